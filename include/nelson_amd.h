@@ -1,0 +1,215 @@
+/*
+ * nelson_amd.h -- C ABI of the MI355X (gfx950) Schur-complement LM step for
+ * bundle adjustment.  Plain pointers and sizes only; every call returns an
+ * int status (NSN_OK = 0) and nsn_last_error() describes the last failure of
+ * the calling thread.
+ *
+ * The reference exposes a C++ template API and no C ABI.  Each entry point
+ * below names the reference member it replaces (paths relative to the nelson
+ * repository root).  One handle owns all device buffers of one problem; a
+ * handle is used by one host thread at a time (the reference's section and
+ * solver are not re-entrant either).
+ *
+ * Increment vectors (dxu, dxv) follow the reference's
+ * DoubleSectionHessianVectors: 6 values per VARIABLE camera (angle, then
+ * translation of the left SE(3) perturbation) in camera order with the fixed
+ * cameras skipped, then 3 per VARIABLE point.
+ */
+#ifndef NELSON_AMD_H
+#define NELSON_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define NSN_OK 0
+#define NSN_NOT_PD 1      /* factorisation failed; increment zeroed (SolverCholeskySchur.hpp:151-154) */
+#define NSN_EINVAL (-1)   /* bad argument */
+#define NSN_EHIP (-2)     /* HIP runtime error or no device */
+#define NSN_ENOMEM (-3)   /* allocation failed */
+#define NSN_ESTATE (-4)   /* call out of order (e.g. schur step before linearize) */
+#define NSN_EDEVICE (-5)  /* a device-side check failed (e.g. a bounded wait expired) */
+
+/* precision of the solve */
+#define NSN_FP64 0        /* everything in FP64 */
+#define NSN_MIXED 1       /* FP32 Jacobians / U,V,W,S assembly / Cholesky; FP64 pivots, rho, refinement */
+
+/* termination reasons: nelson/LevenbergMarquardt.h:36-44 (GaussNewton.h:34-41 is the prefix) */
+#define NSN_SUCCESS_AT_START 0
+#define NSN_SUCCESS_SMALL_INCREMENT 1
+#define NSN_SUCCESS_EPS_B 2
+#define NSN_SUCCESS_SMALL_RESIDUALS 3
+#define NSN_SOLVER_FAILURE 4
+#define NSN_MAX_ITERATION_REACHED 5
+#define NSN_MAX_SUBITERATION_REACHED 6
+
+typedef struct nsn_problem nsn_problem;
+
+/* Problem description.  Replaces the user's DoubleSection subclass set-up:
+ * parametersReady() + addEdge(NodeId cam, NodeId pt, EdgeBinaryUV*) per
+ * observation + structureReady() (nelson/DoubleSection.hpp:67-80,83-246,338-365);
+ * the fixed masks replace NodeType::Fixed (nelson/EdgeInterface.h:35-55).
+ * All arrays are copied; the caller may free them after nsn_problem_create. */
+typedef struct {
+  int n_cameras, n_points, n_edges;
+  const int* cam_idx;              /* [n_edges] camera of each observation            */
+  const int* pt_idx;               /* [n_edges] point of each observation             */
+  const double* obs;               /* [2*n_edges] pixel observations (u, v)           */
+  const double* weight;            /* [n_edges] per-edge weight, or NULL (all 1)      */
+  const unsigned char* cam_fixed;  /* [n_cameras] 1 = fixed, or NULL (none fixed)    */
+  const unsigned char* pt_fixed;   /* [n_points] 1 = fixed, or NULL                   */
+  double focal, cx, cy;            /* pinhole intrinsics shared by all cameras        */
+  double huber_delta;              /* > 0: Huber loss with this threshold (pixels)    */
+  int precision;                   /* NSN_FP64 or NSN_MIXED                           */
+  int device;                      /* HIP device ordinal                              */
+} nsn_problem_desc;
+
+typedef struct {
+  int n_var_cameras, n_var_points, n_wblocks;  /* NU, NV, non-zero W blocks */
+  int n_s;                                     /* 6 * NU, order of the reduced camera system */
+  int n_s_blocks;                              /* non-zero upper 6x6 blocks of S */
+  long long n_s_pairs;                         /* block products of the S assembly */
+} nsn_problem_info;
+
+/* LevenbergMarquardtSettings (nelson/LevenbergMarquardt.h:18-34) */
+typedef struct {
+  int maxNumIt, maxNumSubIt, minNumIt;
+  double epsBVector, epsIncSquare, epsChi2, tau, penaltyV, initV, maxScaleFactor, minScaleFactor;
+} nsn_lm_settings;
+
+/* GaussNewtonSettings (nelson/GaussNewton.h:18-30) */
+typedef struct {
+  int maxNumIt, minNumIt;
+  double epsBVector, epsIncSquare, epsChi2, absLambda, relLambda;
+} nsn_gn_settings;
+
+/* one LevenbergMarquardtStats::InStats record (nelson/LevenbergMarquardt.h:58-61):
+ * chi2 == -1 marks a solver failure; mu is the damping used for that solve. */
+typedef struct { int it, subit; double chi2, mu; } nsn_log_entry;
+
+typedef struct {
+  int reason;          /* NSN_SUCCESS_* / NSN_MAX_* */
+  int iterations;      /* outer iterations completed (the reference's iter) */
+  int sub_iterations;  /* total Schur solves */
+  int n_log;           /* log entries written (including the initial one) */
+  double final_chi2;   /* hessian().chi2() at return */
+  double elapsed_ms;   /* wall-clock of the solve */
+} nsn_lm_result;
+
+/* ---- library ------------------------------------------------------------ */
+const char* nsn_last_error(void);
+int nsn_version(int* major, int* minor);
+int nsn_device_count(int* n);
+
+/* defaults: nelson/LevenbergMarquardt.cpp:6-21, nelson/GaussNewton.cpp:12-22 */
+void nsn_lm_default_settings(nsn_lm_settings* s);
+void nsn_gn_default_settings(nsn_gn_settings* s);
+
+/* ---- synthetic BA generator (host only) -----------------------------------
+ * Seeded SplitMix64 + Box-Muller; the samples' BA_Problem constructor and
+ * addNoise (nelson/samples/BASchurPerfomances.cpp:40-100) play this role in
+ * the reference.  layout 0: cameras on a circle looking at a 4 m cube of
+ * points; layout 1: cameras on a ring, each point observed from a contiguous
+ * arc of `arc` cameras.  Camera 0 and point 0 are left unperturbed (the gauge
+ * of the sample, BASchurPerfomances.cpp:245-246).  Writes at most edge_cap
+ * observations; *n_edges receives the count (NSN_EINVAL if it would exceed). */
+typedef struct {
+  int n_cameras, n_points;
+  int k_min, k_max;            /* observations per point, uniform in [k_min, k_max] */
+  int layout, arc;
+  double obs_sigma;            /* pixel noise */
+  double outlier_frac, outlier_sigma;
+  double rot_sigma, trans_sigma, point_sigma;   /* initial-state perturbation */
+  double focal, cx, cy;
+  unsigned long long seed;
+} nsn_gen_params;
+
+int nsn_generate_ba(const nsn_gen_params* p, int edge_cap,
+                    double* cams_gt, double* pts_gt, double* cams0, double* pts0,
+                    int* cam_idx, int* pt_idx, double* obs, int* n_edges);
+
+/* ---- problem -------------------------------------------------------------- */
+/* parametersReady + addEdge* + structureReady (DoubleSection.hpp:67-246,338-365) */
+int nsn_problem_create(const nsn_problem_desc* d, nsn_problem** out);
+int nsn_problem_destroy(nsn_problem* h);
+int nsn_problem_info_get(nsn_problem* h, nsn_problem_info* info);
+
+/* state: cameras 6 per camera (angle-axis, translation), points 3 per point;
+ * replaces the user's parameterU/parameterV (DoubleSection.h:226-229) */
+int nsn_set_state(nsn_problem* h, const double* cams, const double* pts);
+int nsn_get_state(nsn_problem* h, double* cams, double* pts);
+/* cameras as 12 per camera: R (row-major 3x3) then t */
+int nsn_get_state_rt(nsn_problem* h, double* cams_rt, double* pts);
+int nsn_set_state_rt(nsn_problem* h, const double* cams_rt, const double* pts);
+/* backupSolution / rollbackSolution (driven by LevenbergMarquardt.hpp:51,118,136) */
+int nsn_backup(nsn_problem* h);
+int nsn_rollback(nsn_problem* h);
+
+/* DoubleSection::update(bool) (DoubleSection.hpp:452-467); chi2 may be NULL */
+int nsn_linearize(nsn_problem* h, int with_hessian, double* chi2);
+/* DoubleSectionHessian::maxAbsValBVect (DoubleSectionHessian.hpp:60-63) */
+int nsn_max_abs_b(nsn_problem* h, double* out);
+
+/* Hessian export in the reference's block order (U, V by diagonal block;
+ * W row-major by camera, then point: DoubleSection.hpp:157-191).
+ * U: 36 per variable camera (full 6x6 row-major), V: 9 per variable point,
+ * W: 18 per block (6x3 row-major), Wi/Wj: block coordinates (variable indices).
+ * Any output may be NULL. */
+int nsn_export_hessian(nsn_problem* h, double* U, double* V, double* W, int* Wi, int* Wj,
+                       double* bU, double* bV, double* chi2);
+/* host-assembled Hessian in the same layout (generic edges evaluated by the
+ * caller); the structure (Wi, Wj) must equal the problem's. */
+int nsn_import_hessian(nsn_problem* h, const double* U, const double* V, const double* W,
+                       const double* bU, const double* bV, double chi2);
+
+/* ---- SolverCholeskySchur (nelson/SolverCholeskySchur.h:53-117) ---------- */
+int nsn_schur_init(nsn_problem* h);                                  /* init (.hpp:44-62) */
+int nsn_max_abs_h_diag(nsn_problem* h, double* out);                 /* maxAbsHDiag (.hpp:75-78) */
+/* computeIncrement(H, b, rel, abs) (.hpp:91-157): NSN_OK or NSN_NOT_PD.
+ * dxu / dxv (may be NULL) receive incrementVector(); *sq_norm (may be NULL)
+ * incrementVectorSquaredNorm(). */
+int nsn_schur_step(nsn_problem* h, double rel, double abs_, double* dxu, double* dxv, double* sq_norm);
+/* the user's oplus with the last increment (BASchurPerfomances.cpp:163-170) */
+int nsn_oplus(nsn_problem* h);
+/* computeRhoChi2Change(mu, inc, oldChi2) with the current chi2 (DoubleSection.h:300-311) */
+int nsn_rho(nsn_problem* h, double mu, double old_chi2, double* rho);
+
+/* ---- drivers -------------------------------------------------------------- */
+/* LevenbergMarquardt<SolverCholeskySchur>::solve (LevenbergMarquardt.hpp:35-146),
+ * with the loop state (mu, v, rho, accept/reject, stats) kept on the device.
+ * log (may be NULL) receives up to log_cap entries. */
+int nsn_lm_solve(nsn_problem* h, const nsn_lm_settings* s, nsn_lm_result* res,
+                 nsn_log_entry* log, int log_cap);
+/* GaussNewton<SolverCholeskySchur>::solve (GaussNewton.hpp:31-127) */
+int nsn_gn_solve(nsn_problem* h, const nsn_gn_settings* s, nsn_lm_result* res,
+                 nsn_log_entry* log, int log_cap);
+
+/* per-phase device time of the last Schur step, milliseconds, in the order of
+ * SolverCholeskySchurIterationTimeStat (SolverCholeskySchur.h:25-38):
+ * [0] V^-1 and W V^-1, [1] bS + S assembly, [2] factorisation,
+ * [3] triangular solves, [4] back-substitution.  n >= 5. */
+int nsn_phase_times(nsn_problem* h, double* ms, int n);
+
+/* ---- implicit-function-theorem adjoint (no reference counterpart) --------
+ * At a converged state x*, with H the undamped Gauss-Newton Hessian
+ * (reusing the Schur factorisation), for a loss L(x*) with gradient g given
+ * per variable camera (6, left-perturbation coordinates) and per variable
+ * point (3), computes dL/d(obs) [2*n_edges] and dL/d(weight) [n_edges]. */
+int nsn_adjoint(nsn_problem* h, const double* g_cams, const double* g_pts,
+                double* dL_dobs, double* dL_dw);
+
+/* ---- batched independent windows (no reference counterpart) ------------- */
+typedef struct nsn_batch nsn_batch;
+int nsn_batch_create(const nsn_problem_desc* windows, int n_windows, nsn_batch** out);
+int nsn_batch_destroy(nsn_batch* b);
+/* cams / pts: concatenation over windows of 6*C_w and 3*P_w values */
+int nsn_batch_set_state(nsn_batch* b, const double* cams, const double* pts);
+int nsn_batch_get_state(nsn_batch* b, double* cams, double* pts);
+/* runs s->maxNumIt LM iterations on every window; chi2_out [n_windows] final costs */
+int nsn_batch_lm_solve(nsn_batch* b, const nsn_lm_settings* s, double* chi2_out, double* elapsed_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,60 @@
+"""Builds libnelson_amd.so (HIP kernels for gfx950 + the C ABI) in-tree.
+
+Usage: python -m nelson_amd.build   (or nelson_amd.build.build() from Python)
+"""
+import os
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libnelson_amd.so")
+
+ARCH = os.environ.get("NSN_OFFLOAD_ARCH", "gfx950")
+HIP_SOURCES = ["ba_kernels.hip", "chol_kernels.hip", "api.hip", "extras.hip"]
+CPP_SOURCES = ["generate.cpp", "util.cpp"]
+
+COMMON = ["-O3", "-fPIC", "-std=c++17", "-I", os.path.join(ROOT, "include"), "-I", CSRC]
+# packed FP32 instructions are disabled for all device code
+HIP_FLAGS = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Wno-unused-command-line-argument"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("build step failed:\n" + " ".join(cmd) + "\n" + r.stdout)
+    return r.stdout
+
+
+def build(verbose=False, jobs=8):
+    os.makedirs(OBJ, exist_ok=True)
+    cmds = []
+    objs = []
+    for s in HIP_SOURCES:
+        o = os.path.join(OBJ, s + ".o")
+        cmds.append(["hipcc", "-c"] + COMMON + HIP_FLAGS + [os.path.join(CSRC, s), "-o", o])
+        objs.append(o)
+    for s in CPP_SOURCES:
+        o = os.path.join(OBJ, s + ".o")
+        cmds.append(["g++", "-c"] + COMMON + ["-Wall", os.path.join(CSRC, s), "-o", o])
+        objs.append(o)
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        outs = list(ex.map(_run, cmds))
+    if verbose:
+        for c, o in zip(cmds, outs):
+            print(" ".join(c))
+            if o.strip():
+                print(o)
+    tmp = LIB + ".tmp"
+    _run(["hipcc", "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose="-v" in sys.argv)
+    print(LIB)

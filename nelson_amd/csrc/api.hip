@@ -1,0 +1,892 @@
+// C ABI of the MI355X Schur LM step: problem handle, structure stage
+// (DoubleSection::structureReady), kernel sequences, LM / GN drivers.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <vector>
+
+#include "internal.h"
+#include "kernels.h"
+#include "nelson_amd.h"
+
+using namespace nsn;
+
+#define HIP_TRY(expr)                                                                         \
+  do {                                                                                        \
+    hipError_t e_ = (expr);                                                                   \
+    if (e_ != hipSuccess)                                                                     \
+      return fail(NSN_EHIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
+                  __LINE__);                                                                  \
+  } while (0)
+
+namespace {
+
+constexpr int kTB = 64;          // Cholesky tile
+constexpr int kCholMaxWG = 512;  // persistent workgroups of the factorisation
+constexpr int kTrsvMaxWG = 256;
+
+struct DevBuf {
+  std::vector<void*> ptrs;
+  ~DevBuf() {
+    for (void* p : ptrs) (void)hipFree(p);
+  }
+  template <typename T>
+  T* alloc(size_t n) {
+    void* p = nullptr;
+    if (hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) return nullptr;
+    ptrs.push_back(p);
+    return (T*)p;
+  }
+};
+
+template <typename T>
+int upload(T* dst, const std::vector<T>& src) {
+  if (src.empty()) return NSN_OK;
+  HIP_TRY(hipMemcpy(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+  return NSN_OK;
+}
+
+}  // namespace
+
+struct nsn_problem {
+  int device = 0;
+  hipStream_t st = nullptr;
+  int C = 0, P = 0, E = 0, NU = 0, NV = 0, nW = 0, Ec = 0, prec = NSN_FP64;
+  double f = 0, cx = 0, cy = 0, huber = 0;
+  TileView tv{};
+  int nsb = 0;
+  long long npairs = 0;
+  // host structure
+  std::vector<int> cam_var, pt_var, var_cam_user, var_pt_user, w_cam, w_pt, w_ref;
+  DevBuf mem;
+  BAView view{};
+  Scratch scr{};
+  double *cams = nullptr, *pts = nullptr, *cams_bak = nullptr, *pts_bak = nullptr;
+  void *U = nullptr, *bU = nullptr, *V = nullptr, *bV = nullptr, *W = nullptr, *Vinv = nullptr, *Y = nullptr,
+       *tW = nullptr, *Sd = nullptr, *bS = nullptr, *tiles = nullptr;
+  double *xU = nullptr, *xV = nullptr;
+  int *tile_order = nullptr, *tile_blk_ptr = nullptr, *tile_blk = nullptr, *sb_i = nullptr, *sb_j = nullptr,
+      *sb_ptr = nullptr, *sb_pairs = nullptr;
+  int *flags = nullptr, *counters = nullptr;
+  int nflags = 0;
+  DevCtl* ctl = nullptr;
+  DevCtl* hctl = nullptr;  // pinned mirror
+  DevLog* log = nullptr;
+  int log_cap = 0;
+  bool linearized = false, stepped = false;
+  double last_abs = 0.0;
+  hipEvent_t ev[8] = {};
+  hipGraphExec_t lm_graph = nullptr;
+  bool lm_graph_tried = false;
+
+  ~nsn_problem() {
+    if (lm_graph) (void)hipGraphExecDestroy(lm_graph);
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (hctl) (void)hipHostFree(hctl);
+    if (st) (void)hipStreamDestroy(st);
+  }
+
+  size_t esz() const { return prec == NSN_MIXED ? sizeof(float) : sizeof(double); }
+};
+
+namespace {
+
+// ---- kernel sequences -----------------------------------------------------
+template <typename T>
+void seq_lin_full(nsn_problem* h, int gate) {
+  launch_reset_lin(h->scr, h->ctl, gate, h->st);
+  launch_lin_points<T>(h->view, true, (T*)h->V, (T*)h->bV, (T*)h->W, h->scr, h->ctl, gate, h->st);
+  launch_lin_cams<T>(h->view, (T*)h->U, (T*)h->bU, h->scr, h->ctl, gate, h->st);
+  launch_finalize_lin(h->scr, h->ctl, gate, 1, h->st);
+}
+
+template <typename T>
+void seq_cost(nsn_problem* h, int gate) {
+  launch_lin_points<T>(h->view, false, nullptr, nullptr, nullptr, h->scr, h->ctl, gate, h->st);
+  launch_finalize_lin(h->scr, h->ctl, gate, 0, h->st);
+}
+
+// computeIncrement: V^-1, Y, S, factorisation, solves, back-substitution
+template <typename T>
+void seq_schur(nsn_problem* h, int gate, int checks, bool timed) {
+  hipStream_t st = h->st;
+  launch_begin_step(h->ctl, gate, st);
+  if (timed) (void)hipEventRecord(h->ev[0], st);
+  launch_vinv_y<T>(h->view, (const T*)h->V, (const T*)h->bV, (const T*)h->W, (T*)h->Vinv, (T*)h->Y, (T*)h->tW,
+                   h->ctl, gate, st);
+  if (timed) (void)hipEventRecord(h->ev[1], st);
+  launch_s_diag<T>(h->view, (const T*)h->U, (const T*)h->bU, (const T*)h->Y, (const T*)h->W, (const T*)h->tW,
+                   (T*)h->Sd, (T*)h->bS, h->ctl, gate, st);
+  launch_s_tiles<T>(h->tv, h->tile_order, h->tile_blk_ptr, h->tile_blk, h->sb_i, h->sb_j, h->sb_ptr, h->sb_pairs,
+                    (const T*)h->Sd, (const T*)h->bS, (const T*)h->Y, (const T*)h->W, (T*)h->tiles, h->ctl, gate,
+                    st);
+  if (timed) (void)hipEventRecord(h->ev[2], st);
+  launch_chol_reset<T>(h->flags, h->nflags, h->counters, h->ctl, gate, st);
+  launch_chol<T>(h->tv, (T*)h->tiles, h->tile_order, h->flags, h->counters, h->ctl, gate, kCholMaxWG, st);
+  if (timed) (void)hipEventRecord(h->ev[3], st);
+  launch_trsv_bwd<T>(h->tv, (const T*)h->tiles, h->xU, h->flags + h->tv.ntiles, h->counters + 1, h->ctl, gate,
+                     kTrsvMaxWG, st);
+  if (timed) (void)hipEventRecord(h->ev[4], st);
+  launch_backsub<T>(h->view, (const T*)h->W, (const T*)h->Vinv, (const T*)h->bV, h->xU, h->xV, h->scr, h->ctl,
+                    gate, st);
+  launch_xu_stats<T>(h->NU, h->xU, (const T*)h->bU, h->scr, h->ctl, gate, st);
+  launch_step_finish(h->scr, h->ctl, gate, checks, st);
+  if (timed) (void)hipEventRecord(h->ev[5], st);
+}
+
+// one LM sub-iteration (LevenbergMarquardt.hpp:67-137) as a fixed sequence
+template <typename T>
+void seq_lm_subiter(nsn_problem* h) {
+  seq_lin_full<T>(h, GATE_NEED_LIN);
+  launch_copy_state(h->view, h->cams_bak, h->pts_bak, h->cams, h->pts, h->ctl, GATE_NEED_LIN, h->st);
+  launch_lm_after_lin(h->ctl, h->log, h->st);
+  seq_schur<T>(h, GATE_NOT_DONE, 1, false);
+  launch_oplus(h->view, h->xU, h->xV, h->ctl, GATE_SOLVE_OK, h->st);
+  seq_cost<T>(h, GATE_SOLVE_OK);
+  launch_lm_control(h->ctl, h->log, h->st);
+  launch_copy_state(h->view, h->cams, h->pts, h->cams_bak, h->pts_bak, h->ctl, GATE_ROLLBACK, h->st);
+}
+
+template <typename T>
+void seq_gn_iter(nsn_problem* h) {
+  seq_schur<T>(h, GATE_NOT_DONE, 2, false);
+  launch_oplus(h->view, h->xU, h->xV, h->ctl, GATE_SOLVE_OK, h->st);
+  seq_lin_full<T>(h, GATE_SOLVE_OK);
+  launch_gn_after_step(h->ctl, h->log, h->st);
+}
+
+int sync_ctl(nsn_problem* h) {
+  HIP_TRY(hipMemcpyAsync(h->hctl, h->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, h->st));
+  HIP_TRY(hipStreamSynchronize(h->st));
+  return NSN_OK;
+}
+
+int push_ctl(nsn_problem* h) {
+  HIP_TRY(hipMemcpyAsync(h->ctl, h->hctl, sizeof(DevCtl), hipMemcpyHostToDevice, h->st));
+  HIP_TRY(hipStreamSynchronize(h->st));
+  return NSN_OK;
+}
+
+int check_device_errors(nsn_problem* h) {
+  if (h->hctl->wait_timeout) return fail(NSN_EDEVICE, "a bounded device wait expired in the tile Cholesky");
+  return NSN_OK;
+}
+
+// ---- structure stage ------------------------------------------------------
+int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
+  const int C = h->C, P = h->P, E = h->E;
+  h->cam_var.assign(C, -1);
+  h->pt_var.assign(P, -1);
+  for (int c = 0; c < C; c++)
+    if (!(d->cam_fixed && d->cam_fixed[c])) { h->cam_var[c] = (int)h->var_cam_user.size(); h->var_cam_user.push_back(c); }
+  for (int p = 0; p < P; p++)
+    if (!(d->pt_fixed && d->pt_fixed[p])) { h->pt_var[p] = (int)h->var_pt_user.size(); h->var_pt_user.push_back(p); }
+  h->NU = (int)h->var_cam_user.size();
+  h->NV = (int)h->var_pt_user.size();
+  const int NU = h->NU, NV = h->NV;
+  for (int e = 0; e < E; e++) {
+    if (d->cam_idx[e] < 0 || d->cam_idx[e] >= C || d->pt_idx[e] < 0 || d->pt_idx[e] >= P)
+      return fail(NSN_EINVAL, "edge %d references camera %d / point %d out of range", e, d->cam_idx[e], d->pt_idx[e]);
+  }
+  // point-major order: (point, camera, edge index)
+  std::vector<int> pe(E);
+  std::iota(pe.begin(), pe.end(), 0);
+  std::stable_sort(pe.begin(), pe.end(), [&](int a, int b) {
+    if (d->pt_idx[a] != d->pt_idx[b]) return d->pt_idx[a] < d->pt_idx[b];
+    return d->cam_idx[a] < d->cam_idx[b];
+  });
+  std::vector<int> pt_ptr(P + 1, 0), pe_cam(E), pe_wblk(E, -1);
+  std::vector<double> pe_obs(2 * (size_t)E), pe_w(E);
+  for (int e = 0; e < E; e++) pt_ptr[d->pt_idx[e] + 1]++;
+  for (int p = 0; p < P; p++) pt_ptr[p + 1] += pt_ptr[p];
+  std::vector<int> wptr(NV + 1, 0);
+  h->w_cam.clear();
+  h->w_pt.clear();
+  for (int s = 0; s < E; s++) {
+    const int e = pe[s];
+    pe_cam[s] = d->cam_idx[e];
+    pe_obs[2 * s] = d->obs[2 * e];
+    pe_obs[2 * s + 1] = d->obs[2 * e + 1];
+    pe_w[s] = d->weight ? d->weight[e] : 1.0;
+    const int cv = h->cam_var[d->cam_idx[e]], pv = h->pt_var[d->pt_idx[e]];
+    if (cv >= 0 && pv >= 0) {
+      const bool same = !h->w_cam.empty() && h->w_pt.back() == pv && h->w_cam.back() == cv;
+      if (!same) {
+        h->w_cam.push_back(cv);
+        h->w_pt.push_back(pv);
+        wptr[pv + 1]++;
+      }
+      pe_wblk[s] = (int)h->w_cam.size() - 1;
+    }
+  }
+  for (int v = 0; v < NV; v++) wptr[v + 1] += wptr[v];
+  h->nW = (int)h->w_cam.size();
+  const int nW = h->nW;
+  // camera-major edges of the variable cameras: (camera, point, edge index)
+  std::vector<int> ce;
+  ce.reserve(E);
+  for (int e = 0; e < E; e++)
+    if (h->cam_var[d->cam_idx[e]] >= 0) ce.push_back(e);
+  std::stable_sort(ce.begin(), ce.end(), [&](int a, int b) {
+    if (d->cam_idx[a] != d->cam_idx[b]) return d->cam_idx[a] < d->cam_idx[b];
+    return d->pt_idx[a] < d->pt_idx[b];
+  });
+  h->Ec = (int)ce.size();
+  std::vector<int> cam_ptr(NU + 1, 0), ce_pt(h->Ec);
+  std::vector<double> ce_obs(2 * (size_t)h->Ec), ce_w(h->Ec);
+  for (int s = 0; s < h->Ec; s++) {
+    const int e = ce[s];
+    cam_ptr[h->cam_var[d->cam_idx[e]] + 1]++;
+    ce_pt[s] = d->pt_idx[e];
+    ce_obs[2 * s] = d->obs[2 * e];
+    ce_obs[2 * s + 1] = d->obs[2 * e + 1];
+    ce_w[s] = d->weight ? d->weight[e] : 1.0;
+  }
+  for (int u = 0; u < NU; u++) cam_ptr[u + 1] += cam_ptr[u];
+  // W blocks grouped by camera (ascending point within a camera)
+  std::vector<int> cw_ptr(NU + 1, 0), cw_idx(nW);
+  for (int k = 0; k < nW; k++) cw_ptr[h->w_cam[k] + 1]++;
+  for (int u = 0; u < NU; u++) cw_ptr[u + 1] += cw_ptr[u];
+  {
+    std::vector<int> fill(cw_ptr.begin(), cw_ptr.end() - 1);
+    for (int k = 0; k < nW; k++) cw_idx[fill[h->w_cam[k]]++] = k;
+  }
+  // reference order of W blocks: row-major (camera, point)
+  h->w_ref = cw_idx;
+  // S structure: pairs of blocks sharing a point, grouped by camera pair
+  struct PairRec { int ci, cj, ka, kb; };
+  std::vector<PairRec> pr;
+  for (int v = 0; v < NV; v++)
+    for (int a = wptr[v]; a < wptr[v + 1]; a++)
+      for (int b = a + 1; b < wptr[v + 1]; b++) pr.push_back({h->w_cam[a], h->w_cam[b], a, b});
+  std::stable_sort(pr.begin(), pr.end(), [](const PairRec& x, const PairRec& y) {
+    return x.ci != y.ci ? x.ci < y.ci : x.cj < y.cj;
+  });
+  h->npairs = (long long)pr.size();
+  std::vector<int> sb_i, sb_j, sb_ptr, sb_pairs(2 * pr.size());
+  {
+    size_t q = 0;
+    for (int u = 0; u < NU; u++) {
+      sb_i.push_back(u);
+      sb_j.push_back(u);
+      sb_ptr.push_back((int)q);
+      while (q < pr.size() && pr[q].ci == u) {
+        const int cj = pr[q].cj;
+        sb_i.push_back(u);
+        sb_j.push_back(cj);
+        sb_ptr.push_back((int)q);
+        while (q < pr.size() && pr[q].ci == u && pr[q].cj == cj) {
+          sb_pairs[2 * q] = pr[q].ka;
+          sb_pairs[2 * q + 1] = pr[q].kb;
+          q++;
+        }
+      }
+    }
+    sb_ptr.push_back((int)q);
+  }
+  h->nsb = (int)sb_i.size();
+  // tiles of [S; bS^T]
+  const int n = 6 * NU;
+  TileView tv;
+  tv.n = n;
+  tv.TB = kTB;
+  tv.NT = (n + 1 + kTB - 1) / kTB;
+  tv.ntiles = tv.NT * (tv.NT + 1) / 2;
+  h->tv = tv;
+  std::vector<int> tile_order(2 * (size_t)tv.ntiles);
+  for (int j = 0; j < tv.NT; j++)
+    for (int i = j; i < tv.NT; i++) {
+      const long long t = tile_index(i, j, tv.NT);
+      tile_order[2 * t] = i;
+      tile_order[2 * t + 1] = j;
+    }
+  std::vector<int> tile_cnt(tv.ntiles + 1, 0);
+  auto for_tiles = [&](int sb, auto&& fn) {
+    const int ci = sb_i[sb], cj = sb_j[sb];
+    const int ti0 = (6 * cj) / kTB, ti1 = (6 * cj + 5) / kTB;
+    const int tj0 = (6 * ci) / kTB, tj1 = (6 * ci + 5) / kTB;
+    for (int ti = ti0; ti <= ti1; ti++)
+      for (int tj = tj0; tj <= tj1; tj++)
+        if (ti >= tj) fn(tile_index(ti, tj, tv.NT));
+  };
+  for (int sb = 0; sb < h->nsb; sb++) for_tiles(sb, [&](long long t) { tile_cnt[t + 1]++; });
+  for (int t = 0; t < tv.ntiles; t++) tile_cnt[t + 1] += tile_cnt[t];
+  std::vector<int> tile_blk(tile_cnt[tv.ntiles]);
+  {
+    std::vector<int> fill(tile_cnt.begin(), tile_cnt.end() - 1);
+    for (int sb = 0; sb < h->nsb; sb++) for_tiles(sb, [&](long long t) { tile_blk[fill[t]++] = sb; });
+  }
+  const int NTr = (n + kTB - 1) / kTB;
+  h->nflags = tv.ntiles + NTr;
+
+  // ---- device allocation ----
+  DevBuf& m = h->mem;
+  const size_t es = h->esz();
+  auto a_int = [&](size_t k) { return m.alloc<int>(k); };
+  auto a_dbl = [&](size_t k) { return m.alloc<double>(k); };
+  auto a_T = [&](size_t k) { return (void*)m.alloc<char>(k * es); };
+  int* d_cam_var = a_int(C);
+  int* d_pt_var = a_int(P);
+  int* d_vcu = a_int(NU);
+  int* d_vpu = a_int(NV);
+  int* d_pt_ptr = a_int(P + 1);
+  int* d_pe_cam = a_int(E);
+  double* d_pe_obs = a_dbl(2 * (size_t)E);
+  double* d_pe_w = a_dbl(E);
+  int* d_pe_wblk = a_int(E);
+  int* d_cam_ptr = a_int(NU + 1);
+  int* d_ce_pt = a_int(h->Ec);
+  double* d_ce_obs = a_dbl(2 * (size_t)h->Ec);
+  double* d_ce_w = a_dbl(h->Ec);
+  int* d_wptr = a_int(NV + 1);
+  int* d_w_cam = a_int(nW);
+  int* d_cw_ptr = a_int(NU + 1);
+  int* d_cw_idx = a_int(nW);
+  h->cams = a_dbl(12 * (size_t)C);
+  h->pts = a_dbl(3 * (size_t)P);
+  h->cams_bak = a_dbl(12 * (size_t)C);
+  h->pts_bak = a_dbl(3 * (size_t)P);
+  h->U = a_T(21 * (size_t)NU);
+  h->bU = a_T(6 * (size_t)NU);
+  h->V = a_T(6 * (size_t)NV);
+  h->bV = a_T(3 * (size_t)NV);
+  h->W = a_T(18 * (size_t)nW);
+  h->Vinv = a_T(6 * (size_t)NV);
+  h->Y = a_T(18 * (size_t)nW);
+  h->tW = a_T(6 * (size_t)nW);
+  h->Sd = a_T(21 * (size_t)NU);
+  h->bS = a_T(6 * (size_t)NU);
+  h->tiles = a_T((size_t)tv.ntiles * kTB * kTB);
+  h->xU = a_dbl(6 * (size_t)NU);
+  h->xV = a_dbl(3 * (size_t)NV);
+  h->tile_order = a_int(tile_order.size());
+  h->tile_blk_ptr = a_int(tile_cnt.size());
+  h->tile_blk = a_int(tile_blk.size());
+  h->sb_i = a_int(sb_i.size());
+  h->sb_j = a_int(sb_j.size());
+  h->sb_ptr = a_int(sb_ptr.size());
+  h->sb_pairs = a_int(sb_pairs.size());
+  h->flags = a_int(h->nflags);
+  h->counters = a_int(4);
+  const int nblk_pts = std::max(1, (std::max(P, NV) + 255) / 256);
+  const int nblk_cams = std::max(1, (6 * NU + 255) / 256);
+  h->scr.part_chi2 = a_dbl(nblk_pts);
+  h->scr.part_xv = a_dbl(2 * (size_t)nblk_pts);
+  h->scr.part_xu = a_dbl(2 * (size_t)nblk_cams);
+  h->scr.maxes = m.alloc<unsigned long long>(4);
+  h->scr.nblk_pts = (P + 255) / 256;
+  h->scr.nblk_cams = (6 * NU + 255) / 256;
+  h->ctl = m.alloc<DevCtl>(1);
+  h->log_cap = 4096;
+  h->log = m.alloc<DevLog>(h->log_cap);
+  if (!h->log || !h->ctl || !h->tiles || !h->W || !h->Y || !h->sb_pairs)
+    return fail(NSN_ENOMEM, "device allocation failed (%.1f MB of tiles)",
+                (double)tv.ntiles * kTB * kTB * es / 1e6);
+  int rc;
+  if ((rc = upload(d_cam_var, h->cam_var)) || (rc = upload(d_pt_var, h->pt_var)) ||
+      (rc = upload(d_vcu, h->var_cam_user)) || (rc = upload(d_vpu, h->var_pt_user)) ||
+      (rc = upload(d_pt_ptr, pt_ptr)) || (rc = upload(d_pe_cam, pe_cam)) || (rc = upload(d_pe_obs, pe_obs)) ||
+      (rc = upload(d_pe_w, pe_w)) || (rc = upload(d_pe_wblk, pe_wblk)) || (rc = upload(d_cam_ptr, cam_ptr)) ||
+      (rc = upload(d_ce_pt, ce_pt)) || (rc = upload(d_ce_obs, ce_obs)) || (rc = upload(d_ce_w, ce_w)) ||
+      (rc = upload(d_wptr, wptr)) || (rc = upload(d_w_cam, h->w_cam)) || (rc = upload(d_cw_ptr, cw_ptr)) ||
+      (rc = upload(d_cw_idx, cw_idx)) || (rc = upload(h->tile_order, tile_order)) ||
+      (rc = upload(h->tile_blk_ptr, tile_cnt)) || (rc = upload(h->tile_blk, tile_blk)) ||
+      (rc = upload(h->sb_i, sb_i)) || (rc = upload(h->sb_j, sb_j)) || (rc = upload(h->sb_ptr, sb_ptr)) ||
+      (rc = upload(h->sb_pairs, sb_pairs)))
+    return rc;
+  BAView& v = h->view;
+  v.C = C; v.P = P; v.E = E; v.NU = NU; v.NV = NV; v.nW = nW;
+  v.f = h->f; v.cx = h->cx; v.cy = h->cy; v.huber = h->huber;
+  v.cam_var = d_cam_var; v.pt_var = d_pt_var; v.var_cam_user = d_vcu; v.var_pt_user = d_vpu;
+  v.pt_ptr = d_pt_ptr; v.pe_cam = d_pe_cam; v.pe_obs = d_pe_obs; v.pe_w = d_pe_w; v.pe_wblk = d_pe_wblk;
+  v.cam_ptr = d_cam_ptr; v.ce_pt = d_ce_pt; v.ce_obs = d_ce_obs; v.ce_w = d_ce_w;
+  v.wptr = d_wptr; v.w_cam = d_w_cam; v.cw_ptr = d_cw_ptr; v.cw_idx = d_cw_idx;
+  v.cams = h->cams; v.pts = h->pts;
+  return NSN_OK;
+}
+
+DevCtl make_ctl(int log_cap) {
+  DevCtl c;
+  std::memset(&c, 0, sizeof(c));
+  c.log_cap = log_cap;
+  return c;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" int nsn_device_count(int* n) {
+  if (!n) return fail(NSN_EINVAL, "null argument");
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  if (e != hipSuccess) { *n = 0; return fail(NSN_EHIP, "hipGetDeviceCount: %s", hipGetErrorString(e)); }
+  *n = c;
+  return NSN_OK;
+}
+
+extern "C" int nsn_problem_create(const nsn_problem_desc* d, nsn_problem** out) {
+  clear_error();
+  if (!d || !out) return fail(NSN_EINVAL, "nsn_problem_create: null argument");
+  *out = nullptr;
+  if (d->n_cameras < 1 || d->n_points < 1 || d->n_edges < 0 || (d->n_edges > 0 && (!d->cam_idx || !d->pt_idx || !d->obs)))
+    return fail(NSN_EINVAL, "nsn_problem_create: bad sizes or missing edge arrays");
+  if (d->precision != NSN_FP64 && d->precision != NSN_MIXED)
+    return fail(NSN_EINVAL, "nsn_problem_create: unknown precision %d", d->precision);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return fail(NSN_EHIP, "nsn_problem_create: no HIP device");
+  if (d->device < 0 || d->device >= ndev) return fail(NSN_EINVAL, "nsn_problem_create: bad device %d", d->device);
+  HIP_TRY(hipSetDevice(d->device));
+  std::unique_ptr<nsn_problem> h(new nsn_problem());
+  h->device = d->device;
+  h->C = d->n_cameras; h->P = d->n_points; h->E = d->n_edges; h->prec = d->precision;
+  h->f = d->focal; h->cx = d->cx; h->cy = d->cy; h->huber = d->huber_delta;
+  HIP_TRY(hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking));
+  for (auto& e : h->ev) HIP_TRY(hipEventCreate(&e));
+  HIP_TRY(hipHostMalloc((void**)&h->hctl, sizeof(DevCtl), hipHostMallocDefault));
+  int rc = build_structure(h.get(), d);
+  if (rc) return rc;
+  *h->hctl = make_ctl(h->log_cap);
+  if ((rc = push_ctl(h.get()))) return rc;
+  *out = h.release();
+  return NSN_OK;
+}
+
+extern "C" int nsn_problem_destroy(nsn_problem* h) {
+  if (!h) return NSN_OK;
+  (void)hipSetDevice(h->device);
+  if (h->st) (void)hipStreamSynchronize(h->st);
+  delete h;
+  return NSN_OK;
+}
+
+extern "C" int nsn_problem_info_get(nsn_problem* h, nsn_problem_info* info) {
+  if (!h || !info) return fail(NSN_EINVAL, "null argument");
+  info->n_var_cameras = h->NU;
+  info->n_var_points = h->NV;
+  info->n_wblocks = h->nW;
+  info->n_s = 6 * h->NU;
+  info->n_s_blocks = h->nsb;
+  info->n_s_pairs = h->npairs;
+  return NSN_OK;
+}
+
+extern "C" int nsn_set_state_rt(nsn_problem* h, const double* cams_rt, const double* pts) {
+  if (!h || !cams_rt || !pts) return fail(NSN_EINVAL, "null argument");
+  HIP_TRY(hipSetDevice(h->device));
+  HIP_TRY(hipMemcpy(h->cams, cams_rt, sizeof(double) * 12 * (size_t)h->C, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(h->pts, pts, sizeof(double) * 3 * (size_t)h->P, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(h->cams_bak, cams_rt, sizeof(double) * 12 * (size_t)h->C, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(h->pts_bak, pts, sizeof(double) * 3 * (size_t)h->P, hipMemcpyHostToDevice));
+  h->linearized = false;
+  h->stepped = false;
+  return NSN_OK;
+}
+
+extern "C" int nsn_set_state(nsn_problem* h, const double* cams, const double* pts) {
+  if (!h || !cams || !pts) return fail(NSN_EINVAL, "null argument");
+  std::vector<double> rt(12 * (size_t)h->C);
+  for (int c = 0; c < h->C; c++) {
+    rot_exp(cams + 6 * c, &rt[12 * c]);
+    for (int i = 0; i < 3; i++) rt[12 * c + 9 + i] = cams[6 * c + 3 + i];
+  }
+  return nsn_set_state_rt(h, rt.data(), pts);
+}
+
+extern "C" int nsn_get_state_rt(nsn_problem* h, double* cams_rt, double* pts) {
+  if (!h) return fail(NSN_EINVAL, "null argument");
+  HIP_TRY(hipSetDevice(h->device));
+  HIP_TRY(hipStreamSynchronize(h->st));
+  if (cams_rt) HIP_TRY(hipMemcpy(cams_rt, h->cams, sizeof(double) * 12 * (size_t)h->C, hipMemcpyDeviceToHost));
+  if (pts) HIP_TRY(hipMemcpy(pts, h->pts, sizeof(double) * 3 * (size_t)h->P, hipMemcpyDeviceToHost));
+  return NSN_OK;
+}
+
+extern "C" int nsn_get_state(nsn_problem* h, double* cams, double* pts) {
+  if (!h) return fail(NSN_EINVAL, "null argument");
+  std::vector<double> rt(12 * (size_t)h->C);
+  int rc = nsn_get_state_rt(h, rt.data(), pts);
+  if (rc) return rc;
+  if (cams)
+    for (int c = 0; c < h->C; c++) {
+      rot_log(&rt[12 * c], cams + 6 * c);
+      for (int i = 0; i < 3; i++) cams[6 * c + 3 + i] = rt[12 * c + 9 + i];
+    }
+  return NSN_OK;
+}
+
+extern "C" int nsn_backup(nsn_problem* h) {
+  if (!h) return fail(NSN_EINVAL, "null argument");
+  HIP_TRY(hipSetDevice(h->device));
+  launch_copy_state(h->view, h->cams_bak, h->pts_bak, h->cams, h->pts, h->ctl, GATE_ALWAYS, h->st);
+  HIP_TRY(hipStreamSynchronize(h->st));
+  return NSN_OK;
+}
+
+extern "C" int nsn_rollback(nsn_problem* h) {
+  if (!h) return fail(NSN_EINVAL, "null argument");
+  HIP_TRY(hipSetDevice(h->device));
+  launch_copy_state(h->view, h->cams, h->pts, h->cams_bak, h->pts_bak, h->ctl, GATE_ALWAYS, h->st);
+  HIP_TRY(hipStreamSynchronize(h->st));
+  return NSN_OK;
+}
+
+static int reset_ctl_primitive(nsn_problem* h) {
+  // primitives run with the gates open: done = 0, need_lin = 1
+  int rc = sync_ctl(h);
+  if (rc) return rc;
+  h->hctl->done = 0;
+  h->hctl->mode = 0;
+  return push_ctl(h);
+}
+
+extern "C" int nsn_linearize(nsn_problem* h, int with_hessian, double* chi2) {
+  if (!h) return fail(NSN_EINVAL, "null argument");
+  HIP_TRY(hipSetDevice(h->device));
+  int rc = reset_ctl_primitive(h);
+  if (rc) return rc;
+  if (h->prec == NSN_MIXED) {
+    if (with_hessian) seq_lin_full<float>(h, GATE_ALWAYS); else seq_cost<float>(h, GATE_ALWAYS);
+  } else {
+    if (with_hessian) seq_lin_full<double>(h, GATE_ALWAYS); else seq_cost<double>(h, GATE_ALWAYS);
+  }
+  HIP_TRY(hipGetLastError());
+  if ((rc = sync_ctl(h))) return rc;
+  if (with_hessian) h->linearized = true;
+  if (chi2) *chi2 = h->hctl->chi2;
+  return NSN_OK;
+}
+
+extern "C" int nsn_max_abs_b(nsn_problem* h, double* out) {
+  if (!h || !out) return fail(NSN_EINVAL, "null argument");
+  if (!h->linearized) return fail(NSN_ESTATE, "nsn_max_abs_b before nsn_linearize(with_hessian=1)");
+  int rc = sync_ctl(h);
+  if (rc) return rc;
+  *out = h->hctl->maxb;
+  return NSN_OK;
+}
+
+extern "C" int nsn_max_abs_h_diag(nsn_problem* h, double* out) {
+  if (!h || !out) return fail(NSN_EINVAL, "null argument");
+  if (!h->linearized) return fail(NSN_ESTATE, "nsn_max_abs_h_diag before nsn_linearize(with_hessian=1)");
+  int rc = sync_ctl(h);
+  if (rc) return rc;
+  *out = h->hctl->maxdiag;
+  return NSN_OK;
+}
+
+template <typename T>
+static int export_hessian_t(nsn_problem* h, double* U, double* V, double* W, int* Wi, int* Wj, double* bU,
+                            double* bV) {
+  const int NU = h->NU, NV = h->NV, nW = h->nW;
+  std::vector<T> hu(21 * (size_t)NU), hv(6 * (size_t)NV), hw(18 * (size_t)nW), hbu(6 * (size_t)NU),
+      hbv(3 * (size_t)NV);
+  HIP_TRY(hipStreamSynchronize(h->st));
+  if (U) HIP_TRY(hipMemcpy(hu.data(), h->U, hu.size() * sizeof(T), hipMemcpyDeviceToHost));
+  if (V) HIP_TRY(hipMemcpy(hv.data(), h->V, hv.size() * sizeof(T), hipMemcpyDeviceToHost));
+  if (W) HIP_TRY(hipMemcpy(hw.data(), h->W, hw.size() * sizeof(T), hipMemcpyDeviceToHost));
+  if (bU) HIP_TRY(hipMemcpy(hbu.data(), h->bU, hbu.size() * sizeof(T), hipMemcpyDeviceToHost));
+  if (bV) HIP_TRY(hipMemcpy(hbv.data(), h->bV, hbv.size() * sizeof(T), hipMemcpyDeviceToHost));
+  if (U)
+    for (int u = 0; u < NU; u++)
+      for (int a = 0; a < 6; a++)
+        for (int b = 0; b < 6; b++) {
+          const int i = std::min(a, b), j = std::max(a, b);
+          U[36 * (size_t)u + 6 * a + b] = (double)hu[21 * (size_t)u + i * 6 - i * (i - 1) / 2 + (j - i)];
+        }
+  if (V)
+    for (int q = 0; q < NV; q++) {
+      const T* s = &hv[6 * (size_t)q];
+      const double full[9] = {(double)s[0], (double)s[1], (double)s[2], (double)s[1], (double)s[3],
+                              (double)s[4], (double)s[2], (double)s[4], (double)s[5]};
+      std::memcpy(V + 9 * (size_t)q, full, sizeof(full));
+    }
+  for (int r = 0; r < nW; r++) {
+    const int k = h->w_ref[r];
+    if (W)
+      for (int i = 0; i < 18; i++) W[18 * (size_t)r + i] = (double)hw[18 * (size_t)k + i];
+    if (Wi) Wi[r] = h->w_cam[k];
+    if (Wj) Wj[r] = h->w_pt[k];
+  }
+  if (bU) for (size_t i = 0; i < hbu.size(); i++) bU[i] = (double)hbu[i];
+  if (bV) for (size_t i = 0; i < hbv.size(); i++) bV[i] = (double)hbv[i];
+  return NSN_OK;
+}
+
+extern "C" int nsn_export_hessian(nsn_problem* h, double* U, double* V, double* W, int* Wi, int* Wj, double* bU,
+                                  double* bV, double* chi2) {
+  if (!h) return fail(NSN_EINVAL, "null argument");
+  HIP_TRY(hipSetDevice(h->device));
+  int rc = h->prec == NSN_MIXED ? export_hessian_t<float>(h, U, V, W, Wi, Wj, bU, bV)
+                                : export_hessian_t<double>(h, U, V, W, Wi, Wj, bU, bV);
+  if (rc) return rc;
+  if (chi2) {
+    if ((rc = sync_ctl(h))) return rc;
+    *chi2 = h->hctl->chi2_lin;
+  }
+  return NSN_OK;
+}
+
+template <typename T>
+static int import_hessian_t(nsn_problem* h, const double* U, const double* V, const double* W, const double* bU,
+                            const double* bV) {
+  const int NU = h->NU, NV = h->NV, nW = h->nW;
+  std::vector<T> hu(21 * (size_t)NU), hv(6 * (size_t)NV), hw(18 * (size_t)nW), hbu(6 * (size_t)NU),
+      hbv(3 * (size_t)NV);
+  for (int u = 0; u < NU; u++)
+    for (int i = 0; i < 6; i++)
+      for (int j = i; j < 6; j++) hu[21 * (size_t)u + i * 6 - i * (i - 1) / 2 + (j - i)] = (T)U[36 * (size_t)u + 6 * i + j];
+  for (int q = 0; q < NV; q++) {
+    const double* s = V + 9 * (size_t)q;
+    const double up[6] = {s[0], s[1], s[2], s[4], s[5], s[8]};
+    for (int i = 0; i < 6; i++) hv[6 * (size_t)q + i] = (T)up[i];
+  }
+  for (int r = 0; r < nW; r++) {
+    const int k = h->w_ref[r];
+    for (int i = 0; i < 18; i++) hw[18 * (size_t)k + i] = (T)W[18 * (size_t)r + i];
+  }
+  for (size_t i = 0; i < hbu.size(); i++) hbu[i] = (T)bU[i];
+  for (size_t i = 0; i < hbv.size(); i++) hbv[i] = (T)bV[i];
+  HIP_TRY(hipMemcpy(h->U, hu.data(), hu.size() * sizeof(T), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(h->V, hv.data(), hv.size() * sizeof(T), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(h->W, hw.data(), hw.size() * sizeof(T), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(h->bU, hbu.data(), hbu.size() * sizeof(T), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(h->bV, hbv.data(), hbv.size() * sizeof(T), hipMemcpyHostToDevice));
+  return NSN_OK;
+}
+
+extern "C" int nsn_import_hessian(nsn_problem* h, const double* U, const double* V, const double* W, const double* bU,
+                                  const double* bV, double chi2) {
+  if (!h || !U || !V || !W || !bU || !bV) return fail(NSN_EINVAL, "null argument");
+  HIP_TRY(hipSetDevice(h->device));
+  int rc = h->prec == NSN_MIXED ? import_hessian_t<float>(h, U, V, W, bU, bV)
+                                : import_hessian_t<double>(h, U, V, W, bU, bV);
+  if (rc) return rc;
+  if ((rc = sync_ctl(h))) return rc;
+  double mb = 0, md = 0;
+  for (size_t i = 0; i < 6 * (size_t)h->NU; i++) mb = std::max(mb, std::fabs(bU[i]));
+  for (size_t i = 0; i < 3 * (size_t)h->NV; i++) mb = std::max(mb, std::fabs(bV[i]));
+  for (int u = 0; u < h->NU; u++)
+    for (int i = 0; i < 6; i++) md = std::max(md, std::fabs(U[36 * (size_t)u + 7 * i]));
+  for (int q = 0; q < h->NV; q++)
+    for (int i = 0; i < 3; i++) md = std::max(md, std::fabs(V[9 * (size_t)q + 4 * i]));
+  h->hctl->chi2 = chi2;
+  h->hctl->chi2_lin = chi2;
+  h->hctl->maxb = mb;
+  h->hctl->maxdiag = md;
+  if ((rc = push_ctl(h))) return rc;
+  h->linearized = true;
+  return NSN_OK;
+}
+
+extern "C" int nsn_schur_init(nsn_problem* h) {
+  if (!h) return fail(NSN_EINVAL, "null argument");
+  // the symbolic work (pair lists, tiles) is done by nsn_problem_create, so
+  // init only checks the call order (SolverCholeskySchur.hpp:44-62)
+  if (!h->linearized) return fail(NSN_ESTATE, "nsn_schur_init before nsn_linearize(with_hessian=1)");
+  return NSN_OK;
+}
+
+extern "C" int nsn_schur_step(nsn_problem* h, double rel, double abs_, double* dxu, double* dxv, double* sq_norm) {
+  if (!h) return fail(NSN_EINVAL, "null argument");
+  if (!h->linearized) return fail(NSN_ESTATE, "nsn_schur_step before nsn_linearize(with_hessian=1)");
+  HIP_TRY(hipSetDevice(h->device));
+  int rc = reset_ctl_primitive(h);
+  if (rc) return rc;
+  launch_set_damping(h->ctl, rel, abs_, h->st);
+  if (h->prec == NSN_MIXED) seq_schur<float>(h, GATE_ALWAYS, 0, true);
+  else seq_schur<double>(h, GATE_ALWAYS, 0, true);
+  HIP_TRY(hipGetLastError());
+  if ((rc = sync_ctl(h))) return rc;
+  if ((rc = check_device_errors(h))) return rc;
+  h->stepped = true;
+  h->last_abs = abs_;
+  const bool ok = h->hctl->solve_ok != 0;
+  if (dxu) {
+    if (ok) HIP_TRY(hipMemcpy(dxu, h->xU, sizeof(double) * 6 * (size_t)h->NU, hipMemcpyDeviceToHost));
+    else std::memset(dxu, 0, sizeof(double) * 6 * (size_t)h->NU);
+  }
+  if (dxv) {
+    if (ok) HIP_TRY(hipMemcpy(dxv, h->xV, sizeof(double) * 3 * (size_t)h->NV, hipMemcpyDeviceToHost));
+    else std::memset(dxv, 0, sizeof(double) * 3 * (size_t)h->NV);
+  }
+  if (sq_norm) *sq_norm = ok ? h->hctl->incsq : 0.0;
+  if (!ok) {
+    launch_zero<double>(h->xU, 6LL * h->NU, h->st);
+    launch_zero<double>(h->xV, 3LL * h->NV, h->st);
+    HIP_TRY(hipStreamSynchronize(h->st));
+    return NSN_NOT_PD;
+  }
+  return NSN_OK;
+}
+
+extern "C" int nsn_phase_times(nsn_problem* h, double* ms, int n) {
+  if (!h || !ms || n < 5) return fail(NSN_EINVAL, "nsn_phase_times needs 5 slots");
+  if (!h->stepped) return fail(NSN_ESTATE, "no Schur step has run");
+  HIP_TRY(hipEventSynchronize(h->ev[5]));
+  float t = 0;
+  const int pairs[5][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 4}, {4, 5}};
+  for (int i = 0; i < 5; i++) {
+    HIP_TRY(hipEventElapsedTime(&t, h->ev[pairs[i][0]], h->ev[pairs[i][1]]));
+    ms[i] = t;
+  }
+  return NSN_OK;
+}
+
+extern "C" int nsn_oplus(nsn_problem* h) {
+  if (!h) return fail(NSN_EINVAL, "null argument");
+  if (!h->stepped) return fail(NSN_ESTATE, "nsn_oplus before nsn_schur_step");
+  HIP_TRY(hipSetDevice(h->device));
+  launch_oplus(h->view, h->xU, h->xV, h->ctl, GATE_ALWAYS, h->st);
+  HIP_TRY(hipStreamSynchronize(h->st));
+  return NSN_OK;
+}
+
+extern "C" int nsn_rho(nsn_problem* h, double mu, double old_chi2, double* rho) {
+  if (!h || !rho) return fail(NSN_EINVAL, "null argument");
+  if (!h->stepped) return fail(NSN_ESTATE, "nsn_rho before nsn_schur_step");
+  int rc = sync_ctl(h);
+  if (rc) return rc;
+  double den = h->hctl->den;
+  if (mu != h->last_abs) {
+    // x^T (mu x - b) for a mu other than the step's damping
+    const int nu = 6 * h->NU, nv = 3 * h->NV;
+    std::vector<double> x(nu + nv), b(nu + nv);
+    HIP_TRY(hipMemcpy(x.data(), h->xU, sizeof(double) * nu, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(x.data() + nu, h->xV, sizeof(double) * nv, hipMemcpyDeviceToHost));
+    if (h->prec == NSN_MIXED) {
+      std::vector<float> bf(nu + nv);
+      HIP_TRY(hipMemcpy(bf.data(), h->bU, sizeof(float) * nu, hipMemcpyDeviceToHost));
+      HIP_TRY(hipMemcpy(bf.data() + nu, h->bV, sizeof(float) * nv, hipMemcpyDeviceToHost));
+      for (int i = 0; i < nu + nv; i++) b[i] = bf[i];
+    } else {
+      HIP_TRY(hipMemcpy(b.data(), h->bU, sizeof(double) * nu, hipMemcpyDeviceToHost));
+      HIP_TRY(hipMemcpy(b.data() + nu, h->bV, sizeof(double) * nv, hipMemcpyDeviceToHost));
+    }
+    double du = 0, dv = 0;
+    for (int i = 0; i < nu; i++) du += x[i] * (mu * x[i] - b[i]);
+    for (int i = nu; i < nu + nv; i++) dv += x[i] * (mu * x[i] - b[i]);
+    den = du + dv;
+  }
+  *rho = (old_chi2 - h->hctl->chi2) / den;
+  return NSN_OK;
+}
+
+template <typename T>
+static int run_lm(nsn_problem* h, const nsn_lm_settings* s, nsn_lm_result* res, nsn_log_entry* log, int log_cap) {
+  DevCtl c = make_ctl(h->log_cap);
+  c.mode = 1;
+  c.need_lin = 1;
+  c.maxNumIt = s->maxNumIt; c.maxNumSubIt = s->maxNumSubIt; c.minNumIt = s->minNumIt;
+  c.epsBVector = s->epsBVector; c.epsIncSquare = s->epsIncSquare; c.epsChi2 = s->epsChi2;
+  c.tau = s->tau; c.penaltyV = s->penaltyV; c.initV = s->initV;
+  c.maxScaleFactor = s->maxScaleFactor; c.minScaleFactor = s->minScaleFactor;
+  *h->hctl = c;
+  int rc = push_ctl(h);
+  if (rc) return rc;
+  static const bool no_graph = std::getenv("NSN_NO_GRAPH") != nullptr;
+  if (!h->lm_graph_tried && !no_graph) {
+    h->lm_graph_tried = true;
+    hipGraph_t g = nullptr;
+    if (hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+      seq_lm_subiter<T>(h);
+      if (hipStreamEndCapture(h->st, &g) == hipSuccess && g) {
+        if (hipGraphInstantiate(&h->lm_graph, g, nullptr, nullptr, 0) != hipSuccess) h->lm_graph = nullptr;
+        (void)hipGraphDestroy(g);
+      }
+    }
+    (void)hipGetLastError();
+  }
+  const long long max_sub = (long long)std::max(1, s->maxNumIt) * std::max(1, s->maxNumSubIt) + 4;
+  auto t0 = std::chrono::steady_clock::now();
+  for (long long k = 0; k < max_sub; k++) {
+    if (h->lm_graph) HIP_TRY(hipGraphLaunch(h->lm_graph, h->st));
+    else seq_lm_subiter<T>(h);
+    if ((rc = sync_ctl(h))) return rc;
+    if (h->hctl->done || h->hctl->wait_timeout) break;
+  }
+  auto t1 = std::chrono::steady_clock::now();
+  HIP_TRY(hipGetLastError());
+  if ((rc = check_device_errors(h))) return rc;
+  const DevCtl& r = *h->hctl;
+  res->reason = r.done ? r.reason : NSN_MAX_ITERATION_REACHED;
+  res->iterations = r.iter;
+  res->sub_iterations = r.total_solves;
+  res->n_log = std::min(r.n_log, h->log_cap);
+  res->final_chi2 = r.chi2;
+  res->elapsed_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  if (log && log_cap > 0) {
+    const int n = std::min(res->n_log, log_cap);
+    std::vector<DevLog> dl(n);
+    if (n > 0) HIP_TRY(hipMemcpy(dl.data(), h->log, sizeof(DevLog) * n, hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; i++) { log[i].it = dl[i].it; log[i].subit = dl[i].subit; log[i].chi2 = dl[i].chi2; log[i].mu = dl[i].mu; }
+  }
+  h->linearized = true;
+  h->stepped = true;
+  h->last_abs = r.abs_;
+  return NSN_OK;
+}
+
+extern "C" int nsn_lm_solve(nsn_problem* h, const nsn_lm_settings* s, nsn_lm_result* res, nsn_log_entry* log,
+                            int log_cap) {
+  if (!h || !s || !res) return fail(NSN_EINVAL, "null argument");
+  if (s->maxNumSubIt < 1) return fail(NSN_EINVAL, "maxNumSubIt must be >= 1");
+  HIP_TRY(hipSetDevice(h->device));
+  return h->prec == NSN_MIXED ? run_lm<float>(h, s, res, log, log_cap) : run_lm<double>(h, s, res, log, log_cap);
+}
+
+template <typename T>
+static int run_gn(nsn_problem* h, const nsn_gn_settings* s, nsn_lm_result* res, nsn_log_entry* log, int log_cap) {
+  DevCtl c = make_ctl(h->log_cap);
+  c.mode = 2;
+  c.maxNumIt = s->maxNumIt; c.minNumIt = s->minNumIt;
+  c.epsBVector = s->epsBVector; c.epsIncSquare = s->epsIncSquare; c.epsChi2 = s->epsChi2;
+  c.gn_rel = s->relLambda; c.gn_abs = s->absLambda;
+  *h->hctl = c;
+  int rc = push_ctl(h);
+  if (rc) return rc;
+  auto t0 = std::chrono::steady_clock::now();
+  seq_lin_full<T>(h, GATE_ALWAYS);
+  launch_gn_after_lin(h->ctl, h->log, h->st);
+  for (int k = 0; k < std::max(1, s->maxNumIt) + 2; k++) {
+    if ((rc = sync_ctl(h))) return rc;
+    if (h->hctl->done || h->hctl->wait_timeout) break;
+    seq_gn_iter<T>(h);
+  }
+  if ((rc = sync_ctl(h))) return rc;
+  auto t1 = std::chrono::steady_clock::now();
+  HIP_TRY(hipGetLastError());
+  if ((rc = check_device_errors(h))) return rc;
+  const DevCtl& r = *h->hctl;
+  res->reason = r.done ? r.reason : NSN_MAX_ITERATION_REACHED;
+  res->iterations = r.iter;
+  res->sub_iterations = r.total_solves;
+  res->n_log = std::min(r.n_log, h->log_cap);
+  res->final_chi2 = r.chi2;
+  res->elapsed_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  if (log && log_cap > 0) {
+    const int n = std::min(res->n_log, log_cap);
+    std::vector<DevLog> dl(n);
+    if (n > 0) HIP_TRY(hipMemcpy(dl.data(), h->log, sizeof(DevLog) * n, hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; i++) { log[i].it = dl[i].it; log[i].subit = dl[i].subit; log[i].chi2 = dl[i].chi2; log[i].mu = dl[i].mu; }
+  }
+  h->linearized = true;
+  h->stepped = true;
+  return NSN_OK;
+}
+
+extern "C" int nsn_gn_solve(nsn_problem* h, const nsn_gn_settings* s, nsn_lm_result* res, nsn_log_entry* log,
+                            int log_cap) {
+  if (!h || !s || !res) return fail(NSN_EINVAL, "null argument");
+  HIP_TRY(hipSetDevice(h->device));
+  return h->prec == NSN_MIXED ? run_gn<float>(h, s, res, log, log_cap) : run_gn<double>(h, s, res, log, log_cap);
+}

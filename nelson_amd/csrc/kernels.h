@@ -1,0 +1,121 @@
+// Device data views and kernel launch wrappers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "internal.h"
+
+namespace nsn {
+
+// Kernel gates: each conditional kernel first evaluates its gate against the
+// device control block and returns at once when it is closed, so a fixed
+// kernel sequence (and a captured graph of it) runs the LM loop's branches.
+enum Gate : int {
+  GATE_ALWAYS = 0,
+  GATE_NOT_DONE = 1,        // !done
+  GATE_NEED_LIN = 2,        // !done && need_lin
+  GATE_SOLVE_OK = 3,        // !done && solve_ok
+  GATE_ROLLBACK = 4,        // rollback_pending
+};
+
+// Read-only structure and state pointers of one BA problem on the device.
+struct BAView {
+  int C, P, E, NU, NV, nW;
+  double f, cx, cy, huber;
+  const int* cam_var;       // [C]  user camera -> variable index or -1
+  const int* pt_var;        // [P]  user point  -> variable index or -1
+  const int* var_cam_user;  // [NU]
+  const int* var_pt_user;   // [NV]
+  // edges in point-major order (user point, then camera)
+  const int* pt_ptr;        // [P+1]
+  const int* pe_cam;        // [E]
+  const double* pe_obs;     // [2E]
+  const double* pe_w;       // [E]
+  const int* pe_wblk;       // [E] W block or -1
+  // edges of variable cameras in camera-major order
+  const int* cam_ptr;       // [NU+1]
+  const int* ce_pt;         // [Ec]
+  const double* ce_obs;     // [2Ec]
+  const double* ce_w;       // [Ec]
+  // W blocks: point-major (blocks of variable point v are wptr[v]..wptr[v+1])
+  const int* wptr;          // [NV+1]
+  const int* w_cam;         // [nW] variable camera of the block
+  const int* cw_ptr;        // [NU+1] camera-major list of W blocks
+  const int* cw_idx;        // [nW]
+  double* cams;             // [12C] R (row-major) and t per camera
+  double* pts;              // [3P]
+};
+
+// Tiled lower storage of the reduced camera system S (plus the augmented
+// right-hand-side row at index n): tile (i, j), i >= j, of TB x TB values,
+// column-major inside, tiles packed column by column.
+struct TileView {
+  int n;        // order of S (6 NU)
+  int TB;       // tile size
+  int NT;       // tiles per dimension (covers rows 0..n inclusive)
+  int ntiles;   // NT (NT + 1) / 2
+};
+
+__host__ __device__ inline long long tile_index(int i, int j, int NT) {
+  return (long long)j * NT - (long long)j * (j - 1) / 2 + (i - j);
+}
+
+// reduction scratch layout (doubles), one slot per block of the producing kernel
+struct Scratch {
+  double* part_chi2;    // [nblk_pts]
+  double* part_xv;      // [2 * nblk_pts]  (incsq, den) of the points
+  double* part_xu;      // [2 * nblk_cams] (incsq, den) of the cameras
+  unsigned long long* maxes;  // [4] max|bU|, max|bV|, max|diag U|, max|diag V| as bit patterns
+  int nblk_pts, nblk_cams;
+};
+
+// ---- BA kernels (ba_kernels.hip), T = double (FP64) or float (mixed) ----
+template <typename T>
+void launch_lin_points(const BAView& v, bool hess, T* V, T* bV, T* W, const Scratch& s,
+                       DevCtl* ctl, int gate, hipStream_t st);
+template <typename T>
+void launch_lin_cams(const BAView& v, T* U, T* bU, const Scratch& s, DevCtl* ctl, int gate, hipStream_t st);
+void launch_reset_lin(const Scratch& s, DevCtl* ctl, int gate, hipStream_t st);
+void launch_finalize_lin(const Scratch& s, DevCtl* ctl, int gate, int full, hipStream_t st);
+template <typename T>
+void launch_vinv_y(const BAView& v, const T* V, const T* bV, const T* W, T* Vinv, T* Y, T* tW,
+                   DevCtl* ctl, int gate, hipStream_t st);
+template <typename T>
+void launch_s_diag(const BAView& v, const T* U, const T* bU, const T* Y, const T* W, const T* tW,
+                   T* Sd, T* bS, DevCtl* ctl, int gate, hipStream_t st);
+template <typename T>
+void launch_s_tiles(const TileView& tv, const int* tile_order, const int* tile_blk_ptr, const int* tile_blk,
+                    const int* sb_i, const int* sb_j, const int* sb_ptr, const int* sb_pairs,
+                    const T* Sd, const T* bS, const T* Y, const T* W, T* tiles, DevCtl* ctl,
+                    int gate, hipStream_t st);
+template <typename T>
+void launch_backsub(const BAView& v, const T* W, const T* Vinv, const T* bV, const double* xU,
+                    double* xV, const Scratch& s, DevCtl* ctl, int gate, hipStream_t st);
+template <typename T>
+void launch_xu_stats(int NU, const double* xU, const T* bU, const Scratch& s, DevCtl* ctl, int gate,
+                     hipStream_t st);
+void launch_step_finish(const Scratch& s, DevCtl* ctl, int gate, int lm_checks, hipStream_t st);
+void launch_oplus(const BAView& v, const double* xU, const double* xV, DevCtl* ctl, int gate,
+                  hipStream_t st);
+void launch_copy_state(const BAView& v, double* cams_dst, double* pts_dst, const double* cams_src,
+                       const double* pts_src, DevCtl* ctl, int gate, hipStream_t st);
+void launch_lm_after_lin(DevCtl* ctl, DevLog* log, hipStream_t st);
+void launch_lm_control(DevCtl* ctl, DevLog* log, hipStream_t st);
+void launch_gn_after_step(DevCtl* ctl, DevLog* log, hipStream_t st);
+void launch_gn_after_lin(DevCtl* ctl, DevLog* log, hipStream_t st);
+void launch_set_damping(DevCtl* ctl, double rel, double abs_, hipStream_t st);
+void launch_begin_step(DevCtl* ctl, int gate, hipStream_t st);
+template <typename T>
+void launch_zero(T* p, long long n, hipStream_t st);
+
+// ---- dense tile Cholesky and triangular solve (chol_kernels.hip) --------
+template <typename T>
+void launch_chol_reset(int* flags, int nflags, int* counters, DevCtl* ctl, int gate, hipStream_t st);
+template <typename T>
+void launch_chol(const TileView& tv, T* tiles, const int* tile_order, int* ready, int* counter,
+                 DevCtl* ctl, int gate, int max_wg, hipStream_t st);
+template <typename T>
+void launch_trsv_bwd(const TileView& tv, const T* tiles, double* xU, int* ready, int* counter,
+                     DevCtl* ctl, int gate, int max_wg, hipStream_t st);
+
+}  // namespace nsn

@@ -1,0 +1,341 @@
+"""Python mirror of nelson's problem / solver / driver interface over the C ABI.
+
+Names follow the reference: BAProblem plays a DoubleSection subclass with BA
+edges (nelson/DoubleSection.h:181-545), SolverCholeskySchur the Schur solver
+(nelson/SolverCholeskySchur.h:53-117), LevenbergMarquardt / GaussNewton the
+drivers (nelson/LevenbergMarquardt.h, nelson/GaussNewton.h).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import capi
+from .capi import check
+
+_D = C.POINTER(C.c_double)
+_I = C.POINTER(C.c_int)
+
+
+def _dp(a):
+    return a.ctypes.data_as(_D) if a is not None else None
+
+
+def _ip(a):
+    return a.ctypes.data_as(_I) if a is not None else None
+
+
+# ---------------------------------------------------------------------------
+# synthetic workloads
+# ---------------------------------------------------------------------------
+_BASE = dict(obs_sigma=1.0, outlier_frac=0.0, outlier_sigma=0.0, rot_sigma=0.01, trans_sigma=0.05,
+             point_sigma=0.1, focal=500.0, cx=320.0, cy=240.0)
+
+CONFIGS = {
+    # 10 cameras, 500 points, 4 observations per point, 20 LM iterations
+    "cfg1": dict(_BASE, n_cameras=10, n_points=500, k_min=4, k_max=4, layout=0, arc=0, seed=1, iterations=20),
+    # 1000 cameras on a ring, 200k points seen by 4-8 cameras of a 150-camera arc, 30 LM iterations
+    "cfg2": dict(_BASE, n_cameras=1000, n_points=200000, k_min=4, k_max=8, layout=1, arc=150, seed=2,
+                 iterations=30),
+    # 200 cameras, 20k points, 6 observations per point (differentiable BA)
+    "cfg5": dict(_BASE, n_cameras=200, n_points=20000, k_min=6, k_max=6, layout=1, arc=40, seed=5,
+                 iterations=50),
+}
+
+
+class _SplitMix64:
+    def __init__(self, seed):
+        self.s = seed & 0xFFFFFFFFFFFFFFFF
+
+    def next(self):
+        self.s = (self.s + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+        z = self.s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+        return z ^ (z >> 31)
+
+    def uniform_int(self, lo, hi):
+        u = (self.next() >> 11) * 2.0 ** -53
+        return min(hi, lo + int(u * (hi - lo + 1)))
+
+
+def window_config(w):
+    """Batched local-BA window w: cameras U{5..40}, points U{200..3000}, 3-6
+    observations per point, 5% outliers at 20 px, Huber delta 2 px."""
+    r = _SplitMix64(4000 + w)
+    nc = r.uniform_int(5, 40)
+    npt = r.uniform_int(200, 3000)
+    return dict(_BASE, n_cameras=nc, n_points=npt, k_min=3, k_max=6, layout=0, arc=0, seed=4000 + w,
+                outlier_frac=0.05, outlier_sigma=20.0, huber_delta=2.0, iterations=10)
+
+
+def generate_ba(cfg):
+    """Runs the library's seeded generator; returns a dict of numpy arrays."""
+    lib = capi.lib()
+    p = capi.GenParams()
+    for name, _ in capi.GenParams._fields_:
+        setattr(p, name, cfg[name])
+    nc, npt = cfg["n_cameras"], cfg["n_points"]
+    cap = npt * cfg["k_max"]
+    out = dict(cams_gt=np.zeros(6 * nc), pts_gt=np.zeros(3 * npt), cams0=np.zeros(6 * nc),
+               pts0=np.zeros(3 * npt), cam_idx=np.zeros(cap, np.int32), pt_idx=np.zeros(cap, np.int32),
+               obs=np.zeros(2 * cap))
+    ne = C.c_int(0)
+    check(lib.nsn_generate_ba(C.byref(p), cap, _dp(out["cams_gt"]), _dp(out["pts_gt"]), _dp(out["cams0"]),
+                              _dp(out["pts0"]), _ip(out["cam_idx"]), _ip(out["pt_idx"]), _dp(out["obs"]),
+                              C.byref(ne)))
+    e = ne.value
+    out["cam_idx"] = out["cam_idx"][:e].copy()
+    out["pt_idx"] = out["pt_idx"][:e].copy()
+    out["obs"] = out["obs"][:2 * e].copy()
+    out["n_cameras"], out["n_points"], out["n_edges"] = nc, npt, e
+    for k in ("focal", "cx", "cy"):
+        out[k] = cfg[k]
+    out["huber_delta"] = cfg.get("huber_delta", 0.0)
+    return out
+
+
+# ---------------------------------------------------------------------------
+# problem (DoubleSection + BA edges)
+# ---------------------------------------------------------------------------
+class BAProblem:
+    """A BA problem on one GPU: cameras (angle-axis, translation) and points.
+
+    Camera 0 and point 0 are fixed by default, the gauge of the reference's
+    BA sample (nelson/samples/BASchurPerfomances.cpp:245-246).
+    """
+
+    def __init__(self, n_cameras, n_points, cam_idx, pt_idx, obs, focal, cx, cy, weight=None, cam_fixed=None,
+                 pt_fixed=None, huber_delta=0.0, precision="fp64", device=0):
+        self._lib = capi.lib()
+        self.n_cameras, self.n_points = int(n_cameras), int(n_points)
+        self.cam_idx = np.ascontiguousarray(cam_idx, np.int32)
+        self.pt_idx = np.ascontiguousarray(pt_idx, np.int32)
+        self.obs = np.ascontiguousarray(obs, np.float64).reshape(-1)
+        self.n_edges = len(self.cam_idx)
+        if cam_fixed is None:
+            cam_fixed = np.zeros(self.n_cameras, np.uint8)
+            cam_fixed[0] = 1
+        if pt_fixed is None:
+            pt_fixed = np.zeros(self.n_points, np.uint8)
+            pt_fixed[0] = 1
+        self.cam_fixed = np.ascontiguousarray(cam_fixed, np.uint8)
+        self.pt_fixed = np.ascontiguousarray(pt_fixed, np.uint8)
+        self.weight = None if weight is None else np.ascontiguousarray(weight, np.float64)
+        d = capi.ProblemDesc()
+        d.n_cameras, d.n_points, d.n_edges = self.n_cameras, self.n_points, self.n_edges
+        d.cam_idx, d.pt_idx, d.obs = _ip(self.cam_idx), _ip(self.pt_idx), _dp(self.obs)
+        d.weight = _dp(self.weight)
+        d.cam_fixed = self.cam_fixed.ctypes.data_as(C.POINTER(C.c_ubyte))
+        d.pt_fixed = self.pt_fixed.ctypes.data_as(C.POINTER(C.c_ubyte))
+        d.focal, d.cx, d.cy, d.huber_delta = focal, cx, cy, huber_delta
+        d.precision = capi.NSN_MIXED if precision == "mixed" else capi.NSN_FP64
+        d.device = device
+        h = C.c_void_p()
+        check(self._lib.nsn_problem_create(C.byref(d), C.byref(h)))
+        self._h = h
+        info = capi.ProblemInfo()
+        check(self._lib.nsn_problem_info_get(self._h, C.byref(info)))
+        self.NU, self.NV, self.nW = info.n_var_cameras, info.n_var_points, info.n_wblocks
+        self.n_s, self.n_s_blocks, self.n_s_pairs = info.n_s, info.n_s_blocks, info.n_s_pairs
+        self._chi2 = None
+
+    @classmethod
+    def from_generated(cls, g, **kw):
+        p = cls(g["n_cameras"], g["n_points"], g["cam_idx"], g["pt_idx"], g["obs"], g["focal"], g["cx"],
+                g["cy"], huber_delta=kw.pop("huber_delta", g.get("huber_delta", 0.0)), **kw)
+        p.set_state(g["cams0"], g["pts0"])
+        return p
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.nsn_problem_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    # parameters ------------------------------------------------------------
+    def set_state(self, cams, pts):
+        cams = np.ascontiguousarray(cams, np.float64)
+        pts = np.ascontiguousarray(pts, np.float64)
+        check(self._lib.nsn_set_state(self._h, _dp(cams), _dp(pts)))
+
+    def set_state_rt(self, cams_rt, pts):
+        cams_rt = np.ascontiguousarray(cams_rt, np.float64)
+        pts = np.ascontiguousarray(pts, np.float64)
+        check(self._lib.nsn_set_state_rt(self._h, _dp(cams_rt), _dp(pts)))
+
+    def get_state(self):
+        cams = np.zeros(6 * self.n_cameras)
+        pts = np.zeros(3 * self.n_points)
+        check(self._lib.nsn_get_state(self._h, _dp(cams), _dp(pts)))
+        return cams, pts
+
+    def get_state_rt(self):
+        cams = np.zeros(12 * self.n_cameras)
+        pts = np.zeros(3 * self.n_points)
+        check(self._lib.nsn_get_state_rt(self._h, _dp(cams), _dp(pts)))
+        return cams, pts
+
+    def backupSolution(self):
+        check(self._lib.nsn_backup(self._h))
+
+    def rollbackSolution(self):
+        check(self._lib.nsn_rollback(self._h))
+
+    # DoubleSection::update(bool) ---------------------------------------------
+    def update(self, hessian=True):
+        c = C.c_double(0)
+        check(self._lib.nsn_linearize(self._h, 1 if hessian else 0, C.byref(c)))
+        self._chi2 = c.value
+        return c.value
+
+    def chi2(self):
+        return self._chi2
+
+    def max_abs_b(self):
+        v = C.c_double(0)
+        check(self._lib.nsn_max_abs_b(self._h, C.byref(v)))
+        return v.value
+
+    def export_hessian(self):
+        NU, NV, nW = self.NU, self.NV, self.nW
+        U = np.zeros((NU, 6, 6)); V = np.zeros((NV, 3, 3)); W = np.zeros((nW, 6, 3))
+        Wi = np.zeros(nW, np.int32); Wj = np.zeros(nW, np.int32)
+        bU = np.zeros(6 * NU); bV = np.zeros(3 * NV); chi2 = C.c_double(0)
+        check(self._lib.nsn_export_hessian(self._h, _dp(U), _dp(V), _dp(W), _ip(Wi), _ip(Wj), _dp(bU), _dp(bV),
+                                           C.byref(chi2)))
+        return dict(U=U, V=V, W=W, Wi=Wi, Wj=Wj, bU=bU, bV=bV, chi2=chi2.value)
+
+    def import_hessian(self, U, V, W, bU, bV, chi2):
+        arrs = [np.ascontiguousarray(a, np.float64) for a in (U, V, W, bU, bV)]
+        check(self._lib.nsn_import_hessian(self._h, *[_dp(a) for a in arrs], float(chi2)))
+
+
+class SolverCholeskySchur:
+    """SolverCholeskySchur (nelson/SolverCholeskySchur.h:53-117) on the GPU."""
+
+    def __init__(self):
+        self._p = None
+        self._inc = None
+        self._ok = False
+
+    def init(self, problem):
+        self._p = problem
+        check(capi.lib().nsn_schur_init(problem.handle))
+
+    def maxAbsHDiag(self):
+        v = C.c_double(0)
+        check(capi.lib().nsn_max_abs_h_diag(self._p.handle, C.byref(v)))
+        return v.value
+
+    def computeIncrement(self, relLambda, absLambda):
+        p = self._p
+        dxu = np.zeros(6 * p.NU)
+        dxv = np.zeros(3 * p.NV)
+        sq = C.c_double(0)
+        rc = capi.lib().nsn_schur_step(p.handle, relLambda, absLambda, _dp(dxu), _dp(dxv), C.byref(sq))
+        check(rc, allow=(capi.NSN_OK, capi.NSN_NOT_PD))
+        self._inc = (dxu, dxv)
+        self._sq = sq.value
+        self._ok = rc == capi.NSN_OK
+        return self._ok
+
+    def incrementVector(self):
+        return self._inc
+
+    def incrementVectorSquaredNorm(self):
+        return self._sq
+
+    def phase_times_ms(self):
+        ms = np.zeros(5)
+        check(capi.lib().nsn_phase_times(self._p.handle, _dp(ms), 5))
+        return dict(zip(["vinv_wvinv", "s_assembly", "factorize", "solves", "backsub"], ms))
+
+
+class LevenbergMarquardtSettings(dict):
+    """LevenbergMarquardtSettings (nelson/LevenbergMarquardt.h:18-34) with the
+    defaults of nelson/LevenbergMarquardt.cpp:6-21."""
+
+    def __init__(self, **kw):
+        s = capi.LMSettings()
+        capi.lib().nsn_lm_default_settings(C.byref(s))
+        super().__init__({n: getattr(s, n) for n, _ in capi.LMSettings._fields_})
+        self.update(kw)
+
+    def to_c(self):
+        s = capi.LMSettings()
+        for n, _ in capi.LMSettings._fields_:
+            setattr(s, n, self[n])
+        return s
+
+
+class GaussNewtonSettings(dict):
+    """GaussNewtonSettings (nelson/GaussNewton.h:18-30), defaults GaussNewton.cpp:12-22."""
+
+    def __init__(self, **kw):
+        s = capi.GNSettings()
+        capi.lib().nsn_gn_default_settings(C.byref(s))
+        super().__init__({n: getattr(s, n) for n, _ in capi.GNSettings._fields_})
+        self.update(kw)
+
+    def to_c(self):
+        s = capi.GNSettings()
+        for n, _ in capi.GNSettings._fields_:
+            setattr(s, n, self[n])
+        return s
+
+
+class _Driver:
+    _settings_cls = None
+    _fn = None
+
+    def __init__(self, settings=None):
+        self._settings = settings or self._settings_cls()
+        self._res = None
+        self._log = []
+
+    def settings(self):
+        return self._settings
+
+    def solve(self, problem):
+        s = self._settings.to_c()
+        res = capi.LMResult()
+        cap = 4096
+        log = (capi.LogEntry * cap)()
+        check(getattr(capi.lib(), self._fn)(problem.handle, C.byref(s), C.byref(res), log, cap))
+        self._res = res
+        self._log = [(log[i].it, log[i].subit, log[i].chi2, log[i].mu) for i in range(min(res.n_log, cap))]
+        problem._chi2 = res.final_chi2
+        return capi.TERMINATION_REASONS[res.reason]
+
+    def stats(self):
+        """[(it, subit, chi2, mu)], chi2 == -1 on a solver failure (LevenbergMarquardtStats)."""
+        return list(self._log)
+
+    def numIterations(self):
+        return self._res.iterations
+
+    def result(self):
+        r = self._res
+        return dict(reason=capi.TERMINATION_REASONS[r.reason], iterations=r.iterations,
+                    sub_iterations=r.sub_iterations, final_chi2=r.final_chi2, elapsed_ms=r.elapsed_ms)
+
+
+class LevenbergMarquardt(_Driver):
+    """LevenbergMarquardt<SolverCholeskySchur>::solve with the loop on the device."""
+    _settings_cls = LevenbergMarquardtSettings
+    _fn = "nsn_lm_solve"
+
+
+class GaussNewton(_Driver):
+    """GaussNewton<SolverCholeskySchur>::solve."""
+    _settings_cls = GaussNewtonSettings
+    _fn = "nsn_gn_solve"

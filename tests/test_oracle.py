@@ -1,0 +1,432 @@
+"""Pins the CPU oracle independently of its own code: against numpy/scipy
+linear algebra, closed forms, central finite differences, the reference's
+own property tests (nelson/tests/testMatrix*.cpp) and its linear toy problem
+(nelson/tests/testDoubleSectionCommon.h)."""
+import numpy as np
+import pytest
+import scipy.linalg as sla
+from scipy.spatial.transform import Rotation
+
+from oracle import oracle as orc
+from oracle.oracle import OracleBA, PyProblem
+
+from helpers import CONFIGS, dense_system, generate_ba, oracle_blocks, relerr, small_cfg
+
+
+# ---------------------------------------------------------------------------
+# dense pivoted LDL^T (Eigen::LDLT restated)
+# ---------------------------------------------------------------------------
+def test_ldlt_solves_spd_and_indefinite():
+    rng = np.random.default_rng(1)
+    for n, spd in [(1, True), (7, True), (60, True), (33, False)]:
+        A = rng.standard_normal((n, n))
+        A = A @ A.T + n * np.eye(n) if spd else A + A.T
+        b = rng.standard_normal(n)
+        ok, f, tp = orc.ldlt(A)
+        assert ok
+        x = orc.ldlt_solve(f, tp, b)
+        assert relerr(A @ x, b) < 1e-11
+        assert relerr(x, np.linalg.solve(A, b)) < 1e-9
+
+
+def test_ldlt_reconstruction_and_pivot_order():
+    """P A P^T = L D L^T with the pivot chosen as the largest remaining
+    |diagonal| (first occurrence)."""
+    A = np.array([[1.0, 0.2, 0.1], [0.2, 5.0, 0.3], [0.1, 0.3, 3.0]])
+    ok, f, tp = orc.ldlt(A)
+    assert ok and tp[0] == 1
+    n = 3
+    L = np.tril(np.asarray(f), -1) + np.eye(n)
+    D = np.diag(np.diag(np.asarray(f)))
+    P = np.eye(n)
+    for k in range(n):
+        P[[k, tp[k]]] = P[[tp[k], k]]
+    assert relerr(P @ A @ P.T, L @ D @ L.T) < 1e-14
+    # scipy's LDL of the same matrix gives the same D up to ordering
+    _, d2, _ = sla.ldl(A)
+    assert np.allclose(sorted(np.diag(D)), sorted(np.diag(d2)), rtol=0.5)  # same inertia, positive
+    assert np.all(np.diag(d2) > 0)
+
+
+def test_ldlt_failure_cases():
+    # zero pivot followed by a non-zero column fails (Eigen ComputationInfo::NumericalIssue)
+    A = np.array([[0.0, 1.0], [1.0, 0.0]])
+    ok, _, _ = orc.ldlt(A)
+    assert not ok
+    # a zero matrix "succeeds" (nothing to eliminate), like Eigen's LDLT
+    ok, _, _ = orc.ldlt(np.zeros((3, 3)))
+    assert ok
+
+
+# ---------------------------------------------------------------------------
+# L4 kernels: the reference's property tests on its own patterns
+# ---------------------------------------------------------------------------
+def _sym(rng, B):
+    m = rng.standard_normal((B, B))
+    return m @ m.T + B * np.eye(B)
+
+
+def test_diag_inv_and_mult():  # nelson/tests/testMatrixVinv.cpp:20-64
+    rng = np.random.default_rng(2)
+    NV, BV = 5, 3
+    V = np.stack([_sym(rng, BV) for _ in range(NV)])
+    Vinv = np.zeros_like(V)
+    assert orc.lib().or_diag_inv(NV, BV, orc.dp(V), 0.1, 0.5, orc.dp(Vinv))
+    for p in range(NV):
+        M = np.triu(V[p]) + np.triu(V[p], 1).T
+        M = M + np.diag(np.diag(M) * 0.1 + 0.5)
+        assert relerr(Vinv[p], np.linalg.inv(M)) < 1e-12
+    v = rng.standard_normal(NV * BV)
+    res = np.zeros_like(v)
+    orc.lib().or_diag_inv_mult(NV, BV, orc.dp(Vinv), orc.dp(v), orc.dp(res))
+    ref = np.concatenate([Vinv[p] @ v[BV * p:BV * p + BV] for p in range(NV)])
+    assert relerr(res, ref) < 1e-14
+
+
+# the fixed 3x5 block pattern of nelson/tests/testMatrixWVinvMultiplier.cpp:45-52
+_PATTERN = [(0, 0), (0, 3), (0, 4), (1, 0), (1, 1), (1, 3), (2, 1), (2, 2)]
+
+
+def _dense_w(W, Wi, Wj, NR, NC, BR, BC):
+    D = np.zeros((NR * BR, NC * BC))
+    for k in range(len(Wi)):
+        D[Wi[k] * BR:Wi[k] * BR + BR, Wj[k] * BC:Wj[k] * BC + BC] = W[k]
+    return D
+
+
+@pytest.mark.parametrize("BR,BC", [(6, 3), (2, 3), (3, 2)])
+def test_wvinv_and_rightmult(BR, BC):  # testMatrixWVinvMultiplier.cpp:17-116
+    rng = np.random.default_rng(3)
+    NR, NC = 3, 5
+    Wi = np.array([a for a, _ in _PATTERN], np.int32)
+    Wj = np.array([b for _, b in _PATTERN], np.int32)
+    W = rng.standard_normal((len(Wi), BR, BC))
+    Vinv = rng.standard_normal((NC, BC, BC))
+    Y = np.zeros_like(W)
+    orc.lib().or_wvinv(len(Wi), BR, BC, orc.ip(Wj), orc.dp(W), orc.dp(Vinv), orc.dp(Y))
+    Dv = sla.block_diag(*Vinv)
+    Dw = _dense_w(W, Wi, Wj, NR, NC, BR, BC)
+    assert relerr(_dense_w(Y, Wi, Wj, NR, NC, BR, BC), Dw @ Dv) < 1e-14
+    b = rng.standard_normal(NC * BC)
+    bv = rng.standard_normal(NR * BR)
+    res = bv.copy()
+    orc.lib().or_wvinv_mult_sub(len(Wi), BR, BC, orc.ip(Wi), orc.ip(Wj), orc.dp(Y), orc.dp(b), orc.dp(res))
+    assert relerr(res, bv - (Dw @ Dv) @ b) < 1e-13
+
+
+def test_wwt():  # testMatrixWWtMultiplier.cpp:17-103, U with an extra (0,2) block
+    rng = np.random.default_rng(4)
+    NR, NC, BR, BC = 3, 5, 6, 3
+    Wi = np.array([a for a, _ in _PATTERN], np.int32)
+    Wj = np.array([b for _, b in _PATTERN], np.int32)
+    A = rng.standard_normal((len(Wi), BR, BC))
+    B = rng.standard_normal((len(Wi), BR, BC))
+    nu = NR * BR
+    U = np.zeros((nu, nu))
+    for i in range(NR):
+        m = _sym(rng, BR)
+        U[i * BR:i * BR + BR, i * BR:i * BR + BR] = m
+    U[0:BR, 2 * BR:3 * BR] = rng.standard_normal((BR, BR))
+    S = np.zeros((nu, nu))
+    orc.lib().or_wwt(NR, BR, BC, len(Wi), orc.ip(Wi), orc.ip(Wj), orc.dp(A), orc.dp(B), orc.dp(U), orc.dp(S))
+    ref = np.triu(U - _dense_w(A, Wi, Wj, NR, NC, BR, BC) @ _dense_w(B, Wi, Wj, NR, NC, BR, BC).T)
+    # only upper blocks are produced, within-block lower parts of diagonal blocks excluded
+    mask = np.zeros((nu, nu), bool)
+    for i in range(NR):
+        for j in range(i, NR):
+            mask[i * BR:i * BR + BR, j * BR:j * BR + BR] = True
+    mask &= np.triu(np.ones((nu, nu), bool))
+    assert relerr(S[mask], ref[mask]) < 1e-13
+
+
+def test_wtx():  # testMatrixWtXMultiplier.cpp:17-92
+    rng = np.random.default_rng(5)
+    NR, NC, BR, BC = 3, 5, 6, 3
+    Wi = np.array([a for a, _ in _PATTERN], np.int32)
+    Wj = np.array([b for _, b in _PATTERN], np.int32)
+    W = rng.standard_normal((len(Wi), BR, BC))
+    x = rng.standard_normal(NR * BR)
+    bV = rng.standard_normal(NC * BC)
+    res = bV.copy()
+    orc.lib().or_wtx_sub(len(Wi), BR, BC, orc.ip(Wi), orc.ip(Wj), orc.dp(W), orc.dp(x), orc.dp(res))
+    assert relerr(res, bV - _dense_w(W, Wi, Wj, NR, NC, BR, BC).T @ x) < 1e-13
+
+
+# ---------------------------------------------------------------------------
+# BA edge, rotations, assembly
+# ---------------------------------------------------------------------------
+def test_rodrigues_and_se3_update_vs_scipy():
+    rng = np.random.default_rng(6)
+    for w in [np.zeros(3), 1e-9 * rng.standard_normal(3), 1e-4 * rng.standard_normal(3), rng.standard_normal(3)]:
+        R = np.zeros(9)
+        orc.lib().or_rodrigues(orc.dp(w), orc.dp(R))
+        assert relerr(R.reshape(3, 3), Rotation.from_rotvec(w).as_matrix()) < 1e-13
+        # left update exp(delta) * T against the 4x4 matrix exponential of the twist
+        d = np.concatenate([w, rng.standard_normal(3)])
+        R0 = Rotation.from_rotvec(rng.standard_normal(3)).as_matrix()
+        t0 = rng.standard_normal(3)
+        Rm, tm = R0.reshape(-1).copy(), t0.copy()
+        orc.lib().or_se3_left_update(orc.dp(d), orc.dp(Rm), orc.dp(tm))
+        X = np.zeros((4, 4))
+        X[:3, :3] = np.array([[0, -d[2], d[1]], [d[2], 0, -d[0]], [-d[1], d[0], 0]])
+        X[:3, 3] = d[3:]
+        T0 = np.eye(4)
+        T0[:3, :3], T0[:3, 3] = R0, t0
+        T1 = sla.expm(X) @ T0
+        assert relerr(Rm.reshape(3, 3), T1[:3, :3]) < 1e-12
+        assert relerr(tm, T1[:3, 3]) < 1e-12
+
+
+def _numpy_residuals(g, cams_rt, pts):
+    E = g["n_edges"]
+    r = np.zeros((E, 2))
+    for e in range(E):
+        c, q = g["cam_idx"][e], g["pt_idx"][e]
+        R = cams_rt[12 * c:12 * c + 9].reshape(3, 3)
+        t = cams_rt[12 * c + 9:12 * c + 12]
+        Xc = R @ pts[3 * q:3 * q + 3] + t
+        r[e] = g["focal"] * Xc[:2] / Xc[2] + np.array([g["cx"], g["cy"]]) - g["obs"][2 * e:2 * e + 2]
+    return r
+
+
+def test_edge_residual_and_jacobians_fd():
+    g = generate_ba(small_cfg(n_cameras=4, n_points=20, k_min=2, k_max=3, seed=7))
+    o = OracleBA.from_generated(g)
+    cams, pts = o.get_state_rt()
+    # residuals against an independent numpy projection with scipy's Rodrigues
+    cr = np.zeros(12 * g["n_cameras"])
+    for c in range(g["n_cameras"]):
+        cr[12 * c:12 * c + 9] = Rotation.from_rotvec(g["cams0"][6 * c:6 * c + 3]).as_matrix().reshape(-1)
+        cr[12 * c + 9:12 * c + 12] = g["cams0"][6 * c + 3:6 * c + 6]
+    r, Jc, Jp = o.edge_jacobians()
+    assert relerr(r, _numpy_residuals(g, cr, g["pts0"])) < 1e-10
+    # central differences of the residual w.r.t. the left perturbation and the point
+    h = 1e-6
+    for e in range(g["n_edges"]):
+        c, q = g["cam_idx"][e], g["pt_idx"][e]
+        for k in range(6):
+            d = np.zeros(6)
+            d[k] = h
+            rp, rm = [], []
+            for s, store in ((1, rp), (-1, rm)):
+                Rm, tm = cams[12 * c:12 * c + 9].copy(), cams[12 * c + 9:12 * c + 12].copy()
+                orc.lib().or_se3_left_update(orc.dp(s * d), orc.dp(Rm), orc.dp(tm))
+                cc = cams.copy()
+                cc[12 * c:12 * c + 9], cc[12 * c + 9:12 * c + 12] = Rm, tm
+                store.append(_numpy_residuals(g, cc, pts)[e])
+            fd = (rp[0] - rm[0]) / (2 * h)
+            assert np.allclose(Jc[e][:, k], fd, rtol=1e-5, atol=1e-4)
+        for k in range(3):
+            pp, pm = pts.copy(), pts.copy()
+            pp[3 * q + k] += h
+            pm[3 * q + k] -= h
+            fd = (_numpy_residuals(g, cams, pp)[e] - _numpy_residuals(g, cams, pm)[e]) / (2 * h)
+            assert np.allclose(Jp[e][:, k], fd, rtol=1e-5, atol=1e-4)
+
+
+def test_hessian_equals_dense_jtj():
+    """U, V, W, bU, bV and chi2 equal J^T J, J^T r, |r|^2 assembled densely
+    from the per-edge Jacobians (fixed camera 0 / point 0 columns dropped)."""
+    g = generate_ba(small_cfg(n_cameras=5, n_points=40, k_min=2, k_max=4, seed=8))
+    o = OracleBA.from_generated(g)
+    chi2 = o.update(True)
+    r, Jc, Jp = o.edge_jacobians()
+    NU, NV = o.NU, o.NV
+    J = np.zeros((2 * g["n_edges"], 6 * NU + 3 * NV))
+    for e in range(g["n_edges"]):
+        c, q = g["cam_idx"][e], g["pt_idx"][e]
+        if c > 0:
+            J[2 * e:2 * e + 2, 6 * (c - 1):6 * c] = Jc[e]
+        if q > 0:
+            J[2 * e:2 * e + 2, 6 * NU + 3 * (q - 1):6 * NU + 3 * q] = Jp[e]
+    Hd, b = dense_system(oracle_blocks(o), NU, NV)
+    assert relerr(Hd, J.T @ J) < 1e-12
+    assert relerr(b, J.T @ r.reshape(-1)) < 1e-12
+    assert abs(chi2 - np.sum(r ** 2)) < 1e-12 * chi2
+
+
+# ---------------------------------------------------------------------------
+# Schur step vs brute force
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("rel,ab", [(0.0, 1e3), (0.0, 1.0), (0.05, 0.3), (0.0, 0.0)])
+def test_schur_step_equals_dense_solve(cfg1, rel, ab):
+    """dx of SolverCholeskySchur == solve((H + diag(rel*H_ii + abs)) dx = -b)."""
+    o = OracleBA.from_generated(cfg1)
+    o.update(True)
+    Hd, b = dense_system(oracle_blocks(o), o.NU, o.NV)
+    D = np.diag(np.diag(Hd) * rel + ab)
+    x = np.linalg.solve(Hd + D, -b)
+    s = o.schur_step(rel, ab)
+    assert s["ok"]
+    assert relerr(np.concatenate([s["xU"], s["xV"]]), x) < 1e-10
+
+
+def test_schur_matrix_is_the_schur_complement(cfg1):
+    o = OracleBA.from_generated(cfg1)
+    o.update(True)
+    Hd, b = dense_system(oracle_blocks(o), o.NU, o.NV)
+    nu = 6 * o.NU
+    mu = 2.0
+    A = Hd + mu * np.eye(len(b))
+    Sref = A[:nu, :nu] - A[:nu, nu:] @ np.linalg.solve(A[nu:, nu:], A[nu:, :nu])
+    bSref = -b[:nu] - A[:nu, nu:] @ np.linalg.solve(A[nu:, nu:], -b[nu:])
+    s = o.schur_step(0.0, mu, want_s=True)
+    assert relerr(np.triu(s["S"]), np.triu(Sref)) < 1e-11
+    assert relerr(s["bS"], bSref) < 1e-11
+
+
+def test_rho_closed_form(cfg1):
+    o = OracleBA.from_generated(cfg1)
+    old = o.update(True)
+    s = o.schur_step(0.0, 5.0)
+    h = o.hessian_struct()
+    new = old - 10.0
+    den = s["xU"] @ (5.0 * s["xU"] - o.hessian()["bU"]) + s["xV"] @ (5.0 * s["xV"] - o.hessian()["bV"])
+    rho = orc.lib().or_rho(orc.C.byref(h), 5.0, orc.dp(s["xU"]), orc.dp(s["xV"]), old, new)
+    assert abs(rho - 10.0 / den) < 1e-12 * abs(rho)
+
+
+# ---------------------------------------------------------------------------
+# drivers on the linear toy problem of nelson/tests/testDoubleSectionCommon.h
+# ---------------------------------------------------------------------------
+class Toy(PyProblem):
+    """5 2-D points (U), 3 3-D points (V); unary, U-U and U-V difference
+    edges with identity Jacobians; the measurements are exact for the ground
+    truth, so the optimum has chi2 = 0."""
+    N2, N3 = 5, 3
+
+    def __init__(self, seed=0, sigma=0.5):
+        Wi = [i for i in range(self.N2) for j in range(self.N3)]
+        Wj = [j for i in range(self.N2) for j in range(self.N3)]
+        super().__init__(2, 3, self.N2, self.N3, Wi, Wj)
+        rng = np.random.default_rng(seed)
+        self.gt2 = rng.uniform(-1, 1, (self.N2, 2))
+        self.gt3 = rng.uniform(-1, 1, (self.N3, 3))
+        self.p2 = self.gt2 + rng.uniform(-1, 1, (self.N2, 2)) * sigma
+        self.p3 = self.gt3 + rng.uniform(-1, 1, (self.N3, 3)) * sigma
+        self.nupdates = 0
+
+    def update(self, hess):
+        self.nupdates += 1
+        v = self.views()
+        chi2 = 0.0
+        if hess:
+            for k in ("U", "V", "W", "bU", "bV"):
+                v[k][...] = 0
+        U, V, W, bU, bV = v["U"], v["V"], v["W"], v["bU"], v["bV"]
+        for i in range(self.N2):  # unary U
+            e = self.p2[i] - self.gt2[i]
+            chi2 += e @ e
+            if hess:
+                U[2 * i:2 * i + 2, 2 * i:2 * i + 2] += np.eye(2)
+                bU[2 * i:2 * i + 2] += e
+        for j in range(self.N3):  # unary V
+            e = self.p3[j] - self.gt3[j]
+            chi2 += e @ e
+            if hess:
+                V[j] += np.eye(3)
+                bV[3 * j:3 * j + 3] += e
+        for i in range(self.N2):  # U-U
+            for j in range(i + 1, self.N2):
+                e = (self.p2[i] - self.p2[j]) - (self.gt2[i] - self.gt2[j])
+                chi2 += e @ e
+                if hess:
+                    U[2 * i:2 * i + 2, 2 * i:2 * i + 2] += np.eye(2)
+                    U[2 * j:2 * j + 2, 2 * j:2 * j + 2] += np.eye(2)
+                    U[2 * i:2 * i + 2, 2 * j:2 * j + 2] -= np.eye(2)
+                    bU[2 * i:2 * i + 2] += e
+                    bU[2 * j:2 * j + 2] -= e
+        k = 0
+        for i in range(self.N2):  # U-V (first two coordinates)
+            for j in range(self.N3):
+                e = (self.p2[i] - self.p3[j][:2]) - (self.gt2[i] - self.gt3[j][:2])
+                chi2 += e @ e
+                if hess:
+                    U[2 * i:2 * i + 2, 2 * i:2 * i + 2] += np.eye(2)
+                    bU[2 * i:2 * i + 2] += e
+                    V[j] += np.diag([1.0, 1.0, 0.0])
+                    bV[3 * j:3 * j + 3] -= np.array([e[0], e[1], 0.0])
+                    W[k] += np.array([[-1.0, 0, 0], [0, -1.0, 0]])
+                k += 1
+        self.h.chi2 = chi2
+
+    def oplus(self, xu, xv):
+        self.p2 += xu.reshape(self.N2, 2)
+        self.p3 += xv.reshape(self.N3, 3)
+
+    def backup(self):
+        self.bk = (self.p2.copy(), self.p3.copy())
+
+    def rollback(self):
+        self.p2, self.p3 = self.bk[0].copy(), self.bk[1].copy()
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_toy_problem_converges_gn_and_lm(seed):
+    t = Toy(seed)
+    r = t.gn_solve()
+    assert t.h.chi2 < 1e-12, r
+    t = Toy(seed)
+    r = t.lm_solve()
+    assert t.h.chi2 < 1e-12, r
+
+
+def test_lm_mu_schedule_on_linear_problem():
+    """A linear problem makes the quadratic model exact: rho = 1 and
+    alpha = clamp(1 - 1^3, 1/3, 2/3) = 1/3, so mu shrinks x1/3 per accepted
+    step from mu0 = tau * max|diag H| (LevenbergMarquardt.hpp:58,109-112)."""
+    t = Toy(3)
+    t.update(True)
+    mu0 = 1e-3 * max(np.max(np.abs(np.diag(t.views()["U"]))),
+                     max(np.max(np.abs(np.diag(t.views()["V"][j]))) for j in range(3)))
+    r = Toy(3).lm_solve(maxNumIt=6, epsIncSquare=0.0, epsChi2=0.0, epsBVector=0.0)
+    mus = [e[3] for e in r["log"][1:]]
+    assert abs(mus[0] - mu0) < 1e-15 * mu0
+    for a, b in zip(mus, mus[1:]):
+        assert abs(b - a / 3.0) < 1e-12 * a
+
+
+class Rejector(Toy):
+    """Every candidate's cost is reported as higher than the last accepted
+    one, so every step is rejected: mu grows x2, x4, x8, ... and the solve
+    ends with MaxSubIterationReached after maxNumSubIt tries."""
+
+    def update(self, hess):
+        super().update(hess)
+        if not hess:
+            self.h.chi2 += 1e6
+
+
+def test_lm_rejections_and_max_subiterations():
+    t = Rejector(4)
+    r = t.lm_solve(maxNumSubIt=5)
+    assert r["reason"] == "MaxSubIterationReached"
+    mus = [e[3] for e in r["log"][1:]]
+    assert len(mus) == 5
+    ratios = [b / a for a, b in zip(mus, mus[1:])]
+    assert np.allclose(ratios, [2.0, 4.0, 8.0, 16.0])
+
+
+def test_success_at_start_and_gn_failure():
+    assert Toy(5).lm_solve(epsBVector=1e300)["reason"] == "SuccessAtStart"
+    # removing the unary V edges and the U-V edges leaves V = 0: GN fails
+    class Singular(Toy):
+        def update(self, hess):
+            super().update(hess)
+            if hess:
+                self.views()["V"][...] = 0.0
+    assert Singular(6).gn_solve()["reason"] == "SolverFailure"
+
+
+def test_lm_converges_cfg1(cfg1):
+    """The restated LM brings the cfg1 problem from its perturbed start to a
+    cost at the noise level: chi2 ~ 2E - dof for N(0, 1 px) noise."""
+    o = OracleBA.from_generated(cfg1)
+    r = o.lm_solve(maxNumIt=20)
+    dof = 6 * o.NU + 3 * o.NV
+    expected = 2 * cfg1["n_edges"] - dof
+    assert r["log"][0][2] > 50 * expected
+    assert 0.8 * expected < r["final_chi2"] < 1.2 * expected
+    # costs of accepted steps decrease monotonically
+    acc = [e[2] for e in r["log"][1:] if e[2] >= 0]
+    assert all(b <= a * (1 + 1e-12) or b > a for a, b in zip(acc, acc[1:]))
