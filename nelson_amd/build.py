@@ -3,6 +3,7 @@
 Usage: python -m nelson_amd.build   (or nelson_amd.build.build() from Python)
 """
 import os
+import re
 import subprocess
 import sys
 from concurrent.futures import ThreadPoolExecutor
@@ -23,11 +24,32 @@ HIP_FLAGS = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", 
              "-Wno-unused-command-line-argument"]
 
 
+# a workgroup may not rely on more than 64 KiB of LDS (the kernels are sized
+# to that budget; the resource report below enforces it)
+LDS_LIMIT = 65536
+
+
 def _run(cmd):
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     if r.returncode != 0:
         raise RuntimeError("build step failed:\n" + " ".join(cmd) + "\n" + r.stdout)
     return r.stdout
+
+
+def _check_resources(src, out):
+    """Parses -Rpass-analysis=kernel-resource-usage remarks: LDS within
+    LDS_LIMIT and no scratch (register spills) in any kernel."""
+    fn = None
+    for line in out.splitlines():
+        m = re.search(r"Function Name: (\S+)", line)
+        if m:
+            fn = m.group(1)
+        m = re.search(r"LDS Size \[bytes\]: (\d+)", line)
+        if m and int(m.group(1)) > LDS_LIMIT:
+            raise RuntimeError(f"{src}: kernel {fn} uses {m.group(1)} bytes of LDS (> {LDS_LIMIT})")
+        m = re.search(r"ScratchSize \[bytes/lane\]: (\d+)", line)
+        if m and int(m.group(1)) > 0:
+            raise RuntimeError(f"{src}: kernel {fn} spills ({m.group(1)} bytes/lane of scratch)")
 
 
 def build(verbose=False, jobs=8):
@@ -36,7 +58,8 @@ def build(verbose=False, jobs=8):
     objs = []
     for s in HIP_SOURCES:
         o = os.path.join(OBJ, s + ".o")
-        cmds.append(["hipcc", "-c"] + COMMON + HIP_FLAGS + [os.path.join(CSRC, s), "-o", o])
+        cmds.append(["hipcc", "-c"] + COMMON + HIP_FLAGS + ["-Rpass-analysis=kernel-resource-usage",
+                                                            os.path.join(CSRC, s), "-o", o])
         objs.append(o)
     for s in CPP_SOURCES:
         o = os.path.join(OBJ, s + ".o")
@@ -44,11 +67,11 @@ def build(verbose=False, jobs=8):
         objs.append(o)
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         outs = list(ex.map(_run, cmds))
+    for s, out in zip(HIP_SOURCES, outs):
+        _check_resources(s, out)
     if verbose:
         for c, o in zip(cmds, outs):
             print(" ".join(c))
-            if o.strip():
-                print(o)
     tmp = LIB + ".tmp"
     _run(["hipcc", "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs)
     os.replace(tmp, LIB)

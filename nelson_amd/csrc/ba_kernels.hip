@@ -894,7 +894,7 @@ void launch_xu_stats(int NU, const double* xU, const T* bU, const Scratch& s, De
 }
 
 void launch_step_finish(const Scratch& s, DevCtl* ctl, int gate, int lm_checks, hipStream_t st) {
-  hipLaunchKernelGGL(k_step_finish, dim3(1), dim3(256), 0, st, s.part_xu, s.nblk_cams, s.part_xv, s.nblk_pts, ctl,
+  hipLaunchKernelGGL(k_step_finish, dim3(1), dim3(256), 0, st, s.part_xu, s.nblk_cams, s.part_xv, s.nblk_xv, ctl,
                      gate, lm_checks);
 }
 

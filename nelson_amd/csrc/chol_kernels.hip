@@ -82,7 +82,7 @@ __device__ __forceinline__ void publish_flag(int* flag) {
 // (FP64); g0 = global index of the tile's first row.  Rows >= n (the bS row
 // and the identity padding) take pivot 1 and never count as failures.
 template <int TB>
-__device__ void potrf_lds(double* A, int g0, int n, DevCtl* ctl, double* sinv) {
+__device__ void potrf_lds(double* A, int g0, int n, DevCtl* ctl) {
   const int tid = threadIdx.x;
   constexpr int NS = 256 / TB;
   for (int k = 0; k < TB; k++) {
@@ -95,10 +95,9 @@ __device__ void potrf_lds(double* A, int g0, int n, DevCtl* ctl, double* sinv) {
         piv = 1.0;
       } else piv = sqrt(akk);
       A[k * TB + k] = piv;
-      *sinv = 1.0 / piv;
     }
     __syncthreads();
-    const double inv = *sinv;
+    const double inv = 1.0 / A[k * TB + k];
     if (tid > k && tid < TB) A[k * TB + tid] *= inv;
     __syncthreads();
     const int i = tid % TB, js = tid / TB;
@@ -125,25 +124,35 @@ __device__ void trsm_lds(T* Cs, const T* Ls) {
   }
 }
 
+// LDS budget: exactly 64 KiB per workgroup (tile buffer + staging buffer,
+// plus a double tile for the FP64 pivot block of the FP32 path, which then
+// has half-size buffers).  The dequeued tile index is passed through the
+// first word of the tile buffer, which is dead between tiles.
 template <typename T, int TB>
 __global__ __launch_bounds__(256) void k_chol(TileView tv, T* __restrict__ tiles, const int* __restrict__ tile_order,
                                               int* ready, int* counter, DevCtl* ctl, int gate) {
   static_assert(TB == 64, "the MFMA tiling below assumes 64x64 tiles and 4 waves");
+  constexpr int KC = TB / 2;  // staged K columns per GEMM chunk
   if (!chol_gate(ctl, gate)) return;
   typedef typename Mfma<T>::acc_t acc_t;
-  __shared__ T As[TB * TB];
-  __shared__ T Bs[TB * TB];
+  __shared__ T Ct[TB * TB];        // C tile during potrf / trsm
+  __shared__ T stage[2 * TB * KC];  // A | B chunks during GEMM, L_jj during trsm
   __shared__ double Dd[(sizeof(T) == 8) ? 1 : TB * TB];
-  __shared__ int sh_t;
-  __shared__ double sh_inv;
+  static_assert(sizeof(T) * TB * TB * 2 + ((sizeof(T) == 8) ? 8 : 8 * TB * TB) <= 65536 + 8,
+                "LDS budget");
+  T* As = stage;
+  T* Bs = stage + TB * KC;
+  int* sh_t = (int*)Ct;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int wr = wv & 1, wc = wv >> 1;
   const int NT = tv.NT;
   const size_t T2 = (size_t)TB * TB;
-  for (;;) {
-    if (threadIdx.x == 0) sh_t = atomicAdd(counter, 1);
+  // a workgroup takes at most ntiles tiles: the loop is bounded
+  for (int round = 0; round < tv.ntiles; round++) {
     __syncthreads();
-    const int t = sh_t;
+    if (threadIdx.x == 0) *sh_t = atomicAdd(counter, 1);
+    __syncthreads();
+    const int t = *sh_t;
     __syncthreads();
     if (t >= tv.ntiles) break;
     const int i = tile_order[2 * t], j = tile_order[2 * t + 1];
@@ -169,28 +178,30 @@ __global__ __launch_bounds__(256) void k_chol(TileView tv, T* __restrict__ tiles
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       const T* Ag = tiles + (size_t)tik * T2;
       const T* Bg = tiles + (size_t)tjk * T2;
-      for (int e = threadIdx.x; e < TB * TB; e += 256) {
-        As[e] = Ag[e];
-        Bs[e] = Bg[e];
-      }
-      __syncthreads();
-      // C[r][c] -= sum_m A[r][m] B[c][m]
+      for (int kk = 0; kk < TB; kk += KC) {
+        for (int e = threadIdx.x; e < TB * KC; e += 256) {
+          As[e] = Ag[(size_t)kk * TB + e];
+          Bs[e] = Bg[(size_t)kk * TB + e];
+        }
+        __syncthreads();
+        // C[r][c] -= sum_m A[r][kk+m] B[c][kk+m]
 #pragma unroll 4
-      for (int m = 0; m < TB; m += 4) {
-        const int mm = m + (lane >> 4);
-        T a[2], b[2];
+        for (int m = 0; m < KC; m += 4) {
+          const int mm = m + (lane >> 4);
+          T a[2], b[2];
 #pragma unroll
-        for (int bi = 0; bi < 2; bi++) a[bi] = -As[mm * TB + wr * 32 + bi * 16 + (lane & 15)];
+          for (int bi = 0; bi < 2; bi++) a[bi] = -As[mm * TB + wr * 32 + bi * 16 + (lane & 15)];
 #pragma unroll
-        for (int bj = 0; bj < 2; bj++) b[bj] = Bs[mm * TB + wc * 32 + bj * 16 + (lane & 15)];
+          for (int bj = 0; bj < 2; bj++) b[bj] = Bs[mm * TB + wc * 32 + bj * 16 + (lane & 15)];
 #pragma unroll
-        for (int bi = 0; bi < 2; bi++)
+          for (int bi = 0; bi < 2; bi++)
 #pragma unroll
-          for (int bj = 0; bj < 2; bj++) acc[bi][bj] = Mfma<T>::fma(a[bi], b[bj], acc[bi][bj]);
+            for (int bj = 0; bj < 2; bj++) acc[bi][bj] = Mfma<T>::fma(a[bi], b[bj], acc[bi][bj]);
+        }
+        __syncthreads();
       }
-      __syncthreads();
     }
-    // accumulators -> Bs (column-major)
+    // accumulators -> Ct (column-major)
 #pragma unroll
     for (int bi = 0; bi < 2; bi++)
 #pragma unroll
@@ -199,17 +210,17 @@ __global__ __launch_bounds__(256) void k_chol(TileView tv, T* __restrict__ tiles
         for (int vv = 0; vv < 4; vv++) {
           const int r = wr * 32 + bi * 16 + 4 * (lane >> 4) + vv;
           const int c = wc * 32 + bj * 16 + (lane & 15);
-          Bs[c * TB + r] = acc[bi][bj][vv];
+          Ct[c * TB + r] = acc[bi][bj][vv];
         }
     __syncthreads();
     if (i == j) {
       if constexpr (sizeof(T) == 8) {
-        potrf_lds<TB>((double*)Bs, j * TB, tv.n, ctl, &sh_inv);
+        potrf_lds<TB>((double*)Ct, j * TB, tv.n, ctl);
       } else {
-        for (int e = threadIdx.x; e < TB * TB; e += 256) Dd[e] = (double)Bs[e];
+        for (int e = threadIdx.x; e < TB * TB; e += 256) Dd[e] = (double)Ct[e];
         __syncthreads();
-        potrf_lds<TB>(Dd, j * TB, tv.n, ctl, &sh_inv);
-        for (int e = threadIdx.x; e < TB * TB; e += 256) Bs[e] = (T)Dd[e];
+        potrf_lds<TB>(Dd, j * TB, tv.n, ctl);
+        for (int e = threadIdx.x; e < TB * TB; e += 256) Ct[e] = (T)Dd[e];
         __syncthreads();
       }
     } else {
@@ -218,11 +229,11 @@ __global__ __launch_bounds__(256) void k_chol(TileView tv, T* __restrict__ tiles
       __syncthreads();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       const T* Lg = tiles + (size_t)tjj * T2;
-      for (int e = threadIdx.x; e < TB * TB; e += 256) As[e] = Lg[e];
+      for (int e = threadIdx.x; e < TB * TB; e += 256) stage[e] = Lg[e];
       __syncthreads();
-      trsm_lds<T, TB>(Bs, As);
+      trsm_lds<T, TB>(Ct, stage);
     }
-    for (int e = threadIdx.x; e < TB * TB; e += 256) Cg[e] = Bs[e];
+    for (int e = threadIdx.x; e < TB * TB; e += 256) Cg[e] = Ct[e];
     publish_flag(ready + t);
   }
 }
@@ -241,7 +252,8 @@ __global__ __launch_bounds__(256) void k_trsv_bwd(TileView tv, const T* __restri
   const int NTr = (n + TB - 1) / TB;
   const size_t T2 = (size_t)TB * TB;
   const int tid = threadIdx.x;
-  for (;;) {
+  // a workgroup takes at most NTr tile rows: the loop is bounded
+  for (int round = 0; round < NTr; round++) {
     if (tid == 0) sh_t = atomicAdd(counter, 1);
     __syncthreads();
     const int t = sh_t;

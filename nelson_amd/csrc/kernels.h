@@ -66,7 +66,7 @@ struct Scratch {
   double* part_xv;      // [2 * nblk_pts]  (incsq, den) of the points
   double* part_xu;      // [2 * nblk_cams] (incsq, den) of the cameras
   unsigned long long* maxes;  // [4] max|bU|, max|bV|, max|diag U|, max|diag V| as bit patterns
-  int nblk_pts, nblk_cams;
+  int nblk_pts, nblk_cams, nblk_xv;  // blocks of the point pass (P), xU stats (6 NU), back-substitution (NV)
 };
 
 // ---- BA kernels (ba_kernels.hip), T = double (FP64) or float (mixed) ----
