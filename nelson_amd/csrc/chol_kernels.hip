@@ -29,12 +29,17 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 
 template <typename T>
 struct Mfma;
+// 16x16x4 MFMA on one wave: lane l supplies A[l%16][l/16] and B[l/16][l%16];
+// output register v of lane l holds D[row(v, l)][l%16] where the f32 form
+// keeps four consecutive rows per lane group (row = 4*(l/16) + v) and the
+// f64 form interleaves them (row = 4*v + l/16).
 template <>
 struct Mfma<double> {
   typedef v4d acc_t;
   static __device__ __forceinline__ acc_t fma(double a, double b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int v, int lane) { return 4 * v + (lane >> 4); }
 };
 template <>
 struct Mfma<float> {
@@ -42,6 +47,7 @@ struct Mfma<float> {
   static __device__ __forceinline__ acc_t fma(float a, float b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int v, int lane) { return 4 * (lane >> 4) + v; }
 };
 
 __device__ __forceinline__ bool chol_gate(const DevCtl* c, int gate) {
@@ -164,7 +170,7 @@ __global__ __launch_bounds__(256) void k_chol(TileView tv, T* __restrict__ tiles
       for (int bj = 0; bj < 2; bj++)
 #pragma unroll
         for (int vv = 0; vv < 4; vv++) {
-          const int r = wr * 32 + bi * 16 + 4 * (lane >> 4) + vv;
+          const int r = wr * 32 + bi * 16 + Mfma<T>::row(vv, lane);
           const int c = wc * 32 + bj * 16 + (lane & 15);
           acc[bi][bj][vv] = Cg[c * TB + r];
         }
@@ -208,7 +214,7 @@ __global__ __launch_bounds__(256) void k_chol(TileView tv, T* __restrict__ tiles
       for (int bj = 0; bj < 2; bj++)
 #pragma unroll
         for (int vv = 0; vv < 4; vv++) {
-          const int r = wr * 32 + bi * 16 + 4 * (lane >> 4) + vv;
+          const int r = wr * 32 + bi * 16 + Mfma<T>::row(vv, lane);
           const int c = wc * 32 + bj * 16 + (lane & 15);
           Ct[c * TB + r] = acc[bi][bj][vv];
         }
