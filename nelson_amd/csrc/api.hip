@@ -29,7 +29,7 @@ namespace {
 
 constexpr int kTB = 64;          // Cholesky tile
 constexpr int kCholMaxWG = 512;  // persistent workgroups of the factorisation
-constexpr int kTrsvMaxWG = 256;
+constexpr int kTrsvMaxWG = 1024;
 
 struct DevBuf {
   std::vector<void*> ptrs;
@@ -74,7 +74,8 @@ struct nsn_problem {
   int *tile_order = nullptr, *tile_blk_ptr = nullptr, *tile_blk = nullptr, *sb_i = nullptr, *sb_j = nullptr,
       *sb_ptr = nullptr, *sb_pairs = nullptr;
   int *flags = nullptr, *counters = nullptr;
-  int nflags = 0;
+  int nflags = 0, NR = 0;
+  double* parts = nullptr;  // backward-solve partial products, NR x NR x TB
   DevCtl* ctl = nullptr;
   DevCtl* hctl = nullptr;  // pinned mirror
   DevLog* log = nullptr;
@@ -131,8 +132,8 @@ void seq_schur(nsn_problem* h, int gate, int checks, bool timed) {
   launch_chol_reset<T>(h->flags, h->nflags, h->counters, h->ctl, gate, st);
   launch_chol<T>(h->tv, (T*)h->tiles, h->tile_order, h->flags, h->counters, h->ctl, gate, kCholMaxWG, st);
   if (timed) (void)hipEventRecord(h->ev[3], st);
-  launch_trsv_bwd<T>(h->tv, (const T*)h->tiles, h->xU, h->flags + h->tv.ntiles, h->counters + 1, h->ctl, gate,
-                     kTrsvMaxWG, st);
+  launch_trsv_bwd<T>(h->tv, (const T*)h->tiles, h->xU, h->parts, h->flags + h->tv.ntiles,
+                     h->flags + h->tv.ntiles + h->NR, h->counters + 1, h->ctl, gate, kTrsvMaxWG, st);
   if (timed) (void)hipEventRecord(h->ev[4], st);
   launch_backsub<T>(h->view, (const T*)h->W, (const T*)h->Vinv, (const T*)h->bV, h->xU, h->xV, h->scr, h->ctl,
                     gate, st);
@@ -323,8 +324,8 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
     std::vector<int> fill(tile_cnt.begin(), tile_cnt.end() - 1);
     for (int sb = 0; sb < h->nsb; sb++) for_tiles(sb, [&](long long t) { tile_blk[fill[t]++] = sb; });
   }
-  const int NTr = (n + kTB - 1) / kTB;
-  h->nflags = tv.ntiles + NTr;
+  h->NR = (n + kTB - 1) / kTB;
+  h->nflags = tv.ntiles + h->NR + h->NR * h->NR;
 
   // ---- device allocation ----
   DevBuf& m = h->mem;
@@ -365,6 +366,7 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
   h->bS = a_T(6 * (size_t)NU);
   h->tiles = a_T((size_t)tv.ntiles * kTB * kTB);
   h->xU = a_dbl(6 * (size_t)NU);
+  h->parts = a_dbl((size_t)h->NR * h->NR * kTB);
   h->xV = a_dbl(3 * (size_t)NV);
   h->tile_order = a_int(tile_order.size());
   h->tile_blk_ptr = a_int(tile_cnt.size());

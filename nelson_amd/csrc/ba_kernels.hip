@@ -460,30 +460,40 @@ __global__ __launch_bounds__(256) void k_s_tiles(TileView tv, const int* __restr
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int a = lane / 6, bb = lane % 6;
   const int b1 = tile_blk_ptr[t + 1];
   for (int b = tile_blk_ptr[t] + wv; b < b1; b += 4) {
     const int sb = tile_blk[b];
     const int i = sb_i[sb], j = sb_j[sb];
-    if (lane < 36) {
-      const int a = lane / 6, bb = lane % 6;
-      const int R = 6 * j + bb, Cc = 6 * i + a;
-      const bool inside = R >= r0 && R < r0 + TB && Cc >= c0 && Cc < c0 + TB;
-      if (inside) {
-        if (i == j) {
-          if (bb >= a) lt[(Cc - c0) * TB + (R - r0)] = Sd[21 * (size_t)i + up6(a, bb)];
-        } else {
-          T s = (T)0;
-          const int p1 = sb_ptr[sb + 1];
-          for (int pr = sb_ptr[sb]; pr < p1; pr++) {
-            const int ka = sb_pairs[2 * pr], kb = sb_pairs[2 * pr + 1];
-            const T* y = Y + 18 * (size_t)ka + 3 * a;
-            const T* w = W + 18 * (size_t)kb + 3 * bb;
-            s += y[0] * w[0] + y[1] * w[1] + y[2] * w[2];
-          }
-          lt[(Cc - c0) * TB + (R - r0)] = -s;
+    const int R = 6 * j + bb, Cc = 6 * i + a;
+    const bool inside = lane < 36 && R >= r0 && R < r0 + TB && Cc >= c0 && Cc < c0 + TB;
+    if (i == j) {
+      if (inside && bb >= a) lt[(Cc - c0) * TB + (R - r0)] = Sd[21 * (size_t)i + up6(a, bb)];
+      continue;
+    }
+    // S_ij = -sum_p Y_ip W_jp^T: lane (a, bb) < 36 owns one entry; the pair
+    // indices of up to 64 pairs are loaded at once and broadcast by readlane
+    T s = (T)0;
+    const int p0 = sb_ptr[sb], np = sb_ptr[sb + 1] - p0;
+    for (int base = 0; base < np; base += 64) {
+      const int cnt = min(64, np - base);
+      int my_ka = 0, my_kb = 0;
+      if (lane < cnt) {
+        my_ka = sb_pairs[2 * (p0 + base + lane)];
+        my_kb = sb_pairs[2 * (p0 + base + lane) + 1];
+      }
+#pragma unroll 4
+      for (int q = 0; q < cnt; q++) {
+        const int ka = __builtin_amdgcn_readlane(my_ka, q);
+        const int kb = __builtin_amdgcn_readlane(my_kb, q);
+        if (lane < 36) {
+          const T* y = Y + 18 * (size_t)ka + 3 * a;
+          const T* w = W + 18 * (size_t)kb + 3 * bb;
+          s += y[0] * w[0] + y[1] * w[1] + y[2] * w[2];
         }
       }
     }
+    if (inside) lt[(Cc - c0) * TB + (R - r0)] = -s;
   }
   __syncthreads();
   T* out = tiles + (size_t)t * TB * TB;

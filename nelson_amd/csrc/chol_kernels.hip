@@ -84,81 +84,135 @@ __device__ __forceinline__ void publish_flag(int* flag) {
   if (threadIdx.x == 0) __hip_atomic_store(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// unblocked right-looking Cholesky of a TB x TB column-major tile in LDS
-// (FP64); g0 = global index of the tile's first row.  Rows >= n (the bS row
-// and the identity padding) take pivot 1 and never count as failures.
+// orders this wave's LDS accesses (cross-lane exchange through LDS inside one
+// wave without a workgroup barrier)
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Blocked right-looking Cholesky of a TB x TB column-major FP64 tile in LDS,
+// in 16-column steps: the 16x16 diagonal block by one wave (no workgroup
+// barrier inside), the panel below it by one thread per row, the trailing
+// lower part by all threads; three workgroup barriers per step.
+// g0 = global index of the tile's first row; rows >= n (the bS row and the
+// identity padding) take pivot 1 and never count as failures.
 template <int TB>
-__device__ void potrf_lds(double* A, int g0, int n, DevCtl* ctl) {
-  const int tid = threadIdx.x;
-  constexpr int NS = 256 / TB;
-  for (int k = 0; k < TB; k++) {
-    if (tid == 0) {
-      const double akk = A[k * TB + k];
-      double piv;
-      if (g0 + k >= n) piv = 1.0;
-      else if (!(akk > 0.0) || !isfinite(akk)) {
-        atomicAdd(&ctl->fail_s, 1);
-        piv = 1.0;
-      } else piv = sqrt(akk);
-      A[k * TB + k] = piv;
+__device__ __forceinline__ void potrf_lds(double* A, int g0, int n, DevCtl* ctl) {
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  for (int kb = 0; kb < TB; kb += 16) {
+    if (wv == 0) {
+      for (int k = 0; k < 16; k++) {
+        const double d = A[(kb + k) * TB + kb + k];
+        double piv;
+        if (g0 + kb + k >= n) piv = 1.0;
+        else if (!(d > 0.0) || !isfinite(d)) {
+          if (lane == 0) atomicAdd(&ctl->fail_s, 1);
+          piv = 1.0;
+        } else piv = sqrt(d);
+        const double inv = 1.0 / piv;
+        double lrk = 0.0;
+        if (lane > k && lane < 16) {
+          lrk = A[(kb + k) * TB + kb + lane] * inv;
+          A[(kb + k) * TB + kb + lane] = lrk;
+        }
+        wave_lds_sync();
+        if (lane == 0) A[(kb + k) * TB + kb + k] = piv;
+        if (lane > k && lane < 16)
+          for (int c = k + 1; c <= lane; c++) A[(kb + c) * TB + kb + lane] -= lrk * A[(kb + k) * TB + kb + c];
+        wave_lds_sync();
+      }
     }
     __syncthreads();
-    const double inv = 1.0 / A[k * TB + k];
-    if (tid > k && tid < TB) A[k * TB + tid] *= inv;
+    const int R0 = kb + 16, m = TB - R0;
+    if (m == 0) break;
+    if (tid < m) {
+      const int r = R0 + tid;
+      double x[16];
+#pragma unroll
+      for (int c = 0; c < 16; c++) x[c] = A[(kb + c) * TB + r];
+#pragma unroll
+      for (int c = 0; c < 16; c++) {
+        double s = x[c];
+#pragma unroll
+        for (int q = 0; q < c; q++) s -= x[q] * A[(kb + q) * TB + kb + c];
+        x[c] = s / A[(kb + c) * TB + kb + c];
+      }
+#pragma unroll
+      for (int c = 0; c < 16; c++) A[(kb + c) * TB + r] = x[c];
+    }
     __syncthreads();
-    const int i = tid % TB, js = tid / TB;
-    if (i > k) {
-      const double lik = A[k * TB + i];
-      for (int jj = k + 1 + js; jj <= i; jj += NS) A[jj * TB + i] -= lik * A[k * TB + jj];
+    for (int e = tid; e < m * m; e += 256) {
+      const int r = R0 + e % m, c = R0 + e / m;
+      if (c <= r) {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < 16; q++) s += A[(kb + q) * TB + r] * A[(kb + q) * TB + c];
+        A[c * TB + r] -= s;
+      }
     }
     __syncthreads();
   }
 }
 
-// X L^T = C in place (C in Cs, L lower in Ls), both TB x TB column-major
+// X L^T = C in place, C (TB x TB) and L (lower) column-major in LDS.  One
+// wave, one row per lane; the columns are solved in blocks of 16 held in
+// registers (earlier blocks re-read from LDS), the L entries are broadcast
+// reads.  The other waves wait at the closing barrier.
 template <typename T, int TB>
-__device__ void trsm_lds(T* Cs, const T* Ls) {
+__device__ __forceinline__ void trsm_lds(T* Cs, const T* Ls) {
+  static_assert(TB == 64, "one row per lane");
+  constexpr int CB = 16;
   const int tid = threadIdx.x;
-  constexpr int NS = 256 / TB;
-  const int r = tid % TB, cs = tid / TB;
-  for (int c = 0; c < TB; c++) {
-    if (tid < TB) Cs[c * TB + tid] = Cs[c * TB + tid] / Ls[c * TB + c];
-    __syncthreads();
-    const T xr = Cs[c * TB + r];
-    for (int c2 = c + 1 + cs; c2 < TB; c2 += NS) Cs[c2 * TB + r] -= xr * Ls[c * TB + c2];
-    __syncthreads();
+  if (tid < 64) {
+    for (int cb = 0; cb < TB; cb += CB) {
+      T x[CB];
+#pragma unroll
+      for (int c = 0; c < CB; c++) x[c] = Cs[(cb + c) * TB + tid];
+      for (int q = 0; q < cb; q++) {
+        const T xq = Cs[q * TB + tid];
+#pragma unroll
+        for (int c = 0; c < CB; c++) x[c] -= xq * Ls[q * TB + cb + c];
+      }
+#pragma unroll
+      for (int c = 0; c < CB; c++) {
+        T s = x[c];
+#pragma unroll
+        for (int q = 0; q < c; q++) s -= x[q] * Ls[(cb + q) * TB + cb + c];
+        x[c] = s / Ls[(cb + c) * TB + cb + c];
+      }
+#pragma unroll
+      for (int c = 0; c < CB; c++) Cs[(cb + c) * TB + tid] = x[c];
+    }
   }
+  __syncthreads();
 }
 
-// LDS budget: exactly 64 KiB per workgroup (tile buffer + staging buffer,
-// plus a double tile for the FP64 pivot block of the FP32 path, which then
-// has half-size buffers).  The dequeued tile index is passed through the
-// first word of the tile buffer, which is dead between tiles.
+// Tile DAG factorisation.  Per tile (i, j): C_ij -= sum_k L_ik L_jk^T with
+// the 16x16x4 MFMA, operands loaded straight from global memory into the
+// MFMA operand registers (each wave owns a 32x32 quadrant of C); dependency
+// waits are batched (one acquire fence for every run of already-published
+// k); then POTRF (i == j, FP64 in LDS) or the triangular solve against
+// L_jj (i > j).  LDS: 64 KiB = two TB x TB FP64 regions.
 template <typename T, int TB>
 __global__ __launch_bounds__(256) void k_chol(TileView tv, T* __restrict__ tiles, const int* __restrict__ tile_order,
                                               int* ready, int* counter, DevCtl* ctl, int gate) {
   static_assert(TB == 64, "the MFMA tiling below assumes 64x64 tiles and 4 waves");
-  constexpr int KC = TB / 2;  // staged K columns per GEMM chunk
   if (!chol_gate(ctl, gate)) return;
   typedef typename Mfma<T>::acc_t acc_t;
-  __shared__ T Ct[TB * TB];        // C tile during potrf / trsm
-  __shared__ T stage[2 * TB * KC];  // A | B chunks during GEMM, L_jj during trsm
-  __shared__ double Dd[(sizeof(T) == 8) ? 1 : TB * TB];
-  static_assert(sizeof(T) * TB * TB * 2 + ((sizeof(T) == 8) ? 8 : 8 * TB * TB) <= 65536 + 8,
-                "LDS budget");
-  T* As = stage;
-  T* Bs = stage + TB * KC;
-  int* sh_t = (int*)Ct;
+  __shared__ double lds[2 * TB * TB];
+  double* R0d = lds;                 // FP64 tile (POTRF)
+  T* R0 = (T*)lds;                   // C tile (TRSM)
+  T* R1 = (T*)(lds + TB * TB);       // L_jj (TRSM)
+  int* shv = (int*)(lds + TB * TB);  // scalars, in region 1 while it is unused
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int wr = wv & 1, wc = wv >> 1;
   const int NT = tv.NT;
   const size_t T2 = (size_t)TB * TB;
-  // a workgroup takes at most ntiles tiles: the loop is bounded
-  for (int round = 0; round < tv.ntiles; round++) {
+  for (int round = 0; round < tv.ntiles; round++) {  // at most ntiles tiles per workgroup
+    if (threadIdx.x == 0) shv[0] = atomicAdd(counter, 1);
     __syncthreads();
-    if (threadIdx.x == 0) *sh_t = atomicAdd(counter, 1);
-    __syncthreads();
-    const int t = *sh_t;
+    const int t = shv[0];
     __syncthreads();
     if (t >= tv.ntiles) break;
     const int i = tile_order[2 * t], j = tile_order[2 * t + 1];
@@ -174,146 +228,155 @@ __global__ __launch_bounds__(256) void k_chol(TileView tv, T* __restrict__ tiles
           const int c = wc * 32 + bj * 16 + (lane & 15);
           acc[bi][bj][vv] = Cg[c * TB + r];
         }
-    for (int k = 0; k < j; k++) {
-      const long long tik = tile_index(i, k, NT), tjk = tile_index(j, k, NT);
+    int k = 0;
+    while (k < j) {
       if (threadIdx.x == 0) {
-        wait_flag(ready + tik, ctl);
-        if (i != j) wait_flag(ready + tjk, ctl);
+        wait_flag(ready + tile_index(i, k, NT), ctl);
+        if (i != j) wait_flag(ready + tile_index(j, k, NT), ctl);
+        int ke = k + 1;
+        while (ke < j && ke - k < 16 &&
+               __hip_atomic_load(ready + tile_index(i, ke, NT), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &&
+               (i == j ||
+                __hip_atomic_load(ready + tile_index(j, ke, NT), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
+          ke++;
+        shv[1] = ke;
       }
       __syncthreads();
+      const int ke = shv[1];
+      __syncthreads();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-      const T* Ag = tiles + (size_t)tik * T2;
-      const T* Bg = tiles + (size_t)tjk * T2;
-      for (int kk = 0; kk < TB; kk += KC) {
-        for (int e = threadIdx.x; e < TB * KC; e += 256) {
-          As[e] = Ag[(size_t)kk * TB + e];
-          Bs[e] = Bg[(size_t)kk * TB + e];
+      for (; k < ke; k++) {
+        const T* Ag = tiles + (size_t)tile_index(i, k, NT) * T2 + wr * 32 + (lane & 15);
+        const T* Bg = tiles + (size_t)tile_index(j, k, NT) * T2 + wc * 32 + (lane & 15);
+        const int kq = lane >> 4;
+#pragma unroll 8
+        for (int m = 0; m < TB; m += 4) {
+          const size_t off = (size_t)(m + kq) * TB;
+          const T a0 = -Ag[off], a1 = -Ag[off + 16];
+          const T b0 = Bg[off], b1 = Bg[off + 16];
+          acc[0][0] = Mfma<T>::fma(a0, b0, acc[0][0]);
+          acc[0][1] = Mfma<T>::fma(a0, b1, acc[0][1]);
+          acc[1][0] = Mfma<T>::fma(a1, b0, acc[1][0]);
+          acc[1][1] = Mfma<T>::fma(a1, b1, acc[1][1]);
         }
-        __syncthreads();
-        // C[r][c] -= sum_m A[r][kk+m] B[c][kk+m]
-#pragma unroll 4
-        for (int m = 0; m < KC; m += 4) {
-          const int mm = m + (lane >> 4);
-          T a[2], b[2];
-#pragma unroll
-          for (int bi = 0; bi < 2; bi++) a[bi] = -As[mm * TB + wr * 32 + bi * 16 + (lane & 15)];
-#pragma unroll
-          for (int bj = 0; bj < 2; bj++) b[bj] = Bs[mm * TB + wc * 32 + bj * 16 + (lane & 15)];
-#pragma unroll
-          for (int bi = 0; bi < 2; bi++)
-#pragma unroll
-            for (int bj = 0; bj < 2; bj++) acc[bi][bj] = Mfma<T>::fma(a[bi], b[bj], acc[bi][bj]);
-        }
-        __syncthreads();
       }
     }
-    // accumulators -> Ct (column-major)
-#pragma unroll
-    for (int bi = 0; bi < 2; bi++)
-#pragma unroll
-      for (int bj = 0; bj < 2; bj++)
-#pragma unroll
-        for (int vv = 0; vv < 4; vv++) {
-          const int r = wr * 32 + bi * 16 + Mfma<T>::row(vv, lane);
-          const int c = wc * 32 + bj * 16 + (lane & 15);
-          Ct[c * TB + r] = acc[bi][bj][vv];
-        }
-    __syncthreads();
     if (i == j) {
-      if constexpr (sizeof(T) == 8) {
-        potrf_lds<TB>((double*)Ct, j * TB, tv.n, ctl);
-      } else {
-        for (int e = threadIdx.x; e < TB * TB; e += 256) Dd[e] = (double)Ct[e];
-        __syncthreads();
-        potrf_lds<TB>(Dd, j * TB, tv.n, ctl);
-        for (int e = threadIdx.x; e < TB * TB; e += 256) Ct[e] = (T)Dd[e];
-        __syncthreads();
-      }
+#pragma unroll
+      for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+        for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+          for (int vv = 0; vv < 4; vv++) {
+            const int r = wr * 32 + bi * 16 + Mfma<T>::row(vv, lane);
+            const int c = wc * 32 + bj * 16 + (lane & 15);
+            R0d[c * TB + r] = (double)acc[bi][bj][vv];
+          }
+      __syncthreads();
+      potrf_lds<TB>(R0d, j * TB, tv.n, ctl);
+      for (int e = threadIdx.x; e < TB * TB; e += 256) Cg[e] = (T)R0d[e];
     } else {
+#pragma unroll
+      for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+        for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+          for (int vv = 0; vv < 4; vv++) {
+            const int r = wr * 32 + bi * 16 + Mfma<T>::row(vv, lane);
+            const int c = wc * 32 + bj * 16 + (lane & 15);
+            R0[c * TB + r] = acc[bi][bj][vv];
+          }
       const long long tjj = tile_index(j, j, NT);
       if (threadIdx.x == 0) wait_flag(ready + tjj, ctl);
       __syncthreads();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       const T* Lg = tiles + (size_t)tjj * T2;
-      for (int e = threadIdx.x; e < TB * TB; e += 256) stage[e] = Lg[e];
+      for (int e = threadIdx.x; e < TB * TB; e += 256) R1[e] = Lg[e];
       __syncthreads();
-      trsm_lds<T, TB>(Ct, stage);
+      trsm_lds<T, TB>(R0, R1);
+      for (int e = threadIdx.x; e < TB * TB; e += 256) Cg[e] = R0[e];
     }
-    for (int e = threadIdx.x; e < TB * TB; e += 256) Cg[e] = Ct[e];
     publish_flag(ready + t);
   }
 }
 
 // Backward solve L^T x = y, y = row n of the factor (the forward solve's
-// result), persistent over tile rows in reverse order.
+// result), as a task DAG over the NR tile rows covering 0..n-1.  Tasks in
+// queue order, for k = NR-1 down to 0:
+//   D_k      : x_k = L_kk^-T (y_k - sum_{k' > k} g_{k',k}), the sum taken in
+//              a fixed order (deterministic), one wave for the substitution;
+//   G_{k,i}  : g_{k,i} = L_ki^T x_k for i = k-1 .. 0, into its own slot.
+// Every task waits only on tasks earlier in the queue.
 template <typename T, int TB>
-__global__ __launch_bounds__(256) void k_trsv_bwd(TileView tv, const T* __restrict__ tiles, double* xU, int* ready,
+__global__ __launch_bounds__(256) void k_trsv_bwd(TileView tv, const T* __restrict__ tiles, double* xU,
+                                                  double* __restrict__ parts, int* xready, int* gready,
                                                   int* counter, DevCtl* ctl, int gate) {
+  static_assert(TB == 64, "64-row tiles");
   if (!chol_gate(ctl, gate)) return;
-  __shared__ T Ls[TB * TB];
-  __shared__ double acc[TB], xk[TB], part[256 / TB][TB];
+  __shared__ double acc[4][TB];
+  __shared__ double xs[TB];
   __shared__ int sh_t;
-  constexpr int NS = 256 / TB;
   const int n = tv.n, NT = tv.NT;
-  const int NTr = (n + TB - 1) / TB;
+  const int NR = (n + TB - 1) / TB;
+  const int ntask = NR + NR * (NR - 1) / 2;
   const size_t T2 = (size_t)TB * TB;
-  const int tid = threadIdx.x;
-  // a workgroup takes at most NTr tile rows: the loop is bounded
-  for (int round = 0; round < NTr; round++) {
+  const int tid = threadIdx.x, lane = tid & 63;
+  for (int round = 0; round < ntask; round++) {  // bounded
     if (tid == 0) sh_t = atomicAdd(counter, 1);
     __syncthreads();
-    const int t = sh_t;
+    int t = sh_t;
     __syncthreads();
-    if (t >= NTr) break;
-    const int i = NTr - 1 - t;
-    const T* ytile = tiles + (size_t)tile_index(n / TB, i, NT) * T2;
-    if (tid < TB) {
-      const int R = i * TB + tid;
-      acc[tid] = (R < n) ? (double)ytile[tid * TB + (n % TB)] : 0.0;
-    }
-    for (int k = NTr - 1; k > i; k--) {
-      if (tid == 0) wait_flag(ready + k, ctl);
+    if (t >= ntask) break;
+    // decode: block of k holds D_k then G_{k,k-1..0}; blocks for k = NR-1 down
+    int k = NR - 1;
+    while (t >= k + 1) { t -= k + 1; k--; }
+    if (t == 0) {
+      // ---- D_k ----
+      const T* ytile = tiles + (size_t)tile_index(n / TB, k, NT) * T2;
+      if (tid == 0)
+        for (int kp = NR - 1; kp > k; kp--) wait_flag(gready + (size_t)kp * NR + k, ctl);
       __syncthreads();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-      if (tid < TB) xk[tid] = (k * TB + tid < n) ? xU[k * TB + tid] : 0.0;
-      const T* Lg = tiles + (size_t)tile_index(k, i, NT) * T2;
-      for (int e = tid; e < TB * TB; e += 256) Ls[e] = Lg[e];
-      __syncthreads();
       {
-        const int c = tid % TB, rs = tid / TB;
+        const int c = tid % TB, g = tid / TB;  // 4 interleaved partial sums per entry
         double s = 0.0;
-        for (int r = rs * (TB / NS); r < (rs + 1) * (TB / NS); r++) s += (double)Ls[c * TB + r] * xk[r];
-        part[rs][c] = s;
+        for (int kp = NR - 1 - g; kp > k; kp -= 4) s += parts[((size_t)kp * NR + k) * TB + c];
+        acc[g][c] = s;
       }
       __syncthreads();
-      if (tid < TB) {
-        double s = 0.0;
-        for (int q = 0; q < NS; q++) s += part[q][tid];
-        acc[tid] -= s;
+      if (tid < 64) {
+        const int R = k * TB + tid;
+        double a = (R < n) ? (double)ytile[tid * TB + (n % TB)] : 0.0;
+        a -= ((acc[0][tid] + acc[1][tid]) + acc[2][tid]) + acc[3][tid];
+        const T* Dg = tiles + (size_t)tile_index(k, k, NT) * T2;
+        for (int c = TB - 1; c >= 0; c--) {
+          double xc = 0.0;
+          if (k * TB + c < n) xc = __shfl(a, c) / (double)Dg[c * TB + c];
+          if (tid == c) a = xc;
+          if (tid < c) a -= (double)Dg[tid * TB + c] * xc;
+        }
+        if (R < n) xU[R] = a;
       }
+      publish_flag(xready + k);
+    } else {
+      // ---- G_{k,i} ----
+      const int i = k - t;
+      if (tid == 0) wait_flag(xready + k, ctl);
       __syncthreads();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      if (tid < TB) xs[tid] = (k * TB + tid < n) ? xU[k * TB + tid] : 0.0;
+      __syncthreads();
+      const T* Lg = tiles + (size_t)tile_index(k, i, NT) * T2;
+      const int c = tid >> 2, sub = tid & 3;
+      double s = 0.0;
+#pragma unroll
+      for (int r = sub * 16; r < sub * 16 + 16; r++) s += (double)Lg[c * TB + r] * xs[r];
+      s += __shfl_xor(s, 1);
+      s += __shfl_xor(s, 2);
+      if (sub == 0) parts[((size_t)k * NR + i) * TB + c] = s;
+      (void)lane;
+      publish_flag(gready + (size_t)k * NR + i);
     }
-    const T* Dg = tiles + (size_t)tile_index(i, i, NT) * T2;
-    for (int e = tid; e < TB * TB; e += 256) Ls[e] = Dg[e];
-    __syncthreads();
-    // one wave: lane c holds acc[c]; columns from the last to the first
-    if (tid < 64) {
-      double a = (tid < TB) ? acc[tid] : 0.0;
-      for (int c = TB - 1; c >= 0; c--) {
-        const int R = i * TB + c;
-        double xc = 0.0;
-        if (R < n) xc = __shfl(a, c) / (double)Ls[c * TB + c];
-        if (tid == c) a = xc;
-        if (tid < c) a -= (double)Ls[tid * TB + c] * xc;
-      }
-      if (tid < TB) acc[tid] = a;
-    }
-    __syncthreads();
-    if (tid < TB) {
-      const int R = i * TB + tid;
-      if (R < n) xU[R] = acc[tid];
-    }
-    publish_flag(ready + i);
   }
 }
 
@@ -339,20 +402,22 @@ void launch_chol(const TileView& tv, T* tiles, const int* tile_order, int* ready
 }
 
 template <typename T>
-void launch_trsv_bwd(const TileView& tv, const T* tiles, double* xU, int* ready, int* counter, DevCtl* ctl, int gate,
-                     int max_wg, hipStream_t st) {
-  const int NTr = (tv.n + 63) / 64;
-  int g = NTr < max_wg ? NTr : max_wg;
-  hipLaunchKernelGGL((k_trsv_bwd<T, 64>), dim3(g), dim3(256), 0, st, tv, tiles, xU, ready, counter, ctl, gate);
+void launch_trsv_bwd(const TileView& tv, const T* tiles, double* xU, double* parts, int* xready, int* gready,
+                     int* counter, DevCtl* ctl, int gate, int max_wg, hipStream_t st) {
+  const int NR = (tv.n + 63) / 64;
+  const int ntask = NR + NR * (NR - 1) / 2;
+  int g = ntask < max_wg ? ntask : max_wg;
+  hipLaunchKernelGGL((k_trsv_bwd<T, 64>), dim3(g), dim3(256), 0, st, tv, tiles, xU, parts, xready, gready, counter,
+                     ctl, gate);
 }
 
 template void launch_chol_reset<double>(int*, int, int*, DevCtl*, int, hipStream_t);
 template void launch_chol_reset<float>(int*, int, int*, DevCtl*, int, hipStream_t);
 template void launch_chol<double>(const TileView&, double*, const int*, int*, int*, DevCtl*, int, int, hipStream_t);
 template void launch_chol<float>(const TileView&, float*, const int*, int*, int*, DevCtl*, int, int, hipStream_t);
-template void launch_trsv_bwd<double>(const TileView&, const double*, double*, int*, int*, DevCtl*, int, int,
-                                      hipStream_t);
-template void launch_trsv_bwd<float>(const TileView&, const float*, double*, int*, int*, DevCtl*, int, int,
-                                     hipStream_t);
+template void launch_trsv_bwd<double>(const TileView&, const double*, double*, double*, int*, int*, int*, DevCtl*,
+                                      int, int, hipStream_t);
+template void launch_trsv_bwd<float>(const TileView&, const float*, double*, double*, int*, int*, int*, DevCtl*,
+                                     int, int, hipStream_t);
 
 }  // namespace nsn

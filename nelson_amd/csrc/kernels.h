@@ -115,7 +115,7 @@ template <typename T>
 void launch_chol(const TileView& tv, T* tiles, const int* tile_order, int* ready, int* counter,
                  DevCtl* ctl, int gate, int max_wg, hipStream_t st);
 template <typename T>
-void launch_trsv_bwd(const TileView& tv, const T* tiles, double* xU, int* ready, int* counter,
-                     DevCtl* ctl, int gate, int max_wg, hipStream_t st);
+void launch_trsv_bwd(const TileView& tv, const T* tiles, double* xU, double* parts, int* xready, int* gready,
+                     int* counter, DevCtl* ctl, int gate, int max_wg, hipStream_t st);
 
 }  // namespace nsn
