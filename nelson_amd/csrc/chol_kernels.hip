@@ -91,100 +91,133 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_wave_barrier();
 }
 
-// Blocked right-looking Cholesky of a TB x TB column-major FP64 tile in LDS,
-// in 16-column steps: the 16x16 diagonal block by one wave (no workgroup
-// barrier inside), the panel below it by one thread per row, the trailing
-// lower part by all threads; three workgroup barriers per step.
-// g0 = global index of the tile's first row; rows >= n (the bS row and the
-// identity padding) take pivot 1 and never count as failures.
+// ---------------------------------------------------------------------------
+// Diagonal-tile factorisation and triangular solve on one 64x64 tile in LDS.
+// Both are split into two 32-column panels.  A panel is processed by one
+// wave with one row per lane and the panel row in registers: each step takes
+// its pivot from the owning lane (readlane), publishes column k once through
+// a 64-entry LDS buffer and every lane updates its own row; the 32x32 product
+// between the panels runs on the MFMA units of the four waves.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+template <typename T>
+__device__ __forceinline__ T readlane_t(T v, int l);
+template <>
+__device__ __forceinline__ double readlane_t<double>(double v, int l) { return readlane_f64(v, l); }
+template <>
+__device__ __forceinline__ float readlane_t<float>(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// right-looking Cholesky of columns c0 .. c0+31 of the 64-row FP64 tile A
+// (rows >= c0), wave 0 only; column k is broadcast from the owning lanes by
+// readlane.  Rows with global index >= n (the bS row and the identity
+// padding) take pivot 1 and never count as failures.
 template <int TB>
-__device__ __forceinline__ void potrf_lds(double* A, int g0, int n, DevCtl* ctl) {
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  for (int kb = 0; kb < TB; kb += 16) {
-    if (wv == 0) {
-      for (int k = 0; k < 16; k++) {
-        const double d = A[(kb + k) * TB + kb + k];
-        double piv;
-        if (g0 + kb + k >= n) piv = 1.0;
-        else if (!(d > 0.0) || !isfinite(d)) {
-          if (lane == 0) atomicAdd(&ctl->fail_s, 1);
-          piv = 1.0;
-        } else piv = sqrt(d);
-        const double inv = 1.0 / piv;
-        double lrk = 0.0;
-        if (lane > k && lane < 16) {
-          lrk = A[(kb + k) * TB + kb + lane] * inv;
-          A[(kb + k) * TB + kb + lane] = lrk;
-        }
-        wave_lds_sync();
-        if (lane == 0) A[(kb + k) * TB + kb + k] = piv;
-        if (lane > k && lane < 16)
-          for (int c = k + 1; c <= lane; c++) A[(kb + c) * TB + kb + lane] -= lrk * A[(kb + k) * TB + kb + c];
-        wave_lds_sync();
-      }
+__device__ __forceinline__ void potrf_panel32(double* A, int c0, int g0, int n, DevCtl* ctl) {
+  const int lane = threadIdx.x;
+  double a[32];
+#pragma unroll
+  for (int c = 0; c < 32; c++) a[c] = A[(c0 + c) * TB + lane];
+#pragma unroll
+  for (int k = 0; k < 32; k++) {
+    const double d = readlane_f64(a[k], c0 + k);
+    double piv;
+    if (g0 + c0 + k >= n) piv = 1.0;
+    else if (!(d > 0.0) || !isfinite(d)) {
+      if (lane == 0) atomicAdd(&ctl->fail_s, 1);
+      piv = 1.0;
+    } else piv = sqrt(d);
+    if (lane == c0 + k) a[k] = piv;
+    else if (lane > c0 + k) a[k] = a[k] / piv;
+#pragma unroll
+    for (int c = k + 1; c < 32; c++) {
+      const double lck = readlane_f64(a[k], c0 + c);
+      if (lane >= c0 + c) a[c] -= a[k] * lck;
     }
-    __syncthreads();
-    const int R0 = kb + 16, m = TB - R0;
-    if (m == 0) break;
-    if (tid < m) {
-      const int r = R0 + tid;
-      double x[16];
+  }
 #pragma unroll
-      for (int c = 0; c < 16; c++) x[c] = A[(kb + c) * TB + r];
+  for (int c = 0; c < 32; c++)
+    if (lane >= c0 + c) A[(c0 + c) * TB + lane] = a[c];
+}
+
+// X[:, c0..c0+31] = C[:, c0..c0+31] L[c0.., c0..]^-T for all 64 rows (one per
+// lane), wave 0 only: step c loads row c of the L block as one vector (lane q
+// holds L[c0+c][c0+q]) and broadcasts its entries by readlane.
+template <typename T, int TB>
+__device__ __forceinline__ void trsm_panel32(T* Cs, const T* Ls, int c0) {
+  const int lane = threadIdx.x;
+  T x[32];
 #pragma unroll
-      for (int c = 0; c < 16; c++) {
-        double s = x[c];
+  for (int c = 0; c < 32; c++) x[c] = Cs[(c0 + c) * TB + lane];
 #pragma unroll
-        for (int q = 0; q < c; q++) s -= x[q] * A[(kb + q) * TB + kb + c];
-        x[c] = s / A[(kb + c) * TB + kb + c];
-      }
+  for (int c = 0; c < 32; c++) {
+    const T lrow = Ls[(c0 + (lane & 31)) * TB + c0 + c];
+    T s = x[c];
 #pragma unroll
-      for (int c = 0; c < 16; c++) A[(kb + c) * TB + r] = x[c];
+    for (int q = 0; q < c; q++) s -= x[q] * readlane_t<T>(lrow, q);
+    x[c] = s / readlane_t<T>(lrow, c);
+  }
+#pragma unroll
+  for (int c = 0; c < 32; c++) Cs[(c0 + c) * TB + lane] = x[c];
+}
+
+// D[32+.., 32+..] -= P P^T or D[:, 32+..] -= X Lb^T with the 16x16x4 MFMA:
+// out rows [r0, r0+nr), out cols [32, 64), K = columns [0, 32);
+// left operand rows from `Lft` (column-major, TB rows), right operand rows
+// (the output columns) from `Rgt`.  nr = 32 (lower 2x2 blocks, 3 waves) or
+// 64 (4x2 blocks, 4 waves, two blocks each).
+template <typename T, int TB>
+__device__ __forceinline__ void mfma_update32(T* D, const T* Lft, const T* Rgt, int r0, int nr, bool lower) {
+  typedef typename Mfma<T>::acc_t acc_t;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int nbr = nr / 16;  // row blocks
+  for (int blk = wv; blk < nbr * 2; blk += 4) {
+    const int bi = blk % nbr, bj = blk / nbr;
+    if (lower && bi < bj) continue;
+    const int rr = r0 + bi * 16, cc = 32 + bj * 16;
+    acc_t acc;
+#pragma unroll
+    for (int vv = 0; vv < 4; vv++) acc[vv] = D[(cc + (lane & 15)) * TB + rr + Mfma<T>::row(vv, lane)];
+#pragma unroll
+    for (int m = 0; m < 32; m += 4) {
+      const int mm = m + (lane >> 4);
+      const T a = -Lft[mm * TB + rr + (lane & 15)];
+      const T b = Rgt[mm * TB + cc + (lane & 15)];
+      acc = Mfma<T>::fma(a, b, acc);
     }
-    __syncthreads();
-    for (int e = tid; e < m * m; e += 256) {
-      const int r = R0 + e % m, c = R0 + e / m;
-      if (c <= r) {
-        double s = 0.0;
 #pragma unroll
-        for (int q = 0; q < 16; q++) s += A[(kb + q) * TB + r] * A[(kb + q) * TB + c];
-        A[c * TB + r] -= s;
-      }
-    }
-    __syncthreads();
+    for (int vv = 0; vv < 4; vv++) D[(cc + (lane & 15)) * TB + rr + Mfma<T>::row(vv, lane)] = acc[vv];
   }
 }
 
-// X L^T = C in place, C (TB x TB) and L (lower) column-major in LDS.  One
-// wave, one row per lane; the columns are solved in blocks of 16 held in
-// registers (earlier blocks re-read from LDS), the L entries are broadcast
-// reads.  The other waves wait at the closing barrier.
+// Cholesky of a 64x64 FP64 tile in LDS (lower triangle written).
+template <int TB>
+__device__ __forceinline__ void potrf_lds(double* A, int g0, int n, DevCtl* ctl) {
+  static_assert(TB == 64, "two 32-column panels");
+  if (threadIdx.x < 64) potrf_panel32<TB>(A, 0, g0, n, ctl);
+  __syncthreads();
+  mfma_update32<double, TB>(A, A, A, 32, 32, true);
+  __syncthreads();
+  if (threadIdx.x < 64) potrf_panel32<TB>(A, 32, g0, n, ctl);
+  __syncthreads();
+}
+
+// X L^T = C in place, C (TB x TB) and L (lower) column-major in LDS.
 template <typename T, int TB>
 __device__ __forceinline__ void trsm_lds(T* Cs, const T* Ls) {
-  static_assert(TB == 64, "one row per lane");
-  constexpr int CB = 16;
-  const int tid = threadIdx.x;
-  if (tid < 64) {
-    for (int cb = 0; cb < TB; cb += CB) {
-      T x[CB];
-#pragma unroll
-      for (int c = 0; c < CB; c++) x[c] = Cs[(cb + c) * TB + tid];
-      for (int q = 0; q < cb; q++) {
-        const T xq = Cs[q * TB + tid];
-#pragma unroll
-        for (int c = 0; c < CB; c++) x[c] -= xq * Ls[q * TB + cb + c];
-      }
-#pragma unroll
-      for (int c = 0; c < CB; c++) {
-        T s = x[c];
-#pragma unroll
-        for (int q = 0; q < c; q++) s -= x[q] * Ls[(cb + q) * TB + cb + c];
-        x[c] = s / Ls[(cb + c) * TB + cb + c];
-      }
-#pragma unroll
-      for (int c = 0; c < CB; c++) Cs[(cb + c) * TB + tid] = x[c];
-    }
-  }
+  static_assert(TB == 64, "two 32-column panels");
+  if (threadIdx.x < 64) trsm_panel32<T, TB>(Cs, Ls, 0);
+  __syncthreads();
+  mfma_update32<T, TB>(Cs, Cs, Ls, 0, 64, false);
+  __syncthreads();
+  if (threadIdx.x < 64) trsm_panel32<T, TB>(Cs, Ls, 32);
   __syncthreads();
 }
 
