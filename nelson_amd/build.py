@@ -44,10 +44,10 @@ def _check_resources(src, out):
         m = re.search(r"Function Name: (\S+)", line)
         if m:
             fn = m.group(1)
-        m = re.search(r"LDS Size \[bytes\]: (\d+)", line)
+        m = re.search(r"LDS Size \[bytes[^\]]*\]: (\d+)", line)
         if m and int(m.group(1)) > LDS_LIMIT:
             raise RuntimeError(f"{src}: kernel {fn} uses {m.group(1)} bytes of LDS (> {LDS_LIMIT})")
-        m = re.search(r"ScratchSize \[bytes/lane\]: (\d+)", line)
+        m = re.search(r"ScratchSize \[bytes[^\]]*\]: (\d+)", line)
         if m and int(m.group(1)) > 0:
             raise RuntimeError(f"{src}: kernel {fn} spills ({m.group(1)} bytes/lane of scratch)")
 

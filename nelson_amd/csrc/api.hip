@@ -28,7 +28,7 @@ using namespace nsn;
 namespace {
 
 constexpr int kTB = 64;          // Cholesky tile
-constexpr int kCholMaxWG = 512;  // persistent workgroups of the factorisation
+constexpr int kCholMaxWG = 256;  // persistent workgroups of the factorisation
 constexpr int kTrsvMaxWG = 1024;
 
 struct DevBuf {

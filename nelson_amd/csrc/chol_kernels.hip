@@ -92,20 +92,20 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 // ---------------------------------------------------------------------------
-// Diagonal-tile factorisation and triangular solve on one 64x64 tile in LDS.
-// Both are split into two 32-column panels.  A panel is processed by one
-// wave with one row per lane and the panel row in registers: each step takes
-// its pivot from the owning lane (readlane), publishes column k once through
-// a 64-entry LDS buffer and every lane updates its own row; the 32x32 product
-// between the panels runs on the MFMA units of the four waves.
+// Diagonal-tile factorisation and triangular solve on one 64x64 tile in LDS,
+// in four 16-column panels.  A panel is processed by one wave with one row
+// per lane and the panel row in registers: each step takes its pivot from the
+// owning lane and broadcasts the pivot column by readlane; between panels
+// the trailing update runs on the MFMA units of the four waves.
 // ---------------------------------------------------------------------------
+constexpr int kPW = 16;  // panel width
+
 __device__ __forceinline__ double readlane_f64(double v, int l) {
   const long long b = __double_as_longlong(v);
   const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
   const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
-
 template <typename T>
 __device__ __forceinline__ T readlane_t(T v, int l);
 template <>
@@ -115,79 +115,103 @@ __device__ __forceinline__ float readlane_t<float>(float v, int l) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
 
-// right-looking Cholesky of columns c0 .. c0+31 of the 64-row FP64 tile A
-// (rows >= c0), wave 0 only; column k is broadcast from the owning lanes by
-// readlane.  Rows with global index >= n (the bS row and the identity
-// padding) take pivot 1 and never count as failures.
+// The reciprocal of the factor's diagonal entry c is kept in the strictly
+// upper triangle of the diagonal tile, which nothing else uses: (0, c+1) for
+// c < TB-1 and (1, TB-1) for the last one.  Rows 0 and 1 are never written by
+// the panel updates (rows >= 16) nor by the panels below the diagonal.
 template <int TB>
-__device__ __forceinline__ void potrf_panel32(double* A, int c0, int g0, int n, DevCtl* ctl) {
+__device__ __forceinline__ int rdiag_slot(int c) { return c < TB - 1 ? (c + 1) * TB : (TB - 1) * TB + 1; }
+
+// 1/sqrt(d) from the hardware estimate refined by two Newton steps
+__device__ __forceinline__ double rsqrt_f64(double d) {
+  double r = __builtin_amdgcn_rsq(d);
+  r = r * (1.5 - 0.5 * d * r * r);
+  r = r * (1.5 - 0.5 * d * r * r);
+  return r;
+}
+
+// right-looking Cholesky of columns c0 .. c0+15 of the 64-row FP64 tile A
+// (rows >= c0), wave 0 only.  Rows with global index >= n (the bS row and the
+// identity padding) take pivot 1 and never count as failures.  The pivot
+// reciprocals go to their slots in the upper triangle.
+template <int TB>
+__device__ __forceinline__ void potrf_panel(double* A, int c0, int g0, int n, DevCtl* ctl) {
   const int lane = threadIdx.x;
-  double a[32];
+  double a[kPW];
 #pragma unroll
-  for (int c = 0; c < 32; c++) a[c] = A[(c0 + c) * TB + lane];
+  for (int c = 0; c < kPW; c++) a[c] = A[(c0 + c) * TB + lane];
 #pragma unroll
-  for (int k = 0; k < 32; k++) {
+  for (int k = 0; k < kPW; k++) {
     const double d = readlane_f64(a[k], c0 + k);
-    double piv;
-    if (g0 + c0 + k >= n) piv = 1.0;
-    else if (!(d > 0.0) || !isfinite(d)) {
+    double piv, r;
+    if (g0 + c0 + k >= n) {
+      piv = 1.0;
+      r = 1.0;
+    } else if (!(d > 0.0) || !isfinite(d)) {
       if (lane == 0) atomicAdd(&ctl->fail_s, 1);
       piv = 1.0;
-    } else piv = sqrt(d);
-    if (lane == c0 + k) a[k] = piv;
-    else if (lane > c0 + k) a[k] = a[k] / piv;
+      r = 1.0;
+    } else {
+      r = rsqrt_f64(d);
+      piv = d * r;
+    }
+    if (lane == c0 + k) {
+      a[k] = piv;
+      A[rdiag_slot<TB>(c0 + k)] = r;
+    } else if (lane > c0 + k) {
+      a[k] = a[k] * r;
+    }
 #pragma unroll
-    for (int c = k + 1; c < 32; c++) {
+    for (int c = k + 1; c < kPW; c++) {
       const double lck = readlane_f64(a[k], c0 + c);
       if (lane >= c0 + c) a[c] -= a[k] * lck;
     }
   }
 #pragma unroll
-  for (int c = 0; c < 32; c++)
+  for (int c = 0; c < kPW; c++)
     if (lane >= c0 + c) A[(c0 + c) * TB + lane] = a[c];
 }
 
-// X[:, c0..c0+31] = C[:, c0..c0+31] L[c0.., c0..]^-T for all 64 rows (one per
-// lane), wave 0 only: step c loads row c of the L block as one vector (lane q
-// holds L[c0+c][c0+q]) and broadcasts its entries by readlane.
+// X[:, c0..c0+15] = C[:, c0..c0+15] L[c0.., c0..]^-T for all 64 rows (one
+// per lane), wave 0 only: step c loads row c of the L block as one vector
+// (lane q holds L[c0+c][c0+q]) and broadcasts its entries by readlane; the
+// diagonal is applied through the reciprocals kept in L's upper triangle.
 template <typename T, int TB>
-__device__ __forceinline__ void trsm_panel32(T* Cs, const T* Ls, int c0) {
+__device__ __forceinline__ void trsm_panel(T* Cs, const T* Ls, int c0) {
   const int lane = threadIdx.x;
-  T x[32];
+  T x[kPW];
 #pragma unroll
-  for (int c = 0; c < 32; c++) x[c] = Cs[(c0 + c) * TB + lane];
+  for (int c = 0; c < kPW; c++) x[c] = Cs[(c0 + c) * TB + lane];
 #pragma unroll
-  for (int c = 0; c < 32; c++) {
-    const T lrow = Ls[(c0 + (lane & 31)) * TB + c0 + c];
+  for (int c = 0; c < kPW; c++) {
+    const T lrow = Ls[(c0 + (lane & (kPW - 1))) * TB + c0 + c];
     T s = x[c];
 #pragma unroll
     for (int q = 0; q < c; q++) s -= x[q] * readlane_t<T>(lrow, q);
-    x[c] = s / readlane_t<T>(lrow, c);
+    x[c] = s * Ls[rdiag_slot<TB>(c0 + c)];
   }
 #pragma unroll
-  for (int c = 0; c < 32; c++) Cs[(c0 + c) * TB + lane] = x[c];
+  for (int c = 0; c < kPW; c++) Cs[(c0 + c) * TB + lane] = x[c];
 }
 
-// D[32+.., 32+..] -= P P^T or D[:, 32+..] -= X Lb^T with the 16x16x4 MFMA:
-// out rows [r0, r0+nr), out cols [32, 64), K = columns [0, 32);
-// left operand rows from `Lft` (column-major, TB rows), right operand rows
-// (the output columns) from `Rgt`.  nr = 32 (lower 2x2 blocks, 3 waves) or
-// 64 (4x2 blocks, 4 waves, two blocks each).
+// D[r, c] -= sum_{m < K} Lft[r][k0+m] Rgt[c][k0+m] for rows [r0, TB),
+// columns [c0, TB), K = kPW, with the 16x16x4 MFMA on the four waves; with
+// `lower` only the blocks on or below the diagonal (c block <= r block).
 template <typename T, int TB>
-__device__ __forceinline__ void mfma_update32(T* D, const T* Lft, const T* Rgt, int r0, int nr, bool lower) {
+__device__ __forceinline__ void mfma_update(T* D, const T* Lft, const T* Rgt, int r0, int c0, int k0, bool lower) {
   typedef typename Mfma<T>::acc_t acc_t;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int nbr = nr / 16;  // row blocks
-  for (int blk = wv; blk < nbr * 2; blk += 4) {
+  const int nbr = (TB - r0) / 16, nbc = (TB - c0) / 16;
+  for (int blk = wv; blk < nbr * nbc; blk += 4) {
     const int bi = blk % nbr, bj = blk / nbr;
-    if (lower && bi < bj) continue;
-    const int rr = r0 + bi * 16, cc = 32 + bj * 16;
+    const int rr = r0 + bi * 16, cc = c0 + bj * 16;
+    if (lower && cc > rr) continue;
     acc_t acc;
 #pragma unroll
     for (int vv = 0; vv < 4; vv++) acc[vv] = D[(cc + (lane & 15)) * TB + rr + Mfma<T>::row(vv, lane)];
 #pragma unroll
-    for (int m = 0; m < 32; m += 4) {
-      const int mm = m + (lane >> 4);
+    for (int m = 0; m < kPW; m += 4) {
+      const int mm = k0 + m + (lane >> 4);
       const T a = -Lft[mm * TB + rr + (lane & 15)];
       const T b = Rgt[mm * TB + cc + (lane & 15)];
       acc = Mfma<T>::fma(a, b, acc);
@@ -197,28 +221,32 @@ __device__ __forceinline__ void mfma_update32(T* D, const T* Lft, const T* Rgt, 
   }
 }
 
-// Cholesky of a 64x64 FP64 tile in LDS (lower triangle written).
+// Cholesky of a 64x64 FP64 tile in LDS (lower triangle written, the
+// reciprocals of its diagonal in the upper triangle).
 template <int TB>
 __device__ __forceinline__ void potrf_lds(double* A, int g0, int n, DevCtl* ctl) {
-  static_assert(TB == 64, "two 32-column panels");
-  if (threadIdx.x < 64) potrf_panel32<TB>(A, 0, g0, n, ctl);
-  __syncthreads();
-  mfma_update32<double, TB>(A, A, A, 32, 32, true);
-  __syncthreads();
-  if (threadIdx.x < 64) potrf_panel32<TB>(A, 32, g0, n, ctl);
-  __syncthreads();
+  for (int kb = 0; kb < TB; kb += kPW) {
+    if (threadIdx.x < 64) potrf_panel<TB>(A, kb, g0, n, ctl);
+    __syncthreads();
+    if (kb + kPW < TB) {
+      mfma_update<double, TB>(A, A, A, kb + kPW, kb + kPW, kb, true);
+      __syncthreads();
+    }
+  }
 }
 
-// X L^T = C in place, C (TB x TB) and L (lower) column-major in LDS.
+// X L^T = C in place, C (TB x TB) and L (lower, diagonal reciprocals in the
+// upper triangle) column-major in LDS.
 template <typename T, int TB>
 __device__ __forceinline__ void trsm_lds(T* Cs, const T* Ls) {
-  static_assert(TB == 64, "two 32-column panels");
-  if (threadIdx.x < 64) trsm_panel32<T, TB>(Cs, Ls, 0);
-  __syncthreads();
-  mfma_update32<T, TB>(Cs, Cs, Ls, 0, 64, false);
-  __syncthreads();
-  if (threadIdx.x < 64) trsm_panel32<T, TB>(Cs, Ls, 32);
-  __syncthreads();
+  for (int kb = 0; kb < TB; kb += kPW) {
+    if (threadIdx.x < 64) trsm_panel<T, TB>(Cs, Ls, kb);
+    __syncthreads();
+    if (kb + kPW < TB) {
+      mfma_update<T, TB>(Cs, Cs, Ls, 0, kb + kPW, kb, false);
+      __syncthreads();
+    }
+  }
 }
 
 // Tile DAG factorisation.  Per tile (i, j): C_ij -= sum_k L_ik L_jk^T with
