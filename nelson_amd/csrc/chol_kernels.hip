@@ -130,21 +130,23 @@ __device__ __forceinline__ double rsqrt_f64(double d) {
   return r;
 }
 
-// right-looking Cholesky of columns c0 .. c0+15 of the 64-row FP64 tile A
-// (rows >= c0), wave 0 only.  Rows with global index >= n (the bS row and the
+// right-looking Cholesky of columns C0 .. C0+15 of the 64-row FP64 tile A
+// (rows >= C0), wave 0 only.  Rows with global index >= n (the bS row and the
 // identity padding) take pivot 1 and never count as failures.  The pivot
-// reciprocals go to their slots in the upper triangle.
-template <int TB>
-__device__ __forceinline__ void potrf_panel(double* A, int c0, int g0, int n, DevCtl* ctl) {
+// reciprocals go to their slots in the upper triangle.  Entries above the
+// diagonal in the registers are garbage and never read or written back, so
+// the updates run on every lane without predication.
+template <int TB, int C0>
+__device__ __forceinline__ void potrf_panel(double* A, int g0, int n, DevCtl* ctl) {
   const int lane = threadIdx.x;
   double a[kPW];
 #pragma unroll
-  for (int c = 0; c < kPW; c++) a[c] = A[(c0 + c) * TB + lane];
+  for (int c = 0; c < kPW; c++) a[c] = A[(C0 + c) * TB + lane];
 #pragma unroll
   for (int k = 0; k < kPW; k++) {
-    const double d = readlane_f64(a[k], c0 + k);
+    const double d = readlane_f64(a[k], C0 + k);
     double piv, r;
-    if (g0 + c0 + k >= n) {
+    if (g0 + C0 + k >= n) {
       piv = 1.0;
       r = 1.0;
     } else if (!(d > 0.0) || !isfinite(d)) {
@@ -155,43 +157,39 @@ __device__ __forceinline__ void potrf_panel(double* A, int c0, int g0, int n, De
       r = rsqrt_f64(d);
       piv = d * r;
     }
-    if (lane == c0 + k) {
-      a[k] = piv;
-      A[rdiag_slot<TB>(c0 + k)] = r;
-    } else if (lane > c0 + k) {
-      a[k] = a[k] * r;
-    }
+    a[k] = (lane == C0 + k) ? piv : a[k] * r;
 #pragma unroll
-    for (int c = k + 1; c < kPW; c++) {
-      const double lck = readlane_f64(a[k], c0 + c);
-      if (lane >= c0 + c) a[c] -= a[k] * lck;
-    }
+    for (int c = k + 1; c < kPW; c++) a[c] -= a[k] * readlane_f64(a[k], C0 + c);
+    if (lane == C0 + k) A[rdiag_slot<TB>(C0 + k)] = r;
   }
 #pragma unroll
   for (int c = 0; c < kPW; c++)
-    if (lane >= c0 + c) A[(c0 + c) * TB + lane] = a[c];
+    if (lane >= C0 + c) A[(C0 + c) * TB + lane] = a[c];
 }
 
-// X[:, c0..c0+15] = C[:, c0..c0+15] L[c0.., c0..]^-T for all 64 rows (one
+// X[:, C0..C0+15] = C[:, C0..C0+15] L[C0.., C0..]^-T for all 64 rows (one
 // per lane), wave 0 only: step c loads row c of the L block as one vector
-// (lane q holds L[c0+c][c0+q]) and broadcasts its entries by readlane; the
+// (lane q holds L[C0+c][C0+q]) and broadcasts its entries by readlane; the
 // diagonal is applied through the reciprocals kept in L's upper triangle.
-template <typename T, int TB>
-__device__ __forceinline__ void trsm_panel(T* Cs, const T* Ls, int c0) {
+template <typename T, int TB, int C0>
+__device__ __forceinline__ void trsm_panel(T* Cs, const T* Ls) {
   const int lane = threadIdx.x;
-  T x[kPW];
-#pragma unroll
-  for (int c = 0; c < kPW; c++) x[c] = Cs[(c0 + c) * TB + lane];
+  T x[kPW], rd[kPW];
 #pragma unroll
   for (int c = 0; c < kPW; c++) {
-    const T lrow = Ls[(c0 + (lane & (kPW - 1))) * TB + c0 + c];
+    x[c] = Cs[(C0 + c) * TB + lane];
+    rd[c] = Ls[rdiag_slot<TB>(C0 + c)];
+  }
+#pragma unroll
+  for (int c = 0; c < kPW; c++) {
+    const T lrow = Ls[(C0 + (lane & (kPW - 1))) * TB + C0 + c];
     T s = x[c];
 #pragma unroll
     for (int q = 0; q < c; q++) s -= x[q] * readlane_t<T>(lrow, q);
-    x[c] = s * Ls[rdiag_slot<TB>(c0 + c)];
+    x[c] = s * rd[c];
   }
 #pragma unroll
-  for (int c = 0; c < kPW; c++) Cs[(c0 + c) * TB + lane] = x[c];
+  for (int c = 0; c < kPW; c++) Cs[(C0 + c) * TB + lane] = x[c];
 }
 
 // D[r, c] -= sum_{m < K} Lft[r][k0+m] Rgt[c][k0+m] for rows [r0, TB),
@@ -223,30 +221,44 @@ __device__ __forceinline__ void mfma_update(T* D, const T* Lft, const T* Rgt, in
 
 // Cholesky of a 64x64 FP64 tile in LDS (lower triangle written, the
 // reciprocals of its diagonal in the upper triangle).
+template <int TB, int KB>
+__device__ __forceinline__ void potrf_step(double* A, int g0, int n, DevCtl* ctl) {
+  if (threadIdx.x < 64) potrf_panel<TB, KB>(A, g0, n, ctl);
+  __syncthreads();
+  if constexpr (KB + kPW < TB) {
+    mfma_update<double, TB>(A, A, A, KB + kPW, KB + kPW, KB, true);
+    __syncthreads();
+  }
+}
+
 template <int TB>
 __device__ __forceinline__ void potrf_lds(double* A, int g0, int n, DevCtl* ctl) {
-  for (int kb = 0; kb < TB; kb += kPW) {
-    if (threadIdx.x < 64) potrf_panel<TB>(A, kb, g0, n, ctl);
-    __syncthreads();
-    if (kb + kPW < TB) {
-      mfma_update<double, TB>(A, A, A, kb + kPW, kb + kPW, kb, true);
-      __syncthreads();
-    }
-  }
+  static_assert(TB == 4 * kPW, "four panels");
+  potrf_step<TB, 0>(A, g0, n, ctl);
+  potrf_step<TB, kPW>(A, g0, n, ctl);
+  potrf_step<TB, 2 * kPW>(A, g0, n, ctl);
+  potrf_step<TB, 3 * kPW>(A, g0, n, ctl);
 }
 
 // X L^T = C in place, C (TB x TB) and L (lower, diagonal reciprocals in the
 // upper triangle) column-major in LDS.
+template <typename T, int TB, int KB>
+__device__ __forceinline__ void trsm_step(T* Cs, const T* Ls) {
+  if (threadIdx.x < 64) trsm_panel<T, TB, KB>(Cs, Ls);
+  __syncthreads();
+  if constexpr (KB + kPW < TB) {
+    mfma_update<T, TB>(Cs, Cs, Ls, 0, KB + kPW, KB, false);
+    __syncthreads();
+  }
+}
+
 template <typename T, int TB>
 __device__ __forceinline__ void trsm_lds(T* Cs, const T* Ls) {
-  for (int kb = 0; kb < TB; kb += kPW) {
-    if (threadIdx.x < 64) trsm_panel<T, TB>(Cs, Ls, kb);
-    __syncthreads();
-    if (kb + kPW < TB) {
-      mfma_update<T, TB>(Cs, Cs, Ls, 0, kb + kPW, kb, false);
-      __syncthreads();
-    }
-  }
+  static_assert(TB == 4 * kPW, "four panels");
+  trsm_step<T, TB, 0>(Cs, Ls);
+  trsm_step<T, TB, kPW>(Cs, Ls);
+  trsm_step<T, TB, 2 * kPW>(Cs, Ls);
+  trsm_step<T, TB, 3 * kPW>(Cs, Ls);
 }
 
 // Tile DAG factorisation.  Per tile (i, j): C_ij -= sum_k L_ik L_jk^T with
