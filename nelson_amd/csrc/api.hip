@@ -69,7 +69,7 @@ struct nsn_problem {
   Scratch scr{};
   double *cams = nullptr, *pts = nullptr, *cams_bak = nullptr, *pts_bak = nullptr;
   void *U = nullptr, *bU = nullptr, *V = nullptr, *bV = nullptr, *W = nullptr, *Vinv = nullptr, *Y = nullptr,
-       *tW = nullptr, *Sd = nullptr, *bS = nullptr, *tiles = nullptr;
+       *tW = nullptr, *Sd = nullptr, *bS = nullptr, *tiles = nullptr, *inv = nullptr;
   double *xU = nullptr, *xV = nullptr;
   int *tile_order = nullptr, *tile_blk_ptr = nullptr, *tile_blk = nullptr, *sb_i = nullptr, *sb_j = nullptr,
       *sb_ptr = nullptr, *sb_pairs = nullptr;
@@ -130,10 +130,12 @@ void seq_schur(nsn_problem* h, int gate, int checks, bool timed) {
                     st);
   if (timed) (void)hipEventRecord(h->ev[2], st);
   launch_chol_reset<T>(h->flags, h->nflags, h->counters, h->ctl, gate, st);
-  launch_chol<T>(h->tv, (T*)h->tiles, h->tile_order, h->flags, h->counters, h->ctl, gate, kCholMaxWG, st);
+  launch_chol<T>(h->tv, (T*)h->tiles, (T*)h->inv, h->tile_order, h->flags, h->counters, h->ctl, gate, kCholMaxWG,
+                 st);
   if (timed) (void)hipEventRecord(h->ev[3], st);
-  launch_trsv_bwd<T>(h->tv, (const T*)h->tiles, h->xU, h->parts, h->flags + h->tv.ntiles,
-                     h->flags + h->tv.ntiles + h->NR, h->counters + 1, h->ctl, gate, kTrsvMaxWG, st);
+  launch_trsv<T>(h->tv, (const T*)h->tiles, (const T*)h->inv, true, nullptr, h->xU, h->parts,
+                 h->flags + h->tv.ntiles, h->flags + h->tv.ntiles + h->NR, h->counters + 1, h->ctl, gate, kTrsvMaxWG,
+                 st);
   if (timed) (void)hipEventRecord(h->ev[4], st);
   launch_backsub<T>(h->view, (const T*)h->W, (const T*)h->Vinv, (const T*)h->bV, h->xU, h->xV, h->scr, h->ctl,
                     gate, st);
@@ -365,6 +367,7 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
   h->Sd = a_T(21 * (size_t)NU);
   h->bS = a_T(6 * (size_t)NU);
   h->tiles = a_T((size_t)tv.ntiles * kTB * kTB);
+  h->inv = a_T((size_t)tv.NT * kTB * kTB);
   h->xU = a_dbl(6 * (size_t)NU);
   h->parts = a_dbl((size_t)h->NR * h->NR * kTB);
   h->xV = a_dbl(3 * (size_t)NV);
@@ -389,7 +392,7 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
   h->ctl = m.alloc<DevCtl>(1);
   h->log_cap = 4096;
   h->log = m.alloc<DevLog>(h->log_cap);
-  if (!h->log || !h->ctl || !h->tiles || !h->W || !h->Y || !h->sb_pairs)
+  if (!h->log || !h->ctl || !h->tiles || !h->inv || !h->W || !h->Y || !h->sb_pairs)
     return fail(NSN_ENOMEM, "device allocation failed (%.1f MB of tiles)",
                 (double)tv.ntiles * kTB * kTB * es / 1e6);
   int rc;

@@ -92,11 +92,13 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 // ---------------------------------------------------------------------------
-// Diagonal-tile factorisation and triangular solve on one 64x64 tile in LDS,
-// in four 16-column panels.  A panel is processed by one wave with one row
-// per lane and the panel row in registers: each step takes its pivot from the
-// owning lane and broadcasts the pivot column by readlane; between panels
-// the trailing update runs on the MFMA units of the four waves.
+// Diagonal-tile factorisation on one 64x64 FP64 tile in LDS, in four
+// 16-column panels.  A panel is processed by one wave with one row per lane
+// and the panel row in registers: each step takes its pivot from the owning
+// lane and broadcasts the pivot column by readlane; between panels the
+// trailing update runs on the MFMA units of the four waves.  The factored
+// tile is then inverted, so that the solves against it (the tiles below it
+// and the backward substitution) become matrix products.
 // ---------------------------------------------------------------------------
 constexpr int kPW = 16;  // panel width
 
@@ -105,14 +107,6 @@ __device__ __forceinline__ double readlane_f64(double v, int l) {
   const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
   const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-template <typename T>
-__device__ __forceinline__ T readlane_t(T v, int l);
-template <>
-__device__ __forceinline__ double readlane_t<double>(double v, int l) { return readlane_f64(v, l); }
-template <>
-__device__ __forceinline__ float readlane_t<float>(float v, int l) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
 
 // The reciprocal of the factor's diagonal entry c is kept in the strictly
@@ -167,31 +161,6 @@ __device__ __forceinline__ void potrf_panel(double* A, int g0, int n, DevCtl* ct
     if (lane >= C0 + c) A[(C0 + c) * TB + lane] = a[c];
 }
 
-// X[:, C0..C0+15] = C[:, C0..C0+15] L[C0.., C0..]^-T for all 64 rows (one
-// per lane), wave 0 only: step c loads row c of the L block as one vector
-// (lane q holds L[C0+c][C0+q]) and broadcasts its entries by readlane; the
-// diagonal is applied through the reciprocals kept in L's upper triangle.
-template <typename T, int TB, int C0>
-__device__ __forceinline__ void trsm_panel(T* Cs, const T* Ls) {
-  const int lane = threadIdx.x;
-  T x[kPW], rd[kPW];
-#pragma unroll
-  for (int c = 0; c < kPW; c++) {
-    x[c] = Cs[(C0 + c) * TB + lane];
-    rd[c] = Ls[rdiag_slot<TB>(C0 + c)];
-  }
-#pragma unroll
-  for (int c = 0; c < kPW; c++) {
-    const T lrow = Ls[(C0 + (lane & (kPW - 1))) * TB + C0 + c];
-    T s = x[c];
-#pragma unroll
-    for (int q = 0; q < c; q++) s -= x[q] * readlane_t<T>(lrow, q);
-    x[c] = s * rd[c];
-  }
-#pragma unroll
-  for (int c = 0; c < kPW; c++) Cs[(C0 + c) * TB + lane] = x[c];
-}
-
 // D[r, c] -= sum_{m < K} Lft[r][k0+m] Rgt[c][k0+m] for rows [r0, TB),
 // columns [c0, TB), K = kPW, with the 16x16x4 MFMA on the four waves; with
 // `lower` only the blocks on or below the diagonal (c block <= r block).
@@ -240,43 +209,87 @@ __device__ __forceinline__ void potrf_lds(double* A, int g0, int n, DevCtl* ctl)
   potrf_step<TB, 3 * kPW>(A, g0, n, ctl);
 }
 
-// X L^T = C in place, C (TB x TB) and L (lower, diagonal reciprocals in the
-// upper triangle) column-major in LDS.
-template <typename T, int TB, int KB>
-__device__ __forceinline__ void trsm_step(T* Cs, const T* Ls) {
-  if (threadIdx.x < 64) trsm_panel<T, TB, KB>(Cs, Ls);
+// Li = L^-1 for the lower-triangular 64x64 FP64 tile L in LDS (diagonal
+// reciprocals in their upper-triangle slots); Li is written lower with
+// zeros above the diagonal.  By 16x16 blocks: the four diagonal blocks are
+// inverted by forward substitution (wave b, lane j < 16 solves
+// L_bb x = e_j), then the off-diagonal blocks level by level,
+//   Li_ib,jb = -Li_ib,ib * sum_{kb = jb}^{ib-1} L_ib,kb Li_kb,jb,
+// one wave per block on the MFMA units.  The inner product is staged in the
+// strictly upper block (jb, ib) of Li, which is cleared at the end.
+template <int TB>
+__device__ __forceinline__ void tri_inverse(const double* L, double* Li) {
+  static_assert(TB == 64, "four 16x16 blocks");
+  typedef Mfma<double>::acc_t acc_t;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane < 16) {
+    const int b0 = wv * 16, j = lane;
+    double x[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      double s = (i == j) ? 1.0 : 0.0;
+#pragma unroll
+      for (int k = 0; k < i; k++) s -= L[(b0 + k) * TB + b0 + i] * x[k];
+      x[i] = (i >= j) ? s * L[rdiag_slot<TB>(b0 + i)] : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < 16; i++) Li[(b0 + j) * TB + b0 + i] = x[i];
+  }
   __syncthreads();
-  if constexpr (KB + kPW < TB) {
-    mfma_update<T, TB>(Cs, Cs, Ls, 0, KB + kPW, KB, false);
+#pragma unroll
+  for (int lev = 1; lev < 4; lev++) {
+    if (wv < 4 - lev) {
+      const int jb = wv, ib = wv + lev;
+      acc_t t = {0.0, 0.0, 0.0, 0.0};
+      for (int kb = jb; kb < ib; kb++) {
+#pragma unroll
+        for (int m = 0; m < 16; m += 4) {
+          const int mm = 16 * kb + m + (lane >> 4);
+          t = Mfma<double>::fma(L[mm * TB + 16 * ib + (lane & 15)], Li[(16 * jb + (lane & 15)) * TB + mm], t);
+        }
+      }
+#pragma unroll
+      for (int v = 0; v < 4; v++) Li[(16 * ib + Mfma<double>::row(v, lane)) * TB + 16 * jb + (lane & 15)] = t[v];
+      wave_lds_sync();
+      acc_t o = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int m = 0; m < 16; m += 4) {
+        const int mm = 16 * ib + m + (lane >> 4);
+        o = Mfma<double>::fma(-Li[mm * TB + 16 * ib + (lane & 15)], Li[mm * TB + 16 * jb + (lane & 15)], o);
+      }
+#pragma unroll
+      for (int v = 0; v < 4; v++) Li[(16 * jb + (lane & 15)) * TB + 16 * ib + Mfma<double>::row(v, lane)] = o[v];
+    }
     __syncthreads();
   }
-}
-
-template <typename T, int TB>
-__device__ __forceinline__ void trsm_lds(T* Cs, const T* Ls) {
-  static_assert(TB == 4 * kPW, "four panels");
-  trsm_step<T, TB, 0>(Cs, Ls);
-  trsm_step<T, TB, kPW>(Cs, Ls);
-  trsm_step<T, TB, 2 * kPW>(Cs, Ls);
-  trsm_step<T, TB, 3 * kPW>(Cs, Ls);
+  // clear the staging blocks (jb, ib), jb < ib: (0,1) (0,2) (0,3) (1,2) (1,3) (2,3)
+  for (int e = threadIdx.x; e < 6 * 256; e += 256) {
+    const int blk = e >> 8, w = e & 255;
+    const int jb = blk < 3 ? 0 : (blk < 5 ? 1 : 2);
+    const int ib = blk < 3 ? blk + 1 : (blk < 5 ? blk - 1 : 3);
+    Li[(16 * ib + (w >> 4)) * TB + 16 * jb + (w & 15)] = 0.0;
+  }
+  __syncthreads();
 }
 
 // Tile DAG factorisation.  Per tile (i, j): C_ij -= sum_k L_ik L_jk^T with
 // the 16x16x4 MFMA, operands loaded straight from global memory into the
 // MFMA operand registers (each wave owns a 32x32 quadrant of C); dependency
 // waits are batched (one acquire fence for every run of already-published
-// k); then POTRF (i == j, FP64 in LDS) or the triangular solve against
-// L_jj (i > j).  LDS: 64 KiB = two TB x TB FP64 regions.
+// k); then, for i == j, POTRF and the inverse L_jj^-1 (FP64 in LDS; the
+// inverse goes to inv[j]), or, for i > j, L_ij = C_ij L_jj^-T as one more
+// MFMA product.  LDS: 64 KiB = two TB x TB FP64 regions.
 template <typename T, int TB>
-__global__ __launch_bounds__(256) void k_chol(TileView tv, T* __restrict__ tiles, const int* __restrict__ tile_order,
-                                              int* ready, int* counter, DevCtl* ctl, int gate) {
+__global__ __launch_bounds__(256) void k_chol(TileView tv, T* __restrict__ tiles, T* __restrict__ inv,
+                                              const int* __restrict__ tile_order, int* ready, int* counter,
+                                              DevCtl* ctl, int gate) {
   static_assert(TB == 64, "the MFMA tiling below assumes 64x64 tiles and 4 waves");
   if (!chol_gate(ctl, gate)) return;
   typedef typename Mfma<T>::acc_t acc_t;
   __shared__ double lds[2 * TB * TB];
   double* R0d = lds;                 // FP64 tile (POTRF)
-  T* R0 = (T*)lds;                   // C tile (TRSM)
-  T* R1 = (T*)(lds + TB * TB);       // L_jj (TRSM)
+  double* R1d = lds + TB * TB;       // its inverse
+  T* R0 = (T*)lds;                   // C tile, then X (off-diagonal solve)
   int* shv = (int*)(lds + TB * TB);  // scalars, in region 1 while it is unused
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int wr = wv & 1, wc = wv >> 1;
@@ -347,7 +360,12 @@ __global__ __launch_bounds__(256) void k_chol(TileView tv, T* __restrict__ tiles
           }
       __syncthreads();
       potrf_lds<TB>(R0d, j * TB, tv.n, ctl);
-      for (int e = threadIdx.x; e < TB * TB; e += 256) Cg[e] = (T)R0d[e];
+      tri_inverse<TB>(R0d, R1d);
+      T* Ig = inv + (size_t)j * T2;
+      for (int e = threadIdx.x; e < TB * TB; e += 256) {
+        Cg[e] = (T)R0d[e];
+        Ig[e] = (T)R1d[e];
+      }
     } else {
 #pragma unroll
       for (int bi = 0; bi < 2; bi++)
@@ -363,92 +381,182 @@ __global__ __launch_bounds__(256) void k_chol(TileView tv, T* __restrict__ tiles
       if (threadIdx.x == 0) wait_flag(ready + tjj, ctl);
       __syncthreads();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-      const T* Lg = tiles + (size_t)tjj * T2;
-      for (int e = threadIdx.x; e < TB * TB; e += 256) R1[e] = Lg[e];
+      // X = C L_jj^-T with the inverse from the diagonal tile: A operand C
+      // from LDS, B operand Linv^T straight from global memory
+      const T* Ig = inv + (size_t)j * T2 + wc * 32 + (lane & 15);
+      const T* Cs = R0 + wr * 32 + (lane & 15);
+      const int kq = lane >> 4;
+#pragma unroll
+      for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+        for (int bj = 0; bj < 2; bj++) acc[bi][bj] = acc_t{};
+#pragma unroll 8
+      for (int m = 0; m < TB; m += 4) {
+        const int off = (m + kq) * TB;
+        const T a0 = Cs[off], a1 = Cs[off + 16];
+        const T b0 = Ig[off], b1 = Ig[off + 16];
+        acc[0][0] = Mfma<T>::fma(a0, b0, acc[0][0]);
+        acc[0][1] = Mfma<T>::fma(a0, b1, acc[0][1]);
+        acc[1][0] = Mfma<T>::fma(a1, b0, acc[1][0]);
+        acc[1][1] = Mfma<T>::fma(a1, b1, acc[1][1]);
+      }
       __syncthreads();
-      trsm_lds<T, TB>(R0, R1);
+#pragma unroll
+      for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+        for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+          for (int vv = 0; vv < 4; vv++) {
+            const int r = wr * 32 + bi * 16 + Mfma<T>::row(vv, lane);
+            const int c = wc * 32 + bj * 16 + (lane & 15);
+            R0[c * TB + r] = acc[bi][bj][vv];
+          }
+      __syncthreads();
       for (int e = threadIdx.x; e < TB * TB; e += 256) Cg[e] = R0[e];
     }
     publish_flag(ready + t);
   }
 }
 
-// Backward solve L^T x = y, y = row n of the factor (the forward solve's
-// result), as a task DAG over the NR tile rows covering 0..n-1.  Tasks in
-// queue order, for k = NR-1 down to 0:
-//   D_k      : x_k = L_kk^-T (y_k - sum_{k' > k} g_{k',k}), the sum taken in
-//              a fixed order (deterministic), one wave for the substitution;
-//   G_{k,i}  : g_{k,i} = L_ki^T x_k for i = k-1 .. 0, into its own slot.
-// Every task waits only on tasks earlier in the queue.
-template <typename T, int TB>
-__global__ __launch_bounds__(256) void k_trsv_bwd(TileView tv, const T* __restrict__ tiles, double* xU,
-                                                  double* __restrict__ parts, int* xready, int* gready,
-                                                  int* counter, DevCtl* ctl, int gate) {
+// 64 x 64 tile times a 64-vector in LDS, FP64 accumulation, with the tile
+// prefetched into registers (16 entries per thread) before the vector is
+// known.  TRANS: out[c] = sum_r A[r][c] x[r], thread (c = tid/4, part
+// tid%4) holds rows 16*part .. +15 of column c (contiguous); combined by two
+// lane exchanges.  Otherwise: out[r] = sum_c A[r][c] x[c], thread (r =
+// tid%64, wave w) holds columns 16*w .. +15 of row r (coalesced loads);
+// combined across the four waves through LDS.  Both sums run in a fixed
+// order.
+template <typename T, int TB, bool TRANS>
+struct TileGemv {
+  T a[16];
+  __device__ __forceinline__ void load(const T* A) {
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < 16; q++)
+      a[q] = TRANS ? A[(tid >> 2) * TB + (tid & 3) * 16 + q] : A[((tid >> 6) * 16 + q) * TB + (tid & 63)];
+  }
+  // returns the full sum in every thread whose index() is valid (TRANS: all
+  // four threads of a column; otherwise: wave 0); red: 4 x TB doubles of LDS
+  __device__ __forceinline__ double apply(const double* x, double* red) const {
+    const int tid = threadIdx.x;
+    double s = 0.0;
+    const int base = TRANS ? (tid & 3) * 16 : (tid >> 6) * 16;
+#pragma unroll
+    for (int q = 0; q < 16; q++) s += (double)a[q] * x[base + q];
+    if (TRANS) {
+      s += __shfl_xor(s, 1);
+      s += __shfl_xor(s, 2);
+      return s;
+    }
+    red[(tid >> 6) * TB + (tid & 63)] = s;
+    __syncthreads();
+    s = 0.0;
+    if (tid < TB) s = ((red[tid] + red[TB + tid]) + red[2 * TB + tid]) + red[3 * TB + tid];
+    __syncthreads();
+    return s;
+  }
+  static __device__ __forceinline__ int index() { return TRANS ? (threadIdx.x >> 2) : (int)threadIdx.x; }
+  static __device__ __forceinline__ bool owner() { return TRANS ? (threadIdx.x & 3) == 0 : threadIdx.x < TB; }
+};
+
+// Triangular solve with the tile factor and the inverses of its diagonal
+// tiles (inv), as a task DAG over the NR = ceil(n / TB) tile rows of S.
+//   BWD (L^T x = y): step s handles block k = NR-1-s: task D_k, then G_{k,i}
+//                    for i = k-2 down to 0;
+//   forward (L z = r): step s handles block k = s: task D_k, then G_{i,k} for
+//                    i = k+2 up to NR-1.
+// D_k:  v = rhs_k - sum_p part(k <- p) - N sol_nb,  sol_k = Linv_kk^(T) v,
+//       nb the neighbouring block (k+1 backward, k-1 forward) and N its tile
+//       (L_{k+1,k}^T or L_{k,k-1}); the parts of the other solved blocks p
+//       are summed in a fixed order (deterministic).
+// G:    part(i <- k) = L_ki^T x_k (backward) or L_ik z_k (forward), into its
+//       own slot.
+// rhs == nullptr takes the right-hand side from row n of the factor (the
+// forward solve of bS that the factorisation leaves there).  Every task waits
+// only for tasks earlier in the queue, so the DAG cannot deadlock.
+template <typename T, int TB, bool BWD>
+__global__ __launch_bounds__(256) void k_trsv(TileView tv, const T* __restrict__ tiles, const T* __restrict__ inv,
+                                              const double* rhs, double* sol, double* __restrict__ parts,
+                                              int* xready, int* gready, int* counter, DevCtl* ctl, int gate) {
   static_assert(TB == 64, "64-row tiles");
   if (!chol_gate(ctl, gate)) return;
-  __shared__ double acc[4][TB];
+  typedef TileGemv<T, TB, BWD> Gemv;
+  __shared__ double red[4 * TB];
   __shared__ double xs[TB];
+  __shared__ double acc[4][TB];
   __shared__ int sh_t;
   const int n = tv.n, NT = tv.NT;
   const int NR = (n + TB - 1) / TB;
-  const int ntask = NR + NR * (NR - 1) / 2;
+  const int ntask = NR + (NR >= 2 ? (NR - 1) * (NR - 2) / 2 : 0);
   const size_t T2 = (size_t)TB * TB;
-  const int tid = threadIdx.x, lane = tid & 63;
+  const int tid = threadIdx.x;
   for (int round = 0; round < ntask; round++) {  // bounded
     if (tid == 0) sh_t = atomicAdd(counter, 1);
     __syncthreads();
     int t = sh_t;
     __syncthreads();
     if (t >= ntask) break;
-    // decode: block of k holds D_k then G_{k,k-1..0}; blocks for k = NR-1 down
-    int k = NR - 1;
-    while (t >= k + 1) { t -= k + 1; k--; }
+    int s = 0;
+    while (t >= 1 + (NR - 2 - s > 0 ? NR - 2 - s : 0)) {
+      t -= 1 + (NR - 2 - s > 0 ? NR - 2 - s : 0);
+      s++;
+    }
+    const int k = BWD ? NR - 1 - s : s;
     if (t == 0) {
       // ---- D_k ----
-      const T* ytile = tiles + (size_t)tile_index(n / TB, k, NT) * T2;
-      if (tid == 0)
-        for (int kp = NR - 1; kp > k; kp--) wait_flag(gready + (size_t)kp * NR + k, ctl);
+      const int nb = BWD ? k + 1 : k - 1;
+      const bool has_nb = nb >= 0 && nb < NR;
+      Gemv gd, gn;
+      gd.load(inv + (size_t)k * T2);
+      if (has_nb) gn.load(tiles + (size_t)(BWD ? tile_index(nb, k, NT) : tile_index(k, nb, NT)) * T2);
+      double r0 = 0.0;
+      if (tid < TB && k * TB + tid < n)
+        r0 = rhs ? rhs[k * TB + tid] : (double)tiles[(size_t)tile_index(n / TB, k, NT) * T2 + tid * TB + (n % TB)];
+      if (tid == 0) {
+        if (has_nb) wait_flag(xready + nb, ctl);
+        if (BWD)
+          for (int p = NR - 1; p > k + 1; p--) wait_flag(gready + (size_t)k * NR + p, ctl);
+        else
+          for (int p = 0; p < k - 1; p++) wait_flag(gready + (size_t)k * NR + p, ctl);
+      }
       __syncthreads();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       {
-        const int c = tid % TB, g = tid / TB;  // 4 interleaved partial sums per entry
-        double s = 0.0;
-        for (int kp = NR - 1 - g; kp > k; kp -= 4) s += parts[((size_t)kp * NR + k) * TB + c];
-        acc[g][c] = s;
+        // 4 interleaved partial sums per entry over the other solved blocks
+        const int e = tid % TB, g = tid / TB;
+        double a = 0.0;
+        if (BWD)
+          for (int p = NR - 1 - g; p > k + 1; p -= 4) a += parts[((size_t)k * NR + p) * TB + e];
+        else
+          for (int p = g; p < k - 1; p += 4) a += parts[((size_t)k * NR + p) * TB + e];
+        acc[g][e] = a;
+        if (tid < TB) xs[tid] = (has_nb && nb * TB + tid < n) ? sol[nb * TB + tid] : 0.0;
       }
       __syncthreads();
-      if (tid < 64) {
-        const int R = k * TB + tid;
-        double a = (R < n) ? (double)ytile[tid * TB + (n % TB)] : 0.0;
-        a -= ((acc[0][tid] + acc[1][tid]) + acc[2][tid]) + acc[3][tid];
-        const T* Dg = tiles + (size_t)tile_index(k, k, NT) * T2;
-        for (int c = TB - 1; c >= 0; c--) {
-          double xc = 0.0;
-          if (k * TB + c < n) xc = __shfl(a, c) / (double)Dg[c * TB + c];
-          if (tid == c) a = xc;
-          if (tid < c) a -= (double)Dg[tid * TB + c] * xc;
-        }
-        if (R < n) xU[R] = a;
-      }
+      double h = 0.0;
+      if (has_nb) h = gn.apply(xs, red);
+      __syncthreads();
+      if (has_nb && Gemv::owner()) acc[0][Gemv::index()] += h;  // neighbour term first, then the parts
+      __syncthreads();
+      if (tid < TB)
+        xs[tid] = (k * TB + tid < n) ? r0 - (((acc[0][tid] + acc[1][tid]) + acc[2][tid]) + acc[3][tid]) : 0.0;
+      __syncthreads();
+      const double x = gd.apply(xs, red);
+      if (Gemv::owner() && k * TB + Gemv::index() < n) sol[k * TB + Gemv::index()] = x;
       publish_flag(xready + k);
     } else {
-      // ---- G_{k,i} ----
-      const int i = k - t;
+      // ---- G: contribution of block k to block i ----
+      const int i = BWD ? k - 1 - t : k + 1 + t;
+      Gemv gg;
+      gg.load(tiles + (size_t)(BWD ? tile_index(k, i, NT) : tile_index(i, k, NT)) * T2);
       if (tid == 0) wait_flag(xready + k, ctl);
       __syncthreads();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-      if (tid < TB) xs[tid] = (k * TB + tid < n) ? xU[k * TB + tid] : 0.0;
+      if (tid < TB) xs[tid] = (k * TB + tid < n) ? sol[k * TB + tid] : 0.0;
       __syncthreads();
-      const T* Lg = tiles + (size_t)tile_index(k, i, NT) * T2;
-      const int c = tid >> 2, sub = tid & 3;
-      double s = 0.0;
-#pragma unroll
-      for (int r = sub * 16; r < sub * 16 + 16; r++) s += (double)Lg[c * TB + r] * xs[r];
-      s += __shfl_xor(s, 1);
-      s += __shfl_xor(s, 2);
-      if (sub == 0) parts[((size_t)k * NR + i) * TB + c] = s;
-      (void)lane;
-      publish_flag(gready + (size_t)k * NR + i);
+      const double v = gg.apply(xs, red);
+      if (Gemv::owner()) parts[((size_t)i * NR + k) * TB + Gemv::index()] = v;
+      publish_flag(gready + (size_t)i * NR + k);
     }
   }
 }
@@ -468,29 +576,37 @@ void launch_chol_reset(int* flags, int nflags, int* counters, DevCtl* ctl, int g
 }
 
 template <typename T>
-void launch_chol(const TileView& tv, T* tiles, const int* tile_order, int* ready, int* counter, DevCtl* ctl, int gate,
-                 int max_wg, hipStream_t st) {
+void launch_chol(const TileView& tv, T* tiles, T* inv, const int* tile_order, int* ready, int* counter, DevCtl* ctl,
+                 int gate, int max_wg, hipStream_t st) {
   int g = tv.ntiles < max_wg ? tv.ntiles : max_wg;
-  hipLaunchKernelGGL((k_chol<T, 64>), dim3(g), dim3(256), 0, st, tv, tiles, tile_order, ready, counter, ctl, gate);
+  hipLaunchKernelGGL((k_chol<T, 64>), dim3(g), dim3(256), 0, st, tv, tiles, inv, tile_order, ready, counter, ctl,
+                     gate);
 }
 
 template <typename T>
-void launch_trsv_bwd(const TileView& tv, const T* tiles, double* xU, double* parts, int* xready, int* gready,
-                     int* counter, DevCtl* ctl, int gate, int max_wg, hipStream_t st) {
+void launch_trsv(const TileView& tv, const T* tiles, const T* inv, bool backward, const double* rhs, double* sol,
+                 double* parts, int* xready, int* gready, int* counter, DevCtl* ctl, int gate, int max_wg,
+                 hipStream_t st) {
   const int NR = (tv.n + 63) / 64;
-  const int ntask = NR + NR * (NR - 1) / 2;
+  const int ntask = NR + (NR >= 2 ? (NR - 1) * (NR - 2) / 2 : 0);
   int g = ntask < max_wg ? ntask : max_wg;
-  hipLaunchKernelGGL((k_trsv_bwd<T, 64>), dim3(g), dim3(256), 0, st, tv, tiles, xU, parts, xready, gready, counter,
-                     ctl, gate);
+  if (backward)
+    hipLaunchKernelGGL((k_trsv<T, 64, true>), dim3(g), dim3(256), 0, st, tv, tiles, inv, rhs, sol, parts, xready,
+                       gready, counter, ctl, gate);
+  else
+    hipLaunchKernelGGL((k_trsv<T, 64, false>), dim3(g), dim3(256), 0, st, tv, tiles, inv, rhs, sol, parts, xready,
+                       gready, counter, ctl, gate);
 }
 
 template void launch_chol_reset<double>(int*, int, int*, DevCtl*, int, hipStream_t);
 template void launch_chol_reset<float>(int*, int, int*, DevCtl*, int, hipStream_t);
-template void launch_chol<double>(const TileView&, double*, const int*, int*, int*, DevCtl*, int, int, hipStream_t);
-template void launch_chol<float>(const TileView&, float*, const int*, int*, int*, DevCtl*, int, int, hipStream_t);
-template void launch_trsv_bwd<double>(const TileView&, const double*, double*, double*, int*, int*, int*, DevCtl*,
-                                      int, int, hipStream_t);
-template void launch_trsv_bwd<float>(const TileView&, const float*, double*, double*, int*, int*, int*, DevCtl*,
-                                     int, int, hipStream_t);
+template void launch_chol<double>(const TileView&, double*, double*, const int*, int*, int*, DevCtl*, int, int,
+                                  hipStream_t);
+template void launch_chol<float>(const TileView&, float*, float*, const int*, int*, int*, DevCtl*, int, int,
+                                 hipStream_t);
+template void launch_trsv<double>(const TileView&, const double*, const double*, bool, const double*, double*,
+                                  double*, int*, int*, int*, DevCtl*, int, int, hipStream_t);
+template void launch_trsv<float>(const TileView&, const float*, const float*, bool, const double*, double*, double*,
+                                 int*, int*, int*, DevCtl*, int, int, hipStream_t);
 
 }  // namespace nsn

@@ -111,11 +111,15 @@ void launch_zero(T* p, long long n, hipStream_t st);
 // ---- dense tile Cholesky and triangular solve (chol_kernels.hip) --------
 template <typename T>
 void launch_chol_reset(int* flags, int nflags, int* counters, DevCtl* ctl, int gate, hipStream_t st);
+// factor in place; inv[j] receives L_jj^-1 (TB x TB, column-major)
 template <typename T>
-void launch_chol(const TileView& tv, T* tiles, const int* tile_order, int* ready, int* counter,
+void launch_chol(const TileView& tv, T* tiles, T* inv, const int* tile_order, int* ready, int* counter,
                  DevCtl* ctl, int gate, int max_wg, hipStream_t st);
+// L^T sol = rhs (backward) or L sol = rhs (forward) over the first n rows;
+// rhs == nullptr: row n of the factor.  xready: NR flags, gready: NR*NR.
 template <typename T>
-void launch_trsv_bwd(const TileView& tv, const T* tiles, double* xU, double* parts, int* xready, int* gready,
-                     int* counter, DevCtl* ctl, int gate, int max_wg, hipStream_t st);
+void launch_trsv(const TileView& tv, const T* tiles, const T* inv, bool backward, const double* rhs, double* sol,
+                 double* parts, int* xready, int* gready, int* counter, DevCtl* ctl, int gate, int max_wg,
+                 hipStream_t st);
 
 }  // namespace nsn
