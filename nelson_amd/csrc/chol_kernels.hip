@@ -316,16 +316,21 @@ __global__ __launch_bounds__(256) void k_chol(TileView tv, T* __restrict__ tiles
         }
     int k = 0;
     while (k < j) {
-      if (threadIdx.x == 0) {
-        wait_flag(ready + tile_index(i, k, NT), ctl);
-        if (i != j) wait_flag(ready + tile_index(j, k, NT), ctl);
-        int ke = k + 1;
-        while (ke < j && ke - k < 16 &&
-               __hip_atomic_load(ready + tile_index(i, ke, NT), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &&
+      // wave 0: lane 0 waits for step k, then lane l checks step k + l; the
+      // batch is the run of published steps starting at k (at least one)
+      if (threadIdx.x < 64) {
+        if (lane == 0) {
+          wait_flag(ready + tile_index(i, k, NT), ctl);
+          if (i != j) wait_flag(ready + tile_index(j, k, NT), ctl);
+        }
+        bool ok = false;
+        if (k + lane < j)
+          ok = __hip_atomic_load(ready + tile_index(i, k + lane, NT), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &&
                (i == j ||
-                __hip_atomic_load(ready + tile_index(j, ke, NT), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
-          ke++;
-        shv[1] = ke;
+                __hip_atomic_load(ready + tile_index(j, k + lane, NT), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        const unsigned long long m = __ballot(ok);
+        const int run = ~m == 0ull ? 64 : __builtin_ctzll(~m);
+        if (lane == 0) shv[1] = k + (run > 0 ? run : 1);
       }
       __syncthreads();
       const int ke = shv[1];
@@ -461,19 +466,22 @@ struct TileGemv {
 
 // Triangular solve with the tile factor and the inverses of its diagonal
 // tiles (inv), as a task DAG over the NR = ceil(n / TB) tile rows of S.
-//   BWD (L^T x = y): step s handles block k = NR-1-s: task D_k, then G_{k,i}
-//                    for i = k-2 down to 0;
-//   forward (L z = r): step s handles block k = s: task D_k, then G_{i,k} for
-//                    i = k+2 up to NR-1.
-// D_k:  v = rhs_k - sum_p part(k <- p) - N sol_nb,  sol_k = Linv_kk^(T) v,
-//       nb the neighbouring block (k+1 backward, k-1 forward) and N its tile
-//       (L_{k+1,k}^T or L_{k,k-1}); the parts of the other solved blocks p
-//       are summed in a fixed order (deterministic).
-// G:    part(i <- k) = L_ki^T x_k (backward) or L_ik z_k (forward), into its
-//       own slot.
+//   backward (L^T x = y): step s handles block k = NR-1-s: task D_k, then
+//                         G_{k,i} for i = k-2 down to 0;
+//   forward (L z = r):    step s handles block k = s: task D_k, then G_{k,i}
+//                         for i = k+2 up to NR-1.
+// G_{k,i} adds block k's contribution to block i (L_ki^T x_k backward,
+// L_ik z_k forward) to the running sum of the blocks solved before k, so
+// part(i, k) = part(i, k_prev) + contribution: one chain per destination i,
+// in solve order (deterministic).
+// D_k:  v = rhs_k - (part(k, last) + N sol_nb),  sol_k = Linv_kk^(T) v, with
+//       nb the neighbouring block (k+1 backward, k-1 forward), N its tile
+//       (L_{k+1,k}^T or L_{k,k-1}) and part(k, last) the chain of every other
+//       solved block.
 // rhs == nullptr takes the right-hand side from row n of the factor (the
 // forward solve of bS that the factorisation leaves there).  Every task waits
-// only for tasks earlier in the queue, so the DAG cannot deadlock.
+// for at most two tasks, both earlier in the queue, so the DAG cannot
+// deadlock.
 template <typename T, int TB, bool BWD>
 __global__ __launch_bounds__(256) void k_trsv(TileView tv, const T* __restrict__ tiles, const T* __restrict__ inv,
                                               const double* rhs, double* sol, double* __restrict__ parts,
@@ -483,13 +491,14 @@ __global__ __launch_bounds__(256) void k_trsv(TileView tv, const T* __restrict__
   typedef TileGemv<T, TB, BWD> Gemv;
   __shared__ double red[4 * TB];
   __shared__ double xs[TB];
-  __shared__ double acc[4][TB];
+  __shared__ double hs[TB];
   __shared__ int sh_t;
   const int n = tv.n, NT = tv.NT;
   const int NR = (n + TB - 1) / TB;
   const int ntask = NR + (NR >= 2 ? (NR - 1) * (NR - 2) / 2 : 0);
   const size_t T2 = (size_t)TB * TB;
   const int tid = threadIdx.x;
+  const int step = BWD ? -1 : 1;  // solve order
   for (int round = 0; round < ntask; round++) {  // bounded
     if (tid == 0) sh_t = atomicAdd(counter, 1);
     __syncthreads();
@@ -504,8 +513,9 @@ __global__ __launch_bounds__(256) void k_trsv(TileView tv, const T* __restrict__
     const int k = BWD ? NR - 1 - s : s;
     if (t == 0) {
       // ---- D_k ----
-      const int nb = BWD ? k + 1 : k - 1;
-      const bool has_nb = nb >= 0 && nb < NR;
+      const int nb = k - step;       // solved just before k
+      const int pl = k - 2 * step;   // last block of k's partial-sum chain
+      const bool has_nb = nb >= 0 && nb < NR, has_part = pl >= 0 && pl < NR;
       Gemv gd, gn;
       gd.load(inv + (size_t)k * T2);
       if (has_nb) gn.load(tiles + (size_t)(BWD ? tile_index(nb, k, NT) : tile_index(k, nb, NT)) * T2);
@@ -514,48 +524,45 @@ __global__ __launch_bounds__(256) void k_trsv(TileView tv, const T* __restrict__
         r0 = rhs ? rhs[k * TB + tid] : (double)tiles[(size_t)tile_index(n / TB, k, NT) * T2 + tid * TB + (n % TB)];
       if (tid == 0) {
         if (has_nb) wait_flag(xready + nb, ctl);
-        if (BWD)
-          for (int p = NR - 1; p > k + 1; p--) wait_flag(gready + (size_t)k * NR + p, ctl);
-        else
-          for (int p = 0; p < k - 1; p++) wait_flag(gready + (size_t)k * NR + p, ctl);
+        if (has_part) wait_flag(gready + (size_t)k * NR + pl, ctl);
       }
       __syncthreads();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-      {
-        // 4 interleaved partial sums per entry over the other solved blocks
-        const int e = tid % TB, g = tid / TB;
-        double a = 0.0;
-        if (BWD)
-          for (int p = NR - 1 - g; p > k + 1; p -= 4) a += parts[((size_t)k * NR + p) * TB + e];
-        else
-          for (int p = g; p < k - 1; p += 4) a += parts[((size_t)k * NR + p) * TB + e];
-        acc[g][e] = a;
-        if (tid < TB) xs[tid] = (has_nb && nb * TB + tid < n) ? sol[nb * TB + tid] : 0.0;
+      double pk = 0.0;
+      if (tid < TB) {
+        xs[tid] = (has_nb && nb * TB + tid < n) ? sol[nb * TB + tid] : 0.0;
+        if (has_part) pk = parts[((size_t)k * NR + pl) * TB + tid];
       }
       __syncthreads();
       double h = 0.0;
       if (has_nb) h = gn.apply(xs, red);
+      if (Gemv::owner()) hs[Gemv::index()] = h;
       __syncthreads();
-      if (has_nb && Gemv::owner()) acc[0][Gemv::index()] += h;  // neighbour term first, then the parts
-      __syncthreads();
-      if (tid < TB)
-        xs[tid] = (k * TB + tid < n) ? r0 - (((acc[0][tid] + acc[1][tid]) + acc[2][tid]) + acc[3][tid]) : 0.0;
+      if (tid < TB) xs[tid] = (k * TB + tid < n) ? r0 - (pk + hs[tid]) : 0.0;
       __syncthreads();
       const double x = gd.apply(xs, red);
       if (Gemv::owner() && k * TB + Gemv::index() < n) sol[k * TB + Gemv::index()] = x;
       publish_flag(xready + k);
     } else {
-      // ---- G: contribution of block k to block i ----
+      // ---- G_{k,i}: part(i, k) = part(i, k - step) + contribution of block k ----
       const int i = BWD ? k - 1 - t : k + 1 + t;
+      const int kp = k - step;  // previous link of i's chain, if it is a source of i
+      const bool has_prev = BWD ? kp < NR : kp >= 0;
       Gemv gg;
       gg.load(tiles + (size_t)(BWD ? tile_index(k, i, NT) : tile_index(i, k, NT)) * T2);
-      if (tid == 0) wait_flag(xready + k, ctl);
+      if (tid == 0) {
+        wait_flag(xready + k, ctl);
+        if (has_prev) wait_flag(gready + (size_t)i * NR + kp, ctl);
+      }
       __syncthreads();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       if (tid < TB) xs[tid] = (k * TB + tid < n) ? sol[k * TB + tid] : 0.0;
       __syncthreads();
       const double v = gg.apply(xs, red);
-      if (Gemv::owner()) parts[((size_t)i * NR + k) * TB + Gemv::index()] = v;
+      if (Gemv::owner()) {
+        const int e = Gemv::index();
+        parts[((size_t)i * NR + k) * TB + e] = (has_prev ? parts[((size_t)i * NR + kp) * TB + e] : 0.0) + v;
+      }
       publish_flag(gready + (size_t)i * NR + k);
     }
   }
