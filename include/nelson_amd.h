@@ -188,8 +188,16 @@ int nsn_gn_solve(nsn_problem* h, const nsn_gn_settings* s, nsn_lm_result* res,
 /* per-phase device time of the last Schur step, milliseconds, in the order of
  * SolverCholeskySchurIterationTimeStat (SolverCholeskySchur.h:25-38):
  * [0] V^-1 and W V^-1, [1] bS + S assembly, [2] factorisation,
- * [3] triangular solves, [4] back-substitution.  n >= 5. */
+ * [3] triangular solves, [4] back-substitution, and with n >= 6
+ * [5] iterative refinement (mixed precision).  n >= 5. */
 int nsn_phase_times(nsn_problem* h, double* ms, int n);
+
+/* ---- mixed precision (no reference counterpart) ---------------------------
+ * Number of iterative-refinement steps applied to every NSN_MIXED increment
+ * (0..8; default 2): each step evaluates the FP64 residual of the damped
+ * normal equations from the edges and adds the FP32 Schur solve of it.  The
+ * gain-ratio denominator then uses the FP64 gradient.  Ignored for NSN_FP64. */
+int nsn_set_refinement(nsn_problem* h, int steps);
 
 /* ---- implicit-function-theorem adjoint (no reference counterpart) --------
  * At a converged state x*, with H the undamped Gauss-Newton Hessian

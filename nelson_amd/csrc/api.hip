@@ -75,7 +75,11 @@ struct nsn_problem {
       *sb_ptr = nullptr, *sb_pairs = nullptr;
   int *flags = nullptr, *counters = nullptr;
   int nflags = 0, NR = 0;
-  double* parts = nullptr;  // backward-solve partial products, NR x NR x TB
+  double* parts = nullptr;  // triangular-solve partial sums, NR x NR x TB
+  // iterative refinement (mixed precision)
+  int ir_steps = 0;
+  double *rU = nullptr, *rV = nullptr, *bU64 = nullptr, *bV64 = nullptr, *rS = nullptr, *zS = nullptr,
+         *zU = nullptr;
   DevCtl* ctl = nullptr;
   DevCtl* hctl = nullptr;  // pinned mirror
   DevLog* log = nullptr;
@@ -139,9 +143,29 @@ void seq_schur(nsn_problem* h, int gate, int checks, bool timed) {
   if (timed) (void)hipEventRecord(h->ev[4], st);
   launch_backsub<T>(h->view, (const T*)h->W, (const T*)h->Vinv, (const T*)h->bV, h->xU, h->xV, h->scr, h->ctl,
                     gate, st);
-  launch_xu_stats<T>(h->NU, h->xU, (const T*)h->bU, h->scr, h->ctl, gate, st);
-  launch_step_finish(h->scr, h->ctl, gate, checks, st);
   if (timed) (void)hipEventRecord(h->ev[5], st);
+  const int ir = (h->prec == NSN_MIXED) ? h->ir_steps : 0;
+  const int n = h->tv.n, NR = h->NR;
+  int* xready = h->flags + h->tv.ntiles;
+  int* gready = xready + NR;
+  for (int it = 0; it < ir; it++) {
+    // x += M^-1 (-b - (H + D) x), M the FP32 Schur solve
+    launch_ir_residual(h->view, h->xU, h->xV, h->rU, h->rV, h->bU64, h->bV64, h->ctl, gate, st);
+    launch_ir_reduce<T>(h->view, (const T*)h->Y, h->rU, h->rV, h->rS, h->ctl, gate, st);
+    launch_chol_reset<T>(xready, NR + NR * NR, h->counters + 1, h->ctl, gate, st);
+    launch_trsv<T>(h->tv, (const T*)h->tiles, (const T*)h->inv, false, h->rS, h->zS, h->parts, xready, gready,
+                   h->counters + 1, h->ctl, gate, kTrsvMaxWG, st);
+    launch_chol_reset<T>(xready, NR + NR * NR, h->counters + 1, h->ctl, gate, st);
+    launch_trsv<T>(h->tv, (const T*)h->tiles, (const T*)h->inv, true, h->zS, h->zU, h->parts, xready, gready,
+                   h->counters + 1, h->ctl, gate, kTrsvMaxWG, st);
+    launch_add(h->xU, h->zU, n, h->ctl, gate, st);
+    launch_backsub<T>(h->view, (const T*)h->W, (const T*)h->Vinv, (const T*)h->bV, h->zU, h->xV, h->scr, h->ctl,
+                      gate, st, h->rV, 1, h->bV64);
+  }
+  if (ir > 0) launch_xu_stats<double>(h->NU, h->xU, h->bU64, h->scr, h->ctl, gate, st);
+  else launch_xu_stats<T>(h->NU, h->xU, (const T*)h->bU, h->scr, h->ctl, gate, st);
+  launch_step_finish(h->scr, h->ctl, gate, checks, st);
+  if (timed) (void)hipEventRecord(h->ev[6], st);
 }
 
 // one LM sub-iteration (LevenbergMarquardt.hpp:67-137) as a fixed sequence
@@ -350,6 +374,7 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
   double* d_ce_w = a_dbl(h->Ec);
   int* d_wptr = a_int(NV + 1);
   int* d_w_cam = a_int(nW);
+  int* d_w_pt = a_int(nW);
   int* d_cw_ptr = a_int(NU + 1);
   int* d_cw_idx = a_int(nW);
   h->cams = a_dbl(12 * (size_t)C);
@@ -370,6 +395,15 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
   h->inv = a_T((size_t)tv.NT * kTB * kTB);
   h->xU = a_dbl(6 * (size_t)NU);
   h->parts = a_dbl((size_t)h->NR * h->NR * kTB);
+  if (h->prec == NSN_MIXED) {
+    h->rU = a_dbl(6 * (size_t)NU);
+    h->rV = a_dbl(3 * (size_t)NV);
+    h->bU64 = a_dbl(6 * (size_t)NU);
+    h->bV64 = a_dbl(3 * (size_t)NV);
+    h->rS = a_dbl(6 * (size_t)NU);
+    h->zS = a_dbl(6 * (size_t)NU);
+    h->zU = a_dbl(6 * (size_t)NU);
+  }
   h->xV = a_dbl(3 * (size_t)NV);
   h->tile_order = a_int(tile_order.size());
   h->tile_blk_ptr = a_int(tile_cnt.size());
@@ -379,7 +413,7 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
   h->sb_ptr = a_int(sb_ptr.size());
   h->sb_pairs = a_int(sb_pairs.size());
   h->flags = a_int(h->nflags);
-  h->counters = a_int(4);
+  h->counters = a_int(8);
   const int nblk_pts = std::max(1, (std::max(P, NV) + 255) / 256);
   const int nblk_cams = std::max(1, (6 * NU + 255) / 256);
   h->scr.part_chi2 = a_dbl(nblk_pts);
@@ -392,7 +426,8 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
   h->ctl = m.alloc<DevCtl>(1);
   h->log_cap = 4096;
   h->log = m.alloc<DevLog>(h->log_cap);
-  if (!h->log || !h->ctl || !h->tiles || !h->inv || !h->W || !h->Y || !h->sb_pairs)
+  if (!h->log || !h->ctl || !h->tiles || !h->inv || !h->W || !h->Y || !h->sb_pairs ||
+      (h->prec == NSN_MIXED && !h->zU))
     return fail(NSN_ENOMEM, "device allocation failed (%.1f MB of tiles)",
                 (double)tv.ntiles * kTB * kTB * es / 1e6);
   int rc;
@@ -401,7 +436,7 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
       (rc = upload(d_pt_ptr, pt_ptr)) || (rc = upload(d_pe_cam, pe_cam)) || (rc = upload(d_pe_obs, pe_obs)) ||
       (rc = upload(d_pe_w, pe_w)) || (rc = upload(d_pe_wblk, pe_wblk)) || (rc = upload(d_cam_ptr, cam_ptr)) ||
       (rc = upload(d_ce_pt, ce_pt)) || (rc = upload(d_ce_obs, ce_obs)) || (rc = upload(d_ce_w, ce_w)) ||
-      (rc = upload(d_wptr, wptr)) || (rc = upload(d_w_cam, h->w_cam)) || (rc = upload(d_cw_ptr, cw_ptr)) ||
+      (rc = upload(d_wptr, wptr)) || (rc = upload(d_w_cam, h->w_cam)) || (rc = upload(d_w_pt, h->w_pt)) || (rc = upload(d_cw_ptr, cw_ptr)) ||
       (rc = upload(d_cw_idx, cw_idx)) || (rc = upload(h->tile_order, tile_order)) ||
       (rc = upload(h->tile_blk_ptr, tile_cnt)) || (rc = upload(h->tile_blk, tile_blk)) ||
       (rc = upload(h->sb_i, sb_i)) || (rc = upload(h->sb_j, sb_j)) || (rc = upload(h->sb_ptr, sb_ptr)) ||
@@ -413,7 +448,7 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
   v.cam_var = d_cam_var; v.pt_var = d_pt_var; v.var_cam_user = d_vcu; v.var_pt_user = d_vpu;
   v.pt_ptr = d_pt_ptr; v.pe_cam = d_pe_cam; v.pe_obs = d_pe_obs; v.pe_w = d_pe_w; v.pe_wblk = d_pe_wblk;
   v.cam_ptr = d_cam_ptr; v.ce_pt = d_ce_pt; v.ce_obs = d_ce_obs; v.ce_w = d_ce_w;
-  v.wptr = d_wptr; v.w_cam = d_w_cam; v.cw_ptr = d_cw_ptr; v.cw_idx = d_cw_idx;
+  v.wptr = d_wptr; v.w_cam = d_w_cam; v.w_pt = d_w_pt; v.cw_ptr = d_cw_ptr; v.cw_idx = d_cw_idx;
   v.cams = h->cams; v.pts = h->pts;
   return NSN_OK;
 }
@@ -455,6 +490,7 @@ extern "C" int nsn_problem_create(const nsn_problem_desc* d, nsn_problem** out) 
   std::unique_ptr<nsn_problem> h(new nsn_problem());
   h->device = d->device;
   h->C = d->n_cameras; h->P = d->n_points; h->E = d->n_edges; h->prec = d->precision;
+  h->ir_steps = d->precision == NSN_MIXED ? 2 : 0;
   h->f = d->focal; h->cx = d->cx; h->cy = d->cy; h->huber = d->huber_delta;
   HIP_TRY(hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking));
   for (auto& e : h->ev) HIP_TRY(hipEventCreate(&e));
@@ -738,13 +774,26 @@ extern "C" int nsn_schur_step(nsn_problem* h, double rel, double abs_, double* d
 extern "C" int nsn_phase_times(nsn_problem* h, double* ms, int n) {
   if (!h || !ms || n < 5) return fail(NSN_EINVAL, "nsn_phase_times needs 5 slots");
   if (!h->stepped) return fail(NSN_ESTATE, "no Schur step has run");
-  HIP_TRY(hipEventSynchronize(h->ev[5]));
+  HIP_TRY(hipEventSynchronize(h->ev[6]));
   float t = 0;
-  const int pairs[5][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 4}, {4, 5}};
-  for (int i = 0; i < 5; i++) {
+  const int pairs[6][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 4}, {4, 5}, {5, 6}};
+  for (int i = 0; i < std::min(n, 6); i++) {
     HIP_TRY(hipEventElapsedTime(&t, h->ev[pairs[i][0]], h->ev[pairs[i][1]]));
     ms[i] = t;
   }
+  return NSN_OK;
+}
+
+extern "C" int nsn_set_refinement(nsn_problem* h, int steps) {
+  if (!h) return fail(NSN_EINVAL, "null argument");
+  if (steps < 0 || steps > 8) return fail(NSN_EINVAL, "nsn_set_refinement: steps must be in 0..8, got %d", steps);
+  if (steps != h->ir_steps && h->lm_graph) {
+    // the captured LM sub-iteration holds the old sequence
+    (void)hipGraphExecDestroy(h->lm_graph);
+    h->lm_graph = nullptr;
+    h->lm_graph_tried = false;
+  }
+  h->ir_steps = steps;
   return NSN_OK;
 }
 
@@ -769,7 +818,10 @@ extern "C" int nsn_rho(nsn_problem* h, double mu, double old_chi2, double* rho) 
     std::vector<double> x(nu + nv), b(nu + nv);
     HIP_TRY(hipMemcpy(x.data(), h->xU, sizeof(double) * nu, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(x.data() + nu, h->xV, sizeof(double) * nv, hipMemcpyDeviceToHost));
-    if (h->prec == NSN_MIXED) {
+    if (h->prec == NSN_MIXED && h->ir_steps > 0) {
+      HIP_TRY(hipMemcpy(b.data(), h->bU64, sizeof(double) * nu, hipMemcpyDeviceToHost));
+      HIP_TRY(hipMemcpy(b.data() + nu, h->bV64, sizeof(double) * nv, hipMemcpyDeviceToHost));
+    } else if (h->prec == NSN_MIXED) {
       std::vector<float> bf(nu + nv);
       HIP_TRY(hipMemcpy(bf.data(), h->bU, sizeof(float) * nu, hipMemcpyDeviceToHost));
       HIP_TRY(hipMemcpy(bf.data() + nu, h->bV, sizeof(float) * nv, hipMemcpyDeviceToHost));

@@ -509,7 +509,8 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_backsub(BAView v, const T* __restrict__ W, const T* __restrict__ Vinv,
                                                  const T* __restrict__ bV, const double* __restrict__ xU,
                                                  double* __restrict__ xV, double* __restrict__ part,
-                                                 DevCtl* ctl, int gate) {
+                                                 DevCtl* ctl, int gate, const double* __restrict__ rhs_v,
+                                                 int add, const double* __restrict__ bden) {
   if (!gate_open(ctl, gate)) return;
   __shared__ double sh[4];
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -528,13 +529,29 @@ __global__ __launch_bounds__(256) void k_backsub(BAView v, const T* __restrict__
         s2 += (double)w[3 * i + 2] * xi;
       }
     }
-    const double bv0 = bV[3 * (size_t)q], bv1 = bV[3 * (size_t)q + 1], bv2 = bV[3 * (size_t)q + 2];
-    const double t0 = -bv0 - s0, t1 = -bv1 - s1, t2 = -bv2 - s2;
+    double bv0, bv1, bv2;
+    if (bden) {
+      bv0 = bden[3 * (size_t)q]; bv1 = bden[3 * (size_t)q + 1]; bv2 = bden[3 * (size_t)q + 2];
+    } else {
+      bv0 = bV[3 * (size_t)q]; bv1 = bV[3 * (size_t)q + 1]; bv2 = bV[3 * (size_t)q + 2];
+    }
+    double t0, t1, t2;
+    if (rhs_v) {
+      t0 = rhs_v[3 * (size_t)q] - s0; t1 = rhs_v[3 * (size_t)q + 1] - s1; t2 = rhs_v[3 * (size_t)q + 2] - s2;
+    } else {
+      t0 = -(double)bV[3 * (size_t)q] - s0; t1 = -(double)bV[3 * (size_t)q + 1] - s1;
+      t2 = -(double)bV[3 * (size_t)q + 2] - s2;
+    }
     const T* m = Vinv + 6 * (size_t)q;
     const double m0 = m[0], m1 = m[1], m2 = m[2], m3 = m[3], m4 = m[4], m5 = m[5];
-    const double x0 = m0 * t0 + m1 * t1 + m2 * t2;
-    const double x1 = m1 * t0 + m3 * t1 + m4 * t2;
-    const double x2 = m2 * t0 + m4 * t1 + m5 * t2;
+    double x0 = m0 * t0 + m1 * t1 + m2 * t2;
+    double x1 = m1 * t0 + m3 * t1 + m4 * t2;
+    double x2 = m2 * t0 + m4 * t1 + m5 * t2;
+    if (add) {
+      x0 += xV[3 * (size_t)q];
+      x1 += xV[3 * (size_t)q + 1];
+      x2 += xV[3 * (size_t)q + 2];
+    }
     xV[3 * (size_t)q] = x0;
     xV[3 * (size_t)q + 1] = x1;
     xV[3 * (size_t)q + 2] = x2;
@@ -548,6 +565,170 @@ __global__ __launch_bounds__(256) void k_backsub(BAView v, const T* __restrict__
     part[2 * blockIdx.x] = a;
     part[2 * blockIdx.x + 1] = b;
   }
+}
+
+// ---------------------------------------------------------------------------
+// Iterative refinement of the mixed-precision increment.  The FP32 Schur
+// solve M approximates the damped normal equations (H + D) x = -b; each
+// refinement step takes the FP64 residual
+//   res = -b - (H + D) x,  H = sum_e J_e^T om_e J_e,  b = sum_e J_e^T om_e r_e,
+//   D = diag(rel H_ii + abs) (SolverCholeskySchur.hpp:121-124,
+//   MatrixDiagInv.hpp:36-40),
+// evaluated matrix-free from the edges with J, r and om in FP64 at the
+// current state (no FP32 block enters it), and adds M^-1 res to x.  The
+// FP64 gradient b is written as well: the gain-ratio denominator
+// x^T (mu x - b) uses it (DoubleSection.h:300-311).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_ir_res_points(BAView v, const double* __restrict__ xU,
+                                                       const double* __restrict__ xV, double* __restrict__ rV,
+                                                       double* __restrict__ bV64, DevCtl* ctl, int gate) {
+  if (!gate_open(ctl, gate)) return;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= v.P) return;
+  const int pv = v.pt_var[p];
+  if (pv < 0) return;
+  const double X0 = v.pts[3 * p], X1 = v.pts[3 * p + 1], X2 = v.pts[3 * p + 2];
+  const double d0 = xV[3 * (size_t)pv], d1 = xV[3 * (size_t)pv + 1], d2 = xV[3 * (size_t)pv + 2];
+  double b[3] = {0, 0, 0}, hq[3] = {0, 0, 0}, dg[3] = {0, 0, 0};
+  const int e1 = v.pt_ptr[p + 1];
+  for (int e = v.pt_ptr[p]; e < e1; e++) {
+    const int c = v.pe_cam[e];
+    double R[12];
+#pragma unroll
+    for (int i = 0; i < 12; i++) R[i] = v.cams[12 * c + i];
+    const Proj pr = project(v, R, X0, X1, X2, v.pe_obs[2 * e], v.pe_obs[2 * e + 1], v.pe_w[e]);
+    double g0[3], g1[3];
+    jac_pt<double>(v, pr, R, g0, g1);
+    double q0 = g0[0] * d0 + g0[1] * d1 + g0[2] * d2;
+    double q1 = g1[0] * d0 + g1[1] * d1 + g1[2] * d2;
+    const int cv = v.cam_var[c];
+    if (cv >= 0) {
+      double j0[6], j1[6];
+      jac_cam<double>(v, pr, j0, j1);
+      const double* xc = xU + 6 * (size_t)cv;
+#pragma unroll
+      for (int i = 0; i < 6; i++) {
+        q0 += j0[i] * xc[i];
+        q1 += j1[i] * xc[i];
+      }
+    }
+    const double om = pr.om;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      b[k] += om * (g0[k] * pr.r0 + g1[k] * pr.r1);
+      hq[k] += om * (g0[k] * q0 + g1[k] * q1);
+      dg[k] += om * (g0[k] * g0[k] + g1[k] * g1[k]);
+    }
+  }
+  const double rel = ctl->rel, ab = ctl->abs_;
+  const double d[3] = {d0, d1, d2};
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    rV[3 * (size_t)pv + k] = -b[k] - hq[k] - (rel * dg[k] + ab) * d[k];
+    bV64[3 * (size_t)pv + k] = b[k];
+  }
+}
+
+// one workgroup per variable camera; deterministic reduction
+__global__ __launch_bounds__(256) void k_ir_res_cams(BAView v, const double* __restrict__ xU,
+                                                     const double* __restrict__ xV, double* __restrict__ rU,
+                                                     double* __restrict__ bU64, DevCtl* ctl, int gate) {
+  if (!gate_open(ctl, gate)) return;
+  __shared__ double red[4][18];
+  const int u = blockIdx.x;
+  const int c = v.var_cam_user[u];
+  double R[12];
+#pragma unroll
+  for (int i = 0; i < 12; i++) R[i] = v.cams[12 * c + i];
+  double xc[6];
+#pragma unroll
+  for (int i = 0; i < 6; i++) xc[i] = xU[6 * (size_t)u + i];
+  double acc[18];
+#pragma unroll
+  for (int i = 0; i < 18; i++) acc[i] = 0.0;
+  const int e1 = v.cam_ptr[u + 1];
+  for (int e = v.cam_ptr[u] + threadIdx.x; e < e1; e += 256) {
+    const int q = v.ce_pt[e];
+    const Proj pr = project(v, R, v.pts[3 * q], v.pts[3 * q + 1], v.pts[3 * q + 2], v.ce_obs[2 * e],
+                            v.ce_obs[2 * e + 1], v.ce_w[e]);
+    double j0[6], j1[6];
+    jac_cam<double>(v, pr, j0, j1);
+    double q0 = 0.0, q1 = 0.0;
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+      q0 += j0[i] * xc[i];
+      q1 += j1[i] * xc[i];
+    }
+    const int pv = v.pt_var[q];
+    if (pv >= 0) {
+      double g0[3], g1[3];
+      jac_pt<double>(v, pr, R, g0, g1);
+      const double* xp = xV + 3 * (size_t)pv;
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        q0 += g0[k] * xp[k];
+        q1 += g1[k] * xp[k];
+      }
+    }
+    const double om = pr.om;
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+      acc[i] += om * (j0[i] * pr.r0 + j1[i] * pr.r1);
+      acc[6 + i] += om * (j0[i] * q0 + j1[i] * q1);
+      acc[12 + i] += om * (j0[i] * j0[i] + j1[i] * j1[i]);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 18; i++) {
+    const double s = wave_sum(acc[i]);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][i] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int i = threadIdx.x;
+    double t[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) t[k] = ((red[0][6 * k + i] + red[1][6 * k + i]) + red[2][6 * k + i]) + red[3][6 * k + i];
+    const double rel = ctl->rel, ab = ctl->abs_;
+    rU[6 * (size_t)u + i] = -t[0] - t[1] - (rel * t[2] + ab) * xc[i];
+    bU64[6 * (size_t)u + i] = t[0];
+  }
+}
+
+// rS_u = rU_u - sum_{k in W blocks of u} Y_k rV_{point(k)}; one workgroup per camera
+template <typename T>
+__global__ __launch_bounds__(256) void k_ir_reduce(BAView v, const T* __restrict__ Y, const double* __restrict__ rU,
+                                                   const double* __restrict__ rV, double* __restrict__ rS,
+                                                   DevCtl* ctl, int gate) {
+  if (!gate_open(ctl, gate)) return;
+  __shared__ double red[4][6];
+  const int u = blockIdx.x;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  const int i1 = v.cw_ptr[u + 1];
+  for (int ii = v.cw_ptr[u] + threadIdx.x; ii < i1; ii += 256) {
+    const int k = v.cw_idx[ii];
+    const double* r = rV + 3 * (size_t)v.w_pt[k];
+    const double r0 = r[0], r1 = r[1], r2 = r[2];
+    const T* y = Y + 18 * (size_t)k;
+#pragma unroll
+    for (int a = 0; a < 6; a++) acc[a] += (double)y[3 * a] * r0 + (double)y[3 * a + 1] * r1 + (double)y[3 * a + 2] * r2;
+  }
+#pragma unroll
+  for (int a = 0; a < 6; a++) {
+    const double s = wave_sum(acc[a]);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][a] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int a = threadIdx.x;
+    rS[6 * (size_t)u + a] = rU[6 * (size_t)u + a] - (((red[0][a] + red[1][a]) + red[2][a]) + red[3][a]);
+  }
+}
+
+__global__ void k_add(double* __restrict__ x, const double* __restrict__ z, long long n, DevCtl* ctl, int gate) {
+  if (!gate_open(ctl, gate)) return;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] += z[i];
 }
 
 template <typename T>
@@ -889,10 +1070,31 @@ void launch_s_tiles(const TileView& tv, const int* tile_order, const int* tile_b
 
 template <typename T>
 void launch_backsub(const BAView& v, const T* W, const T* Vinv, const T* bV, const double* xU, double* xV,
-                    const Scratch& s, DevCtl* ctl, int gate, hipStream_t st) {
+                    const Scratch& s, DevCtl* ctl, int gate, hipStream_t st, const double* rhs_v, int add,
+                    const double* bden) {
   if (v.NV == 0) return;
   hipLaunchKernelGGL((k_backsub<T>), dim3(nblocks(v.NV, 256)), dim3(256), 0, st, v, W, Vinv, bV, xU, xV,
-                     s.part_xv, ctl, gate);
+                     s.part_xv, ctl, gate, rhs_v, add, bden);
+}
+
+void launch_ir_residual(const BAView& v, const double* xU, const double* xV, double* rU, double* rV, double* bU64,
+                        double* bV64, DevCtl* ctl, int gate, hipStream_t st) {
+  if (v.P > 0)
+    hipLaunchKernelGGL(k_ir_res_points, dim3(nblocks(v.P, 256)), dim3(256), 0, st, v, xU, xV, rV, bV64, ctl, gate);
+  if (v.NU > 0)
+    hipLaunchKernelGGL(k_ir_res_cams, dim3(v.NU), dim3(256), 0, st, v, xU, xV, rU, bU64, ctl, gate);
+}
+
+template <typename T>
+void launch_ir_reduce(const BAView& v, const T* Y, const double* rU, const double* rV, double* rS, DevCtl* ctl,
+                      int gate, hipStream_t st) {
+  if (v.NU == 0) return;
+  hipLaunchKernelGGL((k_ir_reduce<T>), dim3(v.NU), dim3(256), 0, st, v, Y, rU, rV, rS, ctl, gate);
+}
+
+void launch_add(double* x, const double* z, long long n, DevCtl* ctl, int gate, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_add, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, x, z, n, ctl, gate);
 }
 
 template <typename T>
@@ -958,7 +1160,10 @@ void launch_zero(T* p, long long n, hipStream_t st) {
                                   const int*, const int*, const int*, const T*, const T*, const T*,          \
                                   const T*, T*, DevCtl*, int, hipStream_t);                                  \
   template void launch_backsub<T>(const BAView&, const T*, const T*, const T*, const double*, double*,       \
-                                  const Scratch&, DevCtl*, int, hipStream_t);                                \
+                                  const Scratch&, DevCtl*, int, hipStream_t, const double*, int,             \
+                                  const double*);                                                            \
+  template void launch_ir_reduce<T>(const BAView&, const T*, const double*, const double*, double*, DevCtl*,  \
+                                    int, hipStream_t);                                                       \
   template void launch_xu_stats<T>(int, const double*, const T*, const Scratch&, DevCtl*, int, hipStream_t); \
   template void launch_zero<T>(T*, long long, hipStream_t);
 

@@ -40,6 +40,7 @@ struct BAView {
   // W blocks: point-major (blocks of variable point v are wptr[v]..wptr[v+1])
   const int* wptr;          // [NV+1]
   const int* w_cam;         // [nW] variable camera of the block
+  const int* w_pt;          // [nW] variable point of the block
   const int* cw_ptr;        // [NU+1] camera-major list of W blocks
   const int* cw_idx;        // [nW]
   double* cams;             // [12C] R (row-major) and t per camera
@@ -88,9 +89,23 @@ void launch_s_tiles(const TileView& tv, const int* tile_order, const int* tile_b
                     const int* sb_i, const int* sb_j, const int* sb_ptr, const int* sb_pairs,
                     const T* Sd, const T* bS, const T* Y, const T* W, T* tiles, DevCtl* ctl,
                     int gate, hipStream_t st);
+// xV = V^-1 (rhsV - W^T xU) with rhsV = -bV when rhs_v is null; with `add`
+// the result is added to xV; the increment statistics use bden (FP64
+// gradient) when given, else bV
 template <typename T>
 void launch_backsub(const BAView& v, const T* W, const T* Vinv, const T* bV, const double* xU,
-                    double* xV, const Scratch& s, DevCtl* ctl, int gate, hipStream_t st);
+                    double* xV, const Scratch& s, DevCtl* ctl, int gate, hipStream_t st,
+                    const double* rhs_v = nullptr, int add = 0, const double* bden = nullptr);
+// FP64 residual of the damped normal equations at the increment (xU, xV):
+// rU/rV = -b - (H + D) x from the edges, and the FP64 gradient bU64/bV64
+void launch_ir_residual(const BAView& v, const double* xU, const double* xV, double* rU, double* rV,
+                        double* bU64, double* bV64, DevCtl* ctl, int gate, hipStream_t st);
+// rS = rU - W V^-1 rV (Schur right-hand side of a correction)
+template <typename T>
+void launch_ir_reduce(const BAView& v, const T* Y, const double* rU, const double* rV, double* rS, DevCtl* ctl,
+                      int gate, hipStream_t st);
+// x += z
+void launch_add(double* x, const double* z, long long n, DevCtl* ctl, int gate, hipStream_t st);
 template <typename T>
 void launch_xu_stats(int NU, const double* xU, const T* bU, const Scratch& s, DevCtl* ctl, int gate,
                      hipStream_t st);

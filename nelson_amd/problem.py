@@ -161,6 +161,11 @@ class BAProblem:
     def handle(self):
         return self._h
 
+    def set_refinement(self, steps):
+        """Iterative-refinement steps of every mixed-precision increment
+        (0..8, default 2; ignored in FP64)."""
+        check(self._lib.nsn_set_refinement(self._h, int(steps)))
+
     # parameters ------------------------------------------------------------
     def set_state(self, cams, pts):
         cams = np.ascontiguousarray(cams, np.float64)
@@ -183,6 +188,11 @@ class BAProblem:
         pts = np.zeros(3 * self.n_points)
         check(self._lib.nsn_get_state_rt(self._h, _dp(cams), _dp(pts)))
         return cams, pts
+
+    def oplus(self):
+        """The BA sample's oplus (nelson/samples/BASchurPerfomances.cpp:163-170)
+        with the last Schur increment, which stays on the device."""
+        check(self._lib.nsn_oplus(self._h))
 
     def backupSolution(self):
         check(self._lib.nsn_backup(self._h))
@@ -255,9 +265,9 @@ class SolverCholeskySchur:
         return self._sq
 
     def phase_times_ms(self):
-        ms = np.zeros(5)
-        check(capi.lib().nsn_phase_times(self._p.handle, _dp(ms), 5))
-        return dict(zip(["vinv_wvinv", "s_assembly", "factorize", "solves", "backsub"], ms))
+        ms = np.zeros(6)
+        check(capi.lib().nsn_phase_times(self._p.handle, _dp(ms), 6))
+        return dict(zip(["vinv_wvinv", "s_assembly", "factorize", "solves", "backsub", "refine"], ms))
 
 
 class LevenbergMarquardtSettings(dict):
