@@ -67,6 +67,37 @@ def chol_timing(nsn, p, reps=3):
     return {k: v / reps for k, v in acc.items()}
 
 
+def adjoint_timing(nsn):
+    """Differentiable BA (200 cameras / 20k points / 120k observations): LM to
+    convergence, then the implicit-function-theorem adjoint of
+    L = |t - t_gt|^2; its wall time against one forward LM iteration."""
+    import numpy as np
+    from nelson_amd.autograd import translation_loss_grad
+    g = nsn.generate_ba(dict(nsn.CONFIGS["cfg5"]))
+    p = nsn.BAProblem.from_generated(g)
+    lm = nsn.LevenbergMarquardt(nsn.LevenbergMarquardtSettings(maxNumIt=50))
+    t0 = time.perf_counter()
+    reason = lm.solve(p)
+    t1 = time.perf_counter()
+    its = lm.numIterations()
+    t = p.camera_translations()
+    gc = translation_loss_grad(t, 2.0 * (t - g["cams_gt"].reshape(-1, 6)[:, 3:6]))
+    out = {"workload": f"BA {g['n_cameras']} cameras / {g['n_points']} points / {g['n_edges']} observations, "
+                       "L = |t - t_gt|^2", "lm_reason": reason, "lm_iterations": its,
+           "lm_ms_per_iteration": 1e3 * (t1 - t0) / max(its, 1)}
+    for steps in (0, 2):
+        p.adjoint(g_cams=gc, hessian_steps=steps)  # warm
+        reps = 5
+        a0 = time.perf_counter()
+        for _ in range(reps):
+            p.adjoint(g_cams=gc, hessian_steps=steps)
+        a1 = time.perf_counter()
+        ms = 1e3 * (a1 - a0) / reps
+        out[f"adjoint_ms_hessian_steps_{steps}"] = ms
+        out[f"adjoint_over_lm_iteration_hessian_steps_{steps}"] = ms / out["lm_ms_per_iteration"]
+    return out
+
+
 def cpu_baseline(g, budget_s):
     """The oracle (C restatement of the reference path, 1 thread) timed on a
     bounded sample: one LM iteration's work = update(true) + one Schur step +
@@ -95,6 +126,7 @@ def main():
     ap.add_argument("--config", default="cfg2")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=60.0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -136,6 +168,10 @@ def main():
                        "fp64_final_chi2": s64["final_chi2"], "mixed_final_chi2": sm["final_chi2"],
                        "final_cost_rel_err": abs(sm["final_chi2"] - s64["final_chi2"]) / s64["final_chi2"]}
         log(f"bench: mixed small {mixed_small}")
+    adjoint = None
+    if not args.no_adjoint:
+        adjoint = adjoint_timing(nsn)
+        log(f"bench: adjoint {adjoint}")
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(g, args.cpu_budget)
@@ -163,6 +199,7 @@ def main():
         "phases_ms": ph,
         "mixed": mixed,
         "mixed_small": mixed_small,
+        "adjoint": adjoint,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -199,12 +199,29 @@ int nsn_phase_times(nsn_problem* h, double* ms, int n);
  * gain-ratio denominator then uses the FP64 gradient.  Ignored for NSN_FP64. */
 int nsn_set_refinement(nsn_problem* h, int steps);
 
+/* replaces the observations [2*n_edges] and / or the per-edge weights
+ * [n_edges] (either may be NULL) of an existing problem, in the user's edge
+ * order; the next nsn_linearize / solve uses them.  The reference keeps the
+ * measurements inside its user edges; this is the equivalent of editing them
+ * between solves. */
+int nsn_set_observations(nsn_problem* h, const double* obs, const double* weight);
+
 /* ---- implicit-function-theorem adjoint (no reference counterpart) --------
- * At a converged state x*, with H the undamped Gauss-Newton Hessian
- * (reusing the Schur factorisation), for a loss L(x*) with gradient g given
- * per variable camera (6, left-perturbation coordinates) and per variable
- * point (3), computes dL/d(obs) [2*n_edges] and dL/d(weight) [n_edges]. */
-int nsn_adjoint(nsn_problem* h, const double* g_cams, const double* g_pts,
+ * At a converged state x* (gradient b(x*) = 0) and for a loss L(x*) with
+ * gradient g given per camera [6*n_cameras] (left-perturbation coordinates
+ * (omega, upsilon) of the BA sample's oplus, BASchurPerfomances.cpp:163-170)
+ * and per point [3*n_points], in user order (entries of fixed nodes are
+ * ignored; either array may be NULL for zero), computes
+ *   lambda = (db/dx)^-1 g,  dL/d(obs) = -lambda^T db/d(obs),
+ *   dL/d(weight) = -lambda^T db/d(weight)
+ * into dL_dobs [2*n_edges] and dL_dw [n_edges] (either may be NULL), in the
+ * user's edge order.  db/dx is the Gauss-Newton Hessian, factorised once
+ * (undamped) with the Schur solver; hessian_steps > 0 refines lambda against
+ * the exact Hessian, whose products are central differences of the FP64
+ * gradient along lambda, each step reusing the same factorisation.
+ * Cost: one linearisation, one Schur factorisation and 1 + hessian_steps
+ * Schur solves.  NSN_NOT_PD if the Hessian is not positive definite. */
+int nsn_adjoint(nsn_problem* h, const double* g_cams, const double* g_pts, int hessian_steps,
                 double* dL_dobs, double* dL_dw);
 
 /* ---- batched independent windows (no reference counterpart) ------------- */

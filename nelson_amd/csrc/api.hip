@@ -76,6 +76,14 @@ struct nsn_problem {
   int *flags = nullptr, *counters = nullptr;
   int nflags = 0, NR = 0;
   double* parts = nullptr;  // triangular-solve partial sums, NR x NR x TB
+  // edge permutations and observation arrays (nsn_set_observations)
+  std::vector<int> pe_order, ce_order;
+  double *d_pe_obs = nullptr, *d_pe_w = nullptr, *d_ce_obs = nullptr, *d_ce_w = nullptr;
+  // adjoint work buffers (allocated on the first nsn_adjoint)
+  DevBuf adj_mem;
+  double *gU = nullptr, *gV = nullptr, *lamU = nullptr, *lamV = nullptr, *tU = nullptr, *tV = nullptr,
+         *bpU = nullptr, *bpV = nullptr, *bmU = nullptr, *bmV = nullptr, *dU = nullptr, *dV = nullptr,
+         *zeroU = nullptr, *zeroV = nullptr, *dobs = nullptr, *dw = nullptr;
   // iterative refinement (mixed precision)
   int ir_steps = 0;
   double *rU = nullptr, *rV = nullptr, *bU64 = nullptr, *bV64 = nullptr, *rS = nullptr, *zS = nullptr,
@@ -118,6 +126,27 @@ void seq_cost(nsn_problem* h, int gate) {
   launch_finalize_lin(h->scr, h->ctl, gate, 0, h->st);
 }
 
+// xU = (S^-1 (rhsU - W V^-1 rhsV)), xV = V^-1 (rhsV - W^T xU) with the
+// current factorisation (xV += ... with add); the increment statistics of the
+// back-substitution use bden when given
+template <typename T>
+void seq_schur_solve(nsn_problem* h, int gate, const double* rhsU, const double* rhsV, double* xU, double* xV,
+                     int add, const double* bden) {
+  hipStream_t st = h->st;
+  const int NR = h->NR;
+  int* xready = h->flags + h->tv.ntiles;
+  int* gready = xready + NR;
+  launch_ir_reduce<T>(h->view, (const T*)h->Y, rhsU, rhsV, h->rS, h->ctl, gate, st);
+  launch_chol_reset<T>(xready, NR + NR * NR, h->counters + 1, h->ctl, gate, st);
+  launch_trsv<T>(h->tv, (const T*)h->tiles, (const T*)h->inv, false, h->rS, h->zS, h->parts, xready, gready,
+                 h->counters + 1, h->ctl, gate, kTrsvMaxWG, st);
+  launch_chol_reset<T>(xready, NR + NR * NR, h->counters + 1, h->ctl, gate, st);
+  launch_trsv<T>(h->tv, (const T*)h->tiles, (const T*)h->inv, true, h->zS, xU, h->parts, xready, gready,
+                 h->counters + 1, h->ctl, gate, kTrsvMaxWG, st);
+  launch_backsub<T>(h->view, (const T*)h->W, (const T*)h->Vinv, (const T*)h->bV, xU, xV, h->scr, h->ctl, gate, st,
+                    rhsV, add, bden);
+}
+
 // computeIncrement: V^-1, Y, S, factorisation, solves, back-substitution
 template <typename T>
 void seq_schur(nsn_problem* h, int gate, int checks, bool timed) {
@@ -145,22 +174,11 @@ void seq_schur(nsn_problem* h, int gate, int checks, bool timed) {
                     gate, st);
   if (timed) (void)hipEventRecord(h->ev[5], st);
   const int ir = (h->prec == NSN_MIXED) ? h->ir_steps : 0;
-  const int n = h->tv.n, NR = h->NR;
-  int* xready = h->flags + h->tv.ntiles;
-  int* gready = xready + NR;
   for (int it = 0; it < ir; it++) {
     // x += M^-1 (-b - (H + D) x), M the FP32 Schur solve
     launch_ir_residual(h->view, h->xU, h->xV, h->rU, h->rV, h->bU64, h->bV64, h->ctl, gate, st);
-    launch_ir_reduce<T>(h->view, (const T*)h->Y, h->rU, h->rV, h->rS, h->ctl, gate, st);
-    launch_chol_reset<T>(xready, NR + NR * NR, h->counters + 1, h->ctl, gate, st);
-    launch_trsv<T>(h->tv, (const T*)h->tiles, (const T*)h->inv, false, h->rS, h->zS, h->parts, xready, gready,
-                   h->counters + 1, h->ctl, gate, kTrsvMaxWG, st);
-    launch_chol_reset<T>(xready, NR + NR * NR, h->counters + 1, h->ctl, gate, st);
-    launch_trsv<T>(h->tv, (const T*)h->tiles, (const T*)h->inv, true, h->zS, h->zU, h->parts, xready, gready,
-                   h->counters + 1, h->ctl, gate, kTrsvMaxWG, st);
-    launch_add(h->xU, h->zU, n, h->ctl, gate, st);
-    launch_backsub<T>(h->view, (const T*)h->W, (const T*)h->Vinv, (const T*)h->bV, h->zU, h->xV, h->scr, h->ctl,
-                      gate, st, h->rV, 1, h->bV64);
+    seq_schur_solve<T>(h, gate, h->rU, h->rV, h->zU, h->xV, 1, h->bV64);
+    launch_add(h->xU, h->zU, h->tv.n, h->ctl, gate, st);
   }
   if (ir > 0) launch_xu_stats<double>(h->NU, h->xU, h->bU64, h->scr, h->ctl, gate, st);
   else launch_xu_stats<T>(h->NU, h->xU, (const T*)h->bU, h->scr, h->ctl, gate, st);
@@ -368,6 +386,7 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
   double* d_pe_obs = a_dbl(2 * (size_t)E);
   double* d_pe_w = a_dbl(E);
   int* d_pe_wblk = a_int(E);
+  int* d_pe_edge = a_int(E);
   int* d_cam_ptr = a_int(NU + 1);
   int* d_ce_pt = a_int(h->Ec);
   double* d_ce_obs = a_dbl(2 * (size_t)h->Ec);
@@ -395,15 +414,13 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
   h->inv = a_T((size_t)tv.NT * kTB * kTB);
   h->xU = a_dbl(6 * (size_t)NU);
   h->parts = a_dbl((size_t)h->NR * h->NR * kTB);
-  if (h->prec == NSN_MIXED) {
-    h->rU = a_dbl(6 * (size_t)NU);
-    h->rV = a_dbl(3 * (size_t)NV);
-    h->bU64 = a_dbl(6 * (size_t)NU);
-    h->bV64 = a_dbl(3 * (size_t)NV);
-    h->rS = a_dbl(6 * (size_t)NU);
-    h->zS = a_dbl(6 * (size_t)NU);
-    h->zU = a_dbl(6 * (size_t)NU);
-  }
+  h->rU = a_dbl(6 * (size_t)NU);
+  h->rV = a_dbl(3 * (size_t)NV);
+  h->bU64 = a_dbl(6 * (size_t)NU);
+  h->bV64 = a_dbl(3 * (size_t)NV);
+  h->rS = a_dbl(6 * (size_t)NU);
+  h->zS = a_dbl(6 * (size_t)NU);
+  h->zU = a_dbl(6 * (size_t)NU);
   h->xV = a_dbl(3 * (size_t)NV);
   h->tile_order = a_int(tile_order.size());
   h->tile_blk_ptr = a_int(tile_cnt.size());
@@ -427,14 +444,14 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
   h->log_cap = 4096;
   h->log = m.alloc<DevLog>(h->log_cap);
   if (!h->log || !h->ctl || !h->tiles || !h->inv || !h->W || !h->Y || !h->sb_pairs ||
-      (h->prec == NSN_MIXED && !h->zU))
+      !h->zU)
     return fail(NSN_ENOMEM, "device allocation failed (%.1f MB of tiles)",
                 (double)tv.ntiles * kTB * kTB * es / 1e6);
   int rc;
   if ((rc = upload(d_cam_var, h->cam_var)) || (rc = upload(d_pt_var, h->pt_var)) ||
       (rc = upload(d_vcu, h->var_cam_user)) || (rc = upload(d_vpu, h->var_pt_user)) ||
       (rc = upload(d_pt_ptr, pt_ptr)) || (rc = upload(d_pe_cam, pe_cam)) || (rc = upload(d_pe_obs, pe_obs)) ||
-      (rc = upload(d_pe_w, pe_w)) || (rc = upload(d_pe_wblk, pe_wblk)) || (rc = upload(d_cam_ptr, cam_ptr)) ||
+      (rc = upload(d_pe_w, pe_w)) || (rc = upload(d_pe_wblk, pe_wblk)) || (rc = upload(d_pe_edge, pe)) || (rc = upload(d_cam_ptr, cam_ptr)) ||
       (rc = upload(d_ce_pt, ce_pt)) || (rc = upload(d_ce_obs, ce_obs)) || (rc = upload(d_ce_w, ce_w)) ||
       (rc = upload(d_wptr, wptr)) || (rc = upload(d_w_cam, h->w_cam)) || (rc = upload(d_w_pt, h->w_pt)) || (rc = upload(d_cw_ptr, cw_ptr)) ||
       (rc = upload(d_cw_idx, cw_idx)) || (rc = upload(h->tile_order, tile_order)) ||
@@ -446,10 +463,13 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
   v.C = C; v.P = P; v.E = E; v.NU = NU; v.NV = NV; v.nW = nW;
   v.f = h->f; v.cx = h->cx; v.cy = h->cy; v.huber = h->huber;
   v.cam_var = d_cam_var; v.pt_var = d_pt_var; v.var_cam_user = d_vcu; v.var_pt_user = d_vpu;
-  v.pt_ptr = d_pt_ptr; v.pe_cam = d_pe_cam; v.pe_obs = d_pe_obs; v.pe_w = d_pe_w; v.pe_wblk = d_pe_wblk;
+  v.pt_ptr = d_pt_ptr; v.pe_cam = d_pe_cam; v.pe_obs = d_pe_obs; v.pe_w = d_pe_w; v.pe_wblk = d_pe_wblk; v.pe_edge = d_pe_edge;
   v.cam_ptr = d_cam_ptr; v.ce_pt = d_ce_pt; v.ce_obs = d_ce_obs; v.ce_w = d_ce_w;
   v.wptr = d_wptr; v.w_cam = d_w_cam; v.w_pt = d_w_pt; v.cw_ptr = d_cw_ptr; v.cw_idx = d_cw_idx;
   v.cams = h->cams; v.pts = h->pts;
+  h->pe_order = std::move(pe);
+  h->ce_order = std::move(ce);
+  h->d_pe_obs = d_pe_obs; h->d_pe_w = d_pe_w; h->d_ce_obs = d_ce_obs; h->d_ce_w = d_ce_w;
   return NSN_OK;
 }
 
@@ -837,6 +857,142 @@ extern "C" int nsn_rho(nsn_problem* h, double mu, double old_chi2, double* rho) 
   }
   *rho = (old_chi2 - h->hctl->chi2) / den;
   return NSN_OK;
+}
+
+extern "C" int nsn_set_observations(nsn_problem* h, const double* obs, const double* weight) {
+  if (!h) return fail(NSN_EINVAL, "null argument");
+  HIP_TRY(hipSetDevice(h->device));
+  const int E = h->E, Ec = h->Ec;
+  if (obs) {
+    std::vector<double> po(2 * (size_t)E), co(2 * (size_t)Ec);
+    for (int s = 0; s < E; s++) {
+      po[2 * (size_t)s] = obs[2 * (size_t)h->pe_order[s]];
+      po[2 * (size_t)s + 1] = obs[2 * (size_t)h->pe_order[s] + 1];
+    }
+    for (int s = 0; s < Ec; s++) {
+      co[2 * (size_t)s] = obs[2 * (size_t)h->ce_order[s]];
+      co[2 * (size_t)s + 1] = obs[2 * (size_t)h->ce_order[s] + 1];
+    }
+    int rc;
+    if ((rc = upload(h->d_pe_obs, po)) || (rc = upload(h->d_ce_obs, co))) return rc;
+  }
+  if (weight) {
+    std::vector<double> pw(E), cw(Ec);
+    for (int s = 0; s < E; s++) pw[s] = weight[h->pe_order[s]];
+    for (int s = 0; s < Ec; s++) cw[s] = weight[h->ce_order[s]];
+    int rc;
+    if ((rc = upload(h->d_pe_w, pw)) || (rc = upload(h->d_ce_w, cw))) return rc;
+  }
+  h->linearized = false;
+  h->stepped = false;
+  return NSN_OK;
+}
+
+namespace {
+
+int alloc_adjoint(nsn_problem* h) {
+  if (h->gU) return NSN_OK;
+  DevBuf& m = h->adj_mem;
+  const size_t nu = 6 * (size_t)h->NU, nv = 3 * (size_t)h->NV;
+  double** us[] = {&h->gU, &h->lamU, &h->tU, &h->bpU, &h->bmU, &h->dU, &h->zeroU};
+  double** vs[] = {&h->gV, &h->lamV, &h->tV, &h->bpV, &h->bmV, &h->dV, &h->zeroV};
+  for (double** p : us) *p = m.alloc<double>(nu);
+  for (double** p : vs) *p = m.alloc<double>(nv);
+  h->dobs = m.alloc<double>(2 * (size_t)h->E);
+  h->dw = m.alloc<double>(h->E);
+  for (double** p : us) if (!*p) return fail(NSN_ENOMEM, "adjoint buffers");
+  for (double** p : vs) if (!*p) return fail(NSN_ENOMEM, "adjoint buffers");
+  if (!h->dobs || !h->dw) return fail(NSN_ENOMEM, "adjoint buffers");
+  HIP_TRY(hipMemset(h->zeroU, 0, sizeof(double) * std::max<size_t>(nu, 1)));
+  HIP_TRY(hipMemset(h->zeroV, 0, sizeof(double) * std::max<size_t>(nv, 1)));
+  return NSN_OK;
+}
+
+// gradient b of the cost at the current state perturbed by oplus(t),
+// FP64, into (bu, bv); the state is restored afterwards
+void grad_at_offset(nsn_problem* h, const double* tu, const double* tv, double* bu, double* bv) {
+  hipStream_t st = h->st;
+  launch_copy_state(h->view, h->cams_bak, h->pts_bak, h->cams, h->pts, h->ctl, GATE_ALWAYS, st);
+  launch_oplus(h->view, tu, tv, h->ctl, GATE_ALWAYS, st);
+  launch_ir_residual(h->view, h->zeroU, h->zeroV, h->rU, h->rV, bu, bv, h->ctl, GATE_ALWAYS, st);
+  launch_copy_state(h->view, h->cams, h->pts, h->cams_bak, h->pts_bak, h->ctl, GATE_ALWAYS, st);
+}
+
+template <typename T>
+int adjoint_t(nsn_problem* h, const double* g_cams, const double* g_pts, int steps, double* dL_dobs,
+              double* dL_dw) {
+  hipStream_t st = h->st;
+  const int NU = h->NU, NV = h->NV;
+  const long long nu = 6LL * NU, nv = 3LL * NV;
+  int rc;
+  if ((rc = alloc_adjoint(h))) return rc;
+  if ((rc = reset_ctl_primitive(h))) return rc;
+  // the Gauss-Newton Hessian at x* and its Schur factorisation (undamped)
+  seq_lin_full<T>(h, GATE_ALWAYS);
+  launch_set_damping(h->ctl, 0.0, 0.0, st);
+  seq_schur<T>(h, GATE_ALWAYS, 0, false);
+  if ((rc = sync_ctl(h))) return rc;
+  if ((rc = check_device_errors(h))) return rc;
+  if (!h->hctl->solve_ok)
+    return fail(NSN_NOT_PD, "nsn_adjoint: the Gauss-Newton Hessian at this state is not positive definite");
+  // loss gradient in variable order
+  std::vector<double> gu(nu), gv(nv);
+  for (int u = 0; u < NU; u++)
+    for (int i = 0; i < 6; i++) gu[6 * (size_t)u + i] = g_cams ? g_cams[6 * (size_t)h->var_cam_user[u] + i] : 0.0;
+  for (int q = 0; q < NV; q++)
+    for (int i = 0; i < 3; i++) gv[3 * (size_t)q + i] = g_pts ? g_pts[3 * (size_t)h->var_pt_user[q] + i] : 0.0;
+  if ((rc = upload(h->gU, gu)) || (rc = upload(h->gV, gv))) return rc;
+  // lambda = H^-1 g with the Schur factorisation
+  seq_schur_solve<T>(h, GATE_ALWAYS, h->gU, h->gV, h->lamU, h->lamV, 0, nullptr);
+  // refinement against the exact Hessian: lambda += H^-1 (g - B lambda), with
+  // B lambda = (b(x + eps lambda) - b(x - eps lambda)) / (2 eps), the central
+  // difference of the FP64 gradient along lambda
+  std::vector<double> lu(nu), lv(nv);
+  for (int k = 0; k < steps; k++) {
+    if (nu) HIP_TRY(hipMemcpyAsync(lu.data(), h->lamU, sizeof(double) * nu, hipMemcpyDeviceToHost, st));
+    if (nv) HIP_TRY(hipMemcpyAsync(lv.data(), h->lamV, sizeof(double) * nv, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    double mx = 0.0;
+    for (double x : lu) mx = std::max(mx, std::fabs(x));
+    for (double x : lv) mx = std::max(mx, std::fabs(x));
+    if (!(mx > 0.0) || !std::isfinite(mx)) break;
+    const double eps = 1e-6 / mx;
+    launch_axpby(nu, eps, h->lamU, 0.0, h->lamU, h->tU, st);
+    launch_axpby(nv, eps, h->lamV, 0.0, h->lamV, h->tV, st);
+    grad_at_offset(h, h->tU, h->tV, h->bpU, h->bpV);
+    launch_axpby(nu, -1.0, h->tU, 0.0, h->tU, h->tU, st);
+    launch_axpby(nv, -1.0, h->tV, 0.0, h->tV, h->tV, st);
+    grad_at_offset(h, h->tU, h->tV, h->bmU, h->bmV);
+    // r = g - (bp - bm) / (2 eps), into (bp)
+    launch_axpby(nu, 1.0, h->bpU, -1.0, h->bmU, h->bpU, st);
+    launch_axpby(nv, 1.0, h->bpV, -1.0, h->bmV, h->bpV, st);
+    launch_axpby(nu, 1.0, h->gU, -0.5 / eps, h->bpU, h->bpU, st);
+    launch_axpby(nv, 1.0, h->gV, -0.5 / eps, h->bpV, h->bpV, st);
+    seq_schur_solve<T>(h, GATE_ALWAYS, h->bpU, h->bpV, h->dU, h->dV, 0, nullptr);
+    launch_axpby(nu, 1.0, h->lamU, 1.0, h->dU, h->lamU, st);
+    launch_axpby(nv, 1.0, h->lamV, 1.0, h->dV, h->lamV, st);
+  }
+  launch_adjoint_edges(h->view, h->lamU, h->lamV, h->dobs, h->dw, st);
+  HIP_TRY(hipGetLastError());
+  if ((rc = sync_ctl(h))) return rc;
+  if ((rc = check_device_errors(h))) return rc;
+  if (dL_dobs) HIP_TRY(hipMemcpy(dL_dobs, h->dobs, sizeof(double) * 2 * (size_t)h->E, hipMemcpyDeviceToHost));
+  if (dL_dw) HIP_TRY(hipMemcpy(dL_dw, h->dw, sizeof(double) * (size_t)h->E, hipMemcpyDeviceToHost));
+  // the factorisation and increment buffers now hold the adjoint's values
+  h->stepped = false;
+  return NSN_OK;
+}
+
+}  // namespace
+
+extern "C" int nsn_adjoint(nsn_problem* h, const double* g_cams, const double* g_pts, int hessian_steps,
+                           double* dL_dobs, double* dL_dw) {
+  if (!h) return fail(NSN_EINVAL, "null argument");
+  if (hessian_steps < 0 || hessian_steps > 16)
+    return fail(NSN_EINVAL, "nsn_adjoint: hessian_steps must be in 0..16, got %d", hessian_steps);
+  HIP_TRY(hipSetDevice(h->device));
+  return h->prec == NSN_MIXED ? adjoint_t<float>(h, g_cams, g_pts, hessian_steps, dL_dobs, dL_dw)
+                              : adjoint_t<double>(h, g_cams, g_pts, hessian_steps, dL_dobs, dL_dw);
 }
 
 template <typename T>

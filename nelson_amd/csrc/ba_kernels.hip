@@ -725,6 +725,72 @@ __global__ __launch_bounds__(256) void k_ir_reduce(BAView v, const T* __restrict
   }
 }
 
+// ---------------------------------------------------------------------------
+// Implicit-function-theorem adjoint of the converged solve, per edge.  At x*
+// the first-order condition b(x*, theta) = sum_e w_e psi_e J_e^T r_e = 0
+// holds; with lambda = (db/dx)^-1 g for the loss gradient g,
+//   dL/dz_e = -lambda^T db/dz_e = w_e (psi a + 2 psi'(s) (a . r) r),
+//   dL/dw_e = -lambda^T db/dw_e = -psi (a . r),
+// where a = J_c lambda_c + J_p lambda_p, s = |r|^2, psi = rho'(s) (1, or
+// delta / sqrt(s) on the Huber branch) and psi' its derivative.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_adjoint_edges(BAView v, const double* __restrict__ lamU,
+                                                       const double* __restrict__ lamV, double* __restrict__ dobs,
+                                                       double* __restrict__ dw) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= v.P) return;
+  const double X0 = v.pts[3 * p], X1 = v.pts[3 * p + 1], X2 = v.pts[3 * p + 2];
+  const int pv = v.pt_var[p];
+  const int e1 = v.pt_ptr[p + 1];
+  for (int e = v.pt_ptr[p]; e < e1; e++) {
+    const int c = v.pe_cam[e];
+    double R[12];
+#pragma unroll
+    for (int i = 0; i < 12; i++) R[i] = v.cams[12 * c + i];
+    const double we = v.pe_w[e];
+    const Proj pr = project(v, R, X0, X1, X2, v.pe_obs[2 * e], v.pe_obs[2 * e + 1], we);
+    double a0 = 0.0, a1 = 0.0;
+    if (pv >= 0) {
+      double g0[3], g1[3];
+      jac_pt<double>(v, pr, R, g0, g1);
+      const double* l = lamV + 3 * (size_t)pv;
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        a0 += g0[k] * l[k];
+        a1 += g1[k] * l[k];
+      }
+    }
+    const int cv = v.cam_var[c];
+    if (cv >= 0) {
+      double j0[6], j1[6];
+      jac_cam<double>(v, pr, j0, j1);
+      const double* l = lamU + 6 * (size_t)cv;
+#pragma unroll
+      for (int i = 0; i < 6; i++) {
+        a0 += j0[i] * l[i];
+        a1 += j1[i] * l[i];
+      }
+    }
+    const double s = pr.r0 * pr.r0 + pr.r1 * pr.r1;
+    double psi = 1.0, dpsi = 0.0;
+    if (v.huber > 0.0 && s > v.huber * v.huber) {
+      const double rs = sqrt(s);
+      psi = v.huber / rs;
+      dpsi = -0.5 * v.huber / (s * rs);
+    }
+    const double ar = a0 * pr.r0 + a1 * pr.r1;
+    const int ue = v.pe_edge[e];
+    dobs[2 * (size_t)ue] = we * (psi * a0 + 2.0 * dpsi * ar * pr.r0);
+    dobs[2 * (size_t)ue + 1] = we * (psi * a1 + 2.0 * dpsi * ar * pr.r1);
+    dw[ue] = -psi * ar;
+  }
+}
+
+__global__ void k_axpby(long long n, double a, const double* x, double b, const double* y, double* out) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = a * x[i] + b * y[i];
+}
+
 __global__ void k_add(double* __restrict__ x, const double* __restrict__ z, long long n, DevCtl* ctl, int gate) {
   if (!gate_open(ctl, gate)) return;
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1090,6 +1156,17 @@ void launch_ir_reduce(const BAView& v, const T* Y, const double* rU, const doubl
                       int gate, hipStream_t st) {
   if (v.NU == 0) return;
   hipLaunchKernelGGL((k_ir_reduce<T>), dim3(v.NU), dim3(256), 0, st, v, Y, rU, rV, rS, ctl, gate);
+}
+
+void launch_axpby(long long n, double a, const double* x, double b, const double* y, double* out, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_axpby, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, a, x, b, y, out);
+}
+
+void launch_adjoint_edges(const BAView& v, const double* lamU, const double* lamV, double* dobs, double* dw,
+                          hipStream_t st) {
+  if (v.P == 0) return;
+  hipLaunchKernelGGL(k_adjoint_edges, dim3(nblocks(v.P, 256)), dim3(256), 0, st, v, lamU, lamV, dobs, dw);
 }
 
 void launch_add(double* x, const double* z, long long n, DevCtl* ctl, int gate, hipStream_t st) {

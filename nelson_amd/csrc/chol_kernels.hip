@@ -596,6 +596,7 @@ void launch_trsv(const TileView& tv, const T* tiles, const T* inv, bool backward
                  hipStream_t st) {
   const int NR = (tv.n + 63) / 64;
   const int ntask = NR + (NR >= 2 ? (NR - 1) * (NR - 2) / 2 : 0);
+  if (ntask == 0) return;
   int g = ntask < max_wg ? ntask : max_wg;
   if (backward)
     hipLaunchKernelGGL((k_trsv<T, 64, true>), dim3(g), dim3(256), 0, st, tv, tiles, inv, rhs, sol, parts, xready,

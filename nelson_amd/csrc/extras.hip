@@ -5,9 +5,6 @@
 
 using nsn::fail;
 
-extern "C" int nsn_adjoint(nsn_problem*, const double*, const double*, double*, double*) {
-  return fail(NSN_EINVAL, "nsn_adjoint: not available in this build");
-}
 extern "C" int nsn_batch_create(const nsn_problem_desc*, int, nsn_batch** out) {
   if (out) *out = nullptr;
   return fail(NSN_EINVAL, "nsn_batch_create: not available in this build");

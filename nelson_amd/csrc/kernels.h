@@ -32,6 +32,7 @@ struct BAView {
   const double* pe_obs;     // [2E]
   const double* pe_w;       // [E]
   const int* pe_wblk;       // [E] W block or -1
+  const int* pe_edge;       // [E] user index of the edge
   // edges of variable cameras in camera-major order
   const int* cam_ptr;       // [NU+1]
   const int* ce_pt;         // [Ec]
@@ -106,6 +107,12 @@ void launch_ir_reduce(const BAView& v, const T* Y, const double* rU, const doubl
                       int gate, hipStream_t st);
 // x += z
 void launch_add(double* x, const double* z, long long n, DevCtl* ctl, int gate, hipStream_t st);
+// out = a x + b y (out may alias x or y)
+void launch_axpby(long long n, double a, const double* x, double b, const double* y, double* out, hipStream_t st);
+// per-edge adjoint outputs at the current state from the multipliers
+// (lamU, lamV): dL/d(obs) [2E] and dL/d(weight) [E], user edge order
+void launch_adjoint_edges(const BAView& v, const double* lamU, const double* lamV, double* dobs, double* dw,
+                          hipStream_t st);
 template <typename T>
 void launch_xu_stats(int NU, const double* xU, const T* bU, const Scratch& s, DevCtl* ctl, int gate,
                      hipStream_t st);

@@ -189,6 +189,44 @@ class BAProblem:
         check(self._lib.nsn_get_state_rt(self._h, _dp(cams), _dp(pts)))
         return cams, pts
 
+    def set_observations(self, obs=None, weight=None):
+        """Replaces the observations [E, 2] and / or per-edge weights [E]
+        (user edge order) for the next linearisation."""
+        o = None if obs is None else np.ascontiguousarray(obs, np.float64).reshape(-1)
+        w = None if weight is None else np.ascontiguousarray(weight, np.float64).reshape(-1)
+        if o is not None and o.size != 2 * self.n_edges:
+            raise ValueError("obs must hold 2 values per edge")
+        if w is not None and w.size != self.n_edges:
+            raise ValueError("weight must hold one value per edge")
+        check(self._lib.nsn_set_observations(self._h, _dp(o), _dp(w)))
+        if o is not None:
+            self.obs = o.copy()
+        if w is not None:
+            self.weight = w.copy()
+
+    def adjoint(self, g_cams=None, g_pts=None, hessian_steps=2):
+        """Implicit-function-theorem adjoint at the current (converged) state.
+
+        g_cams [C, 6]: dL/d(camera) in the left-perturbation coordinates
+        (omega, upsilon) of oplus; g_pts [P, 3]: dL/d(point).  Returns
+        (dL/d(obs) [E, 2], dL/d(weight) [E]).  hessian_steps refines the
+        multipliers against the exact Hessian (0: Gauss-Newton Hessian)."""
+        gc = None if g_cams is None else np.ascontiguousarray(g_cams, np.float64).reshape(-1)
+        gp = None if g_pts is None else np.ascontiguousarray(g_pts, np.float64).reshape(-1)
+        if gc is not None and gc.size != 6 * self.n_cameras:
+            raise ValueError("g_cams must hold 6 values per camera")
+        if gp is not None and gp.size != 3 * self.n_points:
+            raise ValueError("g_pts must hold 3 values per point")
+        dobs = np.zeros(2 * self.n_edges)
+        dw = np.zeros(self.n_edges)
+        check(self._lib.nsn_adjoint(self._h, _dp(gc), _dp(gp), int(hessian_steps), _dp(dobs), _dp(dw)))
+        return dobs.reshape(-1, 2), dw
+
+    def camera_translations(self):
+        """[C, 3] translations t of the current state (X_cam = R X + t)."""
+        cams, _ = self.get_state_rt()
+        return cams.reshape(-1, 12)[:, 9:12].copy()
+
     def oplus(self):
         """The BA sample's oplus (nelson/samples/BASchurPerfomances.cpp:163-170)
         with the last Schur increment, which stays on the device."""
