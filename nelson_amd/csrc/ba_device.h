@@ -1,0 +1,452 @@
+// Device helpers of the BA kernels shared by the single-problem kernels
+// (ba_kernels.hip) and the batched-window solver (batch_kernels.hip): the
+// reprojection edge, its Jacobians, the BA oplus, reductions and the LM
+// control steps.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+
+namespace nsn {
+
+__device__ __forceinline__ bool gate_open(const DevCtl* c, int gate) {
+  switch (gate) {
+    case GATE_ALWAYS: return true;
+    case GATE_NOT_DONE: return !c->done;
+    case GATE_NEED_LIN: return !c->done && c->need_lin;
+    case GATE_SOLVE_OK: return !c->done && c->solve_ok;
+    case GATE_ROLLBACK: return c->rollback_pending != 0;
+  }
+  return true;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  return v;
+}
+
+// deterministic block sum for blockDim.x == 256; result valid in thread 0
+__device__ __forceinline__ double block_sum256(double v, double* sh) {
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) r = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+  __syncthreads();
+  return r;
+}
+
+__device__ __forceinline__ void atomic_max_nonneg(unsigned long long* a, double v) {
+  if (v > 0.0) atomicMax(a, (unsigned long long)__double_as_longlong(v));
+}
+
+// packed upper index of a 6x6 symmetric block, i <= j
+__host__ __device__ constexpr int up6(int i, int j) { return i * 6 - i * (i - 1) / 2 + (j - i); }
+
+// ---------------------------------------------------------------------------
+// the reprojection edge
+// ---------------------------------------------------------------------------
+struct Proj {
+  double Xc0, Xc1, Xc2, iz, r0, r1, cost, om;
+};
+
+__device__ __forceinline__ Proj project(const BAView& v, const double* __restrict__ R, double X0,
+                                        double X1, double X2, double o0, double o1, double we) {
+  Proj p;
+  p.Xc0 = R[0] * X0 + R[1] * X1 + R[2] * X2 + R[9];
+  p.Xc1 = R[3] * X0 + R[4] * X1 + R[5] * X2 + R[10];
+  p.Xc2 = R[6] * X0 + R[7] * X1 + R[8] * X2 + R[11];
+  p.iz = 1.0 / p.Xc2;
+  p.r0 = v.f * p.Xc0 * p.iz + v.cx - o0;
+  p.r1 = v.f * p.Xc1 * p.iz + v.cy - o1;
+  double s = p.r0 * p.r0 + p.r1 * p.r1;
+  double psi = 1.0;
+  if (v.huber > 0.0 && s > v.huber * v.huber) {
+    double nn = sqrt(s);
+    p.cost = 2.0 * v.huber * nn - v.huber * v.huber;
+    psi = v.huber / nn;
+  } else {
+    p.cost = s;
+  }
+  p.cost *= we;
+  p.om = we * psi;
+  return p;
+}
+
+// d(r)/d(omega, upsilon) for the left SE(3) perturbation: D [-[Xc]x | I]
+template <typename T>
+__device__ __forceinline__ void jac_cam(const BAView& v, const Proj& p, T* j0, T* j1) {
+  const double a = v.f * p.iz;
+  const double b0 = -a * p.Xc0 * p.iz;
+  const double b1 = -a * p.Xc1 * p.iz;
+  j0[0] = (T)(b0 * p.Xc1);
+  j0[1] = (T)(a * p.Xc2 - b0 * p.Xc0);
+  j0[2] = (T)(-a * p.Xc1);
+  j0[3] = (T)a;
+  j0[4] = (T)0;
+  j0[5] = (T)b0;
+  j1[0] = (T)(-a * p.Xc2 + b1 * p.Xc1);
+  j1[1] = (T)(-b1 * p.Xc0);
+  j1[2] = (T)(a * p.Xc0);
+  j1[3] = (T)0;
+  j1[4] = (T)a;
+  j1[5] = (T)b1;
+}
+
+// d(r)/d(X) = D R
+template <typename T>
+__device__ __forceinline__ void jac_pt(const BAView& v, const Proj& p, const double* __restrict__ R,
+                                       T* g0, T* g1) {
+  const double a = v.f * p.iz;
+  const double b0 = -a * p.Xc0 * p.iz;
+  const double b1 = -a * p.Xc1 * p.iz;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    g0[k] = (T)(a * R[k] + b0 * R[6 + k]);
+    g1[k] = (T)(a * R[3 + k] + b1 * R[6 + k]);
+  }
+}
+
+// oplus of the BA sample (BASchurPerfomances.cpp:163-170): cameras
+// T <- exp(delta) T (R <- Exp(w) R, t <- Exp(w) t + V(w) u), points additive
+__device__ __forceinline__ void so3_coeffs(double th2, double& A, double& B, double& C) {
+  const double th = sqrt(th2);
+  if (th < 1e-3) {
+    A = 1.0 - th2 / 6.0 + th2 * th2 / 120.0;
+    B = 0.5 - th2 / 24.0 + th2 * th2 / 720.0;
+    C = 1.0 / 6.0 - th2 / 120.0 + th2 * th2 / 5040.0;
+  } else {
+    const double s = sin(th), h = sin(0.5 * th);
+    A = s / th;
+    B = 2.0 * h * h / th2;
+    C = (th - s) / (th2 * th);
+  }
+}
+
+// oplus of the BA sample (BASchurPerfomances.cpp:163-170) for variable
+// camera idx: T <- exp(delta) T (R <- Exp(w) R, t <- Exp(w) t + V(w) u)
+__device__ __forceinline__ void oplus_cam(const BAView& v, int idx, const double* __restrict__ xU) {
+  const int c = v.var_cam_user[idx];
+  const double* d = xU + 6 * (size_t)idx;
+  const double w0 = d[0], w1 = d[1], w2 = d[2];
+  double A, B, Cc;
+  so3_coeffs(w0 * w0 + w1 * w1 + w2 * w2, A, B, Cc);
+  const double K[9] = {0, -w2, w1, w2, 0, -w0, -w1, w0, 0};
+  double E[9], Vm[9];
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      const double k2 = K[i * 3 + 0] * K[0 * 3 + j] + K[i * 3 + 1] * K[1 * 3 + j] + K[i * 3 + 2] * K[2 * 3 + j];
+      const double id = (i == j) ? 1.0 : 0.0;
+      E[i * 3 + j] = id + A * K[i * 3 + j] + B * k2;
+      Vm[i * 3 + j] = id + B * K[i * 3 + j] + Cc * k2;
+    }
+  double* cm = v.cams + 12 * (size_t)c;
+  double R[9], t[3];
+#pragma unroll
+  for (int i = 0; i < 9; i++) R[i] = cm[i];
+#pragma unroll
+  for (int i = 0; i < 3; i++) t[i] = cm[9 + i];
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+      cm[i * 3 + j] = E[i * 3 + 0] * R[0 * 3 + j] + E[i * 3 + 1] * R[1 * 3 + j] + E[i * 3 + 2] * R[2 * 3 + j];
+    cm[9 + i] = E[i * 3 + 0] * t[0] + E[i * 3 + 1] * t[1] + E[i * 3 + 2] * t[2] + Vm[i * 3 + 0] * d[3] +
+                Vm[i * 3 + 1] * d[4] + Vm[i * 3 + 2] * d[5];
+  }
+}
+
+// points are additive
+__device__ __forceinline__ void oplus_pt(const BAView& v, int q, const double* __restrict__ xV) {
+  const int p = v.var_pt_user[q];
+#pragma unroll
+  for (int i = 0; i < 3; i++) v.pts[3 * (size_t)p + i] += xV[3 * (size_t)q + i];
+}
+
+__device__ inline void log_add(DevCtl* c, DevLog* log, int it, int subit, double chi2, double mu) {
+  if (log && c->n_log < c->log_cap) {
+    log[c->n_log].it = it;
+    log[c->n_log].subit = subit;
+    log[c->n_log].chi2 = chi2;
+    log[c->n_log].mu = mu;
+  }
+  c->n_log++;
+}
+
+// after a full linearisation in the LM loop: the start of the solve
+// (LevenbergMarquardt.hpp:41-61) or the end of an outer iteration (:132-141)
+__device__ inline void lm_after_lin(DevCtl* c, DevLog* log) {
+  if (c->done || !c->need_lin) return;
+  if (!c->started) {
+    log_add(c, log, -1, -1, c->chi2, 0.0);
+    c->started = 1;
+    if (c->maxb <= c->epsBVector) {
+      c->done = 1;
+      c->reason = 0;  // SuccessAtStart
+      c->need_lin = 0;
+      return;
+    }
+    c->oldChi2 = c->chi2;
+    c->mu = c->tau * c->maxdiag;
+    c->curV = c->initV;
+    c->iter = 0;
+  } else {
+    c->oldChi2 = c->chi2;
+    c->iter++;
+  }
+  c->subiter = 0;
+  c->need_lin = 0;
+  if (c->iter >= c->maxNumIt) {
+    c->done = 1;
+    c->reason = 5;  // MaxIterationReached
+    return;
+  }
+  c->rel = 0.0;
+  c->abs_ = c->mu;
+}
+
+// LevenbergMarquardt.hpp:77-130: log, rho, accept/reject, damping schedule
+__device__ inline void lm_control(DevCtl* c, DevLog* log) {
+  if (c->done) return;
+  double rho = -1.0;
+  if (c->solve_ok) {
+    log_add(c, log, c->iter, c->subiter, c->chi2, c->mu);
+    rho = (c->oldChi2 - c->chi2) / c->den;
+  } else {
+    log_add(c, log, c->iter, c->subiter, -1.0, c->mu);
+  }
+  c->rho = rho;
+  c->accepted = 0;
+  if (c->solve_ok && rho > 0) {
+    if (c->iter > c->minNumIt && c->maxb <= c->epsBVector) {
+      c->done = 1;
+      c->reason = 2;  // SuccessEpsB
+      return;
+    }
+    if (c->iter > c->minNumIt && c->chi2 < c->epsChi2) {
+      c->done = 1;
+      c->reason = 3;  // SuccessSmallResiduals
+      return;
+    }
+    const double fc = 2.0 * rho - 1.0;
+    double alpha = fmin(c->maxScaleFactor, 1.0 - fc * fc * fc);
+    alpha = fmax(c->minScaleFactor, alpha);
+    c->mu = c->mu * alpha;
+    c->curV = c->initV;
+    c->accepted = 1;
+  } else {
+    c->rollback_pending = 1;
+    c->mu = c->mu * c->curV;
+    c->curV = c->curV * c->penaltyV;
+  }
+  c->subiter++;
+  if (c->subiter >= c->maxNumSubIt) {
+    c->done = 1;
+    c->reason = 6;  // MaxSubIterationReached
+  }
+  if (c->accepted && !c->done) c->need_lin = 1;
+  c->rel = 0.0;
+  c->abs_ = c->mu;
+}
+
+
+// one point of the point-major linearisation pass: V_p, bV_p, the W blocks
+// of its edges and the chi2 of every edge of p, with max |bV| and max diag V
+// of the point (nelson/BaseSection.cpp:88-169, updateH22Block / updateH12Block)
+template <typename T, bool HESS>
+__device__ __forceinline__ void lin_point(const BAView& v, int p, T* __restrict__ V, T* __restrict__ bV,
+                                          T* __restrict__ W, double& chi2, double& mb, double& md) {
+  const double X0 = v.pts[3 * p], X1 = v.pts[3 * p + 1], X2 = v.pts[3 * p + 2];
+  const int pv = v.pt_var[p];
+  T vv[6] = {0, 0, 0, 0, 0, 0};
+  T bv[3] = {0, 0, 0};
+  T w[18];
+  int cur = -1;
+  const int e0 = v.pt_ptr[p], e1 = v.pt_ptr[p + 1];
+  for (int e = e0; e < e1; e++) {
+    const int c = v.pe_cam[e];
+    double R[12];
+#pragma unroll
+    for (int i = 0; i < 12; i++) R[i] = v.cams[12 * c + i];
+    const Proj pr = project(v, R, X0, X1, X2, v.pe_obs[2 * e], v.pe_obs[2 * e + 1], v.pe_w[e]);
+    chi2 += pr.cost;
+    if (HESS) {
+      const int cv = v.cam_var[c];
+      if (pv < 0) continue;
+      T g0[3], g1[3];
+      jac_pt<T>(v, pr, R, g0, g1);
+      const T om = (T)pr.om, r0 = (T)pr.r0, r1 = (T)pr.r1;
+      vv[0] += om * (g0[0] * g0[0] + g1[0] * g1[0]);
+      vv[1] += om * (g0[0] * g0[1] + g1[0] * g1[1]);
+      vv[2] += om * (g0[0] * g0[2] + g1[0] * g1[2]);
+      vv[3] += om * (g0[1] * g0[1] + g1[1] * g1[1]);
+      vv[4] += om * (g0[1] * g0[2] + g1[1] * g1[2]);
+      vv[5] += om * (g0[2] * g0[2] + g1[2] * g1[2]);
+#pragma unroll
+      for (int k = 0; k < 3; k++) bv[k] += om * (g0[k] * r0 + g1[k] * r1);
+      if (cv >= 0) {
+        T j0[6], j1[6];
+        jac_cam<T>(v, pr, j0, j1);
+        const int blk = v.pe_wblk[e];
+        if (blk != cur) {
+          if (cur >= 0) {
+#pragma unroll
+            for (int i = 0; i < 18; i++) W[18 * (size_t)cur + i] = w[i];
+          }
+#pragma unroll
+          for (int i = 0; i < 18; i++) w[i] = (T)0;
+          cur = blk;
+        }
+#pragma unroll
+        for (int i = 0; i < 6; i++)
+#pragma unroll
+          for (int j = 0; j < 3; j++) w[i * 3 + j] += om * (j0[i] * g0[j] + j1[i] * g1[j]);
+      }
+    }
+  }
+  if (HESS) {
+    if (cur >= 0) {
+#pragma unroll
+      for (int i = 0; i < 18; i++) W[18 * (size_t)cur + i] = w[i];
+    }
+    if (pv >= 0) {
+#pragma unroll
+      for (int i = 0; i < 6; i++) V[6 * (size_t)pv + i] = vv[i];
+#pragma unroll
+      for (int i = 0; i < 3; i++) bV[3 * (size_t)pv + i] = bv[i];
+      mb = fmax(fabs((double)bv[0]), fmax(fabs((double)bv[1]), fabs((double)bv[2])));
+      md = fmax(fabs((double)vv[0]), fmax(fabs((double)vv[3]), fabs((double)vv[5])));
+    }
+  }
+}
+
+// damped V_q^-1 through its Cholesky factor in FP64 (MatrixDiagInv.hpp:25-140),
+// Y = W V^-1 and t = W V^-1 (-bV) for the W blocks of point q; false if the
+// damped block is not positive definite (V^-1 is then zero)
+template <typename T>
+__device__ __forceinline__ bool vinv_y_point(const BAView& v, int q, const T* __restrict__ V,
+                                             const T* __restrict__ bV, const T* __restrict__ W,
+                                             T* __restrict__ Vinv, T* __restrict__ Y, T* __restrict__ tW, double rel,
+                                             double ab) {
+  double a00 = V[6 * (size_t)q + 0], a01 = V[6 * (size_t)q + 1], a02 = V[6 * (size_t)q + 2];
+  double a11 = V[6 * (size_t)q + 3], a12 = V[6 * (size_t)q + 4], a22 = V[6 * (size_t)q + 5];
+  if (rel != 0.0 || ab != 0.0) {
+    a00 = a00 * (1.0 + rel) + ab;
+    a11 = a11 * (1.0 + rel) + ab;
+    a22 = a22 * (1.0 + rel) + ab;
+  }
+  double vi[6];
+  bool ok = a00 > 0.0;
+  const double l00 = sqrt(a00);
+  const double l10 = a01 / l00, l20 = a02 / l00;
+  const double d1 = a11 - l10 * l10;
+  ok = ok && d1 > 0.0;
+  const double l11 = sqrt(d1);
+  const double l21 = (a12 - l20 * l10) / l11;
+  const double d2 = a22 - l20 * l20 - l21 * l21;
+  ok = ok && d2 > 0.0;
+  const double l22 = sqrt(d2);
+  const double m00 = 1.0 / l00, m11 = 1.0 / l11, m22 = 1.0 / l22;
+  const double m10 = -l10 * m00 * m11;
+  const double m21 = -l21 * m11 * m22;
+  const double m20 = -(l20 * m00 + l21 * m10) * m22;
+  vi[0] = m00 * m00 + m10 * m10 + m20 * m20;
+  vi[1] = m10 * m11 + m20 * m21;
+  vi[2] = m20 * m22;
+  vi[3] = m11 * m11 + m21 * m21;
+  vi[4] = m21 * m22;
+  vi[5] = m22 * m22;
+  ok = ok && isfinite(vi[0]) && isfinite(vi[3]) && isfinite(vi[5]);
+  if (!ok)
+    for (int i = 0; i < 6; i++) vi[i] = 0.0;
+#pragma unroll
+  for (int i = 0; i < 6; i++) Vinv[6 * (size_t)q + i] = (T)vi[i];
+  const double b0 = bV[3 * (size_t)q], b1 = bV[3 * (size_t)q + 1], b2 = bV[3 * (size_t)q + 2];
+  // x0 = V^-1 (-bV)
+  const double x0 = -(vi[0] * b0 + vi[1] * b1 + vi[2] * b2);
+  const double x1 = -(vi[1] * b0 + vi[3] * b1 + vi[4] * b2);
+  const double x2 = -(vi[2] * b0 + vi[4] * b1 + vi[5] * b2);
+  const T M[9] = {(T)vi[0], (T)vi[1], (T)vi[2], (T)vi[1], (T)vi[3], (T)vi[4], (T)vi[2], (T)vi[4], (T)vi[5]};
+  const int k1 = v.wptr[q + 1];
+  for (int k = v.wptr[q]; k < k1; k++) {
+    const T* w = W + 18 * (size_t)k;
+    T y[18];
+    T t[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+      const T w0 = w[3 * i], w1 = w[3 * i + 1], w2 = w[3 * i + 2];
+#pragma unroll
+      for (int j = 0; j < 3; j++) y[3 * i + j] = w0 * M[j] + w1 * M[3 + j] + w2 * M[6 + j];
+      t[i] = (T)((double)w0 * x0 + (double)w1 * x1 + (double)w2 * x2);
+    }
+#pragma unroll
+    for (int i = 0; i < 18; i++) Y[18 * (size_t)k + i] = y[i];
+#pragma unroll
+    for (int i = 0; i < 6; i++) tW[6 * (size_t)k + i] = t[i];
+  }
+  return ok;
+}
+
+// back-substitution of point q: xV_q = V_q^-1 (rhs_q - sum_k W_k^T xU_c(k)),
+// rhs = -bV or rhs_v, added to xV with `add`; inc = |xV_q|^2 and
+// den = xV_q^T (mu xV_q - b_q) with b = bden (FP64) or bV
+template <typename T>
+__device__ __forceinline__ void backsub_point(const BAView& v, int q, const T* __restrict__ W,
+                                              const T* __restrict__ Vinv, const T* __restrict__ bV,
+                                              const double* __restrict__ xU, double* __restrict__ xV, double mu,
+                                              const double* __restrict__ rhs_v, int add,
+                                              const double* __restrict__ bden, double& inc, double& den) {
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  const int k1 = v.wptr[q + 1];
+  for (int k = v.wptr[q]; k < k1; k++) {
+    const T* w = W + 18 * (size_t)k;
+    const double* x = xU + 6 * (size_t)v.w_cam[k];
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+      const double xi = x[i];
+      s0 += (double)w[3 * i] * xi;
+      s1 += (double)w[3 * i + 1] * xi;
+      s2 += (double)w[3 * i + 2] * xi;
+    }
+  }
+  double bv0, bv1, bv2;
+  if (bden) {
+    bv0 = bden[3 * (size_t)q]; bv1 = bden[3 * (size_t)q + 1]; bv2 = bden[3 * (size_t)q + 2];
+  } else {
+    bv0 = bV[3 * (size_t)q]; bv1 = bV[3 * (size_t)q + 1]; bv2 = bV[3 * (size_t)q + 2];
+  }
+  double t0, t1, t2;
+  if (rhs_v) {
+    t0 = rhs_v[3 * (size_t)q] - s0; t1 = rhs_v[3 * (size_t)q + 1] - s1; t2 = rhs_v[3 * (size_t)q + 2] - s2;
+  } else {
+    t0 = -(double)bV[3 * (size_t)q] - s0; t1 = -(double)bV[3 * (size_t)q + 1] - s1;
+    t2 = -(double)bV[3 * (size_t)q + 2] - s2;
+  }
+  const T* m = Vinv + 6 * (size_t)q;
+  const double m0 = m[0], m1 = m[1], m2 = m[2], m3 = m[3], m4 = m[4], m5 = m[5];
+  double x0 = m0 * t0 + m1 * t1 + m2 * t2;
+  double x1 = m1 * t0 + m3 * t1 + m4 * t2;
+  double x2 = m2 * t0 + m4 * t1 + m5 * t2;
+  if (add) {
+    x0 += xV[3 * (size_t)q];
+    x1 += xV[3 * (size_t)q + 1];
+    x2 += xV[3 * (size_t)q + 2];
+  }
+  xV[3 * (size_t)q] = x0;
+  xV[3 * (size_t)q + 1] = x1;
+  xV[3 * (size_t)q + 2] = x2;
+  inc = x0 * x0 + x1 * x1 + x2 * x2;
+  den = x0 * (mu * x0 - bv0) + x1 * (mu * x1 - bv1) + x2 * (mu * x2 - bv2);
+}
+
+}  // namespace nsn
