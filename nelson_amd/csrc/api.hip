@@ -14,6 +14,7 @@
 #include "internal.h"
 #include "kernels.h"
 #include "nelson_amd.h"
+#include "structure.h"
 
 using namespace nsn;
 
@@ -227,116 +228,29 @@ int check_device_errors(nsn_problem* h) {
 // ---- structure stage ------------------------------------------------------
 int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
   const int C = h->C, P = h->P, E = h->E;
-  h->cam_var.assign(C, -1);
-  h->pt_var.assign(P, -1);
-  for (int c = 0; c < C; c++)
-    if (!(d->cam_fixed && d->cam_fixed[c])) { h->cam_var[c] = (int)h->var_cam_user.size(); h->var_cam_user.push_back(c); }
-  for (int p = 0; p < P; p++)
-    if (!(d->pt_fixed && d->pt_fixed[p])) { h->pt_var[p] = (int)h->var_pt_user.size(); h->var_pt_user.push_back(p); }
-  h->NU = (int)h->var_cam_user.size();
-  h->NV = (int)h->var_pt_user.size();
-  const int NU = h->NU, NV = h->NV;
-  for (int e = 0; e < E; e++) {
-    if (d->cam_idx[e] < 0 || d->cam_idx[e] >= C || d->pt_idx[e] < 0 || d->pt_idx[e] >= P)
-      return fail(NSN_EINVAL, "edge %d references camera %d / point %d out of range", e, d->cam_idx[e], d->pt_idx[e]);
-  }
-  // point-major order: (point, camera, edge index)
-  std::vector<int> pe(E);
-  std::iota(pe.begin(), pe.end(), 0);
-  std::stable_sort(pe.begin(), pe.end(), [&](int a, int b) {
-    if (d->pt_idx[a] != d->pt_idx[b]) return d->pt_idx[a] < d->pt_idx[b];
-    return d->cam_idx[a] < d->cam_idx[b];
-  });
-  std::vector<int> pt_ptr(P + 1, 0), pe_cam(E), pe_wblk(E, -1);
-  std::vector<double> pe_obs(2 * (size_t)E), pe_w(E);
-  for (int e = 0; e < E; e++) pt_ptr[d->pt_idx[e] + 1]++;
-  for (int p = 0; p < P; p++) pt_ptr[p + 1] += pt_ptr[p];
-  std::vector<int> wptr(NV + 1, 0);
-  h->w_cam.clear();
-  h->w_pt.clear();
-  for (int s = 0; s < E; s++) {
-    const int e = pe[s];
-    pe_cam[s] = d->cam_idx[e];
-    pe_obs[2 * s] = d->obs[2 * e];
-    pe_obs[2 * s + 1] = d->obs[2 * e + 1];
-    pe_w[s] = d->weight ? d->weight[e] : 1.0;
-    const int cv = h->cam_var[d->cam_idx[e]], pv = h->pt_var[d->pt_idx[e]];
-    if (cv >= 0 && pv >= 0) {
-      const bool same = !h->w_cam.empty() && h->w_pt.back() == pv && h->w_cam.back() == cv;
-      if (!same) {
-        h->w_cam.push_back(cv);
-        h->w_pt.push_back(pv);
-        wptr[pv + 1]++;
-      }
-      pe_wblk[s] = (int)h->w_cam.size() - 1;
-    }
-  }
-  for (int v = 0; v < NV; v++) wptr[v + 1] += wptr[v];
-  h->nW = (int)h->w_cam.size();
-  const int nW = h->nW;
-  // camera-major edges of the variable cameras: (camera, point, edge index)
-  std::vector<int> ce;
-  ce.reserve(E);
-  for (int e = 0; e < E; e++)
-    if (h->cam_var[d->cam_idx[e]] >= 0) ce.push_back(e);
-  std::stable_sort(ce.begin(), ce.end(), [&](int a, int b) {
-    if (d->cam_idx[a] != d->cam_idx[b]) return d->cam_idx[a] < d->cam_idx[b];
-    return d->pt_idx[a] < d->pt_idx[b];
-  });
-  h->Ec = (int)ce.size();
-  std::vector<int> cam_ptr(NU + 1, 0), ce_pt(h->Ec);
-  std::vector<double> ce_obs(2 * (size_t)h->Ec), ce_w(h->Ec);
-  for (int s = 0; s < h->Ec; s++) {
-    const int e = ce[s];
-    cam_ptr[h->cam_var[d->cam_idx[e]] + 1]++;
-    ce_pt[s] = d->pt_idx[e];
-    ce_obs[2 * s] = d->obs[2 * e];
-    ce_obs[2 * s + 1] = d->obs[2 * e + 1];
-    ce_w[s] = d->weight ? d->weight[e] : 1.0;
-  }
-  for (int u = 0; u < NU; u++) cam_ptr[u + 1] += cam_ptr[u];
-  // W blocks grouped by camera (ascending point within a camera)
-  std::vector<int> cw_ptr(NU + 1, 0), cw_idx(nW);
-  for (int k = 0; k < nW; k++) cw_ptr[h->w_cam[k] + 1]++;
-  for (int u = 0; u < NU; u++) cw_ptr[u + 1] += cw_ptr[u];
-  {
-    std::vector<int> fill(cw_ptr.begin(), cw_ptr.end() - 1);
-    for (int k = 0; k < nW; k++) cw_idx[fill[h->w_cam[k]]++] = k;
-  }
-  // reference order of W blocks: row-major (camera, point)
-  h->w_ref = cw_idx;
-  // S structure: pairs of blocks sharing a point, grouped by camera pair
-  struct PairRec { int ci, cj, ka, kb; };
-  std::vector<PairRec> pr;
-  for (int v = 0; v < NV; v++)
-    for (int a = wptr[v]; a < wptr[v + 1]; a++)
-      for (int b = a + 1; b < wptr[v + 1]; b++) pr.push_back({h->w_cam[a], h->w_cam[b], a, b});
-  std::stable_sort(pr.begin(), pr.end(), [](const PairRec& x, const PairRec& y) {
-    return x.ci != y.ci ? x.ci < y.ci : x.cj < y.cj;
-  });
-  h->npairs = (long long)pr.size();
-  std::vector<int> sb_i, sb_j, sb_ptr, sb_pairs(2 * pr.size());
-  {
-    size_t q = 0;
-    for (int u = 0; u < NU; u++) {
-      sb_i.push_back(u);
-      sb_j.push_back(u);
-      sb_ptr.push_back((int)q);
-      while (q < pr.size() && pr[q].ci == u) {
-        const int cj = pr[q].cj;
-        sb_i.push_back(u);
-        sb_j.push_back(cj);
-        sb_ptr.push_back((int)q);
-        while (q < pr.size() && pr[q].ci == u && pr[q].cj == cj) {
-          sb_pairs[2 * q] = pr[q].ka;
-          sb_pairs[2 * q + 1] = pr[q].kb;
-          q++;
-        }
-      }
-    }
-    sb_ptr.push_back((int)q);
-  }
-  h->nsb = (int)sb_i.size();
+  HostStructure hs;
+  int rc0 = host_structure(d, hs);
+  if (rc0) return rc0;
+  h->cam_var = hs.cam_var;
+  h->pt_var = hs.pt_var;
+  h->var_cam_user = hs.var_cam_user;
+  h->var_pt_user = hs.var_pt_user;
+  h->w_cam = hs.w_cam;
+  h->w_pt = hs.w_pt;
+  h->w_ref = hs.w_ref;
+  h->NU = hs.NU;
+  h->NV = hs.NV;
+  h->nW = hs.nW;
+  h->Ec = hs.Ec;
+  h->npairs = hs.npairs;
+  h->nsb = hs.nsb;
+  const int NU = h->NU, NV = h->NV, nW = h->nW;
+  std::vector<int>& pe = hs.pe;
+  std::vector<int>& ce = hs.ce;
+  const std::vector<int>&pt_ptr = hs.pt_ptr, &pe_cam = hs.pe_cam, &pe_wblk = hs.pe_wblk, &wptr = hs.wptr,
+  &cam_ptr = hs.cam_ptr, &ce_pt = hs.ce_pt, &cw_ptr = hs.cw_ptr, &cw_idx = hs.cw_idx, &sb_i = hs.sb_i,
+  &sb_j = hs.sb_j, &sb_ptr = hs.sb_ptr, &sb_pairs = hs.sb_pairs;
+  const std::vector<double>&pe_obs = hs.pe_obs, &pe_w = hs.pe_w, &ce_obs = hs.ce_obs, &ce_w = hs.ce_w;
   // tiles of [S; bS^T]
   const int n = 6 * NU;
   TileView tv;

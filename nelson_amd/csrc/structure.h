@@ -1,0 +1,39 @@
+// Structure stage of a BA problem on the host (the equivalent of
+// DoubleSection::parametersReady / addEdge / structureReady,
+// nelson/DoubleSection.hpp:30-246, 338-365): variable numbering, edge
+// orders, W blocks and the S block pattern with its (W, W) pair lists.
+#pragma once
+
+#include <vector>
+
+#include "nelson_amd.h"
+
+namespace nsn {
+
+struct HostStructure {
+  int C = 0, P = 0, E = 0, NU = 0, NV = 0, nW = 0, Ec = 0, nsb = 0;
+  long long npairs = 0;
+  std::vector<int> cam_var, pt_var;            // user -> variable index or -1
+  std::vector<int> var_cam_user, var_pt_user;  // variable -> user index
+  // edges in point-major order (user point, then camera)
+  std::vector<int> pe;  // user edge index of each slot
+  std::vector<int> pt_ptr, pe_cam, pe_wblk;
+  std::vector<double> pe_obs, pe_w;
+  // W blocks (one per variable (camera, point) pair), point-major
+  std::vector<int> wptr, w_cam, w_pt;
+  // edges of the variable cameras in camera-major order (camera, then point)
+  std::vector<int> ce;  // user edge index of each slot
+  std::vector<int> cam_ptr, ce_pt;
+  std::vector<double> ce_obs, ce_w;
+  // W blocks grouped by camera; w_ref = the reference's row-major W order
+  std::vector<int> cw_ptr, cw_idx, w_ref;
+  // S blocks (i <= j; the diagonal ones with empty pair lists) and, for the
+  // off-diagonal ones, the (W_i, W_j) block pairs sharing a point
+  std::vector<int> sb_i, sb_j, sb_ptr, sb_pairs;
+};
+
+// fills s from the problem description; NSN_OK or NSN_EINVAL (with the
+// library's error message) for edges out of range
+int host_structure(const nsn_problem_desc* d, HostStructure& s);
+
+}  // namespace nsn
