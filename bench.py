@@ -98,6 +98,34 @@ def adjoint_timing(nsn):
     return out
 
 
+def batch_timing(nsn, n_windows=512, iters=10):
+    """Batched local BA: 512 independent windows (cameras U{5..40}, points
+    U{200..3000}, 3-6 observations per point, 5% outliers at 20 px, Huber
+    delta 2 px), mixed precision, LM with `iters` iterations on every window
+    in one persistent launch; launch shape from the autotuner."""
+    import numpy as np
+    wins = [nsn.generate_ba(nsn.window_config(w)) for w in range(n_windows)]
+    t0 = time.perf_counter()
+    b = nsn.BABatch(wins, precision="mixed")
+    t1 = time.perf_counter()
+    s = nsn.LevenbergMarquardtSettings(maxNumIt=iters)
+    b.tune(s)
+    cams0, pts0 = b.get_state()
+    times = []
+    for _ in range(3):
+        b.set_state(cams0, pts0)
+        chi2, ms = b.lm_solve(s)
+        times.append(ms)
+    reasons, its, subs = b.results()
+    ms = min(times)
+    return {"workload": f"{n_windows} windows, {sum(g['n_cameras'] for g in wins)} cameras / "
+                        f"{sum(g['n_points'] for g in wins)} points / {sum(g['n_edges'] for g in wins)} observations, "
+                        f"mixed precision, Huber 2 px, LM {iters} iterations",
+            "batch_solves_per_s": 1e3 / ms, "windows_per_s": n_windows * 1e3 / ms, "ms_per_batch_solve": ms,
+            "mean_iterations": float(np.mean(its)), "mean_sub_iterations": float(np.mean(subs)),
+            "config": b.config(), "create_s": t1 - t0}
+
+
 def cpu_baseline(g, budget_s):
     """The oracle (C restatement of the reference path, 1 thread) timed on a
     bounded sample: one LM iteration's work = update(true) + one Schur step +
@@ -127,6 +155,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=60.0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -168,6 +197,10 @@ def main():
                        "fp64_final_chi2": s64["final_chi2"], "mixed_final_chi2": sm["final_chi2"],
                        "final_cost_rel_err": abs(sm["final_chi2"] - s64["final_chi2"]) / s64["final_chi2"]}
         log(f"bench: mixed small {mixed_small}")
+    batch = None
+    if not args.no_batch:
+        batch = batch_timing(nsn)
+        log(f"bench: batch {batch}")
     adjoint = None
     if not args.no_adjoint:
         adjoint = adjoint_timing(nsn)
@@ -200,6 +233,7 @@ def main():
         "mixed": mixed,
         "mixed_small": mixed_small,
         "adjoint": adjoint,
+        "batch": batch,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

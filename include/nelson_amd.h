@@ -224,15 +224,32 @@ int nsn_set_observations(nsn_problem* h, const double* obs, const double* weight
 int nsn_adjoint(nsn_problem* h, const double* g_cams, const double* g_pts, int hessian_steps,
                 double* dL_dobs, double* dL_dw);
 
-/* ---- batched independent windows (no reference counterpart) ------------- */
+/* ---- batched independent windows (no reference counterpart) -------------
+ * Many independent BA problems ("windows", uneven sizes, each with at most 42
+ * variable cameras) solved by LevenbergMarquardt<SolverCholeskySchur>
+ * (LevenbergMarquardt.hpp:35-146) in one persistent launch: each workgroup
+ * runs whole windows.  All windows share precision and device. */
 typedef struct nsn_batch nsn_batch;
 int nsn_batch_create(const nsn_problem_desc* windows, int n_windows, nsn_batch** out);
 int nsn_batch_destroy(nsn_batch* b);
-/* cams / pts: concatenation over windows of 6*C_w and 3*P_w values */
+/* cams / pts: concatenation over windows of 6*C_w (angle-axis, t) and 3*P_w values */
 int nsn_batch_set_state(nsn_batch* b, const double* cams, const double* pts);
 int nsn_batch_get_state(nsn_batch* b, double* cams, double* pts);
-/* runs s->maxNumIt LM iterations on every window; chi2_out [n_windows] final costs */
+/* LM with the settings s on every window from the current state; chi2_out
+ * [n_windows] (may be NULL) final costs, *elapsed_ms (may be NULL) device
+ * time of the launch.  The first call runs nsn_batch_tune unless a launch
+ * shape was set. */
 int nsn_batch_lm_solve(nsn_batch* b, const nsn_lm_settings* s, double* chi2_out, double* elapsed_ms);
+/* per-window termination reason (NSN_* reasons), outer iterations and Schur
+ * solves of the last nsn_batch_lm_solve; any array may be NULL */
+int nsn_batch_results(nsn_batch* b, int* reasons, int* iterations, int* sub_iterations);
+/* autotuner: times every launch shape (Cholesky panel width 4 / 8 / 16 x
+ * 256 / 512 / 1024 persistent workgroups) for up to 2 LM iterations on this
+ * batch (state restored after each) and keeps the fastest */
+int nsn_batch_tune(nsn_batch* b, const nsn_lm_settings* s, int* panel_width, int* grid);
+int nsn_batch_set_config(nsn_batch* b, int panel_width, int grid);
+/* the launch shape in use and the tuner's timings (ms, in candidate order) */
+int nsn_batch_config(nsn_batch* b, int* panel_width, int* grid, double* tune_ms, int n_tune);
 
 #ifdef __cplusplus
 }

@@ -12,5 +12,6 @@ from .capi import (  # noqa: F401
 )
 from .problem import (  # noqa: F401
     BAProblem, SolverCholeskySchur, LevenbergMarquardt, LevenbergMarquardtSettings, GaussNewton,
-    GaussNewtonSettings, generate_ba, CONFIGS,
+    GaussNewtonSettings, generate_ba, window_config, CONFIGS,
 )
+from .batch import BABatch  # noqa: F401

@@ -115,6 +115,10 @@ SIGNATURES = {
     "nsn_batch_set_state": (C.c_int, [_P, _D, _D]),
     "nsn_batch_get_state": (C.c_int, [_P, _D, _D]),
     "nsn_batch_lm_solve": (C.c_int, [_P, C.POINTER(LMSettings), _D, _D]),
+    "nsn_batch_results": (C.c_int, [_P, _I, _I, _I]),
+    "nsn_batch_tune": (C.c_int, [_P, C.POINTER(LMSettings), _I, _I]),
+    "nsn_batch_set_config": (C.c_int, [_P, C.c_int, C.c_int]),
+    "nsn_batch_config": (C.c_int, [_P, _I, _I, _D, C.c_int]),
 }
 
 _lib = None

@@ -1,0 +1,90 @@
+"""Batched independent windows (nsn_batch_*): one persistent launch runs the
+LM solve of every window.  FP64 batch results match the single-problem GPU
+path and the oracle (final cost 1e-9, same termination); the mixed-precision
+batch reaches the FP64 final cost to 1e-6; the autotuner's pick and the state
+round trip behave.
+"""
+import numpy as np
+import pytest
+
+from nelson_amd import BABatch, BAProblem, CONFIGS, LevenbergMarquardt, LevenbergMarquardtSettings, generate_ba, \
+    window_config
+from oracle.oracle import OracleBA
+
+pytestmark = pytest.mark.gpu
+
+
+def _windows(n, seed=0, huber=False):
+    rng = np.random.default_rng(seed)
+    out = []
+    for w in range(n):
+        nc = int(rng.integers(3, 20))
+        npt = int(rng.integers(30, 400))
+        cfg = dict(CONFIGS["cfg1"], n_cameras=nc, n_points=npt, k_min=3, k_max=min(6, nc), seed=500 + 17 * w + seed)
+        if huber:
+            cfg.update(outlier_frac=0.05, outlier_sigma=20.0, huber_delta=2.0)
+        out.append(generate_ba(cfg))
+    return out
+
+
+def _single(g, precision="fp64", **kw):
+    p = BAProblem.from_generated(g, precision=precision)
+    lm = LevenbergMarquardt(LevenbergMarquardtSettings(**kw))
+    reason = lm.solve(p)
+    return lm.result()["final_chi2"], reason, lm.numIterations()
+
+
+@pytest.mark.parametrize("huber", [False, True])
+def test_batch_fp64_matches_single_problem(huber):
+    wins = _windows(24, seed=1 if huber else 0, huber=huber)
+    b = BABatch(wins, precision="fp64")
+    kw = dict(maxNumIt=15)
+    chi2, ms = b.lm_solve(LevenbergMarquardtSettings(**kw))
+    reasons, its, _ = b.results()
+    for w, g in enumerate(wins):
+        c1, r1, i1 = _single(g, **kw)
+        assert abs(chi2[w] - c1) <= 1e-9 * c1, (w, chi2[w], c1)
+        conv = {"SuccessSmallIncrement", "MaxIterationReached", "MaxSubIterationReached"}
+        assert reasons[w] == r1 or {reasons[w], r1} <= conv, (w, reasons[w], r1)
+    print(f"\nbatch of 24 windows (huber={huber}): {ms:.2f} ms, iterations {list(its)}")
+
+
+def test_batch_vs_oracle():
+    wins = _windows(6, seed=3)
+    b = BABatch(wins, precision="fp64")
+    chi2, _ = b.lm_solve(LevenbergMarquardtSettings(maxNumIt=20))
+    for w, g in enumerate(wins):
+        ro = OracleBA.from_generated(g).lm_solve(maxNumIt=20)
+        assert abs(chi2[w] - ro["final_chi2"]) <= 1e-9 * ro["final_chi2"], (w, chi2[w], ro["final_chi2"])
+
+
+def test_batch_mixed_reaches_fp64_cost():
+    wins = [generate_ba(window_config(w)) for w in range(8)]
+    s = LevenbergMarquardtSettings(maxNumIt=50)
+    c64, _ = BABatch(wins, precision="fp64").lm_solve(s)
+    bm = BABatch(wins, precision="mixed")
+    cm, ms = bm.lm_solve(s)
+    err = np.abs(cm - c64) / c64
+    print(f"\nmixed vs FP64 batch (8 windows of the local-BA workload): max final-cost rel err {err.max():.2e}, "
+          f"config {bm.config()}")
+    assert err.max() <= 1e-6
+
+
+def test_batch_tune_and_state_roundtrip():
+    wins = _windows(10, seed=5)
+    b = BABatch(wins, precision="mixed")
+    cams0, pts0 = b.get_state()
+    # angle-axis read back through the rotation matrix, as BAProblem does
+    ref = np.concatenate([BAProblem.from_generated(g).get_state()[0] for g in wins])
+    assert np.allclose(cams0, ref, rtol=0, atol=1e-12)
+    assert np.array_equal(pts0, np.concatenate([g["pts0"] for g in wins]))
+    s = LevenbergMarquardtSettings(maxNumIt=10)
+    nb, grid = b.tune(s)
+    assert nb in (4, 8, 16) and grid in (256, 512, 1024)
+    cams1, pts1 = b.get_state()
+    assert np.array_equal(cams1, cams0) and np.array_equal(pts1, pts0)  # the tuner restores the state
+    c_a, _ = b.lm_solve(s)
+    b.set_state(cams0, pts0)
+    b.set_config(4, 3)
+    c_b, _ = b.lm_solve(s)
+    assert np.allclose(c_a, c_b, rtol=1e-6)
