@@ -75,7 +75,24 @@ def build(verbose=False, jobs=8):
     tmp = LIB + ".tmp"
     _run(["hipcc", "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs)
     os.replace(tmp, LIB)
+    build_cpp_tests()
     return LIB
+
+
+CPP_TESTS = os.path.join(ROOT, "tests", "cpp")
+
+
+def build_cpp_tests():
+    """The C++ programs under tests/cpp, linked against the library (rpath to
+    the in-tree copy)."""
+    out = []
+    for f in sorted(os.listdir(CPP_TESTS)):
+        if f.endswith(".cpp"):
+            exe = os.path.join(CPP_TESTS, f[:-4])
+            _run(["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), os.path.join(CPP_TESTS, f),
+                  "-L", HERE, "-lnelson_amd", "-Wl,-rpath,$ORIGIN/../../nelson_amd", "-o", exe])
+            out.append(exe)
+    return out
 
 
 if __name__ == "__main__":

@@ -1,0 +1,91 @@
+// The C++ interface (nelson_amd.hpp) used the way the reference's tests use
+// DoubleSection / SolverCholeskySchur / LevenbergMarquardt / GaussNewton
+// (nelson/tests/testDoubleSectionCommon.hpp:55-130): Config 1's BA problem
+// (10 cameras, 500 points, 2000 observations) built through parameterU /
+// parameterV / addEdge / structureReady, one Schur step, an LM solve and a
+// GN solve.  Prints one JSON object per solve on stdout; the GPU test
+// compares them with the oracle.
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+#include "nelson_amd.hpp"
+
+using namespace nelson_amd;
+
+static void build(BASection& pss, const std::vector<double>& cams, const std::vector<double>& pts,
+                  const std::vector<int>& ci, const std::vector<int>& pi, const std::vector<double>& obs, int C,
+                  int P) {
+  pss.setNumParameters(C, P);
+  for (int i = 0; i < C; i++)
+    for (int k = 0; k < 3; k++) {
+      pss.parameterU(i).w[k] = cams[6 * i + k];
+      pss.parameterU(i).t[k] = cams[6 * i + 3 + k];
+    }
+  for (int j = 0; j < P; j++)
+    for (int k = 0; k < 3; k++) pss.parameterV(j)[k] = pts[3 * j + k];
+  pss.setFixedU(0);  // gauge of the BA sample (BASchurPerfomances.cpp:245-246)
+  pss.setFixedV(0);
+  pss.parametersReady();
+  for (size_t e = 0; e < ci.size(); e++) pss.addEdge(ci[e], pi[e], obs[2 * e], obs[2 * e + 1]);
+  pss.structureReady();
+}
+
+static void print_log(const char* name, TerminationReason tc, int its, double chi2, const Stats& st) {
+  std::printf("{\"solve\": \"%s\", \"reason\": \"%s\", \"iterations\": %d, \"final_chi2\": %.17g, \"log\": [", name,
+              toString(tc).c_str(), its, chi2);
+  for (size_t i = 0; i < st.log.size(); i++)
+    std::printf("%s[%d, %d, %.17g, %.17g]", i ? ", " : "", st.log[i].it, st.log[i].subit, st.log[i].chi2,
+                st.log[i].mu);
+  std::printf("]}\n");
+}
+
+int main() {
+  nsn_gen_params g{};
+  g.n_cameras = 10; g.n_points = 500; g.k_min = 4; g.k_max = 4; g.layout = 0; g.arc = 0; g.seed = 1;
+  g.obs_sigma = 1.0; g.outlier_frac = 0.0; g.outlier_sigma = 0.0; g.rot_sigma = 0.01; g.trans_sigma = 0.05;
+  g.point_sigma = 0.1; g.focal = 500.0; g.cx = 320.0; g.cy = 240.0;
+  const int C = g.n_cameras, P = g.n_points, cap = P * g.k_max;
+  std::vector<double> cgt(6 * C), pgt(3 * P), c0(6 * C), p0(3 * P), obs(2 * cap);
+  std::vector<int> ci(cap), pi(cap);
+  int ne = 0;
+  check(nsn_generate_ba(&g, cap, cgt.data(), pgt.data(), c0.data(), p0.data(), ci.data(), pi.data(), obs.data(), &ne),
+        "nsn_generate_ba");
+  ci.resize(ne);
+  pi.resize(ne);
+  obs.resize(2 * ne);
+  try {
+    {
+      BASection pss(g.focal, g.cx, g.cy);
+      build(pss, c0, p0, ci, pi, obs, C, P);
+      pss.update(true);
+      SolverCholeskySchur solver;
+      solver.init(pss);
+      const bool ok = solver.computeIncrement(pss, 0.0, 62.0);
+      std::printf("{\"solve\": \"step\", \"ok\": %s, \"sq_norm\": %.17g, \"max_abs_h_diag\": %.17g, \"du0\": %.17g}\n",
+                  ok ? "true" : "false", solver.incrementVectorSquaredNorm(), solver.maxAbsHDiag(),
+                  solver.incrementU()[0]);
+    }
+    {
+      BASection pss(g.focal, g.cx, g.cy);
+      build(pss, c0, p0, ci, pi, obs, C, P);
+      LevenbergMarquardt<SolverCholeskySchur> lm;
+      lm.settings().maxNumIt = 20;
+      const TerminationReason tc = lm.solve(pss);
+      print_log("lm", tc, lm.numIterations(), pss.update(false), lm.stats());
+    }
+    {
+      BASection pss(g.focal, g.cx, g.cy);
+      build(pss, c0, p0, ci, pi, obs, C, P);
+      GaussNewton<SolverCholeskySchur> gn;
+      gn.settings().maxNumIt = 5;
+      gn.settings().epsIncSquare = 0.0;
+      const TerminationReason tc = gn.solve(pss);
+      print_log("gn", tc, gn.numIterations(), pss.chi2(), gn.stats());
+    }
+  } catch (const Error& e) {
+    std::fprintf(stderr, "nelson_amd error %d: %s\n", e.code(), e.what());
+    return 1;
+  }
+  return 0;
+}

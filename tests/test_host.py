@@ -118,3 +118,14 @@ def test_problem_create_without_gpu_fails_loudly():
     with pytest.raises(capi.NelsonError) as ei:
         nsn.BAProblem.from_generated(g)
     assert ei.value.code == capi.NSN_EHIP
+
+
+def test_cpp_interface_compiles_and_links():
+    """include/nelson_amd.hpp compiles as C++17 and the facade test program
+    is built next to the library (it runs in the GPU suite)."""
+    import subprocess
+    src = os.path.join(ROOT, "tests", "cpp", "test_facade.cpp")
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", "-I", os.path.join(ROOT, "include"),
+                        src], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    assert r.returncode == 0, r.stdout
+    assert os.path.exists(os.path.join(ROOT, "tests", "cpp", "test_facade"))
