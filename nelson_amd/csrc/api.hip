@@ -72,6 +72,8 @@ struct nsn_problem {
   void *U = nullptr, *bU = nullptr, *V = nullptr, *bV = nullptr, *W = nullptr, *Vinv = nullptr, *Y = nullptr,
        *tW = nullptr, *Sd = nullptr, *bS = nullptr, *tiles = nullptr, *inv = nullptr;
   double *xU = nullptr, *xV = nullptr;
+  int* chol_tasks = nullptr;  // worker task list of the factorisation
+  int n_chol_tasks = 0;
   int *tile_order = nullptr, *tile_blk_ptr = nullptr, *tile_blk = nullptr, *sb_i = nullptr, *sb_j = nullptr,
       *sb_ptr = nullptr, *sb_pairs = nullptr;
   int *flags = nullptr, *counters = nullptr;
@@ -164,8 +166,11 @@ void seq_schur(nsn_problem* h, int gate, int checks, bool timed) {
                     st);
   if (timed) (void)hipEventRecord(h->ev[2], st);
   launch_chol_reset<T>(h->flags, h->nflags, h->counters, h->ctl, gate, st);
-  launch_chol<T>(h->tv, (T*)h->tiles, (T*)h->inv, h->tile_order, h->flags, h->counters, h->ctl, gate, kCholMaxWG,
-                 st);
+  {
+    int* pre_sub = h->flags + h->tv.ntiles + h->NR + h->NR * h->NR;
+    launch_chol<T>(h->tv, (T*)h->tiles, (T*)h->inv, h->chol_tasks, h->n_chol_tasks, h->flags, pre_sub,
+                   pre_sub + h->tv.NT, h->counters, h->ctl, gate, kCholMaxWG, st);
+  }
   if (timed) (void)hipEventRecord(h->ev[3], st);
   launch_trsv<T>(h->tv, (const T*)h->tiles, (const T*)h->inv, true, nullptr, h->xU, h->parts,
                  h->flags + h->tv.ntiles, h->flags + h->tv.ntiles + h->NR, h->counters + 1, h->ctl, gate, kTrsvMaxWG,
@@ -283,7 +288,12 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
     for (int sb = 0; sb < h->nsb; sb++) for_tiles(sb, [&](long long t) { tile_blk[fill[t]++] = sb; });
   }
   h->NR = (n + kTB - 1) / kTB;
-  h->nflags = tv.ntiles + h->NR + h->NR * h->NR;
+  h->nflags = tv.ntiles + h->NR + h->NR * h->NR + 2 * tv.NT;
+  int n_tasks = 0;
+  chol_task_list(tv.NT, nullptr, &n_tasks);
+  std::vector<int> chol_tasks(3 * (size_t)std::max(n_tasks, 1));
+  chol_task_list(tv.NT, chol_tasks.data(), &n_tasks);
+  h->n_chol_tasks = n_tasks;
 
   // ---- device allocation ----
   DevBuf& m = h->mem;
@@ -337,6 +347,7 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
   h->zU = a_dbl(6 * (size_t)NU);
   h->xV = a_dbl(3 * (size_t)NV);
   h->tile_order = a_int(tile_order.size());
+  h->chol_tasks = a_int(chol_tasks.size());
   h->tile_blk_ptr = a_int(tile_cnt.size());
   h->tile_blk = a_int(tile_blk.size());
   h->sb_i = a_int(sb_i.size());
@@ -368,7 +379,7 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
       (rc = upload(d_pe_w, pe_w)) || (rc = upload(d_pe_wblk, pe_wblk)) || (rc = upload(d_pe_edge, pe)) || (rc = upload(d_cam_ptr, cam_ptr)) ||
       (rc = upload(d_ce_pt, ce_pt)) || (rc = upload(d_ce_obs, ce_obs)) || (rc = upload(d_ce_w, ce_w)) ||
       (rc = upload(d_wptr, wptr)) || (rc = upload(d_w_cam, h->w_cam)) || (rc = upload(d_w_pt, h->w_pt)) || (rc = upload(d_cw_ptr, cw_ptr)) ||
-      (rc = upload(d_cw_idx, cw_idx)) || (rc = upload(h->tile_order, tile_order)) ||
+      (rc = upload(d_cw_idx, cw_idx)) || (rc = upload(h->tile_order, tile_order)) || (rc = upload(h->chol_tasks, chol_tasks)) ||
       (rc = upload(h->tile_blk_ptr, tile_cnt)) || (rc = upload(h->tile_blk, tile_blk)) ||
       (rc = upload(h->sb_i, sb_i)) || (rc = upload(h->sb_j, sb_j)) || (rc = upload(h->sb_ptr, sb_ptr)) ||
       (rc = upload(h->sb_pairs, sb_pairs)))

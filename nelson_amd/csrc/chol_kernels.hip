@@ -126,39 +126,36 @@ __device__ __forceinline__ double rsqrt_f64(double d) {
 
 // right-looking Cholesky of columns C0 .. C0+15 of the 64-row FP64 tile A
 // (rows >= C0), wave 0 only.  Rows with global index >= n (the bS row and the
-// identity padding) take pivot 1 and never count as failures.  The pivot
-// reciprocals go to their slots in the upper triangle.  Entries above the
-// diagonal in the registers are garbage and never read or written back, so
-// the updates run on every lane without predication.
+// identity padding) take pivot 1 and never count as failures; a failed pivot
+// is replaced by 1 and counted in `bad`.  The step is branch-free: every
+// lane computes the same pivot, the reciprocal square root of the selected
+// value, and the updates run on every lane (entries above the diagonal in the
+// registers are garbage and never written back).  The pivot reciprocals go to
+// their slots in the upper triangle after the panel.
 template <int TB, int C0>
-__device__ __forceinline__ void potrf_panel(double* A, int g0, int n, DevCtl* ctl) {
+__device__ __forceinline__ void potrf_panel(double* A, int g0, int n, int& bad) {
   const int lane = threadIdx.x;
   double a[kPW];
 #pragma unroll
   for (int c = 0; c < kPW; c++) a[c] = A[(C0 + c) * TB + lane];
+  double myr = 1.0;
 #pragma unroll
   for (int k = 0; k < kPW; k++) {
     const double d = readlane_f64(a[k], C0 + k);
-    double piv, r;
-    if (g0 + C0 + k >= n) {
-      piv = 1.0;
-      r = 1.0;
-    } else if (!(d > 0.0) || !isfinite(d)) {
-      if (lane == 0) atomicAdd(&ctl->fail_s, 1);
-      piv = 1.0;
-      r = 1.0;
-    } else {
-      r = rsqrt_f64(d);
-      piv = d * r;
-    }
+    const bool pad = g0 + C0 + k >= n;
+    const bool good = d > 0.0 && d < __builtin_inf();
+    bad |= (!pad && !good) ? 1 : 0;
+    const double r = (pad || !good) ? 1.0 : rsqrt_f64(d);
+    const double piv = (pad || !good) ? 1.0 : d * r;
     a[k] = (lane == C0 + k) ? piv : a[k] * r;
+    myr = (lane == C0 + k) ? r : myr;
 #pragma unroll
     for (int c = k + 1; c < kPW; c++) a[c] -= a[k] * readlane_f64(a[k], C0 + c);
-    if (lane == C0 + k) A[rdiag_slot<TB>(C0 + k)] = r;
   }
 #pragma unroll
   for (int c = 0; c < kPW; c++)
     if (lane >= C0 + c) A[(C0 + c) * TB + lane] = a[c];
+  if (lane >= C0 && lane < C0 + kPW) A[rdiag_slot<TB>(lane)] = myr;
 }
 
 // D[r, c] -= sum_{m < K} Lft[r][k0+m] Rgt[c][k0+m] for rows [r0, TB),
@@ -191,8 +188,8 @@ __device__ __forceinline__ void mfma_update(T* D, const T* Lft, const T* Rgt, in
 // Cholesky of a 64x64 FP64 tile in LDS (lower triangle written, the
 // reciprocals of its diagonal in the upper triangle).
 template <int TB, int KB>
-__device__ __forceinline__ void potrf_step(double* A, int g0, int n, DevCtl* ctl) {
-  if (threadIdx.x < 64) potrf_panel<TB, KB>(A, g0, n, ctl);
+__device__ __forceinline__ void potrf_step(double* A, int g0, int n, int& bad) {
+  if (threadIdx.x < 64) potrf_panel<TB, KB>(A, g0, n, bad);
   __syncthreads();
   if constexpr (KB + kPW < TB) {
     mfma_update<double, TB>(A, A, A, KB + kPW, KB + kPW, KB, true);
@@ -203,10 +200,12 @@ __device__ __forceinline__ void potrf_step(double* A, int g0, int n, DevCtl* ctl
 template <int TB>
 __device__ __forceinline__ void potrf_lds(double* A, int g0, int n, DevCtl* ctl) {
   static_assert(TB == 4 * kPW, "four panels");
-  potrf_step<TB, 0>(A, g0, n, ctl);
-  potrf_step<TB, kPW>(A, g0, n, ctl);
-  potrf_step<TB, 2 * kPW>(A, g0, n, ctl);
-  potrf_step<TB, 3 * kPW>(A, g0, n, ctl);
+  int bad = 0;
+  potrf_step<TB, 0>(A, g0, n, bad);
+  potrf_step<TB, kPW>(A, g0, n, bad);
+  potrf_step<TB, 2 * kPW>(A, g0, n, bad);
+  potrf_step<TB, 3 * kPW>(A, g0, n, bad);
+  if (threadIdx.x == 0 && bad) atomicAdd(&ctl->fail_s, 1);
 }
 
 // Li = L^-1 for the lower-triangular 64x64 FP64 tile L in LDS (diagonal
@@ -272,154 +271,265 @@ __device__ __forceinline__ void tri_inverse(const double* L, double* Li) {
   __syncthreads();
 }
 
-// Tile DAG factorisation.  Per tile (i, j): C_ij -= sum_k L_ik L_jk^T with
-// the 16x16x4 MFMA, operands loaded straight from global memory into the
-// MFMA operand registers (each wave owns a 32x32 quadrant of C); dependency
-// waits are batched (one acquire fence for every run of already-published
-// k); then, for i == j, POTRF and the inverse L_jj^-1 (FP64 in LDS; the
-// inverse goes to inv[j]), or, for i > j, L_ij = C_ij L_jj^-T as one more
-// MFMA product.  LDS: 64 KiB = two TB x TB FP64 regions.
+// ---------------------------------------------------------------------------
+// Tile DAG factorisation, one persistent launch.
+//
+// Workgroup 0 runs the diagonal chain, the critical path of the DAG: for
+// column j it takes the pre-updated diagonal tile C_jj, subtracts
+// L_{j,j-1} L_{j,j-1}^T with L_{j,j-1} still in its LDS from the previous
+// step, factorises it in FP64 (POTRF) and inverts the factor; then it takes
+// the pre-updated sub-diagonal tile C_{j+1,j} and forms L_{j+1,j} =
+// C_{j+1,j} L_jj^-T as an MFMA product, keeping it for step j+1.  Nothing on
+// that path crosses workgroups except the pre-updated tiles, which are
+// ready long before they are needed.
+//
+// The other workgroups take tasks from an atomic counter, grouped in slots
+// j = 0, 1, ...:
+//   slot j: pre-update of C_{j+1,j} and of C_{j+1,j+1} with every k < j
+//           (for j >= 1; written back in place, then flagged), followed by
+//           the tiles (i, j), i >= j+2: C_ij -= sum_{k<j} L_ik L_jk^T, then
+//           L_ij = C_ij L_jj^-T with the inverse published by workgroup 0.
+// A task of slot j only needs tiles of slots < j, the diagonal chain up to
+// step j, and (pre-updates) sub-diagonal tiles of steps < j; step j of the
+// chain needs only slots < j and the pre-update of C_{j+1,j} in slot j,
+// which comes before every task that waits for step j.  A workgroup
+// finishes a task before taking the next and tasks are taken in order, so
+// every awaited task belongs to a running workgroup: the DAG cannot
+// deadlock.  Every wait is bounded.
+//
+// MFMA tiling: each of the four waves owns a 32x32 quadrant of the 64x64
+// tile in accumulator registers; update operands come straight from global
+// memory into the operand registers, dependency waits are batched (lane l of
+// wave 0 checks step k + l, one acquire fence per run of published steps).
+// LDS: 64 KiB = two TB x TB FP64 regions.
+// ---------------------------------------------------------------------------
+enum CholTask : int { CT_TILE = 0, CT_PRE_SUB = 1, CT_PRE_DIAG = 2 };
+
+template <typename T>
+struct CholAcc {
+  typename Mfma<T>::acc_t a[2][2];
+};
+
+// acc <- tile (global, column-major TB x TB)
+template <typename T, int TB>
+__device__ __forceinline__ void acc_load(CholAcc<T>& acc, const T* Cg, int lane, int wr, int wc) {
+#pragma unroll
+  for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int vv = 0; vv < 4; vv++) {
+        const int r = wr * 32 + bi * 16 + Mfma<T>::row(vv, lane);
+        const int c = wc * 32 + bj * 16 + (lane & 15);
+        acc.a[bi][bj][vv] = Cg[c * TB + r];
+      }
+}
+
+// acc -> LDS tile (column-major), element type U
+template <typename T, typename U, int TB>
+__device__ __forceinline__ void acc_store(const CholAcc<T>& acc, U* D, int lane, int wr, int wc) {
+#pragma unroll
+  for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int vv = 0; vv < 4; vv++) {
+        const int r = wr * 32 + bi * 16 + Mfma<T>::row(vv, lane);
+        const int c = wc * 32 + bj * 16 + (lane & 15);
+        D[c * TB + r] = (U)acc.a[bi][bj][vv];
+      }
+}
+
+// acc -= sum_{k < kend} L_ik L_jk^T over published tiles (batched waits)
+template <typename T, int TB>
+__device__ __forceinline__ void acc_update(CholAcc<T>& acc, const T* __restrict__ tiles, int* ready, DevCtl* ctl,
+                                           int* shv, int i, int j, int kend, int NT, int lane, int wr, int wc) {
+  const size_t T2 = (size_t)TB * TB;
+  int k = 0;
+  while (k < kend) {
+    if (threadIdx.x < 64) {
+      if (lane == 0) {
+        wait_flag(ready + tile_index(i, k, NT), ctl);
+        if (i != j) wait_flag(ready + tile_index(j, k, NT), ctl);
+      }
+      bool ok = false;
+      if (k + lane < kend)
+        ok = __hip_atomic_load(ready + tile_index(i, k + lane, NT), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &&
+             (i == j ||
+              __hip_atomic_load(ready + tile_index(j, k + lane, NT), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+      const unsigned long long m = __ballot(ok);
+      const int run = ~m == 0ull ? 64 : __builtin_ctzll(~m);
+      if (lane == 0) shv[1] = k + (run > 0 ? run : 1);
+    }
+    __syncthreads();
+    const int ke = shv[1];
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    for (; k < ke; k++) {
+      const T* Ag = tiles + (size_t)tile_index(i, k, NT) * T2 + wr * 32 + (lane & 15);
+      const T* Bg = tiles + (size_t)tile_index(j, k, NT) * T2 + wc * 32 + (lane & 15);
+      const int kq = lane >> 4;
+#pragma unroll 8
+      for (int m = 0; m < TB; m += 4) {
+        const size_t off = (size_t)(m + kq) * TB;
+        const T a0 = -Ag[off], a1 = -Ag[off + 16];
+        const T b0 = Bg[off], b1 = Bg[off + 16];
+        acc.a[0][0] = Mfma<T>::fma(a0, b0, acc.a[0][0]);
+        acc.a[0][1] = Mfma<T>::fma(a0, b1, acc.a[0][1]);
+        acc.a[1][0] = Mfma<T>::fma(a1, b0, acc.a[1][0]);
+        acc.a[1][1] = Mfma<T>::fma(a1, b1, acc.a[1][1]);
+      }
+    }
+  }
+}
+
+// acc (+)= A B^T for TB x TB operands: A column-major in LDS (element type
+// T), B column-major (element type U, LDS or global); `sub` subtracts
+template <typename T, typename U, int TB>
+__device__ __forceinline__ void acc_gemm_nt(CholAcc<T>& acc, const T* A, const U* B, bool sub, int lane, int wr,
+                                            int wc) {
+  const T* As = A + wr * 32 + (lane & 15);
+  const U* Bs = B + wc * 32 + (lane & 15);
+  const int kq = lane >> 4;
+#pragma unroll 8
+  for (int m = 0; m < TB; m += 4) {
+    const int off = (m + kq) * TB;
+    const T a0 = sub ? -As[off] : As[off], a1 = sub ? -As[off + 16] : As[off + 16];
+    const T b0 = (T)Bs[off], b1 = (T)Bs[off + 16];
+    acc.a[0][0] = Mfma<T>::fma(a0, b0, acc.a[0][0]);
+    acc.a[0][1] = Mfma<T>::fma(a0, b1, acc.a[0][1]);
+    acc.a[1][0] = Mfma<T>::fma(a1, b0, acc.a[1][0]);
+    acc.a[1][1] = Mfma<T>::fma(a1, b1, acc.a[1][1]);
+  }
+}
+
+// tile copies between global memory and LDS: every load of a thread is
+// issued before the first use (the copies sit on the diagonal chain)
+template <typename T, int TB>
+__device__ __forceinline__ void lds_to_global(T* Cg, const T* R) {
+  constexpr int NQ = TB * TB / 256;
+  T v[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; q++) v[q] = R[threadIdx.x + 256 * q];
+#pragma unroll
+  for (int q = 0; q < NQ; q++) Cg[threadIdx.x + 256 * q] = v[q];
+}
+
+template <typename T, int TB>
+__device__ __forceinline__ void global_to_lds(T* R, const T* Cg) {
+  constexpr int NQ = TB * TB / 256;
+  T v[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; q++) v[q] = Cg[threadIdx.x + 256 * q];
+#pragma unroll
+  for (int q = 0; q < NQ; q++) R[threadIdx.x + 256 * q] = v[q];
+}
+
 template <typename T, int TB>
 __global__ __launch_bounds__(256) void k_chol(TileView tv, T* __restrict__ tiles, T* __restrict__ inv,
-                                              const int* __restrict__ tile_order, int* ready, int* counter,
-                                              DevCtl* ctl, int gate) {
-  static_assert(TB == 64, "the MFMA tiling below assumes 64x64 tiles and 4 waves");
+                                              const int* __restrict__ tasks, int ntasks, int* ready, int* pre_sub,
+                                              int* pre_diag, int* counter, DevCtl* ctl, int gate) {
+  static_assert(TB == 64, "the MFMA tiling assumes 64x64 tiles and 4 waves");
   if (!chol_gate(ctl, gate)) return;
-  typedef typename Mfma<T>::acc_t acc_t;
   __shared__ double lds[2 * TB * TB];
-  double* R0d = lds;                 // FP64 tile (POTRF)
+  double* R0d = lds;                 // FP64 diagonal tile (POTRF)
   double* R1d = lds + TB * TB;       // its inverse
-  T* R0 = (T*)lds;                   // C tile, then X (off-diagonal solve)
+  T* R0 = (T*)lds;                   // C / X tiles of the off-diagonal solves
   int* shv = (int*)(lds + TB * TB);  // scalars, in region 1 while it is unused
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int wr = wv & 1, wc = wv >> 1;
   const int NT = tv.NT;
   const size_t T2 = (size_t)TB * TB;
-  for (int round = 0; round < tv.ntiles; round++) {  // at most ntiles tiles per workgroup
-    if (threadIdx.x == 0) shv[0] = atomicAdd(counter, 1);
-    __syncthreads();
-    const int t = shv[0];
-    __syncthreads();
-    if (t >= tv.ntiles) break;
-    const int i = tile_order[2 * t], j = tile_order[2 * t + 1];
-    T* Cg = tiles + (size_t)t * T2;
-    acc_t acc[2][2];
-#pragma unroll
-    for (int bi = 0; bi < 2; bi++)
-#pragma unroll
-      for (int bj = 0; bj < 2; bj++)
-#pragma unroll
-        for (int vv = 0; vv < 4; vv++) {
-          const int r = wr * 32 + bi * 16 + Mfma<T>::row(vv, lane);
-          const int c = wc * 32 + bj * 16 + (lane & 15);
-          acc[bi][bj][vv] = Cg[c * TB + r];
-        }
-    int k = 0;
-    while (k < j) {
-      // wave 0: lane 0 waits for step k, then lane l checks step k + l; the
-      // batch is the run of published steps starting at k (at least one)
-      if (threadIdx.x < 64) {
-        if (lane == 0) {
-          wait_flag(ready + tile_index(i, k, NT), ctl);
-          if (i != j) wait_flag(ready + tile_index(j, k, NT), ctl);
-        }
-        bool ok = false;
-        if (k + lane < j)
-          ok = __hip_atomic_load(ready + tile_index(i, k + lane, NT), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &&
-               (i == j ||
-                __hip_atomic_load(ready + tile_index(j, k + lane, NT), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-        const unsigned long long m = __ballot(ok);
-        const int run = ~m == 0ull ? 64 : __builtin_ctzll(~m);
-        if (lane == 0) shv[1] = k + (run > 0 ? run : 1);
+  if (blockIdx.x == 0) {
+    // ---- the diagonal chain ----
+    for (int j = 0; j < NT; j++) {
+      T* Cg = tiles + (size_t)tile_index(j, j, NT) * T2;
+      if (j >= 2) {
+        if (threadIdx.x == 0) wait_flag(pre_diag + j, ctl);
+        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       }
+      CholAcc<T> acc;
+      acc_load<T, TB>(acc, Cg, lane, wr, wc);
+      if (j >= 1) acc_gemm_nt<T, T, TB>(acc, R0, R0, true, lane, wr, wc);  // - L_{j,j-1} L_{j,j-1}^T
       __syncthreads();
-      const int ke = shv[1];
-      __syncthreads();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-      for (; k < ke; k++) {
-        const T* Ag = tiles + (size_t)tile_index(i, k, NT) * T2 + wr * 32 + (lane & 15);
-        const T* Bg = tiles + (size_t)tile_index(j, k, NT) * T2 + wc * 32 + (lane & 15);
-        const int kq = lane >> 4;
-#pragma unroll 8
-        for (int m = 0; m < TB; m += 4) {
-          const size_t off = (size_t)(m + kq) * TB;
-          const T a0 = -Ag[off], a1 = -Ag[off + 16];
-          const T b0 = Bg[off], b1 = Bg[off + 16];
-          acc[0][0] = Mfma<T>::fma(a0, b0, acc[0][0]);
-          acc[0][1] = Mfma<T>::fma(a0, b1, acc[0][1]);
-          acc[1][0] = Mfma<T>::fma(a1, b0, acc[1][0]);
-          acc[1][1] = Mfma<T>::fma(a1, b1, acc[1][1]);
-        }
-      }
-    }
-    if (i == j) {
-#pragma unroll
-      for (int bi = 0; bi < 2; bi++)
-#pragma unroll
-        for (int bj = 0; bj < 2; bj++)
-#pragma unroll
-          for (int vv = 0; vv < 4; vv++) {
-            const int r = wr * 32 + bi * 16 + Mfma<T>::row(vv, lane);
-            const int c = wc * 32 + bj * 16 + (lane & 15);
-            R0d[c * TB + r] = (double)acc[bi][bj][vv];
-          }
+      acc_store<T, double, TB>(acc, R0d, lane, wr, wc);
       __syncthreads();
       potrf_lds<TB>(R0d, j * TB, tv.n, ctl);
       tri_inverse<TB>(R0d, R1d);
       T* Ig = inv + (size_t)j * T2;
+#pragma unroll 4
       for (int e = threadIdx.x; e < TB * TB; e += 256) {
         Cg[e] = (T)R0d[e];
         Ig[e] = (T)R1d[e];
       }
-    } else {
+      publish_flag(ready + tile_index(j, j, NT));
+      if (j + 1 < NT) {
+        T* Sg = tiles + (size_t)tile_index(j + 1, j, NT) * T2;
+        if (j >= 1) {
+          if (threadIdx.x == 0) wait_flag(pre_sub + j, ctl);
+          __syncthreads();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        }
+        global_to_lds<T, TB>(R0, Sg);
+        __syncthreads();
+        CholAcc<T> x;
 #pragma unroll
-      for (int bi = 0; bi < 2; bi++)
+        for (int bi = 0; bi < 2; bi++)
 #pragma unroll
-        for (int bj = 0; bj < 2; bj++)
-#pragma unroll
-          for (int vv = 0; vv < 4; vv++) {
-            const int r = wr * 32 + bi * 16 + Mfma<T>::row(vv, lane);
-            const int c = wc * 32 + bj * 16 + (lane & 15);
-            R0[c * TB + r] = acc[bi][bj][vv];
-          }
-      const long long tjj = tile_index(j, j, NT);
-      if (threadIdx.x == 0) wait_flag(ready + tjj, ctl);
+          for (int bj = 0; bj < 2; bj++) x.a[bi][bj] = typename Mfma<T>::acc_t{};
+        acc_gemm_nt<T, double, TB>(x, R0, R1d, false, lane, wr, wc);  // C L_jj^-T
+        __syncthreads();
+        acc_store<T, T, TB>(x, R0, lane, wr, wc);  // kept for step j + 1
+        __syncthreads();
+        lds_to_global<T, TB>(Sg, R0);
+        publish_flag(ready + tile_index(j + 1, j, NT));
+      }
+    }
+    return;
+  }
+  // ---- workers ----
+  for (int round = 0; round < ntasks; round++) {  // at most ntasks tasks per workgroup
+    if (threadIdx.x == 0) shv[0] = atomicAdd(counter, 1);
+    __syncthreads();
+    const int t = shv[0];
+    __syncthreads();
+    if (t >= ntasks) break;
+    const int type = tasks[3 * t], i = tasks[3 * t + 1], j = tasks[3 * t + 2];
+    if (type == CT_TILE) {
+      T* Cg = tiles + (size_t)tile_index(i, j, NT) * T2;
+      CholAcc<T> acc;
+      acc_load<T, TB>(acc, Cg, lane, wr, wc);
+      acc_update<T, TB>(acc, tiles, ready, ctl, shv, i, j, j, NT, lane, wr, wc);
+      acc_store<T, T, TB>(acc, R0, lane, wr, wc);
+      if (threadIdx.x == 0) wait_flag(ready + tile_index(j, j, NT), ctl);
       __syncthreads();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-      // X = C L_jj^-T with the inverse from the diagonal tile: A operand C
-      // from LDS, B operand Linv^T straight from global memory
-      const T* Ig = inv + (size_t)j * T2 + wc * 32 + (lane & 15);
-      const T* Cs = R0 + wr * 32 + (lane & 15);
-      const int kq = lane >> 4;
+      // X = C L_jj^-T with the inverse published by the diagonal chain
+      CholAcc<T> x;
 #pragma unroll
       for (int bi = 0; bi < 2; bi++)
 #pragma unroll
-        for (int bj = 0; bj < 2; bj++) acc[bi][bj] = acc_t{};
-#pragma unroll 8
-      for (int m = 0; m < TB; m += 4) {
-        const int off = (m + kq) * TB;
-        const T a0 = Cs[off], a1 = Cs[off + 16];
-        const T b0 = Ig[off], b1 = Ig[off + 16];
-        acc[0][0] = Mfma<T>::fma(a0, b0, acc[0][0]);
-        acc[0][1] = Mfma<T>::fma(a0, b1, acc[0][1]);
-        acc[1][0] = Mfma<T>::fma(a1, b0, acc[1][0]);
-        acc[1][1] = Mfma<T>::fma(a1, b1, acc[1][1]);
-      }
+        for (int bj = 0; bj < 2; bj++) x.a[bi][bj] = typename Mfma<T>::acc_t{};
+      acc_gemm_nt<T, T, TB>(x, R0, inv + (size_t)j * T2, false, lane, wr, wc);
       __syncthreads();
-#pragma unroll
-      for (int bi = 0; bi < 2; bi++)
-#pragma unroll
-        for (int bj = 0; bj < 2; bj++)
-#pragma unroll
-          for (int vv = 0; vv < 4; vv++) {
-            const int r = wr * 32 + bi * 16 + Mfma<T>::row(vv, lane);
-            const int c = wc * 32 + bj * 16 + (lane & 15);
-            R0[c * TB + r] = acc[bi][bj][vv];
-          }
+      acc_store<T, T, TB>(x, R0, lane, wr, wc);
       __syncthreads();
-      for (int e = threadIdx.x; e < TB * TB; e += 256) Cg[e] = R0[e];
+      lds_to_global<T, TB>(Cg, R0);
+      publish_flag(ready + tile_index(i, j, NT));
+    } else {
+      // pre-update of C_{j+1,j} (CT_PRE_SUB, i = j+1) or C_{j+1,j+1}
+      // (CT_PRE_DIAG, i = j+1) with every k < j, written back in place
+      const int jc = type == CT_PRE_SUB ? j : j + 1;
+      T* Cg = tiles + (size_t)tile_index(i, jc, NT) * T2;
+      CholAcc<T> acc;
+      acc_load<T, TB>(acc, Cg, lane, wr, wc);
+      acc_update<T, TB>(acc, tiles, ready, ctl, shv, i, jc, j, NT, lane, wr, wc);
+      acc_store<T, T, TB>(acc, R0, lane, wr, wc);
+      __syncthreads();
+      lds_to_global<T, TB>(Cg, R0);
+      publish_flag(type == CT_PRE_SUB ? pre_sub + j : pre_diag + j + 1);
     }
-    publish_flag(ready + t);
   }
 }
 
@@ -583,11 +693,11 @@ void launch_chol_reset(int* flags, int nflags, int* counters, DevCtl* ctl, int g
 }
 
 template <typename T>
-void launch_chol(const TileView& tv, T* tiles, T* inv, const int* tile_order, int* ready, int* counter, DevCtl* ctl,
-                 int gate, int max_wg, hipStream_t st) {
-  int g = tv.ntiles < max_wg ? tv.ntiles : max_wg;
-  hipLaunchKernelGGL((k_chol<T, 64>), dim3(g), dim3(256), 0, st, tv, tiles, inv, tile_order, ready, counter, ctl,
-                     gate);
+void launch_chol(const TileView& tv, T* tiles, T* inv, const int* tasks, int ntasks, int* ready, int* pre_sub,
+                 int* pre_diag, int* counter, DevCtl* ctl, int gate, int max_wg, hipStream_t st) {
+  int g = 1 + (ntasks < max_wg - 1 ? ntasks : max_wg - 1);
+  hipLaunchKernelGGL((k_chol<T, 64>), dim3(g), dim3(256), 0, st, tv, tiles, inv, tasks, ntasks, ready, pre_sub,
+                     pre_diag, counter, ctl, gate);
 }
 
 template <typename T>
@@ -608,10 +718,10 @@ void launch_trsv(const TileView& tv, const T* tiles, const T* inv, bool backward
 
 template void launch_chol_reset<double>(int*, int, int*, DevCtl*, int, hipStream_t);
 template void launch_chol_reset<float>(int*, int, int*, DevCtl*, int, hipStream_t);
-template void launch_chol<double>(const TileView&, double*, double*, const int*, int*, int*, DevCtl*, int, int,
-                                  hipStream_t);
-template void launch_chol<float>(const TileView&, float*, float*, const int*, int*, int*, DevCtl*, int, int,
-                                 hipStream_t);
+template void launch_chol<double>(const TileView&, double*, double*, const int*, int, int*, int*, int*, int*, DevCtl*,
+                                  int, int, hipStream_t);
+template void launch_chol<float>(const TileView&, float*, float*, const int*, int, int*, int*, int*, int*, DevCtl*,
+                                 int, int, hipStream_t);
 template void launch_trsv<double>(const TileView&, const double*, const double*, bool, const double*, double*,
                                   double*, int*, int*, int*, DevCtl*, int, int, hipStream_t);
 template void launch_trsv<float>(const TileView&, const float*, const float*, bool, const double*, double*, double*,
