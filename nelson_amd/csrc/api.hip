@@ -70,7 +70,7 @@ struct nsn_problem {
   Scratch scr{};
   double *cams = nullptr, *pts = nullptr, *cams_bak = nullptr, *pts_bak = nullptr;
   void *U = nullptr, *bU = nullptr, *V = nullptr, *bV = nullptr, *W = nullptr, *Vinv = nullptr, *Y = nullptr,
-       *tW = nullptr, *Sd = nullptr, *bS = nullptr, *tiles = nullptr, *inv = nullptr;
+       *tW = nullptr, *Sd = nullptr, *bS = nullptr, *tiles = nullptr, *inv = nullptr, *presub = nullptr;
   double *xU = nullptr, *xV = nullptr;
   int* chol_tasks = nullptr;  // worker task list of the factorisation
   int n_chol_tasks = 0;
@@ -168,8 +168,8 @@ void seq_schur(nsn_problem* h, int gate, int checks, bool timed) {
   launch_chol_reset<T>(h->flags, h->nflags, h->counters, h->ctl, gate, st);
   {
     int* pre_sub = h->flags + h->tv.ntiles + h->NR + h->NR * h->NR;
-    launch_chol<T>(h->tv, (T*)h->tiles, (T*)h->inv, h->chol_tasks, h->n_chol_tasks, h->flags, pre_sub,
-                   pre_sub + h->tv.NT, h->counters, h->ctl, gate, kCholMaxWG, st);
+    launch_chol<T>(h->tv, (T*)h->tiles, (T*)h->inv, (T*)h->presub, h->chol_tasks, h->n_chol_tasks, h->flags,
+                   pre_sub, pre_sub + h->tv.NT, h->counters, h->ctl, gate, kCholMaxWG, st);
   }
   if (timed) (void)hipEventRecord(h->ev[3], st);
   launch_trsv<T>(h->tv, (const T*)h->tiles, (const T*)h->inv, true, nullptr, h->xU, h->parts,
@@ -336,6 +336,7 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
   h->bS = a_T(6 * (size_t)NU);
   h->tiles = a_T((size_t)tv.ntiles * kTB * kTB);
   h->inv = a_T((size_t)tv.NT * kTB * kTB);
+  h->presub = a_T((size_t)tv.NT * kTB * kTB);
   h->xU = a_dbl(6 * (size_t)NU);
   h->parts = a_dbl((size_t)h->NR * h->NR * kTB);
   h->rU = a_dbl(6 * (size_t)NU);
@@ -368,7 +369,7 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
   h->ctl = m.alloc<DevCtl>(1);
   h->log_cap = 4096;
   h->log = m.alloc<DevLog>(h->log_cap);
-  if (!h->log || !h->ctl || !h->tiles || !h->inv || !h->W || !h->Y || !h->sb_pairs ||
+  if (!h->log || !h->ctl || !h->tiles || !h->inv || !h->presub || !h->W || !h->Y || !h->sb_pairs ||
       !h->zU)
     return fail(NSN_ENOMEM, "device allocation failed (%.1f MB of tiles)",
                 (double)tv.ntiles * kTB * kTB * es / 1e6);

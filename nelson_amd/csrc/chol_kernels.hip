@@ -158,53 +158,56 @@ __device__ __forceinline__ void potrf_panel(double* A, int g0, int n, int& bad) 
   if (lane >= C0 && lane < C0 + kPW) A[rdiag_slot<TB>(lane)] = myr;
 }
 
-// D[r, c] -= sum_{m < K} Lft[r][k0+m] Rgt[c][k0+m] for rows [r0, TB),
-// columns [c0, TB), K = kPW, with the 16x16x4 MFMA on the four waves; with
-// `lower` only the blocks on or below the diagonal (c block <= r block).
-template <typename T, int TB>
-__device__ __forceinline__ void mfma_update(T* D, const T* Lft, const T* Rgt, int r0, int c0, int k0, bool lower) {
-  typedef typename Mfma<T>::acc_t acc_t;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int nbr = (TB - r0) / 16, nbc = (TB - c0) / 16;
-  for (int blk = wv; blk < nbr * nbc; blk += 4) {
-    const int bi = blk % nbr, bj = blk / nbr;
-    const int rr = r0 + bi * 16, cc = c0 + bj * 16;
-    if (lower && cc > rr) continue;
-    acc_t acc;
+// D_(rb, cb) -= A_(rb, p) A_(cb, p)^T for one 16x16 block of the FP64 tile
+// A in LDS (column-major), with the 16 columns of panel p; one wave, four
+// 16x16x4 MFMAs
+template <int TB>
+__device__ __forceinline__ void block_update(double* A, int rb, int cb, int k0) {
+  typedef Mfma<double>::acc_t acc_t;
+  const int lane = threadIdx.x & 63;
+  acc_t acc;
 #pragma unroll
-    for (int vv = 0; vv < 4; vv++) acc[vv] = D[(cc + (lane & 15)) * TB + rr + Mfma<T>::row(vv, lane)];
+  for (int vv = 0; vv < 4; vv++) acc[vv] = A[(16 * cb + (lane & 15)) * TB + 16 * rb + Mfma<double>::row(vv, lane)];
 #pragma unroll
-    for (int m = 0; m < kPW; m += 4) {
-      const int mm = k0 + m + (lane >> 4);
-      const T a = -Lft[mm * TB + rr + (lane & 15)];
-      const T b = Rgt[mm * TB + cc + (lane & 15)];
-      acc = Mfma<T>::fma(a, b, acc);
-    }
-#pragma unroll
-    for (int vv = 0; vv < 4; vv++) D[(cc + (lane & 15)) * TB + rr + Mfma<T>::row(vv, lane)] = acc[vv];
+  for (int m = 0; m < kPW; m += 4) {
+    const int mm = k0 + m + (lane >> 4);
+    acc = Mfma<double>::fma(-A[mm * TB + 16 * rb + (lane & 15)], A[mm * TB + 16 * cb + (lane & 15)], acc);
   }
+#pragma unroll
+  for (int vv = 0; vv < 4; vv++) A[(16 * cb + (lane & 15)) * TB + 16 * rb + Mfma<double>::row(vv, lane)] = acc[vv];
 }
 
 // Cholesky of a 64x64 FP64 tile in LDS (lower triangle written, the
-// reciprocals of its diagonal in the upper triangle).
-template <int TB, int KB>
-__device__ __forceinline__ void potrf_step(double* A, int g0, int n, int& bad) {
-  if (threadIdx.x < 64) potrf_panel<TB, KB>(A, g0, n, bad);
+// reciprocals of its diagonal in the upper triangle), four 16-column panels
+// with look-ahead: after panel p the four waves first update block column
+// p+1 (what panel p+1 needs), then wave 0 factorises panel p+1 while waves
+// 1-3 apply panel p to the block columns beyond it.
+template <int TB, int P>
+__device__ __forceinline__ void potrf_next(double* A, int g0, int n, int& bad) {
+  const int wv = threadIdx.x >> 6;
+  if (wv < 3 - P) block_update<TB>(A, P + 1 + wv, P + 1, kPW * P);
   __syncthreads();
-  if constexpr (KB + kPW < TB) {
-    mfma_update<double, TB>(A, A, A, KB + kPW, KB + kPW, KB, true);
-    __syncthreads();
+  if (wv == 0) {
+    potrf_panel<TB, kPW * (P + 1)>(A, g0, n, bad);
+  } else if (P == 0) {
+    // blocks (2,2), (3,2), (3,3) with panel 0
+    const int rb = wv == 3 ? 3 : wv + 1, cb = wv == 1 ? 2 : (wv == 2 ? 2 : 3);
+    block_update<TB>(A, rb, cb, 0);
+  } else if (P == 1 && wv == 1) {
+    block_update<TB>(A, 3, 3, kPW);
   }
+  __syncthreads();
 }
 
 template <int TB>
 __device__ __forceinline__ void potrf_lds(double* A, int g0, int n, DevCtl* ctl) {
   static_assert(TB == 4 * kPW, "four panels");
   int bad = 0;
-  potrf_step<TB, 0>(A, g0, n, bad);
-  potrf_step<TB, kPW>(A, g0, n, bad);
-  potrf_step<TB, 2 * kPW>(A, g0, n, bad);
-  potrf_step<TB, 3 * kPW>(A, g0, n, bad);
+  if (threadIdx.x < 64) potrf_panel<TB, 0>(A, g0, n, bad);
+  __syncthreads();
+  potrf_next<TB, 0>(A, g0, n, bad);
+  potrf_next<TB, 1>(A, g0, n, bad);
+  potrf_next<TB, 2>(A, g0, n, bad);
   if (threadIdx.x == 0 && bad) atomicAdd(&ctl->fail_s, 1);
 }
 
@@ -285,10 +288,13 @@ __device__ __forceinline__ void tri_inverse(const double* L, double* Li) {
 //
 // The other workgroups take tasks from an atomic counter, grouped in slots
 // j = 0, 1, ...:
-//   slot j: pre-update of C_{j+1,j} and of C_{j+1,j+1} with every k < j
-//           (for j >= 1; written back in place, then flagged), followed by
-//           the tiles (i, j), i >= j+2: C_ij -= sum_{k<j} L_ik L_jk^T, then
-//           L_ij = C_ij L_jj^-T with the inverse published by workgroup 0.
+//   slot j: pre-update of C_{j+1,j} with every k < j into the scratch tile
+//           presub[j], and of C_{j+1,j+1} (j >= 1) in place, each flagged;
+//           L_{j+1,j} = C_{j+1,j} L_jj^-T into the tile storage for the
+//           other tasks (the chain keeps its own copy and never publishes
+//           it); then the tiles (i, j), i >= j+2: C_ij -= sum_{k<j} L_ik
+//           L_jk^T, L_ij = C_ij L_jj^-T with the inverse published by
+//           workgroup 0.
 // A task of slot j only needs tiles of slots < j, the diagonal chain up to
 // step j, and (pre-updates) sub-diagonal tiles of steps < j; step j of the
 // chain needs only slots < j and the pre-update of C_{j+1,j} in slot j,
@@ -303,7 +309,7 @@ __device__ __forceinline__ void tri_inverse(const double* L, double* Li) {
 // wave 0 checks step k + l, one acquire fence per run of published steps).
 // LDS: 64 KiB = two TB x TB FP64 regions.
 // ---------------------------------------------------------------------------
-enum CholTask : int { CT_TILE = 0, CT_PRE_SUB = 1, CT_PRE_DIAG = 2 };
+enum CholTask : int { CT_TILE = 0, CT_PRE_SUB = 1, CT_PRE_DIAG = 2, CT_SUB = 3 };
 
 template <typename T>
 struct CholAcc {
@@ -427,8 +433,9 @@ __device__ __forceinline__ void global_to_lds(T* R, const T* Cg) {
 
 template <typename T, int TB>
 __global__ __launch_bounds__(256) void k_chol(TileView tv, T* __restrict__ tiles, T* __restrict__ inv,
-                                              const int* __restrict__ tasks, int ntasks, int* ready, int* pre_sub,
-                                              int* pre_diag, int* counter, DevCtl* ctl, int gate) {
+                                              T* __restrict__ presub, const int* __restrict__ tasks, int ntasks,
+                                              int* ready, int* pre_sub, int* pre_diag, int* counter, DevCtl* ctl,
+                                              int gate) {
   static_assert(TB == 64, "the MFMA tiling assumes 64x64 tiles and 4 waves");
   if (!chol_gate(ctl, gate)) return;
   __shared__ double lds[2 * TB * TB];
@@ -442,15 +449,10 @@ __global__ __launch_bounds__(256) void k_chol(TileView tv, T* __restrict__ tiles
   const size_t T2 = (size_t)TB * TB;
   if (blockIdx.x == 0) {
     // ---- the diagonal chain ----
+    CholAcc<T> acc;  // the next diagonal tile, loaded one step ahead
+    acc_load<T, TB>(acc, tiles, lane, wr, wc);
     for (int j = 0; j < NT; j++) {
       T* Cg = tiles + (size_t)tile_index(j, j, NT) * T2;
-      if (j >= 2) {
-        if (threadIdx.x == 0) wait_flag(pre_diag + j, ctl);
-        __syncthreads();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-      }
-      CholAcc<T> acc;
-      acc_load<T, TB>(acc, Cg, lane, wr, wc);
       if (j >= 1) acc_gemm_nt<T, T, TB>(acc, R0, R0, true, lane, wr, wc);  // - L_{j,j-1} L_{j,j-1}^T
       __syncthreads();
       acc_store<T, double, TB>(acc, R0d, lane, wr, wc);
@@ -465,13 +467,17 @@ __global__ __launch_bounds__(256) void k_chol(TileView tv, T* __restrict__ tiles
       }
       publish_flag(ready + tile_index(j, j, NT));
       if (j + 1 < NT) {
-        T* Sg = tiles + (size_t)tile_index(j + 1, j, NT) * T2;
-        if (j >= 1) {
-          if (threadIdx.x == 0) wait_flag(pre_sub + j, ctl);
-          __syncthreads();
-          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        // the pre-updated C_{j+1,j+1} (in flight during the product below)
+        if (threadIdx.x == 0) {
+          if (j + 1 >= 2) wait_flag(pre_diag + j + 1, ctl);
+          wait_flag(pre_sub + j, ctl);
         }
-        global_to_lds<T, TB>(R0, Sg);
+        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        acc_load<T, TB>(acc, tiles + (size_t)tile_index(j + 1, j + 1, NT) * T2, lane, wr, wc);
+        // L_{j+1,j} for the next step, from the pre-updated C_{j+1,j}; a
+        // worker forms the same product for the tile storage (CT_SUB)
+        global_to_lds<T, TB>(R0, presub + (size_t)j * T2);
         __syncthreads();
         CholAcc<T> x;
 #pragma unroll
@@ -482,8 +488,6 @@ __global__ __launch_bounds__(256) void k_chol(TileView tv, T* __restrict__ tiles
         __syncthreads();
         acc_store<T, T, TB>(x, R0, lane, wr, wc);  // kept for step j + 1
         __syncthreads();
-        lds_to_global<T, TB>(Sg, R0);
-        publish_flag(ready + tile_index(j + 1, j, NT));
       }
     }
     return;
@@ -517,17 +521,39 @@ __global__ __launch_bounds__(256) void k_chol(TileView tv, T* __restrict__ tiles
       __syncthreads();
       lds_to_global<T, TB>(Cg, R0);
       publish_flag(ready + tile_index(i, j, NT));
+    } else if (type == CT_SUB) {
+      // L_{j+1,j} = C_{j+1,j} L_jj^-T into the tile storage (the diagonal
+      // chain keeps its own copy)
+      if (threadIdx.x == 0) {
+        wait_flag(pre_sub + j, ctl);
+        wait_flag(ready + tile_index(j, j, NT), ctl);
+      }
+      __syncthreads();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      global_to_lds<T, TB>(R0, presub + (size_t)j * T2);
+      __syncthreads();
+      CholAcc<T> x;
+#pragma unroll
+      for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+        for (int bj = 0; bj < 2; bj++) x.a[bi][bj] = typename Mfma<T>::acc_t{};
+      acc_gemm_nt<T, T, TB>(x, R0, inv + (size_t)j * T2, false, lane, wr, wc);
+      __syncthreads();
+      acc_store<T, T, TB>(x, R0, lane, wr, wc);
+      __syncthreads();
+      lds_to_global<T, TB>(tiles + (size_t)tile_index(i, j, NT) * T2, R0);
+      publish_flag(ready + tile_index(i, j, NT));
     } else {
-      // pre-update of C_{j+1,j} (CT_PRE_SUB, i = j+1) or C_{j+1,j+1}
-      // (CT_PRE_DIAG, i = j+1) with every k < j, written back in place
+      // pre-update with every k < j of C_{j+1,j} (CT_PRE_SUB, into the
+      // scratch tile presub[j]) or of C_{j+1,j+1} (CT_PRE_DIAG, in place)
       const int jc = type == CT_PRE_SUB ? j : j + 1;
-      T* Cg = tiles + (size_t)tile_index(i, jc, NT) * T2;
       CholAcc<T> acc;
-      acc_load<T, TB>(acc, Cg, lane, wr, wc);
+      acc_load<T, TB>(acc, tiles + (size_t)tile_index(i, jc, NT) * T2, lane, wr, wc);
       acc_update<T, TB>(acc, tiles, ready, ctl, shv, i, jc, j, NT, lane, wr, wc);
       acc_store<T, T, TB>(acc, R0, lane, wr, wc);
       __syncthreads();
-      lds_to_global<T, TB>(Cg, R0);
+      lds_to_global<T, TB>(type == CT_PRE_SUB ? presub + (size_t)j * T2 : tiles + (size_t)tile_index(i, jc, NT) * T2,
+                           R0);
       publish_flag(type == CT_PRE_SUB ? pre_sub + j : pre_diag + j + 1);
     }
   }
@@ -693,11 +719,11 @@ void launch_chol_reset(int* flags, int nflags, int* counters, DevCtl* ctl, int g
 }
 
 template <typename T>
-void launch_chol(const TileView& tv, T* tiles, T* inv, const int* tasks, int ntasks, int* ready, int* pre_sub,
-                 int* pre_diag, int* counter, DevCtl* ctl, int gate, int max_wg, hipStream_t st) {
+void launch_chol(const TileView& tv, T* tiles, T* inv, T* presub, const int* tasks, int ntasks, int* ready,
+                 int* pre_sub, int* pre_diag, int* counter, DevCtl* ctl, int gate, int max_wg, hipStream_t st) {
   int g = 1 + (ntasks < max_wg - 1 ? ntasks : max_wg - 1);
-  hipLaunchKernelGGL((k_chol<T, 64>), dim3(g), dim3(256), 0, st, tv, tiles, inv, tasks, ntasks, ready, pre_sub,
-                     pre_diag, counter, ctl, gate);
+  hipLaunchKernelGGL((k_chol<T, 64>), dim3(g), dim3(256), 0, st, tv, tiles, inv, presub, tasks, ntasks, ready,
+                     pre_sub, pre_diag, counter, ctl, gate);
 }
 
 template <typename T>
@@ -718,10 +744,10 @@ void launch_trsv(const TileView& tv, const T* tiles, const T* inv, bool backward
 
 template void launch_chol_reset<double>(int*, int, int*, DevCtl*, int, hipStream_t);
 template void launch_chol_reset<float>(int*, int, int*, DevCtl*, int, hipStream_t);
-template void launch_chol<double>(const TileView&, double*, double*, const int*, int, int*, int*, int*, int*, DevCtl*,
-                                  int, int, hipStream_t);
-template void launch_chol<float>(const TileView&, float*, float*, const int*, int, int*, int*, int*, int*, DevCtl*,
-                                 int, int, hipStream_t);
+template void launch_chol<double>(const TileView&, double*, double*, double*, const int*, int, int*, int*, int*,
+                                  int*, DevCtl*, int, int, hipStream_t);
+template void launch_chol<float>(const TileView&, float*, float*, float*, const int*, int, int*, int*, int*, int*,
+                                 DevCtl*, int, int, hipStream_t);
 template void launch_trsv<double>(const TileView&, const double*, const double*, bool, const double*, double*,
                                   double*, int*, int*, int*, DevCtl*, int, int, hipStream_t);
 template void launch_trsv<float>(const TileView&, const float*, const float*, bool, const double*, double*, double*,

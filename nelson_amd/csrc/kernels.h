@@ -133,26 +133,33 @@ void launch_zero(T* p, long long n, hipStream_t st);
 // ---- dense tile Cholesky and triangular solve (chol_kernels.hip) --------
 template <typename T>
 void launch_chol_reset(int* flags, int nflags, int* counters, DevCtl* ctl, int gate, hipStream_t st);
-// factor in place; inv[j] receives L_jj^-1 (TB x TB, column-major).  tasks:
-// the worker task list (3 ints per task: kind, i, j; chol_task_list);
-// ready: ntiles flags, pre_sub / pre_diag: NT flags each; counter zeroed
+// factor in place; inv[j] receives L_jj^-1 (TB x TB, column-major); presub:
+// NT scratch tiles.  tasks: the worker task list (3 ints per task: kind, i,
+// j; chol_task_list); ready: ntiles flags, pre_sub / pre_diag: NT flags
+// each; counter zeroed
 template <typename T>
-void launch_chol(const TileView& tv, T* tiles, T* inv, const int* tasks, int ntasks, int* ready, int* pre_sub,
-                 int* pre_diag, int* counter, DevCtl* ctl, int gate, int max_wg, hipStream_t st);
-// the worker task list of the factorisation, in queue order
+void launch_chol(const TileView& tv, T* tiles, T* inv, T* presub, const int* tasks, int ntasks, int* ready,
+                 int* pre_sub, int* pre_diag, int* counter, DevCtl* ctl, int gate, int max_wg, hipStream_t st);
+// the worker task list of the factorisation, in queue order (kinds: 0 tile
+// (i >= j+2), 1 pre-update of (j+1, j), 2 pre-update of (j+1, j+1), 3 the
+// sub-diagonal tile (j+1, j))
 inline void chol_task_list(int NT, int* out, int* ntasks) {
   int t = 0;
+  auto add = [&](int kind, int i, int j) {
+    if (out) {
+      out[3 * t] = kind;
+      out[3 * t + 1] = i;
+      out[3 * t + 2] = j;
+    }
+    t++;
+  };
   for (int j = 0; j < NT; j++) {
-    if (j >= 1 && j + 1 < NT) {
-      if (out) { out[3 * t] = 1; out[3 * t + 1] = j + 1; out[3 * t + 2] = j; }
-      t++;
-      if (out) { out[3 * t] = 2; out[3 * t + 1] = j + 1; out[3 * t + 2] = j; }
-      t++;
+    if (j + 1 < NT) {
+      add(1, j + 1, j);
+      if (j >= 1) add(2, j + 1, j);
+      add(3, j + 1, j);
     }
-    for (int i = j + 2; i < NT; i++) {
-      if (out) { out[3 * t] = 0; out[3 * t + 1] = i; out[3 * t + 2] = j; }
-      t++;
-    }
+    for (int i = j + 2; i < NT; i++) add(0, i, j);
   }
   *ntasks = t;
 }
