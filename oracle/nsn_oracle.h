@@ -51,6 +51,7 @@ void or_ldlt_solve(const double* a, int n, const int* transp, double* x);
  * on factorisation failure (increment zeroed, SolverCholeskySchur.hpp:151-154).
  * work/phase outputs: if S_out != NULL it receives the damped Schur matrix
  * (nu x nu row-major upper), if bS_out != NULL the reduced rhs.            */
+int or_schur_assemble(const or_hessian* h, double rel, double abs_, double* S_out, double* bS_out);
 int or_schur_step(const or_hessian* h, double rel, double abs_, double* xU, double* xV,
                   double* S_out, double* bS_out);
 

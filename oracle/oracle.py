@@ -80,6 +80,7 @@ def lib():
             "or_ldlt_factor": (C.c_int, [_D, C.c_int, _I]),
             "or_ldlt_solve": (None, [_D, C.c_int, _I, _D]),
             "or_schur_step": (C.c_int, [C.POINTER(Hessian), C.c_double, C.c_double, _D, _D, _D, _D]),
+            "or_schur_assemble": (C.c_int, [C.POINTER(Hessian), C.c_double, C.c_double, _D, _D]),
             "or_max_abs_h_diag": (C.c_double, [C.POINTER(Hessian)]),
             "or_rho": (C.c_double, [C.POINTER(Hessian), C.c_double, _D, _D, C.c_double, C.c_double]),
             "or_diag_inv": (C.c_int, [C.c_int, C.c_int, _D, C.c_double, C.c_double, _D]),
@@ -254,6 +255,15 @@ class OracleBA:
             out["S"] = S
             out["bS"] = bS
         return out
+
+    def schur_assemble(self, rel, abs_):
+        """The damped reduced system (S, bS) without factorising it."""
+        h = lib().or_ba_hessian(self._h)
+        nu = 6 * self.NU
+        S = np.zeros((nu, nu))
+        bS = np.zeros(max(nu, 1))
+        ok = lib().or_schur_assemble(h, rel, abs_, dp(S), dp(bS))
+        return dict(ok=bool(ok), S=S, bS=bS[:nu])
 
     def oplus(self, xU, xV):
         lib().or_ba_oplus(self._h, dp(np.ascontiguousarray(xU)), dp(np.ascontiguousarray(xV)))
