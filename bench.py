@@ -204,8 +204,16 @@ def main():
     flops = n ** 3 / 3.0
     chol_s = ph["factorize"] / 1e3
     achieved = flops / chol_s / 1e12
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic_k_chol_cfg2_fp64.json")
+    if args.config == "cfg2" and os.path.exists(tfile):
+        with open(tfile) as f:
+            t = json.load(f)
+        traffic = t["hbm_read_bytes_per_launch"] + t["hbm_written_bytes_per_launch"]
     roofline = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
+                "traffic_note": "HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, copy-calibrated), from "
+                                "profiles/traffic_k_chol_cfg2_fp64.json",
                 "kernel": "k_chol<double,64> (tile Cholesky of S, n=%d, n^3/3 flop)" % n,
                 "kernel_ms": ph["factorize"]}
     mixed = None
