@@ -243,13 +243,14 @@ int nsn_batch_lm_solve(nsn_batch* b, const nsn_lm_settings* s, double* chi2_out,
 /* per-window termination reason (NSN_* reasons), outer iterations and Schur
  * solves of the last nsn_batch_lm_solve; any array may be NULL */
 int nsn_batch_results(nsn_batch* b, int* reasons, int* iterations, int* sub_iterations);
-/* autotuner: times every launch shape (Cholesky panel width 4 / 8 / 16 x
- * 256 / 512 / 1024 persistent workgroups) for up to 2 LM iterations on this
- * batch (state restored after each) and keeps the fastest */
-int nsn_batch_tune(nsn_batch* b, const nsn_lm_settings* s, int* panel_width, int* grid);
-int nsn_batch_set_config(nsn_batch* b, int panel_width, int grid);
+/* autotuner: times every launch shape -- factorisation variant (8 or 16:
+ * column panels of that width; 64: 64x64 tiles with MFMA products) x 256 /
+ * 512 / 1024 persistent workgroups -- for up to 2 LM iterations on this batch
+ * (state restored after each) and keeps the fastest */
+int nsn_batch_tune(nsn_batch* b, const nsn_lm_settings* s, int* variant, int* grid);
+int nsn_batch_set_config(nsn_batch* b, int variant, int grid);
 /* the launch shape in use and the tuner's timings (ms, in candidate order) */
-int nsn_batch_config(nsn_batch* b, int* panel_width, int* grid, double* tune_ms, int n_tune);
+int nsn_batch_config(nsn_batch* b, int* variant, int* grid, double* tune_ms, int n_tune);
 
 #ifdef __cplusplus
 }

@@ -29,7 +29,7 @@ using namespace nsn;
 namespace {
 
 constexpr int kMaxWinS = 256;  // largest S order of a window (6 variable cameras each)
-const int kNbCand[3] = {4, 8, 16};
+const int kNbCand[3] = {8, 16, 64};
 const int kGridCand[3] = {256, 512, 1024};
 
 }  // namespace
@@ -109,7 +109,7 @@ int save_state(nsn_batch* b, bool restore) {
   return NSN_OK;
 }
 
-// times every (panel width, grid) candidate on the batch itself for a few LM
+// times every (factorisation variant, grid) candidate on the batch itself for a few LM
 // iterations from the current state (restored after each run) and keeps the
 // fastest
 int autotune(nsn_batch* b, const nsn_lm_settings* s) {
@@ -177,7 +177,7 @@ extern "C" int nsn_batch_create(const nsn_problem_desc* wd, int n_windows, nsn_b
     size_t cam_var, pt_var, vcu, vpu, pt_ptr, pe_cam, pe_wblk, pe_edge, cam_ptr, ce_pt, wptr, w_cam, w_pt, cw_ptr,
         cw_idx, sb_i, sb_j, sb_ptr, sb_pairs;
     size_t pe_obs, pe_w, ce_obs, ce_w, xU, xV;
-    size_t U, bU, V, bV, W, Vinv, Y, tW, S;
+    size_t U, bU, V, bV, W, Vinv, Y, tW, S, Linv;
     size_t cams, pts;
   };
   std::vector<WinOff> wo(n_windows);
@@ -212,13 +212,14 @@ extern "C" int nsn_batch_create(const nsn_problem_desc* wd, int n_windows, nsn_b
     o.cw_ptr = push_i(s.cw_ptr); o.cw_idx = push_i(s.cw_idx);
     o.sb_i = push_i(s.sb_i); o.sb_j = push_i(s.sb_j); o.sb_ptr = push_i(s.sb_ptr); o.sb_pairs = push_i(s.sb_pairs);
     o.pe_obs = push_d(s.pe_obs); o.pe_w = push_d(s.pe_w); o.ce_obs = push_d(s.ce_obs); o.ce_w = push_d(s.ce_w);
-    const int n = 6 * s.NU;
-    o.xU = take_d(n); o.xV = take_d(3 * (size_t)s.NV);
+    const int n = 6 * s.NU, nP = (n + 63) / 64 * 64;
+    o.xU = take_d(nP); o.xV = take_d(3 * (size_t)s.NV);
     o.U = take_t(21 * (size_t)s.NU); o.bU = take_t(6 * (size_t)s.NU);
     o.V = take_t(6 * (size_t)s.NV); o.bV = take_t(3 * (size_t)s.NV);
     o.W = take_t(18 * (size_t)s.nW); o.Vinv = take_t(6 * (size_t)s.NV);
     o.Y = take_t(18 * (size_t)s.nW); o.tW = take_t(6 * (size_t)s.nW);
-    o.S = take_t((size_t)n * n);
+    o.S = take_t((size_t)nP * nP);
+    o.Linv = take_t((size_t)(nP / 64) * 4096);
     o.cams = 12 * (size_t)b->totC;
     o.pts = 3 * (size_t)b->totP;
     b->totC += s.C;
@@ -262,11 +263,13 @@ extern "C" int nsn_batch_create(const nsn_problem_desc* wd, int n_windows, nsn_b
     v.cw_idx = di + o.cw_idx;
     v.cams = b->cams + o.cams; v.pts = b->pts + o.pts;
     x.n = 6 * s.NU;
+    x.nP = (x.n + 63) / 64 * 64;
     x.nsb = s.nsb;
     x.sb_i = di + o.sb_i; x.sb_j = di + o.sb_j; x.sb_ptr = di + o.sb_ptr; x.sb_pairs = di + o.sb_pairs;
     x.U = dt + es * o.U; x.bU = dt + es * o.bU; x.V = dt + es * o.V; x.bV = dt + es * o.bV;
     x.W = dt + es * o.W; x.Vinv = dt + es * o.Vinv; x.Y = dt + es * o.Y; x.tW = dt + es * o.tW;
     x.S = dt + es * o.S;
+    x.Linv = dt + es * o.Linv;
     x.xU = dd + o.xU; x.xV = dd + o.xV;
     x.cams_bak = bak_c + o.cams; x.pts_bak = bak_p + o.pts;
   }
@@ -325,8 +328,8 @@ extern "C" int nsn_batch_tune(nsn_batch* b, const nsn_lm_settings* s, int* nb, i
 
 extern "C" int nsn_batch_set_config(nsn_batch* b, int nb, int grid) {
   if (!b) return fail(NSN_EINVAL, "null argument");
-  if ((nb != 4 && nb != 8 && nb != 16) || grid < 1)
-    return fail(NSN_EINVAL, "nsn_batch_set_config: panel width 4, 8 or 16 and grid >= 1");
+  if ((nb != 8 && nb != 16 && nb != 64) || grid < 1)
+    return fail(NSN_EINVAL, "nsn_batch_set_config: variant 8, 16 (panel width) or 64 (tiles) and grid >= 1");
   b->nb = nb;
   b->grid = grid;
   b->tuned = true;

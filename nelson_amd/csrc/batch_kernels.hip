@@ -3,16 +3,21 @@
 // Levenberg-Marquardt solve of a window (nelson/LevenbergMarquardt.hpp:35-146
 // with the Schur step of nelson/SolverCholeskySchur.hpp:91-157) on its own:
 // linearisation, V^-1 / W V^-1, S assembly from the (W, W) pair lists, a
-// dense blocked Cholesky of S (n <= 256) whose panels are factorised in FP64
-// in LDS, the two triangular solves, back-substitution, oplus, cost and the
-// accept / reject schedule.  Windows never wait for each other, so the launch
-// needs no inter-workgroup synchronisation; every loop is bounded.
+// dense Cholesky of S (n <= 256, padded to a multiple of 64) with FP64 pivot
+// blocks, the two triangular solves, back-substitution, oplus, cost and the
+// accept / reject schedule.  Two factorisation variants, picked per batch by
+// the autotuner: 64x64 tiles (the POTRF / inverse / MFMA tile routines of the
+// large-problem factorisation, solves as products with the diagonal-tile
+// inverses) or NB-column panels factorised in LDS with column sweeps.
+// Windows never wait for each other, so the launch needs no inter-workgroup
+// synchronisation; every loop is bounded.
 // The per-edge, per-point and LM-control arithmetic is shared with the
 // single-problem kernels (ba_device.h).
 #include <hip/hip_runtime.h>
 
 #include "ba_device.h"
 #include "batch.h"
+#include "tile_device.h"
 
 namespace nsn {
 
@@ -118,18 +123,19 @@ __device__ void win_copy(const BAView& v, double* cd, double* pd, const double* 
   for (int i = threadIdx.x; i < 3 * v.P; i += kBT) pd[i] = ps[i];
 }
 
-// dense Cholesky of the n x n lower S (column-major) in place: panels of NB
-// columns are factorised in FP64 in LDS (pan: NB columns of kMaxS rows), the
-// trailing matrix is updated from the panel; false if a pivot is not positive
+// panel variant: Cholesky of the n x n lower S (column-major, leading
+// dimension ld) in place; panels of NB columns are factorised in FP64 in LDS
+// (pan: NB columns of kMaxS rows), the trailing matrix is updated from the
+// panel; false if a pivot is not positive
 template <typename T, int NB>
-__device__ bool win_cholesky(T* S, int n, double* pan) {
+__device__ bool win_cholesky_panels(T* S, int n, int ld, double* pan) {
   const int tid = threadIdx.x;
   bool ok = true;
   for (int k0 = 0; k0 < n; k0 += NB) {
     const int nb = min(NB, n - k0), m = n - k0;
     for (int e = tid; e < nb * m; e += kBT) {
       const int cc = e / m, r = e % m;
-      pan[cc * kMaxS + r] = (double)S[(size_t)(k0 + cc) * n + k0 + r];
+      pan[cc * kMaxS + r] = (double)S[(size_t)(k0 + cc) * ld + k0 + r];
     }
     __syncthreads();
     for (int cc = 0; cc < nb; cc++) {
@@ -150,7 +156,7 @@ __device__ bool win_cholesky(T* S, int n, double* pan) {
     }
     for (int e = tid; e < nb * m; e += kBT) {
       const int cc = e / m, r = e % m;
-      if (r >= cc) S[(size_t)(k0 + cc) * n + k0 + r] = (T)pan[cc * kMaxS + r];
+      if (r >= cc) S[(size_t)(k0 + cc) * ld + k0 + r] = (T)pan[cc * kMaxS + r];
     }
     const int mt = m - nb;
     for (int e = tid; e < mt * mt; e += kBT) {
@@ -159,7 +165,7 @@ __device__ bool win_cholesky(T* S, int n, double* pan) {
         double s = 0.0;
 #pragma unroll 4
         for (int cc = 0; cc < nb; cc++) s += pan[cc * kMaxS + nb + ii] * pan[cc * kMaxS + nb + jj];
-        T* a = S + (size_t)(k0 + nb + jj) * n + k0 + nb + ii;
+        T* a = S + (size_t)(k0 + nb + jj) * ld + k0 + nb + ii;
         *a = (T)((double)*a - s);
       }
     }
@@ -168,13 +174,213 @@ __device__ bool win_cholesky(T* S, int n, double* pan) {
   return ok;
 }
 
+// column sweeps L y = b, then L^T x = y, on the panel variant's factor; b in
+// yv (LDS), x to xU
+template <typename T>
+__device__ void win_solve_sweeps(const T* S, int n, int ld, double* yv, double* ys, double* xU) {
+  const int tid = threadIdx.x;
+  for (int cc = 0; cc < n; cc++) {
+    const double yc = yv[cc] / (double)S[(size_t)cc * ld + cc];
+    if (tid == 0) ys[cc] = yc;
+    if (tid > cc && tid < n) yv[tid] -= (double)S[(size_t)cc * ld + tid] * yc;
+    __syncthreads();
+  }
+  for (int cc = n - 1; cc >= 0; cc--) {
+    const double xc = ys[cc] / (double)S[(size_t)cc * ld + cc];
+    if (tid == 0) xU[cc] = xc;
+    if (tid < cc) ys[tid] -= (double)S[(size_t)tid * ld + cc] * xc;
+    __syncthreads();
+  }
+}
+
+// ---- tile variant -----------------------------------------------------------
+// 64x64 tile (ti, tj) of the padded S (column-major, leading dimension ld)
+template <typename T>
+__device__ __forceinline__ T* tile_at(T* S, int ld, int ti, int tj) {
+  return S + (size_t)(64 * tj) * ld + 64 * ti;
+}
+
+template <typename T, typename U>
+__device__ __forceinline__ void tile_to_lds(U* R, const T* A, int ld) {
+  T v[16];
+#pragma unroll
+  for (int q = 0; q < 16; q++) {
+    const int e = threadIdx.x + 256 * q;
+    v[q] = A[(size_t)(e >> 6) * ld + (e & 63)];
+  }
+#pragma unroll
+  for (int q = 0; q < 16; q++) R[threadIdx.x + 256 * q] = (U)v[q];
+}
+
+template <typename T, typename U>
+__device__ __forceinline__ void lds_to_tile(T* A, int ld, const U* R) {
+  U v[16];
+#pragma unroll
+  for (int q = 0; q < 16; q++) v[q] = R[threadIdx.x + 256 * q];
+#pragma unroll
+  for (int q = 0; q < 16; q++) {
+    const int e = threadIdx.x + 256 * q;
+    A[(size_t)(e >> 6) * ld + (e & 63)] = (T)v[q];
+  }
+}
+
+// acc (+)= A B^T for 64x64 operands in global memory (leading dimension ld)
+template <typename T>
+__device__ __forceinline__ void acc_gemm_nt_g(CholAcc<T>& acc, const T* A, const T* B, int ld, bool sub, int lane,
+                                              int wr, int wc) {
+  const T* As = A + wr * 32 + (lane & 15);
+  const T* Bs = B + wc * 32 + (lane & 15);
+  const int kq = lane >> 4;
+#pragma unroll 8
+  for (int m = 0; m < 64; m += 4) {
+    const size_t off = (size_t)(m + kq) * ld;
+    const T a0 = sub ? -As[off] : As[off], a1 = sub ? -As[off + 16] : As[off + 16];
+    const T b0 = Bs[off], b1 = Bs[off + 16];
+    acc.a[0][0] = Mfma<T>::fma(a0, b0, acc.a[0][0]);
+    acc.a[0][1] = Mfma<T>::fma(a0, b1, acc.a[0][1]);
+    acc.a[1][0] = Mfma<T>::fma(a1, b0, acc.a[1][0]);
+    acc.a[1][1] = Mfma<T>::fma(a1, b1, acc.a[1][1]);
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void acc_zero(CholAcc<T>& acc) {
+#pragma unroll
+  for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++) acc.a[bi][bj] = typename Mfma<T>::acc_t{};
+}
+
+// right-looking tile Cholesky of the padded S (nP x nP, NT = nP / 64 <= 4
+// tiles per side): per tile column j the diagonal tile is factorised and
+// inverted in FP64 in LDS (potrf_lds / tri_inverse), its inverse kept in
+// Linv[j], the tiles below become C L_jj^-T and the trailing tiles are
+// updated, all products on MFMA.  Pivot failures are counted in c->fail_s.
+template <typename T>
+__device__ void win_cholesky_tiles(const BWin& w, DevCtl* c, double* lds) {
+  double* R0d = lds;
+  double* R1d = lds + 4096;
+  T* R0 = (T*)lds;
+  T* S = tp<T>(w.S);
+  T* Li = tp<T>(w.Linv);
+  const int ld = w.nP, NTw = w.nP / 64;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int wr = wv & 1, wc = wv >> 1;
+  for (int j = 0; j < NTw; j++) {
+    tile_to_lds<T, double>(R0d, tile_at(S, ld, j, j), ld);
+    __syncthreads();
+    potrf_lds<64>(R0d, 64 * j, w.n, c);
+    tri_inverse<64>(R0d, R1d);
+    lds_to_tile<T, double>(tile_at(S, ld, j, j), ld, R0d);
+    {
+      T* Ig = Li + (size_t)j * 4096;
+#pragma unroll 4
+      for (int e = threadIdx.x; e < 4096; e += 256) Ig[e] = (T)R1d[e];
+    }
+    __syncthreads();
+    for (int i = j + 1; i < NTw; i++) {
+      tile_to_lds<T, T>(R0, tile_at(S, ld, i, j), ld);
+      __syncthreads();
+      CholAcc<T> x;
+      acc_zero(x);
+      acc_gemm_nt<T, double, 64>(x, R0, R1d, false, lane, wr, wc);
+      __syncthreads();
+      acc_store<T, T, 64>(x, R0, lane, wr, wc);
+      __syncthreads();
+      lds_to_tile<T, T>(tile_at(S, ld, i, j), ld, R0);
+    }
+    __syncthreads();
+    for (int k = j + 1; k < NTw; k++)
+      for (int i = k; i < NTw; i++) {
+        CholAcc<T> acc;
+        T* Cg = tile_at(S, ld, i, k);
+#pragma unroll
+        for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+          for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+            for (int vv = 0; vv < 4; vv++)
+              acc.a[bi][bj][vv] = Cg[(size_t)(wc * 32 + bj * 16 + (lane & 15)) * ld + wr * 32 + bi * 16 +
+                                     Mfma<T>::row(vv, lane)];
+        acc_gemm_nt_g<T>(acc, tile_at(S, ld, i, j), tile_at(S, ld, k, j), ld, true, lane, wr, wc);
+#pragma unroll
+        for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+          for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+            for (int vv = 0; vv < 4; vv++)
+              Cg[(size_t)(wc * 32 + bj * 16 + (lane & 15)) * ld + wr * 32 + bi * 16 + Mfma<T>::row(vv, lane)] =
+                  acc.a[bi][bj][vv];
+      }
+    __syncthreads();
+  }
+}
+
+// out[0..63] = M v (TRANS = false) or M^T v (TRANS = true) for a 64x64
+// column-major tile M (leading dimension ld) and v in LDS; four partial sums
+// per entry combined in a fixed order through red (4 x 64 doubles); the
+// result is valid in threads 0..63 (returned) after the call
+template <typename T, bool TRANS>
+__device__ __forceinline__ double tile_gemv(const T* M, int ld, const double* v, double* red) {
+  const int tid = threadIdx.x, e = tid & 63, part = tid >> 6;
+  double s = 0.0;
+#pragma unroll
+  for (int q = 0; q < 16; q++) {
+    const int k = part * 16 + q;
+    s += (double)(TRANS ? M[(size_t)e * ld + k] : M[(size_t)k * ld + e]) * v[k];
+  }
+  red[part * 64 + e] = s;
+  __syncthreads();
+  double r = 0.0;
+  if (tid < 64) r = ((red[e] + red[64 + e]) + red[128 + e]) + red[192 + e];
+  __syncthreads();
+  return r;
+}
+
+// L y = b then L^T x = y by 64-row blocks with the diagonal-tile inverses:
+// y_k = Linv_kk (b_k - sum_{i<k} L_ki y_i), x_k = Linv_kk^T (y_k - sum_{i>k}
+// L_ik^T x_i).  b: xU (global, nP entries, zero beyond n), overwritten by x.
+// ws: LDS workspace (nP + 64 + 256 doubles)
+template <typename T>
+__device__ void win_solve_tiles(const BWin& w, double* ws) {
+  const T* S = tp<T>(w.S);
+  const T* Li = tp<T>(w.Linv);
+  const int ld = w.nP, NTw = w.nP / 64, tid = threadIdx.x;
+  double* yv = ws;                 // nP: y, then x
+  double* tv = ws + w.nP;          // 64
+  double* red = tv + 64;           // 256
+  for (int e = tid; e < w.nP; e += 256) yv[e] = w.xU[e];
+  __syncthreads();
+  for (int k = 0; k < NTw; k++) {
+    double acc = 0.0;
+    for (int i = 0; i < k; i++) acc += tile_gemv<T, false>(tile_at(S, ld, k, i), ld, yv + 64 * i, red);
+    if (tid < 64) tv[tid] = yv[64 * k + tid] - acc;
+    __syncthreads();
+    const double y = tile_gemv<T, false>(Li + (size_t)k * 4096, 64, tv, red);
+    if (tid < 64) yv[64 * k + tid] = y;
+    __syncthreads();
+  }
+  for (int k = NTw - 1; k >= 0; k--) {
+    double acc = 0.0;
+    for (int i = k + 1; i < NTw; i++) acc += tile_gemv<T, true>(tile_at(S, ld, i, k), ld, yv + 64 * i, red);
+    if (tid < 64) tv[tid] = yv[64 * k + tid] - acc;
+    __syncthreads();
+    const double x = tile_gemv<T, true>(Li + (size_t)k * 4096, 64, tv, red);
+    if (tid < 64) yv[64 * k + tid] = x;
+    __syncthreads();
+  }
+  for (int e = tid; e < w.n; e += 256) w.xU[e] = yv[e];
+  __syncthreads();
+}
+
 // one Schur step with damping (rel, abs) = (c->rel, c->abs_); writes xU, xV
 // and the step statistics (the small-increment test of
-// LevenbergMarquardt.hpp:73-75 included)
-template <typename T, int NB>
-__device__ void win_schur(const BWin& w, DevCtl* c, double* pan, double* yv, double* ys, double* sh, int* shi) {
+// LevenbergMarquardt.hpp:73-75 included).  ALG: 8 / 16 = panel variant of
+// that width, 64 = tile variant.  lds: the workgroup's 64 KiB; sh: 4 doubles.
+template <typename T, int ALG>
+__device__ void win_schur(const BWin& w, DevCtl* c, double* lds, double* sh) {
   const BAView& v = w.v;
-  const int tid = threadIdx.x, n = w.n;
+  const int tid = threadIdx.x, n = w.n, ld = w.nP;
   const double rel = c->rel, ab = c->abs_;
   const T* V = tp<T>(w.V);
   const T* bV = tp<T>(w.bV);
@@ -185,13 +391,20 @@ __device__ void win_schur(const BWin& w, DevCtl* c, double* pan, double* yv, dou
   T* Y = tp<T>(w.Y);
   T* tW = tp<T>(w.tW);
   T* S = tp<T>(w.S);
-  if (tid == 0) shi[0] = 0;
+  if (tid == 0) {
+    c->fail_v = 0;
+    c->fail_s = 0;
+  }
   __syncthreads();
   int fv = 0;
   for (int q = tid; q < v.NV; q += kBT)
     if (!vinv_y_point<T>(v, q, V, bV, W, Vinv, Y, tW, rel, ab)) fv = 1;
-  if (fv) atomicOr(shi, 1);
-  for (int e = tid; e < n * n; e += kBT) S[e] = (T)0;
+  if (fv) atomicAdd(&c->fail_v, 1);
+  // the padded S: zeros, identity on the padding diagonal
+  for (int e = tid; e < ld * ld; e += kBT) {
+    const int r = e % ld, cc = e / ld;
+    S[e] = (r == cc && r >= n) ? (T)1 : (T)0;
+  }
   __syncthreads();
   // S blocks: S_ii = U_i - sum_k Y_k W_k^T (+ damping), S_ij = -sum_pairs Y_a W_b^T
   // (SolverCholeskySchur.hpp:114-124, MatrixWWtMultiplier.hpp:124-153);
@@ -216,7 +429,7 @@ __device__ void win_schur(const BWin& w, DevCtl* c, double* pan, double* yv, dou
           val += rel * uu;
           val += ab;
         }
-        S[(size_t)(6 * i + a) * n + 6 * i + bb] = (T)val;
+        S[(size_t)(6 * i + a) * ld + 6 * i + bb] = (T)val;
       } else {
         const int p1 = w.sb_ptr[sb + 1];
         for (int pp = w.sb_ptr[sb]; pp < p1; pp++) {
@@ -224,32 +437,33 @@ __device__ void win_schur(const BWin& w, DevCtl* c, double* pan, double* yv, dou
           const T* ww = W + 18 * (size_t)w.sb_pairs[2 * pp + 1] + 3 * bb;
           s += y[0] * ww[0] + y[1] * ww[1] + y[2] * ww[2];
         }
-        S[(size_t)(6 * i + a) * n + 6 * j + bb] = -s;
+        S[(size_t)(6 * i + a) * ld + 6 * j + bb] = -s;
       }
     }
   }
-  // bS = -bU - W V^-1 (bV-part): one thread per entry
-  if (tid < n) {
-    const int u = tid / 6, aa = tid % 6;
-    T s = (T)0;
-    const int k1 = v.cw_ptr[u + 1];
-    for (int kk = v.cw_ptr[u]; kk < k1; kk++) s += tW[6 * (size_t)v.cw_idx[kk] + aa];
-    yv[tid] = -(double)bU[6 * (size_t)u + aa] - (double)s;
+  // bS = -bU - W V^-1 (bV-part): one thread per entry (the tile variant keeps
+  // it in xU, padded with zeros; the panel variant in LDS)
+  double* yv = lds + 4096;
+  if (tid < ld) {
+    double val = 0.0;
+    if (tid < n) {
+      const int u = tid / 6, aa = tid % 6;
+      T s = (T)0;
+      const int k1 = v.cw_ptr[u + 1];
+      for (int kk = v.cw_ptr[u]; kk < k1; kk++) s += tW[6 * (size_t)v.cw_idx[kk] + aa];
+      val = -(double)bU[6 * (size_t)u + aa] - (double)s;
+    }
+    if constexpr (ALG == 64) w.xU[tid] = val;
+    else yv[tid] = val;
   }
   __syncthreads();
-  const bool ok_s = win_cholesky<T, NB>(S, n, pan);
-  // forward solve L y = bS, then backward L^T x = y (column sweeps)
-  for (int cc = 0; cc < n; cc++) {
-    const double yc = yv[cc] / (double)S[(size_t)cc * n + cc];
-    if (tid == 0) ys[cc] = yc;
-    if (tid > cc && tid < n) yv[tid] -= (double)S[(size_t)cc * n + tid] * yc;
-    __syncthreads();
-  }
-  for (int cc = n - 1; cc >= 0; cc--) {
-    const double xc = ys[cc] / (double)S[(size_t)cc * n + cc];
-    if (tid == 0) w.xU[cc] = xc;
-    if (tid < cc) ys[tid] -= (double)S[(size_t)tid * n + cc] * xc;
-    __syncthreads();
+  if constexpr (ALG == 64) {
+    win_cholesky_tiles<T>(w, c, lds);
+    win_solve_tiles<T>(w, lds);
+  } else {
+    const bool ok = win_cholesky_panels<T, ALG>(S, n, ld, lds);
+    if (!ok && tid == 0) atomicAdd(&c->fail_s, 1);
+    win_solve_sweeps<T>(S, n, ld, yv, yv + 256, w.xU);
   }
   double inc = 0.0, den = 0.0;
   for (int q = tid; q < v.NV; q += kBT) {
@@ -266,7 +480,9 @@ __device__ void win_schur(const BWin& w, DevCtl* c, double* pan, double* yv, dou
   }
   const double a1 = bsum(incu, sh), b1 = bsum(denu, sh), c1 = bsum(inc, sh), d1 = bsum(den, sh);
   if (tid == 0) {
-    const int ok = (shi[0] == 0 && ok_s) ? 1 : 0;
+    const int fails = __hip_atomic_load(&c->fail_v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) +
+                      __hip_atomic_load(&c->fail_s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int ok = fails == 0 ? 1 : 0;
     c->solve_ok = ok;
     c->incsq = a1 + c1;
     c->den = b1 + d1;
@@ -280,8 +496,8 @@ __device__ void win_schur(const BWin& w, DevCtl* c, double* pan, double* yv, dou
 
 // the LM loop of one window (LevenbergMarquardt.hpp:35-146); the control
 // scalars live in c and are broadcast through LDS after every update
-template <typename T, int NB>
-__device__ void win_lm(const BWin& w, DevCtl* c, double* pan, double* yv, double* ys, double* sh, int* shi) {
+template <typename T, int ALG>
+__device__ void win_lm(const BWin& w, DevCtl* c, double* lds, double* sh, int* shi) {
   const int tid = threadIdx.x;
   const long long maxsub = (long long)max(1, c->maxNumIt) * max(1, c->maxNumSubIt) + 4;
   for (long long it = 0; it < maxsub; it++) {
@@ -304,7 +520,7 @@ __device__ void win_lm(const BWin& w, DevCtl* c, double* pan, double* yv, double
       __syncthreads();
       if (shi[1]) break;
     }
-    win_schur<T, NB>(w, c, pan, yv, ys, sh, shi);
+    win_schur<T, ALG>(w, c, lds, sh);
     __syncthreads();
     if (tid == 0) shi[3] = !c->done && c->solve_ok;
     __syncthreads();
@@ -330,21 +546,31 @@ __device__ void win_lm(const BWin& w, DevCtl* c, double* pan, double* yv, double
   }
 }
 
-template <typename T, int NB>
+// LDS: one 64 KiB buffer; the tile variant's two 64x64 FP64 tiles or the
+// panel variant's panel and vectors, with the reduction / control scalars in
+// its last 96 bytes, which only the tile factorisation (inside a Schur step)
+// overwrites
+template <typename T, int ALG>
 __global__ __launch_bounds__(256) void k_batch_lm(const BWin* __restrict__ wins, int nwin, DevCtl* ctls,
                                                   int* counter) {
-  __shared__ double pan[NB * kMaxS];
-  __shared__ double yv[kMaxS], ys[kMaxS];
-  __shared__ double sh[4];
-  __shared__ int shi[8];
+  __shared__ double lds[8192];
+  double* sh = lds + 8184;
+  int* shi = (int*)(lds + 8176);
   for (int round = 0; round < nwin; round++) {  // at most nwin windows per workgroup
     if (threadIdx.x == 0) shi[7] = atomicAdd(counter, 1);
     __syncthreads();
     const int wi = shi[7];
     __syncthreads();
     if (wi >= nwin) break;
-    const BWin w = wins[wi];
-    win_lm<T, NB>(w, ctls + wi, pan, yv, ys, sh, shi);
+    if constexpr (ALG == 64) {
+      // the tile variant reads the descriptor's fields where it uses them
+      // (its factorisation needs the registers)
+      const BWin& w = wins[wi];
+      win_lm<T, ALG>(w, ctls + wi, lds, sh, shi);
+    } else {
+      const BWin w = wins[wi];
+      win_lm<T, ALG>(w, ctls + wi, lds, sh, shi);
+    }
     __syncthreads();
   }
 }
@@ -356,9 +582,9 @@ int launch_batch_lm(const BWin* wins, int nwin, DevCtl* ctls, int* counter, int 
   const int g = grid < nwin ? grid : nwin;
   if (g <= 0) return 0;
   switch (nb) {
-    case 4: hipLaunchKernelGGL((k_batch_lm<T, 4>), dim3(g), dim3(kBT), 0, st, wins, nwin, ctls, counter); break;
     case 8: hipLaunchKernelGGL((k_batch_lm<T, 8>), dim3(g), dim3(kBT), 0, st, wins, nwin, ctls, counter); break;
     case 16: hipLaunchKernelGGL((k_batch_lm<T, 16>), dim3(g), dim3(kBT), 0, st, wins, nwin, ctls, counter); break;
+    case 64: hipLaunchKernelGGL((k_batch_lm<T, 64>), dim3(g), dim3(kBT), 0, st, wins, nwin, ctls, counter); break;
     default: return -1;
   }
   return 0;

@@ -80,11 +80,32 @@ def test_batch_tune_and_state_roundtrip():
     assert np.array_equal(pts0, np.concatenate([g["pts0"] for g in wins]))
     s = LevenbergMarquardtSettings(maxNumIt=10)
     nb, grid = b.tune(s)
-    assert nb in (4, 8, 16) and grid in (256, 512, 1024)
+    assert nb in (8, 16, 64) and grid in (256, 512, 1024)
     cams1, pts1 = b.get_state()
     assert np.array_equal(cams1, cams0) and np.array_equal(pts1, pts0)  # the tuner restores the state
     c_a, _ = b.lm_solve(s)
     b.set_state(cams0, pts0)
-    b.set_config(4, 3)
+    b.set_config(8, 3)
     c_b, _ = b.lm_solve(s)
     assert np.allclose(c_a, c_b, rtol=1e-6)
+
+
+def test_batch_variants_agree():
+    """The tile (64) and panel (8, 16) factorisation variants give the same
+    FP64 solves, on windows of one to four 64-row tiles."""
+    rng = np.random.default_rng(7)
+    wins = []
+    for w, nc in enumerate([4, 11, 12, 25, 40, 43]):
+        npt = int(rng.integers(100, 500))
+        wins.append(generate_ba(dict(CONFIGS["cfg1"], n_cameras=nc, n_points=npt, k_min=3, k_max=6, seed=900 + w)))
+    s = LevenbergMarquardtSettings(maxNumIt=8)
+    res = {}
+    for variant in (8, 16, 64):
+        b = BABatch(wins, precision="fp64")
+        b.set_config(variant, 4)
+        res[variant], _ = b.lm_solve(s)
+    for variant in (8, 16):
+        assert np.allclose(res[variant], res[64], rtol=1e-9, atol=0), (variant, res[variant], res[64])
+    for w, g in enumerate(wins):
+        c1, _, _ = _single(g, maxNumIt=8)
+        assert abs(res[64][w] - c1) <= 1e-9 * c1
