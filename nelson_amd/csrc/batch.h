@@ -5,13 +5,26 @@
 
 namespace nsn {
 
+constexpr int kSChunk = 32;  // terms per chunk of the S assembly
+constexpr int kPart = 42;    // partial sums per chunk: the 6x6 block, then 6 of bS
+
 struct BWin {
   BAView v;                                   // the window's structure and state
   int n;                                      // order of S (6 NU), <= 256
   int nP;                                     // n padded to a multiple of 64
   int nsb;                                    // S blocks (i <= j)
-  const int *sb_i, *sb_j, *sb_ptr, *sb_pairs;  // and their (W, W) pair lists
+  const int *sb_i, *sb_j;                     // their camera pair
+  // the terms Y_a W_b^T of every S block ((a, a) over the camera's W blocks
+  // for a diagonal block, the (W, W) pairs sharing a point otherwise), block
+  // after block (terms: int2 (a, b)), cut into chunks of at most kSChunk
+  // terms; the chunks are numbered block after block (block b owns chunks
+  // blk_ch_ptr[b] .. blk_ch_ptr[b + 1]) and listed longest first in ch
+  // (int4: first term, end term, chunk number, 1 for a diagonal block);
+  // sb_diag[u]: the diagonal block of camera u
+  int nch;
+  const int *terms, *ch, *blk_ch_ptr, *sb_diag;
   void *U, *bU, *V, *bV, *W, *Vinv, *Y, *tW;  // Hessian blocks (element type of the batch)
+  void* part;                                 // kPart partial sums per chunk
   void* S;                                    // nP x nP, column-major, lower
   void* Linv;                                 // nP / 64 inverses of the diagonal factor tiles
   double *xU, *xV, *cams_bak, *pts_bak;       // xU: nP entries
@@ -23,5 +36,9 @@ struct BWin {
 // (panels of that width) or 64 (64x64 tiles).
 template <typename T>
 int launch_batch_lm(const BWin* wins, int nwin, DevCtl* ctls, int* counter, int nb, int grid, hipStream_t st);
+
+// zeroes the window counter of the persistent launch (a kernel, so that the
+// launch can be captured in a graph without a memset node)
+void launch_batch_reset(int* counter, hipStream_t st);
 
 }  // namespace nsn

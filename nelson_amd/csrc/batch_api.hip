@@ -8,7 +8,9 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <utility>
 #include <vector>
 
 #include "batch.h"
@@ -50,9 +52,12 @@ struct nsn_batch {
   bool tuned = false;
   double tune_ms[9] = {};
   hipEvent_t e0 = nullptr, e1 = nullptr;
+  // one instantiated graph (counter reset + persistent kernel) per launch shape
+  std::map<std::pair<int, int>, hipGraphExec_t> graphs;
 
   ~nsn_batch() {
     if (st) (void)hipStreamSynchronize(st);
+    for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
     for (void* p : allocs) (void)hipFree(p);
     if (hctls) (void)hipHostFree(hctls);
     if (e0) (void)hipEventDestroy(e0);
@@ -71,6 +76,83 @@ namespace {
 
 size_t esz(int prec) { return prec == NSN_MIXED ? sizeof(float) : sizeof(double); }
 
+// the S assembly's work list: every block's terms cut into chunks of at most
+// kSChunk, so that one lane sums one chunk whatever the block sizes
+// (MatrixWWtMultiplier.hpp:124-153 sums the same terms block by block)
+struct SChunks {
+  std::vector<int> terms, ch, blk_ch_ptr, sb_diag;
+};
+
+void s_chunks(const HostStructure& s, SChunks& o) {
+  std::vector<int> beg, end;
+  o.terms.clear();
+  o.blk_ch_ptr.assign(s.nsb + 1, 0);
+  o.sb_diag.assign(s.NU, 0);
+  std::vector<int> diag;
+  for (int b = 0; b < s.nsb; b++) {
+    const int t0 = (int)o.terms.size() / 2;
+    const bool dg = s.sb_i[b] == s.sb_j[b];
+    if (dg) {
+      const int u = s.sb_i[b];
+      o.sb_diag[u] = b;
+      for (int kk = s.cw_ptr[u]; kk < s.cw_ptr[u + 1]; kk++) {
+        o.terms.push_back(s.cw_idx[kk]);
+        o.terms.push_back(s.cw_idx[kk]);
+      }
+    } else {
+      o.terms.insert(o.terms.end(), s.sb_pairs.begin() + 2 * (size_t)s.sb_ptr[b],
+                     s.sb_pairs.begin() + 2 * (size_t)s.sb_ptr[b + 1]);
+    }
+    const int t1 = (int)o.terms.size() / 2;
+    for (int t = t0; t < t1; t += kSChunk) {
+      beg.push_back(t);
+      end.push_back(std::min(t1, t + kSChunk));
+      diag.push_back(dg ? 1 : 0);
+    }
+    o.blk_ch_ptr[b + 1] = (int)beg.size();
+  }
+  const int nch = (int)beg.size();
+  std::vector<int> ord(nch);
+  for (int c = 0; c < nch; c++) ord[c] = c;
+  std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return end[x] - beg[x] > end[y] - beg[y]; });
+  o.ch.assign(4 * (size_t)nch, 0);
+  for (int r = 0; r < nch; r++) {
+    o.ch[4 * r] = beg[ord[r]];
+    o.ch[4 * r + 1] = end[ord[r]];
+    o.ch[4 * r + 2] = ord[r];
+    o.ch[4 * r + 3] = diag[ord[r]];
+  }
+}
+
+// the batch solve as a graph: the counter reset and the persistent kernel,
+// captured once per (variant, grid) and replayed by every later solve
+int batch_graph(nsn_batch* b, int nb, int grid, hipGraphExec_t* out) {
+  auto it = b->graphs.find({nb, grid});
+  if (it != b->graphs.end()) {
+    *out = it->second;
+    return NSN_OK;
+  }
+  hipGraph_t g = nullptr;
+  HIP_TRY(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+  launch_batch_reset(b->counter, b->st);
+  const int lrc = b->prec == NSN_MIXED ? launch_batch_lm<float>(b->dw, b->nwin, b->ctls, b->counter, nb, grid, b->st)
+                                       : launch_batch_lm<double>(b->dw, b->nwin, b->ctls, b->counter, nb, grid, b->st);
+  const hipError_t ec = hipStreamEndCapture(b->st, &g);
+  if (lrc || ec != hipSuccess || !g) {
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+    if (lrc) return fail(NSN_EINVAL, "batch: unsupported factorisation variant %d", nb);
+    return fail(NSN_EHIP, "batch: graph capture failed: %s", hipGetErrorString(ec));
+  }
+  hipGraphExec_t x = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ei != hipSuccess) return fail(NSN_EHIP, "batch: graph instantiation failed: %s", hipGetErrorString(ei));
+  b->graphs[{nb, grid}] = x;
+  *out = x;
+  return NSN_OK;
+}
+
 int run_batch(nsn_batch* b, const nsn_lm_settings* s, int nb, int grid, float* ms) {
   for (int w = 0; w < b->nwin; w++) {
     DevCtl& c = b->hctls[w];
@@ -82,13 +164,12 @@ int run_batch(nsn_batch* b, const nsn_lm_settings* s, int nb, int grid, float* m
     c.tau = s->tau; c.penaltyV = s->penaltyV; c.initV = s->initV;
     c.maxScaleFactor = s->maxScaleFactor; c.minScaleFactor = s->minScaleFactor;
   }
+  hipGraphExec_t graph = nullptr;
+  int rc = batch_graph(b, nb, grid, &graph);
+  if (rc) return rc;
   HIP_TRY(hipMemcpyAsync(b->ctls, b->hctls, sizeof(DevCtl) * b->nwin, hipMemcpyHostToDevice, b->st));
-  HIP_TRY(hipMemsetAsync(b->counter, 0, sizeof(int), b->st));
   HIP_TRY(hipEventRecord(b->e0, b->st));
-  int rc = b->prec == NSN_MIXED ? launch_batch_lm<float>(b->dw, b->nwin, b->ctls, b->counter, nb, grid, b->st)
-                                : launch_batch_lm<double>(b->dw, b->nwin, b->ctls, b->counter, nb, grid, b->st);
-  if (rc) return fail(NSN_EINVAL, "batch: unsupported panel width %d", nb);
-  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipGraphLaunch(graph, b->st));
   HIP_TRY(hipEventRecord(b->e1, b->st));
   HIP_TRY(hipMemcpyAsync(b->hctls, b->ctls, sizeof(DevCtl) * b->nwin, hipMemcpyDeviceToHost, b->st));
   HIP_TRY(hipStreamSynchronize(b->st));
@@ -175,9 +256,10 @@ extern "C" int nsn_batch_create(const nsn_problem_desc* wd, int n_windows, nsn_b
   size_t tcount = 0;
   struct WinOff {
     size_t cam_var, pt_var, vcu, vpu, pt_ptr, pe_cam, pe_wblk, pe_edge, cam_ptr, ce_pt, wptr, w_cam, w_pt, cw_ptr,
-        cw_idx, sb_i, sb_j, sb_ptr, sb_pairs;
+        cw_idx, sb_i, sb_j, terms, ch, blk_ch_ptr, sb_diag;
+    int nch;
     size_t pe_obs, pe_w, ce_obs, ce_w, xU, xV;
-    size_t U, bU, V, bV, W, Vinv, Y, tW, S, Linv;
+    size_t U, bU, V, bV, W, Vinv, Y, tW, S, Linv, part;
     size_t cams, pts;
   };
   std::vector<WinOff> wo(n_windows);
@@ -210,7 +292,18 @@ extern "C" int nsn_batch_create(const nsn_problem_desc* wd, int n_windows, nsn_b
     o.cam_ptr = push_i(s.cam_ptr); o.ce_pt = push_i(s.ce_pt);
     o.wptr = push_i(s.wptr); o.w_cam = push_i(s.w_cam); o.w_pt = push_i(s.w_pt);
     o.cw_ptr = push_i(s.cw_ptr); o.cw_idx = push_i(s.cw_idx);
-    o.sb_i = push_i(s.sb_i); o.sb_j = push_i(s.sb_j); o.sb_ptr = push_i(s.sb_ptr); o.sb_pairs = push_i(s.sb_pairs);
+    o.sb_i = push_i(s.sb_i); o.sb_j = push_i(s.sb_j);
+    {
+      SChunks sc;
+      s_chunks(s, sc);
+      ipool.resize((ipool.size() + 3) / 4 * 4, 0);  // int2 / int4 loads
+      o.terms = push_i(sc.terms);
+      ipool.resize((ipool.size() + 3) / 4 * 4, 0);
+      o.ch = push_i(sc.ch);
+      o.blk_ch_ptr = push_i(sc.blk_ch_ptr);
+      o.sb_diag = push_i(sc.sb_diag);
+      o.nch = (int)sc.ch.size() / 4;
+    }
     o.pe_obs = push_d(s.pe_obs); o.pe_w = push_d(s.pe_w); o.ce_obs = push_d(s.ce_obs); o.ce_w = push_d(s.ce_w);
     const int n = 6 * s.NU, nP = (n + 63) / 64 * 64;
     o.xU = take_d(nP); o.xV = take_d(3 * (size_t)s.NV);
@@ -220,6 +313,7 @@ extern "C" int nsn_batch_create(const nsn_problem_desc* wd, int n_windows, nsn_b
     o.Y = take_t(18 * (size_t)s.nW); o.tW = take_t(6 * (size_t)s.nW);
     o.S = take_t((size_t)nP * nP);
     o.Linv = take_t((size_t)(nP / 64) * 4096);
+    o.part = take_t(kPart * (size_t)o.nch);
     o.cams = 12 * (size_t)b->totC;
     o.pts = 3 * (size_t)b->totP;
     b->totC += s.C;
@@ -265,7 +359,10 @@ extern "C" int nsn_batch_create(const nsn_problem_desc* wd, int n_windows, nsn_b
     x.n = 6 * s.NU;
     x.nP = (x.n + 63) / 64 * 64;
     x.nsb = s.nsb;
-    x.sb_i = di + o.sb_i; x.sb_j = di + o.sb_j; x.sb_ptr = di + o.sb_ptr; x.sb_pairs = di + o.sb_pairs;
+    x.sb_i = di + o.sb_i; x.sb_j = di + o.sb_j;
+    x.nch = o.nch;
+    x.terms = di + o.terms; x.ch = di + o.ch; x.blk_ch_ptr = di + o.blk_ch_ptr; x.sb_diag = di + o.sb_diag;
+    x.part = dt + es * o.part;
     x.U = dt + es * o.U; x.bU = dt + es * o.bU; x.V = dt + es * o.V; x.bV = dt + es * o.bV;
     x.W = dt + es * o.W; x.Vinv = dt + es * o.Vinv; x.Y = dt + es * o.Y; x.tW = dt + es * o.tW;
     x.S = dt + es * o.S;

@@ -25,7 +25,6 @@ namespace {
 
 constexpr int kBT = 256;      // threads per window
 constexpr int kMaxS = 256;    // largest S order of a window
-constexpr int kTeams = 7;     // 36-lane teams of the S assembly
 
 // deterministic block reductions of per-thread values (4 waves, fixed order)
 __device__ __forceinline__ double bsum(double v, double* sh) {
@@ -377,6 +376,87 @@ __device__ void win_solve_tiles(const BWin& w, double* ws) {
 // and the step statistics (the small-increment test of
 // LevenbergMarquardt.hpp:73-75 included).  ALG: 8 / 16 = panel variant of
 // that width, 64 = tile variant.  lds: the workgroup's 64 KiB; sh: 4 doubles.
+// the 18 values of a 6x3 block (Y or W) of edge k; blocks start at a
+// multiple of 18 elements of a 256-byte aligned array, so 8-byte (float) or
+// 16-byte (double) vector loads are aligned
+__device__ __forceinline__ void load18(const float* p, float* o) {
+  const float2* q = (const float2*)p;
+#pragma unroll
+  for (int i = 0; i < 9; i++) {
+    const float2 t = q[i];
+    o[2 * i] = t.x;
+    o[2 * i + 1] = t.y;
+  }
+}
+
+__device__ __forceinline__ void load18(const double* p, double* o) {
+  const double2* q = (const double2*)p;
+#pragma unroll
+  for (int i = 0; i < 9; i++) {
+    const double2 t = q[i];
+    o[2 * i] = t.x;
+    o[2 * i + 1] = t.y;
+  }
+}
+
+// phase 1 of the S assembly: a lane sums the terms Y_a W_b^T of one chunk
+// (at most kSChunk terms of one block) into 36 partial sums, and for a
+// diagonal block also the (W V^-1 bV)_a of the bS row of its camera into 6
+// more, stored under the chunk's number; the terms are taken U at a time (all
+// their loads issued before the products), the last group padded with repeats
+// of its first term that are not added
+template <typename T>
+__device__ void win_s_chunks(const BWin& w) {
+  constexpr int U = sizeof(T) == 4 ? 4 : 2;
+  const T* Y = tp<T>(w.Y);
+  const T* W = tp<T>(w.W);
+  const T* tW = tp<T>(w.tW);
+  T* part = tp<T>(w.part);
+  const int nch = w.nch;
+  const int4* chs = (const int4*)w.ch;
+  const int2* terms = (const int2*)w.terms;
+  for (int c = threadIdx.x; c < nch; c += kBT) {
+    const int4 cr = chs[c];
+    T acc[36];
+#pragma unroll
+    for (int e = 0; e < 36; e++) acc[e] = (T)0;
+    for (int t = cr.x; t < cr.y; t += U) {
+      int2 k[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) k[u] = terms[t + u < cr.y ? t + u : t];
+      T y[U][18], ww[U][18];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        load18(Y + 18 * (size_t)k[u].x, y[u]);
+        load18(W + 18 * (size_t)k[u].y, ww[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        if (t + u >= cr.y) break;
+#pragma unroll
+        for (int a = 0; a < 6; a++)
+#pragma unroll
+          for (int bb = 0; bb < 6; bb++)
+            acc[6 * a + bb] += y[u][3 * a] * ww[u][3 * bb] + y[u][3 * a + 1] * ww[u][3 * bb + 1] +
+                               y[u][3 * a + 2] * ww[u][3 * bb + 2];
+      }
+    }
+    T* o = part + kPart * (size_t)cr.z;
+#pragma unroll
+    for (int e = 0; e < 36; e++) o[e] = acc[e];
+    if (cr.w) {
+      T bs[6] = {(T)0, (T)0, (T)0, (T)0, (T)0, (T)0};
+      for (int t = cr.x; t < cr.y; t++) {
+        const T* q = tW + 6 * (size_t)terms[t].x;
+#pragma unroll
+        for (int i = 0; i < 6; i++) bs[i] += q[i];
+      }
+#pragma unroll
+      for (int i = 0; i < 6; i++) o[36 + i] = bs[i];
+    }
+  }
+}
+
 template <typename T, int ALG>
 __device__ void win_schur(const BWin& w, DevCtl* c, double* lds, double* sh) {
   const BAView& v = w.v;
@@ -407,37 +487,32 @@ __device__ void win_schur(const BWin& w, DevCtl* c, double* lds, double* sh) {
   }
   __syncthreads();
   // S blocks: S_ii = U_i - sum_k Y_k W_k^T (+ damping), S_ij = -sum_pairs Y_a W_b^T
-  // (SolverCholeskySchur.hpp:114-124, MatrixWWtMultiplier.hpp:124-153);
-  // a 36-lane team per block, lane (a, bb) owns one entry
-  const int team = tid / 36, l36 = tid % 36, a = l36 / 6, bb = l36 % 6;
-  if (team < kTeams) {
-    for (int sb = team; sb < w.nsb; sb += kTeams) {
+  // (SolverCholeskySchur.hpp:114-124, MatrixWWtMultiplier.hpp:124-153).
+  // Phase 1: one lane per chunk of a block's terms, the whole 6x6 product per
+  // term; phase 2: one thread per block entry sums its chunks in term order
+  // (deterministic)
+  win_s_chunks<T>(w);
+  __syncthreads();
+  {
+    const T* part = tp<T>(w.part);
+    for (int e = tid; e < 36 * w.nsb; e += kBT) {
+      const int sb = e / 36, r = e - 36 * sb, a = r / 6, bb = r - 6 * (r / 6);
       const int i = w.sb_i[sb], j = w.sb_j[sb];
-      T s = (T)0;
+      if (i == j && bb < a) continue;
+      T sum = (T)0;
+      const int c1 = w.blk_ch_ptr[sb + 1];
+#pragma unroll 8
+      for (int cc = w.blk_ch_ptr[sb]; cc < c1; cc++) sum += part[kPart * (size_t)cc + r];
       if (i == j) {
-        if (bb < a) continue;
-        const int k1 = v.cw_ptr[i + 1];
-        for (int kk = v.cw_ptr[i]; kk < k1; kk++) {
-          const int k = v.cw_idx[kk];
-          const T* y = Y + 18 * (size_t)k + 3 * a;
-          const T* ww = W + 18 * (size_t)k + 3 * bb;
-          s += y[0] * ww[0] + y[1] * ww[1] + y[2] * ww[2];
-        }
         const double uu = (double)U[21 * (size_t)i + up6(a, bb)];
-        double val = uu - (double)s;
+        double val = uu - (double)sum;
         if (a == bb && (rel != 0.0 || ab != 0.0)) {
           val += rel * uu;
           val += ab;
         }
         S[(size_t)(6 * i + a) * ld + 6 * i + bb] = (T)val;
       } else {
-        const int p1 = w.sb_ptr[sb + 1];
-        for (int pp = w.sb_ptr[sb]; pp < p1; pp++) {
-          const T* y = Y + 18 * (size_t)w.sb_pairs[2 * pp] + 3 * a;
-          const T* ww = W + 18 * (size_t)w.sb_pairs[2 * pp + 1] + 3 * bb;
-          s += y[0] * ww[0] + y[1] * ww[1] + y[2] * ww[2];
-        }
-        S[(size_t)(6 * i + a) * ld + 6 * j + bb] = -s;
+        S[(size_t)(6 * i + a) * ld + 6 * j + bb] = -sum;
       }
     }
   }
@@ -448,9 +523,11 @@ __device__ void win_schur(const BWin& w, DevCtl* c, double* lds, double* sh) {
     double val = 0.0;
     if (tid < n) {
       const int u = tid / 6, aa = tid % 6;
+      const T* part = tp<T>(w.part);
+      const int sb = w.sb_diag[u], c1 = w.blk_ch_ptr[sb + 1];
       T s = (T)0;
-      const int k1 = v.cw_ptr[u + 1];
-      for (int kk = v.cw_ptr[u]; kk < k1; kk++) s += tW[6 * (size_t)v.cw_idx[kk] + aa];
+#pragma unroll 8
+      for (int cc = w.blk_ch_ptr[sb]; cc < c1; cc++) s += part[kPart * (size_t)cc + 36 + aa];
       val = -(double)bU[6 * (size_t)u + aa] - (double)s;
     }
     if constexpr (ALG == 64) w.xU[tid] = val;
@@ -575,7 +652,15 @@ __global__ __launch_bounds__(256) void k_batch_lm(const BWin* __restrict__ wins,
   }
 }
 
+__global__ void k_batch_reset(int* counter) {
+  if (threadIdx.x == 0) *counter = 0;
+}
+
 }  // namespace
+
+void launch_batch_reset(int* counter, hipStream_t st) {
+  hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, st, counter);
+}
 
 template <typename T>
 int launch_batch_lm(const BWin* wins, int nwin, DevCtl* ctls, int* counter, int nb, int grid, hipStream_t st) {
