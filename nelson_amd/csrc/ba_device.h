@@ -330,10 +330,11 @@ __device__ __forceinline__ void lin_point(const BAView& v, int p, T* __restrict_
   }
 }
 
-// damped V_q^-1 through its Cholesky factor in FP64 (MatrixDiagInv.hpp:25-140),
-// Y = W V^-1 and t = W V^-1 (-bV) for the W blocks of point q; false if the
-// damped block is not positive definite (V^-1 is then zero)
-template <typename T>
+// damped V_q^-1 through its Cholesky factor L in FP64 (MatrixDiagInv.hpp:25-140),
+// Y = W V^-1 (or, with ZF, Y = W L^-T, so that W V^-1 W'^T = Y Y'^T) and
+// t = W V^-1 (-bV) for the W blocks of point q; false if the damped block is
+// not positive definite (V^-1 is then zero)
+template <typename T, bool ZF = false>
 __device__ __forceinline__ bool vinv_y_point(const BAView& v, int q, const T* __restrict__ V,
                                              const T* __restrict__ bV, const T* __restrict__ W,
                                              T* __restrict__ Vinv, T* __restrict__ Y, T* __restrict__ tW, double rel,
@@ -376,7 +377,18 @@ __device__ __forceinline__ bool vinv_y_point(const BAView& v, int q, const T* __
   const double x0 = -(vi[0] * b0 + vi[1] * b1 + vi[2] * b2);
   const double x1 = -(vi[1] * b0 + vi[3] * b1 + vi[4] * b2);
   const double x2 = -(vi[2] * b0 + vi[4] * b1 + vi[5] * b2);
-  const T M[9] = {(T)vi[0], (T)vi[1], (T)vi[2], (T)vi[1], (T)vi[3], (T)vi[4], (T)vi[2], (T)vi[4], (T)vi[5]};
+  // y_i = w_i M: M = V^-1, or M = L^-T (zero when V failed)
+  T M[9];
+  if constexpr (ZF) {
+    const double z = ok ? 1.0 : 0.0;
+    M[0] = (T)(z * m00); M[1] = (T)(z * m10); M[2] = (T)(z * m20);
+    M[3] = (T)0;         M[4] = (T)(z * m11); M[5] = (T)(z * m21);
+    M[6] = (T)0;         M[7] = (T)0;         M[8] = (T)(z * m22);
+  } else {
+    M[0] = (T)vi[0]; M[1] = (T)vi[1]; M[2] = (T)vi[2];
+    M[3] = (T)vi[1]; M[4] = (T)vi[3]; M[5] = (T)vi[4];
+    M[6] = (T)vi[2]; M[7] = (T)vi[4]; M[8] = (T)vi[5];
+  }
   const int k1 = v.wptr[q + 1];
   for (int k = v.wptr[q]; k < k1; k++) {
     const T* w = W + 18 * (size_t)k;

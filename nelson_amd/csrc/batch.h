@@ -14,7 +14,7 @@ struct BWin {
   int nP;                                     // n padded to a multiple of 64
   int nsb;                                    // S blocks (i <= j)
   const int *sb_i, *sb_j;                     // their camera pair
-  // the terms Y_a W_b^T of every S block ((a, a) over the camera's W blocks
+  // the terms Z_a Z_b^T of every S block ((a, a) over the camera's W blocks
   // for a diagonal block, the (W, W) pairs sharing a point otherwise), block
   // after block (terms: int2 (a, b)), cut into chunks of at most kSChunk
   // terms; the chunks are numbered block after block (block b owns chunks
@@ -23,7 +23,7 @@ struct BWin {
   // sb_diag[u]: the diagonal block of camera u
   int nch;
   const int *terms, *ch, *blk_ch_ptr, *sb_diag;
-  void *U, *bU, *V, *bV, *W, *Vinv, *Y, *tW;  // Hessian blocks (element type of the batch)
+  void *U, *bU, *V, *bV, *W, *Vinv, *Y, *tW;  // Hessian blocks (element type of the batch); Y = W L^-T
   void* part;                                 // kPart partial sums per chunk
   void* S;                                    // nP x nP, column-major, lower
   void* Linv;                                 // nP / 64 inverses of the diagonal factor tiles
@@ -31,7 +31,8 @@ struct BWin {
 };
 
 // One persistent launch: `grid` workgroups of 256 threads take windows from
-// *counter (zeroed by the caller) and run each window's LM solve with the
+// counter[0] (zeroed by the caller) in the order counter[1 .. nwin] (largest
+// windows first, so that the last windows taken are short) and run each window's LM solve with the
 // settings in its control block.  nb: the factorisation variant, 8 or 16
 // (panels of that width) or 64 (64x64 tiles).
 template <typename T>

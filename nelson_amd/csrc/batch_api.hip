@@ -331,7 +331,7 @@ extern "C" int nsn_batch_create(const nsn_problem_desc* wd, int n_windows, nsn_b
   double* bak_p = (double*)b->alloc(sizeof(double) * 3 * (size_t)b->totP);
   b->ctls = (DevCtl*)b->alloc(sizeof(DevCtl) * n_windows);
   b->dw = (BWin*)b->alloc(sizeof(BWin) * n_windows);
-  b->counter = (int*)b->alloc(sizeof(int));
+  b->counter = (int*)b->alloc(sizeof(int) * (1 + (size_t)n_windows));
   if (!di || !dd || !dt || !b->cams || !b->pts || !b->cams_save || !b->pts_save || !bak_c || !bak_p || !b->ctls ||
       !b->dw || !b->counter)
     return fail(NSN_ENOMEM, "nsn_batch_create: device allocation failed (%.1f MB)",
@@ -371,6 +371,15 @@ extern "C" int nsn_batch_create(const nsn_problem_desc* wd, int n_windows, nsn_b
     x.cams_bak = bak_c + o.cams; x.pts_bak = bak_p + o.pts;
   }
   HIP_TRY(hipMemcpy(b->dw, b->hw.data(), sizeof(BWin) * n_windows, hipMemcpyHostToDevice));
+  {
+    // windows in decreasing order of work (edges and S-assembly terms)
+    std::vector<long long> work(n_windows);
+    for (int w = 0; w < n_windows; w++) work[w] = (long long)hs[w].E + hs[w].nW + hs[w].npairs;
+    std::vector<int> order(1 + (size_t)n_windows, 0);
+    for (int w = 0; w < n_windows; w++) order[1 + w] = w;
+    std::stable_sort(order.begin() + 1, order.end(), [&](int x, int y) { return work[x] > work[y]; });
+    HIP_TRY(hipMemcpy(b->counter, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice));
+  }
   HIP_TRY(hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking));
   HIP_TRY(hipEventCreate(&b->e0));
   HIP_TRY(hipEventCreate(&b->e1));

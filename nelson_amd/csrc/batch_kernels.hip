@@ -1,5 +1,5 @@
 // Batched independent BA windows: one persistent launch in which each
-// workgroup takes windows from an atomic counter and runs the complete
+// workgroup takes windows (largest first) from an atomic counter and runs the complete
 // Levenberg-Marquardt solve of a window (nelson/LevenbergMarquardt.hpp:35-146
 // with the Schur step of nelson/SolverCholeskySchur.hpp:91-157) on its own:
 // linearisation, V^-1 / W V^-1, S assembly from the (W, W) pair lists, a
@@ -399,7 +399,9 @@ __device__ __forceinline__ void load18(const double* p, double* o) {
   }
 }
 
-// phase 1 of the S assembly: a lane sums the terms Y_a W_b^T of one chunk
+// phase 1 of the S assembly: a lane sums the terms Z_a Z_b^T (Z = W L^-T,
+// V = L L^T, so Z_a Z_b^T = W_a V^-1 W_b^T, and one array serves both
+// operands: half the bytes of Y and W, a diagonal term reads one block) of one chunk
 // (at most kSChunk terms of one block) into 36 partial sums, and for a
 // diagonal block also the (W V^-1 bV)_a of the bS row of its camera into 6
 // more, stored under the chunk's number; the terms are taken U at a time (all
@@ -408,8 +410,7 @@ __device__ __forceinline__ void load18(const double* p, double* o) {
 template <typename T>
 __device__ void win_s_chunks(const BWin& w) {
   constexpr int U = sizeof(T) == 4 ? 4 : 2;
-  const T* Y = tp<T>(w.Y);
-  const T* W = tp<T>(w.W);
+  const T* Y = tp<T>(w.Y);  // Z
   const T* tW = tp<T>(w.tW);
   T* part = tp<T>(w.part);
   const int nch = w.nch;
@@ -428,7 +429,7 @@ __device__ void win_s_chunks(const BWin& w) {
 #pragma unroll
       for (int u = 0; u < U; u++) {
         load18(Y + 18 * (size_t)k[u].x, y[u]);
-        load18(W + 18 * (size_t)k[u].y, ww[u]);
+        load18(Y + 18 * (size_t)k[u].y, ww[u]);
       }
 #pragma unroll
       for (int u = 0; u < U; u++) {
@@ -478,7 +479,7 @@ __device__ void win_schur(const BWin& w, DevCtl* c, double* lds, double* sh) {
   __syncthreads();
   int fv = 0;
   for (int q = tid; q < v.NV; q += kBT)
-    if (!vinv_y_point<T>(v, q, V, bV, W, Vinv, Y, tW, rel, ab)) fv = 1;
+    if (!vinv_y_point<T, true>(v, q, V, bV, W, Vinv, Y, tW, rel, ab)) fv = 1;
   if (fv) atomicAdd(&c->fail_v, 1);
   // the padded S: zeros, identity on the padding diagonal
   for (int e = tid; e < ld * ld; e += kBT) {
@@ -486,7 +487,7 @@ __device__ void win_schur(const BWin& w, DevCtl* c, double* lds, double* sh) {
     S[e] = (r == cc && r >= n) ? (T)1 : (T)0;
   }
   __syncthreads();
-  // S blocks: S_ii = U_i - sum_k Y_k W_k^T (+ damping), S_ij = -sum_pairs Y_a W_b^T
+  // S blocks: S_ii = U_i - sum_k Z_k Z_k^T (+ damping), S_ij = -sum_pairs Z_a Z_b^T
   // (SolverCholeskySchur.hpp:114-124, MatrixWWtMultiplier.hpp:124-153).
   // Phase 1: one lane per chunk of a block's terms, the whole 6x6 product per
   // term; phase 2: one thread per block entry sums its chunks in term order
@@ -634,7 +635,10 @@ __global__ __launch_bounds__(256) void k_batch_lm(const BWin* __restrict__ wins,
   double* sh = lds + 8184;
   int* shi = (int*)(lds + 8176);
   for (int round = 0; round < nwin; round++) {  // at most nwin windows per workgroup
-    if (threadIdx.x == 0) shi[7] = atomicAdd(counter, 1);
+    if (threadIdx.x == 0) {
+      const int k = atomicAdd(counter, 1);
+      shi[7] = k < nwin ? counter[1 + k] : nwin;
+    }
     __syncthreads();
     const int wi = shi[7];
     __syncthreads();
