@@ -52,7 +52,7 @@ __device__ __forceinline__ T* tp(void* p) {
 // max |b| and max |diag H|
 template <typename T>
 __device__ void win_linearize(const BWin& w, DevCtl* c, double* sh) {
-  const BAView& v = w.v;
+  const BAView v = w.v;  // a copy: the stores below cannot alias it
   T* V = tp<T>(w.V);
   T* bV = tp<T>(w.bV);
   T* W = tp<T>(w.W);
@@ -111,8 +111,9 @@ __device__ void win_linearize(const BWin& w, DevCtl* c, double* sh) {
 template <typename T>
 __device__ void win_cost(const BWin& w, DevCtl* c, double* sh) {
   double chi2 = 0.0, mb = 0.0, md = 0.0;
-  for (int p = threadIdx.x; p < w.v.P; p += kBT)
-    lin_point<T, false>(w.v, p, nullptr, nullptr, nullptr, chi2, mb, md);
+  const BAView v = w.v;
+  for (int p = threadIdx.x; p < v.P; p += kBT)
+    lin_point<T, false>(v, p, nullptr, nullptr, nullptr, chi2, mb, md);
   const double tot = bsum(chi2, sh);
   if (threadIdx.x == 0) c->chi2 = tot;
 }
@@ -460,7 +461,7 @@ __device__ void win_s_chunks(const BWin& w) {
 
 template <typename T, int ALG>
 __device__ void win_schur(const BWin& w, DevCtl* c, double* lds, double* sh) {
-  const BAView& v = w.v;
+  const BAView v = w.v;  // a copy: the stores below cannot alias it
   const int tid = threadIdx.x, n = w.n, ld = w.nP;
   const double rel = c->rel, ab = c->abs_;
   const T* V = tp<T>(w.V);
@@ -603,9 +604,13 @@ __device__ void win_lm(const BWin& w, DevCtl* c, double* lds, double* sh, int* s
     if (tid == 0) shi[3] = !c->done && c->solve_ok;
     __syncthreads();
     if (shi[3]) {
-      for (int idx = tid; idx < w.v.NU + w.v.NV; idx += kBT) {
-        if (idx < w.v.NU) oplus_cam(w.v, idx, w.xU);
-        else oplus_pt(w.v, idx - w.v.NU, w.xV);
+      {
+        const BAView v = w.v;
+        const double *xU = w.xU, *xV = w.xV;
+        for (int idx = tid; idx < v.NU + v.NV; idx += kBT) {
+          if (idx < v.NU) oplus_cam(v, idx, xU);
+          else oplus_pt(v, idx - v.NU, xV);
+        }
       }
       __syncthreads();
       win_cost<T>(w, c, sh);
