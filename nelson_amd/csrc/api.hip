@@ -139,7 +139,7 @@ void seq_schur_solve(nsn_problem* h, int gate, const double* rhsU, const double*
   const int NR = h->NR;
   int* xready = h->flags + h->tv.ntiles;
   int* gready = xready + NR;
-  launch_ir_reduce<T>(h->view, (const T*)h->Y, rhsU, rhsV, h->rS, h->ctl, gate, st);
+  launch_ir_reduce<T>(h->view, (const T*)h->W, (const T*)h->Vinv, rhsU, rhsV, h->rS, h->ctl, gate, st);
   launch_chol_reset<T>(xready, NR + NR * NR, h->counters + 1, h->ctl, gate, st);
   launch_trsv<T>(h->tv, (const T*)h->tiles, (const T*)h->inv, false, h->rS, h->zS, h->parts, xready, gready,
                  h->counters + 1, h->ctl, gate, kTrsvMaxWG, st);
@@ -159,11 +159,10 @@ void seq_schur(nsn_problem* h, int gate, int checks, bool timed) {
   launch_vinv_y<T>(h->view, (const T*)h->V, (const T*)h->bV, (const T*)h->W, (T*)h->Vinv, (T*)h->Y, (T*)h->tW,
                    h->ctl, gate, st);
   if (timed) (void)hipEventRecord(h->ev[1], st);
-  launch_s_diag<T>(h->view, (const T*)h->U, (const T*)h->bU, (const T*)h->Y, (const T*)h->W, (const T*)h->tW,
-                   (T*)h->Sd, (T*)h->bS, h->ctl, gate, st);
+  launch_s_diag<T>(h->view, (const T*)h->U, (const T*)h->bU, (const T*)h->Y, (const T*)h->tW, (T*)h->Sd,
+                   (T*)h->bS, h->ctl, gate, st);
   launch_s_tiles<T>(h->tv, h->tile_order, h->tile_blk_ptr, h->tile_blk, h->sb_i, h->sb_j, h->sb_ptr, h->sb_pairs,
-                    (const T*)h->Sd, (const T*)h->bS, (const T*)h->Y, (const T*)h->W, (T*)h->tiles, h->ctl, gate,
-                    st);
+                    (const T*)h->Sd, (const T*)h->bS, (const T*)h->Y, (T*)h->tiles, h->ctl, gate, st);
   if (timed) (void)hipEventRecord(h->ev[2], st);
   launch_chol_reset<T>(h->flags, h->nflags, h->counters, h->ctl, gate, st);
   {

@@ -18,6 +18,7 @@
 
 #include "ba_device.h"
 #include "kernels.h"
+#include "tile_device.h"
 
 namespace nsn {
 
@@ -131,31 +132,33 @@ __global__ __launch_bounds__(256) void k_finalize_lin(const double* __restrict__
 
 // ---------------------------------------------------------------------------
 // V^-1 (MatrixDiagInv::compute, nelson/MatrixDiagInv.hpp:25-140) fused with
-// Y = W V^-1 (MatrixWVinvMultiplier::multiply, MatrixWVinvMultiplier.hpp:23-106)
-// and t = W V^-1 (-bV) (the W * V^-1 bV term of bS, SolverCholeskySchur.hpp:114).
-// The damped 3x3 block is inverted through its Cholesky factor in FP64.
+// the W V^-1 products (MatrixWVinvMultiplier::multiply,
+// MatrixWVinvMultiplier.hpp:23-106) in the symmetric form Z = W L^-T, where
+// V = L L^T is the damped 3x3 block's Cholesky factor (FP64): the S terms
+// W_a V^-1 W_b^T = Z_a Z_b^T then read one array; and t = W V^-1 (-bV) (the
+// W * V^-1 bV term of bS, SolverCholeskySchur.hpp:114).
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void k_vinv_y(BAView v, const T* __restrict__ V, const T* __restrict__ bV,
                                                 const T* __restrict__ W, T* __restrict__ Vinv,
-                                                T* __restrict__ Y, T* __restrict__ tW, DevCtl* ctl, int gate) {
+                                                T* __restrict__ Z, T* __restrict__ tW, DevCtl* ctl, int gate) {
   if (!gate_open(ctl, gate)) return;
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= v.NV) return;
-  if (!vinv_y_point<T>(v, q, V, bV, W, Vinv, Y, tW, ctl->rel, ctl->abs_)) atomicAdd(&ctl->fail_v, 1);
+  if (!vinv_y_point<T, true>(v, q, V, bV, W, Vinv, Z, tW, ctl->rel, ctl->abs_)) atomicAdd(&ctl->fail_v, 1);
 }
 
 // ---------------------------------------------------------------------------
 // diagonal S blocks and bS, one workgroup per variable camera:
-//   S_uu = U_u - sum_k Y_k W_k^T, then S_ii += rel * U_ii + abs
+//   S_uu = U_u - sum_k Z_k Z_k^T, then S_ii += rel * U_ii + abs
 //   bS_u = -bU_u - sum_k t_k
 // (SolverCholeskySchur.hpp:114-124; MatrixWWtMultiplier.hpp:139-153 for the
 // diagonal pairs)
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void k_s_diag(BAView v, const T* __restrict__ U, const T* __restrict__ bU,
-                                                const T* __restrict__ Y, const T* __restrict__ W,
-                                                const T* __restrict__ tW, T* __restrict__ Sd,
+                                                const T* __restrict__ Z, const T* __restrict__ tW,
+                                                T* __restrict__ Sd,
                                                 T* __restrict__ bS, DevCtl* ctl, int gate) {
   if (!gate_open(ctl, gate)) return;
   __shared__ double red[4][27];
@@ -166,14 +169,14 @@ __global__ __launch_bounds__(256) void k_s_diag(BAView v, const T* __restrict__ 
   const int i1 = v.cw_ptr[u + 1];
   for (int ii = v.cw_ptr[u] + threadIdx.x; ii < i1; ii += 256) {
     const int k = v.cw_idx[ii];
-    T y[18], w[18];
+    T z[18];
 #pragma unroll
-    for (int i = 0; i < 18; i++) { y[i] = Y[18 * (size_t)k + i]; w[i] = W[18 * (size_t)k + i]; }
+    for (int i = 0; i < 18; i++) z[i] = Z[18 * (size_t)k + i];
 #pragma unroll
     for (int a = 0; a < 6; a++)
 #pragma unroll
       for (int b = a; b < 6; b++)
-        acc[up6(a, b)] += y[3 * a] * w[3 * b] + y[3 * a + 1] * w[3 * b + 1] + y[3 * a + 2] * w[3 * b + 2];
+        acc[up6(a, b)] += z[3 * a] * z[3 * b] + z[3 * a + 1] * z[3 * b + 1] + z[3 * a + 2] * z[3 * b + 2];
 #pragma unroll
     for (int a = 0; a < 6; a++) acc[21 + a] += tW[6 * (size_t)k + a];
   }
@@ -207,21 +210,27 @@ __global__ __launch_bounds__(256) void k_s_diag(BAView v, const T* __restrict__ 
 // ---------------------------------------------------------------------------
 // S tiles: one workgroup per lower TB x TB tile of [S; bS^T]. The tile is
 // built in LDS (zeros, identity padding, the bS row, diagonal blocks from Sd,
-// off-diagonal blocks S_ij = -sum_p Y_ip W_jp^T gathered over their pair
-// lists, MatrixWWtMultiplier.hpp:124-153) and written once, coalesced.
+// off-diagonal blocks S_ij = -sum_p Z_ip Z_jp^T over their pair lists,
+// MatrixWWtMultiplier.hpp:124-153) and written once, coalesced.
 // An upper block (i, j), i <= j, is stored transposed at rows 6j.., cols 6i..
+// A wave builds one block at a time: lane (s, a), s < kSlots, a < 6, sums row
+// a of the products of the pairs p = s, s + kSlots, ... (kSlots pairs in
+// flight per step), and the kSlots partial rows are added in slot order
+// through LDS (deterministic).
 // ---------------------------------------------------------------------------
+constexpr int kSlots = 10;
+
 template <typename T, int TB>
 __global__ __launch_bounds__(256) void k_s_tiles(TileView tv, const int* __restrict__ tile_order,
                                                  const int* __restrict__ tile_blk_ptr,
                                                  const int* __restrict__ tile_blk, const int* __restrict__ sb_i,
                                                  const int* __restrict__ sb_j, const int* __restrict__ sb_ptr,
                                                  const int* __restrict__ sb_pairs, const T* __restrict__ Sd,
-                                                 const T* __restrict__ bS, const T* __restrict__ Y,
-                                                 const T* __restrict__ W, T* __restrict__ tiles, DevCtl* ctl,
-                                                 int gate) {
+                                                 const T* __restrict__ bS, const T* __restrict__ Z,
+                                                 T* __restrict__ tiles, DevCtl* ctl, int gate) {
   if (!gate_open(ctl, gate)) return;
   __shared__ T lt[TB * TB];
+  __shared__ T red[4][kSlots * 36];
   const int t = blockIdx.x;
   const int ti = tile_order[2 * t], tj = tile_order[2 * t + 1];
   const int r0 = ti * TB, c0 = tj * TB, n = tv.n;
@@ -235,7 +244,9 @@ __global__ __launch_bounds__(256) void k_s_tiles(TileView tv, const int* __restr
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int a = lane / 6, bb = lane % 6;
+  const int slot = lane / 6, ra = lane - 6 * (lane / 6);  // partial-sum lanes (slot < kSlots)
+  const int a = lane / 6, bb = lane % 6;                   // output lanes (lane < 36)
+  T* rd = red[wv];
   const int b1 = tile_blk_ptr[t + 1];
   for (int b = tile_blk_ptr[t] + wv; b < b1; b += 4) {
     const int sb = tile_blk[b];
@@ -246,29 +257,29 @@ __global__ __launch_bounds__(256) void k_s_tiles(TileView tv, const int* __restr
       if (inside && bb >= a) lt[(Cc - c0) * TB + (R - r0)] = Sd[21 * (size_t)i + up6(a, bb)];
       continue;
     }
-    // S_ij = -sum_p Y_ip W_jp^T: lane (a, bb) < 36 owns one entry; the pair
-    // indices of up to 64 pairs are loaded at once and broadcast by readlane
-    T s = (T)0;
-    const int p0 = sb_ptr[sb], np = sb_ptr[sb + 1] - p0;
-    for (int base = 0; base < np; base += 64) {
-      const int cnt = min(64, np - base);
-      int my_ka = 0, my_kb = 0;
-      if (lane < cnt) {
-        my_ka = sb_pairs[2 * (p0 + base + lane)];
-        my_kb = sb_pairs[2 * (p0 + base + lane) + 1];
+    T acc[6] = {(T)0, (T)0, (T)0, (T)0, (T)0, (T)0};
+    if (slot < kSlots) {
+      const int p0 = sb_ptr[sb], p1 = sb_ptr[sb + 1];
+#pragma unroll 2
+      for (int p = p0 + slot; p < p1; p += kSlots) {
+        const int ka = sb_pairs[2 * p], kb = sb_pairs[2 * p + 1];
+        const T* y = Z + 18 * (size_t)ka + 3 * ra;
+        const T* w = Z + 18 * (size_t)kb;
+        const T y0 = y[0], y1 = y[1], y2 = y[2];
+#pragma unroll
+        for (int c = 0; c < 6; c++) acc[c] += y0 * w[3 * c] + y1 * w[3 * c + 1] + y2 * w[3 * c + 2];
       }
-#pragma unroll 4
-      for (int q = 0; q < cnt; q++) {
-        const int ka = __builtin_amdgcn_readlane(my_ka, q);
-        const int kb = __builtin_amdgcn_readlane(my_kb, q);
-        if (lane < 36) {
-          const T* y = Y + 18 * (size_t)ka + 3 * a;
-          const T* w = W + 18 * (size_t)kb + 3 * bb;
-          s += y[0] * w[0] + y[1] * w[1] + y[2] * w[2];
-        }
-      }
+#pragma unroll
+      for (int c = 0; c < 6; c++) rd[lane * 6 + c] = acc[c];
     }
-    if (inside) lt[(Cc - c0) * TB + (R - r0)] = -s;
+    wave_lds_sync();
+    if (inside) {
+      T sum = (T)0;
+#pragma unroll
+      for (int s2 = 0; s2 < kSlots; s2++) sum += rd[(s2 * 6 + a) * 6 + bb];
+      lt[(Cc - c0) * TB + (R - r0)] = -sum;
+    }
+    wave_lds_sync();
   }
   __syncthreads();
   T* out = tiles + (size_t)t * TB * TB;
@@ -429,21 +440,26 @@ __global__ __launch_bounds__(256) void k_ir_res_cams(BAView v, const double* __r
 
 // rS_u = rU_u - sum_{k in W blocks of u} Y_k rV_{point(k)}; one workgroup per camera
 template <typename T>
-__global__ __launch_bounds__(256) void k_ir_reduce(BAView v, const T* __restrict__ Y, const double* __restrict__ rU,
-                                                   const double* __restrict__ rV, double* __restrict__ rS,
-                                                   DevCtl* ctl, int gate) {
+__global__ __launch_bounds__(256) void k_ir_reduce(BAView v, const T* __restrict__ W, const T* __restrict__ Vinv,
+                                                   const double* __restrict__ rU, const double* __restrict__ rV,
+                                                   double* __restrict__ rS, DevCtl* ctl, int gate) {
   if (!gate_open(ctl, gate)) return;
   __shared__ double red[4][6];
   const int u = blockIdx.x;
   double acc[6] = {0, 0, 0, 0, 0, 0};
   const int i1 = v.cw_ptr[u + 1];
   for (int ii = v.cw_ptr[u] + threadIdx.x; ii < i1; ii += 256) {
-    const int k = v.cw_idx[ii];
-    const double* r = rV + 3 * (size_t)v.w_pt[k];
+    const int k = v.cw_idx[ii], q = v.w_pt[k];
+    const double* r = rV + 3 * (size_t)q;
+    const T* m = Vinv + 6 * (size_t)q;
     const double r0 = r[0], r1 = r[1], r2 = r[2];
-    const T* y = Y + 18 * (size_t)k;
+    // V^-1 r (symmetric, upper triangle stored row by row)
+    const double g0 = (double)m[0] * r0 + (double)m[1] * r1 + (double)m[2] * r2;
+    const double g1 = (double)m[1] * r0 + (double)m[3] * r1 + (double)m[4] * r2;
+    const double g2 = (double)m[2] * r0 + (double)m[4] * r1 + (double)m[5] * r2;
+    const T* w = W + 18 * (size_t)k;
 #pragma unroll
-    for (int a = 0; a < 6; a++) acc[a] += (double)y[3 * a] * r0 + (double)y[3 * a + 1] * r1 + (double)y[3 * a + 2] * r2;
+    for (int a = 0; a < 6; a++) acc[a] += (double)w[3 * a] * g0 + (double)w[3 * a + 1] * g1 + (double)w[3 * a + 2] * g2;
   }
 #pragma unroll
   for (int a = 0; a < 6; a++) {
@@ -705,30 +721,30 @@ void launch_finalize_lin(const Scratch& s, DevCtl* ctl, int gate, int full, hipS
 }
 
 template <typename T>
-void launch_vinv_y(const BAView& v, const T* V, const T* bV, const T* W, T* Vinv, T* Y, T* tW, DevCtl* ctl,
+void launch_vinv_y(const BAView& v, const T* V, const T* bV, const T* W, T* Vinv, T* Z, T* tW, DevCtl* ctl,
                    int gate, hipStream_t st) {
   if (v.NV == 0) return;
-  hipLaunchKernelGGL((k_vinv_y<T>), dim3(nblocks(v.NV, 256)), dim3(256), 0, st, v, V, bV, W, Vinv, Y, tW, ctl,
+  hipLaunchKernelGGL((k_vinv_y<T>), dim3(nblocks(v.NV, 256)), dim3(256), 0, st, v, V, bV, W, Vinv, Z, tW, ctl,
                      gate);
 }
 
 template <typename T>
-void launch_s_diag(const BAView& v, const T* U, const T* bU, const T* Y, const T* W, const T* tW, T* Sd, T* bS,
-                   DevCtl* ctl, int gate, hipStream_t st) {
+void launch_s_diag(const BAView& v, const T* U, const T* bU, const T* Z, const T* tW, T* Sd, T* bS, DevCtl* ctl,
+                   int gate, hipStream_t st) {
   if (v.NU == 0) return;
-  hipLaunchKernelGGL((k_s_diag<T>), dim3(v.NU), dim3(256), 0, st, v, U, bU, Y, W, tW, Sd, bS, ctl, gate);
+  hipLaunchKernelGGL((k_s_diag<T>), dim3(v.NU), dim3(256), 0, st, v, U, bU, Z, tW, Sd, bS, ctl, gate);
 }
 
 template <typename T>
 void launch_s_tiles(const TileView& tv, const int* tile_order, const int* tile_blk_ptr, const int* tile_blk,
                     const int* sb_i, const int* sb_j, const int* sb_ptr, const int* sb_pairs, const T* Sd,
-                    const T* bS, const T* Y, const T* W, T* tiles, DevCtl* ctl, int gate, hipStream_t st) {
+                    const T* bS, const T* Z, T* tiles, DevCtl* ctl, int gate, hipStream_t st) {
   if (tv.TB == 64)
     hipLaunchKernelGGL((k_s_tiles<T, 64>), dim3(tv.ntiles), dim3(256), 0, st, tv, tile_order, tile_blk_ptr,
-                       tile_blk, sb_i, sb_j, sb_ptr, sb_pairs, Sd, bS, Y, W, tiles, ctl, gate);
+                       tile_blk, sb_i, sb_j, sb_ptr, sb_pairs, Sd, bS, Z, tiles, ctl, gate);
   else if (tv.TB == 32)
     hipLaunchKernelGGL((k_s_tiles<T, 32>), dim3(tv.ntiles), dim3(256), 0, st, tv, tile_order, tile_blk_ptr,
-                       tile_blk, sb_i, sb_j, sb_ptr, sb_pairs, Sd, bS, Y, W, tiles, ctl, gate);
+                       tile_blk, sb_i, sb_j, sb_ptr, sb_pairs, Sd, bS, Z, tiles, ctl, gate);
 }
 
 template <typename T>
@@ -749,10 +765,10 @@ void launch_ir_residual(const BAView& v, const double* xU, const double* xV, dou
 }
 
 template <typename T>
-void launch_ir_reduce(const BAView& v, const T* Y, const double* rU, const double* rV, double* rS, DevCtl* ctl,
-                      int gate, hipStream_t st) {
+void launch_ir_reduce(const BAView& v, const T* W, const T* Vinv, const double* rU, const double* rV, double* rS,
+                      DevCtl* ctl, int gate, hipStream_t st) {
   if (v.NU == 0) return;
-  hipLaunchKernelGGL((k_ir_reduce<T>), dim3(v.NU), dim3(256), 0, st, v, Y, rU, rV, rS, ctl, gate);
+  hipLaunchKernelGGL((k_ir_reduce<T>), dim3(v.NU), dim3(256), 0, st, v, W, Vinv, rU, rV, rS, ctl, gate);
 }
 
 void launch_axpby(long long n, double a, const double* x, double b, const double* y, double* out, hipStream_t st) {
@@ -828,16 +844,16 @@ void launch_zero(T* p, long long n, hipStream_t st) {
   template void launch_lin_cams<T>(const BAView&, T*, T*, const Scratch&, DevCtl*, int, hipStream_t);       \
   template void launch_vinv_y<T>(const BAView&, const T*, const T*, const T*, T*, T*, T*, DevCtl*, int,      \
                                  hipStream_t);                                                               \
-  template void launch_s_diag<T>(const BAView&, const T*, const T*, const T*, const T*, const T*, T*, T*,    \
-                                 DevCtl*, int, hipStream_t);                                                 \
+  template void launch_s_diag<T>(const BAView&, const T*, const T*, const T*, const T*, T*, T*, DevCtl*,    \
+                                 int, hipStream_t);                                                          \
   template void launch_s_tiles<T>(const TileView&, const int*, const int*, const int*, const int*,           \
-                                  const int*, const int*, const int*, const T*, const T*, const T*,          \
-                                  const T*, T*, DevCtl*, int, hipStream_t);                                  \
+                                  const int*, const int*, const int*, const T*, const T*, const T*, T*,      \
+                                  DevCtl*, int, hipStream_t);                                                \
   template void launch_backsub<T>(const BAView&, const T*, const T*, const T*, const double*, double*,       \
                                   const Scratch&, DevCtl*, int, hipStream_t, const double*, int,             \
                                   const double*);                                                            \
-  template void launch_ir_reduce<T>(const BAView&, const T*, const double*, const double*, double*, DevCtl*,  \
-                                    int, hipStream_t);                                                       \
+  template void launch_ir_reduce<T>(const BAView&, const T*, const T*, const double*, const double*, double*, \
+                                    DevCtl*, int, hipStream_t);                                              \
   template void launch_xu_stats<T>(int, const double*, const T*, const Scratch&, DevCtl*, int, hipStream_t); \
   template void launch_zero<T>(T*, long long, hipStream_t);
 

@@ -79,16 +79,18 @@ template <typename T>
 void launch_lin_cams(const BAView& v, T* U, T* bU, const Scratch& s, DevCtl* ctl, int gate, hipStream_t st);
 void launch_reset_lin(const Scratch& s, DevCtl* ctl, int gate, hipStream_t st);
 void launch_finalize_lin(const Scratch& s, DevCtl* ctl, int gate, int full, hipStream_t st);
+// Vinv = damped V^-1, Z = W L^-T (V = L L^T, so W_a V^-1 W_b^T = Z_a Z_b^T)
+// and t = W V^-1 (-bV)
 template <typename T>
-void launch_vinv_y(const BAView& v, const T* V, const T* bV, const T* W, T* Vinv, T* Y, T* tW,
+void launch_vinv_y(const BAView& v, const T* V, const T* bV, const T* W, T* Vinv, T* Z, T* tW,
                    DevCtl* ctl, int gate, hipStream_t st);
 template <typename T>
-void launch_s_diag(const BAView& v, const T* U, const T* bU, const T* Y, const T* W, const T* tW,
+void launch_s_diag(const BAView& v, const T* U, const T* bU, const T* Z, const T* tW,
                    T* Sd, T* bS, DevCtl* ctl, int gate, hipStream_t st);
 template <typename T>
 void launch_s_tiles(const TileView& tv, const int* tile_order, const int* tile_blk_ptr, const int* tile_blk,
                     const int* sb_i, const int* sb_j, const int* sb_ptr, const int* sb_pairs,
-                    const T* Sd, const T* bS, const T* Y, const T* W, T* tiles, DevCtl* ctl,
+                    const T* Sd, const T* bS, const T* Z, T* tiles, DevCtl* ctl,
                     int gate, hipStream_t st);
 // xV = V^-1 (rhsV - W^T xU) with rhsV = -bV when rhs_v is null; with `add`
 // the result is added to xV; the increment statistics use bden (FP64
@@ -103,8 +105,8 @@ void launch_ir_residual(const BAView& v, const double* xU, const double* xV, dou
                         double* bU64, double* bV64, DevCtl* ctl, int gate, hipStream_t st);
 // rS = rU - W V^-1 rV (Schur right-hand side of a correction)
 template <typename T>
-void launch_ir_reduce(const BAView& v, const T* Y, const double* rU, const double* rV, double* rS, DevCtl* ctl,
-                      int gate, hipStream_t st);
+void launch_ir_reduce(const BAView& v, const T* W, const T* Vinv, const double* rU, const double* rV, double* rS,
+                      DevCtl* ctl, int gate, hipStream_t st);
 // x += z
 void launch_add(double* x, const double* z, long long n, DevCtl* ctl, int gate, hipStream_t st);
 // out = a x + b y (out may alias x or y)
