@@ -30,7 +30,10 @@ namespace {
 
 constexpr int kTB = 64;          // Cholesky tile
 constexpr int kCholMaxWG = 256;  // persistent workgroups of the factorisation
-constexpr int kTrsvMaxWG = 1024;
+// workgroups of a triangular solve: the chain and at most 255 destination
+// owners, one per CU, so that all of them are resident together whenever one
+// of them owns several destinations
+constexpr int kTrsvMaxWG = 256;
 
 struct DevBuf {
   std::vector<void*> ptrs;
@@ -142,10 +145,10 @@ void seq_schur_solve(nsn_problem* h, int gate, const double* rhsU, const double*
   launch_ir_reduce<T>(h->view, (const T*)h->W, (const T*)h->Vinv, rhsU, rhsV, h->rS, h->ctl, gate, st);
   launch_chol_reset<T>(xready, NR + NR * NR, h->counters + 1, h->ctl, gate, st);
   launch_trsv<T>(h->tv, (const T*)h->tiles, (const T*)h->inv, false, h->rS, h->zS, h->parts, xready, gready,
-                 h->counters + 1, h->ctl, gate, kTrsvMaxWG, st);
+                 h->ctl, gate, kTrsvMaxWG, st);
   launch_chol_reset<T>(xready, NR + NR * NR, h->counters + 1, h->ctl, gate, st);
   launch_trsv<T>(h->tv, (const T*)h->tiles, (const T*)h->inv, true, h->zS, xU, h->parts, xready, gready,
-                 h->counters + 1, h->ctl, gate, kTrsvMaxWG, st);
+                 h->ctl, gate, kTrsvMaxWG, st);
   launch_backsub<T>(h->view, (const T*)h->W, (const T*)h->Vinv, (const T*)h->bV, xU, xV, h->scr, h->ctl, gate, st,
                     rhsV, add, bden);
 }
@@ -172,8 +175,7 @@ void seq_schur(nsn_problem* h, int gate, int checks, bool timed) {
   }
   if (timed) (void)hipEventRecord(h->ev[3], st);
   launch_trsv<T>(h->tv, (const T*)h->tiles, (const T*)h->inv, true, nullptr, h->xU, h->parts,
-                 h->flags + h->tv.ntiles, h->flags + h->tv.ntiles + h->NR, h->counters + 1, h->ctl, gate, kTrsvMaxWG,
-                 st);
+                 h->flags + h->tv.ntiles, h->flags + h->tv.ntiles + h->NR, h->ctl, gate, kTrsvMaxWG, st);
   if (timed) (void)hipEventRecord(h->ev[4], st);
   launch_backsub<T>(h->view, (const T*)h->W, (const T*)h->Vinv, (const T*)h->bV, h->xU, h->xV, h->scr, h->ctl,
                     gate, st);

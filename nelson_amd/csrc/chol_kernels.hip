@@ -268,105 +268,146 @@ __global__ __launch_bounds__(256) void k_chol(TileView tv, T* __restrict__ tiles
 }
 
 // Triangular solve with the tile factor and the inverses of its diagonal
-// tiles (inv), as a task DAG over the NR = ceil(n / TB) tile rows of S.
-//   backward (L^T x = y): step s handles block k = NR-1-s: task D_k, then
-//                         G_{k,i} for i = k-2 down to 0;
-//   forward (L z = r):    step s handles block k = s: task D_k, then G_{k,i}
-//                         for i = k+2 up to NR-1.
-// G_{k,i} adds block k's contribution to block i (L_ki^T x_k backward,
-// L_ik z_k forward) to the running sum of the blocks solved before k, so
-// part(i, k) = part(i, k_prev) + contribution: one chain per destination i,
-// in solve order (deterministic).
-// D_k:  v = rhs_k - (part(k, last) + N sol_nb),  sol_k = Linv_kk^(T) v, with
-//       nb the neighbouring block (k+1 backward, k-1 forward), N its tile
-//       (L_{k+1,k}^T or L_{k,k-1}) and part(k, last) the chain of every other
-//       solved block.
+// tiles (inv), over the NR = ceil(n / TB) tile rows of S, in solve order
+// (backward, L^T x = y: blocks NR-1 .. 0; forward, L z = r: blocks 0 .. NR-1).
+// Workgroup 0 is the chain: for the block k of each solve step it forms
+//   v = rhs_k - (far_k + sum_{d=1..kNear} N_{k,d} x_{k-d})
+//   x_k = Linv_kk^(T) v
+// with the kNear previous solutions kept in its LDS (N_{k,d} = L_{k,k-d}
+// forward, L_{k+d,k}^T backward) and the tiles of the next step prefetched
+// into registers, and publishes x_k (xready[k]).  Every other workgroup owns
+// destination blocks (solve index kNear+1+w, +G, ...): it takes the
+// solutions in solve order and accumulates, in its LDS and in source order
+// (deterministic), far_i = sum of N x_k over the sources more than kNear steps
+// before i, publishing far_i (tready[i]) after its last far source.  The
+// chain waits only for destination sums whose sources it has published, and
+// the destination workgroups only for published solutions: no cycle.
 // rhs == nullptr takes the right-hand side from row n of the factor (the
-// forward solve of bS that the factorisation leaves there).  Every task waits
-// for at most two tasks, both earlier in the queue, so the DAG cannot
-// deadlock.
+// forward solve of bS that the factorisation leaves there).
+constexpr int kNear = 2;
+constexpr int kMaxOwned = 16;  // destination blocks per workgroup
+
 template <typename T, int TB, bool BWD>
 __global__ __launch_bounds__(256) void k_trsv(TileView tv, const T* __restrict__ tiles, const T* __restrict__ inv,
-                                              const double* rhs, double* sol, double* __restrict__ parts,
-                                              int* xready, int* gready, int* counter, DevCtl* ctl, int gate) {
+                                              const double* rhs, double* sol, double* __restrict__ far,
+                                              int* xready, int* tready, DevCtl* ctl, int gate) {
   static_assert(TB == 64, "64-row tiles");
   if (!chol_gate(ctl, gate)) return;
   typedef TileGemv<T, TB, BWD> Gemv;
   __shared__ double red[4 * TB];
   __shared__ double xs[TB];
   __shared__ double hs[TB];
-  __shared__ int sh_t;
+  __shared__ double xr[kNear][TB];
+  __shared__ double acc[kMaxOwned][TB];
   const int n = tv.n, NT = tv.NT;
   const int NR = (n + TB - 1) / TB;
-  const int ntask = NR + (NR >= 2 ? (NR - 1) * (NR - 2) / 2 : 0);
   const size_t T2 = (size_t)TB * TB;
   const int tid = threadIdx.x;
   const int step = BWD ? -1 : 1;  // solve order
-  for (int round = 0; round < ntask; round++) {  // bounded
-    if (tid == 0) sh_t = atomicAdd(counter, 1);
-    __syncthreads();
-    int t = sh_t;
-    __syncthreads();
-    if (t >= ntask) break;
-    int s = 0;
-    while (t >= 1 + (NR - 2 - s > 0 ? NR - 2 - s : 0)) {
-      t -= 1 + (NR - 2 - s > 0 ? NR - 2 - s : 0);
-      s++;
-    }
-    const int k = BWD ? NR - 1 - s : s;
-    if (t == 0) {
-      // ---- D_k ----
-      const int nb = k - step;       // solved just before k
-      const int pl = k - 2 * step;   // last block of k's partial-sum chain
-      const bool has_nb = nb >= 0 && nb < NR, has_part = pl >= 0 && pl < NR;
-      Gemv gd, gn;
+  auto blk = [&](int s) { return BWD ? NR - 1 - s : s; };
+  auto valid = [&](int b) { return b >= 0 && b < NR; };
+  // the tile applied to x_k for destination i (L_ik forward, L_ki^T backward)
+  auto ntile = [&](int i, int k) { return tiles + (size_t)(BWD ? tile_index(k, i, NT) : tile_index(i, k, NT)) * T2; };
+  if (blockIdx.x == 0) {
+    // ---- the chain ----
+    Gemv gd, gn[kNear];
+    double r0 = 0.0;
+    auto prefetch = [&](int s) {
+      const int k = blk(s);
       gd.load(inv + (size_t)k * T2);
-      if (has_nb) gn.load(tiles + (size_t)(BWD ? tile_index(nb, k, NT) : tile_index(k, nb, NT)) * T2);
-      double r0 = 0.0;
+#pragma unroll
+      for (int d = 1; d <= kNear; d++)
+        if (valid(k - d * step)) gn[d - 1].load(ntile(k, k - d * step));
+      r0 = 0.0;
       if (tid < TB && k * TB + tid < n)
         r0 = rhs ? rhs[k * TB + tid] : (double)tiles[(size_t)tile_index(n / TB, k, NT) * T2 + tid * TB + (n % TB)];
-      if (tid == 0) {
-        if (has_nb) wait_flag(xready + nb, ctl);
-        if (has_part) wait_flag(gready + (size_t)k * NR + pl, ctl);
+    };
+    prefetch(0);
+    for (int s = 0; s < NR; s++) {
+      const int k = blk(s);
+      const bool has_far = s > kNear;
+      // one thread waits and acquires; the barrier orders the rest of the
+      // workgroup after it (the other waves keep their prefetches in flight)
+      if (tid == 0 && has_far) {
+        wait_flag(tready + k, ctl);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       }
       __syncthreads();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       double pk = 0.0;
-      if (tid < TB) {
-        xs[tid] = (has_nb && nb * TB + tid < n) ? sol[nb * TB + tid] : 0.0;
-        if (has_part) pk = parts[((size_t)k * NR + pl) * TB + tid];
-      }
-      __syncthreads();
+      if (tid < TB && has_far) pk = far[(size_t)k * TB + tid];
+      // the near blocks' contributions, one reduction for all of them
       double h = 0.0;
-      if (has_nb) h = gn.apply(xs, red);
+      {
+        const int base = BWD ? (tid & 3) * 16 : (tid >> 6) * 16;
+#pragma unroll
+        for (int d = 1; d <= kNear; d++) {
+          if (d > s) continue;
+          const double* xv = xr[(s - d) % kNear];
+#pragma unroll
+          for (int q = 0; q < 16; q++) h += (double)gn[d - 1].a[q] * xv[base + q];
+        }
+        if (BWD) {
+          h += __shfl_xor(h, 1);
+          h += __shfl_xor(h, 2);
+        } else {
+          red[(tid >> 6) * TB + (tid & 63)] = h;
+          __syncthreads();
+          h = 0.0;
+          if (tid < TB) h = ((red[tid] + red[TB + tid]) + red[2 * TB + tid]) + red[3 * TB + tid];
+        }
+      }
       if (Gemv::owner()) hs[Gemv::index()] = h;
       __syncthreads();
       if (tid < TB) xs[tid] = (k * TB + tid < n) ? r0 - (pk + hs[tid]) : 0.0;
       __syncthreads();
       const double x = gd.apply(xs, red);
-      if (Gemv::owner() && k * TB + Gemv::index() < n) sol[k * TB + Gemv::index()] = x;
-      publish_flag(xready + k);
-    } else {
-      // ---- G_{k,i}: part(i, k) = part(i, k - step) + contribution of block k ----
-      const int i = BWD ? k - 1 - t : k + 1 + t;
-      const int kp = k - step;  // previous link of i's chain, if it is a source of i
-      const bool has_prev = BWD ? kp < NR : kp >= 0;
-      Gemv gg;
-      gg.load(tiles + (size_t)(BWD ? tile_index(k, i, NT) : tile_index(i, k, NT)) * T2);
-      if (tid == 0) {
-        wait_flag(xready + k, ctl);
-        if (has_prev) wait_flag(gready + (size_t)i * NR + kp, ctl);
-      }
-      __syncthreads();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-      if (tid < TB) xs[tid] = (k * TB + tid < n) ? sol[k * TB + tid] : 0.0;
-      __syncthreads();
-      const double v = gg.apply(xs, red);
       if (Gemv::owner()) {
         const int e = Gemv::index();
-        parts[((size_t)i * NR + k) * TB + e] = (has_prev ? parts[((size_t)i * NR + kp) * TB + e] : 0.0) + v;
+        const bool in = k * TB + e < n;
+        if (in) sol[k * TB + e] = x;
+        xr[s % kNear][e] = in ? x : 0.0;
       }
-      publish_flag(gready + (size_t)i * NR + k);
+      publish_flag(xready + k);
+      if (s + 1 < NR) prefetch(s + 1);
+    }
+    return;
+  }
+  // ---- destination sums ----
+  const int G = gridDim.x - 1, w = blockIdx.x - 1;
+  const int first = kNear + 1 + w;  // solve index of the first owned destination
+  if (first >= NR) return;
+  const int nown = (NR - 1 - first) / G + 1;  // <= kMaxOwned (launch)
+  for (int m = 0; m < nown; m++)
+    if (tid < TB) acc[m][tid] = 0.0;
+  Gemv gg;
+  int m0 = 0;  // first owned destination still open
+  gg.load(ntile(blk(first), blk(0)));
+  for (int s = 0; s < NR && m0 < nown; s++) {  // source steps
+    const int k = blk(s);
+    if (tid == 0) {
+      wait_flag(xready + k, ctl);
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+    if (tid < TB) xs[tid] = (k * TB + tid < n) ? sol[k * TB + tid] : 0.0;
+    __syncthreads();
+    for (int m = m0; m < nown; m++) {
+      const double v = gg.apply(xs, red);
+      if (Gemv::owner()) acc[m][Gemv::index()] += v;
+      // next product: the following destination, or the first open one of the next source
+      int nm = m + 1, ns = s;
+      if (nm == nown) {
+        nm = m0 + (first + G * m0 - kNear - 1 == s ? 1 : 0);
+        ns = s + 1;
+      }
+      if (nm < nown && ns < NR) gg.load(ntile(blk(first + G * nm), blk(ns)));
+    }
+    // destination m0 is complete after its last far source
+    if (first + G * m0 - kNear - 1 == s) {
+      const int d = blk(first + G * m0);
+      if (Gemv::owner()) far[(size_t)d * TB + Gemv::index()] = acc[m0][Gemv::index()];
+      publish_flag(tready + d);
+      m0++;
     }
   }
 }
@@ -395,18 +436,20 @@ void launch_chol(const TileView& tv, T* tiles, T* inv, T* presub, const int* tas
 
 template <typename T>
 void launch_trsv(const TileView& tv, const T* tiles, const T* inv, bool backward, const double* rhs, double* sol,
-                 double* parts, int* xready, int* gready, int* counter, DevCtl* ctl, int gate, int max_wg,
-                 hipStream_t st) {
+                 double* far, int* xready, int* tready, DevCtl* ctl, int gate, int max_wg, hipStream_t st) {
   const int NR = (tv.n + 63) / 64;
-  const int ntask = NR + (NR >= 2 ? (NR - 1) * (NR - 2) / 2 : 0);
-  if (ntask == 0) return;
-  int g = ntask < max_wg ? ntask : max_wg;
+  if (NR == 0) return;
+  // the chain plus G workgroups owning the nd destinations with far sources
+  const int nd = NR > kNear + 1 ? NR - kNear - 1 : 0;
+  int G = nd < max_wg - 1 ? nd : max_wg - 1;
+  if (G > 0 && (nd + G - 1) / G > kMaxOwned) G = (nd + kMaxOwned - 1) / kMaxOwned;
+  const int g = 1 + G;
   if (backward)
-    hipLaunchKernelGGL((k_trsv<T, 64, true>), dim3(g), dim3(256), 0, st, tv, tiles, inv, rhs, sol, parts, xready,
-                       gready, counter, ctl, gate);
+    hipLaunchKernelGGL((k_trsv<T, 64, true>), dim3(g), dim3(256), 0, st, tv, tiles, inv, rhs, sol, far, xready,
+                       tready, ctl, gate);
   else
-    hipLaunchKernelGGL((k_trsv<T, 64, false>), dim3(g), dim3(256), 0, st, tv, tiles, inv, rhs, sol, parts, xready,
-                       gready, counter, ctl, gate);
+    hipLaunchKernelGGL((k_trsv<T, 64, false>), dim3(g), dim3(256), 0, st, tv, tiles, inv, rhs, sol, far, xready,
+                       tready, ctl, gate);
 }
 
 template void launch_chol_reset<double>(int*, int, int*, DevCtl*, int, hipStream_t);
@@ -416,8 +459,8 @@ template void launch_chol<double>(const TileView&, double*, double*, double*, co
 template void launch_chol<float>(const TileView&, float*, float*, float*, const int*, int, int*, int*, int*, int*,
                                  DevCtl*, int, int, hipStream_t);
 template void launch_trsv<double>(const TileView&, const double*, const double*, bool, const double*, double*,
-                                  double*, int*, int*, int*, DevCtl*, int, int, hipStream_t);
+                                  double*, int*, int*, DevCtl*, int, int, hipStream_t);
 template void launch_trsv<float>(const TileView&, const float*, const float*, bool, const double*, double*, double*,
-                                 int*, int*, int*, DevCtl*, int, int, hipStream_t);
+                                 int*, int*, DevCtl*, int, int, hipStream_t);
 
 }  // namespace nsn

@@ -166,10 +166,10 @@ inline void chol_task_list(int NT, int* out, int* ntasks) {
   *ntasks = t;
 }
 // L^T sol = rhs (backward) or L sol = rhs (forward) over the first n rows;
-// rhs == nullptr: row n of the factor.  xready: NR flags, gready: NR*NR.
+// rhs == nullptr: row n of the factor.  xready, tready: NR flags each,
+// zeroed before the launch; far: NR x 64 doubles of scratch.
 template <typename T>
 void launch_trsv(const TileView& tv, const T* tiles, const T* inv, bool backward, const double* rhs, double* sol,
-                 double* parts, int* xready, int* gready, int* counter, DevCtl* ctl, int gate, int max_wg,
-                 hipStream_t st);
+                 double* far, int* xready, int* tready, DevCtl* ctl, int gate, int max_wg, hipStream_t st);
 
 }  // namespace nsn
