@@ -89,7 +89,7 @@ struct nsn_problem {
   DevBuf adj_mem;
   double *gU = nullptr, *gV = nullptr, *lamU = nullptr, *lamV = nullptr, *tU = nullptr, *tV = nullptr,
          *bpU = nullptr, *bpV = nullptr, *bmU = nullptr, *bmV = nullptr, *dU = nullptr, *dV = nullptr,
-         *zeroU = nullptr, *zeroV = nullptr, *dobs = nullptr, *dw = nullptr;
+         *zeroU = nullptr, *zeroV = nullptr, *dobs = nullptr, *dw = nullptr, *adj_eps = nullptr;
   // iterative refinement (mixed precision)
   int ir_steps = 0;
   double *rU = nullptr, *rV = nullptr, *bU64 = nullptr, *bV64 = nullptr, *rS = nullptr, *zS = nullptr,
@@ -827,9 +827,10 @@ int alloc_adjoint(nsn_problem* h) {
   for (double** p : vs) *p = m.alloc<double>(nv);
   h->dobs = m.alloc<double>(2 * (size_t)h->E);
   h->dw = m.alloc<double>(h->E);
+  h->adj_eps = m.alloc<double>(1);
   for (double** p : us) if (!*p) return fail(NSN_ENOMEM, "adjoint buffers");
   for (double** p : vs) if (!*p) return fail(NSN_ENOMEM, "adjoint buffers");
-  if (!h->dobs || !h->dw) return fail(NSN_ENOMEM, "adjoint buffers");
+  if (!h->dobs || !h->dw || !h->adj_eps) return fail(NSN_ENOMEM, "adjoint buffers");
   HIP_TRY(hipMemset(h->zeroU, 0, sizeof(double) * std::max<size_t>(nu, 1)));
   HIP_TRY(hipMemset(h->zeroV, 0, sizeof(double) * std::max<size_t>(nv, 1)));
   return NSN_OK;
@@ -874,27 +875,14 @@ int adjoint_t(nsn_problem* h, const double* g_cams, const double* g_pts, int ste
   // refinement against the exact Hessian: lambda += H^-1 (g - B lambda), with
   // B lambda = (b(x + eps lambda) - b(x - eps lambda)) / (2 eps), the central
   // difference of the FP64 gradient along lambda
-  std::vector<double> lu(nu), lv(nv);
+  // (all on the device: eps = 1e-6 / max |lambda| lives in adj_eps)
   for (int k = 0; k < steps; k++) {
-    if (nu) HIP_TRY(hipMemcpyAsync(lu.data(), h->lamU, sizeof(double) * nu, hipMemcpyDeviceToHost, st));
-    if (nv) HIP_TRY(hipMemcpyAsync(lv.data(), h->lamV, sizeof(double) * nv, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    double mx = 0.0;
-    for (double x : lu) mx = std::max(mx, std::fabs(x));
-    for (double x : lv) mx = std::max(mx, std::fabs(x));
-    if (!(mx > 0.0) || !std::isfinite(mx)) break;
-    const double eps = 1e-6 / mx;
-    launch_axpby(nu, eps, h->lamU, 0.0, h->lamU, h->tU, st);
-    launch_axpby(nv, eps, h->lamV, 0.0, h->lamV, h->tV, st);
+    launch_adj_eps(h->lamU, nu, h->lamV, nv, h->adj_eps, st);
+    launch_adj_offset(h->lamU, nu, h->lamV, nv, h->adj_eps, 1.0, h->tU, h->tV, st);
     grad_at_offset(h, h->tU, h->tV, h->bpU, h->bpV);
-    launch_axpby(nu, -1.0, h->tU, 0.0, h->tU, h->tU, st);
-    launch_axpby(nv, -1.0, h->tV, 0.0, h->tV, h->tV, st);
+    launch_adj_offset(h->lamU, nu, h->lamV, nv, h->adj_eps, -1.0, h->tU, h->tV, st);
     grad_at_offset(h, h->tU, h->tV, h->bmU, h->bmV);
-    // r = g - (bp - bm) / (2 eps), into (bp)
-    launch_axpby(nu, 1.0, h->bpU, -1.0, h->bmU, h->bpU, st);
-    launch_axpby(nv, 1.0, h->bpV, -1.0, h->bmV, h->bpV, st);
-    launch_axpby(nu, 1.0, h->gU, -0.5 / eps, h->bpU, h->bpU, st);
-    launch_axpby(nv, 1.0, h->gV, -0.5 / eps, h->bpV, h->bpV, st);
+    launch_adj_resid(h->gU, h->bmU, h->bpU, nu, h->gV, h->bmV, h->bpV, nv, h->adj_eps, st);
     seq_schur_solve<T>(h, GATE_ALWAYS, h->bpU, h->bpV, h->dU, h->dV, 0, nullptr);
     launch_axpby(nu, 1.0, h->lamU, 1.0, h->dU, h->lamU, st);
     launch_axpby(nv, 1.0, h->lamV, 1.0, h->dV, h->lamV, st);

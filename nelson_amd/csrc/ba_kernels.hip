@@ -539,6 +539,44 @@ __global__ void k_axpby(long long n, double a, const double* x, double b, const 
   if (i < n) out[i] = a * x[i] + b * y[i];
 }
 
+// step of the exact-Hessian refinement of the adjoint, on the device:
+// eps = 1e-6 / max |lambda| (0 when lambda is 0 or not finite), one block
+__global__ __launch_bounds__(256) void k_adj_eps(const double* __restrict__ lu, long long nu,
+                                                 const double* __restrict__ lv, long long nv, double* eps) {
+  __shared__ double sh[4];
+  double m = 0.0;
+  for (long long i = threadIdx.x; i < nu; i += 256) m = fmax(m, fabs(lu[i]));
+  for (long long i = threadIdx.x; i < nv; i += 256) m = fmax(m, fabs(lv[i]));
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double mx = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+    eps[0] = (mx > 0.0 && isfinite(mx)) ? 1e-6 / mx : 0.0;
+  }
+}
+
+// t = sgn * eps * lambda
+__global__ void k_adj_offset(const double* __restrict__ lu, long long nu, const double* __restrict__ lv,
+                             long long nv, const double* __restrict__ eps, double sgn, double* __restrict__ tu,
+                             double* __restrict__ tv) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const double e = sgn * eps[0];
+  if (i < nu) tu[i] = e * lu[i];
+  if (i < nv) tv[i] = e * lv[i];
+}
+
+// r = g - (bp - bm) / (2 eps) into bp (0 when eps is 0: no correction)
+__global__ void k_adj_resid(const double* __restrict__ gu, const double* __restrict__ bmu, double* __restrict__ bpu,
+                            long long nu, const double* __restrict__ gv, const double* __restrict__ bmv,
+                            double* __restrict__ bpv, long long nv, const double* __restrict__ eps) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const double e = eps[0];
+  const double c = e > 0.0 ? -0.5 / e : 0.0;
+  if (i < nu) bpu[i] = e > 0.0 ? gu[i] + c * (bpu[i] - bmu[i]) : 0.0;
+  if (i < nv) bpv[i] = e > 0.0 ? gv[i] + c * (bpv[i] - bmv[i]) : 0.0;
+}
+
 __global__ void k_add(double* __restrict__ x, const double* __restrict__ z, long long n, DevCtl* ctl, int gate) {
   if (!gate_open(ctl, gate)) return;
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -769,6 +807,26 @@ void launch_ir_reduce(const BAView& v, const T* W, const T* Vinv, const double* 
                       DevCtl* ctl, int gate, hipStream_t st) {
   if (v.NU == 0) return;
   hipLaunchKernelGGL((k_ir_reduce<T>), dim3(v.NU), dim3(256), 0, st, v, W, Vinv, rU, rV, rS, ctl, gate);
+}
+
+void launch_adj_eps(const double* lu, long long nu, const double* lv, long long nv, double* eps, hipStream_t st) {
+  hipLaunchKernelGGL(k_adj_eps, dim3(1), dim3(256), 0, st, lu, nu, lv, nv, eps);
+}
+
+void launch_adj_offset(const double* lu, long long nu, const double* lv, long long nv, const double* eps, double sgn,
+                       double* tu, double* tv, hipStream_t st) {
+  const long long n = nu > nv ? nu : nv;
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_adj_offset, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, lu, nu, lv, nv, eps, sgn, tu,
+                     tv);
+}
+
+void launch_adj_resid(const double* gu, const double* bmu, double* bpu, long long nu, const double* gv,
+                      const double* bmv, double* bpv, long long nv, const double* eps, hipStream_t st) {
+  const long long n = nu > nv ? nu : nv;
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_adj_resid, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, gu, bmu, bpu, nu, gv, bmv, bpv,
+                     nv, eps);
 }
 
 void launch_axpby(long long n, double a, const double* x, double b, const double* y, double* out, hipStream_t st) {

@@ -111,6 +111,13 @@ void launch_ir_reduce(const BAView& v, const T* W, const T* Vinv, const double* 
 void launch_add(double* x, const double* z, long long n, DevCtl* ctl, int gate, hipStream_t st);
 // out = a x + b y (out may alias x or y)
 void launch_axpby(long long n, double a, const double* x, double b, const double* y, double* out, hipStream_t st);
+// exact-Hessian refinement of the adjoint: eps = 1e-6 / max |lambda| (0 for a
+// zero or non-finite lambda), t = sgn eps lambda, bp = g - (bp - bm) / (2 eps)
+void launch_adj_eps(const double* lu, long long nu, const double* lv, long long nv, double* eps, hipStream_t st);
+void launch_adj_offset(const double* lu, long long nu, const double* lv, long long nv, const double* eps, double sgn,
+                       double* tu, double* tv, hipStream_t st);
+void launch_adj_resid(const double* gu, const double* bmu, double* bpu, long long nu, const double* gv,
+                      const double* bmv, double* bpv, long long nv, const double* eps, hipStream_t st);
 // per-edge adjoint outputs at the current state from the multipliers
 // (lamU, lamV): dL/d(obs) [2E] and dL/d(weight) [E], user edge order
 void launch_adjoint_edges(const BAView& v, const double* lamU, const double* lamV, double* dobs, double* dw,
