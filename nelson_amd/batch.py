@@ -83,14 +83,16 @@ class BABatch:
         check(self._lib.nsn_batch_tune(self._h, C.byref(s), C.byref(nb), C.byref(grid)))
         return nb.value, grid.value
 
-    def set_config(self, panel_width, grid):
-        check(self._lib.nsn_batch_set_config(self._h, int(panel_width), int(grid)))
+    def set_config(self, variant, grid):
+        """Factorisation variant (8 or 16: panels of that width, 64: 64x64
+        tiles) and number of workgroups of the persistent launch."""
+        check(self._lib.nsn_batch_set_config(self._h, int(variant), int(grid)))
 
     def config(self):
         nb, grid = C.c_int(0), C.c_int(0)
         ms = np.zeros(9)
         check(self._lib.nsn_batch_config(self._h, C.byref(nb), C.byref(grid), _dp(ms), 9))
-        return dict(panel_width=nb.value, grid=grid.value, tune_ms=ms.tolist())
+        return dict(variant=nb.value, grid=grid.value, tune_ms=ms.tolist())
 
     def lm_solve(self, settings=None):
         """LM on every window; returns (final chi2 per window, device ms)."""
