@@ -81,7 +81,7 @@ struct nsn_problem {
       *sb_ptr = nullptr, *sb_pairs = nullptr;
   int *flags = nullptr, *counters = nullptr;
   int nflags = 0, NR = 0;
-  double* parts = nullptr;  // triangular-solve partial sums, NR x NR x TB
+  double* parts = nullptr;  // triangular-solve destination sums, NR x TB
   // edge permutations and observation arrays (nsn_set_observations)
   std::vector<int> pe_order, ce_order;
   double *d_pe_obs = nullptr, *d_pe_w = nullptr, *d_ce_obs = nullptr, *d_ce_w = nullptr;
@@ -141,19 +141,19 @@ void seq_schur_solve(nsn_problem* h, int gate, const double* rhsU, const double*
   hipStream_t st = h->st;
   const int NR = h->NR;
   int* xready = h->flags + h->tv.ntiles;
-  int* gready = xready + NR;
+  int* tready = xready + NR;
   launch_ir_reduce<T>(h->view, (const T*)h->W, (const T*)h->Vinv, rhsU, rhsV, h->rS, h->ctl, gate, st);
-  launch_chol_reset<T>(xready, NR + NR * NR, h->counters + 1, h->ctl, gate, st);
-  launch_trsv<T>(h->tv, (const T*)h->tiles, (const T*)h->inv, false, h->rS, h->zS, h->parts, xready, gready,
+  launch_chol_reset<T>(xready, 2 * NR, h->counters + 1, h->ctl, gate, st);
+  launch_trsv<T>(h->tv, (const T*)h->tiles, (const T*)h->inv, false, h->rS, h->zS, h->parts, xready, tready,
                  h->ctl, gate, kTrsvMaxWG, st);
-  launch_chol_reset<T>(xready, NR + NR * NR, h->counters + 1, h->ctl, gate, st);
-  launch_trsv<T>(h->tv, (const T*)h->tiles, (const T*)h->inv, true, h->zS, xU, h->parts, xready, gready,
+  launch_chol_reset<T>(xready, 2 * NR, h->counters + 1, h->ctl, gate, st);
+  launch_trsv<T>(h->tv, (const T*)h->tiles, (const T*)h->inv, true, h->zS, xU, h->parts, xready, tready,
                  h->ctl, gate, kTrsvMaxWG, st);
   launch_backsub<T>(h->view, (const T*)h->W, (const T*)h->Vinv, (const T*)h->bV, xU, xV, h->scr, h->ctl, gate, st,
                     rhsV, add, bden);
 }
 
-// computeIncrement: V^-1, Y, S, factorisation, solves, back-substitution
+// computeIncrement: V^-1, Z, S, factorisation, solves, back-substitution
 template <typename T>
 void seq_schur(nsn_problem* h, int gate, int checks, bool timed) {
   hipStream_t st = h->st;
@@ -169,7 +169,7 @@ void seq_schur(nsn_problem* h, int gate, int checks, bool timed) {
   if (timed) (void)hipEventRecord(h->ev[2], st);
   launch_chol_reset<T>(h->flags, h->nflags, h->counters, h->ctl, gate, st);
   {
-    int* pre_sub = h->flags + h->tv.ntiles + h->NR + h->NR * h->NR;
+    int* pre_sub = h->flags + h->tv.ntiles + 2 * h->NR;
     launch_chol<T>(h->tv, (T*)h->tiles, (T*)h->inv, (T*)h->presub, h->chol_tasks, h->n_chol_tasks, h->flags,
                    pre_sub, pre_sub + h->tv.NT, h->counters, h->ctl, gate, kCholMaxWG, st);
   }
@@ -289,7 +289,9 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
     for (int sb = 0; sb < h->nsb; sb++) for_tiles(sb, [&](long long t) { tile_blk[fill[t]++] = sb; });
   }
   h->NR = (n + kTB - 1) / kTB;
-  h->nflags = tv.ntiles + h->NR + h->NR * h->NR + 2 * tv.NT;
+  // flags: ready (ntiles), the triangular solves' xready / tready (NR each),
+  // pre_sub / pre_diag (NT each)
+  h->nflags = tv.ntiles + 2 * h->NR + 2 * tv.NT;
   int n_tasks = 0;
   chol_task_list(tv.NT, nullptr, &n_tasks);
   std::vector<int> chol_tasks(3 * (size_t)std::max(n_tasks, 1));
@@ -339,7 +341,7 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
   h->inv = a_T((size_t)tv.NT * kTB * kTB);
   h->presub = a_T((size_t)tv.NT * kTB * kTB);
   h->xU = a_dbl(6 * (size_t)NU);
-  h->parts = a_dbl((size_t)h->NR * h->NR * kTB);
+  h->parts = a_dbl((size_t)h->NR * kTB);  // the triangular solves' destination sums
   h->rU = a_dbl(6 * (size_t)NU);
   h->rV = a_dbl(3 * (size_t)NV);
   h->bU64 = a_dbl(6 * (size_t)NU);
