@@ -4,7 +4,7 @@
 // nelson/LevenbergMarquardt.hpp:67-130):
 //   k_lin_points / k_lin_cams  : residuals, Jacobians, V, bV, W (point-major)
 //                                and U, bU (camera-major)        [update(true)]
-//   k_vinv_y                   : damped V^-1, Y = W V^-1, t = W V^-1 (-bV)
+//   k_vinv_y                   : damped V^-1, Z = W L^-T (V = L L^T), t = W V^-1 (-bV)
 //   k_s_diag                   : diagonal S blocks and bS per camera
 //   k_s_tiles                  : off-diagonal S blocks gathered per output tile
 //   (chol_kernels.hip)         : tile Cholesky of [S; bS^T], backward solve
@@ -438,7 +438,7 @@ __global__ __launch_bounds__(256) void k_ir_res_cams(BAView v, const double* __r
   }
 }
 
-// rS_u = rU_u - sum_{k in W blocks of u} Y_k rV_{point(k)}; one workgroup per camera
+// rS_u = rU_u - sum_{k in W blocks of u} W_k V^-1 rV_{point(k)}; one workgroup per camera
 template <typename T>
 __global__ __launch_bounds__(256) void k_ir_reduce(BAView v, const T* __restrict__ W, const T* __restrict__ Vinv,
                                                    const double* __restrict__ rU, const double* __restrict__ rV,

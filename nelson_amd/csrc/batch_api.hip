@@ -435,7 +435,7 @@ extern "C" int nsn_batch_tune(nsn_batch* b, const nsn_lm_settings* s, int* nb, i
 extern "C" int nsn_batch_set_config(nsn_batch* b, int nb, int grid) {
   if (!b) return fail(NSN_EINVAL, "null argument");
   if ((nb != 8 && nb != 16 && nb != 64) || grid < 1)
-    return fail(NSN_EINVAL, "nsn_batch_set_config: variant 8, 16 (panel width) or 64 (tiles) and grid >= 1");
+    return fail(NSN_EINVAL, "nsn_batch_set_config: variant 8 or 16 (panels of that width) or 64 (tiles), grid >= 1");
   b->nb = nb;
   b->grid = grid;
   b->tuned = true;
