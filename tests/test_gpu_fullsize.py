@@ -80,3 +80,25 @@ def test_cfg2_step_solves_normal_equations(cfg2, precision):
         w = _backward_error(Ho, dxu, dxv, rel, ab)
         print(f"\n{precision} rel={rel} abs={ab:.3g}: componentwise backward error {w:.2e}")
         assert w <= (1e-12 if precision == "fp64" else 1e-10), w
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_large_s_triangular_solves_with_shared_owners(precision):
+    """3 000 cameras (n = 17 994, 282 tile rows): more destination blocks of
+    the triangular solves than owner workgroups (255), so owners take several
+    destinations; the FP64 and the refined mixed increment still solve the
+    oracle's damped normal equations (same backward-error bars)."""
+    g = generate_ba(dict(CONFIGS["cfg2"], n_cameras=3000, n_points=60000, arc=150, seed=12))
+    o = OracleBA.from_generated(g)
+    o.update(True)
+    Ho = oracle_blocks(o)
+    p = BAProblem.from_generated(g, precision=precision)
+    p.update(True)
+    s = SolverCholeskySchur()
+    s.init(p)
+    mu = 1e-4 * s.maxAbsHDiag()
+    assert s.computeIncrement(0.0, mu)
+    dxu, dxv = s.incrementVector()
+    w = _backward_error(Ho, dxu, dxv, 0.0, mu)
+    print(f"\n{precision} n={6 * (g['n_cameras'] - 1)}: componentwise backward error {w:.2e}")
+    assert w <= (1e-12 if precision == "fp64" else 1e-10), w
