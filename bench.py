@@ -263,8 +263,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded generator)",
-        "config": {"workload": f"{args.config}: BA {g['n_cameras']} cameras / {g['n_points']} points / "
-                               f"{g['n_edges']} observations, S {n}x{n}, FP64 Schur LM",
+        "config": {"workload": f"BA {g['n_cameras']} cameras / {g['n_points']} points / "
+                               f"{g['n_edges']} observations (ring of cameras), S {n}x{n}, FP64 Schur LM",
+                   "preset": args.config,
                    "cameras": g["n_cameras"], "points": g["n_points"], "observations": g["n_edges"],
                    "lm_iterations": args.steps},
         "fp64": r64,
