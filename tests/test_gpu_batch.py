@@ -132,3 +132,22 @@ def test_batch_graph_replays():
             assert np.array_equal(chi2, first[0]) and np.array_equal(cams, first[1]) and np.array_equal(pts, first[2])
         for w in range(len(wins)):
             assert abs(chi2[w] - ref[w]) <= 1e-9 * ref[w], (rep, w, chi2[w], ref[w])
+
+
+@pytest.mark.parametrize("variant", [8, 64])
+def test_batch_degenerate_windows(variant):
+    """Windows with no variable camera (S of order 0), with one variable
+    camera and single observations, beside an ordinary one: the same final
+    cost and termination as the single-problem path."""
+    wins = []
+    for w, (nc, npt) in enumerate([(1, 30), (2, 20), (5, 60)]):
+        wins.append(generate_ba(dict(CONFIGS["cfg1"], n_cameras=nc, n_points=npt, k_min=1 if nc < 3 else 3,
+                                     k_max=min(nc, 4), seed=300 + w)))
+    b = BABatch(wins, precision="fp64")
+    b.set_config(variant, 3)
+    chi2, _ = b.lm_solve(LevenbergMarquardtSettings(maxNumIt=5))
+    reasons, _, _ = b.results()
+    for w, g in enumerate(wins):
+        c1, r1, _ = _single(g, maxNumIt=5)
+        assert abs(chi2[w] - c1) <= 1e-9 * c1, (w, chi2[w], c1)
+        assert reasons[w] == r1, (w, reasons[w], r1)
