@@ -126,6 +126,17 @@ def batch_timing(nsn, n_windows=512, iters=10):
             "config": b.config(), "create_s": t1 - t0}
 
 
+def measured_peaks(nsn):
+    """The device's own ceilings: FP64 / FP32 16x16x4 MFMA rate on register
+    operands and a 1 GiB streaming copy (nsn_measure_peaks)."""
+    import ctypes as C
+    lib = nsn.capi.lib()
+    f64, f32, bw = C.c_double(0), C.c_double(0), C.c_double(0)
+    nsn.capi.check(lib.nsn_measure_peaks(0, C.byref(f64), C.byref(f32), C.byref(bw)))
+    return {"mfma_f64_tflops": f64.value, "mfma_f32_tflops": f32.value, "copy_gbps": bw.value,
+            "datasheet": {"mfma_f64_tflops": FP64_PEAK_TFLOPS, "mfma_f32_tflops": 157.3, "hbm_gbps": 8000.0}}
+
+
 def cpu_baseline(g, budget_s):
     """The oracle (C restatement of the reference path, one thread) on a
     bounded sample of one LM iteration of the same workload: update(true),
@@ -210,8 +221,12 @@ def main():
         with open(tfile) as f:
             t = json.load(f)
         traffic = t["hbm_read_bytes_per_launch"] + t["hbm_written_bytes_per_launch"]
+    peaks = measured_peaks(nsn)
+    log(f"bench: measured peaks {peaks}")
     roofline = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
+                "measured_peak": peaks["mfma_f64_tflops"],
+                "frac_of_measured_peak": achieved / peaks["mfma_f64_tflops"] if peaks["mfma_f64_tflops"] else None,
                 "traffic_note": "HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, copy-calibrated), from "
                                 "profiles/traffic_k_chol_cfg2_fp64.json",
                 "kernel": "k_chol<double,64> (tile Cholesky of S, n=%d, n^3/3 flop)" % n,
@@ -270,6 +285,7 @@ def main():
                    "lm_iterations": args.steps},
         "fp64": r64,
         "phases_ms": ph,
+        "measured_peaks": peaks,
         "mixed": mixed,
         "mixed_small": mixed_small,
         "adjoint": adjoint,

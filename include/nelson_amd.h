@@ -252,6 +252,13 @@ int nsn_batch_set_config(nsn_batch* b, int variant, int grid);
 /* the launch shape in use and the tuner's timings (ms, in candidate order) */
 int nsn_batch_config(nsn_batch* b, int* variant, int* grid, double* tune_ms, int n_tune);
 
+/* ---- measured ceilings (no counterpart in the reference) ----
+ * For stating roofline fractions against the device itself as well as
+ * against its datasheet: the FP64 and FP32 rate of the 16x16x4 MFMA
+ * instructions on register operands (TFLOP/s) and the bandwidth of a 1 GiB
+ * streaming copy (GB/s, bytes read + written).  Any output may be NULL. */
+int nsn_measure_peaks(int device, double* mfma_f64_tflops, double* mfma_f32_tflops, double* copy_gbps);
+
 #ifdef __cplusplus
 }
 #endif
