@@ -709,6 +709,9 @@ struct or_ba {
   double *cams, *pts, *cams_bak, *pts_bak;   /* cams: 12 per camera (R, t) */
   int* edge_w;                /* edge -> W block or -1 */
   double *er, *eJc, *eJp, *eom; /* per-edge r (2), Jc (12), Jp (6), IRLS weight */
+  int ncp, npp;                 /* unary priors (or_ba_set_priors) */
+  int *cp_id, *pp_id;
+  double *cp_p0, *cp_w, *pp_p0, *pp_w;
   or_hessian H;
 };
 
@@ -784,8 +787,101 @@ void or_ba_destroy(or_ba* b) {
   free(b->cam); free(b->pt); free(b->obs); free(b->w); free(b->cam_var); free(b->pt_var);
   free(b->cams); free(b->pts); free(b->cams_bak); free(b->pts_bak); free(b->edge_w);
   free(b->er); free(b->eJc); free(b->eJp); free(b->eom);
+  free(b->cp_id); free(b->cp_p0); free(b->cp_w); free(b->pp_id); free(b->pp_p0); free(b->pp_w);
   or_hessian_free(&b->H);
   free(b);
+}
+
+/* principal log of a rotation: theta = atan2(|vee(R - R^T)| / 2, (tr R - 1) / 2),
+ * omega = theta / (2 sin theta) vee(R - R^T); near theta = pi the axis comes
+ * from a a^T = ((R + R^T) / 2 - cos theta I) / (1 - cos theta) (its largest
+ * diagonal entry's column), signed like vee(R - R^T) */
+void or_so3_log(const double* R, double* w) {
+  double v[3] = {R[7] - R[5], R[2] - R[6], R[3] - R[1]};
+  double c = 0.5 * (R[0] + R[4] + R[8] - 1.0);
+  double s = 0.5 * sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  double th = atan2(s, c);
+  if (th < 1e-4) {
+    double k = 0.5 + th * th / 12.0;
+    for (int i = 0; i < 3; i++) w[i] = k * v[i];
+  } else if (3.14159265358979323846 - th > 1e-4) {
+    double k = th / (2.0 * s);
+    for (int i = 0; i < 3; i++) w[i] = k * v[i];
+  } else {
+    double B[9];
+    for (int i = 0; i < 3; i++)
+      for (int k = 0; k < 3; k++) B[i * 3 + k] = (0.5 * (R[i * 3 + k] + R[k * 3 + i]) - (i == k ? c : 0.0)) / (1.0 - c);
+    int j = 0;
+    for (int i = 1; i < 3; i++) if (B[i * 4] > B[j * 4]) j = i;
+    double a[3];
+    double d = sqrt(B[j * 4]);
+    for (int i = 0; i < 3; i++) a[i] = B[i * 3 + j] / d;
+    double sg = a[0] * v[0] + a[1] * v[1] + a[2] * v[2] < 0.0 ? -1.0 : 1.0;
+    for (int i = 0; i < 3; i++) w[i] = sg * th * a[i];
+  }
+}
+
+/* Jl^-1(w) = I - K / 2 + (1 / th^2 - (1 + cos th) / (2 th sin th)) K^2, K = [w]x */
+void or_so3_jl_inv(const double* w, double* J) {
+  double th2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+  double th = sqrt(th2);
+  double g = th < 1e-4 ? 1.0 / 12.0 + th2 / 720.0 : 1.0 / th2 - (1.0 + cos(th)) / (2.0 * th * sin(th));
+  double K[9] = {0, -w[2], w[1], w[2], 0, -w[0], -w[1], w[0], 0};
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      double k2 = K[i * 3 + 0] * K[0 * 3 + j] + K[i * 3 + 1] * K[1 * 3 + j] + K[i * 3 + 2] * K[2 * 3 + j];
+      J[i * 3 + j] = (i == j ? 1.0 : 0.0) - 0.5 * K[i * 3 + j] + g * k2;
+    }
+}
+
+void or_ba_set_priors(or_ba* b, int ncp, const int* cam_ids, const double* cam_p0, const double* cam_w,
+                      int npp, const int* pt_ids, const double* pt_p0, const double* pt_w) {
+  free(b->cp_id); free(b->cp_p0); free(b->cp_w); free(b->pp_id); free(b->pp_p0); free(b->pp_w);
+  b->ncp = ncp; b->npp = npp;
+  b->cp_id = (int*)malloc(sizeof(int) * (size_t)(ncp + 1));
+  b->cp_p0 = (double*)malloc(sizeof(double) * 6 * (size_t)(ncp + 1));
+  b->cp_w = (double*)malloc(sizeof(double) * 6 * (size_t)(ncp + 1));
+  b->pp_id = (int*)malloc(sizeof(int) * (size_t)(npp + 1));
+  b->pp_p0 = (double*)malloc(sizeof(double) * 3 * (size_t)(npp + 1));
+  b->pp_w = (double*)malloc(sizeof(double) * 3 * (size_t)(npp + 1));
+  if (ncp) {
+    memcpy(b->cp_id, cam_ids, sizeof(int) * (size_t)ncp);
+    memcpy(b->cp_p0, cam_p0, sizeof(double) * 6 * (size_t)ncp);
+    memcpy(b->cp_w, cam_w, sizeof(double) * 6 * (size_t)ncp);
+  }
+  if (npp) {
+    memcpy(b->pp_id, pt_ids, sizeof(int) * (size_t)npp);
+    memcpy(b->pp_p0, pt_p0, sizeof(double) * 3 * (size_t)npp);
+    memcpy(b->pp_w, pt_w, sizeof(double) * 3 * (size_t)npp);
+  }
+}
+
+/* camera prior k: residual r (6) and, with jac, J (6x6 row-major, weighted) */
+static double cam_prior(const or_ba* b, int k, int jac, double* r, double* J) {
+  const double* R = b->cams + 12 * b->cp_id[k];
+  const double* t = R + 9;
+  const double* p0 = b->cp_p0 + 6 * k;
+  const double* w = b->cp_w + 6 * k;
+  double om[3];
+  or_so3_log(R, om);
+  double cost = 0.0;
+  for (int i = 0; i < 3; i++) r[i] = w[i] * (om[i] - p0[i]);
+  for (int i = 0; i < 3; i++) r[3 + i] = w[3 + i] * (t[i] - p0[3 + i]);
+  for (int i = 0; i < 6; i++) cost += r[i] * r[i];
+  if (jac) {
+    double Jl[9];
+    or_so3_jl_inv(om, Jl);
+    /* d t / d omega = -[t]x, d t / d upsilon = I */
+    double Tx[9] = {0, t[2], -t[1], -t[2], 0, t[0], t[1], -t[0], 0};
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) {
+        J[i * 6 + j] = w[i] * Jl[i * 3 + j];
+        J[i * 6 + 3 + j] = 0.0;
+        J[(3 + i) * 6 + j] = w[3 + i] * Tx[i * 3 + j];
+        J[(3 + i) * 6 + 3 + j] = (i == j) ? w[3 + i] : 0.0;
+      }
+  }
+  return cost;
 }
 
 /* the BA reprojection edge: r = pi(K, R X + t) - z, pinhole with focal f and
@@ -844,6 +940,16 @@ void or_ba_update(or_ba* b, int hessian) {
   double chi2 = 0;
   for (int e = 0; e < b->E; e++)
     chi2 += ba_edge(b, e, hessian, b->er + 2 * e, b->eJc + 12 * e, b->eJp + 6 * e, b->eom + e);
+  /* unary priors after the binary edges: cameras, then points */
+  double rc[6], Jcp[36];
+  for (int k = 0; k < b->ncp; k++) chi2 += cam_prior(b, k, 0, rc, Jcp);
+  for (int k = 0; k < b->npp; k++) {
+    const double* X = b->pts + 3 * b->pp_id[k];
+    for (int i = 0; i < 3; i++) {
+      double ri = b->pp_w[3 * k + i] * (X[i] - b->pp_p0[3 * k + i]);
+      chi2 += ri * ri;
+    }
+  }
   H->chi2 = chi2;
   if (!hessian) return;
   size_t nu = (size_t)6 * H->NU;
@@ -871,6 +977,32 @@ void or_ba_update(or_ba* b, int hessian) {
       double* W = H->W + (size_t)b->edge_w[e] * 18;
       for (int i = 0; i < 6; i++)
         for (int j = 0; j < 3; j++) W[i * 3 + j] += om * (J[i] * G[j] + J[6 + i] * G[3 + j]);
+    }
+  }
+  /* the priors' diagonal blocks and gradient segments (variable nodes) */
+  for (int k = 0; k < b->ncp; k++) {
+    int ci = b->cam_var[b->cp_id[k]];
+    if (ci < 0) continue;
+    cam_prior(b, k, 1, rc, Jcp);
+    for (int i = 0; i < 6; i++) {
+      for (int j = i; j < 6; j++) {
+        double h = 0.0;
+        for (int m = 0; m < 6; m++) h += Jcp[m * 6 + i] * Jcp[m * 6 + j];
+        H->U[((size_t)ci * 6 + i) * nu + (size_t)ci * 6 + j] += h;
+      }
+      double g = 0.0;
+      for (int m = 0; m < 6; m++) g += Jcp[m * 6 + i] * rc[m];
+      H->bU[(size_t)ci * 6 + i] += g;
+    }
+  }
+  for (int k = 0; k < b->npp; k++) {
+    int pj = b->pt_var[b->pp_id[k]];
+    if (pj < 0) continue;
+    const double* X = b->pts + 3 * b->pp_id[k];
+    for (int i = 0; i < 3; i++) {
+      double w = b->pp_w[3 * k + i];
+      H->V[(size_t)pj * 9 + i * 4] += w * w;
+      H->bV[(size_t)pj * 3 + i] += w * w * (X[i] - b->pp_p0[3 * k + i]);
     }
   }
 }

@@ -147,6 +147,21 @@ void or_ba_edge_jacobians(or_ba* b, double* r, double* Jc, double* Jp);
 int  or_ba_num_var_cams(const or_ba* b);
 int  or_ba_num_var_pts(const or_ba* b);
 
+/* unary prior edges (the BA instance of nelson's EdgeUnary,
+ * nelson/EdgeUnary.h:76-93: one residual on one node, accumulated into that
+ * node's diagonal block and gradient segment).  Camera prior k on user
+ * camera cam_ids[k]: r = w .* [log(R) - omega0; t - t0] (6 values of cam_p0
+ * and cam_w per prior); point prior k: r = w .* (X - X0).  Jacobians w.r.t.
+ * the section's perturbations: camera [[Jl^-1(log R), 0], [-[t]x, I]]
+ * (left SE(3) update), point I.  A new call replaces the previous priors;
+ * counts of 0 remove them. */
+void or_ba_set_priors(or_ba* b, int ncp, const int* cam_ids, const double* cam_p0, const double* cam_w,
+                      int npp, const int* pt_ids, const double* pt_p0, const double* pt_w);
+/* log of a rotation matrix (principal angle-axis) and the inverse left
+ * Jacobian of SO(3) at omega (row-major 3x3) */
+void or_so3_log(const double* R, double* w);
+void or_so3_jl_inv(const double* w, double* J);
+
 /* rotation helpers (own restatement of Rodrigues / SE(3) exp) */
 void or_rodrigues(const double* w, double* R);
 void or_se3_left_update(const double* delta, double* R, double* t);

@@ -105,6 +105,9 @@ def lib():
             "or_ba_edge_jacobians": (None, [C.c_void_p, _D, _D, _D]),
             "or_ba_num_var_cams": (C.c_int, [C.c_void_p]),
             "or_ba_num_var_pts": (C.c_int, [C.c_void_p]),
+            "or_ba_set_priors": (None, [C.c_void_p, C.c_int, _I, _D, _D, C.c_int, _I, _D, _D]),
+            "or_so3_log": (None, [_D, _D]),
+            "or_so3_jl_inv": (None, [_D, _D]),
             "or_rodrigues": (None, [_D, _D]),
             "or_se3_left_update": (None, [_D, _D, _D]),
         }
@@ -122,6 +125,22 @@ def dp(a):
 
 def ip(a):
     return None if a is None else a.ctypes.data_as(_I)
+
+
+def so3_log(R):
+    """The oracle's principal log of a 3x3 rotation (angle-axis)."""
+    R = np.ascontiguousarray(R, np.float64).reshape(9)
+    w = np.zeros(3)
+    lib().or_so3_log(dp(R), dp(w))
+    return w
+
+
+def so3_jl_inv(w):
+    """The oracle's inverse left Jacobian of SO(3) at w (3x3)."""
+    w = np.ascontiguousarray(w, np.float64)
+    J = np.zeros(9)
+    lib().or_so3_jl_inv(dp(w), dp(J))
+    return J.reshape(3, 3)
 
 
 def hessian_views(h):
@@ -264,6 +283,15 @@ class OracleBA:
         bS = np.zeros(max(nu, 1))
         ok = lib().or_schur_assemble(h, rel, abs_, dp(S), dp(bS))
         return dict(ok=bool(ok), S=S, bS=bS[:nu])
+
+    def set_priors(self, cam_ids=(), cam_p0=(), cam_w=(), pt_ids=(), pt_p0=(), pt_w=()):
+        """Unary priors (or_ba_set_priors): per camera 6 values (omega0, t0)
+        and 6 weights, per point 3 and 3."""
+        self._pri = [np.ascontiguousarray(cam_ids, np.int32), np.ascontiguousarray(cam_p0, np.float64).ravel(),
+                     np.ascontiguousarray(cam_w, np.float64).ravel(), np.ascontiguousarray(pt_ids, np.int32),
+                     np.ascontiguousarray(pt_p0, np.float64).ravel(), np.ascontiguousarray(pt_w, np.float64).ravel()]
+        ci, cp, cw, pi, pp, pw = self._pri
+        lib().or_ba_set_priors(self._h, len(ci), ip(ci), dp(cp), dp(cw), len(pi), ip(pi), dp(pp), dp(pw))
 
     def oplus(self, xU, xV):
         lib().or_ba_oplus(self._h, dp(np.ascontiguousarray(xU)), dp(np.ascontiguousarray(xV)))

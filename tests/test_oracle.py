@@ -430,3 +430,94 @@ def test_lm_converges_cfg1(cfg1):
     # costs of accepted steps decrease monotonically
     acc = [e[2] for e in r["log"][1:] if e[2] >= 0]
     assert all(b <= a * (1 + 1e-12) or b > a for a, b in zip(acc, acc[1:]))
+
+
+# ---- unary priors -------------------------------------------------------------
+
+def test_so3_log_and_inverse_left_jacobian():
+    """or_so3_log against scipy's rotation vector (angles from 1e-9 to
+    pi - 1e-9); or_so3_jl_inv against the central difference of
+    log(exp(d) R) in d."""
+    rng = np.random.default_rng(3)
+    for k in range(600):
+        ax = rng.normal(size=3)
+        ax /= np.linalg.norm(ax)
+        th = [rng.uniform(0, np.pi), np.pi - 10 ** rng.uniform(-9, -2), 10 ** rng.uniform(-9, -2)][k % 3]
+        R = Rotation.from_rotvec(th * ax).as_matrix()
+        assert np.abs(orc.so3_log(R) - Rotation.from_rotvec(th * ax).as_rotvec()).max() <= 1e-10
+    for w in (np.array([0.3, -0.2, 0.5]), np.array([1e-7, -2e-7, 3e-7]), np.array([1.0, 2.0, -1.5])):
+        R = Rotation.from_rotvec(w).as_matrix()
+        J = np.zeros((3, 3))
+        h = 1e-6
+        for i in range(3):
+            d = np.zeros(3)
+            d[i] = h
+            J[:, i] = (orc.so3_log(Rotation.from_rotvec(d).as_matrix() @ R)
+                       - orc.so3_log(Rotation.from_rotvec(-d).as_matrix() @ R)) / (2 * h)
+        assert np.abs(orc.so3_jl_inv(w) - J).max() <= 1e-7, (w, orc.so3_jl_inv(w), J)
+
+
+def _priors_cfg1():
+    g = generate_ba(CONFIGS["cfg1"])
+    rng = np.random.default_rng(4)
+    cam_ids = np.array([0, 2, 5, 7], np.int32)          # camera 0 is fixed (constant cost only)
+    cam_p0 = np.asarray(g["cams0"]).reshape(-1, 6)[cam_ids] + rng.normal(0, [0.02] * 3 + [0.1] * 3, (4, 6))
+    cam_w = rng.uniform(0.5, 3.0, (4, 6))
+    pt_ids = np.array([0, 3, 17, 250, 499], np.int32)   # point 0 is fixed
+    pt_p0 = np.asarray(g["pts0"]).reshape(-1, 3)[pt_ids] + rng.normal(0, 0.2, (5, 3))
+    pt_w = rng.uniform(0.5, 3.0, (5, 3))
+    return g, dict(cam_ids=cam_ids, cam_p0=cam_p0, cam_w=cam_w, pt_ids=pt_ids, pt_p0=pt_p0, pt_w=pt_w)
+
+
+def _prior_residuals(cams_rt, pts, pri):
+    """The priors' residuals from the state, with scipy's log (independent
+    of the oracle)."""
+    r = []
+    for k, c in enumerate(pri["cam_ids"]):
+        R = cams_rt[12 * c:12 * c + 9].reshape(3, 3)
+        t = cams_rt[12 * c + 9:12 * c + 12]
+        e = np.concatenate([Rotation.from_matrix(R).as_rotvec(), t]) - pri["cam_p0"][k]
+        r.append(pri["cam_w"][k] * e)
+    for k, q in enumerate(pri["pt_ids"]):
+        r.append(pri["pt_w"][k] * (pts[3 * q:3 * q + 3] - pri["pt_p0"][k]))
+    return np.concatenate(r)
+
+
+def test_priors_cost_gradient_hessian_vs_finite_differences():
+    """The priors add sum |r|^2 to the cost, J^T r to b and J^T J to the
+    diagonal blocks of the variable nodes, J the central difference of the
+    residuals under the section's oplus (cameras exp(d) T, points additive)."""
+    g, pri = _priors_cfg1()
+    o0 = OracleBA.from_generated(g)
+    o0.update(True)
+    H0 = oracle_blocks(o0)
+    o = OracleBA.from_generated(g)
+    o.set_priors(**pri)
+    o.update(True)
+    H1 = oracle_blocks(o)
+    cams, pts = o.get_state_rt()
+    r = _prior_residuals(cams, pts, pri)
+    assert abs((H1["chi2"] - H0["chi2"]) - float(r @ r)) <= 1e-12 * H1["chi2"]
+    NU, NV = o.NU, o.NV
+    J = np.zeros((len(r), 6 * NU + 3 * NV))
+    h = 1e-6
+    for i in range(6 * NU + 3 * NV):
+        for s in (1, -1):
+            o.set_state_rt(cams, pts)
+            xu, xv = np.zeros(6 * NU), np.zeros(3 * NV)
+            (xu if i < 6 * NU else xv)[i if i < 6 * NU else i - 6 * NU] = s * h
+            o.oplus(xu, xv)
+            c2, p2 = o.get_state_rt()
+            J[:, i] += s * _prior_residuals(c2, p2, pri) / (2 * h)
+    o.set_state_rt(cams, pts)
+    dU = np.array([H1["U"][u] - H0["U"][u] for u in range(NU)])
+    JtJ = J.T @ J
+    for u in range(NU):
+        assert np.abs(dU[u] - JtJ[6 * u:6 * u + 6, 6 * u:6 * u + 6]).max() <= 1e-6 * max(1.0, np.abs(dU[u]).max())
+    for q in range(NV):
+        blk = JtJ[6 * NU + 3 * q:6 * NU + 3 * q + 3, 6 * NU + 3 * q:6 * NU + 3 * q + 3]
+        assert np.abs((H1["V"][q] - H0["V"][q]) - blk).max() <= 1e-6 * max(1.0, np.abs(blk).max())
+    Jtr = J.T @ r
+    assert np.abs((H1["bU"] - H0["bU"]) - Jtr[:6 * NU]).max() <= 1e-6 * max(1.0, np.abs(Jtr).max())
+    assert np.abs((H1["bV"] - H0["bV"]) - Jtr[6 * NU:]).max() <= 1e-6 * max(1.0, np.abs(Jtr).max())
+    assert np.array_equal(H1["W"], H0["W"])
