@@ -252,6 +252,19 @@ int nsn_batch_set_config(nsn_batch* b, int variant, int grid);
 /* the launch shape in use and the tuner's timings (ms, in candidate order) */
 int nsn_batch_config(nsn_batch* b, int* variant, int* grid, double* tune_ms, int n_tune);
 
+/* ---- unary priors (nelson's EdgeUnary, EdgeUnary.h:76-93, for the BA
+ * section) ----
+ * Camera prior k on user camera cam_ids[k]: residual w .* [log(R) - omega0;
+ * t - t0] with cam_p0 = (omega0 angle-axis, t0) and cam_w 6 weights per
+ * prior; point prior k: w .* (X - X0), pt_p0 / pt_w 3 per prior.  Each adds
+ * |r|^2 to the cost and J^T J, J^T r to its node's diagonal block and
+ * gradient (variable nodes), J w.r.t. the section's perturbations (cameras:
+ * left SE(3) update).  At most one prior per node; a new call replaces all
+ * priors (counts 0 remove them); the next linearisation uses them.  Batched
+ * windows take no priors. */
+int nsn_set_priors(nsn_problem* h, int n_cam, const int* cam_ids, const double* cam_p0, const double* cam_w,
+                   int n_pt, const int* pt_ids, const double* pt_p0, const double* pt_w);
+
 /* ---- measured ceilings (no counterpart in the reference) ----
  * For stating roofline fractions against the device itself as well as
  * against its datasheet: the FP64 and FP32 rate of the 16x16x4 MFMA

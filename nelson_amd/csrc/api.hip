@@ -103,6 +103,7 @@ struct nsn_problem {
   hipEvent_t ev[8] = {};
   hipGraphExec_t lm_graph = nullptr;
   bool lm_graph_tried = false;
+  std::unique_ptr<DevBuf> pri_mem;  // unary priors (nsn_set_priors)
 
   ~nsn_problem() {
     if (lm_graph) (void)hipGraphExecDestroy(lm_graph);
@@ -730,6 +731,70 @@ extern "C" int nsn_phase_times(nsn_problem* h, double* ms, int n) {
     HIP_TRY(hipEventElapsedTime(&t, h->ev[pairs[i][0]], h->ev[pairs[i][1]]));
     ms[i] = t;
   }
+  return NSN_OK;
+}
+
+extern "C" int nsn_set_priors(nsn_problem* h, int n_cam, const int* cam_ids, const double* cam_p0,
+                              const double* cam_w, int n_pt, const int* pt_ids, const double* pt_p0,
+                              const double* pt_w) {
+  if (!h) return fail(NSN_EINVAL, "null argument");
+  if (n_cam < 0 || n_pt < 0) return fail(NSN_EINVAL, "nsn_set_priors: negative count");
+  if ((n_cam > 0 && (!cam_ids || !cam_p0 || !cam_w)) || (n_pt > 0 && (!pt_ids || !pt_p0 || !pt_w)))
+    return fail(NSN_EINVAL, "nsn_set_priors: missing arrays");
+  std::vector<int> cidx(h->C, -1), pidx(h->P, -1);
+  for (int k = 0; k < n_cam; k++) {
+    const int c = cam_ids[k];
+    if (c < 0 || c >= h->C) return fail(NSN_EINVAL, "nsn_set_priors: camera prior %d on camera %d out of range", k, c);
+    if (cidx[c] >= 0) return fail(NSN_EINVAL, "nsn_set_priors: two priors on camera %d", c);
+    for (int i = 0; i < 6; i++)
+      if (!std::isfinite(cam_p0[6 * (size_t)k + i]) || !std::isfinite(cam_w[6 * (size_t)k + i]))
+        return fail(NSN_EINVAL, "nsn_set_priors: camera prior %d is not finite", k);
+    cidx[c] = k;
+  }
+  for (int k = 0; k < n_pt; k++) {
+    const int q = pt_ids[k];
+    if (q < 0 || q >= h->P) return fail(NSN_EINVAL, "nsn_set_priors: point prior %d on point %d out of range", k, q);
+    if (pidx[q] >= 0) return fail(NSN_EINVAL, "nsn_set_priors: two priors on point %d", q);
+    for (int i = 0; i < 3; i++)
+      if (!std::isfinite(pt_p0[3 * (size_t)k + i]) || !std::isfinite(pt_w[3 * (size_t)k + i]))
+        return fail(NSN_EINVAL, "nsn_set_priors: point prior %d is not finite", k);
+    pidx[q] = k;
+  }
+  HIP_TRY(hipSetDevice(h->device));
+  HIP_TRY(hipStreamSynchronize(h->st));  // the old prior arrays may be in use
+  std::unique_ptr<DevBuf> m(new DevBuf());
+  BAView& v = h->view;
+  v.cam_prior = nullptr; v.cam_prior_p = nullptr; v.cam_prior_w = nullptr;
+  v.pt_prior = nullptr; v.pt_prior_p = nullptr; v.pt_prior_w = nullptr;
+  int rc;
+  if (n_cam > 0) {
+    int* di = m->alloc<int>(h->C);
+    double* dp = m->alloc<double>(6 * (size_t)n_cam);
+    double* dw = m->alloc<double>(6 * (size_t)n_cam);
+    if (!di || !dp || !dw) return fail(NSN_ENOMEM, "nsn_set_priors: device allocation failed");
+    if ((rc = upload(di, cidx)) || (rc = upload(dp, std::vector<double>(cam_p0, cam_p0 + 6 * (size_t)n_cam))) ||
+        (rc = upload(dw, std::vector<double>(cam_w, cam_w + 6 * (size_t)n_cam))))
+      return rc;
+    v.cam_prior = di; v.cam_prior_p = dp; v.cam_prior_w = dw;
+  }
+  if (n_pt > 0) {
+    int* di = m->alloc<int>(h->P);
+    double* dp = m->alloc<double>(3 * (size_t)n_pt);
+    double* dw = m->alloc<double>(3 * (size_t)n_pt);
+    if (!di || !dp || !dw) return fail(NSN_ENOMEM, "nsn_set_priors: device allocation failed");
+    if ((rc = upload(di, pidx)) || (rc = upload(dp, std::vector<double>(pt_p0, pt_p0 + 3 * (size_t)n_pt))) ||
+        (rc = upload(dw, std::vector<double>(pt_w, pt_w + 3 * (size_t)n_pt))))
+      return rc;
+    v.pt_prior = di; v.pt_prior_p = dp; v.pt_prior_w = dw;
+  }
+  h->pri_mem = std::move(m);
+  if (h->lm_graph) {  // the captured LM sub-iteration holds the old view
+    (void)hipGraphExecDestroy(h->lm_graph);
+    h->lm_graph = nullptr;
+  }
+  h->lm_graph_tried = false;
+  h->linearized = false;
+  h->stepped = false;
   return NSN_OK;
 }
 

@@ -167,6 +167,81 @@ __device__ __forceinline__ void oplus_cam(const BAView& v, int idx, const double
 }
 
 // points are additive
+// principal log of a rotation (R row-major): theta = atan2(|vee(R - R^T)| / 2,
+// (tr R - 1) / 2), omega = theta / (2 sin theta) vee(R - R^T); near theta = pi
+// the axis comes from a a^T = ((R + R^T) / 2 - cos theta I) / (1 - cos theta)
+__device__ inline void so3_log(const double* R, double* w) {
+  const double v0 = R[7] - R[5], v1 = R[2] - R[6], v2 = R[3] - R[1];
+  const double c = 0.5 * (R[0] + R[4] + R[8] - 1.0);
+  const double s = 0.5 * sqrt(v0 * v0 + v1 * v1 + v2 * v2);
+  const double th = atan2(s, c);
+  if (th < 1e-4) {
+    const double k = 0.5 + th * th / 12.0;
+    w[0] = k * v0; w[1] = k * v1; w[2] = k * v2;
+  } else if (3.14159265358979323846 - th > 1e-4) {
+    const double k = th / (2.0 * s);
+    w[0] = k * v0; w[1] = k * v1; w[2] = k * v2;
+  } else {
+    double B[9];
+    for (int i = 0; i < 3; i++)
+      for (int k = 0; k < 3; k++) B[i * 3 + k] = (0.5 * (R[i * 3 + k] + R[k * 3 + i]) - (i == k ? c : 0.0)) / (1.0 - c);
+    int j = 0;
+    for (int i = 1; i < 3; i++)
+      if (B[i * 4] > B[j * 4]) j = i;
+    const double d = sqrt(B[j * 4]);
+    const double a0 = B[j] / d, a1 = B[3 + j] / d, a2 = B[6 + j] / d;
+    const double sg = a0 * v0 + a1 * v1 + a2 * v2 < 0.0 ? -1.0 : 1.0;
+    w[0] = sg * th * a0; w[1] = sg * th * a1; w[2] = sg * th * a2;
+  }
+}
+
+// inverse left Jacobian of SO(3): I - K / 2 + (1 / th^2 - (1 + cos th) / (2 th sin th)) K^2, K = [w]x
+__device__ inline void so3_jl_inv(const double* w, double* J) {
+  const double th2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+  const double th = sqrt(th2);
+  const double g = th < 1e-4 ? 1.0 / 12.0 + th2 / 720.0 : 1.0 / th2 - (1.0 + cos(th)) / (2.0 * th * sin(th));
+  const double K[9] = {0, -w[2], w[1], w[2], 0, -w[0], -w[1], w[0], 0};
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      const double k2 = K[i * 3 + 0] * K[0 * 3 + j] + K[i * 3 + 1] * K[1 * 3 + j] + K[i * 3 + 2] * K[2 * 3 + j];
+      J[i * 3 + j] = (i == j ? 1.0 : 0.0) - 0.5 * K[i * 3 + j] + g * k2;
+    }
+}
+
+// unary prior k on user camera c (EdgeUnary, nelson/EdgeUnary.h:76-93):
+// r = w .* [log(R) - omega0; t - t0]; with JAC the weighted Jacobian (6x6
+// row-major) w.r.t. the left update exp(d) T: [[Jl^-1(log R), 0], [-[t]x, I]].
+// Returns |r|^2.
+template <bool JAC>
+__device__ inline double cam_prior_eval(const BAView& v, int c, int k, double* r, double* J) {
+  const double* cm = v.cams + 12 * (size_t)c;
+  const double* p0 = v.cam_prior_p + 6 * (size_t)k;
+  const double* w = v.cam_prior_w + 6 * (size_t)k;
+  double R[9], om[3];
+  for (int i = 0; i < 9; i++) R[i] = cm[i];
+  so3_log(R, om);
+  double cost = 0.0;
+  for (int i = 0; i < 3; i++) {
+    r[i] = w[i] * (om[i] - p0[i]);
+    r[3 + i] = w[3 + i] * (cm[9 + i] - p0[3 + i]);
+  }
+  for (int i = 0; i < 6; i++) cost += r[i] * r[i];
+  if (JAC) {
+    double Jl[9];
+    so3_jl_inv(om, Jl);
+    const double t0 = cm[9], t1 = cm[10], t2 = cm[11];
+    const double Tx[9] = {0, t2, -t1, -t2, 0, t0, t1, -t0, 0};
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) {
+        J[i * 6 + j] = w[i] * Jl[i * 3 + j];
+        J[i * 6 + 3 + j] = 0.0;
+        J[(3 + i) * 6 + j] = w[3 + i] * Tx[i * 3 + j];
+        J[(3 + i) * 6 + 3 + j] = (i == j) ? w[3 + i] : 0.0;
+      }
+  }
+  return cost;
+}
+
 __device__ __forceinline__ void oplus_pt(const BAView& v, int q, const double* __restrict__ xV) {
   const int p = v.var_pt_user[q];
 #pragma unroll
@@ -311,6 +386,24 @@ __device__ __forceinline__ void lin_point(const BAView& v, int p, T* __restrict_
         for (int i = 0; i < 6; i++)
 #pragma unroll
           for (int j = 0; j < 3; j++) w[i * 3 + j] += om * (j0[i] * g0[j] + j1[i] * g1[j]);
+      }
+    }
+  }
+  // the point's unary prior (after its edges): r = w .* (X - X0), J = diag(w)
+  if (v.pt_prior) {
+    const int k = v.pt_prior[p];
+    if (k >= 0) {
+      const double* x0 = v.pt_prior_p + 3 * (size_t)k;
+      const double* wp = v.pt_prior_w + 3 * (size_t)k;
+      const double Xp[3] = {X0, X1, X2};
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        const double e = Xp[i] - x0[i], w2 = wp[i] * wp[i];
+        chi2 += w2 * e * e;
+        if (HESS && pv >= 0) {
+          vv[i == 0 ? 0 : (i == 1 ? 3 : 5)] += (T)w2;
+          bv[i] += (T)(w2 * e);
+        }
       }
     }
   }

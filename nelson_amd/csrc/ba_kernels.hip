@@ -36,6 +36,13 @@ __global__ __launch_bounds__(256) void k_lin_points(BAView v, T* __restrict__ V,
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   double chi2 = 0.0, mb = 0.0, md = 0.0;
   if (p < v.P) lin_point<T, HESS>(v, p, V, bV, W, chi2, mb, md);
+  if (v.cam_prior) {  // the cameras' unary priors (cost)
+    for (int c = p; c < v.C; c += gridDim.x * blockDim.x) {
+      const int k = v.cam_prior[c];
+      double r[6];
+      if (k >= 0) chi2 += cam_prior_eval<false>(v, c, k, r, nullptr);
+    }
+  }
   double s = block_sum256(chi2, sh);
   if (threadIdx.x == 0) part_chi2[blockIdx.x] = s;
   if (HESS) {
@@ -58,8 +65,27 @@ __global__ __launch_bounds__(256) void k_lin_cams(BAView v, T* __restrict__ U, T
                                                   int gate) {
   if (!gate_open(ctl, gate)) return;
   __shared__ double red[4][27];
+  __shared__ double pri[27];  // the camera's unary prior: J^T J (21 packed), J^T r
   const int u = blockIdx.x;
   const int c = v.var_cam_user[u];
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < 27; i++) pri[i] = 0.0;
+    const int k = v.cam_prior ? v.cam_prior[c] : -1;
+    if (k >= 0) {
+      double r[6], J[36];
+      cam_prior_eval<true>(v, c, k, r, J);
+      for (int i = 0; i < 6; i++) {
+        for (int j = i; j < 6; j++) {
+          double h = 0.0;
+          for (int m = 0; m < 6; m++) h += J[m * 6 + i] * J[m * 6 + j];
+          pri[up6(i, j)] = h;
+        }
+        double g = 0.0;
+        for (int m = 0; m < 6; m++) g += J[m * 6 + i] * r[m];
+        pri[21 + i] = g;
+      }
+    }
+  }
   double R[12];
 #pragma unroll
   for (int i = 0; i < 12; i++) R[i] = v.cams[12 * c + i];
@@ -89,16 +115,17 @@ __global__ __launch_bounds__(256) void k_lin_cams(BAView v, T* __restrict__ U, T
   __syncthreads();
   if (threadIdx.x < 27) {
     const int i = threadIdx.x;
-    const double s = ((red[0][i] + red[1][i]) + red[2][i]) + red[3][i];
+    const double s = (((red[0][i] + red[1][i]) + red[2][i]) + red[3][i]) + pri[i];
     if (i < 21) U[21 * (size_t)u + i] = (T)s;
     else bU[6 * (size_t)u + (i - 21)] = (T)s;
   }
   if (threadIdx.x == 0) {
     double mb = 0.0, md = 0.0;
     for (int i = 0; i < 6; i++) {
-      mb = fmax(mb, fabs((double)(T)(((red[0][21 + i] + red[1][21 + i]) + red[2][21 + i]) + red[3][21 + i])));
+      mb = fmax(mb, fabs((double)(T)((((red[0][21 + i] + red[1][21 + i]) + red[2][21 + i]) + red[3][21 + i]) +
+                                      pri[21 + i])));
       const int d = up6(i, i);
-      md = fmax(md, fabs((double)(T)(((red[0][d] + red[1][d]) + red[2][d]) + red[3][d])));
+      md = fmax(md, fabs((double)(T)((((red[0][d] + red[1][d]) + red[2][d]) + red[3][d]) + pri[d])));
     }
     atomic_max_nonneg(maxes + 0, mb);
     atomic_max_nonneg(maxes + 2, md);
@@ -363,8 +390,21 @@ __global__ __launch_bounds__(256) void k_ir_res_points(BAView v, const double* _
       dg[k] += om * (g0[k] * g0[k] + g1[k] * g1[k]);
     }
   }
-  const double rel = ctl->rel, ab = ctl->abs_;
   const double d[3] = {d0, d1, d2};
+  if (v.pt_prior) {  // the point's unary prior: J = diag(w)
+    const int k = v.pt_prior[p];
+    if (k >= 0) {
+      const double Xp[3] = {X0, X1, X2};
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        const double w2 = v.pt_prior_w[3 * (size_t)k + i] * v.pt_prior_w[3 * (size_t)k + i];
+        b[i] += w2 * (Xp[i] - v.pt_prior_p[3 * (size_t)k + i]);
+        hq[i] += w2 * d[i];
+        dg[i] += w2;
+      }
+    }
+  }
+  const double rel = ctl->rel, ab = ctl->abs_;
 #pragma unroll
   for (int k = 0; k < 3; k++) {
     rV[3 * (size_t)pv + k] = -b[k] - hq[k] - (rel * dg[k] + ab) * d[k];
@@ -378,6 +418,7 @@ __global__ __launch_bounds__(256) void k_ir_res_cams(BAView v, const double* __r
                                                      double* __restrict__ bU64, DevCtl* ctl, int gate) {
   if (!gate_open(ctl, gate)) return;
   __shared__ double red[4][18];
+  __shared__ double pri[18];  // the camera's unary prior: J^T r, J^T J x, diag(J^T J)
   const int u = blockIdx.x;
   const int c = v.var_cam_user[u];
   double R[12];
@@ -386,6 +427,27 @@ __global__ __launch_bounds__(256) void k_ir_res_cams(BAView v, const double* __r
   double xc[6];
 #pragma unroll
   for (int i = 0; i < 6; i++) xc[i] = xU[6 * (size_t)u + i];
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < 18; i++) pri[i] = 0.0;
+    const int kp = v.cam_prior ? v.cam_prior[c] : -1;
+    if (kp >= 0) {
+      double r[6], J[36];
+      cam_prior_eval<true>(v, c, kp, r, J);
+#pragma unroll
+      for (int m = 0; m < 6; m++) {
+        double jx = 0.0;
+#pragma unroll
+        for (int n = 0; n < 6; n++) jx += J[m * 6 + n] * xc[n];
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+          pri[i] += J[m * 6 + i] * r[m];
+          pri[6 + i] += J[m * 6 + i] * jx;
+          pri[12 + i] += J[m * 6 + i] * J[m * 6 + i];
+        }
+      }
+    }
+  }
   double acc[18];
 #pragma unroll
   for (int i = 0; i < 18; i++) acc[i] = 0.0;
@@ -432,6 +494,9 @@ __global__ __launch_bounds__(256) void k_ir_res_cams(BAView v, const double* __r
     double t[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) t[k] = ((red[0][6 * k + i] + red[1][6 * k + i]) + red[2][6 * k + i]) + red[3][6 * k + i];
+    t[0] += pri[i];
+    t[1] += pri[6 + i];
+    t[2] += pri[12 + i];
     const double rel = ctl->rel, ab = ctl->abs_;
     rU[6 * (size_t)u + i] = -t[0] - t[1] - (rel * t[2] + ab) * xc[i];
     bU64[6 * (size_t)u + i] = t[0];

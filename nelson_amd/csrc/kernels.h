@@ -46,6 +46,15 @@ struct BAView {
   const int* cw_idx;        // [nW]
   double* cams;             // [12C] R (row-major) and t per camera
   double* pts;              // [3P]
+  // unary priors (nsn_set_priors; null pointers: none): per user camera /
+  // point the index of its prior or -1; camera priors omega0, t0 and 6
+  // weights, point priors X0 and 3 weights
+  const int* cam_prior;       // [C]
+  const double* cam_prior_p;  // [6 ncp]
+  const double* cam_prior_w;  // [6 ncp]
+  const int* pt_prior;        // [P]
+  const double* pt_prior_p;   // [3 npp]
+  const double* pt_prior_w;   // [3 npp]
 };
 
 // Tiled lower storage of the reduced camera system S (plus the augmented

@@ -189,6 +189,21 @@ class BAProblem:
         check(self._lib.nsn_get_state_rt(self._h, _dp(cams), _dp(pts)))
         return cams, pts
 
+    def set_priors(self, cam_ids=(), cam_p0=(), cam_w=(), pt_ids=(), pt_p0=(), pt_w=()):
+        """Unary priors (nsn_set_priors): per camera the angle-axis and
+        translation it is pulled to (6 values) and 6 weights, per point 3 and
+        3; replaces earlier priors, used from the next linearisation."""
+        ci = np.ascontiguousarray(cam_ids, np.int32).reshape(-1)
+        cp = np.ascontiguousarray(cam_p0, np.float64).reshape(-1)
+        cw = np.ascontiguousarray(cam_w, np.float64).reshape(-1)
+        pi = np.ascontiguousarray(pt_ids, np.int32).reshape(-1)
+        pp = np.ascontiguousarray(pt_p0, np.float64).reshape(-1)
+        pw = np.ascontiguousarray(pt_w, np.float64).reshape(-1)
+        if cp.size != 6 * ci.size or cw.size != 6 * ci.size or pp.size != 3 * pi.size or pw.size != 3 * pi.size:
+            raise ValueError("priors: 6 values and 6 weights per camera, 3 and 3 per point")
+        check(self._lib.nsn_set_priors(self._h, ci.size, _ip(ci), _dp(cp), _dp(cw), pi.size, _ip(pi), _dp(pp),
+                                       _dp(pw)))
+
     def set_observations(self, obs=None, weight=None):
         """Replaces the observations [E, 2] and / or per-edge weights [E]
         (user edge order) for the next linearisation."""
