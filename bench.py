@@ -126,6 +126,38 @@ def batch_timing(nsn, n_windows=512, iters=10):
             "config": b.config(), "create_s": t1 - t0}
 
 
+LADYBUG = os.path.join(ROOT, "tests", "golden", "ladybug_49_7776.txt")
+
+
+def ladybug_problem(nsn):
+    """The reference's BA sample workload: a BA problem on the ladybug
+    visibility graph (49 cameras / 7 776 points / 31 843 observations) with
+    the sample's geometry (nelson/samples/BASchurPerfomances.cpp:40-92,236-248)."""
+    return nsn.generate_on_graph(nsn.load_visibility(LADYBUG))
+
+
+def ladybug_timing(nsn, g):
+    """Gauss-Newton as the sample runs it (minNumIt = 5, other settings at
+    their defaults; BASchurPerfomances.cpp:266-280) in FP64, and LM to
+    convergence in FP64 and mixed precision, on the GPU."""
+    def timed(make, precision):
+        p = nsn.BAProblem.from_generated(g, precision=precision)
+        make().solve(p)  # warm: graphs captured, buffers allocated
+        p.set_state(g["cams0"], g["pts0"])
+        d = make()
+        t0 = time.perf_counter()
+        reason = d.solve(p)
+        t1 = time.perf_counter()
+        its = d.numIterations()
+        return dict(reason=reason, iterations=its, final_chi2=d.result()["final_chi2"],
+                    wall_s=t1 - t0, it_per_s=its / (t1 - t0))
+    gn = timed(lambda: nsn.GaussNewton(nsn.GaussNewtonSettings(minNumIt=5)), "fp64")
+    lm64 = timed(lambda: nsn.LevenbergMarquardt(nsn.LevenbergMarquardtSettings(maxNumIt=30)), "fp64")
+    lmm = timed(lambda: nsn.LevenbergMarquardt(nsn.LevenbergMarquardtSettings(maxNumIt=30)), "mixed")
+    return {"workload": f"BA on the ladybug visibility graph, {g['n_cameras']} cameras / {g['n_points']} points / "
+                        f"{g['n_edges']} observations, S 288x288", "gn_fp64": gn, "lm_fp64": lm64, "lm_mixed": lmm}
+
+
 def measured_peaks(nsn):
     """The device's own ceilings: FP64 / FP32 16x16x4 MFMA rate on register
     operands and a 1 GiB streaming copy (nsn_measure_peaks)."""
@@ -135,6 +167,20 @@ def measured_peaks(nsn):
     nsn.capi.check(lib.nsn_measure_peaks(0, C.byref(f64), C.byref(f32), C.byref(bw)))
     return {"mfma_f64_tflops": f64.value, "mfma_f32_tflops": f32.value, "copy_gbps": bw.value,
             "datasheet": {"mfma_f64_tflops": FP64_PEAK_TFLOPS, "mfma_f32_tflops": 157.3, "hbm_gbps": 8000.0}}
+
+
+def cpu_baseline_ladybug(g):
+    """The oracle (one thread) running the sample's Gauss-Newton on the
+    ladybug workload end to end."""
+    from oracle.oracle import OracleBA
+    o = OracleBA.from_generated(g)
+    t0 = time.perf_counter()
+    r = o.gn_solve(minNumIt=5)
+    t1 = time.perf_counter()
+    its = max(len(r["log"]) - 1, 1)
+    return dict(value=its / (t1 - t0), unit="GN iter/s", cores=1, kind="port", iterations=its,
+                reason=r["reason"], final_chi2=r["final_chi2"],
+                sample=f"the whole Gauss-Newton solve ({t1 - t0:.2f} s) on {cpu_model()}, 1 thread")
 
 
 def cpu_baseline(g, budget_s):
@@ -255,6 +301,9 @@ def main():
     if not args.no_batch:
         batch = batch_timing(nsn)
         log(f"bench: batch {batch}")
+    lb = ladybug_problem(nsn)
+    ladybug = ladybug_timing(nsn, lb)
+    log(f"bench: ladybug {ladybug}")
     adjoint = None
     if not args.no_adjoint:
         adjoint = adjoint_timing(nsn)
@@ -262,7 +311,9 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(g, args.cpu_budget)
-        log(f"bench: cpu {cpu}")
+        ladybug["cpu_baseline"] = cpu_baseline_ladybug(lb)
+        ladybug["gn_gpu_over_cpu"] = ladybug["gn_fp64"]["it_per_s"] / ladybug["cpu_baseline"]["value"]
+        log(f"bench: cpu {cpu} ladybug {ladybug['cpu_baseline']}")
     its = r64["iterations"]
     value = its / r64["wall_s"]
     out = {
@@ -290,6 +341,7 @@ def main():
         "mixed_small": mixed_small,
         "adjoint": adjoint,
         "batch": batch,
+        "ladybug": ladybug,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -129,6 +129,34 @@ int nsn_generate_ba(const nsn_gen_params* p, int edge_cap,
                     double* cams_gt, double* pts_gt, double* cams0, double* pts0,
                     int* cam_idx, int* pt_idx, double* obs, int* n_edges);
 
+/* Synthetic BA on a given visibility graph (camera-point pairs), with the
+ * geometry of the reference's BA sample (BASchurPerfomances.cpp:40-92, which
+ * builds its problem on the ladybug graph at :236-248): points uniform in
+ * [-point_radius, point_radius]^3, cameras at cam_radius from the origin
+ * looking at it with a random roll; obs = projection + N(0, obs_sigma);
+ * initial state = ground truth perturbed except camera 0 and point 0.
+ * cam_radius must exceed sqrt(3) point_radius.  The sample's values:
+ * focal 100, principal point (50, 50), point_radius 1, cam_radius 2. */
+typedef struct {
+  double focal, cx, cy;
+  double point_radius, cam_radius;
+  double obs_sigma;
+  double rot_sigma, trans_sigma, point_sigma;
+  unsigned long long seed;
+} nsn_graph_gen_params;
+
+int nsn_generate_on_graph(const nsn_graph_gen_params* p, int n_cameras, int n_points, int n_edges,
+                          const int* cam_idx, const int* pt_idx, double* cams_gt, double* pts_gt,
+                          double* cams0, double* pts0, double* obs);
+
+/* Problem::load (nelson/samples/Problems.cpp:8-30): reads a visibility-graph
+ * text file "nCameras nPoints nEdges" + nEdges "camera point" pairs.  With
+ * cam_idx and pt_idx NULL it reads the header only (sizes); otherwise it
+ * fills up to capacity pairs.  Unlike the reference it rejects malformed or
+ * short files and out-of-range pairs (NSN_EINVAL, message names the entry). */
+int nsn_visibility_load(const char* path, int capacity, int* n_cameras, int* n_points, int* n_edges,
+                        int* cam_idx, int* pt_idx);
+
 /* ---- problem -------------------------------------------------------------- */
 /* parametersReady + addEdge* + structureReady (DoubleSection.hpp:67-246,338-365) */
 int nsn_problem_create(const nsn_problem_desc* d, nsn_problem** out);

@@ -12,6 +12,6 @@ from .capi import (  # noqa: F401
 )
 from .problem import (  # noqa: F401
     BAProblem, SolverCholeskySchur, LevenbergMarquardt, LevenbergMarquardtSettings, GaussNewton,
-    GaussNewtonSettings, generate_ba, window_config, CONFIGS,
+    GaussNewtonSettings, generate_ba, window_config, CONFIGS, load_visibility, generate_on_graph, LADYBUG_SAMPLE,
 )
 from .batch import BABatch  # noqa: F401

@@ -16,7 +16,7 @@ LIB = os.path.join(HERE, "libnelson_amd.so")
 
 ARCH = os.environ.get("NSN_OFFLOAD_ARCH", "gfx950")
 HIP_SOURCES = ["ba_kernels.hip", "chol_kernels.hip", "api.hip", "batch_kernels.hip", "batch_api.hip", "calib.hip"]
-CPP_SOURCES = ["generate.cpp", "util.cpp", "structure.cpp"]
+CPP_SOURCES = ["generate.cpp", "util.cpp", "structure.cpp", "io.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-I", os.path.join(ROOT, "include"), "-I", CSRC]
 # packed FP32 instructions are disabled for all device code

@@ -74,6 +74,12 @@ class GenParams(C.Structure):
                 ("focal", C.c_double), ("cx", C.c_double), ("cy", C.c_double), ("seed", C.c_ulonglong)]
 
 
+class GraphGenParams(C.Structure):
+    _fields_ = [("focal", C.c_double), ("cx", C.c_double), ("cy", C.c_double), ("point_radius", C.c_double),
+                ("cam_radius", C.c_double), ("obs_sigma", C.c_double), ("rot_sigma", C.c_double),
+                ("trans_sigma", C.c_double), ("point_sigma", C.c_double), ("seed", C.c_ulonglong)]
+
+
 _P = C.c_void_p
 _D = C.POINTER(C.c_double)
 _I = C.POINTER(C.c_int)
@@ -86,6 +92,9 @@ SIGNATURES = {
     "nsn_lm_default_settings": (None, [C.POINTER(LMSettings)]),
     "nsn_gn_default_settings": (None, [C.POINTER(GNSettings)]),
     "nsn_generate_ba": (C.c_int, [C.POINTER(GenParams), C.c_int, _D, _D, _D, _D, _I, _I, _D, _I]),
+    "nsn_generate_on_graph": (C.c_int, [C.POINTER(GraphGenParams), C.c_int, C.c_int, C.c_int, _I, _I, _D, _D, _D,
+                                        _D, _D]),
+    "nsn_visibility_load": (C.c_int, [C.c_char_p, C.c_int, _I, _I, _I, _I, _I]),
     "nsn_problem_create": (C.c_int, [C.POINTER(ProblemDesc), C.POINTER(_P)]),
     "nsn_problem_destroy": (C.c_int, [_P]),
     "nsn_problem_info_get": (C.c_int, [_P, C.POINTER(ProblemInfo)]),

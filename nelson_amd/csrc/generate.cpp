@@ -134,3 +134,80 @@ extern "C" int nsn_generate_ba(const nsn_gen_params* p, int edge_cap, double* ca
     for (int i = 0; i < 3; i++) pts0[3 * q + i] += p->point_sigma * rng.normal();
   return NSN_OK;
 }
+
+// Synthetic BA on a given visibility graph: the geometry of the reference's BA
+// sample (nelson/samples/BASchurPerfomances.cpp:40-75): points uniform in the
+// cube [-point_radius, point_radius]^3, cameras at cam_radius from the origin
+// in a uniformly random direction, looking at the origin with a random roll.
+// Observations are the projections plus N(0, obs_sigma) pixel noise; the
+// initial state perturbs every camera but 0 and every point but 0 (the
+// sample's addNoise, :77-92, with normal instead of uniform draws).  Draw
+// order: points, cameras (direction, then roll vector), observation noise in
+// edge order, camera perturbations, point perturbations.
+extern "C" int nsn_generate_on_graph(const nsn_graph_gen_params* p, int n_cameras, int n_points, int n_edges,
+                                     const int* cam_idx, const int* pt_idx, double* cams_gt, double* pts_gt,
+                                     double* cams0, double* pts0, double* obs) {
+  if (!p || !cam_idx || !pt_idx || !cams_gt || !pts_gt || !cams0 || !pts0 || !obs)
+    return nsn::fail(NSN_EINVAL, "nsn_generate_on_graph: null argument");
+  const int C = n_cameras, P = n_points, E = n_edges;
+  if (C < 1 || P < 1 || E < 0) return nsn::fail(NSN_EINVAL, "nsn_generate_on_graph: bad sizes");
+  if (!(p->point_radius > 0) || !(p->cam_radius > std::sqrt(3.0) * p->point_radius))
+    return nsn::fail(NSN_EINVAL,
+                     "nsn_generate_on_graph: cam_radius must exceed sqrt(3) point_radius (every point in front)");
+  for (int e = 0; e < E; e++)
+    if (cam_idx[e] < 0 || cam_idx[e] >= C || pt_idx[e] < 0 || pt_idx[e] >= P)
+      return nsn::fail(NSN_EINVAL, "nsn_generate_on_graph: edge %d (%d, %d) out of range", e, cam_idx[e], pt_idx[e]);
+  SplitMix64 rng(p->seed);
+  for (int q = 0; q < P; q++)
+    for (int i = 0; i < 3; i++) pts_gt[3 * q + i] = p->point_radius * (2.0 * rng.uniform() - 1.0);
+  std::vector<double> R(9 * (size_t)C), t(3 * (size_t)C);
+  auto unit = [&rng](double v[3]) {
+    double n2 = 0.0;
+    while (n2 < 1e-12) {
+      for (int i = 0; i < 3; i++) v[i] = 2.0 * rng.uniform() - 1.0;
+      n2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    }
+    const double n = std::sqrt(n2);
+    for (int i = 0; i < 3; i++) v[i] /= n;
+  };
+  for (int c = 0; c < C; c++) {
+    double dir[3], roll[3];
+    unit(dir);
+    double z[3] = {-dir[0], -dir[1], -dir[2]};
+    double dot = 1.0;
+    while (1.0 - std::fabs(dot) < 1e-6) {
+      unit(roll);
+      dot = roll[0] * z[0] + roll[1] * z[1] + roll[2] * z[2];
+    }
+    double y[3] = {z[1] * roll[2] - z[2] * roll[1], z[2] * roll[0] - z[0] * roll[2], z[0] * roll[1] - z[1] * roll[0]};
+    double ny = std::sqrt(y[0] * y[0] + y[1] * y[1] + y[2] * y[2]);
+    for (double& v : y) v /= ny;
+    double x[3] = {y[1] * z[2] - y[2] * z[1], y[2] * z[0] - y[0] * z[2], y[0] * z[1] - y[1] * z[0]};
+    double* Rc = &R[9 * (size_t)c];
+    for (int j = 0; j < 3; j++) { Rc[0 * 3 + j] = x[j]; Rc[1 * 3 + j] = y[j]; Rc[2 * 3 + j] = z[j]; }
+    const double centre[3] = {p->cam_radius * dir[0], p->cam_radius * dir[1], p->cam_radius * dir[2]};
+    for (int i = 0; i < 3; i++)
+      t[3 * c + i] = -(Rc[i * 3 + 0] * centre[0] + Rc[i * 3 + 1] * centre[1] + Rc[i * 3 + 2] * centre[2]);
+    nsn::rot_log(Rc, cams_gt + 6 * (size_t)c);
+    for (int i = 0; i < 3; i++) cams_gt[6 * (size_t)c + 3 + i] = t[3 * c + i];
+  }
+  for (int e = 0; e < E; e++) {
+    const int c = cam_idx[e], q = pt_idx[e];
+    const double* Rc = &R[9 * (size_t)c];
+    const double* X = pts_gt + 3 * (size_t)q;
+    double Xc[3];
+    for (int r = 0; r < 3; r++)
+      Xc[r] = Rc[r * 3 + 0] * X[0] + Rc[r * 3 + 1] * X[1] + Rc[r * 3 + 2] * X[2] + t[3 * c + r];
+    obs[2 * (size_t)e] = p->focal * Xc[0] / Xc[2] + p->cx + p->obs_sigma * rng.normal();
+    obs[2 * (size_t)e + 1] = p->focal * Xc[1] / Xc[2] + p->cy + p->obs_sigma * rng.normal();
+  }
+  std::memcpy(cams0, cams_gt, sizeof(double) * 6 * (size_t)C);
+  std::memcpy(pts0, pts_gt, sizeof(double) * 3 * (size_t)P);
+  for (int c = 1; c < C; c++) {
+    for (int i = 0; i < 3; i++) cams0[6 * (size_t)c + i] += p->rot_sigma * rng.normal();
+    for (int i = 0; i < 3; i++) cams0[6 * (size_t)c + 3 + i] += p->trans_sigma * rng.normal();
+  }
+  for (int q = 1; q < P; q++)
+    for (int i = 0; i < 3; i++) pts0[3 * (size_t)q + i] += p->point_sigma * rng.normal();
+  return NSN_OK;
+}

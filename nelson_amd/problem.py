@@ -94,6 +94,48 @@ def generate_ba(cfg):
     return out
 
 
+def load_visibility(path):
+    """Problem::load (nelson/samples/Problems.cpp:8-30): a visibility-graph
+    text file -> dict(n_cameras, n_points, cam_idx, pt_idx)."""
+    lib = capi.lib()
+    nc, npt, ne = C.c_int(0), C.c_int(0), C.c_int(0)
+    bpath = str(path).encode()
+    check(lib.nsn_visibility_load(bpath, 0, C.byref(nc), C.byref(npt), C.byref(ne), None, None))
+    ci = np.zeros(max(ne.value, 1), np.int32)
+    pi = np.zeros(max(ne.value, 1), np.int32)
+    check(lib.nsn_visibility_load(bpath, ne.value, C.byref(nc), C.byref(npt), C.byref(ne), _ip(ci), _ip(pi)))
+    return dict(n_cameras=nc.value, n_points=npt.value, cam_idx=ci[:ne.value], pt_idx=pi[:ne.value])
+
+
+# the reference's BA sample on a visibility graph
+# (nelson/samples/BASchurPerfomances.cpp:40-75: camera 100 x 100 px, focal 100,
+# points in the unit cube, cameras at focal * 1 / (100 / 2) = 2 from the origin)
+LADYBUG_SAMPLE = dict(focal=100.0, cx=50.0, cy=50.0, point_radius=1.0, cam_radius=2.0, obs_sigma=0.5,
+                      rot_sigma=0.002, trans_sigma=0.002, point_sigma=0.002, seed=49)
+
+
+def generate_on_graph(graph, **params):
+    """nsn_generate_on_graph on a visibility graph (load_visibility's dict);
+    params default to LADYBUG_SAMPLE.  Returns the dict of generate_ba."""
+    lib = capi.lib()
+    cfg = dict(LADYBUG_SAMPLE, **params)
+    p = capi.GraphGenParams()
+    for name, _ in capi.GraphGenParams._fields_:
+        setattr(p, name, cfg[name])
+    nc, npt = int(graph["n_cameras"]), int(graph["n_points"])
+    ci = np.ascontiguousarray(graph["cam_idx"], np.int32)
+    pi = np.ascontiguousarray(graph["pt_idx"], np.int32)
+    e = len(ci)
+    out = dict(cams_gt=np.zeros(6 * nc), pts_gt=np.zeros(3 * npt), cams0=np.zeros(6 * nc), pts0=np.zeros(3 * npt),
+               obs=np.zeros(max(2 * e, 1)))
+    check(lib.nsn_generate_on_graph(C.byref(p), nc, npt, e, _ip(ci), _ip(pi), _dp(out["cams_gt"]),
+                                    _dp(out["pts_gt"]), _dp(out["cams0"]), _dp(out["pts0"]), _dp(out["obs"])))
+    out["obs"] = out["obs"][:2 * e]
+    out.update(cam_idx=ci.copy(), pt_idx=pi.copy(), n_cameras=nc, n_points=npt, n_edges=e, focal=cfg["focal"],
+               cx=cfg["cx"], cy=cfg["cy"], huber_delta=0.0)
+    return out
+
+
 # ---------------------------------------------------------------------------
 # problem (DoubleSection + BA edges)
 # ---------------------------------------------------------------------------
