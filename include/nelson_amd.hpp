@@ -121,6 +121,25 @@ class BASection {
     obs_.push_back(v);
     w_.push_back(weight);
   }
+  // a unary prior edge (EdgeUnaryU / EdgeUnaryV, nelson/DoubleSection.h:361-366,
+  // nelson/EdgeUnary.h:76-93) with a diagonal weight: on camera i
+  // r = weight .* [log R - prior.w; t - prior.t], on point j r = weight .* (X - prior).
+  // At most one per node (NSN_EINVAL at structureReady otherwise).
+  void addEdgeUnaryU(int i, const Camera& prior, const std::array<double, 6>& weight) {
+    if (!ready_) throw Error(NSN_ESTATE, "addEdgeUnaryU before parametersReady");
+    if (h_) throw Error(NSN_ESTATE, "addEdgeUnaryU after structureReady");
+    priCam_.push_back(i);
+    for (int k = 0; k < 3; k++) priCamP_.push_back(prior.w[k]);
+    for (int k = 0; k < 3; k++) priCamP_.push_back(prior.t[k]);
+    priCamW_.insert(priCamW_.end(), weight.begin(), weight.end());
+  }
+  void addEdgeUnaryV(int j, const std::array<double, 3>& prior, const std::array<double, 3>& weight) {
+    if (!ready_) throw Error(NSN_ESTATE, "addEdgeUnaryV before parametersReady");
+    if (h_) throw Error(NSN_ESTATE, "addEdgeUnaryV after structureReady");
+    priPt_.push_back(j);
+    priPtP_.insert(priPtP_.end(), prior.begin(), prior.end());
+    priPtW_.insert(priPtW_.end(), weight.begin(), weight.end());
+  }
   void structureReady(Precision p = Precision::FP64, int device = 0) {
     if (!ready_) throw Error(NSN_ESTATE, "structureReady before parametersReady");
     nsn_problem_desc d{};
@@ -140,6 +159,10 @@ class BASection {
     d.precision = (int)p;
     d.device = device;
     check(nsn_problem_create(&d, &h_), "nsn_problem_create");
+    if (!priCam_.empty() || !priPt_.empty())
+      check(nsn_set_priors(h_, (int)priCam_.size(), priCam_.data(), priCamP_.data(), priCamW_.data(),
+                           (int)priPt_.size(), priPt_.data(), priPtP_.data(), priPtW_.data()),
+            "nsn_set_priors");
     upload();
   }
 
@@ -201,6 +224,8 @@ class BASection {
   std::vector<unsigned char> camFixed_, ptFixed_;
   std::vector<int> cam_, pt_;
   std::vector<double> obs_, w_;
+  std::vector<int> priCam_, priPt_;
+  std::vector<double> priCamP_, priCamW_, priPtP_, priPtW_;
   nsn_problem* h_ = nullptr;
 };
 

@@ -15,7 +15,7 @@ using namespace nelson_amd;
 
 static void build(BASection& pss, const std::vector<double>& cams, const std::vector<double>& pts,
                   const std::vector<int>& ci, const std::vector<int>& pi, const std::vector<double>& obs, int C,
-                  int P) {
+                  int P, bool priors = false) {
   pss.setNumParameters(C, P);
   for (int i = 0; i < C; i++)
     for (int k = 0; k < 3; k++) {
@@ -28,6 +28,26 @@ static void build(BASection& pss, const std::vector<double>& cams, const std::ve
   pss.setFixedV(0);
   pss.parametersReady();
   for (size_t e = 0; e < ci.size(); e++) pss.addEdge(ci[e], pi[e], obs[2 * e], obs[2 * e + 1]);
+  if (priors) {
+    for (int c : {0, 2, 5, 7}) {
+      BASection::Camera pr;
+      std::array<double, 6> w;
+      for (int k = 0; k < 6; k++) {
+        const double off = 0.01 * (k % 3 + 1) * (k < 3 ? 1.0 : 5.0);
+        (k < 3 ? pr.w[k] : pr.t[k - 3]) = cams[6 * c + k] + off;
+        w[k] = 0.5 + 0.25 * ((k + c) % 4);
+      }
+      pss.addEdgeUnaryU(c, pr, w);
+    }
+    for (int q : {0, 3, 17, 250, 499}) {
+      std::array<double, 3> pr, w;
+      for (int k = 0; k < 3; k++) {
+        pr[k] = pts[3 * q + k] + 0.01 * (k % 3 + 1);
+        w[k] = 0.5 + 0.25 * ((k + q) % 4);
+      }
+      pss.addEdgeUnaryV(q, pr, w);
+    }
+  }
   pss.structureReady();
 }
 
@@ -82,6 +102,14 @@ int main() {
       gn.settings().epsIncSquare = 0.0;
       const TerminationReason tc = gn.solve(pss);
       print_log("gn", tc, gn.numIterations(), pss.chi2(), gn.stats());
+    }
+    {
+      BASection pss(g.focal, g.cx, g.cy);
+      build(pss, c0, p0, ci, pi, obs, C, P, true);
+      LevenbergMarquardt<SolverCholeskySchur> lm;
+      lm.settings().maxNumIt = 20;
+      const TerminationReason tc = lm.solve(pss);
+      print_log("lm_priors", tc, lm.numIterations(), pss.update(false), lm.stats());
     }
   } catch (const Error& e) {
     std::fprintf(stderr, "nelson_amd error %d: %s\n", e.code(), e.what());

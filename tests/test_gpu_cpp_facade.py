@@ -58,3 +58,23 @@ def test_cpp_gn(results):
     assert len(d["log"]) == len(ro["log"])
     for a, b in zip(d["log"], ro["log"]):
         assert abs(a[2] - b[2]) <= 1e-9 * abs(b[2])
+
+
+def test_cpp_lm_with_unary_edges(results):
+    """addEdgeUnaryU / addEdgeUnaryV: the priors the program adds (its header
+    states them) on the oracle, the LM log and final cost to 1e-9."""
+    import numpy as np
+    g = generate_ba(CONFIGS["cfg1"])
+    cams0, pts0 = g["cams0"].reshape(-1, 6), g["pts0"].reshape(-1, 3)
+    cam_ids, pt_ids = np.array([0, 2, 5, 7]), np.array([0, 3, 17, 250, 499])
+    k6, k3 = np.arange(6), np.arange(3)
+    cam_p0 = cams0[cam_ids] + 0.01 * (k6 % 3 + 1) * np.where(k6 < 3, 1.0, 5.0)
+    cam_w = 0.5 + 0.25 * ((k6[None, :] + cam_ids[:, None]) % 4)
+    pt_p0 = pts0[pt_ids] + 0.01 * (k3 % 3 + 1)
+    pt_w = 0.5 + 0.25 * ((k3[None, :] + pt_ids[:, None]) % 4)
+    o = OracleBA.from_generated(g)
+    o.set_priors(cam_ids, cam_p0, cam_w, pt_ids, pt_p0, pt_w)
+    ro = o.lm_solve(maxNumIt=20)
+    d = results["lm_priors"]
+    detail = _compare_logs([tuple(e) for e in d["log"]], ro["log"], "cpp lm priors")
+    assert abs(d["final_chi2"] - ro["final_chi2"]) <= 1e-9 * ro["final_chi2"], detail
