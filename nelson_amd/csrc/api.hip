@@ -34,6 +34,9 @@ constexpr int kCholMaxWG = 256;  // persistent workgroups of the factorisation
 // owners, one per CU, so that all of them are resident together whenever one
 // of them owns several destinations
 constexpr int kTrsvMaxWG = 256;
+// the S assembly splits tiles into smaller parts until it has at least this
+// many workgroups (four per CU) or the parts are single camera blocks
+constexpr int kMinSParts = 1024;
 
 struct DevBuf {
   std::vector<void*> ptrs;
@@ -77,8 +80,10 @@ struct nsn_problem {
   double *xU = nullptr, *xV = nullptr;
   int* chol_tasks = nullptr;  // worker task list of the factorisation
   int n_chol_tasks = 0;
-  int *tile_order = nullptr, *tile_blk_ptr = nullptr, *tile_blk = nullptr, *sb_i = nullptr, *sb_j = nullptr,
-      *sb_ptr = nullptr, *sb_pairs = nullptr;
+  int *tile_order = nullptr, *sb_i = nullptr, *sb_j = nullptr, *sb_ptr = nullptr, *sb_pairs = nullptr;
+  // parts of the S assembly: tile, local rectangle (rows lo, hi, cols lo, hi), block list
+  int *part_tile = nullptr, *part_rect = nullptr, *part_ptr = nullptr, *part_blk = nullptr;
+  int n_s_parts = 0;
   int *flags = nullptr, *counters = nullptr;
   int nflags = 0, NR = 0;
   double* parts = nullptr;  // triangular-solve destination sums, NR x TB
@@ -165,8 +170,9 @@ void seq_schur(nsn_problem* h, int gate, int checks, bool timed) {
   if (timed) (void)hipEventRecord(h->ev[1], st);
   launch_s_diag<T>(h->view, (const T*)h->U, (const T*)h->bU, (const T*)h->Y, (const T*)h->tW, (T*)h->Sd,
                    (T*)h->bS, h->ctl, gate, st);
-  launch_s_tiles<T>(h->tv, h->tile_order, h->tile_blk_ptr, h->tile_blk, h->sb_i, h->sb_j, h->sb_ptr, h->sb_pairs,
-                    (const T*)h->Sd, (const T*)h->bS, (const T*)h->Y, (T*)h->tiles, h->ctl, gate, st);
+  launch_s_tiles<T>(h->tv, h->n_s_parts, h->tile_order, h->part_tile, h->part_rect, h->part_ptr, h->part_blk, h->sb_i,
+                    h->sb_j, h->sb_ptr, h->sb_pairs, (const T*)h->Sd, (const T*)h->bS, (const T*)h->Y,
+                    (T*)h->tiles, h->ctl, gate, st);
   if (timed) (void)hipEventRecord(h->ev[2], st);
   launch_chol_reset<T>(h->flags, h->nflags, h->counters, h->ctl, gate, st);
   {
@@ -273,22 +279,60 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
       tile_order[2 * t] = i;
       tile_order[2 * t + 1] = j;
     }
-  std::vector<int> tile_cnt(tv.ntiles + 1, 0);
-  auto for_tiles = [&](int sb, auto&& fn) {
+  // parts of the tiles for the S assembly: rectangles of a tile split at
+  // camera boundaries every g cameras in rows and columns (a large g: one part
+  // per tile).  A small S with many terms per block gets small parts, so the
+  // assembly spreads over the device instead of running in a few workgroups.
+  auto n_seg = [](int a, int len, int u) { return (a + len - 1) / u - a / u + 1; };
+  auto count_parts = [&](int g) {
+    const int u = 6 * g;
+    long long np = 0;
+    for (int t = 0; t < tv.ntiles; t++)
+      np += (long long)n_seg(tile_order[2 * t] * kTB, kTB, u) * n_seg(tile_order[2 * t + 1] * kTB, kTB, u);
+    return np;
+  };
+  int g = 1 << 20;
+  for (int cand : {1 << 20, 4, 2, 1}) {
+    g = cand;
+    if (count_parts(cand) >= kMinSParts) break;
+  }
+  const int u = 6 * g;
+  std::vector<int> part_base(tv.ntiles + 1, 0), part_tile, part_rect;
+  for (int t = 0; t < tv.ntiles; t++) {
+    const int r0 = tile_order[2 * t] * kTB, c0 = tile_order[2 * t + 1] * kTB;
+    const int nr = n_seg(r0, kTB, u), nc = n_seg(c0, kTB, u);
+    part_base[t + 1] = part_base[t] + nr * nc;
+    for (int a = 0; a < nr; a++)
+      for (int b = 0; b < nc; b++) {
+        const int rlo = std::max(r0, (r0 / u + a) * u), rhi = std::min(r0 + kTB, (r0 / u + a + 1) * u);
+        const int clo = std::max(c0, (c0 / u + b) * u), chi = std::min(c0 + kTB, (c0 / u + b + 1) * u);
+        part_tile.push_back(t);
+        part_rect.insert(part_rect.end(), {rlo - r0, rhi - r0, clo - c0, chi - c0});
+      }
+  }
+  const int nparts = part_base[tv.ntiles];
+  auto for_parts = [&](int sb, auto&& fn) {
     const int ci = sb_i[sb], cj = sb_j[sb];
     const int ti0 = (6 * cj) / kTB, ti1 = (6 * cj + 5) / kTB;
     const int tj0 = (6 * ci) / kTB, tj1 = (6 * ci + 5) / kTB;
     for (int ti = ti0; ti <= ti1; ti++)
       for (int tj = tj0; tj <= tj1; tj++)
-        if (ti >= tj) fn(tile_index(ti, tj, tv.NT));
+        if (ti >= tj) {
+          const long long t = tile_index(ti, tj, tv.NT);
+          const int r0 = ti * kTB, c0 = tj * kTB;
+          const int sr = std::max(6 * cj, r0) / u - r0 / u, sc = std::max(6 * ci, c0) / u - c0 / u;
+          fn(part_base[t] + sr * n_seg(c0, kTB, u) + sc);
+        }
   };
-  for (int sb = 0; sb < h->nsb; sb++) for_tiles(sb, [&](long long t) { tile_cnt[t + 1]++; });
-  for (int t = 0; t < tv.ntiles; t++) tile_cnt[t + 1] += tile_cnt[t];
-  std::vector<int> tile_blk(tile_cnt[tv.ntiles]);
+  std::vector<int> part_ptr(nparts + 1, 0);
+  for (int sb = 0; sb < h->nsb; sb++) for_parts(sb, [&](int q) { part_ptr[q + 1]++; });
+  for (int q = 0; q < nparts; q++) part_ptr[q + 1] += part_ptr[q];
+  std::vector<int> part_blk(std::max(part_ptr[nparts], 1));
   {
-    std::vector<int> fill(tile_cnt.begin(), tile_cnt.end() - 1);
-    for (int sb = 0; sb < h->nsb; sb++) for_tiles(sb, [&](long long t) { tile_blk[fill[t]++] = sb; });
+    std::vector<int> fill(part_ptr.begin(), part_ptr.end() - 1);
+    for (int sb = 0; sb < h->nsb; sb++) for_parts(sb, [&](int q) { part_blk[fill[q]++] = sb; });
   }
+  h->n_s_parts = nparts;
   h->NR = (n + kTB - 1) / kTB;
   // flags: ready (ntiles), the triangular solves' xready / tready (NR each),
   // pre_sub / pre_diag (NT each)
@@ -353,8 +397,10 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
   h->xV = a_dbl(3 * (size_t)NV);
   h->tile_order = a_int(tile_order.size());
   h->chol_tasks = a_int(chol_tasks.size());
-  h->tile_blk_ptr = a_int(tile_cnt.size());
-  h->tile_blk = a_int(tile_blk.size());
+  h->part_tile = a_int(part_tile.size());
+  h->part_rect = a_int(part_rect.size());
+  h->part_ptr = a_int(part_ptr.size());
+  h->part_blk = a_int(part_blk.size());
   h->sb_i = a_int(sb_i.size());
   h->sb_j = a_int(sb_j.size());
   h->sb_ptr = a_int(sb_ptr.size());
@@ -385,7 +431,8 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
       (rc = upload(d_ce_pt, ce_pt)) || (rc = upload(d_ce_obs, ce_obs)) || (rc = upload(d_ce_w, ce_w)) ||
       (rc = upload(d_wptr, wptr)) || (rc = upload(d_w_cam, h->w_cam)) || (rc = upload(d_w_pt, h->w_pt)) || (rc = upload(d_cw_ptr, cw_ptr)) ||
       (rc = upload(d_cw_idx, cw_idx)) || (rc = upload(h->tile_order, tile_order)) || (rc = upload(h->chol_tasks, chol_tasks)) ||
-      (rc = upload(h->tile_blk_ptr, tile_cnt)) || (rc = upload(h->tile_blk, tile_blk)) ||
+      (rc = upload(h->part_tile, part_tile)) || (rc = upload(h->part_rect, part_rect)) ||
+      (rc = upload(h->part_ptr, part_ptr)) || (rc = upload(h->part_blk, part_blk)) ||
       (rc = upload(h->sb_i, sb_i)) || (rc = upload(h->sb_j, sb_j)) || (rc = upload(h->sb_ptr, sb_ptr)) ||
       (rc = upload(h->sb_pairs, sb_pairs)))
     return rc;

@@ -235,10 +235,13 @@ __global__ __launch_bounds__(256) void k_s_diag(BAView v, const T* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// S tiles: one workgroup per lower TB x TB tile of [S; bS^T]. The tile is
-// built in LDS (zeros, identity padding, the bS row, diagonal blocks from Sd,
-// off-diagonal blocks S_ij = -sum_p Z_ip Z_jp^T over their pair lists,
-// MatrixWWtMultiplier.hpp:124-153) and written once, coalesced.
+// S tiles: one workgroup per part of a lower TB x TB tile of [S; bS^T]: a
+// rectangle of the tile (rows [rlo, rhi), columns [clo, chi), local) cut at
+// camera boundaries, the whole tile when S has enough tiles to fill the
+// device.  The part is built in LDS (zeros, identity padding, the bS row,
+// diagonal blocks from Sd, off-diagonal blocks S_ij = -sum_p Z_ip Z_jp^T over
+// their pair lists, MatrixWWtMultiplier.hpp:124-153) and written once,
+// coalesced along the tile's columns.
 // An upper block (i, j), i <= j, is stored transposed at rows 6j.., cols 6i..
 // A wave builds one block at a time: lane (s, a), s < kSlots, a < 6, sums row
 // a of the products of the pairs p = s, s + kSlots, ... (kSlots pairs in
@@ -249,8 +252,10 @@ constexpr int kSlots = 10;
 
 template <typename T, int TB>
 __global__ __launch_bounds__(256) void k_s_tiles(TileView tv, const int* __restrict__ tile_order,
-                                                 const int* __restrict__ tile_blk_ptr,
-                                                 const int* __restrict__ tile_blk, const int* __restrict__ sb_i,
+                                                 const int* __restrict__ part_tile,
+                                                 const int* __restrict__ part_rect,
+                                                 const int* __restrict__ part_ptr,
+                                                 const int* __restrict__ part_blk, const int* __restrict__ sb_i,
                                                  const int* __restrict__ sb_j, const int* __restrict__ sb_ptr,
                                                  const int* __restrict__ sb_pairs, const T* __restrict__ Sd,
                                                  const T* __restrict__ bS, const T* __restrict__ Z,
@@ -258,11 +263,15 @@ __global__ __launch_bounds__(256) void k_s_tiles(TileView tv, const int* __restr
   if (!gate_open(ctl, gate)) return;
   __shared__ T lt[TB * TB];
   __shared__ T red[4][kSlots * 36];
-  const int t = blockIdx.x;
+  const int q = blockIdx.x;
+  const int t = part_tile[q];
+  const int rlo = part_rect[4 * q], rhi = part_rect[4 * q + 1];
+  const int clo = part_rect[4 * q + 2], chi = part_rect[4 * q + 3];
+  const int hr = rhi - rlo, npe = hr * (chi - clo);
   const int ti = tile_order[2 * t], tj = tile_order[2 * t + 1];
   const int r0 = ti * TB, c0 = tj * TB, n = tv.n;
-  for (int e = threadIdx.x; e < TB * TB; e += 256) {
-    const int lr = e % TB, lc = e / TB;
+  for (int e = threadIdx.x; e < npe; e += 256) {
+    const int lr = rlo + e % hr, lc = clo + e / hr;
     const int R = r0 + lr, Cc = c0 + lc;
     T val = (T)0;
     if (R == Cc && R >= n) val = (T)1;
@@ -274,14 +283,15 @@ __global__ __launch_bounds__(256) void k_s_tiles(TileView tv, const int* __restr
   const int slot = lane / 6, ra = lane - 6 * (lane / 6);  // partial-sum lanes (slot < kSlots)
   const int a = lane / 6, bb = lane % 6;                   // output lanes (lane < 36)
   T* rd = red[wv];
-  const int b1 = tile_blk_ptr[t + 1];
-  for (int b = tile_blk_ptr[t] + wv; b < b1; b += 4) {
-    const int sb = tile_blk[b];
+  const int b1 = part_ptr[q + 1];
+  for (int b = part_ptr[q] + wv; b < b1; b += 4) {
+    const int sb = part_blk[b];
     const int i = sb_i[sb], j = sb_j[sb];
     const int R = 6 * j + bb, Cc = 6 * i + a;
-    const bool inside = lane < 36 && R >= r0 && R < r0 + TB && Cc >= c0 && Cc < c0 + TB;
+    const bool inside = lane < 36 && R >= r0 + rlo && R < r0 + rhi && Cc >= c0 + clo && Cc < c0 + chi;
+    const int at = (Cc - c0 - clo) * hr + (R - r0 - rlo);
     if (i == j) {
-      if (inside && bb >= a) lt[(Cc - c0) * TB + (R - r0)] = Sd[21 * (size_t)i + up6(a, bb)];
+      if (inside && bb >= a) lt[at] = Sd[21 * (size_t)i + up6(a, bb)];
       continue;
     }
     T acc[6] = {(T)0, (T)0, (T)0, (T)0, (T)0, (T)0};
@@ -304,13 +314,13 @@ __global__ __launch_bounds__(256) void k_s_tiles(TileView tv, const int* __restr
       T sum = (T)0;
 #pragma unroll
       for (int s2 = 0; s2 < kSlots; s2++) sum += rd[(s2 * 6 + a) * 6 + bb];
-      lt[(Cc - c0) * TB + (R - r0)] = -sum;
+      lt[at] = -sum;
     }
     wave_lds_sync();
   }
   __syncthreads();
   T* out = tiles + (size_t)t * TB * TB;
-  for (int e = threadIdx.x; e < TB * TB; e += 256) out[e] = lt[e];
+  for (int e = threadIdx.x; e < npe; e += 256) out[(clo + e / hr) * TB + rlo + e % hr] = lt[e];
 }
 
 // ---------------------------------------------------------------------------
@@ -839,15 +849,16 @@ void launch_s_diag(const BAView& v, const T* U, const T* bU, const T* Z, const T
 }
 
 template <typename T>
-void launch_s_tiles(const TileView& tv, const int* tile_order, const int* tile_blk_ptr, const int* tile_blk,
-                    const int* sb_i, const int* sb_j, const int* sb_ptr, const int* sb_pairs, const T* Sd,
-                    const T* bS, const T* Z, T* tiles, DevCtl* ctl, int gate, hipStream_t st) {
+void launch_s_tiles(const TileView& tv, int nparts, const int* tile_order, const int* part_tile, const int* part_rect,
+                    const int* part_ptr, const int* part_blk, const int* sb_i, const int* sb_j, const int* sb_ptr,
+                    const int* sb_pairs, const T* Sd, const T* bS, const T* Z, T* tiles, DevCtl* ctl, int gate,
+                    hipStream_t st) {
   if (tv.TB == 64)
-    hipLaunchKernelGGL((k_s_tiles<T, 64>), dim3(tv.ntiles), dim3(256), 0, st, tv, tile_order, tile_blk_ptr,
-                       tile_blk, sb_i, sb_j, sb_ptr, sb_pairs, Sd, bS, Z, tiles, ctl, gate);
+    hipLaunchKernelGGL((k_s_tiles<T, 64>), dim3(nparts), dim3(256), 0, st, tv, tile_order, part_tile, part_rect,
+                       part_ptr, part_blk, sb_i, sb_j, sb_ptr, sb_pairs, Sd, bS, Z, tiles, ctl, gate);
   else if (tv.TB == 32)
-    hipLaunchKernelGGL((k_s_tiles<T, 32>), dim3(tv.ntiles), dim3(256), 0, st, tv, tile_order, tile_blk_ptr,
-                       tile_blk, sb_i, sb_j, sb_ptr, sb_pairs, Sd, bS, Z, tiles, ctl, gate);
+    hipLaunchKernelGGL((k_s_tiles<T, 32>), dim3(nparts), dim3(256), 0, st, tv, tile_order, part_tile, part_rect,
+                       part_ptr, part_blk, sb_i, sb_j, sb_ptr, sb_pairs, Sd, bS, Z, tiles, ctl, gate);
 }
 
 template <typename T>
@@ -969,9 +980,9 @@ void launch_zero(T* p, long long n, hipStream_t st) {
                                  hipStream_t);                                                               \
   template void launch_s_diag<T>(const BAView&, const T*, const T*, const T*, const T*, T*, T*, DevCtl*,    \
                                  int, hipStream_t);                                                          \
-  template void launch_s_tiles<T>(const TileView&, const int*, const int*, const int*, const int*,           \
-                                  const int*, const int*, const int*, const T*, const T*, const T*, T*,      \
-                                  DevCtl*, int, hipStream_t);                                                \
+  template void launch_s_tiles<T>(const TileView&, int, const int*, const int*, const int*, const int*,      \
+                                  const int*, const int*, const int*, const int*, const int*, const T*,      \
+                                  const T*, const T*, T*, DevCtl*, int, hipStream_t);                        \
   template void launch_backsub<T>(const BAView&, const T*, const T*, const T*, const double*, double*,       \
                                   const Scratch&, DevCtl*, int, hipStream_t, const double*, int,             \
                                   const double*);                                                            \

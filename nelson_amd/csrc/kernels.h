@@ -96,11 +96,13 @@ void launch_vinv_y(const BAView& v, const T* V, const T* bV, const T* W, T* Vinv
 template <typename T>
 void launch_s_diag(const BAView& v, const T* U, const T* bU, const T* Z, const T* tW,
                    T* Sd, T* bS, DevCtl* ctl, int gate, hipStream_t st);
+// one workgroup per part of a tile (part_tile, part_rect = local rows lo, hi,
+// cols lo, hi, part_ptr / part_blk = the camera blocks inside)
 template <typename T>
-void launch_s_tiles(const TileView& tv, const int* tile_order, const int* tile_blk_ptr, const int* tile_blk,
-                    const int* sb_i, const int* sb_j, const int* sb_ptr, const int* sb_pairs,
-                    const T* Sd, const T* bS, const T* Z, T* tiles, DevCtl* ctl,
-                    int gate, hipStream_t st);
+void launch_s_tiles(const TileView& tv, int nparts, const int* tile_order, const int* part_tile, const int* part_rect,
+                    const int* part_ptr, const int* part_blk, const int* sb_i, const int* sb_j, const int* sb_ptr,
+                    const int* sb_pairs, const T* Sd, const T* bS, const T* Z, T* tiles, DevCtl* ctl, int gate,
+                    hipStream_t st);
 // xV = V^-1 (rhsV - W^T xU) with rhsV = -bV when rhs_v is null; with `add`
 // the result is added to xV; the increment statistics use bden (FP64
 // gradient) when given, else bV
