@@ -63,6 +63,11 @@ typedef struct {
   double huber_delta;              /* > 0: Huber loss with this threshold (pixels)    */
   int precision;                   /* NSN_FP64 or NSN_MIXED                           */
   int device;                      /* HIP device ordinal                              */
+  /* [3*n_cameras] per-camera (f, k1, k2) of the BAL camera model, or NULL for
+   * the shared pinhole above.  BAL model: the camera looks down -z,
+   * p = -(R X + t)_xy / (R X + t)_z, pixel = f (1 + k1 |p|^2 + k2 |p|^4) p;
+   * focal / cx / cy are then unused.  The intrinsics stay constant. */
+  const double* intrinsics;
 } nsn_problem_desc;
 
 typedef struct {
@@ -156,6 +161,18 @@ int nsn_generate_on_graph(const nsn_graph_gen_params* p, int n_cameras, int n_po
  * short files and out-of-range pairs (NSN_EINVAL, message names the entry). */
 int nsn_visibility_load(const char* path, int capacity, int* n_cameras, int* n_points, int* n_edges,
                         int* cam_idx, int* pt_idx);
+
+/* A Bundle Adjustment in the Large (BAL) problem file, the data format of the
+ * ladybug problem the reference's visibility graph comes from
+ * (nelson/samples/Problems.cpp:38): header "nCameras nPoints nObservations",
+ * the observations "camera point x y", 9 numbers per camera (angle-axis,
+ * translation, f, k1, k2), 3 per point.  With every array NULL it reads the
+ * header only; otherwise it fills obs (2 per observation), cams (6 per camera,
+ * the nsn_set_state layout), intrinsics (3 per camera, nsn_problem_desc's
+ * BAL model) and pts (3 per point).  Rejects short, malformed or
+ * out-of-range input (NSN_EINVAL, message names the entry). */
+int nsn_bal_load(const char* path, int capacity, int* n_cameras, int* n_points, int* n_obs,
+                 int* cam_idx, int* pt_idx, double* obs, double* cams, double* intrinsics, double* pts);
 
 /* ---- problem -------------------------------------------------------------- */
 /* parametersReady + addEdge* + structureReady (DoubleSection.hpp:67-246,338-365) */

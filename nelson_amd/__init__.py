@@ -13,5 +13,6 @@ from .capi import (  # noqa: F401
 from .problem import (  # noqa: F401
     BAProblem, SolverCholeskySchur, LevenbergMarquardt, LevenbergMarquardtSettings, GaussNewton,
     GaussNewtonSettings, generate_ba, window_config, CONFIGS, load_visibility, generate_on_graph, LADYBUG_SAMPLE,
+    load_bal,
 )
 from .batch import BABatch  # noqa: F401

@@ -38,7 +38,9 @@ class BABatch:
             pf[0] = 1
             w = g.get("weight")
             w = None if w is None else np.ascontiguousarray(w, np.float64)
-            self._keep += [cam_idx, pt_idx, obs, cf, pf, w]
+            k = g.get("intrinsics")  # the BAL camera model: passed on so that the library can refuse it
+            k = None if k is None else np.ascontiguousarray(k, np.float64).reshape(-1)
+            self._keep += [cam_idx, pt_idx, obs, cf, pf, w, k]
             d.n_cameras, d.n_points, d.n_edges = g["n_cameras"], g["n_points"], len(cam_idx)
             d.cam_idx, d.pt_idx, d.obs, d.weight = _ip(cam_idx), _ip(pt_idx), _dp(obs), _dp(w)
             d.cam_fixed = cf.ctypes.data_as(C.POINTER(C.c_ubyte))
@@ -47,6 +49,7 @@ class BABatch:
             d.huber_delta = g.get("huber_delta", 0.0)
             d.precision = capi.NSN_MIXED if precision == "mixed" else capi.NSN_FP64
             d.device = device
+            d.intrinsics = _dp(k)
             self.C.append(g["n_cameras"])
             self.P.append(g["n_points"])
             self.E.append(len(cam_idx))

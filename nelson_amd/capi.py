@@ -36,6 +36,7 @@ class ProblemDesc(C.Structure):
         ("cam_fixed", C.POINTER(C.c_ubyte)), ("pt_fixed", C.POINTER(C.c_ubyte)),
         ("focal", C.c_double), ("cx", C.c_double), ("cy", C.c_double),
         ("huber_delta", C.c_double), ("precision", C.c_int), ("device", C.c_int),
+        ("intrinsics", C.POINTER(C.c_double)),
     ]
 
 
@@ -95,6 +96,7 @@ SIGNATURES = {
     "nsn_generate_on_graph": (C.c_int, [C.POINTER(GraphGenParams), C.c_int, C.c_int, C.c_int, _I, _I, _D, _D, _D,
                                         _D, _D]),
     "nsn_visibility_load": (C.c_int, [C.c_char_p, C.c_int, _I, _I, _I, _I, _I]),
+    "nsn_bal_load": (C.c_int, [C.c_char_p, C.c_int, _I, _I, _I, _I, _I, _D, _D, _D, _D]),
     "nsn_problem_create": (C.c_int, [C.POINTER(ProblemDesc), C.POINTER(_P)]),
     "nsn_problem_destroy": (C.c_int, [_P]),
     "nsn_problem_info_get": (C.c_int, [_P, C.POINTER(ProblemInfo)]),

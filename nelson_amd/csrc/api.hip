@@ -444,6 +444,13 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
   v.cam_ptr = d_cam_ptr; v.ce_pt = d_ce_pt; v.ce_obs = d_ce_obs; v.ce_w = d_ce_w;
   v.wptr = d_wptr; v.w_cam = d_w_cam; v.w_pt = d_w_pt; v.cw_ptr = d_cw_ptr; v.cw_idx = d_cw_idx;
   v.cams = h->cams; v.pts = h->pts;
+  v.intr = nullptr;
+  if (d->intrinsics) {
+    double* d_intr = a_dbl(3 * (size_t)C);
+    if (!d_intr) return fail(NSN_ENOMEM, "device allocation failed (intrinsics)");
+    if ((rc = upload(d_intr, std::vector<double>(d->intrinsics, d->intrinsics + 3 * (size_t)C)))) return rc;
+    v.intr = d_intr;
+  }
   h->pe_order = std::move(pe);
   h->ce_order = std::move(ce);
   h->d_pe_obs = d_pe_obs; h->d_pe_w = d_pe_w; h->d_ce_obs = d_ce_obs; h->d_ce_w = d_ce_w;
@@ -479,6 +486,11 @@ extern "C" int nsn_problem_create(const nsn_problem_desc* d, nsn_problem** out) 
     return fail(NSN_EINVAL, "nsn_problem_create: bad sizes or missing edge arrays");
   if (d->precision != NSN_FP64 && d->precision != NSN_MIXED)
     return fail(NSN_EINVAL, "nsn_problem_create: unknown precision %d", d->precision);
+  if (d->intrinsics)
+    for (int c = 0; c < d->n_cameras; c++)
+      if (!std::isfinite(d->intrinsics[3 * c]) || d->intrinsics[3 * c] == 0.0 ||
+          !std::isfinite(d->intrinsics[3 * c + 1]) || !std::isfinite(d->intrinsics[3 * c + 2]))
+        return fail(NSN_EINVAL, "nsn_problem_create: camera %d: intrinsics must be finite with f != 0", c);
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
     return fail(NSN_EHIP, "nsn_problem_create: no HIP device");

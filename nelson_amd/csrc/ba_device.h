@@ -54,10 +54,32 @@ __host__ __device__ constexpr int up6(int i, int j) { return i * 6 - i * (i - 1)
 // ---------------------------------------------------------------------------
 // the reprojection edge
 // ---------------------------------------------------------------------------
+// camera record in registers: R (row-major), t, then (f, k1, k2) of the
+// per-camera model when the problem has one (zeros otherwise)
+constexpr int kCamRec = 15;
+
+__device__ __forceinline__ void load_cam(const BAView& v, int c, double* R) {
+#pragma unroll
+  for (int i = 0; i < 12; i++) R[i] = v.cams[12 * c + i];
+  if (v.intr) {
+#pragma unroll
+    for (int i = 0; i < 3; i++) R[12 + i] = v.intr[3 * c + i];
+  } else {
+    R[12] = R[13] = R[14] = 0.0;
+  }
+}
+
+// Xc = R X + t, 1 / Xc_z, the residual, the loss terms, and for the BAL
+// model D = d(pixel)/d(Xc) (2 x 3; the pinhole's D is formed where it is used)
 struct Proj {
   double Xc0, Xc1, Xc2, iz, r0, r1, cost, om;
+  double d00, d01, d02, d10, d11, d12;
 };
 
+// Two camera models:
+//  * shared pinhole (v.intr == null): pixel = f Xc_xy / Xc_z + (cx, cy);
+//  * the BAL model (v.intr: f, k1, k2 per camera, the camera looks down -z):
+//    p = -Xc_xy / Xc_z, pixel = f (1 + k1 |p|^2 + k2 |p|^4) p.
 __device__ __forceinline__ Proj project(const BAView& v, const double* __restrict__ R, double X0,
                                         double X1, double X2, double o0, double o1, double we) {
   Proj p;
@@ -65,8 +87,27 @@ __device__ __forceinline__ Proj project(const BAView& v, const double* __restric
   p.Xc1 = R[3] * X0 + R[4] * X1 + R[5] * X2 + R[10];
   p.Xc2 = R[6] * X0 + R[7] * X1 + R[8] * X2 + R[11];
   p.iz = 1.0 / p.Xc2;
-  p.r0 = v.f * p.Xc0 * p.iz + v.cx - o0;
-  p.r1 = v.f * p.Xc1 * p.iz + v.cy - o1;
+  const double iz = p.iz;
+  if (!v.intr) {
+    p.r0 = v.f * p.Xc0 * p.iz + v.cx - o0;
+    p.r1 = v.f * p.Xc1 * p.iz + v.cy - o1;
+  } else {
+    const double f = R[12], k1 = R[13], k2 = R[14];
+    const double px = -p.Xc0 * iz, py = -p.Xc1 * iz;
+    const double s2 = px * px + py * py;
+    const double dist = 1.0 + s2 * (k1 + k2 * s2);
+    p.r0 = f * dist * px - o0;
+    p.r1 = f * dist * py - o1;
+    // D = f (dist I + g p p^T) dp/dXc, g = d(dist)/d(s2) * 2, dp/dXc = -iz [I | p]
+    const double g = 2.0 * k1 + 4.0 * k2 * s2;
+    const double m00 = f * (dist + g * px * px), m01 = f * g * px * py, m11 = f * (dist + g * py * py);
+    p.d00 = -iz * m00;
+    p.d01 = -iz * m01;
+    p.d02 = -iz * (m00 * px + m01 * py);
+    p.d10 = -iz * m01;
+    p.d11 = -iz * m11;
+    p.d12 = -iz * (m01 * px + m11 * py);
+  }
   double s = p.r0 * p.r0 + p.r1 * p.r1;
   double psi = 1.0;
   if (v.huber > 0.0 && s > v.huber * v.huber) {
@@ -81,37 +122,61 @@ __device__ __forceinline__ Proj project(const BAView& v, const double* __restric
   return p;
 }
 
-// d(r)/d(omega, upsilon) for the left SE(3) perturbation: D [-[Xc]x | I]
+// d(r)/d(omega, upsilon) for the left SE(3) perturbation: D [-[Xc]x | I].
+// The pinhole's D = [[a, 0, b0], [0, a, b1]] (a = f / z, b = -a Xc_xy / z).
 template <typename T>
 __device__ __forceinline__ void jac_cam(const BAView& v, const Proj& p, T* j0, T* j1) {
-  const double a = v.f * p.iz;
-  const double b0 = -a * p.Xc0 * p.iz;
-  const double b1 = -a * p.Xc1 * p.iz;
-  j0[0] = (T)(b0 * p.Xc1);
-  j0[1] = (T)(a * p.Xc2 - b0 * p.Xc0);
-  j0[2] = (T)(-a * p.Xc1);
-  j0[3] = (T)a;
-  j0[4] = (T)0;
-  j0[5] = (T)b0;
-  j1[0] = (T)(-a * p.Xc2 + b1 * p.Xc1);
-  j1[1] = (T)(-b1 * p.Xc0);
-  j1[2] = (T)(a * p.Xc0);
-  j1[3] = (T)0;
-  j1[4] = (T)a;
-  j1[5] = (T)b1;
+  if (!v.intr) {
+    const double a = v.f * p.iz;
+    const double b0 = -a * p.Xc0 * p.iz;
+    const double b1 = -a * p.Xc1 * p.iz;
+    j0[0] = (T)(b0 * p.Xc1);
+    j0[1] = (T)(a * p.Xc2 - b0 * p.Xc0);
+    j0[2] = (T)(-a * p.Xc1);
+    j0[3] = (T)a;
+    j0[4] = (T)0;
+    j0[5] = (T)b0;
+    j1[0] = (T)(-a * p.Xc2 + b1 * p.Xc1);
+    j1[1] = (T)(-b1 * p.Xc0);
+    j1[2] = (T)(a * p.Xc0);
+    j1[3] = (T)0;
+    j1[4] = (T)a;
+    j1[5] = (T)b1;
+    return;
+  }
+  j0[0] = (T)(p.d02 * p.Xc1 - p.d01 * p.Xc2);
+  j0[1] = (T)(p.d00 * p.Xc2 - p.d02 * p.Xc0);
+  j0[2] = (T)(p.d01 * p.Xc0 - p.d00 * p.Xc1);
+  j0[3] = (T)p.d00;
+  j0[4] = (T)p.d01;
+  j0[5] = (T)p.d02;
+  j1[0] = (T)(p.d12 * p.Xc1 - p.d11 * p.Xc2);
+  j1[1] = (T)(p.d10 * p.Xc2 - p.d12 * p.Xc0);
+  j1[2] = (T)(p.d11 * p.Xc0 - p.d10 * p.Xc1);
+  j1[3] = (T)p.d10;
+  j1[4] = (T)p.d11;
+  j1[5] = (T)p.d12;
 }
 
 // d(r)/d(X) = D R
 template <typename T>
 __device__ __forceinline__ void jac_pt(const BAView& v, const Proj& p, const double* __restrict__ R,
                                        T* g0, T* g1) {
-  const double a = v.f * p.iz;
-  const double b0 = -a * p.Xc0 * p.iz;
-  const double b1 = -a * p.Xc1 * p.iz;
+  if (!v.intr) {
+    const double a = v.f * p.iz;
+    const double b0 = -a * p.Xc0 * p.iz;
+    const double b1 = -a * p.Xc1 * p.iz;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      g0[k] = (T)(a * R[k] + b0 * R[6 + k]);
+      g1[k] = (T)(a * R[3 + k] + b1 * R[6 + k]);
+    }
+    return;
+  }
 #pragma unroll
   for (int k = 0; k < 3; k++) {
-    g0[k] = (T)(a * R[k] + b0 * R[6 + k]);
-    g1[k] = (T)(a * R[3 + k] + b1 * R[6 + k]);
+    g0[k] = (T)(p.d00 * R[k] + p.d01 * R[3 + k] + p.d02 * R[6 + k]);
+    g1[k] = (T)(p.d10 * R[k] + p.d11 * R[3 + k] + p.d12 * R[6 + k]);
   }
 }
 
@@ -350,9 +415,8 @@ __device__ __forceinline__ void lin_point(const BAView& v, int p, T* __restrict_
   const int e0 = v.pt_ptr[p], e1 = v.pt_ptr[p + 1];
   for (int e = e0; e < e1; e++) {
     const int c = v.pe_cam[e];
-    double R[12];
-#pragma unroll
-    for (int i = 0; i < 12; i++) R[i] = v.cams[12 * c + i];
+    double R[kCamRec];
+    load_cam(v, c, R);
     const Proj pr = project(v, R, X0, X1, X2, v.pe_obs[2 * e], v.pe_obs[2 * e + 1], v.pe_w[e]);
     chi2 += pr.cost;
     if (HESS) {

@@ -86,9 +86,8 @@ __global__ __launch_bounds__(256) void k_lin_cams(BAView v, T* __restrict__ U, T
       }
     }
   }
-  double R[12];
-#pragma unroll
-  for (int i = 0; i < 12; i++) R[i] = v.cams[12 * c + i];
+  double R[kCamRec];
+  load_cam(v, c, R);
   T acc[27];
 #pragma unroll
   for (int i = 0; i < 27; i++) acc[i] = (T)0;
@@ -373,9 +372,8 @@ __global__ __launch_bounds__(256) void k_ir_res_points(BAView v, const double* _
   const int e1 = v.pt_ptr[p + 1];
   for (int e = v.pt_ptr[p]; e < e1; e++) {
     const int c = v.pe_cam[e];
-    double R[12];
-#pragma unroll
-    for (int i = 0; i < 12; i++) R[i] = v.cams[12 * c + i];
+    double R[kCamRec];
+    load_cam(v, c, R);
     const Proj pr = project(v, R, X0, X1, X2, v.pe_obs[2 * e], v.pe_obs[2 * e + 1], v.pe_w[e]);
     double g0[3], g1[3];
     jac_pt<double>(v, pr, R, g0, g1);
@@ -431,9 +429,8 @@ __global__ __launch_bounds__(256) void k_ir_res_cams(BAView v, const double* __r
   __shared__ double pri[18];  // the camera's unary prior: J^T r, J^T J x, diag(J^T J)
   const int u = blockIdx.x;
   const int c = v.var_cam_user[u];
-  double R[12];
-#pragma unroll
-  for (int i = 0; i < 12; i++) R[i] = v.cams[12 * c + i];
+  double R[kCamRec];
+  load_cam(v, c, R);
   double xc[6];
 #pragma unroll
   for (int i = 0; i < 6; i++) xc[i] = xU[6 * (size_t)u + i];
@@ -567,9 +564,8 @@ __global__ __launch_bounds__(256) void k_adjoint_edges(BAView v, const double* _
   const int e1 = v.pt_ptr[p + 1];
   for (int e = v.pt_ptr[p]; e < e1; e++) {
     const int c = v.pe_cam[e];
-    double R[12];
-#pragma unroll
-    for (int i = 0; i < 12; i++) R[i] = v.cams[12 * c + i];
+    double R[kCamRec];
+    load_cam(v, c, R);
     const double we = v.pe_w[e];
     const Proj pr = project(v, R, X0, X1, X2, v.pe_obs[2 * e], v.pe_obs[2 * e + 1], we);
     double a0 = 0.0, a1 = 0.0;

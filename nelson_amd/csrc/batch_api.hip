@@ -241,6 +241,8 @@ extern "C" int nsn_batch_create(const nsn_problem_desc* wd, int n_windows, nsn_b
     if (d->n_cameras < 1 || d->n_points < 1 || d->n_edges < 0 ||
         (d->n_edges > 0 && (!d->cam_idx || !d->pt_idx || !d->obs)))
       return fail(NSN_EINVAL, "nsn_batch_create: window %d: bad sizes or missing edge arrays", w);
+    if (d->intrinsics)
+      return fail(NSN_EINVAL, "nsn_batch_create: window %d: batched windows use the shared pinhole model only", w);
     int rc = host_structure(d, hs[w]);
     if (rc) return rc;
     if (6 * hs[w].NU > kMaxWinS)

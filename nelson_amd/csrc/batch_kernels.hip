@@ -63,9 +63,8 @@ __device__ void win_linearize(const BWin& w, DevCtl* c, double* sh) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   for (int u = wv; u < v.NU; u += 4) {
     const int cam = v.var_cam_user[u];
-    double R[12];
-#pragma unroll
-    for (int i = 0; i < 12; i++) R[i] = v.cams[12 * cam + i];
+    double R[kCamRec];
+    load_cam(v, cam, R);
     T acc[27];
 #pragma unroll
     for (int i = 0; i < 27; i++) acc[i] = (T)0;

@@ -46,6 +46,7 @@ struct BAView {
   const int* cw_idx;        // [nW]
   double* cams;             // [12C] R (row-major) and t per camera
   double* pts;              // [3P]
+  const double* intr;       // [3C] (f, k1, k2) per camera of the BAL model, or null: shared pinhole
   // unary priors (nsn_set_priors; null pointers: none): per user camera /
   // point the index of its prior or -1; camera priors omega0, t0 and 6
   // weights, point priors X0 and 3 weights

@@ -107,6 +107,25 @@ def load_visibility(path):
     return dict(n_cameras=nc.value, n_points=npt.value, cam_idx=ci[:ne.value], pt_idx=pi[:ne.value])
 
 
+def load_bal(path):
+    """nsn_bal_load: a Bundle Adjustment in the Large problem file -> the dict
+    of generate_ba (cams0 / pts0 the file's state, intrinsics (f, k1, k2) per
+    camera for the BAL camera model), ready for BAProblem.from_generated."""
+    lib = capi.lib()
+    nc, npt, ne = C.c_int(0), C.c_int(0), C.c_int(0)
+    bpath = str(path).encode()
+    z = [None] * 6
+    check(lib.nsn_bal_load(bpath, 0, C.byref(nc), C.byref(npt), C.byref(ne), *z))
+    e = ne.value
+    ci, pi = np.zeros(max(e, 1), np.int32), np.zeros(max(e, 1), np.int32)
+    obs = np.zeros(max(2 * e, 1))
+    cams, intr, pts = np.zeros(6 * nc.value), np.zeros(3 * nc.value), np.zeros(3 * npt.value)
+    check(lib.nsn_bal_load(bpath, e, C.byref(nc), C.byref(npt), C.byref(ne), _ip(ci), _ip(pi), _dp(obs), _dp(cams),
+                           _dp(intr), _dp(pts)))
+    return dict(n_cameras=nc.value, n_points=npt.value, n_edges=e, cam_idx=ci[:e], pt_idx=pi[:e], obs=obs[:2 * e],
+                cams0=cams, pts0=pts, intrinsics=intr, focal=0.0, cx=0.0, cy=0.0, huber_delta=0.0)
+
+
 # the reference's BA sample on a visibility graph
 # (nelson/samples/BASchurPerfomances.cpp:40-75: camera 100 x 100 px, focal 100,
 # points in the unit cube, cameras at focal * 1 / (100 / 2) = 2 from the origin)
@@ -147,7 +166,7 @@ class BAProblem:
     """
 
     def __init__(self, n_cameras, n_points, cam_idx, pt_idx, obs, focal, cx, cy, weight=None, cam_fixed=None,
-                 pt_fixed=None, huber_delta=0.0, precision="fp64", device=0):
+                 pt_fixed=None, huber_delta=0.0, precision="fp64", device=0, intrinsics=None):
         self._lib = capi.lib()
         self.n_cameras, self.n_points = int(n_cameras), int(n_points)
         self.cam_idx = np.ascontiguousarray(cam_idx, np.int32)
@@ -163,6 +182,10 @@ class BAProblem:
         self.cam_fixed = np.ascontiguousarray(cam_fixed, np.uint8)
         self.pt_fixed = np.ascontiguousarray(pt_fixed, np.uint8)
         self.weight = None if weight is None else np.ascontiguousarray(weight, np.float64)
+        # per-camera (f, k1, k2) of the BAL camera model, or None: the shared pinhole
+        self.intrinsics = None if intrinsics is None else np.ascontiguousarray(intrinsics, np.float64).reshape(-1)
+        if self.intrinsics is not None and self.intrinsics.size != 3 * self.n_cameras:
+            raise ValueError("intrinsics must hold (f, k1, k2) per camera")
         d = capi.ProblemDesc()
         d.n_cameras, d.n_points, d.n_edges = self.n_cameras, self.n_points, self.n_edges
         d.cam_idx, d.pt_idx, d.obs = _ip(self.cam_idx), _ip(self.pt_idx), _dp(self.obs)
@@ -172,6 +195,7 @@ class BAProblem:
         d.focal, d.cx, d.cy, d.huber_delta = focal, cx, cy, huber_delta
         d.precision = capi.NSN_MIXED if precision == "mixed" else capi.NSN_FP64
         d.device = device
+        d.intrinsics = _dp(self.intrinsics)
         h = C.c_void_p()
         check(self._lib.nsn_problem_create(C.byref(d), C.byref(h)))
         self._h = h
@@ -184,7 +208,8 @@ class BAProblem:
     @classmethod
     def from_generated(cls, g, **kw):
         p = cls(g["n_cameras"], g["n_points"], g["cam_idx"], g["pt_idx"], g["obs"], g["focal"], g["cx"],
-                g["cy"], huber_delta=kw.pop("huber_delta", g.get("huber_delta", 0.0)), **kw)
+                g["cy"], huber_delta=kw.pop("huber_delta", g.get("huber_delta", 0.0)),
+                intrinsics=kw.pop("intrinsics", g.get("intrinsics")), **kw)
         p.set_state(g["cams0"], g["pts0"])
         return p
 

@@ -712,6 +712,7 @@ struct or_ba {
   int ncp, npp;                 /* unary priors (or_ba_set_priors) */
   int *cp_id, *pp_id;
   double *cp_p0, *cp_w, *pp_p0, *pp_w;
+  double* intr;                 /* per camera (f, k1, k2) of the BAL model, or NULL */
   or_hessian H;
 };
 
@@ -788,6 +789,7 @@ void or_ba_destroy(or_ba* b) {
   free(b->cams); free(b->pts); free(b->cams_bak); free(b->pts_bak); free(b->edge_w);
   free(b->er); free(b->eJc); free(b->eJp); free(b->eom);
   free(b->cp_id); free(b->cp_p0); free(b->cp_w); free(b->pp_id); free(b->pp_p0); free(b->pp_w);
+  free(b->intr);
   or_hessian_free(&b->H);
   free(b);
 }
@@ -884,10 +886,21 @@ static double cam_prior(const or_ba* b, int k, int jac, double* r, double* J) {
   return cost;
 }
 
-/* the BA reprojection edge: r = pi(K, R X + t) - z, pinhole with focal f and
- * principal point (cx, cy); Jacobians w.r.t. the left SE(3) perturbation
- * (omega, upsilon) of the camera and w.r.t. the point.  Cost per edge:
- * w * rho_huber(|r|^2) (rho = identity without Huber); IRLS weight returned. */
+void or_ba_set_intrinsics(or_ba* b, const double* intr) {
+  free(b->intr);
+  b->intr = NULL;
+  if (intr) {
+    b->intr = (double*)malloc(sizeof(double) * 3 * (size_t)b->C);
+    memcpy(b->intr, intr, sizeof(double) * 3 * (size_t)b->C);
+  }
+}
+
+/* the BA reprojection edge: r = pi(R X + t) - z.  pi is the pinhole with
+ * focal f and principal point (cx, cy), or with per-camera intrinsics the
+ * BAL model: q = -(x, y) / z, pi = f (1 + k1 |q|^2 + k2 |q|^4) q.
+ * Jacobians w.r.t. the left SE(3) perturbation (omega, upsilon) of the camera
+ * and w.r.t. the point.  Cost per edge: w * rho_huber(|r|^2) (rho = identity
+ * without Huber); IRLS weight returned. */
 static double ba_edge(const or_ba* b, int e, int jac, double* r, double* Jc, double* Jp, double* om) {
   const double* R = b->cams + 12 * b->cam[e];
   const double* t = R + 9;
@@ -895,8 +908,28 @@ static double ba_edge(const or_ba* b, int e, int jac, double* r, double* Jc, dou
   double Xc[3];
   for (int i = 0; i < 3; i++) Xc[i] = R[i * 3 + 0] * X[0] + R[i * 3 + 1] * X[1] + R[i * 3 + 2] * X[2] + t[i];
   double iz = 1.0 / Xc[2];
-  double u = b->f * Xc[0] * iz + b->cx;
-  double v = b->f * Xc[1] * iz + b->cy;
+  double u, v, D[6];
+  if (!b->intr) {
+    u = b->f * Xc[0] * iz + b->cx;
+    v = b->f * Xc[1] * iz + b->cy;
+    double a = b->f * iz;
+    D[0] = a; D[1] = 0.0; D[2] = -a * Xc[0] * iz;
+    D[3] = 0.0; D[4] = a; D[5] = -a * Xc[1] * iz;
+  } else {
+    const double* K = b->intr + 3 * b->cam[e];
+    double qx = -Xc[0] * iz, qy = -Xc[1] * iz;
+    double s = qx * qx + qy * qy;
+    double dist = 1.0 + K[1] * s + K[2] * s * s;
+    double ddist = K[1] + 2.0 * K[2] * s;       /* d dist / d s */
+    u = K[0] * dist * qx;
+    v = K[0] * dist * qy;
+    /* chain rule: d pi / d q, then d q / d Xc */
+    double du_dqx = K[0] * (dist + qx * ddist * 2.0 * qx), du_dqy = K[0] * qx * ddist * 2.0 * qy;
+    double dv_dqx = K[0] * qy * ddist * 2.0 * qx, dv_dqy = K[0] * (dist + qy * ddist * 2.0 * qy);
+    double dqx_dx = -iz, dqx_dz = Xc[0] * iz * iz, dqy_dy = -iz, dqy_dz = Xc[1] * iz * iz;
+    D[0] = du_dqx * dqx_dx; D[1] = du_dqy * dqy_dy; D[2] = du_dqx * dqx_dz + du_dqy * dqy_dz;
+    D[3] = dv_dqx * dqx_dx; D[4] = dv_dqy * dqy_dy; D[5] = dv_dqx * dqx_dz + dv_dqy * dqy_dz;
+  }
   r[0] = u - b->obs[2 * e];
   r[1] = v - b->obs[2 * e + 1];
   double s = r[0] * r[0] + r[1] * r[1];
@@ -911,9 +944,6 @@ static double ba_edge(const or_ba* b, int e, int jac, double* r, double* Jc, dou
   }
   *om = b->w[e] * psi;
   if (jac) {
-    /* d proj / d Xc */
-    double a = b->f * iz;
-    double D[6] = {a, 0.0, -a * Xc[0] * iz, 0.0, a, -a * Xc[1] * iz};
     /* d Xc / d (omega, upsilon) = [ -[Xc]x | I ] */
     double M[18] = {0.0, Xc[2], -Xc[1], 1.0, 0.0, 0.0,
                     -Xc[2], 0.0, Xc[0], 0.0, 1.0, 0.0,

@@ -157,6 +157,11 @@ int  or_ba_num_var_pts(const or_ba* b);
  * counts of 0 remove them. */
 void or_ba_set_priors(or_ba* b, int ncp, const int* cam_ids, const double* cam_p0, const double* cam_w,
                       int npp, const int* pt_ids, const double* pt_p0, const double* pt_w);
+/* per-camera intrinsics (f, k1, k2; 3 per camera) of the BAL camera model:
+ * q = -(x, y) / z of Xc = R X + t, pixel = f (1 + k1 |q|^2 + k2 |q|^4) q
+ * (the camera looks down -z; focal / cx / cy unused).  NULL restores the
+ * shared pinhole. */
+void or_ba_set_intrinsics(or_ba* b, const double* intr);
 /* log of a rotation matrix (principal angle-axis) and the inverse left
  * Jacobian of SO(3) at omega (row-major 3x3) */
 void or_so3_log(const double* R, double* w);

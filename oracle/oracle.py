@@ -106,6 +106,7 @@ def lib():
             "or_ba_num_var_cams": (C.c_int, [C.c_void_p]),
             "or_ba_num_var_pts": (C.c_int, [C.c_void_p]),
             "or_ba_set_priors": (None, [C.c_void_p, C.c_int, _I, _D, _D, C.c_int, _I, _D, _D]),
+            "or_ba_set_intrinsics": (None, [C.c_void_p, _D]),
             "or_so3_log": (None, [_D, _D]),
             "or_so3_jl_inv": (None, [_D, _D]),
             "or_rodrigues": (None, [_D, _D]),
@@ -211,7 +212,7 @@ class OracleBA:
     """The oracle's BA section built from the same arrays as BAProblem."""
 
     def __init__(self, n_cameras, n_points, cam_idx, pt_idx, obs, focal, cx, cy, cams0, pts0, weight=None,
-                 cam_fixed=None, pt_fixed=None, huber_delta=0.0):
+                 cam_fixed=None, pt_fixed=None, huber_delta=0.0, intrinsics=None):
         L = lib()
         self.C, self.P = n_cameras, n_points
         self.cam_idx = np.ascontiguousarray(cam_idx, np.int32)
@@ -235,12 +236,14 @@ class OracleBA:
                                  dp(pts0))
         self.NU = L.or_ba_num_var_cams(self._h)
         self.NV = L.or_ba_num_var_pts(self._h)
+        if intrinsics is not None:  # the BAL camera model (or_ba_set_intrinsics)
+            L.or_ba_set_intrinsics(self._h, dp(np.ascontiguousarray(intrinsics, np.float64).ravel()))
 
     @classmethod
     def from_generated(cls, g, **kw):
         return cls(g["n_cameras"], g["n_points"], g["cam_idx"], g["pt_idx"], g["obs"], g["focal"], g["cx"],
                    g["cy"], g["cams0"], g["pts0"], huber_delta=kw.pop("huber_delta", g.get("huber_delta", 0.0)),
-                   **kw)
+                   intrinsics=kw.pop("intrinsics", g.get("intrinsics")), **kw)
 
     def __del__(self):
         if getattr(self, "_h", None):
