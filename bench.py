@@ -233,6 +233,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--only-ladybug", action="store_true",
+                    help="only the BA sample's workload on the ladybug graph (its own JSON line)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device copy of 1 GiB 5 times (HBM counter calibration)")
     args = ap.parse_args()
@@ -250,6 +252,13 @@ def main():
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
         sys.exit(2)
     import nelson_amd as nsn
+    if args.only_ladybug:
+        lb = ladybug_problem(nsn)
+        out = ladybug_timing(nsn, lb)
+        if not args.no_cpu:
+            out["cpu_baseline"] = cpu_baseline_ladybug(lb)
+        print(json.dumps(out))
+        return
     cfg = dict(nsn.CONFIGS[args.config])
     g = nsn.generate_ba(cfg)
     log(f"bench: {args.config} C={g['n_cameras']} P={g['n_points']} E={g['n_edges']}")
