@@ -338,9 +338,10 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
   // pre_sub / pre_diag (NT each)
   h->nflags = tv.ntiles + 2 * h->NR + 2 * tv.NT;
   int n_tasks = 0;
-  chol_task_list(tv.NT, nullptr, &n_tasks);
+  chol_task_list(tv.NT, 0, nullptr, &n_tasks);  // the count does not depend on the look-ahead
+  const int ahead = chol_lookahead(std::min(n_tasks, kCholMaxWG - 1));
   std::vector<int> chol_tasks(3 * (size_t)std::max(n_tasks, 1));
-  chol_task_list(tv.NT, chol_tasks.data(), &n_tasks);
+  chol_task_list(tv.NT, ahead, chol_tasks.data(), &n_tasks);
   h->n_chol_tasks = n_tasks;
 
   // ---- device allocation ----

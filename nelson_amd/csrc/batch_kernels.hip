@@ -268,7 +268,7 @@ __device__ void win_cholesky_tiles(const BWin& w, DevCtl* c, double* lds) {
   for (int j = 0; j < NTw; j++) {
     tile_to_lds<T, double>(R0d, tile_at(S, ld, j, j), ld);
     __syncthreads();
-    potrf_lds<64>(R0d, 64 * j, w.n, c);
+    potrf_lds<64>(R0d, R1d, 64 * j, w.n, c);  // R1d is free until the inverse
     tri_inverse<64>(R0d, R1d);
     lds_to_tile<T, double>(tile_at(S, ld, j, j), ld, R0d);
     {

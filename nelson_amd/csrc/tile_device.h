@@ -48,7 +48,7 @@ __device__ __forceinline__ void wave_lds_sync() {
 // Diagonal-tile factorisation on one 64x64 FP64 tile in LDS, in four
 // 16-column panels.  A panel is processed by one wave with one row per lane
 // and the panel row in registers: each step takes its pivot from the owning
-// lane and broadcasts the pivot column by readlane; between panels the
+// lane and broadcasts the pivot column (lane reads and LDS); between panels the
 // trailing update runs on the MFMA units of the four waves.  The factored
 // tile is then inverted, so that the solves against it (the tiles below it
 // and the backward substitution) become matrix products.
@@ -83,10 +83,15 @@ __device__ __forceinline__ double rsqrt_f64(double d) {
 // is replaced by 1 and counted in `bad`.  The step is branch-free: every
 // lane computes the same pivot, the reciprocal square root of the selected
 // value, and the updates run on every lane (entries above the diagonal in the
-// registers are garbage and never written back).  The pivot reciprocals go to
-// their slots in the upper triangle after the panel.
+// registers are garbage and never written back).  After scaling, the entry
+// of column k that the next pivot needs comes by a lane read; the rest of the
+// column goes through the LDS buffer col (two 64-double halves, alternating
+// by step) and every lane reads rows C0+k+2 .. C0+15 back as a broadcast, so
+// those updates overlap the next pivot and their operands land in vector
+// registers.  The pivot reciprocals go to their slots in the upper triangle
+// after the panel.
 template <int TB, int C0>
-__device__ __forceinline__ void potrf_panel(double* A, int g0, int n, int& bad) {
+__device__ __forceinline__ void potrf_panel(double* A, double* col, int g0, int n, int& bad) {
   const int lane = threadIdx.x;
   double a[kPW];
 #pragma unroll
@@ -102,8 +107,19 @@ __device__ __forceinline__ void potrf_panel(double* A, int g0, int n, int& bad) 
     const double piv = (pad || !good) ? 1.0 : d * r;
     a[k] = (lane == C0 + k) ? piv : a[k] * r;
     myr = (lane == C0 + k) ? r : myr;
+    if (k + 1 < kPW) {
+      // the next pivot's column first, by a lane read (the critical path)
+      a[k + 1] -= a[k] * readlane_f64(a[k], C0 + k + 1);
+      if (k + 2 < kPW) {
+        // the rest through LDS: the write and the broadcast reads are LDS
+        // instructions of this one wave, which the LDS executes in order
+        double* cb = col + (k & 1) * 64;
+        cb[lane] = a[k];
+        __builtin_amdgcn_wave_barrier();
 #pragma unroll
-    for (int c = k + 1; c < kPW; c++) a[c] -= a[k] * readlane_f64(a[k], C0 + c);
+        for (int c = k + 2; c < kPW; c++) a[c] -= a[k] * cb[C0 + c];
+      }
+    }
   }
 #pragma unroll
   for (int c = 0; c < kPW; c++)
@@ -136,12 +152,12 @@ __device__ __forceinline__ void block_update(double* A, int rb, int cb, int k0) 
 // p+1 (what panel p+1 needs), then wave 0 factorises panel p+1 while waves
 // 1-3 apply panel p to the block columns beyond it.
 template <int TB, int P>
-__device__ __forceinline__ void potrf_next(double* A, int g0, int n, int& bad) {
+__device__ __forceinline__ void potrf_next(double* A, double* col, int g0, int n, int& bad) {
   const int wv = threadIdx.x >> 6;
   if (wv < 3 - P) block_update<TB>(A, P + 1 + wv, P + 1, kPW * P);
   __syncthreads();
   if (wv == 0) {
-    potrf_panel<TB, kPW * (P + 1)>(A, g0, n, bad);
+    potrf_panel<TB, kPW * (P + 1)>(A, col, g0, n, bad);
   } else if (P == 0) {
     // blocks (2,2), (3,2), (3,3) with panel 0
     const int rb = wv == 3 ? 3 : wv + 1, cb = wv == 1 ? 2 : (wv == 2 ? 2 : 3);
@@ -152,15 +168,16 @@ __device__ __forceinline__ void potrf_next(double* A, int g0, int n, int& bad) {
   __syncthreads();
 }
 
+// col: 128 doubles of LDS that nothing else uses during the call
 template <int TB>
-__device__ __forceinline__ void potrf_lds(double* A, int g0, int n, DevCtl* ctl) {
+__device__ __forceinline__ void potrf_lds(double* A, double* col, int g0, int n, DevCtl* ctl) {
   static_assert(TB == 4 * kPW, "four panels");
   int bad = 0;
-  if (threadIdx.x < 64) potrf_panel<TB, 0>(A, g0, n, bad);
+  if (threadIdx.x < 64) potrf_panel<TB, 0>(A, col, g0, n, bad);
   __syncthreads();
-  potrf_next<TB, 0>(A, g0, n, bad);
-  potrf_next<TB, 1>(A, g0, n, bad);
-  potrf_next<TB, 2>(A, g0, n, bad);
+  potrf_next<TB, 0>(A, col, g0, n, bad);
+  potrf_next<TB, 1>(A, col, g0, n, bad);
+  potrf_next<TB, 2>(A, col, g0, n, bad);
   if (threadIdx.x == 0 && bad) atomicAdd(&ctl->fail_s, 1);
 }
 
@@ -178,6 +195,8 @@ __device__ __forceinline__ void tri_inverse(const double* L, double* Li) {
   typedef Mfma<double>::acc_t acc_t;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   if (lane < 16) {
+    // branch-free: for i < j every term is zero, so x_i = 0 without a test,
+    // and the loads of L (the same for every lane) can all be issued early
     const int b0 = wv * 16, j = lane;
     double x[16];
 #pragma unroll
@@ -185,7 +204,7 @@ __device__ __forceinline__ void tri_inverse(const double* L, double* Li) {
       double s = (i == j) ? 1.0 : 0.0;
 #pragma unroll
       for (int k = 0; k < i; k++) s -= L[(b0 + k) * TB + b0 + i] * x[k];
-      x[i] = (i >= j) ? s * L[rdiag_slot<TB>(b0 + i)] : 0.0;
+      x[i] = s * L[rdiag_slot<TB>(b0 + i)];
     }
 #pragma unroll
     for (int i = 0; i < 16; i++) Li[(b0 + j) * TB + b0 + i] = x[i];
