@@ -96,37 +96,10 @@ __device__ __forceinline__ void publish_flag(int* flag) {
 // ---------------------------------------------------------------------------
 enum CholTask : int { CT_TILE = 0, CT_PRE_SUB = 1, CT_PRE_DIAG = 2, CT_SUB = 3 };
 
-// The operands of one wave for one 64-deep product L_ik L_jk^T: rows wr*32
-// + (lane & 15) (+16) of L_ik and wc*32 + (lane & 15) (+16) of L_jk, columns
-// 4m + lane/16, m = 0..15 (the 16x16x4 MFMA operand layout).
-template <typename T, int TB>
-struct ProdOps {
-  T a0[TB / 4], a1[TB / 4], b0[TB / 4], b1[TB / 4];
-  __device__ __forceinline__ void load(const T* __restrict__ Ag, const T* __restrict__ Bg, int kq) {
-#pragma unroll
-    for (int m = 0; m < TB / 4; m++) {
-      const int off = (4 * m + kq) * TB;
-      a0[m] = Ag[off];
-      a1[m] = Ag[off + 16];
-      b0[m] = Bg[off];
-      b1[m] = Bg[off + 16];
-    }
-  }
-  __device__ __forceinline__ void apply(CholAcc<T>& acc) const {
-#pragma unroll
-    for (int m = 0; m < TB / 4; m++) {
-      acc.a[0][0] = Mfma<T>::fma(-a0[m], b0[m], acc.a[0][0]);
-      acc.a[0][1] = Mfma<T>::fma(-a0[m], b1[m], acc.a[0][1]);
-      acc.a[1][0] = Mfma<T>::fma(-a1[m], b0[m], acc.a[1][0]);
-      acc.a[1][1] = Mfma<T>::fma(-a1[m], b1[m], acc.a[1][1]);
-    }
-  }
-};
-
 // acc -= sum_{k < kend} L_ik L_jk^T over published tiles (batched waits).
-// Inside a run of published steps the operands of product k+1 are loaded
-// into a second register set while the MFMAs of product k run, so a
-// product waits for its loads only at the start of a run.
+// Operands go from global memory straight into the MFMA operand registers
+// (a register double buffer across products measured slower: the diagonal
+// chain shares this kernel's register budget).
 template <typename T, int TB>
 __device__ __forceinline__ void acc_update(CholAcc<T>& acc, const T* __restrict__ tiles, int* ready, DevCtl* ctl,
                                            int* shv, int i, int j, int kend, int NT, int lane, int wr, int wc) {
@@ -151,18 +124,20 @@ __device__ __forceinline__ void acc_update(CholAcc<T>& acc, const T* __restrict_
     const int ke = shv[1];
     __syncthreads();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    const int kq = lane >> 4;
-    auto ag = [&](int kk) { return tiles + (size_t)tile_index(i, kk, NT) * T2 + wr * 32 + (lane & 15); };
-    auto bg = [&](int kk) { return tiles + (size_t)tile_index(j, kk, NT) * T2 + wc * 32 + (lane & 15); };
-    ProdOps<T, TB> X, Y;
-    X.load(ag(k), bg(k), kq);
-    while (true) {
-      if (k + 1 < ke) Y.load(ag(k + 1), bg(k + 1), kq);
-      X.apply(acc);
-      if (++k >= ke) break;
-      if (k + 1 < ke) X.load(ag(k + 1), bg(k + 1), kq);
-      Y.apply(acc);
-      if (++k >= ke) break;
+    for (; k < ke; k++) {
+      const T* Ag = tiles + (size_t)tile_index(i, k, NT) * T2 + wr * 32 + (lane & 15);
+      const T* Bg = tiles + (size_t)tile_index(j, k, NT) * T2 + wc * 32 + (lane & 15);
+      const int kq = lane >> 4;
+#pragma unroll 8
+      for (int m = 0; m < TB; m += 4) {
+        const size_t off = (size_t)(m + kq) * TB;
+        const T a0 = -Ag[off], a1 = -Ag[off + 16];
+        const T b0 = Bg[off], b1 = Bg[off + 16];
+        acc.a[0][0] = Mfma<T>::fma(a0, b0, acc.a[0][0]);
+        acc.a[0][1] = Mfma<T>::fma(a0, b1, acc.a[0][1]);
+        acc.a[1][0] = Mfma<T>::fma(a1, b0, acc.a[1][0]);
+        acc.a[1][1] = Mfma<T>::fma(a1, b1, acc.a[1][1]);
+      }
     }
   }
 }
