@@ -172,11 +172,13 @@ __global__ __launch_bounds__(256) void k_chol(TileView tv, T* __restrict__ tiles
       tri_inverse<TB>(R0d, R1d);
       T* Ig = inv + (size_t)j * T2;
 #pragma unroll 4
-      for (int e = threadIdx.x; e < TB * TB; e += 256) {
-        Cg[e] = (T)R0d[e];
-        Ig[e] = (T)R1d[e];
-      }
+      for (int e = threadIdx.x; e < TB * TB; e += 256) Ig[e] = (T)R1d[e];
       publish_flag(ready + tile_index(j, j, NT));
+      // the factor tile itself is read only after the launch (solves): its
+      // stores go out after the flag, so the release above waits only for
+      // the inverse (R0d is overwritten only after the next barrier)
+#pragma unroll 4
+      for (int e = threadIdx.x; e < TB * TB; e += 256) Cg[e] = (T)R0d[e];
       if (j + 1 < NT) {
         // the pre-updated C_{j+1,j+1} (in flight during the product below)
         if (threadIdx.x == 0) {
@@ -190,14 +192,10 @@ __global__ __launch_bounds__(256) void k_chol(TileView tv, T* __restrict__ tiles
         // worker forms the same product for the tile storage (CT_SUB)
         global_to_lds<T, TB>(R0, presub + (size_t)j * T2);
         __syncthreads();
-        CholAcc<T> x;
-#pragma unroll
-        for (int bi = 0; bi < 2; bi++)
-#pragma unroll
-          for (int bj = 0; bj < 2; bj++) x.a[bi][bj] = typename Mfma<T>::acc_t{};
-        acc_gemm_nt<T, double, TB>(x, R0, R1d, false, lane, wr, wc);  // C L_jj^-T
+        TriProd<T> x;
+        tri_gemm_nt<T, double, TB>(x, R0, R1d, lane, wv);  // C L_jj^-T
         __syncthreads();
-        acc_store<T, T, TB>(x, R0, lane, wr, wc);  // kept for step j + 1
+        tri_store<T, TB>(x, R0, lane, wv);  // kept for step j + 1
         __syncthreads();
       }
     }
@@ -221,14 +219,10 @@ __global__ __launch_bounds__(256) void k_chol(TileView tv, T* __restrict__ tiles
       __syncthreads();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       // X = C L_jj^-T with the inverse published by the diagonal chain
-      CholAcc<T> x;
-#pragma unroll
-      for (int bi = 0; bi < 2; bi++)
-#pragma unroll
-        for (int bj = 0; bj < 2; bj++) x.a[bi][bj] = typename Mfma<T>::acc_t{};
-      acc_gemm_nt<T, T, TB>(x, R0, inv + (size_t)j * T2, false, lane, wr, wc);
+      TriProd<T> x;
+      tri_gemm_nt<T, T, TB>(x, R0, inv + (size_t)j * T2, lane, wv);
       __syncthreads();
-      acc_store<T, T, TB>(x, R0, lane, wr, wc);
+      tri_store<T, TB>(x, R0, lane, wv);
       __syncthreads();
       lds_to_global<T, TB>(Cg, R0);
       publish_flag(ready + tile_index(i, j, NT));
@@ -243,14 +237,10 @@ __global__ __launch_bounds__(256) void k_chol(TileView tv, T* __restrict__ tiles
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       global_to_lds<T, TB>(R0, presub + (size_t)j * T2);
       __syncthreads();
-      CholAcc<T> x;
-#pragma unroll
-      for (int bi = 0; bi < 2; bi++)
-#pragma unroll
-        for (int bj = 0; bj < 2; bj++) x.a[bi][bj] = typename Mfma<T>::acc_t{};
-      acc_gemm_nt<T, T, TB>(x, R0, inv + (size_t)j * T2, false, lane, wr, wc);
+      TriProd<T> x;
+      tri_gemm_nt<T, T, TB>(x, R0, inv + (size_t)j * T2, lane, wv);
       __syncthreads();
-      acc_store<T, T, TB>(x, R0, lane, wr, wc);
+      tri_store<T, TB>(x, R0, lane, wv);
       __syncthreads();
       lds_to_global<T, TB>(tiles + (size_t)tile_index(i, j, NT) * T2, R0);
       publish_flag(ready + tile_index(i, j, NT));

@@ -301,6 +301,44 @@ __device__ __forceinline__ void acc_gemm_nt(CholAcc<T>& acc, const T* A, const U
   }
 }
 
+// X = A B^T for TB x TB column-major operands with B lower triangular
+// (B[c][m] = 0 for m > c, as the inverse of a factor tile): wave w forms the
+// 16-row block w of X for the four 16-column blocks, and column block cb
+// accumulates only m < 16 (cb + 1), 40 MFMAs per wave instead of 64.  The
+// terms left out are exact zeros, so X is the same as the full product's.
+// A in LDS (element type T), B in LDS or global (element type U).
+template <typename T>
+struct TriProd {
+  typename Mfma<T>::acc_t a[4];
+};
+
+template <typename T, typename U, int TB>
+__device__ __forceinline__ void tri_gemm_nt(TriProd<T>& x, const T* A, const U* B, int lane, int wv) {
+  static_assert(TB == 64, "four 16-column blocks");
+  const T* As = A + wv * 16 + (lane & 15);
+  const U* Bs = B + (lane & 15);
+  const int kq = lane >> 4;
+#pragma unroll
+  for (int cb = 0; cb < 4; cb++) x.a[cb] = typename Mfma<T>::acc_t{};
+#pragma unroll
+  for (int m = 0; m < TB; m += 4) {
+    const int off = (m + kq) * TB;
+    const T a = As[off];
+#pragma unroll
+    for (int cb = 0; cb < 4; cb++)
+      if (m < 16 * (cb + 1)) x.a[cb] = Mfma<T>::fma(a, (T)Bs[off + 16 * cb], x.a[cb]);
+  }
+}
+
+// TriProd -> LDS tile (column-major)
+template <typename T, int TB>
+__device__ __forceinline__ void tri_store(const TriProd<T>& x, T* D, int lane, int wv) {
+#pragma unroll
+  for (int cb = 0; cb < 4; cb++)
+#pragma unroll
+    for (int vv = 0; vv < 4; vv++) D[(16 * cb + (lane & 15)) * TB + 16 * wv + Mfma<T>::row(vv, lane)] = x.a[cb][vv];
+}
+
 // tile copies between global memory and LDS: every load of a thread is
 // issued before the first use (the copies sit on the diagonal chain)
 template <typename T, int TB>
