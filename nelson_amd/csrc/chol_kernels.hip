@@ -351,8 +351,14 @@ __global__ __launch_bounds__(256) void k_trsv(TileView tv, const T* __restrict__
       }
       if (Gemv::owner()) hs[Gemv::index()] = h;
       __syncthreads();
+      // publish the previous step's solution here: its stores (and the tile
+      // prefetch, consumed above) are complete by now, so the release does
+      // not wait on this step's path; the destinations need x only two
+      // steps later
+      if (s > 0) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
       if (tid < TB) xs[tid] = (k * TB + tid < n) ? r0 - (pk + hs[tid]) : 0.0;
       __syncthreads();
+      if (s > 0 && tid == 0) __hip_atomic_store(xready + blk(s - 1), 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       const double x = gd.apply(xs, red);
       if (Gemv::owner()) {
         const int e = Gemv::index();
@@ -360,9 +366,9 @@ __global__ __launch_bounds__(256) void k_trsv(TileView tv, const T* __restrict__
         if (in) sol[k * TB + e] = x;
         xr[s % kNear][e] = in ? x : 0.0;
       }
-      publish_flag(xready + k);
       if (s + 1 < NR) prefetch(s + 1);
     }
+    publish_flag(xready + blk(NR - 1));
     return;
   }
   // ---- destination sums ----
