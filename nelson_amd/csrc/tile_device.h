@@ -37,6 +37,15 @@ struct Mfma<float> {
   static __device__ __forceinline__ int row(int v, int lane) { return 4 * (lane >> 4) + v; }
 };
 
+// orders a wave's LDS write before its later reads of other lanes' entries
+// (wavefront-scope fences: no instructions, since one wave's LDS operations
+// execute in order, but the compiler keeps the order)
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // orders this wave's LDS accesses (cross-lane exchange through LDS inside one
 // wave without a workgroup barrier)
 __device__ __forceinline__ void wave_lds_sync() {
@@ -115,7 +124,7 @@ __device__ __forceinline__ void potrf_panel(double* A, double* col, int g0, int 
         // instructions of this one wave, which the LDS executes in order
         double* cb = col + (k & 1) * 64;
         cb[lane] = a[k];
-        __builtin_amdgcn_wave_barrier();
+        wave_sync();
 #pragma unroll
         for (int c = k + 2; c < kPW; c++) a[c] -= a[k] * cb[C0 + c];
       }
