@@ -331,13 +331,10 @@ __global__ __launch_bounds__(256) void k_trsv(TileView tv, const T* __restrict__
       // the near blocks' contributions, one reduction for all of them
       double h = 0.0;
       {
-        const int base = BWD ? (tid & 3) * 16 : (tid >> 6) * 16;
 #pragma unroll
         for (int d = 1; d <= kNear; d++) {
           if (d > s) continue;
-          const double* xv = xr[(s - d) % kNear];
-#pragma unroll
-          for (int q = 0; q < 16; q++) h += (double)gn[d - 1].a[q] * xv[base + q];
+          h += gn[d - 1].partial(xr[(s - d) % kNear]);
         }
         if (BWD) {
           h += __shfl_xor(h, 1);
