@@ -146,14 +146,14 @@ void seq_schur_solve(nsn_problem* h, int gate, const double* rhsU, const double*
                      int add, const double* bden) {
   hipStream_t st = h->st;
   const int NR = h->NR;
-  int* xready = h->flags + h->tv.ntiles;
-  int* tready = xready + NR;
-  launch_ir_reduce<T>(h->view, (const T*)h->W, (const T*)h->Vinv, rhsU, rhsV, h->rS, h->ctl, gate, st);
-  launch_chol_reset<T>(xready, 2 * NR, h->counters + 1, h->ctl, gate, st);
-  launch_trsv<T>(h->tv, (const T*)h->tiles, (const T*)h->inv, false, h->rS, h->zS, h->parts, xready, tready,
-                 h->ctl, gate, kTrsvMaxWG, st);
-  launch_chol_reset<T>(xready, 2 * NR, h->counters + 1, h->ctl, gate, st);
-  launch_trsv<T>(h->tv, (const T*)h->tiles, (const T*)h->inv, true, h->zS, xU, h->parts, xready, tready,
+  // flags of the forward and of the backward solve, cleared by the reduction
+  int* xready_f = h->flags + h->tv.ntiles;
+  int* xready_b = h->flags + h->tv.ntiles + 2 * NR + 2 * h->tv.NT;
+  launch_ir_reduce<T>(h->view, (const T*)h->W, (const T*)h->Vinv, rhsU, rhsV, h->rS, xready_f, xready_b, 2 * NR,
+                      h->ctl, gate, st);
+  launch_trsv<T>(h->tv, (const T*)h->tiles, (const T*)h->inv, false, h->rS, h->zS, h->parts, xready_f,
+                 xready_f + NR, h->ctl, gate, kTrsvMaxWG, st);
+  launch_trsv<T>(h->tv, (const T*)h->tiles, (const T*)h->inv, true, h->zS, xU, h->parts, xready_b, xready_b + NR,
                  h->ctl, gate, kTrsvMaxWG, st);
   launch_backsub<T>(h->view, (const T*)h->W, (const T*)h->Vinv, (const T*)h->bV, xU, xV, h->scr, h->ctl, gate, st,
                     rhsV, add, bden);
@@ -335,8 +335,9 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
   h->n_s_parts = nparts;
   h->NR = (n + kTB - 1) / kTB;
   // flags: ready (ntiles), the triangular solves' xready / tready (NR each),
-  // pre_sub / pre_diag (NT each)
-  h->nflags = tv.ntiles + 2 * h->NR + 2 * tv.NT;
+  // pre_sub / pre_diag (NT each), the backward refinement solve's xready /
+  // tready (NR each)
+  h->nflags = tv.ntiles + 2 * h->NR + 2 * tv.NT + 2 * h->NR;
   int n_tasks = 0;
   chol_task_list(tv.NT, 0, nullptr, &n_tasks);  // the count does not depend on the look-ahead
   const int ahead = chol_lookahead(std::min(n_tasks, kCholMaxWG - 1));

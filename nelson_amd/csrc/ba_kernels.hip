@@ -358,12 +358,9 @@ __global__ __launch_bounds__(256) void k_backsub(BAView v, const T* __restrict__
 // FP64 gradient b is written as well: the gain-ratio denominator
 // x^T (mu x - b) uses it (DoubleSection.h:300-311).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_ir_res_points(BAView v, const double* __restrict__ xU,
-                                                       const double* __restrict__ xV, double* __restrict__ rV,
-                                                       double* __restrict__ bV64, DevCtl* ctl, int gate) {
-  if (!gate_open(ctl, gate)) return;
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= v.P) return;
+__device__ __forceinline__ void ir_res_point(const BAView& v, const double* __restrict__ xU,
+                                             const double* __restrict__ xV, double* __restrict__ rV,
+                                             double* __restrict__ bV64, const DevCtl* ctl, int p) {
   const int pv = v.pt_var[p];
   if (pv < 0) return;
   const double X0 = v.pts[3 * p], X1 = v.pts[3 * p + 1], X2 = v.pts[3 * p + 2];
@@ -421,13 +418,12 @@ __global__ __launch_bounds__(256) void k_ir_res_points(BAView v, const double* _
 }
 
 // one workgroup per variable camera; deterministic reduction
-__global__ __launch_bounds__(256) void k_ir_res_cams(BAView v, const double* __restrict__ xU,
-                                                     const double* __restrict__ xV, double* __restrict__ rU,
-                                                     double* __restrict__ bU64, DevCtl* ctl, int gate) {
-  if (!gate_open(ctl, gate)) return;
-  __shared__ double red[4][18];
-  __shared__ double pri[18];  // the camera's unary prior: J^T r, J^T J x, diag(J^T J)
-  const int u = blockIdx.x;
+// red: 4 x 18 doubles of LDS; pri: 18 (the camera's unary prior: J^T r,
+// J^T J x, diag(J^T J)); the whole workgroup calls it for one camera
+__device__ __forceinline__ void ir_res_cam(const BAView& v, const double* __restrict__ xU,
+                                           const double* __restrict__ xV, double* __restrict__ rU,
+                                           double* __restrict__ bU64, const DevCtl* ctl, int u, double (*red)[18],
+                                           double* pri) {
   const int c = v.var_cam_user[u];
   double R[kCamRec];
   load_cam(v, c, R);
@@ -510,12 +506,39 @@ __global__ __launch_bounds__(256) void k_ir_res_cams(BAView v, const double* __r
   }
 }
 
+// the FP64 residual of the damped normal equations, res = -b - (H + D) x,
+// matrix free from the edges, in one launch: the first nbp workgroups take
+// the points (one thread per point), the next NU one variable camera each
+__global__ __launch_bounds__(256) void k_ir_res(BAView v, const double* __restrict__ xU,
+                                                const double* __restrict__ xV, double* __restrict__ rU,
+                                                double* __restrict__ rV, double* __restrict__ bU64,
+                                                double* __restrict__ bV64, int nbp, DevCtl* ctl, int gate) {
+  if (!gate_open(ctl, gate)) return;
+  __shared__ double red[4][18];
+  __shared__ double pri[18];
+  if ((int)blockIdx.x < nbp) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < v.P) ir_res_point(v, xU, xV, rV, bV64, ctl, p);
+  } else {
+    ir_res_cam(v, xU, xV, rU, bU64, ctl, (int)blockIdx.x - nbp, red, pri);
+  }
+}
+
 // rS_u = rU_u - sum_{k in W blocks of u} W_k V^-1 rV_{point(k)}; one workgroup per camera
 template <typename T>
 __global__ __launch_bounds__(256) void k_ir_reduce(BAView v, const T* __restrict__ W, const T* __restrict__ Vinv,
                                                    const double* __restrict__ rU, const double* __restrict__ rV,
-                                                   double* __restrict__ rS, DevCtl* ctl, int gate) {
+                                                   double* __restrict__ rS, int* clr0, int* clr1, int nclr, DevCtl* ctl,
+                                                   int gate) {
   if (!gate_open(ctl, gate)) return;
+  {
+    // the flags of the two triangular solves that follow (nclr <= 256 NU)
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < nclr) {
+      clr0[g] = 0;
+      clr1[g] = 0;
+    }
+  }
   __shared__ double red[4][6];
   const int u = blockIdx.x;
   double acc[6] = {0, 0, 0, 0, 0, 0};
@@ -868,17 +891,17 @@ void launch_backsub(const BAView& v, const T* W, const T* Vinv, const T* bV, con
 
 void launch_ir_residual(const BAView& v, const double* xU, const double* xV, double* rU, double* rV, double* bU64,
                         double* bV64, DevCtl* ctl, int gate, hipStream_t st) {
-  if (v.P > 0)
-    hipLaunchKernelGGL(k_ir_res_points, dim3(nblocks(v.P, 256)), dim3(256), 0, st, v, xU, xV, rV, bV64, ctl, gate);
-  if (v.NU > 0)
-    hipLaunchKernelGGL(k_ir_res_cams, dim3(v.NU), dim3(256), 0, st, v, xU, xV, rU, bU64, ctl, gate);
+  const int nbp = v.P > 0 ? (int)nblocks(v.P, 256) : 0;
+  if (nbp + v.NU > 0)
+    hipLaunchKernelGGL(k_ir_res, dim3(nbp + v.NU), dim3(256), 0, st, v, xU, xV, rU, rV, bU64, bV64, nbp, ctl, gate);
 }
 
 template <typename T>
 void launch_ir_reduce(const BAView& v, const T* W, const T* Vinv, const double* rU, const double* rV, double* rS,
-                      DevCtl* ctl, int gate, hipStream_t st) {
+                      int* clr0, int* clr1, int nclr, DevCtl* ctl, int gate, hipStream_t st) {
   if (v.NU == 0) return;
-  hipLaunchKernelGGL((k_ir_reduce<T>), dim3(v.NU), dim3(256), 0, st, v, W, Vinv, rU, rV, rS, ctl, gate);
+  hipLaunchKernelGGL((k_ir_reduce<T>), dim3(v.NU), dim3(256), 0, st, v, W, Vinv, rU, rV, rS, clr0, clr1, nclr, ctl,
+                     gate);
 }
 
 void launch_adj_eps(const double* lu, long long nu, const double* lv, long long nv, double* eps, hipStream_t st) {
@@ -983,7 +1006,7 @@ void launch_zero(T* p, long long n, hipStream_t st) {
                                   const Scratch&, DevCtl*, int, hipStream_t, const double*, int,             \
                                   const double*);                                                            \
   template void launch_ir_reduce<T>(const BAView&, const T*, const T*, const double*, const double*, double*, \
-                                    DevCtl*, int, hipStream_t);                                              \
+                                    int*, int*, int, DevCtl*, int, hipStream_t);                             \
   template void launch_xu_stats<T>(int, const double*, const T*, const Scratch&, DevCtl*, int, hipStream_t); \
   template void launch_zero<T>(T*, long long, hipStream_t);
 

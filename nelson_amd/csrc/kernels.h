@@ -120,7 +120,7 @@ void launch_ir_residual(const BAView& v, const double* xU, const double* xV, dou
 // rS = rU - W V^-1 rV (Schur right-hand side of a correction)
 template <typename T>
 void launch_ir_reduce(const BAView& v, const T* W, const T* Vinv, const double* rU, const double* rV, double* rS,
-                      DevCtl* ctl, int gate, hipStream_t st);
+                      int* clr0, int* clr1, int nclr, DevCtl* ctl, int gate, hipStream_t st);
 // x += z
 void launch_add(double* x, const double* z, long long n, DevCtl* ctl, int gate, hipStream_t st);
 // out = a x + b y (out may alias x or y)
