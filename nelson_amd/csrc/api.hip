@@ -140,10 +140,11 @@ void seq_cost(nsn_problem* h, int gate) {
 
 // xU = (S^-1 (rhsU - W V^-1 rhsV)), xV = V^-1 (rhsV - W^T xU) with the
 // current factorisation (xV += ... with add); the increment statistics of the
-// back-substitution use bden when given
+// back-substitution use bden when given; accU, when given, gets xU added
+// (the refinement's camera update, done by the back-substitution kernel)
 template <typename T>
 void seq_schur_solve(nsn_problem* h, int gate, const double* rhsU, const double* rhsV, double* xU, double* xV,
-                     int add, const double* bden) {
+                     int add, const double* bden, double* accU = nullptr) {
   hipStream_t st = h->st;
   const int NR = h->NR;
   // flags of the forward and of the backward solve, cleared by the reduction
@@ -156,7 +157,7 @@ void seq_schur_solve(nsn_problem* h, int gate, const double* rhsU, const double*
   launch_trsv<T>(h->tv, (const T*)h->tiles, (const T*)h->inv, true, h->zS, xU, h->parts, xready_b, xready_b + NR,
                  h->ctl, gate, kTrsvMaxWG, st);
   launch_backsub<T>(h->view, (const T*)h->W, (const T*)h->Vinv, (const T*)h->bV, xU, xV, h->scr, h->ctl, gate, st,
-                    rhsV, add, bden);
+                    rhsV, add, bden, accU, accU ? h->tv.n : 0);
 }
 
 // computeIncrement: V^-1, Z, S, factorisation, solves, back-substitution
@@ -191,8 +192,7 @@ void seq_schur(nsn_problem* h, int gate, int checks, bool timed) {
   for (int it = 0; it < ir; it++) {
     // x += M^-1 (-b - (H + D) x), M the FP32 Schur solve
     launch_ir_residual(h->view, h->xU, h->xV, h->rU, h->rV, h->bU64, h->bV64, h->ctl, gate, st);
-    seq_schur_solve<T>(h, gate, h->rU, h->rV, h->zU, h->xV, 1, h->bV64);
-    launch_add(h->xU, h->zU, h->tv.n, h->ctl, gate, st);
+    seq_schur_solve<T>(h, gate, h->rU, h->rV, h->zU, h->xV, 1, h->bV64, h->xU);
   }
   if (ir > 0) launch_xu_stats<double>(h->NU, h->xU, h->bU64, h->scr, h->ctl, gate, st);
   else launch_xu_stats<T>(h->NU, h->xU, (const T*)h->bU, h->scr, h->ctl, gate, st);

@@ -332,10 +332,14 @@ __global__ __launch_bounds__(256) void k_backsub(BAView v, const T* __restrict__
                                                  const T* __restrict__ bV, const double* __restrict__ xU,
                                                  double* __restrict__ xV, double* __restrict__ part,
                                                  DevCtl* ctl, int gate, const double* __restrict__ rhs_v,
-                                                 int add, const double* __restrict__ bden) {
+                                                 int add, const double* __restrict__ bden,
+                                                 double* __restrict__ accU, int nacc) {
   if (!gate_open(ctl, gate)) return;
   __shared__ double sh[4];
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  // refinement: the camera correction is added to the solution here
+  // (accU += xU, grid-stride), saving a launch
+  for (int g = q; g < nacc; g += gridDim.x * blockDim.x) accU[g] += xU[g];
   double inc = 0.0, den = 0.0;
   if (q < v.NV) backsub_point<T>(v, q, W, Vinv, bV, xU, xV, ctl->abs_, rhs_v, add, bden, inc, den);
   const double a = block_sum256(inc, sh);
@@ -883,10 +887,13 @@ void launch_s_tiles(const TileView& tv, int nparts, const int* tile_order, const
 template <typename T>
 void launch_backsub(const BAView& v, const T* W, const T* Vinv, const T* bV, const double* xU, double* xV,
                     const Scratch& s, DevCtl* ctl, int gate, hipStream_t st, const double* rhs_v, int add,
-                    const double* bden) {
-  if (v.NV == 0) return;
+                    const double* bden, double* accU, int nacc) {
+  if (v.NV == 0) {
+    if (accU && nacc > 0) launch_add(accU, xU, nacc, ctl, gate, st);
+    return;
+  }
   hipLaunchKernelGGL((k_backsub<T>), dim3(nblocks(v.NV, 256)), dim3(256), 0, st, v, W, Vinv, bV, xU, xV,
-                     s.part_xv, ctl, gate, rhs_v, add, bden);
+                     s.part_xv, ctl, gate, rhs_v, add, bden, accU, accU ? nacc : 0);
 }
 
 void launch_ir_residual(const BAView& v, const double* xU, const double* xV, double* rU, double* rV, double* bU64,
@@ -1004,7 +1011,7 @@ void launch_zero(T* p, long long n, hipStream_t st) {
                                   const T*, const T*, T*, DevCtl*, int, hipStream_t);                        \
   template void launch_backsub<T>(const BAView&, const T*, const T*, const T*, const double*, double*,       \
                                   const Scratch&, DevCtl*, int, hipStream_t, const double*, int,             \
-                                  const double*);                                                            \
+                                  const double*, double*, int);                                              \
   template void launch_ir_reduce<T>(const BAView&, const T*, const T*, const double*, const double*, double*, \
                                     int*, int*, int, DevCtl*, int, hipStream_t);                             \
   template void launch_xu_stats<T>(int, const double*, const T*, const Scratch&, DevCtl*, int, hipStream_t); \

@@ -108,11 +108,13 @@ void launch_s_tiles(const TileView& tv, int nparts, const int* tile_order, const
                     hipStream_t st);
 // xV = V^-1 (rhsV - W^T xU) with rhsV = -bV when rhs_v is null; with `add`
 // the result is added to xV; the increment statistics use bden (FP64
-// gradient) when given, else bV
+// gradient) when given, else bV; accU (nacc entries), when given, gets xU
+// added (the refinement's camera update)
 template <typename T>
 void launch_backsub(const BAView& v, const T* W, const T* Vinv, const T* bV, const double* xU,
                     double* xV, const Scratch& s, DevCtl* ctl, int gate, hipStream_t st,
-                    const double* rhs_v = nullptr, int add = 0, const double* bden = nullptr);
+                    const double* rhs_v = nullptr, int add = 0, const double* bden = nullptr,
+                    double* accU = nullptr, int nacc = 0);
 // FP64 residual of the damped normal equations at the increment (xU, xV):
 // rU/rV = -b - (H + D) x from the edges, and the FP64 gradient bU64/bV64
 void launch_ir_residual(const BAView& v, const double* xU, const double* xV, double* rU, double* rV,
