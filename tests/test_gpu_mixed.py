@@ -93,3 +93,26 @@ def test_mixed_lm_final_cost(cfg1):
         print(f"\nmixed LM ({ir} refinement steps): final cost {lm.result()['final_chi2']:.10g} vs oracle "
               f"{ro['final_chi2']:.10g}, rel err {err:.2e}, iterations {lm.numIterations()} vs {ro['iterations']}")
         assert err <= 1e-6
+
+
+def test_refinement_without_variable_points(cfg1):
+    """Every point fixed (no V blocks, S = U): the refinement's camera update
+    then runs without the back-substitution kernel.  After two refinement
+    steps the increment meets the FP64 bar against the oracle, and the mixed
+    LM reaches the oracle's final cost (bar 1e-6)."""
+    pf = np.ones(cfg1["n_points"], np.uint8)
+    o = OracleBA.from_generated(cfg1, pt_fixed=pf)
+    o.update(True)
+    ro = o.schur_step(0.0, 1.0)
+    p = BAProblem.from_generated(cfg1, precision="mixed", pt_fixed=pf)
+    p.update(True)
+    ok, (dxu, dxv) = _step(p, 0.0, 1.0)
+    assert ok and np.asarray(dxv).size == 0
+    err = relerr(dxu, ro["xU"])
+    print(f"no variable points: increment error {err:.2e}")
+    assert err <= 1e-9
+    lm = LevenbergMarquardt(LevenbergMarquardtSettings(maxNumIt=30))
+    lm.solve(p)
+    rl = OracleBA.from_generated(cfg1, pt_fixed=pf).lm_solve(maxNumIt=30)
+    cost = lm.result()["final_chi2"]
+    assert abs(cost - rl["final_chi2"]) <= 1e-6 * rl["final_chi2"], (cost, rl["final_chi2"])
