@@ -332,10 +332,15 @@ double or_max_abs_h_diag(const or_hessian* h) {  /* SolverCholeskySchur.hpp:75-7
   return mv > mu ? mv : mu;
 }
 
-/* the assembly part of or_schur_step (V solves, bS, damped S) without the
- * factorisation and solves: the CPU baseline times the factorisation of the
- * assembled S separately.  S_out (nu x nu) and bS_out may not be NULL. */
-int or_schur_assemble(const or_hessian* h, double rel, double abs_, double* S_out, double* bS_out) {
+/* The reduction half of the Schur step (nelson/SolverCholeskySchur.hpp:91-124):
+ * the damped V factor per point block (vf, BV x BV column-major lower LDL^T),
+ * xv0 = V^-1 (-bV), bS = -bU - W xv0 and the damped S = (U - W V^-1 W^T).upper,
+ * row-major nu x nu.  Returns 0 if a V block fails; the caller frees
+ * *vf, *xv0, *S and *bS (NULL on failure).  or_schur_assemble and
+ * or_schur_step both call it, so the S the CPU baseline times is the S the
+ * step factorises. */
+static int schur_reduce(const or_hessian* h, double rel, double abs_, double** vf_out, double** xv0_out,
+                        double** S_out, double** bS_out) {
   const int BU = h->BU, BV = h->BV, NV = h->NV, nW = h->nW;
   const size_t nu = (size_t)BU * h->NU, nv = (size_t)BV * NV;
   int ok = 1;
@@ -359,7 +364,7 @@ int or_schur_assemble(const or_hessian* h, double rel, double abs_, double* S_ou
       ldlt_nopiv_solve(a, BV, xv0 + (size_t)p * BV);
     }
   }
-  double* S = NULL; double* bS = NULL; int* stp = NULL;
+  double* S = NULL; double* bS = NULL;
   if (ok) {
     /* bS = -bU - W * x0   (SolverCholeskySchur.hpp:114) */
     bS = (double*)malloc(nu * sizeof(double) + 8);
@@ -392,14 +397,23 @@ int or_schur_assemble(const or_hessian* h, double rel, double abs_, double* S_ou
     /* damping (:121-124) */
     if (rel != 0 || abs_ != 0)
       for (size_t i = 0; i < nu; i++) { S[i * nu + i] += rel * h->U[i * nu + i]; S[i * nu + i] += abs_; }
-    if (S_out) memcpy(S_out, S, nu * nu * sizeof(double));
-    if (bS_out) memcpy(bS_out, bS, nu * sizeof(double));
   }
-  free(vf);
-  free(xv0);
-  free(S);
-  free(bS);
-  (void)stp;
+  *vf_out = vf; *xv0_out = xv0; *S_out = S; *bS_out = bS;
+  return ok;
+}
+
+/* the reduction alone (no factorisation or solves): the CPU baseline times
+ * the factorisation of the assembled S separately.  S_out (nu x nu) and
+ * bS_out may not be NULL. */
+int or_schur_assemble(const or_hessian* h, double rel, double abs_, double* S_out, double* bS_out) {
+  const size_t nu = (size_t)h->BU * h->NU;
+  double *vf, *xv0, *S, *bS;
+  int ok = schur_reduce(h, rel, abs_, &vf, &xv0, &S, &bS);
+  if (ok) {
+    memcpy(S_out, S, nu * nu * sizeof(double));
+    memcpy(bS_out, bS, nu * sizeof(double));
+  }
+  free(vf); free(xv0); free(S); free(bS);
   return ok;
 }
 
@@ -407,60 +421,10 @@ int or_schur_step(const or_hessian* h, double rel, double abs_, double* xU, doub
                   double* S_out, double* bS_out) {
   const int BU = h->BU, BV = h->BV, NV = h->NV, nW = h->nW;
   const size_t nu = (size_t)BU * h->NU, nv = (size_t)BV * NV;
-  int ok = 1;
-  /* V solver: per diagonal block, damped LDL^T without pivoting in natural
-   * order, failing on an exactly zero pivot: the simplicial LDL^T that
-   * SolverCholeskySchur uses as its block-diagonal V solver
-   * (SolverCholeskySparse.hpp:41-69, SolverCholeskySparse.h:16):
-   * x0 = V^-1 (-bV) */
-  double* vf = (double*)malloc((size_t)NV * BV * BV * sizeof(double) + 8);
-  double* xv0 = (double*)malloc(nv * sizeof(double) + 8);
-  for (int p = 0; p < NV && ok; p++) {
-    const double* v = h->V + (size_t)p * BV * BV;
-    double* a = vf + (size_t)p * BV * BV;
-    for (int i = 0; i < BV; i++)
-      for (int j = 0; j < BV; j++) a[j * BV + i] = (i >= j) ? v[j * BV + i] : 0.0;
-    if (rel != 0 || abs_ != 0)
-      for (int i = 0; i < BV; i++) a[i * BV + i] = a[i * BV + i] * (1 + rel) + abs_;
-    ok = ldlt_nopiv_factor(a, BV);
-    if (ok) {
-      for (int i = 0; i < BV; i++) xv0[(size_t)p * BV + i] = -h->bV[(size_t)p * BV + i];
-      ldlt_nopiv_solve(a, BV, xv0 + (size_t)p * BV);
-    }
-  }
-  double* S = NULL; double* bS = NULL; int* stp = NULL;
+  double *vf, *xv0, *S, *bS;
+  int* stp = NULL;
+  int ok = schur_reduce(h, rel, abs_, &vf, &xv0, &S, &bS);
   if (ok) {
-    /* bS = -bU - W * x0   (SolverCholeskySchur.hpp:114) */
-    bS = (double*)malloc(nu * sizeof(double) + 8);
-    double* t = (double*)calloc(nu + 1, sizeof(double));
-    for (int k = 0; k < nW; k++) {
-      const double* w = h->W + (size_t)k * BU * BV;
-      for (int r = 0; r < BU; r++) {
-        double s = 0;
-        for (int c = 0; c < BV; c++) s += w[r * BV + c] * xv0[(size_t)h->Wj[k] * BV + c];
-        t[(size_t)h->Wi[k] * BU + r] += s;
-      }
-    }
-    for (size_t i = 0; i < nu; i++) bS[i] = -h->bU[i] - t[i];
-    free(t);
-    /* S = (U - W * V^-1 W^T).upper  (:118), Z_k = V^-1 W_k^T via the V factor */
-    double* Z = (double*)malloc((size_t)nW * BU * BV * sizeof(double) + 8); /* Z_k stored BU x BV = (V^-1 W_k^T)^T */
-    for (int k = 0; k < nW; k++) {
-      const double* w = h->W + (size_t)k * BU * BV;
-      int p = h->Wj[k];
-      for (int r = 0; r < BU; r++) {
-        double x[8];
-        for (int c = 0; c < BV; c++) x[c] = w[r * BV + c];
-        ldlt_nopiv_solve(vf + (size_t)p * BV * BV, BV, x);
-        for (int c = 0; c < BV; c++) Z[(size_t)k * BU * BV + r * BV + c] = x[c];
-      }
-    }
-    S = (double*)malloc(nu * nu * sizeof(double) + 8);
-    or_wwt(h->NU, BU, BV, nW, h->Wi, h->Wj, h->W, Z, h->U, S);
-    free(Z);
-    /* damping (:121-124) */
-    if (rel != 0 || abs_ != 0)
-      for (size_t i = 0; i < nu; i++) { S[i * nu + i] += rel * h->U[i * nu + i]; S[i * nu + i] += abs_; }
     if (S_out) memcpy(S_out, S, nu * nu * sizeof(double));
     if (bS_out) memcpy(bS_out, bS, nu * sizeof(double));
     /* factorize (:136); row-major upper == column-major lower */

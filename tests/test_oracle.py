@@ -521,3 +521,156 @@ def test_priors_cost_gradient_hessian_vs_finite_differences():
     assert np.abs((H1["bU"] - H0["bU"]) - Jtr[:6 * NU]).max() <= 1e-6 * max(1.0, np.abs(Jtr).max())
     assert np.abs((H1["bV"] - H0["bV"]) - Jtr[6 * NU:]).max() <= 1e-6 * max(1.0, np.abs(Jtr).max())
     assert np.array_equal(H1["W"], H0["W"])
+
+
+# ---------------------------------------------------------------------------
+# per-edge weights and the Huber loss of the BA edge, pinned independently:
+# the cost from a numpy projection with a numpy Huber, its gradient by central
+# differences under the section's oplus, the blocks from a dense assembly
+# with the IRLS weight w * psi computed in numpy
+# ---------------------------------------------------------------------------
+def _huber_rho(s, delta):
+    """rho(s) of the squared residual norm s: s inside delta^2, the linear
+    branch 2 delta sqrt(s) - delta^2 outside."""
+    s = np.asarray(s, np.float64)
+    return np.where(s <= delta * delta, s, 2.0 * delta * np.sqrt(s) - delta * delta)
+
+
+def _huber_psi(s, delta):
+    """rho'(s): 1 inside, delta / sqrt(s) outside."""
+    s = np.asarray(s, np.float64)
+    return np.where(s <= delta * delta, 1.0, delta / np.sqrt(np.maximum(s, 1e-300)))
+
+
+def _weighted_huber_problem():
+    g = generate_ba(small_cfg(n_cameras=5, n_points=40, k_min=2, k_max=4, seed=11, outlier_frac=0.3,
+                              outlier_sigma=8.0))
+    # at the ground truth: N(0, 1 px) residuals mostly inside delta, the
+    # outliers outside
+    g = dict(g, cams0=g["cams_gt"].copy(), pts0=g["pts_gt"].copy())
+    rng = np.random.default_rng(12)
+    w = rng.uniform(0.3, 2.0, g["n_edges"])
+    return g, w, 2.0
+
+
+def _cams_rt_from_aa(cams6):
+    C = len(cams6) // 6
+    cr = np.zeros(12 * C)
+    for c in range(C):
+        cr[12 * c:12 * c + 9] = Rotation.from_rotvec(cams6[6 * c:6 * c + 3]).as_matrix().reshape(-1)
+        cr[12 * c + 9:12 * c + 12] = cams6[6 * c + 3:6 * c + 6]
+    return cr
+
+
+def _numpy_cost(g, w, delta, cams_rt, pts):
+    r = _numpy_residuals(g, cams_rt, pts)
+    return float(np.sum(w * _huber_rho(np.sum(r * r, axis=1), delta)))
+
+
+def test_weighted_huber_cost_and_residual_split():
+    """The oracle's cost equals sum_e w_e rho(|r_e|^2) from an independent
+    numpy projection and Huber function, with residuals on both sides of delta."""
+    g, w, delta = _weighted_huber_problem()
+    o = OracleBA.from_generated(g, weight=w, huber_delta=delta)
+    cr = _cams_rt_from_aa(g["cams0"])
+    r = _numpy_residuals(g, cr, g["pts0"])
+    s = np.sum(r * r, axis=1)
+    assert np.sum(s > delta * delta) >= 10 and np.sum(s <= delta * delta) >= 10
+    assert abs(o.update(False) - _numpy_cost(g, w, delta, cr, g["pts0"])) < 1e-12 * o.update(False)
+
+
+def test_weighted_huber_gradient_is_half_the_cost_gradient():
+    """b = sum_e w_e psi_e J_e^T r_e is half the gradient of the cost under
+    the section's perturbations (left SE(3) update of the variable cameras,
+    additive points): central differences (h = 1e-6) of the numpy cost."""
+    g, w, delta = _weighted_huber_problem()
+    o = OracleBA.from_generated(g, weight=w, huber_delta=delta)
+    o.update(True)
+    H = o.hessian()
+    cams, pts = o.get_state_rt()
+    h = 1e-6
+    fd_u = np.zeros(6 * o.NU)
+    for c in range(1, g["n_cameras"]):  # camera 0 is fixed
+        for k in range(6):
+            vals = []
+            for sgn in (1.0, -1.0):
+                d = np.zeros(6)
+                d[k] = sgn * h
+                Rm, tm = cams[12 * c:12 * c + 9].copy(), cams[12 * c + 9:12 * c + 12].copy()
+                orc.lib().or_se3_left_update(orc.dp(d), orc.dp(Rm), orc.dp(tm))
+                cc = cams.copy()
+                cc[12 * c:12 * c + 9], cc[12 * c + 9:12 * c + 12] = Rm, tm
+                vals.append(_numpy_cost(g, w, delta, cc, pts))
+            fd_u[6 * (c - 1) + k] = (vals[0] - vals[1]) / (2 * h)
+    fd_v = np.zeros(3 * o.NV)
+    for q in range(1, g["n_points"]):  # point 0 is fixed
+        for k in range(3):
+            pp, pm = pts.copy(), pts.copy()
+            pp[3 * q + k] += h
+            pm[3 * q + k] -= h
+            fd_v[3 * (q - 1) + k] = (_numpy_cost(g, w, delta, cams, pp) - _numpy_cost(g, w, delta, cams, pm)) / (2 * h)
+    grad = 2.0 * np.concatenate([H["bU"], H["bV"]])
+    assert relerr(grad, np.concatenate([fd_u, fd_v])) < 1e-6
+
+
+def test_weighted_huber_blocks_equal_dense_irls_assembly():
+    """U, V, W, bU, bV equal J^T Omega J and J^T Omega r with
+    Omega = diag(w_e psi(|r_e|^2)) computed in numpy (J from the oracle's
+    per-edge Jacobians, which central differences pin separately)."""
+    g, w, delta = _weighted_huber_problem()
+    o = OracleBA.from_generated(g, weight=w, huber_delta=delta)
+    o.update(True)
+    r, Jc, Jp = o.edge_jacobians()
+    cr = _cams_rt_from_aa(g["cams0"])
+    assert relerr(r, _numpy_residuals(g, cr, g["pts0"])) < 1e-10
+    om = w * _huber_psi(np.sum(r * r, axis=1), delta)
+    NU, NV = o.NU, o.NV
+    J = np.zeros((2 * g["n_edges"], 6 * NU + 3 * NV))
+    for e in range(g["n_edges"]):
+        c, q = g["cam_idx"][e], g["pt_idx"][e]
+        if c > 0:
+            J[2 * e:2 * e + 2, 6 * (c - 1):6 * c] = Jc[e]
+        if q > 0:
+            J[2 * e:2 * e + 2, 6 * NU + 3 * (q - 1):6 * NU + 3 * q] = Jp[e]
+    Om = np.repeat(om, 2)
+    Hd, b = dense_system(oracle_blocks(o), NU, NV)
+    assert relerr(Hd, J.T @ (Om[:, None] * J)) < 1e-12
+    assert relerr(b, J.T @ (Om * r.reshape(-1))) < 1e-12
+    # an unweighted, Huber-free build of the same problem gives different blocks
+    o2 = OracleBA.from_generated(g)
+    o2.update(True)
+    Hd2, _ = dense_system(oracle_blocks(o2), NU, NV)
+    assert relerr(Hd2, Hd) > 1e-3
+
+
+@pytest.mark.parametrize("wt", [1.0, 0.7])
+def test_huber_continuous_at_delta(wt):
+    """rho and the gradient are continuous where |r| crosses delta: one
+    camera, one point, the observation placed at |r| = delta (1 -+ 1e-9)."""
+    delta = 2.0
+    cams6 = np.array([0.1, -0.2, 0.05, 0.3, -0.1, 6.0])
+    pt = np.array([0.2, 0.1, 0.5])
+    cr = _cams_rt_from_aa(cams6)
+    Xc = cr[:9].reshape(3, 3) @ pt + cr[9:]
+    proj = 500.0 * Xc[:2] / Xc[2] + np.array([320.0, 240.0])
+    vals = []
+    for eps in (-1e-9, 1e-9):
+        z = proj - delta * (1.0 + eps) * np.array([0.6, 0.8])
+        o = OracleBA(2, 2, [1], [1], z, 500.0, 320.0, 240.0, np.concatenate([np.zeros(6), cams6]),
+                     np.concatenate([np.zeros(3), pt]), weight=np.array([wt]), huber_delta=delta)
+        c = o.update(True)
+        vals.append((c, o.hessian()["bV"].copy(), o.hessian()["bU"].copy()))
+    (c0, bv0, bu0), (c1, bv1, bu1) = vals
+    assert abs(c0 - wt * delta ** 2) < 1e-8 and abs(c1 - wt * delta ** 2) < 1e-8
+    assert relerr(bv1, bv0) < 1e-8 and relerr(bu1, bu0) < 1e-8
+
+
+def test_schur_assemble_is_the_steps_reduction(cfg1):
+    """or_schur_assemble and or_schur_step share one reduction: the same S
+    and bS bit for bit."""
+    o = OracleBA.from_generated(cfg1, huber_delta=2.0)
+    o.update(True)
+    a = o.schur_assemble(0.01, 3.0)
+    s = o.schur_step(0.01, 3.0, want_s=True)
+    assert a["ok"] and s["ok"]
+    assert np.array_equal(a["S"], s["S"]) and np.array_equal(a["bS"], s["bS"])
