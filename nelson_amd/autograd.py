@@ -8,6 +8,8 @@ which reuses the Schur factorisation of the Gauss-Newton Hessian at the
 solution.  torch only carries tensors here; all the work runs in
 libnelson_amd.so.
 """
+import warnings
+
 import numpy as np
 import torch
 
@@ -25,27 +27,43 @@ def translation_loss_grad(t, g_t):
 
 
 class BASolve(torch.autograd.Function):
-    """translations = BASolve.apply(obs, weight, problem, cams0, pts0, settings, hessian_steps)"""
+    """translations = BASolve.apply(obs, weight, problem, cams0, pts0, settings, hessian_steps)
+
+    The adjoint assumes a converged solve (gradient b(x*) = 0): a solver
+    failure raises, and a solve that stops on an iteration limit warns.  The
+    solution and the observations of this forward are kept and put back into
+    the problem before its backward, so another forward on the same problem in
+    between (e.g. a finite-difference check) does not change the gradients."""
 
     @staticmethod
     def forward(ctx, obs, weight, problem, cams0, pts0, settings=None, hessian_steps=2):
-        problem.set_observations(obs.detach().cpu().double().numpy(), weight.detach().cpu().double().numpy())
+        o = obs.detach().cpu().double().numpy().copy()
+        w = weight.detach().cpu().double().numpy().copy()
+        problem.set_observations(o, w)
         problem.set_state(cams0, pts0)
         lm = LevenbergMarquardt(settings or LevenbergMarquardtSettings())
         reason = lm.solve(problem)
         if reason == "SolverFailure":
             raise RuntimeError("BASolve: the LM solve failed")
+        if reason in ("MaxIterationReached", "MaxSubIterationReached"):
+            warnings.warn(f"BASolve: LM stopped with {reason}; the adjoint assumes a zero gradient at the "
+                          "solution", RuntimeWarning)
         t = problem.camera_translations()
         ctx.problem = problem
         ctx.hessian_steps = hessian_steps
         ctx.t = t
+        ctx.solution = problem.get_state_rt()
+        ctx.obs, ctx.weight = o, w
         ctx.dev, ctx.dtype = obs.device, obs.dtype
         return torch.as_tensor(t, dtype=obs.dtype, device=obs.device)
 
     @staticmethod
     def backward(ctx, grad_t):
+        p = ctx.problem
+        p.set_observations(ctx.obs, ctx.weight)
+        p.set_state_rt(*ctx.solution)
         g = translation_loss_grad(ctx.t, grad_t.detach().cpu().double().numpy())
-        dobs, dw = ctx.problem.adjoint(g_cams=g, hessian_steps=ctx.hessian_steps)
+        dobs, dw = p.adjoint(g_cams=g, hessian_steps=ctx.hessian_steps)
         go = torch.as_tensor(dobs, dtype=ctx.dtype, device=ctx.dev)
         gw = torch.as_tensor(dw, dtype=ctx.dtype, device=ctx.dev)
         return go, gw, None, None, None, None, None

@@ -38,15 +38,31 @@ constexpr int kTrsvMaxWG = 256;
 // many workgroups (four per CU) or the parts are single camera blocks
 constexpr int kMinSParts = 1024;
 
+// Device allocations of one owner.  A failed allocation returns nullptr and
+// sets `failed`, so a sequence of allocations is checked once at its end.
+// fail_after >= 0 makes the allocation with that index fail (the
+// NSN_TEST_FAIL_ALLOC environment variable of nsn_problem_create), so that
+// tests can reach every failure path.
 struct DevBuf {
   std::vector<void*> ptrs;
-  ~DevBuf() {
+  bool failed = false;
+  long fail_after = -1;
+  ~DevBuf() { reset(); }
+  void reset() {
     for (void* p : ptrs) (void)hipFree(p);
+    ptrs.clear();
+    failed = false;
   }
   template <typename T>
   T* alloc(size_t n) {
     void* p = nullptr;
-    if (hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) return nullptr;
+    if (fail_after == 0 || hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) {
+      (void)hipGetLastError();  // an out-of-memory error is not left for a later check to find
+      if (fail_after >= 0) fail_after = -1;
+      failed = true;
+      return nullptr;
+    }
+    if (fail_after > 0) fail_after--;
     ptrs.push_back(p);
     return (T*)p;
   }
@@ -421,10 +437,9 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
   h->ctl = m.alloc<DevCtl>(1);
   h->log_cap = 4096;
   h->log = m.alloc<DevLog>(h->log_cap);
-  if (!h->log || !h->ctl || !h->tiles || !h->inv || !h->presub || !h->W || !h->Y || !h->sb_pairs ||
-      !h->zU)
-    return fail(NSN_ENOMEM, "device allocation failed (%.1f MB of tiles)",
-                (double)tv.ntiles * kTB * kTB * es / 1e6);
+  if (m.failed)
+    return fail(NSN_ENOMEM, "device allocation failed (%.1f MB of tiles, %.1f MB of W blocks)",
+                (double)tv.ntiles * kTB * kTB * es / 1e6, 18.0 * nW * es / 1e6);
   int rc;
   if ((rc = upload(d_cam_var, h->cam_var)) || (rc = upload(d_pt_var, h->pt_var)) ||
       (rc = upload(d_vcu, h->var_cam_user)) || (rc = upload(d_vpu, h->var_pt_user)) ||
@@ -449,7 +464,7 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
   v.intr = nullptr;
   if (d->intrinsics) {
     double* d_intr = a_dbl(3 * (size_t)C);
-    if (!d_intr) return fail(NSN_ENOMEM, "device allocation failed (intrinsics)");
+    if (m.failed) return fail(NSN_ENOMEM, "device allocation failed (intrinsics)");
     if ((rc = upload(d_intr, std::vector<double>(d->intrinsics, d->intrinsics + 3 * (size_t)C)))) return rc;
     v.intr = d_intr;
   }
@@ -503,6 +518,7 @@ extern "C" int nsn_problem_create(const nsn_problem_desc* d, nsn_problem** out) 
   h->C = d->n_cameras; h->P = d->n_points; h->E = d->n_edges; h->prec = d->precision;
   h->ir_steps = d->precision == NSN_MIXED ? 2 : 0;
   h->f = d->focal; h->cx = d->cx; h->cy = d->cy; h->huber = d->huber_delta;
+  if (const char* fa = std::getenv("NSN_TEST_FAIL_ALLOC")) h->mem.fail_after = std::strtol(fa, nullptr, 10);
   HIP_TRY(hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking));
   for (auto& e : h->ev) HIP_TRY(hipEventCreate(&e));
   HIP_TRY(hipHostMalloc((void**)&h->hctl, sizeof(DevCtl), hipHostMallocDefault));
@@ -832,7 +848,7 @@ extern "C" int nsn_set_priors(nsn_problem* h, int n_cam, const int* cam_ids, con
     int* di = m->alloc<int>(h->C);
     double* dp = m->alloc<double>(6 * (size_t)n_cam);
     double* dw = m->alloc<double>(6 * (size_t)n_cam);
-    if (!di || !dp || !dw) return fail(NSN_ENOMEM, "nsn_set_priors: device allocation failed");
+    if (m->failed) return fail(NSN_ENOMEM, "nsn_set_priors: device allocation failed");
     if ((rc = upload(di, cidx)) || (rc = upload(dp, std::vector<double>(cam_p0, cam_p0 + 6 * (size_t)n_cam))) ||
         (rc = upload(dw, std::vector<double>(cam_w, cam_w + 6 * (size_t)n_cam))))
       return rc;
@@ -842,7 +858,7 @@ extern "C" int nsn_set_priors(nsn_problem* h, int n_cam, const int* cam_ids, con
     int* di = m->alloc<int>(h->P);
     double* dp = m->alloc<double>(3 * (size_t)n_pt);
     double* dw = m->alloc<double>(3 * (size_t)n_pt);
-    if (!di || !dp || !dw) return fail(NSN_ENOMEM, "nsn_set_priors: device allocation failed");
+    if (m->failed) return fail(NSN_ENOMEM, "nsn_set_priors: device allocation failed");
     if ((rc = upload(di, pidx)) || (rc = upload(dp, std::vector<double>(pt_p0, pt_p0 + 3 * (size_t)n_pt))) ||
         (rc = upload(dw, std::vector<double>(pt_w, pt_w + 3 * (size_t)n_pt))))
       return rc;
@@ -956,9 +972,13 @@ int alloc_adjoint(nsn_problem* h) {
   h->dobs = m.alloc<double>(2 * (size_t)h->E);
   h->dw = m.alloc<double>(h->E);
   h->adj_eps = m.alloc<double>(1);
-  for (double** p : us) if (!*p) return fail(NSN_ENOMEM, "adjoint buffers");
-  for (double** p : vs) if (!*p) return fail(NSN_ENOMEM, "adjoint buffers");
-  if (!h->dobs || !h->dw || !h->adj_eps) return fail(NSN_ENOMEM, "adjoint buffers");
+  if (m.failed) {  // nothing half-allocated stays behind for the next call
+    m.reset();
+    for (double** p : us) *p = nullptr;
+    for (double** p : vs) *p = nullptr;
+    h->dobs = h->dw = h->adj_eps = nullptr;
+    return fail(NSN_ENOMEM, "nsn_adjoint: device allocation failed");
+  }
   HIP_TRY(hipMemset(h->zeroU, 0, sizeof(double) * std::max<size_t>(nu, 1)));
   HIP_TRY(hipMemset(h->zeroV, 0, sizeof(double) * std::max<size_t>(nv, 1)));
   return NSN_OK;

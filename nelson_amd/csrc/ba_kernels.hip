@@ -833,7 +833,8 @@ static inline unsigned nblocks(long long n, int b) { return (unsigned)((n + b - 
 template <typename T>
 void launch_lin_points(const BAView& v, bool hess, T* V, T* bV, T* W, const Scratch& s, DevCtl* ctl, int gate,
                        hipStream_t st) {
-  if (v.P == 0) return;
+  // always launched (nsn_problem_create requires n_points >= 1): the camera
+  // priors' cost is added here too, so it is counted with every point fixed
   if (hess)
     hipLaunchKernelGGL((k_lin_points<T, true>), dim3(nblocks(v.P, 256)), dim3(256), 0, st, v, V, bV, W,
                        s.part_chi2, s.maxes, ctl, gate);

@@ -7,7 +7,10 @@
 // comes out of the factorisation for free.  The reference factorises S with
 // Eigen's pivoted LDL^T (SolverCholeskyDense.hpp:15-18); for the damped,
 // positive definite S of an LM step the two give the same solution up to
-// rounding, and both report failure when S is not positive definite.
+// rounding.  A semidefinite S whose zero pivots have zero rows (a variable
+// camera that nothing observes, undamped) is accepted by both with a zero
+// solution entry there (potrf_panel); an indefinite S, which Eigen's LDL^T
+// factorises and this LL^T cannot, is reported as a failure.
 //
 // Scheduling: workgroups take tiles from an atomic counter in column-major
 // lower order (j, then i).  Tile (i, j) needs (i, k), (j, k) for k < j and

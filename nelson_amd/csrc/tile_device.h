@@ -89,7 +89,15 @@ __device__ __forceinline__ double rsqrt_f64(double d) {
 // right-looking Cholesky of columns C0 .. C0+15 of the 64-row FP64 tile A
 // (rows >= C0), wave 0 only.  Rows with global index >= n (the bS row and the
 // identity padding) take pivot 1 and never count as failures; a failed pivot
-// is replaced by 1 and counted in `bad`.  The step is branch-free: every
+// is replaced by 1 and counted in `bad`.  A pivot that is exactly zero with
+// an exactly zero column below it (a row and column of S that no term
+// touches, e.g. a variable camera without observations and without damping)
+// is the semidefinite case that the reference's pivoted LDL^T accepts
+// (SolverCholeskyDense.hpp:15-18) and solves with the pseudo-inverse of D:
+// the pivot is kept as 1 with a stored reciprocal of 0, so the inverse of
+// the factor tile has that row and column zero and the solution entry comes
+// out 0, as in the reference; a zero pivot over a non-zero column is a
+// failure, as there.  The step is branch-free: every
 // lane computes the same pivot, the reciprocal square root of the selected
 // value, and the updates run on every lane (entries above the diagonal in the
 // registers are garbage and never written back).  After scaling, the entry
@@ -106,13 +114,16 @@ __device__ __forceinline__ void potrf_panel(double* A, double* col, int g0, int 
 #pragma unroll
   for (int c = 0; c < kPW; c++) a[c] = A[(C0 + c) * TB + lane];
   double myr = 1.0;
+  int zcol = 0;  // this lane's entry below a zero pivot is non-zero
 #pragma unroll
   for (int k = 0; k < kPW; k++) {
     const double d = readlane_f64(a[k], C0 + k);
     const bool pad = g0 + C0 + k >= n;
     const bool good = d > 0.0 && d < __builtin_inf();
-    bad |= (!pad && !good) ? 1 : 0;
-    const double r = (pad || !good) ? 1.0 : rsqrt_f64(d);
+    const bool zero = !pad && d == 0.0;
+    bad |= (!pad && !good && !zero) ? 1 : 0;
+    zcol |= (zero && lane > C0 + k && a[k] != 0.0) ? 1 : 0;
+    const double r = zero ? 0.0 : ((pad || !good) ? 1.0 : rsqrt_f64(d));
     const double piv = (pad || !good) ? 1.0 : d * r;
     a[k] = (lane == C0 + k) ? piv : a[k] * r;
     myr = (lane == C0 + k) ? r : myr;
@@ -134,6 +145,7 @@ __device__ __forceinline__ void potrf_panel(double* A, double* col, int g0, int 
   for (int c = 0; c < kPW; c++)
     if (lane >= C0 + c) A[(C0 + c) * TB + lane] = a[c];
   if (lane >= C0 && lane < C0 + kPW) A[rdiag_slot<TB>(lane)] = myr;
+  if (__ballot(zcol) != 0ull) bad = 1;
 }
 
 // D_(rb, cb) -= A_(rb, p) A_(cb, p)^T for one 16x16 block of the FP64 tile

@@ -206,3 +206,82 @@ def test_termination_reasons(cfg1):
     ro = o2.lm_solve(maxNumIt=50, maxNumSubIt=1, tau=1e-12)
     _compare_logs(lm2.stats(), ro["log"], "subit1")
     assert r == ro["reason"]
+
+
+def _with_unobserved_cameras(n_cameras=15, seed=21, at=(7, 15)):
+    """A problem with variable cameras that no edge observes, inserted at the
+    user indices `at` (one in the middle of S, one at its end; two 64-row
+    tiles): their rows and columns of S are exactly zero without damping."""
+    g = generate_ba(small_cfg(n_cameras=n_cameras, n_points=200, k_min=3, k_max=5, seed=seed))
+    cam_idx = g["cam_idx"].copy()
+    c6 = [g["cams0"].reshape(-1, 6)[c] for c in range(n_cameras)]
+    cg = [g["cams_gt"].reshape(-1, 6)[c] for c in range(n_cameras)]
+    for a in sorted(at):
+        cam_idx[cam_idx >= a] += 1
+        extra = np.array([0.01 * a, -0.02, 0.03, 0.1, -0.2, 0.05 * a])
+        c6.insert(a, extra)
+        cg.insert(a, extra)
+    out = dict(g, n_cameras=n_cameras + len(at), cam_idx=cam_idx.astype(np.int32),
+               cams0=np.concatenate(c6), cams_gt=np.concatenate(cg))
+    return out
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_semidefinite_schur_step_unobserved_camera(precision):
+    """Undamped, S is positive semidefinite with zero rows for the unobserved
+    cameras.  The reference's pivoted LDL^T accepts it and returns a zero
+    increment for those cameras (SolverCholeskyDense.hpp:15-18, D's
+    pseudo-inverse); the GPU factorisation must do the same."""
+    g = _with_unobserved_cameras()
+    p, o = pair(g, precision=precision)
+    p.update(True)
+    o.update(True)
+    s = SolverCholeskySchur()
+    s.init(p)
+    ok = s.computeIncrement(0.0, 0.0)
+    ro = o.schur_step(0.0, 0.0)
+    assert ro["ok"] and ok
+    dxu, dxv = s.incrementVector()
+    for c in (7, 15):  # user index -> variable index (camera 0 is fixed)
+        assert np.all(dxu[6 * (c - 1):6 * c] == 0.0) and np.all(ro["xU"][6 * (c - 1):6 * c] == 0.0)
+    tol = TOL_STEP if precision == "fp64" else 1e-8
+    assert relerr(dxu, ro["xU"]) <= tol
+    assert relerr(dxv, ro["xV"]) <= tol
+
+
+def test_semidefinite_gauss_newton_unobserved_camera():
+    """The reference's default (undamped) Gauss-Newton on a problem with
+    unobserved variable cameras runs to convergence instead of stopping with
+    SolverFailure (GaussNewton.hpp:67-78): same log and reason as the oracle,
+    the unobserved cameras unchanged."""
+    g = _with_unobserved_cameras()
+    p, o = pair(g)
+    gn = GaussNewton(GaussNewtonSettings(maxNumIt=8))
+    reason = gn.solve(p)
+    ro = o.gn_solve(maxNumIt=8)
+    assert reason == ro["reason"] and reason != "SolverFailure"
+    _compare_logs(gn.stats(), ro["log"], "gn-semidefinite")
+    cams, _ = p.get_state()
+    for c in (7, 15):
+        assert np.array_equal(cams[6 * c:6 * c + 6], g["cams0"][6 * c:6 * c + 6])
+
+
+def test_indefinite_s_reports_failure():
+    """An indefinite S (a camera block of U replaced by minus itself through
+    the host-imported Hessian) has negative pivots.  The GPU's LL^T reports
+    NOT_PD and zeroes the increment (SolverCholeskySchur.hpp:151-154).  The
+    reference's pivoted LDL^T factorises indefinite matrices and would return
+    an increment: this is the one documented divergence, and it cannot occur
+    for the J^T J + damping systems of BA."""
+    g = generate_ba(small_cfg(n_cameras=6, n_points=60, k_min=3, k_max=4, seed=22))
+    p, o = pair(g)
+    p.update(True)
+    H = p.export_hessian()
+    U = H["U"].copy()
+    U[2] = -U[2]
+    p.import_hessian(U, H["V"], H["W"], H["bU"], H["bV"], H["chi2"])
+    s = SolverCholeskySchur()
+    s.init(p)
+    assert not s.computeIncrement(0.0, 0.0)
+    dxu, dxv = s.incrementVector()
+    assert np.all(dxu == 0.0) and np.all(dxv == 0.0)
