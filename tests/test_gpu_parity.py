@@ -263,7 +263,8 @@ def test_semidefinite_gauss_newton_unobserved_camera():
     _compare_logs(gn.stats(), ro["log"], "gn-semidefinite")
     cams, _ = p.get_state()
     for c in (7, 15):
-        assert np.array_equal(cams[6 * c:6 * c + 6], g["cams0"][6 * c:6 * c + 6])
+        # unchanged up to the rotation's angle-axis -> matrix -> angle-axis round trip
+        assert relerr(cams[6 * c:6 * c + 6], g["cams0"][6 * c:6 * c + 6]) < 1e-14
 
 
 def test_indefinite_s_reports_failure():
