@@ -83,6 +83,13 @@ double or_hessian_max_abs_b(const or_hessian* h) {
 /* ------------------------------------------------------------------------ */
 #define AC(i, j) a[(size_t)(j) * (size_t)n + (size_t)(i)]
 
+#ifdef _OPENMP
+#include <omp.h>
+static int host_threads(void) { return omp_get_max_threads(); }
+#else
+static int host_threads(void) { return 1; }
+#endif
+
 int or_ldlt_factor(double* a, int n, int* tp) {
   if (n <= 0) return 1;
   if (n == 1) { tp[0] = 0; return 1; }
@@ -111,11 +118,32 @@ int or_ldlt_factor(double* a, int n, int* tp) {
       for (int j = 0; j < k; j++) dot += AC(k, j) * temp[j];
       AC(k, k) -= dot;
       if (rs > 0) {
-        for (int j = 0; j < k; j++) {
-          const double tj = temp[j];
-          const double* colj = &AC(0, j);
-          double* colk = &AC(0, k);
-          for (int i = k + 1; i < n; i++) colk[i] -= colj[i] * tj;
+        /* colk[i] -= sum_j colj[i] temp[j], each entry over j in ascending
+         * order (Eigen's A21 -= A20 * temp).  Rows are split in blocks over
+         * the host threads (OpenMP): every entry sees the same operations in
+         * the same order whatever the thread count, so the factor is the
+         * same bit for bit as the single-threaded one. */
+        double* colk = &AC(0, k);
+        const int nt = host_threads();
+        if (nt == 1 || rs < 256) {
+          for (int j = 0; j < k; j++) {
+            const double tj = temp[j];
+            const double* colj = &AC(0, j);
+            for (int i = k + 1; i < n; i++) colk[i] -= colj[i] * tj;
+          }
+        } else {
+          /* 64-row blocks: a block of colk stays in L1 over the j loop */
+          const int bs = 64;
+          const int nblk = (rs + bs - 1) / bs;
+#pragma omp parallel for schedule(static) num_threads(nt)
+          for (int ib = 0; ib < nblk; ib++) {
+            const int i0 = k + 1 + bs * ib, i1 = i0 + bs < n ? i0 + bs : n;
+            for (int j = 0; j < k; j++) {
+              const double tj = temp[j];
+              const double* colj = &AC(0, j);
+              for (int i = i0; i < i1; i++) colk[i] -= colj[i] * tj;
+            }
+          }
         }
       }
     }

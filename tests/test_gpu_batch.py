@@ -151,3 +151,47 @@ def test_batch_degenerate_windows(variant):
         c1, r1, _ = _single(g, maxNumIt=5)
         assert abs(chi2[w] - c1) <= 1e-9 * c1, (w, chi2[w], c1)
         assert reasons[w] == r1, (w, reasons[w], r1)
+
+
+def test_config4_all_512_windows():
+    """The local-BA batch at its full size: all 512 windows of window_config
+    (cameras U{5..40}, points U{200..3000}, 3-6 observations per point, 5 %
+    outliers at 20 px, Huber 2 px) in one launch per solve.
+      * FP64 batch, the workload's 10 LM iterations: every window's final cost
+        equals the single-problem FP64 GPU path's (1e-9) with the same
+        termination;
+      * run to convergence: the mixed-precision batch reaches every window's
+        FP64 final cost to 1e-6;
+      * the oracle's LM on 8 windows, the largest among them: 1e-9."""
+    wins = [generate_ba(window_config(w)) for w in range(512)]
+    sizes = [(g["n_cameras"], g["n_points"], g["n_edges"]) for g in wins]
+    print(f"\n512 windows: {sum(s[2] for s in sizes)} observations, largest {max(sizes, key=lambda s: s[2])}",
+          flush=True)
+    s10 = LevenbergMarquardtSettings(maxNumIt=10)
+    b64 = BABatch(wins, precision="fp64")
+    c10, ms10 = b64.lm_solve(s10)
+    reasons, _, _ = b64.results()
+    conv = {"SuccessSmallIncrement", "MaxIterationReached", "MaxSubIterationReached"}
+    worst = 0.0
+    for w, g in enumerate(wins):
+        c1, r1, _ = _single(g, maxNumIt=10)
+        worst = max(worst, abs(c10[w] - c1) / c1)
+        assert abs(c10[w] - c1) <= 1e-9 * c1, (w, c10[w], c1)
+        assert reasons[w] == r1 or {reasons[w], r1} <= conv, (w, reasons[w], r1)
+    print(f"FP64 batch (10 iterations, {ms10:.1f} ms) vs single-problem path: max rel err {worst:.2e}", flush=True)
+    # to convergence: mixed vs FP64
+    sconv = LevenbergMarquardtSettings(maxNumIt=60)
+    b64.set_state(*BABatch(wins, precision="fp64").get_state())
+    cf64, _ = b64.lm_solve(sconv)
+    bm = BABatch(wins, precision="mixed")
+    cm, msm = bm.lm_solve(sconv)
+    err = np.abs(cm - cf64) / cf64
+    print(f"mixed vs FP64 batch to convergence: max final-cost rel err {err.max():.2e} "
+          f"(median {np.median(err):.2e}), mixed config {bm.config()}", flush=True)
+    assert err.max() <= 1e-6
+    # the oracle on 8 windows including the largest
+    big = int(np.argmax([s[2] for s in sizes]))
+    pick = [big] + [int(w) for w in np.random.default_rng(4).choice(512, 7, replace=False) if w != big][:7]
+    for w in pick:
+        ro = OracleBA.from_generated(wins[w]).lm_solve(maxNumIt=60)
+        assert abs(cf64[w] - ro["final_chi2"]) <= 1e-9 * ro["final_chi2"], (w, cf64[w], ro["final_chi2"])
