@@ -269,6 +269,18 @@ int nsn_set_observations(nsn_problem* h, const double* obs, const double* weight
 int nsn_adjoint(nsn_problem* h, const double* g_cams, const double* g_pts, int hessian_steps,
                 double* dL_dobs, double* dL_dw);
 
+/* The same adjoint without allocating device memory.  No counterpart in the
+ * reference; for callers that cannot allocate at the point of the call
+ * (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc after set-up).  The work vectors live in a static device array
+ * of 32 MiB (4 Mi doubles) that each device holds for the life of the
+ * process; the problem must fit: 7 x 6 NU + 7 x 3 NV + 3 E doubles plus
+ * rounding of each vector to 256 bytes (NSN_EINVAL otherwise, before any
+ * work).  Calls on one device share the array: calls that overlap on one
+ * device race.  The results equal nsn_adjoint's. */
+int nsn_adjoint_noalloc(nsn_problem* h, const double* g_cams, const double* g_pts, int hessian_steps,
+                        double* dL_dobs, double* dL_dw);
+
 /* ---- batched independent windows (no reference counterpart) -------------
  * Many independent BA problems ("windows", uneven sizes, each with at most 42
  * variable cameras) solved by LevenbergMarquardt<SolverCholeskySchur>

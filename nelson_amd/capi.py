@@ -120,6 +120,7 @@ SIGNATURES = {
     "nsn_phase_times": (C.c_int, [_P, _D, C.c_int]),
     "nsn_set_refinement": (C.c_int, [_P, C.c_int]),
     "nsn_adjoint": (C.c_int, [_P, _D, _D, C.c_int, _D, _D]),
+    "nsn_adjoint_noalloc": (C.c_int, [_P, _D, _D, C.c_int, _D, _D]),
     "nsn_set_observations": (C.c_int, [_P, _D, _D]),
     "nsn_batch_create": (C.c_int, [C.POINTER(ProblemDesc), C.c_int, C.POINTER(_P)]),
     "nsn_batch_destroy": (C.c_int, [_P]),

@@ -8,6 +8,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <vector>
 
@@ -108,6 +109,7 @@ struct nsn_problem {
   double *d_pe_obs = nullptr, *d_pe_w = nullptr, *d_ce_obs = nullptr, *d_ce_w = nullptr;
   // adjoint work buffers (allocated on the first nsn_adjoint)
   DevBuf adj_mem;
+  bool zero_fresh = true;  // zeroU / zeroV still to be cleared after their allocation
   double *gU = nullptr, *gV = nullptr, *lamU = nullptr, *lamV = nullptr, *tU = nullptr, *tV = nullptr,
          *bpU = nullptr, *bpV = nullptr, *bmU = nullptr, *bmV = nullptr, *dU = nullptr, *dV = nullptr,
          *zeroU = nullptr, *zeroV = nullptr, *dobs = nullptr, *dw = nullptr, *adj_eps = nullptr;
@@ -961,47 +963,56 @@ extern "C" int nsn_set_observations(nsn_problem* h, const double* obs, const dou
 
 namespace {
 
-int alloc_adjoint(nsn_problem* h) {
-  if (h->gU) return NSN_OK;
-  DevBuf& m = h->adj_mem;
+// work vectors of the adjoint: seven of the camera order (6 NU), seven of the
+// point order (3 NV), the per-edge outputs and eps
+struct AdjBufs {
+  double *gU, *lamU, *tU, *bpU, *bmU, *dU, *zeroU;
+  double *gV, *lamV, *tV, *bpV, *bmV, *dV, *zeroV;
+  double *dobs, *dw, *eps;
+};
+
+// the adjoint's work vectors of a handle, allocated on its first nsn_adjoint
+int alloc_adjoint(nsn_problem* h, AdjBufs& b) {
   const size_t nu = 6 * (size_t)h->NU, nv = 3 * (size_t)h->NV;
   double** us[] = {&h->gU, &h->lamU, &h->tU, &h->bpU, &h->bmU, &h->dU, &h->zeroU};
   double** vs[] = {&h->gV, &h->lamV, &h->tV, &h->bpV, &h->bmV, &h->dV, &h->zeroV};
-  for (double** p : us) *p = m.alloc<double>(nu);
-  for (double** p : vs) *p = m.alloc<double>(nv);
-  h->dobs = m.alloc<double>(2 * (size_t)h->E);
-  h->dw = m.alloc<double>(h->E);
-  h->adj_eps = m.alloc<double>(1);
-  if (m.failed) {  // nothing half-allocated stays behind for the next call
-    m.reset();
-    for (double** p : us) *p = nullptr;
-    for (double** p : vs) *p = nullptr;
-    h->dobs = h->dw = h->adj_eps = nullptr;
-    return fail(NSN_ENOMEM, "nsn_adjoint: device allocation failed");
+  if (!h->gU) {
+    DevBuf& m = h->adj_mem;
+    for (double** p : us) *p = m.alloc<double>(nu);
+    for (double** p : vs) *p = m.alloc<double>(nv);
+    h->dobs = m.alloc<double>(2 * (size_t)h->E);
+    h->dw = m.alloc<double>(h->E);
+    h->adj_eps = m.alloc<double>(1);
+    if (m.failed) {  // nothing half-allocated stays behind for the next call
+      m.reset();
+      for (double** p : us) *p = nullptr;
+      for (double** p : vs) *p = nullptr;
+      h->dobs = h->dw = h->adj_eps = nullptr;
+      return fail(NSN_ENOMEM, "nsn_adjoint: device allocation failed");
+    }
   }
-  HIP_TRY(hipMemset(h->zeroU, 0, sizeof(double) * std::max<size_t>(nu, 1)));
-  HIP_TRY(hipMemset(h->zeroV, 0, sizeof(double) * std::max<size_t>(nv, 1)));
+  b = AdjBufs{h->gU, h->lamU, h->tU, h->bpU, h->bmU, h->dU, h->zeroU, h->gV, h->lamV, h->tV, h->bpV,
+              h->bmV, h->dV, h->zeroV, h->dobs, h->dw, h->adj_eps};
   return NSN_OK;
 }
 
 // gradient b of the cost at the current state perturbed by oplus(t),
 // FP64, into (bu, bv); the state is restored afterwards
-void grad_at_offset(nsn_problem* h, const double* tu, const double* tv, double* bu, double* bv) {
+void grad_at_offset(nsn_problem* h, const AdjBufs& b, const double* tu, const double* tv, double* bu, double* bv) {
   hipStream_t st = h->st;
   launch_copy_state(h->view, h->cams_bak, h->pts_bak, h->cams, h->pts, h->ctl, GATE_ALWAYS, st);
   launch_oplus(h->view, tu, tv, h->ctl, GATE_ALWAYS, st);
-  launch_ir_residual(h->view, h->zeroU, h->zeroV, h->rU, h->rV, bu, bv, h->ctl, GATE_ALWAYS, st);
+  launch_ir_residual(h->view, b.zeroU, b.zeroV, h->rU, h->rV, bu, bv, h->ctl, GATE_ALWAYS, st);
   launch_copy_state(h->view, h->cams, h->pts, h->cams_bak, h->pts_bak, h->ctl, GATE_ALWAYS, st);
 }
 
 template <typename T>
-int adjoint_t(nsn_problem* h, const double* g_cams, const double* g_pts, int steps, double* dL_dobs,
-              double* dL_dw) {
+int adjoint_t(nsn_problem* h, const AdjBufs& b, const double* g_cams, const double* g_pts, int steps,
+              double* dL_dobs, double* dL_dw) {
   hipStream_t st = h->st;
   const int NU = h->NU, NV = h->NV;
   const long long nu = 6LL * NU, nv = 3LL * NV;
   int rc;
-  if ((rc = alloc_adjoint(h))) return rc;
   if ((rc = reset_ctl_primitive(h))) return rc;
   // the Gauss-Newton Hessian at x* and its Schur factorisation (undamped)
   seq_lin_full<T>(h, GATE_ALWAYS);
@@ -1017,30 +1028,30 @@ int adjoint_t(nsn_problem* h, const double* g_cams, const double* g_pts, int ste
     for (int i = 0; i < 6; i++) gu[6 * (size_t)u + i] = g_cams ? g_cams[6 * (size_t)h->var_cam_user[u] + i] : 0.0;
   for (int q = 0; q < NV; q++)
     for (int i = 0; i < 3; i++) gv[3 * (size_t)q + i] = g_pts ? g_pts[3 * (size_t)h->var_pt_user[q] + i] : 0.0;
-  if ((rc = upload(h->gU, gu)) || (rc = upload(h->gV, gv))) return rc;
+  if ((rc = upload(b.gU, gu)) || (rc = upload(b.gV, gv))) return rc;
   // lambda = H^-1 g with the Schur factorisation
-  seq_schur_solve<T>(h, GATE_ALWAYS, h->gU, h->gV, h->lamU, h->lamV, 0, nullptr);
+  seq_schur_solve<T>(h, GATE_ALWAYS, b.gU, b.gV, b.lamU, b.lamV, 0, nullptr);
   // refinement against the exact Hessian: lambda += H^-1 (g - B lambda), with
   // B lambda = (b(x + eps lambda) - b(x - eps lambda)) / (2 eps), the central
   // difference of the FP64 gradient along lambda
   // (all on the device: eps = 1e-6 / max |lambda| lives in adj_eps)
   for (int k = 0; k < steps; k++) {
-    launch_adj_eps(h->lamU, nu, h->lamV, nv, h->adj_eps, st);
-    launch_adj_offset(h->lamU, nu, h->lamV, nv, h->adj_eps, 1.0, h->tU, h->tV, st);
-    grad_at_offset(h, h->tU, h->tV, h->bpU, h->bpV);
-    launch_adj_offset(h->lamU, nu, h->lamV, nv, h->adj_eps, -1.0, h->tU, h->tV, st);
-    grad_at_offset(h, h->tU, h->tV, h->bmU, h->bmV);
-    launch_adj_resid(h->gU, h->bmU, h->bpU, nu, h->gV, h->bmV, h->bpV, nv, h->adj_eps, st);
-    seq_schur_solve<T>(h, GATE_ALWAYS, h->bpU, h->bpV, h->dU, h->dV, 0, nullptr);
-    launch_axpby(nu, 1.0, h->lamU, 1.0, h->dU, h->lamU, st);
-    launch_axpby(nv, 1.0, h->lamV, 1.0, h->dV, h->lamV, st);
+    launch_adj_eps(b.lamU, nu, b.lamV, nv, b.eps, st);
+    launch_adj_offset(b.lamU, nu, b.lamV, nv, b.eps, 1.0, b.tU, b.tV, st);
+    grad_at_offset(h, b, b.tU, b.tV, b.bpU, b.bpV);
+    launch_adj_offset(b.lamU, nu, b.lamV, nv, b.eps, -1.0, b.tU, b.tV, st);
+    grad_at_offset(h, b, b.tU, b.tV, b.bmU, b.bmV);
+    launch_adj_resid(b.gU, b.bmU, b.bpU, nu, b.gV, b.bmV, b.bpV, nv, b.eps, st);
+    seq_schur_solve<T>(h, GATE_ALWAYS, b.bpU, b.bpV, b.dU, b.dV, 0, nullptr);
+    launch_axpby(nu, 1.0, b.lamU, 1.0, b.dU, b.lamU, st);
+    launch_axpby(nv, 1.0, b.lamV, 1.0, b.dV, b.lamV, st);
   }
-  launch_adjoint_edges(h->view, h->lamU, h->lamV, h->dobs, h->dw, st);
+  launch_adjoint_edges(h->view, b.lamU, b.lamV, b.dobs, b.dw, st);
   HIP_TRY(hipGetLastError());
   if ((rc = sync_ctl(h))) return rc;
   if ((rc = check_device_errors(h))) return rc;
-  if (dL_dobs) HIP_TRY(hipMemcpy(dL_dobs, h->dobs, sizeof(double) * 2 * (size_t)h->E, hipMemcpyDeviceToHost));
-  if (dL_dw) HIP_TRY(hipMemcpy(dL_dw, h->dw, sizeof(double) * (size_t)h->E, hipMemcpyDeviceToHost));
+  if (dL_dobs) HIP_TRY(hipMemcpy(dL_dobs, b.dobs, sizeof(double) * 2 * (size_t)h->E, hipMemcpyDeviceToHost));
+  if (dL_dw) HIP_TRY(hipMemcpy(dL_dw, b.dw, sizeof(double) * (size_t)h->E, hipMemcpyDeviceToHost));
   // the factorisation and increment buffers now hold the adjoint's values
   h->stepped = false;
   return NSN_OK;
@@ -1054,8 +1065,89 @@ extern "C" int nsn_adjoint(nsn_problem* h, const double* g_cams, const double* g
   if (hessian_steps < 0 || hessian_steps > 16)
     return fail(NSN_EINVAL, "nsn_adjoint: hessian_steps must be in 0..16, got %d", hessian_steps);
   HIP_TRY(hipSetDevice(h->device));
-  return h->prec == NSN_MIXED ? adjoint_t<float>(h, g_cams, g_pts, hessian_steps, dL_dobs, dL_dw)
-                              : adjoint_t<double>(h, g_cams, g_pts, hessian_steps, dL_dobs, dL_dw);
+  AdjBufs b;
+  int rc = alloc_adjoint(h, b);
+  if (rc) return rc;
+  if (h->zero_fresh) {
+    HIP_TRY(hipMemset(h->zeroU, 0, sizeof(double) * std::max<size_t>(6 * (size_t)h->NU, 1)));
+    HIP_TRY(hipMemset(h->zeroV, 0, sizeof(double) * std::max<size_t>(3 * (size_t)h->NV, 1)));
+    h->zero_fresh = false;
+  }
+  return h->prec == NSN_MIXED ? adjoint_t<float>(h, b, g_cams, g_pts, hessian_steps, dL_dobs, dL_dw)
+                              : adjoint_t<double>(h, b, g_cams, g_pts, hessian_steps, dL_dobs, dL_dw);
+}
+
+// ---- the adjoint without allocation ---------------------------------------
+// For callers that cannot allocate device memory at the point of the call
+// (graph capture, serving loops): the adjoint's work vectors live in this
+// file-scope device array instead of the handle's lazily allocated buffers.
+// Each device holds its own copy for the life of the process; the host reads
+// its address once per device with hipGetSymbolAddress and passes that
+// address, never the array's name, to the kernels.  Calls on one device share
+// the array, so calls that overlap on one device race.
+constexpr size_t kAdjScratchDoubles = size_t(4) << 20;  // 32 MiB
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double scratch[kAdjScratchDoubles];
+
+namespace {
+
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// the device address of `scratch` on the current device `dev`
+int scratch_address(int dev, double** out) {
+  if (dev < 0 || dev >= kMaxDevices) return fail(NSN_EINVAL, "device %d beyond the scratch table", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(NSN_EHIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+    g_scratch_ptr[dev] = (double*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return NSN_OK;
+}
+
+// doubles of the adjoint's work vectors, each rounded up to 256 bytes
+size_t adj_scratch_need(const nsn_problem* h) {
+  auto r = [](size_t n) { return (std::max<size_t>(n, 1) + 31) / 32 * 32; };
+  return 7 * r(6 * (size_t)h->NU) + 7 * r(3 * (size_t)h->NV) + r(2 * (size_t)h->E) + r(h->E) + r(1);
+}
+
+}  // namespace
+
+extern "C" int nsn_adjoint_noalloc(nsn_problem* h, const double* g_cams, const double* g_pts, int hessian_steps,
+                                   double* dL_dobs, double* dL_dw) {
+  if (!h) return fail(NSN_EINVAL, "null argument");
+  if (hessian_steps < 0 || hessian_steps > 16)
+    return fail(NSN_EINVAL, "nsn_adjoint_noalloc: hessian_steps must be in 0..16, got %d", hessian_steps);
+  const size_t need = adj_scratch_need(h);
+  if (need > kAdjScratchDoubles)
+    return fail(NSN_EINVAL,
+                "nsn_adjoint_noalloc: the problem needs %zu doubles of work space (7 x 6 NU + 7 x 3 NV + 3 E), "
+                "more than the %zu of the static scratch array; use nsn_adjoint",
+                need, kAdjScratchDoubles);
+  HIP_TRY(hipSetDevice(h->device));
+  double* base = nullptr;
+  int rc = scratch_address(h->device, &base);
+  if (rc) return rc;
+  auto r = [](size_t n) { return (std::max<size_t>(n, 1) + 31) / 32 * 32; };
+  const size_t nu = 6 * (size_t)h->NU, nv = 3 * (size_t)h->NV;
+  double* q = base;
+  auto take = [&](size_t n) { double* p = q; q += r(n); return p; };
+  AdjBufs b;
+  b.gU = take(nu); b.lamU = take(nu); b.tU = take(nu); b.bpU = take(nu); b.bmU = take(nu); b.dU = take(nu);
+  b.zeroU = take(nu);
+  b.gV = take(nv); b.lamV = take(nv); b.tV = take(nv); b.bpV = take(nv); b.bmV = take(nv); b.dV = take(nv);
+  b.zeroV = take(nv);
+  b.dobs = take(2 * (size_t)h->E); b.dw = take(h->E); b.eps = take(1);
+  // the array keeps the previous call's contents: the zero vectors are
+  // cleared by a fill kernel on every call
+  launch_zero<double>(b.zeroU, (long long)nu, h->st);
+  launch_zero<double>(b.zeroV, (long long)nv, h->st);
+  return h->prec == NSN_MIXED ? adjoint_t<float>(h, b, g_cams, g_pts, hessian_steps, dL_dobs, dL_dw)
+                              : adjoint_t<double>(h, b, g_cams, g_pts, hessian_steps, dL_dobs, dL_dw);
 }
 
 template <typename T>
