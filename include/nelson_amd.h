@@ -208,6 +208,38 @@ int nsn_export_hessian(nsn_problem* h, double* U, double* V, double* W, int* Wi,
 int nsn_import_hessian(nsn_problem* h, const double* U, const double* V, const double* W,
                        const double* bU, const double* bV, double chi2);
 
+/* A problem given by its block structure instead of BA edges, for sections
+ * whose edges the caller evaluates on the host (the reference's generic
+ * DoubleSection with user edges, nelson/DoubleSection.h:181-545): n_u
+ * parameters of the first section (6x6 diagonal blocks of U), n_v of the
+ * second (3x3 diagonal blocks of V, which must be block diagonal: no V-V
+ * edges), the non-zero 6x3 W blocks (u, v) and the non-zero off-diagonal 6x6
+ * U blocks (i, j), i < j, of U-U edges.  Every parameter is variable (the
+ * caller leaves fixed ones out).  Smaller blocks are embedded by the caller
+ * (include/nelson_amd.hpp pads them with an identity diagonal and zero
+ * gradient, which leaves the solution of the real part unchanged).  The
+ * Hessian is always imported (nsn_import_hessian_blocks); calls that need
+ * edges or a BA state return NSN_ESTATE.  Mixed precision solves the
+ * imported system without iterative refinement. */
+typedef struct {
+  int n_u, n_v;
+  int n_w;
+  const int* w_u;      /* [n_w] */
+  const int* w_v;      /* [n_w] */
+  int n_uoff;
+  const int* uoff_i;   /* [n_uoff] */
+  const int* uoff_j;   /* [n_uoff] */
+  int precision;       /* NSN_FP64 or NSN_MIXED */
+  int device;
+} nsn_block_desc;
+int nsn_problem_create_blocks(const nsn_block_desc* d, nsn_problem** out);
+/* nsn_import_hessian plus the off-diagonal U blocks: Uoff holds 36 values
+ * (6x6 row-major U_ij) per block, in row-major (i, j) order of the blocks
+ * (the UID order of nelson/DoubleSection.hpp:157-191); NULL when the problem
+ * has none.  The W blocks are in row-major (u, v) order as well. */
+int nsn_import_hessian_blocks(nsn_problem* h, const double* U, const double* Uoff, const double* V,
+                              const double* W, const double* bU, const double* bV, double chi2);
+
 /* ---- SolverCholeskySchur (nelson/SolverCholeskySchur.h:53-117) ---------- */
 int nsn_schur_init(nsn_problem* h);                                  /* init (.hpp:44-62) */
 int nsn_max_abs_h_diag(nsn_problem* h, double* out);                 /* maxAbsHDiag (.hpp:75-78) */

@@ -28,6 +28,14 @@ class NelsonError(RuntimeError):
         self.code = code
 
 
+class BlockDesc(C.Structure):
+    _fields_ = [
+        ("n_u", C.c_int), ("n_v", C.c_int), ("n_w", C.c_int), ("w_u", C.POINTER(C.c_int)),
+        ("w_v", C.POINTER(C.c_int)), ("n_uoff", C.c_int), ("uoff_i", C.POINTER(C.c_int)),
+        ("uoff_j", C.POINTER(C.c_int)), ("precision", C.c_int), ("device", C.c_int),
+    ]
+
+
 class ProblemDesc(C.Structure):
     _fields_ = [
         ("n_cameras", C.c_int), ("n_points", C.c_int), ("n_edges", C.c_int),
@@ -119,6 +127,8 @@ SIGNATURES = {
     "nsn_gn_solve": (C.c_int, [_P, C.POINTER(GNSettings), C.POINTER(LMResult), C.POINTER(LogEntry), C.c_int]),
     "nsn_phase_times": (C.c_int, [_P, _D, C.c_int]),
     "nsn_set_refinement": (C.c_int, [_P, C.c_int]),
+    "nsn_problem_create_blocks": (C.c_int, [C.POINTER(BlockDesc), C.POINTER(_P)]),
+    "nsn_import_hessian_blocks": (C.c_int, [_P, _D, _D, _D, _D, _D, _D, C.c_double]),
     "nsn_adjoint": (C.c_int, [_P, _D, _D, C.c_int, _D, _D]),
     "nsn_adjoint_noalloc": (C.c_int, [_P, _D, _D, C.c_int, _D, _D]),
     "nsn_set_observations": (C.c_int, [_P, _D, _D]),

@@ -127,6 +127,13 @@ struct nsn_problem {
   hipGraphExec_t lm_graph = nullptr;
   bool lm_graph_tried = false;
   std::unique_ptr<DevBuf> pri_mem;  // unary priors (nsn_set_priors)
+  // a block problem (nsn_problem_create_blocks) has no edges: its Hessian is
+  // always imported; off-diagonal U blocks (U-U edges) are added to S
+  bool blocks_only = false;
+  bool imported = false;  // the current Hessian came from an import
+  int n_uoff = 0;
+  int *uoff_i = nullptr, *uoff_j = nullptr;
+  void* Uoff = nullptr;
 
   ~nsn_problem() {
     if (lm_graph) (void)hipGraphExecDestroy(lm_graph);
@@ -192,6 +199,7 @@ void seq_schur(nsn_problem* h, int gate, int checks, bool timed) {
   launch_s_tiles<T>(h->tv, h->n_s_parts, h->tile_order, h->part_tile, h->part_rect, h->part_ptr, h->part_blk, h->sb_i,
                     h->sb_j, h->sb_ptr, h->sb_pairs, (const T*)h->Sd, (const T*)h->bS, (const T*)h->Y,
                     (T*)h->tiles, h->ctl, gate, st);
+  launch_add_uoff<T>(h->tv, h->n_uoff, h->uoff_i, h->uoff_j, (const T*)h->Uoff, (T*)h->tiles, h->ctl, gate, st);
   if (timed) (void)hipEventRecord(h->ev[2], st);
   launch_chol_reset<T>(h->flags, h->nflags, h->counters, h->ctl, gate, st);
   {
@@ -206,7 +214,9 @@ void seq_schur(nsn_problem* h, int gate, int checks, bool timed) {
   launch_backsub<T>(h->view, (const T*)h->W, (const T*)h->Vinv, (const T*)h->bV, h->xU, h->xV, h->scr, h->ctl,
                     gate, st);
   if (timed) (void)hipEventRecord(h->ev[5], st);
-  const int ir = (h->prec == NSN_MIXED) ? h->ir_steps : 0;
+  // the FP64 refinement residual is evaluated from the edges, which an
+  // imported Hessian does not come from: imported systems are not refined
+  const int ir = (h->prec == NSN_MIXED && !h->imported) ? h->ir_steps : 0;
   for (int it = 0; it < ir; it++) {
     // x += M^-1 (-b - (H + D) x), M the FP32 Schur solve
     launch_ir_residual(h->view, h->xU, h->xV, h->rU, h->rV, h->bU64, h->bV64, h->ctl, gate, st);
@@ -497,8 +507,16 @@ extern "C" int nsn_device_count(int* n) {
   return NSN_OK;
 }
 
+static int problem_create_impl(const nsn_problem_desc* d, nsn_problem** out, long fail_after);
+
 extern "C" int nsn_problem_create(const nsn_problem_desc* d, nsn_problem** out) {
   clear_error();
+  long fa = -1;
+  if (const char* e = std::getenv("NSN_TEST_FAIL_ALLOC")) fa = std::strtol(e, nullptr, 10);
+  return problem_create_impl(d, out, fa);
+}
+
+static int problem_create_impl(const nsn_problem_desc* d, nsn_problem** out, long fail_after) {
   if (!d || !out) return fail(NSN_EINVAL, "nsn_problem_create: null argument");
   *out = nullptr;
   if (d->n_cameras < 1 || d->n_points < 1 || d->n_edges < 0 || (d->n_edges > 0 && (!d->cam_idx || !d->pt_idx || !d->obs)))
@@ -520,7 +538,7 @@ extern "C" int nsn_problem_create(const nsn_problem_desc* d, nsn_problem** out) 
   h->C = d->n_cameras; h->P = d->n_points; h->E = d->n_edges; h->prec = d->precision;
   h->ir_steps = d->precision == NSN_MIXED ? 2 : 0;
   h->f = d->focal; h->cx = d->cx; h->cy = d->cy; h->huber = d->huber_delta;
-  if (const char* fa = std::getenv("NSN_TEST_FAIL_ALLOC")) h->mem.fail_after = std::strtol(fa, nullptr, 10);
+  h->mem.fail_after = fail_after;
   HIP_TRY(hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking));
   for (auto& e : h->ev) HIP_TRY(hipEventCreate(&e));
   HIP_TRY(hipHostMalloc((void**)&h->hctl, sizeof(DevCtl), hipHostMallocDefault));
@@ -529,6 +547,82 @@ extern "C" int nsn_problem_create(const nsn_problem_desc* d, nsn_problem** out) 
   *h->hctl = make_ctl(h->log_cap);
   if ((rc = push_ctl(h.get()))) return rc;
   *out = h.release();
+  return NSN_OK;
+}
+
+extern "C" int nsn_problem_create_blocks(const nsn_block_desc* d, nsn_problem** out) {
+  clear_error();
+  if (!d || !out) return fail(NSN_EINVAL, "nsn_problem_create_blocks: null argument");
+  *out = nullptr;
+  if (d->n_u < 1 || d->n_v < 0 || d->n_w < 0 || d->n_uoff < 0 || (d->n_w > 0 && (!d->w_u || !d->w_v)) ||
+      (d->n_uoff > 0 && (!d->uoff_i || !d->uoff_j)))
+    return fail(NSN_EINVAL, "nsn_problem_create_blocks: bad sizes or missing block arrays");
+  if (d->precision != NSN_FP64 && d->precision != NSN_MIXED)
+    return fail(NSN_EINVAL, "nsn_problem_create_blocks: unknown precision %d", d->precision);
+  std::vector<long long> seen;
+  seen.reserve(std::max(d->n_w, d->n_uoff));
+  for (int k = 0; k < d->n_w; k++) {
+    if (d->w_u[k] < 0 || d->w_u[k] >= d->n_u || d->w_v[k] < 0 || d->w_v[k] >= d->n_v)
+      return fail(NSN_EINVAL, "nsn_problem_create_blocks: W block %d at (%d, %d) out of range", k, d->w_u[k], d->w_v[k]);
+    seen.push_back((long long)d->w_u[k] * d->n_v + d->w_v[k]);
+  }
+  std::sort(seen.begin(), seen.end());
+  if (std::adjacent_find(seen.begin(), seen.end()) != seen.end())
+    return fail(NSN_EINVAL, "nsn_problem_create_blocks: a W block is listed twice");
+  seen.clear();
+  for (int k = 0; k < d->n_uoff; k++) {
+    if (d->uoff_i[k] < 0 || d->uoff_j[k] >= d->n_u || d->uoff_i[k] >= d->uoff_j[k])
+      return fail(NSN_EINVAL, "nsn_problem_create_blocks: U block %d at (%d, %d) is not strictly upper", k,
+                  d->uoff_i[k], d->uoff_j[k]);
+    seen.push_back((long long)d->uoff_i[k] * d->n_u + d->uoff_j[k]);
+  }
+  std::sort(seen.begin(), seen.end());
+  if (std::adjacent_find(seen.begin(), seen.end()) != seen.end())
+    return fail(NSN_EINVAL, "nsn_problem_create_blocks: an off-diagonal U block is listed twice");
+  // the problem is built as a section whose edges are the W blocks: every
+  // U and V parameter variable; with no V parameter, one fixed placeholder
+  // point keeps the point arrays non-empty (it has no block)
+  const int P = std::max(d->n_v, 1);
+  std::vector<double> obs(2 * (size_t)std::max(d->n_w, 1), 0.0);
+  std::vector<unsigned char> pt_fixed(P, 0);
+  if (d->n_v == 0) pt_fixed[0] = 1;
+  nsn_problem_desc bd{};
+  bd.n_cameras = d->n_u;
+  bd.n_points = P;
+  bd.n_edges = d->n_w;
+  bd.cam_idx = d->w_u;
+  bd.pt_idx = d->w_v;
+  bd.obs = obs.data();
+  bd.pt_fixed = pt_fixed.data();
+  bd.focal = 1.0;
+  bd.precision = d->precision;
+  bd.device = d->device;
+  nsn_problem* h = nullptr;
+  int rc = problem_create_impl(&bd, &h, -1);
+  if (rc) return rc;
+  std::unique_ptr<nsn_problem> hp(h);
+  h->blocks_only = true;
+  if (d->n_uoff > 0) {
+    // stored in UID order (row-major (i, j)); uoff_ref maps the caller's order
+    std::vector<int> ord(d->n_uoff);
+    std::iota(ord.begin(), ord.end(), 0);
+    std::sort(ord.begin(), ord.end(), [&](int a, int b) {
+      return d->uoff_i[a] != d->uoff_i[b] ? d->uoff_i[a] < d->uoff_i[b] : d->uoff_j[a] < d->uoff_j[b];
+    });
+    std::vector<int> ui(d->n_uoff), uj(d->n_uoff);
+    for (int k = 0; k < d->n_uoff; k++) {
+      ui[k] = d->uoff_i[ord[k]];
+      uj[k] = d->uoff_j[ord[k]];
+    }
+    DevBuf& m = h->mem;
+    h->uoff_i = m.alloc<int>(d->n_uoff);
+    h->uoff_j = m.alloc<int>(d->n_uoff);
+    h->Uoff = m.alloc<char>(36 * (size_t)d->n_uoff * h->esz());
+    if (m.failed) return fail(NSN_ENOMEM, "nsn_problem_create_blocks: device allocation failed");
+    if ((rc = upload(h->uoff_i, ui)) || (rc = upload(h->uoff_j, uj))) return rc;
+    h->n_uoff = d->n_uoff;
+  }
+  *out = hp.release();
   return NSN_OK;
 }
 
@@ -552,6 +646,7 @@ extern "C" int nsn_problem_info_get(nsn_problem* h, nsn_problem_info* info) {
 }
 
 extern "C" int nsn_set_state_rt(nsn_problem* h, const double* cams_rt, const double* pts) {
+  if (h && h->blocks_only) return fail(NSN_ESTATE, "%s: a block problem has no edges or state (import its Hessian)", __func__);
   if (!h || !cams_rt || !pts) return fail(NSN_EINVAL, "null argument");
   HIP_TRY(hipSetDevice(h->device));
   HIP_TRY(hipMemcpy(h->cams, cams_rt, sizeof(double) * 12 * (size_t)h->C, hipMemcpyHostToDevice));
@@ -574,6 +669,7 @@ extern "C" int nsn_set_state(nsn_problem* h, const double* cams, const double* p
 }
 
 extern "C" int nsn_get_state_rt(nsn_problem* h, double* cams_rt, double* pts) {
+  if (h && h->blocks_only) return fail(NSN_ESTATE, "%s: a block problem has no edges or state (import its Hessian)", __func__);
   if (!h) return fail(NSN_EINVAL, "null argument");
   HIP_TRY(hipSetDevice(h->device));
   HIP_TRY(hipStreamSynchronize(h->st));
@@ -596,6 +692,7 @@ extern "C" int nsn_get_state(nsn_problem* h, double* cams, double* pts) {
 }
 
 extern "C" int nsn_backup(nsn_problem* h) {
+  if (h && h->blocks_only) return fail(NSN_ESTATE, "%s: a block problem has no edges or state (import its Hessian)", __func__);
   if (!h) return fail(NSN_EINVAL, "null argument");
   HIP_TRY(hipSetDevice(h->device));
   launch_copy_state(h->view, h->cams_bak, h->pts_bak, h->cams, h->pts, h->ctl, GATE_ALWAYS, h->st);
@@ -604,6 +701,7 @@ extern "C" int nsn_backup(nsn_problem* h) {
 }
 
 extern "C" int nsn_rollback(nsn_problem* h) {
+  if (h && h->blocks_only) return fail(NSN_ESTATE, "%s: a block problem has no edges or state (import its Hessian)", __func__);
   if (!h) return fail(NSN_EINVAL, "null argument");
   HIP_TRY(hipSetDevice(h->device));
   launch_copy_state(h->view, h->cams, h->pts, h->cams_bak, h->pts_bak, h->ctl, GATE_ALWAYS, h->st);
@@ -621,7 +719,9 @@ static int reset_ctl_primitive(nsn_problem* h) {
 }
 
 extern "C" int nsn_linearize(nsn_problem* h, int with_hessian, double* chi2) {
+  if (h && h->blocks_only) return fail(NSN_ESTATE, "%s: a block problem has no edges or state (import its Hessian)", __func__);
   if (!h) return fail(NSN_EINVAL, "null argument");
+  h->imported = false;
   HIP_TRY(hipSetDevice(h->device));
   int rc = reset_ctl_primitive(h);
   if (rc) return rc;
@@ -735,6 +835,26 @@ static int import_hessian_t(nsn_problem* h, const double* U, const double* V, co
   return NSN_OK;
 }
 
+template <typename T>
+static int import_uoff_t(nsn_problem* h, const double* Uoff) {
+  std::vector<T> hu(36 * (size_t)h->n_uoff);
+  for (size_t i = 0; i < hu.size(); i++) hu[i] = (T)Uoff[i];
+  HIP_TRY(hipMemcpy(h->Uoff, hu.data(), hu.size() * sizeof(T), hipMemcpyHostToDevice));
+  return NSN_OK;
+}
+
+extern "C" int nsn_import_hessian_blocks(nsn_problem* h, const double* U, const double* Uoff, const double* V,
+                                         const double* W, const double* bU, const double* bV, double chi2) {
+  if (!h) return fail(NSN_EINVAL, "null argument");
+  if (h->n_uoff > 0) {
+    if (!Uoff) return fail(NSN_EINVAL, "nsn_import_hessian_blocks: the problem has off-diagonal U blocks, Uoff is NULL");
+    HIP_TRY(hipSetDevice(h->device));
+    const int rc = h->prec == NSN_MIXED ? import_uoff_t<float>(h, Uoff) : import_uoff_t<double>(h, Uoff);
+    if (rc) return rc;
+  }
+  return nsn_import_hessian(h, U, V, W, bU, bV, chi2);
+}
+
 extern "C" int nsn_import_hessian(nsn_problem* h, const double* U, const double* V, const double* W, const double* bU,
                                   const double* bV, double chi2) {
   if (!h || !U || !V || !W || !bU || !bV) return fail(NSN_EINVAL, "null argument");
@@ -742,6 +862,7 @@ extern "C" int nsn_import_hessian(nsn_problem* h, const double* U, const double*
   int rc = h->prec == NSN_MIXED ? import_hessian_t<float>(h, U, V, W, bU, bV)
                                 : import_hessian_t<double>(h, U, V, W, bU, bV);
   if (rc) return rc;
+  h->imported = true;
   if ((rc = sync_ctl(h))) return rc;
   double mb = 0, md = 0;
   for (size_t i = 0; i < 6 * (size_t)h->NU; i++) mb = std::max(mb, std::fabs(bU[i]));
@@ -816,6 +937,7 @@ extern "C" int nsn_phase_times(nsn_problem* h, double* ms, int n) {
 extern "C" int nsn_set_priors(nsn_problem* h, int n_cam, const int* cam_ids, const double* cam_p0,
                               const double* cam_w, int n_pt, const int* pt_ids, const double* pt_p0,
                               const double* pt_w) {
+  if (h && h->blocks_only) return fail(NSN_ESTATE, "%s: a block problem has no edges or state (import its Hessian)", __func__);
   if (!h) return fail(NSN_EINVAL, "null argument");
   if (n_cam < 0 || n_pt < 0) return fail(NSN_EINVAL, "nsn_set_priors: negative count");
   if ((n_cam > 0 && (!cam_ids || !cam_p0 || !cam_w)) || (n_pt > 0 && (!pt_ids || !pt_p0 || !pt_w)))
@@ -891,6 +1013,7 @@ extern "C" int nsn_set_refinement(nsn_problem* h, int steps) {
 }
 
 extern "C" int nsn_oplus(nsn_problem* h) {
+  if (h && h->blocks_only) return fail(NSN_ESTATE, "%s: a block problem has no edges or state (import its Hessian)", __func__);
   if (!h) return fail(NSN_EINVAL, "null argument");
   if (!h->stepped) return fail(NSN_ESTATE, "nsn_oplus before nsn_schur_step");
   HIP_TRY(hipSetDevice(h->device));
@@ -900,6 +1023,7 @@ extern "C" int nsn_oplus(nsn_problem* h) {
 }
 
 extern "C" int nsn_rho(nsn_problem* h, double mu, double old_chi2, double* rho) {
+  if (h && h->blocks_only) return fail(NSN_ESTATE, "%s: a block problem has no edges or state (import its Hessian)", __func__);
   if (!h || !rho) return fail(NSN_EINVAL, "null argument");
   if (!h->stepped) return fail(NSN_ESTATE, "nsn_rho before nsn_schur_step");
   int rc = sync_ctl(h);
@@ -933,6 +1057,7 @@ extern "C" int nsn_rho(nsn_problem* h, double mu, double old_chi2, double* rho) 
 }
 
 extern "C" int nsn_set_observations(nsn_problem* h, const double* obs, const double* weight) {
+  if (h && h->blocks_only) return fail(NSN_ESTATE, "%s: a block problem has no edges or state (import its Hessian)", __func__);
   if (!h) return fail(NSN_EINVAL, "null argument");
   HIP_TRY(hipSetDevice(h->device));
   const int E = h->E, Ec = h->Ec;
@@ -1061,6 +1186,7 @@ int adjoint_t(nsn_problem* h, const AdjBufs& b, const double* g_cams, const doub
 
 extern "C" int nsn_adjoint(nsn_problem* h, const double* g_cams, const double* g_pts, int hessian_steps,
                            double* dL_dobs, double* dL_dw) {
+  if (h && h->blocks_only) return fail(NSN_ESTATE, "%s: a block problem has no edges or state (import its Hessian)", __func__);
   if (!h) return fail(NSN_EINVAL, "null argument");
   if (hessian_steps < 0 || hessian_steps > 16)
     return fail(NSN_EINVAL, "nsn_adjoint: hessian_steps must be in 0..16, got %d", hessian_steps);
@@ -1119,6 +1245,7 @@ size_t adj_scratch_need(const nsn_problem* h) {
 
 extern "C" int nsn_adjoint_noalloc(nsn_problem* h, const double* g_cams, const double* g_pts, int hessian_steps,
                                    double* dL_dobs, double* dL_dw) {
+  if (h && h->blocks_only) return fail(NSN_ESTATE, "%s: a block problem has no edges or state (import its Hessian)", __func__);
   if (!h) return fail(NSN_EINVAL, "null argument");
   if (hessian_steps < 0 || hessian_steps > 16)
     return fail(NSN_EINVAL, "nsn_adjoint_noalloc: hessian_steps must be in 0..16, got %d", hessian_steps);
@@ -1152,6 +1279,7 @@ extern "C" int nsn_adjoint_noalloc(nsn_problem* h, const double* g_cams, const d
 
 template <typename T>
 static int run_lm(nsn_problem* h, const nsn_lm_settings* s, nsn_lm_result* res, nsn_log_entry* log, int log_cap) {
+  h->imported = false;  // the drivers linearise from the edges
   DevCtl c = make_ctl(h->log_cap);
   c.mode = 1;
   c.need_lin = 1;
@@ -1207,6 +1335,7 @@ static int run_lm(nsn_problem* h, const nsn_lm_settings* s, nsn_lm_result* res, 
 
 extern "C" int nsn_lm_solve(nsn_problem* h, const nsn_lm_settings* s, nsn_lm_result* res, nsn_log_entry* log,
                             int log_cap) {
+  if (h && h->blocks_only) return fail(NSN_ESTATE, "%s: a block problem has no edges or state (import its Hessian)", __func__);
   if (!h || !s || !res) return fail(NSN_EINVAL, "null argument");
   if (s->maxNumSubIt < 1) return fail(NSN_EINVAL, "maxNumSubIt must be >= 1");
   HIP_TRY(hipSetDevice(h->device));
@@ -1215,6 +1344,7 @@ extern "C" int nsn_lm_solve(nsn_problem* h, const nsn_lm_settings* s, nsn_lm_res
 
 template <typename T>
 static int run_gn(nsn_problem* h, const nsn_gn_settings* s, nsn_lm_result* res, nsn_log_entry* log, int log_cap) {
+  h->imported = false;  // the drivers linearise from the edges
   DevCtl c = make_ctl(h->log_cap);
   c.mode = 2;
   c.maxNumIt = s->maxNumIt; c.minNumIt = s->minNumIt;
@@ -1255,6 +1385,7 @@ static int run_gn(nsn_problem* h, const nsn_gn_settings* s, nsn_lm_result* res, 
 
 extern "C" int nsn_gn_solve(nsn_problem* h, const nsn_gn_settings* s, nsn_lm_result* res, nsn_log_entry* log,
                             int log_cap) {
+  if (h && h->blocks_only) return fail(NSN_ESTATE, "%s: a block problem has no edges or state (import its Hessian)", __func__);
   if (!h || !s || !res) return fail(NSN_EINVAL, "null argument");
   HIP_TRY(hipSetDevice(h->device));
   return h->prec == NSN_MIXED ? run_gn<float>(h, s, res, log, log_cap) : run_gn<double>(h, s, res, log, log_cap);

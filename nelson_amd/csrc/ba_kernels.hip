@@ -993,6 +993,34 @@ void launch_begin_step(DevCtl* ctl, int gate, hipStream_t st) {
   hipLaunchKernelGGL(k_begin_step, dim3(1), dim3(64), 0, st, ctl, gate);
 }
 
+// ---------------------------------------------------------------------------
+// off-diagonal U blocks (U-U edges of a generic section, added to S after
+// the assembly: S = U - W V^-1 W^T, MatrixWWtMultiplier.hpp:124-153 copies the
+// U blocks, then subtracts the products).  Block k is the upper block
+// U_{i,j}, i < j, 6x6 row-major; S is stored lower, so U_{i,j}[b][a] goes to
+// entry (6 j + a, 6 i + b).  Distinct blocks touch distinct entries.
+// ---------------------------------------------------------------------------
+template <typename T, int TB>
+__global__ __launch_bounds__(256) void k_add_uoff(TileView tv, int nb, const int* __restrict__ ui,
+                                                  const int* __restrict__ uj, const T* __restrict__ Uoff,
+                                                  T* __restrict__ tiles, DevCtl* ctl, int gate) {
+  if (!gate_open(ctl, gate)) return;
+  const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (e >= 36LL * nb) return;
+  const int k = (int)(e / 36), a = (int)(e % 36) / 6, b = (int)(e % 6);
+  const int R = 6 * uj[k] + a, Cc = 6 * ui[k] + b;
+  const long long t = tile_index(R / TB, Cc / TB, tv.NT);
+  tiles[t * TB * TB + (long long)(Cc % TB) * TB + (R % TB)] += Uoff[36LL * k + 6 * b + a];
+}
+
+template <typename T>
+void launch_add_uoff(const TileView& tv, int nb, const int* ui, const int* uj, const T* Uoff, T* tiles, DevCtl* ctl,
+                     int gate, hipStream_t st) {
+  if (nb <= 0) return;
+  hipLaunchKernelGGL((k_add_uoff<T, 64>), dim3(nblocks(36LL * nb, 256)), dim3(256), 0, st, tv, nb, ui, uj, Uoff,
+                     tiles, ctl, gate);
+}
+
 template <typename T>
 void launch_zero(T* p, long long n, hipStream_t st) {
   if (n <= 0) return;
@@ -1016,7 +1044,9 @@ void launch_zero(T* p, long long n, hipStream_t st) {
   template void launch_ir_reduce<T>(const BAView&, const T*, const T*, const double*, const double*, double*, \
                                     int*, int*, int, DevCtl*, int, hipStream_t);                             \
   template void launch_xu_stats<T>(int, const double*, const T*, const Scratch&, DevCtl*, int, hipStream_t); \
-  template void launch_zero<T>(T*, long long, hipStream_t);
+  template void launch_zero<T>(T*, long long, hipStream_t);                                                 \
+  template void launch_add_uoff<T>(const TileView&, int, const int*, const int*, const T*, T*, DevCtl*, int,  \
+                                   hipStream_t);
 
 NSN_INST(double)
 NSN_INST(float)

@@ -152,6 +152,11 @@ void launch_gn_after_step(DevCtl* ctl, DevLog* log, hipStream_t st);
 void launch_gn_after_lin(DevCtl* ctl, DevLog* log, hipStream_t st);
 void launch_set_damping(DevCtl* ctl, double rel, double abs_, hipStream_t st);
 void launch_begin_step(DevCtl* ctl, int gate, hipStream_t st);
+// adds the off-diagonal U blocks (i < j, 6x6 row-major, nb of them) to the
+// assembled S tiles
+template <typename T>
+void launch_add_uoff(const TileView& tv, int nb, const int* ui, const int* uj, const T* Uoff, T* tiles, DevCtl* ctl,
+                     int gate, hipStream_t st);
 template <typename T>
 void launch_zero(T* p, long long n, hipStream_t st);
 

@@ -80,11 +80,11 @@ int main() {
       build(pss, c0, p0, ci, pi, obs, C, P);
       pss.update(true);
       SolverCholeskySchur solver;
-      solver.init(pss);
-      const bool ok = solver.computeIncrement(pss, 0.0, 62.0);
+      solver.init(pss.hessian().H(), pss.hessian().b());
+      const bool ok = solver.computeIncrement(pss.hessian().H(), pss.hessian().b(), 0.0, 62.0);
       std::printf("{\"solve\": \"step\", \"ok\": %s, \"sq_norm\": %.17g, \"max_abs_h_diag\": %.17g, \"du0\": %.17g}\n",
                   ok ? "true" : "false", solver.incrementVectorSquaredNorm(), solver.maxAbsHDiag(),
-                  solver.incrementU()[0]);
+                  solver.incrementVector().bU()[0]);
     }
     {
       BASection pss(g.focal, g.cx, g.cy);

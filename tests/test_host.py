@@ -129,3 +129,35 @@ def test_cpp_interface_compiles_and_links():
                         src], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     assert r.returncode == 0, r.stdout
     assert os.path.exists(os.path.join(ROOT, "tests", "cpp", "test_facade"))
+
+
+def _block_desc(n_u, n_v, w, uo):
+    lib = capi.lib()
+    d = capi.BlockDesc()
+    wu = np.array([a for a, _ in w] or [0], np.int32)
+    wv = np.array([b for _, b in w] or [0], np.int32)
+    oi = np.array([a for a, _ in uo] or [0], np.int32)
+    oj = np.array([b for _, b in uo] or [0], np.int32)
+    ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))  # noqa: E731
+    d.n_u, d.n_v, d.n_w, d.w_u, d.w_v = n_u, n_v, len(w), ip(wu), ip(wv)
+    d.n_uoff, d.uoff_i, d.uoff_j = len(uo), ip(oi), ip(oj)
+    d.precision, d.device = capi.NSN_FP64, 0
+    h = C.c_void_p()
+    rc = lib.nsn_problem_create_blocks(C.byref(d), C.byref(h))
+    return rc, lib.nsn_last_error().decode()
+
+
+@pytest.mark.parametrize("n_u,n_v,w,uo,what", [
+    (0, 2, [], [], "bad sizes"),
+    (3, 2, [(0, 0), (3, 1)], [], "out of range"),
+    (3, 2, [(0, 1), (0, 1)], [], "listed twice"),
+    (3, 2, [(0, 1)], [(1, 1)], "strictly upper"),
+    (3, 2, [(0, 1)], [(2, 1)], "strictly upper"),
+    (3, 2, [(0, 1)], [(0, 2), (0, 2)], "listed twice"),
+])
+def test_block_problem_structure_is_validated(n_u, n_v, w, uo, what):
+    """nsn_problem_create_blocks checks the block structure on the host
+    before touching a device: repeated or out-of-range W blocks, off-diagonal
+    U blocks that are not strictly upper or repeated."""
+    rc, msg = _block_desc(n_u, n_v, w, uo)
+    assert rc == capi.NSN_EINVAL and what in msg, (rc, msg)
