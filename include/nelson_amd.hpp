@@ -42,6 +42,7 @@
 #include <sstream>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -182,6 +183,8 @@ class HessianVectors {
   };
   const Part& bU() const { return u_; }
   const Part& bV() const { return v_; }
+  // a single section's increment (SingleSection::HessianVecType::segment)
+  ConstSeg segment(int i) const { return u_.segment(i); }
   double squaredNorm() const {
     double s = 0, t = 0;
     for (double x : u_.v_) s += x * x;
@@ -430,6 +433,8 @@ class EdgeBase {
   virtual void applyUpdaters_() = 0;
   int uid_ = -1;          // W block / off-diagonal U block of a binary edge
   bool uidTrans_ = false; // the off-diagonal U block is stored as (j, i)
+  std::vector<int> pairUid_;     // n-ary edges: the off-diagonal U block of each
+  std::vector<char> pairTrans_;  // variable pair (a < b), in pair order
 
  protected:
   void setChi2(double v) { chi2_ = v; }
@@ -438,6 +443,26 @@ class EdgeBase {
   double chi2_ = 0.0;
 };
 
+}  // namespace detail
+
+namespace detail {
+// the off-diagonal updaters may take the reference's transpose flag (the
+// view handed over is already oriented, so it is always false) or not
+template <class D, class = void>
+struct H12TakesFlag : std::false_type {};
+template <class D>
+struct H12TakesFlag<D, std::void_t<decltype(std::declval<D&>().updateH12Block(std::declval<MatRef>(), false))>>
+    : std::true_type {};
+template <class D, class = void>
+struct HijTakesFlag : std::false_type {};
+template <class D>
+struct HijTakesFlag<D, std::void_t<decltype(std::declval<D&>().updateHBlock(0, 0, std::declval<MatRef>(), false))>>
+    : std::true_type {};
+template <class D>
+void updateH12(D* d, MatRef H) {
+  if constexpr (H12TakesFlag<D>::value) d->updateH12Block(H, false);
+  else d->updateH12Block(H);
+}
 }  // namespace detail
 
 // an edge on one U parameter: the derived edge implements update(bool) and
@@ -499,7 +524,7 @@ class EdgeBinaryUV : public detail::EdgeBase {
       d->updateH11Block(H.Ublock(u), H.bUseg(u));
     }
     if (j_.isVariable()) d->updateH22Block(H.Vblock(j_.id()), H.bVseg(j_.id()));
-    if (i_.isVariable() && j_.isVariable()) d->updateH12Block(H.Wblock(uid_));
+    if (i_.isVariable() && j_.isVariable()) detail::updateH12(d, H.Wblock(uid_));
   }
 
  private:
@@ -529,12 +554,53 @@ class EdgeBinaryUU : public detail::EdgeBase {
       const int u = section_->internalU_(j_.id());
       d->updateH22Block(H.Ublock(u), H.bUseg(u));
     }
-    if (i_.isVariable() && j_.isVariable()) d->updateH12Block(H.Uoffblock(uid_, uidTrans_));
+    if (i_.isVariable() && j_.isVariable()) detail::updateH12(d, H.Uoffblock(uid_, uidTrans_));
   }
 
  private:
   Section* section_ = nullptr;
   NodeId i_, j_;
+};
+
+// an edge on N parameters of the first section (nelson/EdgeNary.h:125-150;
+// N = kDynamic: the count is set by addEdge): parameter(i) / parId(i) /
+// numParams(); the derived edge implements update(bool) and
+// updateHBlock(i, MatRef H, VecRef b) for each variable parameter i and
+// updateHBlock(i, j, MatRef H[, bool transpose]) for each variable pair
+// i < j (H: rows of parameter i, columns of parameter j)
+constexpr int kDynamic = -1;
+
+template <class Section, class Derived, int N>
+class EdgeNaryU : public detail::EdgeBase {
+ public:
+  NodeId parId(int i) const { return ids_.at(i); }
+  int numParams() const { return (int)ids_.size(); }
+  const auto& parameter(int i) const { return section_->parameterU(ids_.at(i)); }
+  Section& section() const { return *section_; }
+  void bind_(Section* s, std::vector<NodeId> ids) { section_ = s, ids_ = std::move(ids); }
+  void applyUpdaters_() override {
+    auto& H = section_->hessianStorage_();
+    Derived* d = static_cast<Derived*>(this);
+    const int n = numParams();
+    for (int a = 0; a < n; a++)
+      if (ids_[a].isVariable()) {
+        const int u = section_->internalU_(ids_[a].id());
+        d->updateHBlock(a, H.Ublock(u), H.bUseg(u));
+      }
+    size_t k = 0;
+    for (int a = 0; a < n; a++)
+      for (int b = a + 1; b < n; b++)
+        if (ids_[a].isVariable() && ids_[b].isVariable()) {
+          MatRef Hab = H.Uoffblock(pairUid_[k], pairTrans_[k] != 0);
+          if constexpr (detail::HijTakesFlag<Derived>::value) d->updateHBlock(a, b, Hab, false);
+          else d->updateHBlock(a, b, Hab);
+          k++;
+        }
+  }
+
+ private:
+  Section* section_ = nullptr;
+  std::vector<NodeId> ids_;
 };
 
 // Host storage of the section's Hessian in internal order (U diagonal blocks
@@ -601,8 +667,10 @@ class DoubleSectionHessian : public SchurSystem {
     for (int k = 0; k < nW; k++)
       for (int a = 0; a < BU; a++)
         for (int c = 0; c < BV; c++) W[18 * (size_t)k + 3 * a + c] = W_[(size_t)BU * BV * k + BV * a + c];
-    check(nsn_import_hessian_blocks(problem(), U.data(), nO ? Uo.data() : nullptr, V.data(), W.data(), bU.data(),
-                                    bV.data(), chi2_),
+    double none = 0.0;  // a section without V parameters or W blocks passes empty arrays
+    auto ptr = [&](std::vector<double>& v) { return v.empty() ? &none : v.data(); };
+    check(nsn_import_hessian_blocks(problem(), ptr(U), nO ? Uo.data() : nullptr, ptr(V), ptr(W), ptr(bU), ptr(bV),
+                                    chi2_),
           "nsn_import_hessian_blocks");
     dirty_ = false;
   }
@@ -686,6 +754,8 @@ class DoubleSection {
   using EdgeBinaryUV = nelson_amd::EdgeBinaryUV<Derived, E>;
   template <class E>
   using EdgeBinaryUU = nelson_amd::EdgeBinaryUU<Derived, E>;
+  template <class E, int N = kDynamic>
+  using EdgeNaryU = nelson_amd::EdgeNaryU<Derived, E, N>;
 
   DoubleSection() = default;
   DoubleSection(int numU, int numV) : nU_(numU), nV_(numV) {}
@@ -740,6 +810,16 @@ class DoubleSection {
     edges_.push_back(std::move(own));
   }
 
+  // an n-ary edge on parameters of the first section
+  template <class E, int N>
+  void addEdge(const std::array<NodeId, N>& ids, nelson_amd::EdgeNaryU<Derived, E, N>* e) {
+    addNary(std::vector<NodeId>(ids.begin(), ids.end()), e);
+  }
+  template <class E>
+  void addEdge(const std::vector<NodeId>& ids, nelson_amd::EdgeNaryU<Derived, E, kDynamic>* e) {
+    addNary(ids, e);
+  }
+
   // the internal order of the U parameters (bU and the increment's U part
   // follow it; nelson/DoubleSection.hpp:30-63): u2i[user index] = internal
   void setUser2InternalIndexesU(const std::vector<int>& u2i) {
@@ -758,7 +838,8 @@ class DoubleSection {
     if (!ready_ || structured_) throw Error(NSN_ESTATE, "permuteAMD_SchurU outside parametersReady .. structureReady");
     const int n = numParametersU();
     std::vector<std::set<int>> adj(n);
-    for (auto& e : uu_) adj[e.second.first].insert(e.second.second), adj[e.second.second].insert(e.second.first);
+    for (auto* lst : {&uu_, &nary_})
+      for (auto& e : *lst) adj[e.second.first].insert(e.second.second), adj[e.second.second].insert(e.second.first);
     std::map<int, std::vector<int>> byV;
     for (auto& e : uv_) byV[e.second.second].push_back(e.second.first);
     for (auto& kv : byV)
@@ -790,10 +871,11 @@ class DoubleSection {
     if (structured_) throw Error(NSN_ESTATE, "structureReady called twice");
     std::vector<std::pair<int, int>> w, o;
     for (auto& e : uv_) w.push_back({u2i_[e.second.first], e.second.second});
-    for (auto& e : uu_) {
-      const int a = u2i_[e.second.first], b = u2i_[e.second.second];
-      o.push_back({std::min(a, b), std::max(a, b)});
-    }
+    for (auto* lst : {&uu_, &nary_})
+      for (auto& e : *lst) {
+        const int a = u2i_[e.second.first], b = u2i_[e.second.second];
+        o.push_back({std::min(a, b), std::max(a, b)});
+      }
     auto uniq = [](std::vector<std::pair<int, int>>& v) {
       std::sort(v.begin(), v.end());
       v.erase(std::unique(v.begin(), v.end()), v.end());
@@ -808,6 +890,13 @@ class DoubleSection {
       edges_[e.first]->uid_ =
           (int)(std::lower_bound(o.begin(), o.end(), std::make_pair(std::min(a, b), std::max(a, b))) - o.begin());
       edges_[e.first]->uidTrans_ = a > b;
+    }
+    for (auto& e : nary_) {
+      const int a = u2i_[e.second.first], b = u2i_[e.second.second];
+      auto& ed = *edges_[e.first];
+      ed.pairUid_.push_back(
+          (int)(std::lower_bound(o.begin(), o.end(), std::make_pair(std::min(a, b), std::max(a, b))) - o.begin()));
+      ed.pairTrans_.push_back(a > b ? 1 : 0);
     }
     H_.setup_(this, numParametersU(), numParametersV(), w, o, p, device);
     structured_ = true;
@@ -847,8 +936,28 @@ class DoubleSection {
   HessianT& hessianStorage_() { return H_; }
   int internalU_(int userId) const { return u2i_[userId]; }
 
+ protected:
+  int sizeU_() const { return nU_; }
+
  private:
   Derived& self() { return static_cast<Derived&>(*this); }
+  template <class Edge>
+  void addNary(const std::vector<NodeId>& ids, Edge* e) {
+    std::unique_ptr<detail::EdgeBase> own(e);
+    std::vector<int> vars;
+    for (const NodeId& i : ids) {
+      checkAdd(i, true);
+      if (i.isVariable()) vars.push_back(i.id());
+    }
+    std::vector<int> sorted(vars);
+    std::sort(sorted.begin(), sorted.end());
+    if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
+      throw Error(NSN_EINVAL, "addEdge: an n-ary edge lists a parameter twice");
+    e->bind_(&self(), ids);
+    for (size_t a = 0; a < vars.size(); a++)
+      for (size_t b = a + 1; b < vars.size(); b++) nary_.push_back({edges_.size(), {vars[a], vars[b]}});
+    edges_.push_back(std::move(own));
+  }
   void checkAdd(NodeId i, bool isU) {
     if (!ready_) throw Error(NSN_ESTATE, "addEdge before parametersReady");
     if (structured_) throw Error(NSN_ESTATE, "addEdge after structureReady");
@@ -861,7 +970,52 @@ class DoubleSection {
   std::vector<int> u2i_;
   std::vector<std::unique_ptr<detail::EdgeBase>> edges_;
   std::vector<std::pair<size_t, std::pair<int, int>>> uv_, uu_;  // (edge, (user U id, V / U id))
+  std::vector<std::pair<size_t, std::pair<int, int>>> nary_;    // (edge, variable pair), in pair order
   HessianT H_;
+};
+
+// ---------------------------------------------------------------------------
+// A single section (nelson/SingleSection.hpp:63-368: pose graphs, scan
+// alignment): one parameter set, unary and binary edges between its
+// parameters.  It is a DoubleSection without a second section, so H = U and
+// the GPU factorises it directly (S = U, no V blocks, no W).
+// ---------------------------------------------------------------------------
+namespace detail {
+struct NoParameter {};
+}  // namespace detail
+
+template <class Derived, class Par, int B>
+class SingleSection : public DoubleSection<Derived, Par, detail::NoParameter, B, 1> {
+  using Base = DoubleSection<Derived, Par, detail::NoParameter, B, 1>;
+
+ public:
+  using ParameterType = Par;
+  using HessianVecType = HessianVectors;
+  // edges: update(bool) with setChi2; unary: updateHBlock(MatRef H, VecRef
+  // b); binary: updateH11Block(H, b), updateH22Block(H, b),
+  // updateH12Block(MatRef H) with H the block of rows parameter_1, columns
+  // parameter_2 (already oriented: no transpose flag)
+  template <class E>
+  using EdgeUnary = nelson_amd::EdgeUnaryU<Derived, E>;
+  template <class E>
+  using EdgeBinary = nelson_amd::EdgeBinaryUU<Derived, E>;
+  template <class E, int N = kDynamic>
+  using EdgeNary = nelson_amd::EdgeNaryU<Derived, E, N>;
+
+  SingleSection() = default;
+  explicit SingleSection(int numParameters) : Base(numParameters, 0) {}
+
+  // the user implements parameter(NodeId) (both constnesses)
+  virtual int numParameters() const { return this->sizeU_(); }
+  int numParametersU() const override { return static_cast<const Derived&>(*this).numParameters(); }
+  int numParametersV() const override { return 0; }
+  Par& parameterU(NodeId i) { return static_cast<Derived&>(*this).parameter(i); }
+  const Par& parameterU(NodeId i) const { return static_cast<const Derived&>(*this).parameter(i); }
+
+  // nelson/SingleSection.hpp names of the ordering calls
+  void setUser2InternalIndexes(const std::vector<int>& u2i) { Base::setUser2InternalIndexesU(u2i); }
+  void permuteAMD() { Base::permuteAMD_SchurU(); }
+  const std::vector<int>& user2internalIndexes() const { return Base::user2internalIndexesU(); }
 };
 
 // ---------------------------------------------------------------------------

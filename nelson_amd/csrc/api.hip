@@ -827,11 +827,10 @@ static int import_hessian_t(nsn_problem* h, const double* U, const double* V, co
   }
   for (size_t i = 0; i < hbu.size(); i++) hbu[i] = (T)bU[i];
   for (size_t i = 0; i < hbv.size(); i++) hbv[i] = (T)bV[i];
-  HIP_TRY(hipMemcpy(h->U, hu.data(), hu.size() * sizeof(T), hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(h->V, hv.data(), hv.size() * sizeof(T), hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(h->W, hw.data(), hw.size() * sizeof(T), hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(h->bU, hbu.data(), hbu.size() * sizeof(T), hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(h->bV, hbv.data(), hbv.size() * sizeof(T), hipMemcpyHostToDevice));
+  int rc;
+  if ((rc = upload((T*)h->U, hu)) || (rc = upload((T*)h->V, hv)) || (rc = upload((T*)h->W, hw)) ||
+      (rc = upload((T*)h->bU, hbu)) || (rc = upload((T*)h->bV, hbv)))
+    return rc;
   return NSN_OK;
 }
 

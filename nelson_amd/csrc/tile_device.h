@@ -96,8 +96,8 @@ __device__ __forceinline__ double rsqrt_f64(double d) {
 // (SolverCholeskyDense.hpp:15-18) and solves with the pseudo-inverse of D:
 // the pivot is kept as 1 with a stored reciprocal of 0, so the inverse of
 // the factor tile has that row and column zero and the solution entry comes
-// out 0, as in the reference; a zero pivot over a non-zero column is a
-// failure, as there.  The step is branch-free: every
+// out 0, as in the reference; a zero pivot over a non-zero column (checked
+// after the panel) is a failure, as there.  The step is branch-free: every
 // lane computes the same pivot, the reciprocal square root of the selected
 // value, and the updates run on every lane (entries above the diagonal in the
 // registers are garbage and never written back).  After scaling, the entry
@@ -114,7 +114,7 @@ __device__ __forceinline__ void potrf_panel(double* A, double* col, int g0, int 
 #pragma unroll
   for (int c = 0; c < kPW; c++) a[c] = A[(C0 + c) * TB + lane];
   double myr = 1.0;
-  int zcol = 0;  // this lane's entry below a zero pivot is non-zero
+  unsigned zmask = 0;  // panel columns whose pivot is exactly zero (wave-uniform)
 #pragma unroll
   for (int k = 0; k < kPW; k++) {
     const double d = readlane_f64(a[k], C0 + k);
@@ -122,11 +122,13 @@ __device__ __forceinline__ void potrf_panel(double* A, double* col, int g0, int 
     const bool good = d > 0.0 && d < __builtin_inf();
     const bool zero = !pad && d == 0.0;
     bad |= (!pad && !good && !zero) ? 1 : 0;
-    zcol |= (zero && lane > C0 + k && a[k] != 0.0) ? 1 : 0;
-    const double r = zero ? 0.0 : ((pad || !good) ? 1.0 : rsqrt_f64(d));
+    zmask |= zero ? (1u << k) : 0u;
+    // a zero pivot scales its column by 1: the column is left as it is, and
+    // checked after the panel
+    const double r = (pad || !good) ? 1.0 : rsqrt_f64(d);
     const double piv = (pad || !good) ? 1.0 : d * r;
     a[k] = (lane == C0 + k) ? piv : a[k] * r;
-    myr = (lane == C0 + k) ? r : myr;
+    myr = (lane == C0 + k) ? (zero ? 0.0 : r) : myr;
     if (k + 1 < kPW) {
       // the next pivot's column first, by a lane read (the critical path)
       a[k + 1] -= a[k] * readlane_f64(a[k], C0 + k + 1);
@@ -145,7 +147,12 @@ __device__ __forceinline__ void potrf_panel(double* A, double* col, int g0, int 
   for (int c = 0; c < kPW; c++)
     if (lane >= C0 + c) A[(C0 + c) * TB + lane] = a[c];
   if (lane >= C0 && lane < C0 + kPW) A[rdiag_slot<TB>(lane)] = myr;
-  if (__ballot(zcol) != 0ull) bad = 1;
+  if (zmask != 0u) {  // semidefinite S only: every entry below a zero pivot must be zero
+    int nz = 0;
+#pragma unroll
+    for (int c = 0; c < kPW; c++) nz |= (((zmask >> c) & 1u) && lane > C0 + c && a[c] != 0.0) ? 1 : 0;
+    if (__ballot(nz) != 0ull) bad = 1;
+  }
 }
 
 // D_(rb, cb) -= A_(rb, p) A_(cb, p)^T for one 16x16 block of the FP64 tile

@@ -86,8 +86,16 @@ double or_hessian_max_abs_b(const or_hessian* h) {
 #ifdef _OPENMP
 #include <omp.h>
 static int host_threads(void) { return omp_get_max_threads(); }
+int or_set_threads(int n) {
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+}
 #else
 static int host_threads(void) { return 1; }
+int or_set_threads(int n) {
+  (void)n;
+  return 1;
+}
 #endif
 
 int or_ldlt_factor(double* a, int n, int* tp) {

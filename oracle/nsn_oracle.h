@@ -171,6 +171,10 @@ void or_so3_jl_inv(const double* w, double* J);
 void or_rodrigues(const double* w, double* R);
 void or_se3_left_update(const double* delta, double* R, double* t);
 
+/* host threads of the LDL^T row updates (OpenMP; results do not depend on
+ * the count); n <= 0 only reads it.  Returns the count in use. */
+int or_set_threads(int n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -111,6 +111,7 @@ def lib():
             "or_so3_jl_inv": (None, [_D, _D]),
             "or_rodrigues": (None, [_D, _D]),
             "or_se3_left_update": (None, [_D, _D, _D]),
+            "or_set_threads": (C.c_int, [C.c_int]),
         }
         for n, (r, a) in sig.items():
             f = getattr(L, n)
