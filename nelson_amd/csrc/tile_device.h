@@ -78,26 +78,24 @@ __device__ __forceinline__ double readlane_f64(double v, int l) {
 template <int TB>
 __device__ __forceinline__ int rdiag_slot(int c) { return c < TB - 1 ? (c + 1) * TB : (TB - 1) * TB + 1; }
 
-// 1/sqrt(d) from the hardware estimate refined by two Newton steps
+// 1/sqrt(d) from the hardware estimate y refined by one third-order step,
+// y (1 + e/2 + 3 e^2 / 8) with e = 1 - d y^2 (four dependent operations; two
+// Newton steps take seven)
 __device__ __forceinline__ double rsqrt_f64(double d) {
-  double r = __builtin_amdgcn_rsq(d);
-  r = r * (1.5 - 0.5 * d * r * r);
-  r = r * (1.5 - 0.5 * d * r * r);
-  return r;
+  const double y = __builtin_amdgcn_rsq(d);
+  const double e = __builtin_fma(-(d * y), y, 1.0);
+  return __builtin_fma(y * e, __builtin_fma(e, 0.375, 0.5), y);
 }
 
 // right-looking Cholesky of columns C0 .. C0+15 of the 64-row FP64 tile A
 // (rows >= C0), wave 0 only.  Rows with global index >= n (the bS row and the
-// identity padding) take pivot 1 and never count as failures; a failed pivot
-// is replaced by 1 and counted in `bad`.  A pivot that is exactly zero with
-// an exactly zero column below it (a row and column of S that no term
-// touches, e.g. a variable camera without observations and without damping)
-// is the semidefinite case that the reference's pivoted LDL^T accepts
-// (SolverCholeskyDense.hpp:15-18) and solves with the pseudo-inverse of D:
-// the pivot is kept as 1 with a stored reciprocal of 0, so the inverse of
-// the factor tile has that row and column zero and the solution entry comes
-// out 0, as in the reference; a zero pivot over a non-zero column (checked
-// after the panel) is a failure, as there.  The step is branch-free: every
+// identity padding) take the reciprocal 1 and never count as failures; a
+// failed pivot (not positive and finite) also takes 1 and sets `bad`.  Each
+// column is scaled by its reciprocal square root r on every lane, so the
+// diagonal lane keeps d r = sqrt(d) for a good pivot and d itself for a
+// padded or failed one (nothing reads the diagonal of a factor tile except
+// potrf_lds's check of failed pivots; the solves use the reciprocals, stored
+// in the upper triangle).  The step is branch-free: every
 // lane computes the same pivot, the reciprocal square root of the selected
 // value, and the updates run on every lane (entries above the diagonal in the
 // registers are garbage and never written back).  After scaling, the entry
@@ -109,26 +107,26 @@ __device__ __forceinline__ double rsqrt_f64(double d) {
 // after the panel.
 template <int TB, int C0>
 __device__ __forceinline__ void potrf_panel(double* A, double* col, int g0, int n, int& bad) {
-  const int lane = threadIdx.x;
+  // the lane index is opaque to the optimiser, so that the lane tests below
+  // are formed where they are used: hoisted out of the enclosing loops they
+  // become scalar masks that the register allocator spills to vector lanes
+  // and reads back on the pivot path (measured 3.06 -> 2.92 ms for the
+  // 5994-row factorisation)
+  int lane = threadIdx.x;
+  asm volatile("" : "+v"(lane));
   double a[kPW];
 #pragma unroll
   for (int c = 0; c < kPW; c++) a[c] = A[(C0 + c) * TB + lane];
   double myr = 1.0;
-  unsigned zmask = 0;  // panel columns whose pivot is exactly zero (wave-uniform)
 #pragma unroll
   for (int k = 0; k < kPW; k++) {
     const double d = readlane_f64(a[k], C0 + k);
     const bool pad = g0 + C0 + k >= n;
     const bool good = d > 0.0 && d < __builtin_inf();
-    const bool zero = !pad && d == 0.0;
-    bad |= (!pad && !good && !zero) ? 1 : 0;
-    zmask |= zero ? (1u << k) : 0u;
-    // a zero pivot scales its column by 1: the column is left as it is, and
-    // checked after the panel
+    bad |= (!pad && !good) ? 1 : 0;
     const double r = (pad || !good) ? 1.0 : rsqrt_f64(d);
-    const double piv = (pad || !good) ? 1.0 : d * r;
-    a[k] = (lane == C0 + k) ? piv : a[k] * r;
-    myr = (lane == C0 + k) ? (zero ? 0.0 : r) : myr;
+    a[k] *= r;
+    myr = (lane == C0 + k) ? r : myr;
     if (k + 1 < kPW) {
       // the next pivot's column first, by a lane read (the critical path)
       a[k + 1] -= a[k] * readlane_f64(a[k], C0 + k + 1);
@@ -147,12 +145,6 @@ __device__ __forceinline__ void potrf_panel(double* A, double* col, int g0, int 
   for (int c = 0; c < kPW; c++)
     if (lane >= C0 + c) A[(C0 + c) * TB + lane] = a[c];
   if (lane >= C0 && lane < C0 + kPW) A[rdiag_slot<TB>(lane)] = myr;
-  if (zmask != 0u) {  // semidefinite S only: every entry below a zero pivot must be zero
-    int nz = 0;
-#pragma unroll
-    for (int c = 0; c < kPW; c++) nz |= (((zmask >> c) & 1u) && lane > C0 + c && a[c] != 0.0) ? 1 : 0;
-    if (__ballot(nz) != 0ull) bad = 1;
-  }
 }
 
 // D_(rb, cb) -= A_(rb, p) A_(cb, p)^T for one 16x16 block of the FP64 tile
@@ -186,6 +178,8 @@ __device__ __forceinline__ void potrf_next(double* A, double* col, int g0, int n
   __syncthreads();
   if (wv == 0) {
     potrf_panel<TB, kPW * (P + 1)>(A, col, g0, n, bad);
+    // the last panel leaves the tile's failure flag for every thread
+    if (P == 2 && threadIdx.x == 0) ((volatile int*)(col + 128))[0] = bad;
   } else if (P == 0) {
     // blocks (2,2), (3,2), (3,3) with panel 0
     const int rb = wv == 3 ? 3 : wv + 1, cb = wv == 1 ? 2 : (wv == 2 ? 2 : 3);
@@ -196,7 +190,17 @@ __device__ __forceinline__ void potrf_next(double* A, double* col, int g0, int n
   __syncthreads();
 }
 
-// col: 128 doubles of LDS that nothing else uses during the call
+// col: 129 doubles of LDS that nothing else uses during the call (the
+// broadcast buffer and the failure flag).  When a pivot failed, the tile is
+// checked column by column (all threads, rare): a pivot that is exactly
+// zero (its diagonal entry was left at d = 0) with an exactly zero column
+// below it is the semidefinite case that the reference's pivoted LDL^T
+// accepts (SolverCholeskyDense.hpp:15-18) and solves with the pseudo-inverse
+// of D: its stored reciprocal becomes 0, so the inverse of the factor tile
+// has that row and column zero and the solution entry comes out 0, as in the
+// reference (a row and column of S that no term touches: a variable camera
+// without observations, undamped).  Any other failed pivot, or a zero pivot
+// over a non-zero column, is a failure, as there.
 template <int TB>
 __device__ __forceinline__ void potrf_lds(double* A, double* col, int g0, int n, DevCtl* ctl) {
   static_assert(TB == 4 * kPW, "four panels");
@@ -206,6 +210,22 @@ __device__ __forceinline__ void potrf_lds(double* A, double* col, int g0, int n,
   potrf_next<TB, 0>(A, col, g0, n, bad);
   potrf_next<TB, 1>(A, col, g0, n, bad);
   potrf_next<TB, 2>(A, col, g0, n, bad);
+  if (((volatile int*)(col + 128))[0] != 0) {  // uniform
+    volatile int* fails = (volatile int*)(col + 128) + 1;
+    if (threadIdx.x == 0) *fails = 0;
+    __syncthreads();
+    for (int e = threadIdx.x; e < TB * TB; e += 256) {
+      const int c = e / TB, r = e % TB;
+      const double v = A[c * TB + c];
+      if (g0 + c >= n || (v > 0.0 && v < __builtin_inf())) continue;  // padded or good
+      if (v != 0.0 || (r > c && A[c * TB + r] != 0.0)) *fails = 1;
+    }
+    __syncthreads();
+    if (threadIdx.x < TB && g0 + threadIdx.x < n && A[threadIdx.x * TB + threadIdx.x] == 0.0)
+      A[rdiag_slot<TB>(threadIdx.x)] = 0.0;
+    bad = *fails;
+    __syncthreads();
+  }
   if (threadIdx.x == 0 && bad) atomicAdd(&ctl->fail_s, 1);
 }
 
@@ -221,7 +241,9 @@ template <int TB>
 __device__ __forceinline__ void tri_inverse(const double* L, double* Li) {
   static_assert(TB == 64, "four 16x16 blocks");
   typedef Mfma<double>::acc_t acc_t;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));  // lane tests formed in place (see potrf_panel)
+  const int lane = tid & 63, wv = threadIdx.x >> 6;
   if (lane < 16) {
     // branch-free: for i < j every term is zero, so x_i = 0 without a test,
     // and the loads of L (the same for every lane) can all be issued early
