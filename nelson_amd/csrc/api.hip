@@ -38,6 +38,8 @@ constexpr int kTrsvMaxWG = 256;
 // the S assembly splits tiles into smaller parts until it has at least this
 // many workgroups (four per CU) or the parts are single camera blocks
 constexpr int kMinSParts = 1024;
+// LM / GN sub-iterations queued between two host synchronisations (at most)
+constexpr int kLmReplaysPerSync = 8;
 
 // Device allocations of one owner.  A failed allocation returns nullptr and
 // sets `failed`, so a sequence of allocations is checked once at its end.
@@ -1304,9 +1306,16 @@ static int run_lm(nsn_problem* h, const nsn_lm_settings* s, nsn_lm_result* res, 
   }
   const long long max_sub = (long long)std::max(1, s->maxNumIt) * std::max(1, s->maxNumSubIt) + 4;
   auto t0 = std::chrono::steady_clock::now();
-  for (long long k = 0; k < max_sub; k++) {
-    if (h->lm_graph) HIP_TRY(hipGraphLaunch(h->lm_graph, h->st));
-    else seq_lm_subiter<T>(h);
+  // sub-iterations are queued in batches of 1, 2, 4 and then 8 between host
+  // synchronisations: every kernel is gated on the device's done flag, so the
+  // replays queued after the solve ends do nothing, and a short solve wastes
+  // few of them
+  long long queued = 0;
+  for (int batch = 1; queued < max_sub; batch = std::min(2 * batch, kLmReplaysPerSync)) {
+    for (int b = 0; b < batch && queued < max_sub; b++, queued++) {
+      if (h->lm_graph) HIP_TRY(hipGraphLaunch(h->lm_graph, h->st));
+      else seq_lm_subiter<T>(h);
+    }
     if ((rc = sync_ctl(h))) return rc;
     if (h->hctl->done || h->hctl->wait_timeout) break;
   }
@@ -1355,10 +1364,13 @@ static int run_gn(nsn_problem* h, const nsn_gn_settings* s, nsn_lm_result* res, 
   auto t0 = std::chrono::steady_clock::now();
   seq_lin_full<T>(h, GATE_ALWAYS);
   launch_gn_after_lin(h->ctl, h->log, h->st);
-  for (int k = 0; k < std::max(1, s->maxNumIt) + 2; k++) {
+  // iterations queued in batches between synchronisations, as in run_lm
+  const int max_it = std::max(1, s->maxNumIt) + 2;
+  int queued = 0;
+  for (int batch = 1; queued < max_it; batch = std::min(2 * batch, kLmReplaysPerSync)) {
     if ((rc = sync_ctl(h))) return rc;
     if (h->hctl->done || h->hctl->wait_timeout) break;
-    seq_gn_iter<T>(h);
+    for (int b = 0; b < batch && queued < max_it; b++, queued++) seq_gn_iter<T>(h);
   }
   if ((rc = sync_ctl(h))) return rc;
   auto t1 = std::chrono::steady_clock::now();
