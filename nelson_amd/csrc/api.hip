@@ -126,6 +126,9 @@ struct nsn_problem {
   bool linearized = false, stepped = false;
   double last_abs = 0.0;
   hipEvent_t ev[8] = {};
+  // the LM sub-iteration captured once per problem (a GN iteration is
+  // launched directly: measured on the ladybug workload, replaying it as a
+  // graph was 2 % slower than its queued launches)
   hipGraphExec_t lm_graph = nullptr;
   bool lm_graph_tried = false;
   std::unique_ptr<DevBuf> pri_mem;  // unary priors (nsn_set_priors)
@@ -138,7 +141,7 @@ struct nsn_problem {
   void* Uoff = nullptr;
 
   ~nsn_problem() {
-    if (lm_graph) (void)hipGraphExecDestroy(lm_graph);
+    drop_graphs();
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
     if (hctl) (void)hipHostFree(hctl);
@@ -146,6 +149,14 @@ struct nsn_problem {
   }
 
   size_t esz() const { return prec == NSN_MIXED ? sizeof(float) : sizeof(double); }
+
+  // the captured graph holds the launch sequence and its arguments: it is
+  // dropped when either changes
+  void drop_graphs() {
+    if (lm_graph) (void)hipGraphExecDestroy(lm_graph);
+    lm_graph = nullptr;
+    lm_graph_tried = false;
+  }
 };
 
 namespace {
@@ -990,11 +1001,7 @@ extern "C" int nsn_set_priors(nsn_problem* h, int n_cam, const int* cam_ids, con
     v.pt_prior = di; v.pt_prior_p = dp; v.pt_prior_w = dw;
   }
   h->pri_mem = std::move(m);
-  if (h->lm_graph) {  // the captured LM sub-iteration holds the old view
-    (void)hipGraphExecDestroy(h->lm_graph);
-    h->lm_graph = nullptr;
-  }
-  h->lm_graph_tried = false;
+  h->drop_graphs();  // the captured iterations hold the old view
   h->linearized = false;
   h->stepped = false;
   return NSN_OK;
@@ -1003,12 +1010,7 @@ extern "C" int nsn_set_priors(nsn_problem* h, int n_cam, const int* cam_ids, con
 extern "C" int nsn_set_refinement(nsn_problem* h, int steps) {
   if (!h) return fail(NSN_EINVAL, "null argument");
   if (steps < 0 || steps > 8) return fail(NSN_EINVAL, "nsn_set_refinement: steps must be in 0..8, got %d", steps);
-  if (steps != h->ir_steps && h->lm_graph) {
-    // the captured LM sub-iteration holds the old sequence
-    (void)hipGraphExecDestroy(h->lm_graph);
-    h->lm_graph = nullptr;
-    h->lm_graph_tried = false;
-  }
+  if (steps != h->ir_steps) h->drop_graphs();  // the captured iterations hold the old sequence
   h->ir_steps = steps;
   return NSN_OK;
 }
@@ -1278,6 +1280,25 @@ extern "C" int nsn_adjoint_noalloc(nsn_problem* h, const double* g_cams, const d
                               : adjoint_t<double>(h, b, g_cams, g_pts, hessian_steps, dL_dobs, dL_dw);
 }
 
+// captures one launch sequence on the problem's stream into an executable
+// graph; null when capture is switched off (NSN_NO_GRAPH) or fails, and the
+// caller then launches the sequence directly
+static hipGraphExec_t capture(nsn_problem* h, void (*seq)(nsn_problem*)) {
+  static const bool no_graph = std::getenv("NSN_NO_GRAPH") != nullptr;
+  if (no_graph) return nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t g = nullptr;
+  if (hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+    seq(h);
+    if (hipStreamEndCapture(h->st, &g) == hipSuccess && g) {
+      if (hipGraphInstantiate(&exec, g, nullptr, nullptr, 0) != hipSuccess) exec = nullptr;
+      (void)hipGraphDestroy(g);
+    }
+  }
+  (void)hipGetLastError();
+  return exec;
+}
+
 template <typename T>
 static int run_lm(nsn_problem* h, const nsn_lm_settings* s, nsn_lm_result* res, nsn_log_entry* log, int log_cap) {
   h->imported = false;  // the drivers linearise from the edges
@@ -1291,18 +1312,9 @@ static int run_lm(nsn_problem* h, const nsn_lm_settings* s, nsn_lm_result* res, 
   *h->hctl = c;
   int rc = push_ctl(h);
   if (rc) return rc;
-  static const bool no_graph = std::getenv("NSN_NO_GRAPH") != nullptr;
-  if (!h->lm_graph_tried && !no_graph) {
+  if (!h->lm_graph_tried) {
     h->lm_graph_tried = true;
-    hipGraph_t g = nullptr;
-    if (hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal) == hipSuccess) {
-      seq_lm_subiter<T>(h);
-      if (hipStreamEndCapture(h->st, &g) == hipSuccess && g) {
-        if (hipGraphInstantiate(&h->lm_graph, g, nullptr, nullptr, 0) != hipSuccess) h->lm_graph = nullptr;
-        (void)hipGraphDestroy(g);
-      }
-    }
-    (void)hipGetLastError();
+    h->lm_graph = capture(h, seq_lm_subiter<T>);
   }
   const long long max_sub = (long long)std::max(1, s->maxNumIt) * std::max(1, s->maxNumSubIt) + 4;
   auto t0 = std::chrono::steady_clock::now();
