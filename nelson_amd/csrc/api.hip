@@ -105,7 +105,7 @@ struct nsn_problem {
   int n_s_parts = 0;
   int *flags = nullptr, *counters = nullptr;
   int nflags = 0, NR = 0;
-  double* parts = nullptr;  // triangular-solve destination sums, NR x TB
+  double* parts = nullptr;  // triangular-solve destination sums, two of NR x TB
   // edge permutations and observation arrays (nsn_set_observations)
   std::vector<int> pe_order, ce_order;
   double *d_pe_obs = nullptr, *d_pe_w = nullptr, *d_ce_obs = nullptr, *d_ce_w = nullptr;
@@ -184,16 +184,15 @@ template <typename T>
 void seq_schur_solve(nsn_problem* h, int gate, const double* rhsU, const double* rhsV, double* xU, double* xV,
                      int add, const double* bden, double* accU = nullptr) {
   hipStream_t st = h->st;
-  const int NR = h->NR;
-  // flags of the forward and of the backward solve, cleared by the reduction
-  int* xready_f = h->flags + h->tv.ntiles;
-  int* xready_b = h->flags + h->tv.ntiles + 2 * NR + 2 * h->tv.NT;
-  launch_ir_reduce<T>(h->view, (const T*)h->W, (const T*)h->Vinv, rhsU, rhsV, h->rS, xready_f, xready_b, 2 * NR,
-                      h->ctl, gate, st);
-  launch_trsv<T>(h->tv, (const T*)h->tiles, (const T*)h->inv, false, h->rS, h->zS, h->parts, xready_f,
-                 xready_f + NR, h->ctl, gate, kTrsvMaxWG, st);
-  launch_trsv<T>(h->tv, (const T*)h->tiles, (const T*)h->inv, true, h->zS, xU, h->parts, xready_b, xready_b + NR,
-                 h->ctl, gate, kTrsvMaxWG, st);
+  // the two solves' solutions and destination sums start unset (reduction)
+  double* far_f = h->parts;
+  double* far_b = h->parts + (size_t)h->NR * kTB;
+  double* const unset[4] = {h->zS, xU, far_f, far_b};
+  launch_ir_reduce<T>(h->view, (const T*)h->W, (const T*)h->Vinv, rhsU, rhsV, h->rS, unset, h->tv.n, h->ctl, gate,
+                      st);
+  launch_trsv<T>(h->tv, (const T*)h->tiles, (const T*)h->inv, false, h->rS, h->zS, far_f, h->ctl, gate, kTrsvMaxWG,
+                 st);
+  launch_trsv<T>(h->tv, (const T*)h->tiles, (const T*)h->inv, true, h->zS, xU, far_b, h->ctl, gate, kTrsvMaxWG, st);
   launch_backsub<T>(h->view, (const T*)h->W, (const T*)h->Vinv, (const T*)h->bV, xU, xV, h->scr, h->ctl, gate, st,
                     rhsV, add, bden, accU, accU ? h->tv.n : 0);
 }
@@ -214,15 +213,16 @@ void seq_schur(nsn_problem* h, int gate, int checks, bool timed) {
                     (T*)h->tiles, h->ctl, gate, st);
   launch_add_uoff<T>(h->tv, h->n_uoff, h->uoff_i, h->uoff_j, (const T*)h->Uoff, (T*)h->tiles, h->ctl, gate, st);
   if (timed) (void)hipEventRecord(h->ev[2], st);
-  launch_chol_reset<T>(h->flags, h->nflags, h->counters, h->ctl, gate, st);
+  // also sets the backward solve's solution and destination sums unset
+  launch_chol_reset<T>(h->flags, h->nflags, h->counters, h->xU, h->parts, h->tv.n, h->ctl, gate, st);
   {
-    int* pre_sub = h->flags + h->tv.ntiles + 2 * h->NR;
+    int* pre_sub = h->flags + h->tv.ntiles;
     launch_chol<T>(h->tv, (T*)h->tiles, (T*)h->inv, (T*)h->presub, h->chol_tasks, h->n_chol_tasks, h->flags,
                    pre_sub, pre_sub + h->tv.NT, h->counters, h->ctl, gate, kCholMaxWG, st);
   }
   if (timed) (void)hipEventRecord(h->ev[3], st);
-  launch_trsv<T>(h->tv, (const T*)h->tiles, (const T*)h->inv, true, nullptr, h->xU, h->parts,
-                 h->flags + h->tv.ntiles, h->flags + h->tv.ntiles + h->NR, h->ctl, gate, kTrsvMaxWG, st);
+  launch_trsv<T>(h->tv, (const T*)h->tiles, (const T*)h->inv, true, nullptr, h->xU, h->parts, h->ctl, gate,
+                 kTrsvMaxWG, st);
   if (timed) (void)hipEventRecord(h->ev[4], st);
   launch_backsub<T>(h->view, (const T*)h->W, (const T*)h->Vinv, (const T*)h->bV, h->xU, h->xV, h->scr, h->ctl,
                     gate, st);
@@ -378,7 +378,7 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
   // flags: ready (ntiles), the triangular solves' xready / tready (NR each),
   // pre_sub / pre_diag (NT each), the backward refinement solve's xready /
   // tready (NR each)
-  h->nflags = tv.ntiles + 2 * h->NR + 2 * tv.NT + 2 * h->NR;
+  h->nflags = tv.ntiles + 2 * tv.NT;  // tile flags, pre_sub, pre_diag
   int n_tasks = 0;
   chol_task_list(tv.NT, 0, nullptr, &n_tasks);  // the count does not depend on the look-ahead
   const int ahead = chol_lookahead(std::min(n_tasks, kCholMaxWG - 1));
@@ -429,7 +429,7 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
   h->inv = a_T((size_t)tv.NT * kTB * kTB);
   h->presub = a_T((size_t)tv.NT * kTB * kTB);
   h->xU = a_dbl(6 * (size_t)NU);
-  h->parts = a_dbl((size_t)h->NR * kTB);  // the triangular solves' destination sums
+  h->parts = a_dbl(2 * (size_t)h->NR * kTB);  // the triangular solves' destination sums
   h->rU = a_dbl(6 * (size_t)NU);
   h->rV = a_dbl(3 * (size_t)NV);
   h->bU64 = a_dbl(6 * (size_t)NU);

@@ -532,15 +532,19 @@ __global__ __launch_bounds__(256) void k_ir_res(BAView v, const double* __restri
 template <typename T>
 __global__ __launch_bounds__(256) void k_ir_reduce(BAView v, const T* __restrict__ W, const T* __restrict__ Vinv,
                                                    const double* __restrict__ rU, const double* __restrict__ rV,
-                                                   double* __restrict__ rS, int* clr0, int* clr1, int nclr, DevCtl* ctl,
-                                                   int gate) {
+                                                   double* __restrict__ rS, double* u0, double* u1, double* u2,
+                                                   double* u3, int nunset, DevCtl* ctl, int gate) {
   if (!gate_open(ctl, gate)) return;
   {
-    // the flags of the two triangular solves that follow (nclr <= 256 NU)
+    // the solutions and destination sums of the two triangular solves that
+    // follow start unset (nunset <= 256 NU)
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g < nclr) {
-      clr0[g] = 0;
-      clr1[g] = 0;
+    if (g < nunset) {
+      const double u = __builtin_bit_cast(double, kUnsetBits);
+      u0[g] = u;
+      u1[g] = u;
+      u2[g] = u;
+      u3[g] = u;
     }
   }
   __shared__ double red[4][6];
@@ -906,10 +910,10 @@ void launch_ir_residual(const BAView& v, const double* xU, const double* xV, dou
 
 template <typename T>
 void launch_ir_reduce(const BAView& v, const T* W, const T* Vinv, const double* rU, const double* rV, double* rS,
-                      int* clr0, int* clr1, int nclr, DevCtl* ctl, int gate, hipStream_t st) {
+                      double* const unset[4], int nunset, DevCtl* ctl, int gate, hipStream_t st) {
   if (v.NU == 0) return;
-  hipLaunchKernelGGL((k_ir_reduce<T>), dim3(v.NU), dim3(256), 0, st, v, W, Vinv, rU, rV, rS, clr0, clr1, nclr, ctl,
-                     gate);
+  hipLaunchKernelGGL((k_ir_reduce<T>), dim3(v.NU), dim3(256), 0, st, v, W, Vinv, rU, rV, rS, unset[0], unset[1],
+                     unset[2], unset[3], nunset, ctl, gate);
 }
 
 void launch_adj_eps(const double* lu, long long nu, const double* lv, long long nv, double* eps, hipStream_t st) {
@@ -1042,7 +1046,7 @@ void launch_zero(T* p, long long n, hipStream_t st) {
                                   const Scratch&, DevCtl*, int, hipStream_t, const double*, int,             \
                                   const double*, double*, int);                                              \
   template void launch_ir_reduce<T>(const BAView&, const T*, const T*, const double*, const double*, double*, \
-                                    int*, int*, int, DevCtl*, int, hipStream_t);                             \
+                                    double* const*, int, DevCtl*, int, hipStream_t);                         \
   template void launch_xu_stats<T>(int, const double*, const T*, const Scratch&, DevCtl*, int, hipStream_t); \
   template void launch_zero<T>(T*, long long, hipStream_t);                                                 \
   template void launch_add_uoff<T>(const TileView&, int, const int*, const int*, const T*, T*, DevCtl*, int,  \

@@ -270,105 +270,136 @@ __global__ __launch_bounds__(256) void k_chol(TileView tv, T* __restrict__ tiles
 //   v = rhs_k - (far_k + sum_{d=1..kNear} N_{k,d} x_{k-d})
 //   x_k = Linv_kk^(T) v
 // with the kNear previous solutions kept in its LDS (N_{k,d} = L_{k,k-d}
-// forward, L_{k+d,k}^T backward) and the tiles of the next step prefetched
-// into registers, and publishes x_k (xready[k]).  Every other workgroup owns
-// destination blocks (solve index kNear+1+w, +G, ...): it takes the
+// forward, L_{k+d,k}^T backward) and the tiles, rhs_k and far_k of the next
+// step loaded into registers during the current one (far_k speculatively:
+// an entry still unset is polled again when it is needed).  Every other workgroup
+// owns destination blocks (solve index kNear+1+w, +G, ...): it takes the
 // solutions in solve order and accumulates, in its LDS and in source order
-// (deterministic), far_i = sum of N x_k over the sources more than kNear steps
-// before i, publishing far_i (tready[i]) after its last far source.  The
-// chain waits only for destination sums whose sources it has published, and
-// the destination workgroups only for published solutions: no cycle.
+// (deterministic), far_i = sum of N x_k over the sources more than kNear
+// steps before i, and publishes far_i after its last far source.  The chain
+// waits only for destination sums whose sources it has published, and the
+// destination workgroups only for published solutions: no cycle.
+//
+// Publication carries no flags or fences: sol and far hold kUnsetBits (a
+// signalling NaN, which arithmetic never produces) before the launch, every
+// entry is written once with an agent-scope atomic store, and a reader polls
+// the entries it needs with agent-scope atomic loads until none is unset.
+// Each entry is its own ready flag, so a writer never waits for its stores
+// to drain before signalling, and a reader gets the data with the poll.
 // rhs == nullptr takes the right-hand side from row n of the factor (the
 // forward solve of bS that the factorisation leaves there).
 constexpr int kNear = 2;
 constexpr int kMaxOwned = 16;  // destination blocks per workgroup
 
+__device__ __forceinline__ bool is_unset(double v) { return __builtin_bit_cast(unsigned long long, v) == kUnsetBits; }
+
+__device__ __forceinline__ double load_pub(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void store_pub(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// polls one published entry (per lane) until it is set or the wait gave up
+// (then 0 and ctl->wait_timeout raised); v: a value already loaded from p
+__device__ __forceinline__ double wait_pub(const double* p, double v, DevCtl* ctl) {
+  int it = 0;
+  while (is_unset(v)) {
+    __builtin_amdgcn_s_sleep(1);
+    if (((++it) & 255) == 0) {
+      if (__hip_atomic_load(&ctl->wait_timeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return 0.0;
+      if (it > kWaitLimit) {
+        atomicAdd(&ctl->wait_timeout, 1);
+        return 0.0;
+      }
+    }
+    v = load_pub(p);
+  }
+  return v;
+}
+
 template <typename T, int TB, bool BWD>
 __global__ __launch_bounds__(256) void k_trsv(TileView tv, const T* __restrict__ tiles, const T* __restrict__ inv,
-                                              const double* rhs, double* sol, double* __restrict__ far,
-                                              int* xready, int* tready, DevCtl* ctl, int gate) {
+                                              const double* rhs, double* sol, double* far, DevCtl* ctl, int gate) {
   static_assert(TB == 64, "64-row tiles");
   if (!chol_gate(ctl, gate)) return;
   typedef TileGemv<T, TB, BWD> Gemv;
-  __shared__ double red[4 * TB];
-  __shared__ double xs[TB];
-  __shared__ double hs[TB];
-  __shared__ double xr[kNear][TB];
+  __shared__ __align__(16) double xs[2][TB];
+  __shared__ __align__(16) double xr[kNear][TB];
   __shared__ double acc[kMaxOwned][TB];
   const int n = tv.n, NT = tv.NT;
   const int NR = (n + TB - 1) / TB;
   const size_t T2 = (size_t)TB * TB;
-  const int tid = threadIdx.x;
   const int step = BWD ? -1 : 1;  // solve order
   auto blk = [&](int s) { return BWD ? NR - 1 - s : s; };
   auto valid = [&](int b) { return b >= 0 && b < NR; };
   // the tile applied to x_k for destination i (L_ik forward, L_ki^T backward)
   auto ntile = [&](int i, int k) { return tiles + (size_t)(BWD ? tile_index(k, i, NT) : tile_index(i, k, NT)) * T2; };
+  const bool own = Gemv::owner();
+  const int i0 = Gemv::index(0), i1 = Gemv::index(1);
   if (blockIdx.x == 0) {
     // ---- the chain ----
+    // the loads of a step's tiles are issued as soon as the registers they
+    // replace are consumed (issuing 96 KB takes about a microsecond on one
+    // CU: spread over the step it overlaps the rest of the step's work)
     Gemv gd, gn[kNear];
-    double r0 = 0.0;
-    auto prefetch = [&](int s) {
+    double r0 = 0.0, r1 = 0.0, f0 = 0.0, f1 = 0.0;
+    auto rhs_at = [&](int row) {
+      return rhs ? rhs[row] : (double)tiles[(size_t)tile_index(n / TB, row / TB, NT) * T2 + (row % TB) * TB + (n % TB)];
+    };
+    auto load_near = [&](int s) {
       const int k = blk(s);
-      gd.load(inv + (size_t)k * T2);
 #pragma unroll
       for (int d = 1; d <= kNear; d++)
         if (valid(k - d * step)) gn[d - 1].load(ntile(k, k - d * step));
-      r0 = 0.0;
-      if (tid < TB && k * TB + tid < n)
-        r0 = rhs ? rhs[k * TB + tid] : (double)tiles[(size_t)tile_index(n / TB, k, NT) * T2 + tid * TB + (n % TB)];
     };
-    prefetch(0);
+    auto load_vec = [&](int s) {
+      const int k = blk(s);
+      if (own) {
+        const int a = k * TB + i0, b = k * TB + i1;
+        f0 = (s > kNear && a < n) ? load_pub(far + a) : 0.0;
+        f1 = (s > kNear && b < n) ? load_pub(far + b) : 0.0;
+        r0 = a < n ? rhs_at(a) : 0.0;
+        r1 = b < n ? rhs_at(b) : 0.0;
+      }
+    };
+    load_vec(0);
+    gd.load(inv + (size_t)blk(0) * T2);
+    load_near(0);
     for (int s = 0; s < NR; s++) {
       const int k = blk(s);
-      const bool has_far = s > kNear;
-      // one thread waits and acquires; the barrier orders the rest of the
-      // workgroup after it (the other waves keep their prefetches in flight)
-      if (tid == 0 && has_far) {
-        wait_flag(tready + k, ctl);
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-      }
-      __syncthreads();
-      double pk = 0.0;
-      if (tid < TB && has_far) pk = far[(size_t)k * TB + tid];
       // the near blocks' contributions, one reduction for all of them
-      double h = 0.0;
-      {
+      double h0 = 0.0, h1 = 0.0;
 #pragma unroll
-        for (int d = 1; d <= kNear; d++) {
-          if (d > s) continue;
-          h += gn[d - 1].partial(xr[(s - d) % kNear]);
-        }
-        if (BWD) {
-          h += __shfl_xor(h, 1);
-          h += __shfl_xor(h, 2);
-        } else {
-          red[(tid >> 6) * TB + (tid & 63)] = h;
-          __syncthreads();
-          h = 0.0;
-          if (tid < TB) h = ((red[tid] + red[TB + tid]) + red[2 * TB + tid]) + red[3 * TB + tid];
-        }
+      for (int d = 1; d <= kNear; d++)
+        if (d <= s) gn[d - 1].partial(xr[(s - d) % kNear], h0, h1);
+      if (s + 1 < NR) load_near(s + 1);
+      Gemv::reduce(h0, h1);
+      if (own) {
+        const int a = k * TB + i0, b = k * TB + i1;
+        if (a < n) f0 = wait_pub(far + a, f0, ctl);
+        if (b < n) f1 = wait_pub(far + b, f1, ctl);
+        xs[0][i0] = a < n ? r0 - (f0 + h0) : 0.0;
+        xs[0][i1] = b < n ? r1 - (f1 + h1) : 0.0;
       }
-      if (Gemv::owner()) hs[Gemv::index()] = h;
       __syncthreads();
-      // publish the previous step's solution here: its stores (and the tile
-      // prefetch, consumed above) are complete by now, so the release does
-      // not wait on this step's path; the destinations need x only two
-      // steps later
-      if (s > 0) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-      if (tid < TB) xs[tid] = (k * TB + tid < n) ? r0 - (pk + hs[tid]) : 0.0;
-      __syncthreads();
-      if (s > 0 && tid == 0) __hip_atomic_store(xready + blk(s - 1), 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      const double x = gd.apply(xs, red);
-      if (Gemv::owner()) {
-        const int e = Gemv::index();
-        const bool in = k * TB + e < n;
-        if (in) sol[k * TB + e] = x;
-        xr[s % kNear][e] = in ? x : 0.0;
+      double x0 = 0.0, x1 = 0.0;
+      gd.partial(xs[0], x0, x1);
+      if (s + 1 < NR) gd.load(inv + (size_t)blk(s + 1) * T2);
+      Gemv::reduce(x0, x1);
+      if (own) {
+        const int a = k * TB + i0, b = k * TB + i1;
+        if (a < n) store_pub(sol + a, x0);
+        if (b < n) store_pub(sol + b, x1);
+        xr[s % kNear][i0] = a < n ? x0 : 0.0;
+        xr[s % kNear][i1] = b < n ? x1 : 0.0;
       }
-      if (s + 1 < NR) prefetch(s + 1);
+      // x_k complete in the LDS before the next step reads it (and xs is
+      // read by every wave before it is written again)
+      __syncthreads();
+      if (s + 1 < NR) load_vec(s + 1);
     }
-    publish_flag(xready + blk(NR - 1));
     return;
   }
   // ---- destination sums ----
@@ -376,23 +407,29 @@ __global__ __launch_bounds__(256) void k_trsv(TileView tv, const T* __restrict__
   const int first = kNear + 1 + w;  // solve index of the first owned destination
   if (first >= NR) return;
   const int nown = (NR - 1 - first) / G + 1;  // <= kMaxOwned (launch)
-  for (int m = 0; m < nown; m++)
-    if (tid < TB) acc[m][tid] = 0.0;
+  if (own)
+    for (int m = 0; m < nown; m++) acc[m][i0] = acc[m][i1] = 0.0;
   Gemv gg;
   int m0 = 0;  // first owned destination still open
   gg.load(ntile(blk(first), blk(0)));
   for (int s = 0; s < NR && m0 < nown; s++) {  // source steps
     const int k = blk(s);
-    if (tid == 0) {
-      wait_flag(xready + k, ctl);
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    // wave 0 polls the source's solution into the LDS (double-buffered by
+    // source parity: the barrier of source s orders every read of the buffer
+    // of source s - 2 before this write)
+    if (threadIdx.x < TB) {
+      const int row = k * TB + threadIdx.x;
+      xs[s & 1][threadIdx.x] = row < n ? wait_pub(sol + row, load_pub(sol + row), ctl) : 0.0;
     }
     __syncthreads();
-    if (tid < TB) xs[tid] = (k * TB + tid < n) ? sol[k * TB + tid] : 0.0;
-    __syncthreads();
     for (int m = m0; m < nown; m++) {
-      const double v = gg.apply(xs, red);
-      if (Gemv::owner()) acc[m][Gemv::index()] += v;
+      double v0 = 0.0, v1 = 0.0;
+      gg.partial(xs[s & 1], v0, v1);
+      Gemv::reduce(v0, v1);
+      if (own) {
+        acc[m][i0] += v0;
+        acc[m][i1] += v1;
+      }
       // next product: the following destination, or the first open one of the next source
       int nm = m + 1, ns = s;
       if (nm == nown) {
@@ -404,25 +441,36 @@ __global__ __launch_bounds__(256) void k_trsv(TileView tv, const T* __restrict__
     // destination m0 is complete after its last far source
     if (first + G * m0 - kNear - 1 == s) {
       const int d = blk(first + G * m0);
-      if (Gemv::owner()) far[(size_t)d * TB + Gemv::index()] = acc[m0][Gemv::index()];
-      publish_flag(tready + d);
+      if (own) {
+        if (d * TB + i0 < n) store_pub(far + d * TB + i0, acc[m0][i0]);
+        if (d * TB + i1 < n) store_pub(far + d * TB + i1, acc[m0][i1]);
+      }
       m0++;
     }
   }
 }
 
 template <typename T>
-__global__ void k_chol_reset(int* flags, int nflags, int* counters, DevCtl* ctl, int gate) {
+__global__ void k_chol_reset(int* flags, int nflags, int* counters, double* unset0, double* unset1, int nunset,
+                             DevCtl* ctl, int gate) {
   if (!chol_gate(ctl, gate)) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < nflags) flags[i] = 0;
   if (i < 4) counters[i] = 0;
+  if (i < nunset) {
+    const double u = __builtin_bit_cast(double, kUnsetBits);
+    unset0[i] = u;
+    unset1[i] = u;
+  }
 }
 
 template <typename T>
-void launch_chol_reset(int* flags, int nflags, int* counters, DevCtl* ctl, int gate, hipStream_t st) {
-  const int n = nflags > 4 ? nflags : 4;
-  hipLaunchKernelGGL((k_chol_reset<T>), dim3((n + 255) / 256), dim3(256), 0, st, flags, nflags, counters, ctl, gate);
+void launch_chol_reset(int* flags, int nflags, int* counters, double* unset0, double* unset1, int nunset,
+                       DevCtl* ctl, int gate, hipStream_t st) {
+  int n = nflags > 4 ? nflags : 4;
+  if (nunset > n) n = nunset;
+  hipLaunchKernelGGL((k_chol_reset<T>), dim3((n + 255) / 256), dim3(256), 0, st, flags, nflags, counters, unset0,
+                     unset1, nunset, ctl, gate);
 }
 
 template <typename T>
@@ -435,7 +483,7 @@ void launch_chol(const TileView& tv, T* tiles, T* inv, T* presub, const int* tas
 
 template <typename T>
 void launch_trsv(const TileView& tv, const T* tiles, const T* inv, bool backward, const double* rhs, double* sol,
-                 double* far, int* xready, int* tready, DevCtl* ctl, int gate, int max_wg, hipStream_t st) {
+                 double* far, DevCtl* ctl, int gate, int max_wg, hipStream_t st) {
   const int NR = (tv.n + 63) / 64;
   if (NR == 0) return;
   // the chain plus G workgroups owning the nd destinations with far sources
@@ -444,22 +492,20 @@ void launch_trsv(const TileView& tv, const T* tiles, const T* inv, bool backward
   if (G > 0 && (nd + G - 1) / G > kMaxOwned) G = (nd + kMaxOwned - 1) / kMaxOwned;
   const int g = 1 + G;
   if (backward)
-    hipLaunchKernelGGL((k_trsv<T, 64, true>), dim3(g), dim3(256), 0, st, tv, tiles, inv, rhs, sol, far, xready,
-                       tready, ctl, gate);
+    hipLaunchKernelGGL((k_trsv<T, 64, true>), dim3(g), dim3(256), 0, st, tv, tiles, inv, rhs, sol, far, ctl, gate);
   else
-    hipLaunchKernelGGL((k_trsv<T, 64, false>), dim3(g), dim3(256), 0, st, tv, tiles, inv, rhs, sol, far, xready,
-                       tready, ctl, gate);
+    hipLaunchKernelGGL((k_trsv<T, 64, false>), dim3(g), dim3(256), 0, st, tv, tiles, inv, rhs, sol, far, ctl, gate);
 }
 
-template void launch_chol_reset<double>(int*, int, int*, DevCtl*, int, hipStream_t);
-template void launch_chol_reset<float>(int*, int, int*, DevCtl*, int, hipStream_t);
+template void launch_chol_reset<double>(int*, int, int*, double*, double*, int, DevCtl*, int, hipStream_t);
+template void launch_chol_reset<float>(int*, int, int*, double*, double*, int, DevCtl*, int, hipStream_t);
 template void launch_chol<double>(const TileView&, double*, double*, double*, const int*, int, int*, int*, int*,
                                   int*, DevCtl*, int, int, hipStream_t);
 template void launch_chol<float>(const TileView&, float*, float*, float*, const int*, int, int*, int*, int*, int*,
                                  DevCtl*, int, int, hipStream_t);
 template void launch_trsv<double>(const TileView&, const double*, const double*, bool, const double*, double*,
-                                  double*, int*, int*, DevCtl*, int, int, hipStream_t);
+                                  double*, DevCtl*, int, int, hipStream_t);
 template void launch_trsv<float>(const TileView&, const float*, const float*, bool, const double*, double*, double*,
-                                 int*, int*, DevCtl*, int, int, hipStream_t);
+                                 DevCtl*, int, int, hipStream_t);
 
 }  // namespace nsn

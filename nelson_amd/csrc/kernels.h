@@ -119,10 +119,12 @@ void launch_backsub(const BAView& v, const T* W, const T* Vinv, const T* bV, con
 // rU/rV = -b - (H + D) x from the edges, and the FP64 gradient bU64/bV64
 void launch_ir_residual(const BAView& v, const double* xU, const double* xV, double* rU, double* rV,
                         double* bU64, double* bV64, DevCtl* ctl, int gate, hipStream_t st);
-// rS = rU - W V^-1 rV (Schur right-hand side of a correction)
+// rS = rU - W V^-1 rV (Schur right-hand side of a correction); sets
+// unset[0..3][0..nunset) to kUnsetBits (the solutions and destination sums
+// of the two triangular solves that follow; nunset <= 256 NU)
 template <typename T>
 void launch_ir_reduce(const BAView& v, const T* W, const T* Vinv, const double* rU, const double* rV, double* rS,
-                      int* clr0, int* clr1, int nclr, DevCtl* ctl, int gate, hipStream_t st);
+                      double* const unset[4], int nunset, DevCtl* ctl, int gate, hipStream_t st);
 // x += z
 void launch_add(double* x, const double* z, long long n, DevCtl* ctl, int gate, hipStream_t st);
 // out = a x + b y (out may alias x or y)
@@ -161,8 +163,15 @@ template <typename T>
 void launch_zero(T* p, long long n, hipStream_t st);
 
 // ---- dense tile Cholesky and triangular solve (chol_kernels.hip) --------
+// bit pattern of a not yet published entry of a triangular solve's solution
+// and destination sums: a signalling NaN, which arithmetic never produces
+// (its NaN results are quiet)
+constexpr unsigned long long kUnsetBits = 0x7FF4A5A5A5A5A5A5ull;
+// zeroes the factorisation's flags and counters and sets unset0/unset1[0..nunset)
+// to kUnsetBits (the solution and destination sums of the solve that follows)
 template <typename T>
-void launch_chol_reset(int* flags, int nflags, int* counters, DevCtl* ctl, int gate, hipStream_t st);
+void launch_chol_reset(int* flags, int nflags, int* counters, double* unset0, double* unset1, int nunset,
+                       DevCtl* ctl, int gate, hipStream_t st);
 // factor in place; inv[j] receives L_jj^-1 (TB x TB, column-major); presub:
 // NT scratch tiles.  tasks: the worker task list (3 ints per task: kind, i,
 // j; chol_task_list); ready: ntiles flags, pre_sub / pre_diag: NT flags
@@ -171,10 +180,11 @@ template <typename T>
 void launch_chol(const TileView& tv, T* tiles, T* inv, T* presub, const int* tasks, int ntasks, int* ready,
                  int* pre_sub, int* pre_diag, int* counter, DevCtl* ctl, int gate, int max_wg, hipStream_t st);
 // L^T sol = rhs (backward) or L sol = rhs (forward) over the first n rows;
-// rhs == nullptr: row n of the factor.  xready, tready: NR flags each,
-// zeroed before the launch; far: NR x 64 doubles of scratch.
+// rhs == nullptr: row n of the factor.  far: n doubles of scratch; sol[0..n)
+// and far[0..n) hold kUnsetBits before the launch (launch_chol_reset,
+// launch_ir_reduce); rhs must not alias sol.
 template <typename T>
 void launch_trsv(const TileView& tv, const T* tiles, const T* inv, bool backward, const double* rhs, double* sol,
-                 double* far, int* xready, int* tready, DevCtl* ctl, int gate, int max_wg, hipStream_t st);
+                 double* far, DevCtl* ctl, int gate, int max_wg, hipStream_t st);
 
 }  // namespace nsn

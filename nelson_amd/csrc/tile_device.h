@@ -411,55 +411,80 @@ __device__ __forceinline__ void global_to_lds(T* R, const T* Cg) {
   for (int q = 0; q < NQ; q++) R[threadIdx.x + 256 * q] = v[q];
 }
 
-// 64 x 64 tile times a 64-vector in LDS, FP64 accumulation, with the tile
-// prefetched into registers (16 entries per thread) before the vector is
-// known.  TRANS: out[c] = sum_r A[r][c] x[r], thread (c = tid/4, part
-// tid%4) holds 16 rows of column c: for FP64 tiles rows part + 4q, q =
-// 0..15 (the four threads of a column read 32 contiguous bytes per load;
-// 16 contiguous rows per thread measured 25 % slower), for FP32 tiles rows
-// 16*part .. +15 (wide loads); combined by two lane exchanges.  Otherwise:
-// out[r] = sum_c A[r][c] x[c], thread (r = tid%64, wave w) holds columns
-// 16*w .. +15 of row r (coalesced loads); combined across the four waves
-// through LDS.  Both sums run in a fixed order.
+// 64 x 64 column-major tile (entry (r, c) at A[c * TB + r]) times a
+// 64-vector in LDS with FP64 accumulation; the tile is loaded into registers
+// (16 entries per thread, as pairs of adjacent entries of a column) before
+// the vector is known.  Wave w owns 16 outputs and reduces them within the
+// wave (three lane exchanges), so no barrier separates the partial sums from
+// the result:
+//   TRANS:  out[c] = sum_r A(r, c) x[r] for c = 16w + l/8 and c + 8; lane l
+//           holds the rows 2(l%8) + 16q, +1, q = 0..3, of both columns;
+//   !TRANS: out[r] = sum_c A(r, c) x[c] for r in 16w .. 16w+15; lane l holds
+//           the row pair 16w + 2(l%8), +1 in the columns l/8 + 8q, q = 0..7.
+// In both, eight lanes read 16 adjacent entries of a column per load.
+// reduce() leaves the two sums of a lane group in every lane of the group;
+// the owner lanes (TRANS: l%8 == 0; otherwise l < 8) hand out outputs
+// index(0) and index(1).  All sums run in a fixed order.
 template <typename T, int TB, bool TRANS>
 struct TileGemv {
+  static_assert(TB == 64, "64-row tiles");
+  typedef T T2 __attribute__((ext_vector_type(2)));
   T a[16];
-  // the index into x of this thread's entry q
-  static __device__ __forceinline__ int xi(int q) {
-    if (!TRANS) return (int)(threadIdx.x >> 6) * 16 + q;
-    return sizeof(T) == 8 ? (int)(threadIdx.x & 3) + 4 * q : (int)(threadIdx.x & 3) * 16 + q;
-  }
+  static __device__ __forceinline__ int lane() { return (int)(threadIdx.x & 63); }
+  static __device__ __forceinline__ int wave() { return (int)(threadIdx.x >> 6); }
   __device__ __forceinline__ void load(const T* A) {
-    const int tid = threadIdx.x;
-#pragma unroll
-    for (int q = 0; q < 16; q++) a[q] = TRANS ? A[(tid >> 2) * TB + xi(q)] : A[xi(q) * TB + (tid & 63)];
-  }
-  // this thread's partial sum of the product with x (LDS)
-  __device__ __forceinline__ double partial(const double* x) const {
-    double s = 0.0;
-#pragma unroll
-    for (int q = 0; q < 16; q++) s += (double)a[q] * x[xi(q)];
-    return s;
-  }
-  // returns the full sum in every thread whose index() is valid (TRANS: all
-  // four threads of a column; otherwise: wave 0); red: 4 x TB doubles of LDS
-  __device__ __forceinline__ double apply(const double* x, double* red) const {
-    const int tid = threadIdx.x;
-    double s = partial(x);
+    const int l = lane(), w = wave();
     if (TRANS) {
-      s += __shfl_xor(s, 1);
-      s += __shfl_xor(s, 2);
-      return s;
+      const T* p = A + (size_t)(16 * w + (l >> 3)) * TB + 2 * (l & 7);
+#pragma unroll
+      for (int h = 0; h < 2; h++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const T2 v = *(const T2*)(p + (size_t)8 * h * TB + 16 * q);
+          a[8 * h + 2 * q] = v.x;
+          a[8 * h + 2 * q + 1] = v.y;
+        }
+    } else {
+      const T* p = A + (size_t)(l >> 3) * TB + 16 * w + 2 * (l & 7);
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        const T2 v = *(const T2*)(p + (size_t)8 * q * TB);
+        a[2 * q] = v.x;
+        a[2 * q + 1] = v.y;
+      }
     }
-    red[(tid >> 6) * TB + (tid & 63)] = s;
-    __syncthreads();
-    s = 0.0;
-    if (tid < TB) s = ((red[tid] + red[TB + tid]) + red[2 * TB + tid]) + red[3 * TB + tid];
-    __syncthreads();
-    return s;
   }
-  static __device__ __forceinline__ int index() { return TRANS ? (threadIdx.x >> 2) : (int)threadIdx.x; }
-  static __device__ __forceinline__ bool owner() { return TRANS ? (threadIdx.x & 3) == 0 : threadIdx.x < TB; }
+  // adds this lane's partial sums of its two outputs with x (LDS, 16-byte
+  // aligned) to s0, s1
+  __device__ __forceinline__ void partial(const double* x, double& s0, double& s1) const {
+    const int l = lane();
+    if (TRANS) {
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const double2 xv = *(const double2*)(x + 2 * (l & 7) + 16 * q);
+        s0 += (double)a[2 * q] * xv.x + (double)a[2 * q + 1] * xv.y;
+        s1 += (double)a[8 + 2 * q] * xv.x + (double)a[8 + 2 * q + 1] * xv.y;
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        const double xv = x[(l >> 3) + 8 * q];
+        s0 += (double)a[2 * q] * xv;
+        s1 += (double)a[2 * q + 1] * xv;
+      }
+    }
+  }
+  static __device__ __forceinline__ void reduce(double& s0, double& s1) {
+#pragma unroll
+    for (int m = TRANS ? 1 : 8; m <= (TRANS ? 4 : 32); m *= 2) {
+      s0 += __shfl_xor(s0, m);
+      s1 += __shfl_xor(s1, m);
+    }
+  }
+  static __device__ __forceinline__ bool owner() { return TRANS ? (lane() & 7) == 0 : lane() < 8; }
+  static __device__ __forceinline__ int index(int e) {
+    return TRANS ? 16 * wave() + (lane() >> 3) + 8 * e : 16 * wave() + 2 * lane() + e;
+  }
 };
 
 }  // namespace nsn
