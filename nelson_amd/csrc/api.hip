@@ -375,10 +375,8 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
   }
   h->n_s_parts = nparts;
   h->NR = (n + kTB - 1) / kTB;
-  // flags: ready (ntiles), the triangular solves' xready / tready (NR each),
-  // pre_sub / pre_diag (NT each), the backward refinement solve's xready /
-  // tready (NR each)
-  h->nflags = tv.ntiles + 2 * tv.NT;  // tile flags, pre_sub, pre_diag
+  // flags of the factorisation: ready (ntiles), pre_sub / pre_diag (NT each)
+  h->nflags = tv.ntiles + 2 * tv.NT;
   int n_tasks = 0;
   chol_task_list(tv.NT, 0, nullptr, &n_tasks);  // the count does not depend on the look-ahead
   const int ahead = chol_lookahead(std::min(n_tasks, kCholMaxWG - 1));
