@@ -314,6 +314,7 @@ def main():
                 "kernel_ms": ph["factorize"]}
     mixed = None
     mixed_small = None
+    mixed_ir1 = None
     if not args.no_mixed:
         # mixed precision (FP32 blocks / S / Cholesky, FP64 pivot tiles, cost, gain
         # ratio and 2 iterative-refinement steps) on the same workload, and on the
@@ -323,6 +324,13 @@ def main():
         mixed = dict(rm, refinement_steps=2,
                      final_cost_rel_err_vs_fp64=abs(rm["final_chi2"] - r64["final_chi2"]) / r64["final_chi2"],
                      phases_ms=chol_timing(nsn, pm))
+        # the same with one refinement step (the library's setting
+        # nsn_set_refinement): the speed / accuracy trade-off, not the headline
+        _, r1 = run_lm(nsn, g, "mixed", args.steps, args.warmup, refine=1)
+        mixed_ir1 = {"refinement_steps": 1, "it_per_s": r1["it_per_s"], "ms_per_solve": r1["ms_per_solve"],
+                     "it_per_s_ratio_vs_fp64": r1["it_per_s"] / r64["it_per_s"],
+                     "final_cost_rel_err_vs_fp64": abs(r1["final_chi2"] - r64["final_chi2"]) / r64["final_chi2"]}
+        log(f"bench: mixed, one refinement step {mixed_ir1}")
         # the 10-camera / 500-point problem of the mixed-precision
         # configuration (20 iterations)
         gs = nsn.generate_ba(dict(nsn.CONFIGS["cfg1"]))
@@ -384,6 +392,7 @@ def main():
         "measured_peaks": peaks,
         "mixed": mixed,
         "mixed_small": mixed_small,
+        "mixed_ir1": mixed_ir1,
         "adjoint": adjoint,
         "batch": batch,
         "ladybug": ladybug,
