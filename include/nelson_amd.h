@@ -75,6 +75,9 @@ typedef struct {
   int n_s;                                     /* 6 * NU, order of the reduced camera system */
   int n_s_blocks;                              /* non-zero upper 6x6 blocks of S */
   long long n_s_pairs;                         /* block products of the S assembly */
+  int n_tiles;                                 /* stored 64x64 tiles of [S; bS^T]: its envelope (the
+                                                  sparse S path; fill stays inside it) */
+  int n_tiles_dense;                           /* NT (NT + 1) / 2, the dense count */
 } nsn_problem_info;
 
 /* LevenbergMarquardtSettings (nelson/LevenbergMarquardt.h:18-34) */
@@ -116,7 +119,9 @@ void nsn_gn_default_settings(nsn_gn_settings* s);
  * addNoise (nelson/samples/BASchurPerfomances.cpp:40-100) play this role in
  * the reference.  layout 0: cameras on a circle looking at a 4 m cube of
  * points; layout 1: cameras on a ring, each point observed from a contiguous
- * arc of `arc` cameras.  Camera 0 and point 0 are left unperturbed (the gauge
+ * arc of `arc` cameras; layout 2: the same with arcs that do not wrap round
+ * the ring (cameras along an open path: a banded S).  Camera 0 and point 0
+ * are left unperturbed (the gauge
  * of the sample, BASchurPerfomances.cpp:245-246).  Writes at most edge_cap
  * observations; *n_edges receives the count (NSN_EINVAL if it would exceed). */
 typedef struct {

@@ -50,7 +50,8 @@ class ProblemDesc(C.Structure):
 
 class ProblemInfo(C.Structure):
     _fields_ = [("n_var_cameras", C.c_int), ("n_var_points", C.c_int), ("n_wblocks", C.c_int),
-                ("n_s", C.c_int), ("n_s_blocks", C.c_int), ("n_s_pairs", C.c_longlong)]
+                ("n_s", C.c_int), ("n_s_blocks", C.c_int), ("n_s_pairs", C.c_longlong), ("n_tiles", C.c_int),
+                ("n_tiles_dense", C.c_int)]
 
 
 class LMSettings(C.Structure):

@@ -86,6 +86,7 @@ struct nsn_problem {
   int C = 0, P = 0, E = 0, NU = 0, NV = 0, nW = 0, Ec = 0, prec = NSN_FP64;
   double f = 0, cx = 0, cy = 0, huber = 0;
   TileView tv{};
+  TileProfile prof;  // the host copy of tv's envelope
   int nsb = 0;
   long long npairs = 0;
   // host structure
@@ -138,6 +139,10 @@ struct nsn_problem {
   bool imported = false;  // the current Hessian came from an import
   int n_uoff = 0;
   int *uoff_i = nullptr, *uoff_j = nullptr;
+  // the same blocks on the host, known before the structure stage (the tile
+  // envelope of S contains them)
+  std::vector<int> uoff_i_host, uoff_j_host;
+  int* tprof = nullptr;  // device copy of the tile envelope: first[NT], row_off[NT]
   void* Uoff = nullptr;
 
   ~nsn_problem() {
@@ -280,6 +285,40 @@ int check_device_errors(nsn_problem* h) {
 }
 
 // ---- structure stage ------------------------------------------------------
+
+// the tile envelope of [S; bS^T]: tile row i starts at the first tile
+// column that holds a non-zero block of its rows (the S blocks (ci, cj),
+// ci < cj, of variable cameras sharing a point, at rows 6 cj, columns 6 ci;
+// the extra U-U blocks), at least two tiles before the diagonal (the
+// factorisation's chain reads the two sub-diagonal tiles); the last tile
+// row, which holds the right-hand side, is dense.  NSN_DENSE_S=1 in the
+// environment when a problem is created forces a dense layout (for
+// comparisons: the tiles outside the envelope are exact zeros, so both
+// layouts give the same results bit for bit).
+static void tile_envelope(int NT, const std::vector<int>& sb_i, const std::vector<int>& sb_j,
+                          const std::vector<int>& xi, const std::vector<int>& xj, TileProfile& p) {
+  const char* de = std::getenv("NSN_DENSE_S");
+  const bool dense = de != nullptr && de[0] != '\0' && de[0] != '0';
+  p.first.assign(NT, 0);
+  for (int i = 0; i < NT; i++) p.first[i] = std::max(0, i - 2);
+  auto add = [&](int ci, int cj) {
+    const int tc = (6 * ci) / kTB;
+    for (int ti = (6 * cj) / kTB; ti <= (6 * cj + 5) / kTB && ti < NT; ti++) p.first[ti] = std::min(p.first[ti], tc);
+  };
+  for (size_t b = 0; b < sb_i.size(); b++)
+    if (sb_i[b] != sb_j[b]) add(std::min(sb_i[b], sb_j[b]), std::max(sb_i[b], sb_j[b]));
+  for (size_t b = 0; b < xi.size(); b++) add(std::min(xi[b], xj[b]), std::max(xi[b], xj[b]));
+  if (NT > 0) p.first[NT - 1] = 0;
+  if (dense) std::fill(p.first.begin(), p.first.end(), 0);
+  p.row_off.assign(NT, 0);
+  long long off = 0;
+  for (int i = 0; i < NT; i++) {
+    p.row_off[i] = (int)off;
+    off += i - p.first[i] + 1;
+  }
+  p.ntiles = (int)off;
+}
+
 int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
   const int C = h->C, P = h->P, E = h->E;
   HostStructure hs;
@@ -305,18 +344,20 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
   &cam_ptr = hs.cam_ptr, &ce_pt = hs.ce_pt, &cw_ptr = hs.cw_ptr, &cw_idx = hs.cw_idx, &sb_i = hs.sb_i,
   &sb_j = hs.sb_j, &sb_ptr = hs.sb_ptr, &sb_pairs = hs.sb_pairs;
   const std::vector<double>&pe_obs = hs.pe_obs, &pe_w = hs.pe_w, &ce_obs = hs.ce_obs, &ce_w = hs.ce_w;
-  // tiles of [S; bS^T]
+  // tiles of [S; bS^T] over the envelope of S (tile_envelope)
   const int n = 6 * NU;
-  TileView tv;
+  TileView tv{};
   tv.n = n;
   tv.TB = kTB;
   tv.NT = (n + 1 + kTB - 1) / kTB;
-  tv.ntiles = tv.NT * (tv.NT + 1) / 2;
+  TileProfile& prof = h->prof;
+  tile_envelope(tv.NT, sb_i, sb_j, h->uoff_i_host, h->uoff_j_host, prof);
+  tv.ntiles = prof.ntiles;
   h->tv = tv;
   std::vector<int> tile_order(2 * (size_t)tv.ntiles);
-  for (int j = 0; j < tv.NT; j++)
-    for (int i = j; i < tv.NT; i++) {
-      const long long t = tile_index(i, j, tv.NT);
+  for (int i = 0; i < tv.NT; i++)
+    for (int j = prof.first[i]; j <= i; j++) {
+      const long long t = prof.slot(i, j);
       tile_order[2 * t] = i;
       tile_order[2 * t + 1] = j;
     }
@@ -359,7 +400,7 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
     for (int ti = ti0; ti <= ti1; ti++)
       for (int tj = tj0; tj <= tj1; tj++)
         if (ti >= tj) {
-          const long long t = tile_index(ti, tj, tv.NT);
+          const long long t = prof.slot(ti, tj);
           const int r0 = ti * kTB, c0 = tj * kTB;
           const int sr = std::max(6 * cj, r0) / u - r0 / u, sc = std::max(6 * ci, c0) / u - c0 / u;
           fn(part_base[t] + sr * n_seg(c0, kTB, u) + sc);
@@ -378,10 +419,10 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
   // flags of the factorisation: ready (ntiles), pre_sub / pre_diag (NT each)
   h->nflags = tv.ntiles + 2 * tv.NT;
   int n_tasks = 0;
-  chol_task_list(tv.NT, 0, nullptr, &n_tasks);  // the count does not depend on the look-ahead
+  chol_task_list(tv.NT, 0, nullptr, &n_tasks, prof.first.data());  // the count does not depend on the look-ahead
   const int ahead = chol_lookahead(std::min(n_tasks, kCholMaxWG - 1));
   std::vector<int> chol_tasks(3 * (size_t)std::max(n_tasks, 1));
-  chol_task_list(tv.NT, ahead, chol_tasks.data(), &n_tasks);
+  chol_task_list(tv.NT, ahead, chol_tasks.data(), &n_tasks, prof.first.data());
   h->n_chol_tasks = n_tasks;
 
   // ---- device allocation ----
@@ -424,6 +465,7 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
   h->Sd = a_T(21 * (size_t)NU);
   h->bS = a_T(6 * (size_t)NU);
   h->tiles = a_T((size_t)tv.ntiles * kTB * kTB);
+  h->tprof = a_int(2 * (size_t)tv.NT);
   h->inv = a_T((size_t)tv.NT * kTB * kTB);
   h->presub = a_T((size_t)tv.NT * kTB * kTB);
   h->xU = a_dbl(6 * (size_t)NU);
@@ -476,6 +518,13 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
       (rc = upload(h->sb_i, sb_i)) || (rc = upload(h->sb_j, sb_j)) || (rc = upload(h->sb_ptr, sb_ptr)) ||
       (rc = upload(h->sb_pairs, sb_pairs)))
     return rc;
+  {
+    std::vector<int> tp(prof.first);
+    tp.insert(tp.end(), prof.row_off.begin(), prof.row_off.end());
+    if ((rc = upload(h->tprof, tp))) return rc;
+    h->tv.first = h->tprof;
+    h->tv.row_off = h->tprof + tv.NT;
+  }
   BAView& v = h->view;
   v.C = C; v.P = P; v.E = E; v.NU = NU; v.NV = NV; v.nW = nW;
   v.f = h->f; v.cx = h->cx; v.cy = h->cy; v.huber = h->huber;
@@ -518,7 +567,8 @@ extern "C" int nsn_device_count(int* n) {
   return NSN_OK;
 }
 
-static int problem_create_impl(const nsn_problem_desc* d, nsn_problem** out, long fail_after);
+static int problem_create_impl(const nsn_problem_desc* d, nsn_problem** out, long fail_after,
+                               const std::vector<int>* uoff_i = nullptr, const std::vector<int>* uoff_j = nullptr);
 
 extern "C" int nsn_problem_create(const nsn_problem_desc* d, nsn_problem** out) {
   clear_error();
@@ -527,7 +577,8 @@ extern "C" int nsn_problem_create(const nsn_problem_desc* d, nsn_problem** out) 
   return problem_create_impl(d, out, fa);
 }
 
-static int problem_create_impl(const nsn_problem_desc* d, nsn_problem** out, long fail_after) {
+static int problem_create_impl(const nsn_problem_desc* d, nsn_problem** out, long fail_after,
+                               const std::vector<int>* uoff_i, const std::vector<int>* uoff_j) {
   if (!d || !out) return fail(NSN_EINVAL, "nsn_problem_create: null argument");
   *out = nullptr;
   if (d->n_cameras < 1 || d->n_points < 1 || d->n_edges < 0 || (d->n_edges > 0 && (!d->cam_idx || !d->pt_idx || !d->obs)))
@@ -553,6 +604,10 @@ static int problem_create_impl(const nsn_problem_desc* d, nsn_problem** out, lon
   HIP_TRY(hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking));
   for (auto& e : h->ev) HIP_TRY(hipEventCreate(&e));
   HIP_TRY(hipHostMalloc((void**)&h->hctl, sizeof(DevCtl), hipHostMallocDefault));
+  if (uoff_i && uoff_j) {
+    h->uoff_i_host = *uoff_i;
+    h->uoff_j_host = *uoff_j;
+  }
   int rc = build_structure(h.get(), d);
   if (rc) return rc;
   *h->hctl = make_ctl(h->log_cap);
@@ -608,23 +663,26 @@ extern "C" int nsn_problem_create_blocks(const nsn_block_desc* d, nsn_problem** 
   bd.focal = 1.0;
   bd.precision = d->precision;
   bd.device = d->device;
-  nsn_problem* h = nullptr;
-  int rc = problem_create_impl(&bd, &h, -1);
-  if (rc) return rc;
-  std::unique_ptr<nsn_problem> hp(h);
-  h->blocks_only = true;
-  if (d->n_uoff > 0) {
-    // stored in UID order (row-major (i, j)); uoff_ref maps the caller's order
+  // the U-U blocks, stored in UID order (row-major (i, j)); the tile
+  // envelope of S is laid out with them
+  std::vector<int> ui(d->n_uoff), uj(d->n_uoff);
+  {
     std::vector<int> ord(d->n_uoff);
     std::iota(ord.begin(), ord.end(), 0);
     std::sort(ord.begin(), ord.end(), [&](int a, int b) {
       return d->uoff_i[a] != d->uoff_i[b] ? d->uoff_i[a] < d->uoff_i[b] : d->uoff_j[a] < d->uoff_j[b];
     });
-    std::vector<int> ui(d->n_uoff), uj(d->n_uoff);
     for (int k = 0; k < d->n_uoff; k++) {
       ui[k] = d->uoff_i[ord[k]];
       uj[k] = d->uoff_j[ord[k]];
     }
+  }
+  nsn_problem* h = nullptr;
+  int rc = problem_create_impl(&bd, &h, -1, &ui, &uj);
+  if (rc) return rc;
+  std::unique_ptr<nsn_problem> hp(h);
+  h->blocks_only = true;
+  if (d->n_uoff > 0) {
     DevBuf& m = h->mem;
     h->uoff_i = m.alloc<int>(d->n_uoff);
     h->uoff_j = m.alloc<int>(d->n_uoff);
@@ -653,6 +711,8 @@ extern "C" int nsn_problem_info_get(nsn_problem* h, nsn_problem_info* info) {
   info->n_s = 6 * h->NU;
   info->n_s_blocks = h->nsb;
   info->n_s_pairs = h->npairs;
+  info->n_tiles = h->tv.ntiles;
+  info->n_tiles_dense = h->tv.NT * (h->tv.NT + 1) / 2;
   return NSN_OK;
 }
 

@@ -1013,7 +1013,7 @@ __global__ __launch_bounds__(256) void k_add_uoff(TileView tv, int nb, const int
   if (e >= 36LL * nb) return;
   const int k = (int)(e / 36), a = (int)(e % 36) / 6, b = (int)(e % 6);
   const int R = 6 * uj[k] + a, Cc = 6 * ui[k] + b;
-  const long long t = tile_index(R / TB, Cc / TB, tv.NT);
+  const long long t = tv.slot(R / TB, Cc / TB);
   tiles[t * TB * TB + (long long)(Cc % TB) * TB + (R % TB)] += Uoff[36LL * k + 6 * b + a];
 }
 

@@ -99,26 +99,28 @@ __device__ __forceinline__ void publish_flag(int* flag) {
 // ---------------------------------------------------------------------------
 enum CholTask : int { CT_TILE = 0, CT_PRE_SUB = 1, CT_PRE_DIAG = 2, CT_SUB = 3 };
 
-// acc -= sum_{k < kend} L_ik L_jk^T over published tiles (batched waits).
+// acc -= sum_{k0 <= k < kend} L_ik L_jk^T over published tiles (batched
+// waits); k0 = max(first[i], first[j]), the first k with both tiles inside
+// the envelope.
 // Operands go from global memory straight into the MFMA operand registers
 // (a register double buffer across products measured slower: the diagonal
 // chain shares this kernel's register budget).
 template <typename T, int TB>
 __device__ __forceinline__ void acc_update(CholAcc<T>& acc, const T* __restrict__ tiles, int* ready, DevCtl* ctl,
-                                           int* shv, int i, int j, int kend, int NT, int lane, int wr, int wc) {
+                                           int* shv, int i, int j, int kend, const TileView& tv, int lane, int wr,
+                                           int wc) {
   const size_t T2 = (size_t)TB * TB;
-  int k = 0;
+  int k = max(tv.first[i], tv.first[j]);
   while (k < kend) {
     if (threadIdx.x < 64) {
       if (lane == 0) {
-        wait_flag(ready + tile_index(i, k, NT), ctl);
-        if (i != j) wait_flag(ready + tile_index(j, k, NT), ctl);
+        wait_flag(ready + tv.slot(i, k), ctl);
+        if (i != j) wait_flag(ready + tv.slot(j, k), ctl);
       }
       bool ok = false;
       if (k + lane < kend)
-        ok = __hip_atomic_load(ready + tile_index(i, k + lane, NT), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &&
-             (i == j ||
-              __hip_atomic_load(ready + tile_index(j, k + lane, NT), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        ok = __hip_atomic_load(ready + tv.slot(i, k + lane), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &&
+             (i == j || __hip_atomic_load(ready + tv.slot(j, k + lane), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
       const unsigned long long m = __ballot(ok);
       const int run = ~m == 0ull ? 64 : __builtin_ctzll(~m);
       if (lane == 0) shv[1] = k + (run > 0 ? run : 1);
@@ -128,8 +130,8 @@ __device__ __forceinline__ void acc_update(CholAcc<T>& acc, const T* __restrict_
     __syncthreads();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     for (; k < ke; k++) {
-      const T* Ag = tiles + (size_t)tile_index(i, k, NT) * T2 + wr * 32 + (lane & 15);
-      const T* Bg = tiles + (size_t)tile_index(j, k, NT) * T2 + wc * 32 + (lane & 15);
+      const T* Ag = tiles + (size_t)tv.slot(i, k) * T2 + wr * 32 + (lane & 15);
+      const T* Bg = tiles + (size_t)tv.slot(j, k) * T2 + wc * 32 + (lane & 15);
       const int kq = lane >> 4;
 #pragma unroll 8
       for (int m = 0; m < TB; m += 4) {
@@ -166,7 +168,7 @@ __global__ __launch_bounds__(256) void k_chol(TileView tv, T* __restrict__ tiles
     CholAcc<T> acc;  // the next diagonal tile, loaded one step ahead
     acc_load<T, TB>(acc, tiles, lane, wr, wc);
     for (int j = 0; j < NT; j++) {
-      T* Cg = tiles + (size_t)tile_index(j, j, NT) * T2;
+      T* Cg = tiles + (size_t)tv.slot(j, j) * T2;
       if (j >= 1) acc_gemm_nt<T, T, TB>(acc, R0, R0, true, lane, wr, wc);  // - L_{j,j-1} L_{j,j-1}^T
       __syncthreads();
       acc_store<T, double, TB>(acc, R0d, lane, wr, wc);
@@ -176,7 +178,7 @@ __global__ __launch_bounds__(256) void k_chol(TileView tv, T* __restrict__ tiles
       T* Ig = inv + (size_t)j * T2;
 #pragma unroll 4
       for (int e = threadIdx.x; e < TB * TB; e += 256) Ig[e] = (T)R1d[e];
-      publish_flag(ready + tile_index(j, j, NT));
+      publish_flag(ready + tv.slot(j, j));
       // the factor tile itself is read only after the launch (solves): its
       // stores go out after the flag, so the release above waits only for
       // the inverse (R0d is overwritten only after the next barrier)
@@ -190,7 +192,7 @@ __global__ __launch_bounds__(256) void k_chol(TileView tv, T* __restrict__ tiles
         }
         __syncthreads();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        acc_load<T, TB>(acc, tiles + (size_t)tile_index(j + 1, j + 1, NT) * T2, lane, wr, wc);
+        acc_load<T, TB>(acc, tiles + (size_t)tv.slot(j + 1, j + 1) * T2, lane, wr, wc);
         // L_{j+1,j} for the next step, from the pre-updated C_{j+1,j}; a
         // worker forms the same product for the tile storage (CT_SUB)
         global_to_lds<T, TB>(R0, presub + (size_t)j * T2);
@@ -213,12 +215,12 @@ __global__ __launch_bounds__(256) void k_chol(TileView tv, T* __restrict__ tiles
     if (t >= ntasks) break;
     const int type = tasks[3 * t], i = tasks[3 * t + 1], j = tasks[3 * t + 2];
     if (type == CT_TILE) {
-      T* Cg = tiles + (size_t)tile_index(i, j, NT) * T2;
+      T* Cg = tiles + (size_t)tv.slot(i, j) * T2;
       CholAcc<T> acc;
       acc_load<T, TB>(acc, Cg, lane, wr, wc);
-      acc_update<T, TB>(acc, tiles, ready, ctl, shv, i, j, j, NT, lane, wr, wc);
+      acc_update<T, TB>(acc, tiles, ready, ctl, shv, i, j, j, tv, lane, wr, wc);
       acc_store<T, T, TB>(acc, R0, lane, wr, wc);
-      if (threadIdx.x == 0) wait_flag(ready + tile_index(j, j, NT), ctl);
+      if (threadIdx.x == 0) wait_flag(ready + tv.slot(j, j), ctl);
       __syncthreads();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       // X = C L_jj^-T with the inverse published by the diagonal chain
@@ -228,13 +230,13 @@ __global__ __launch_bounds__(256) void k_chol(TileView tv, T* __restrict__ tiles
       tri_store<T, TB>(x, R0, lane, wv);
       __syncthreads();
       lds_to_global<T, TB>(Cg, R0);
-      publish_flag(ready + tile_index(i, j, NT));
+      publish_flag(ready + tv.slot(i, j));
     } else if (type == CT_SUB) {
       // L_{j+1,j} = C_{j+1,j} L_jj^-T into the tile storage (the diagonal
       // chain keeps its own copy)
       if (threadIdx.x == 0) {
         wait_flag(pre_sub + j, ctl);
-        wait_flag(ready + tile_index(j, j, NT), ctl);
+        wait_flag(ready + tv.slot(j, j), ctl);
       }
       __syncthreads();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
@@ -245,18 +247,18 @@ __global__ __launch_bounds__(256) void k_chol(TileView tv, T* __restrict__ tiles
       __syncthreads();
       tri_store<T, TB>(x, R0, lane, wv);
       __syncthreads();
-      lds_to_global<T, TB>(tiles + (size_t)tile_index(i, j, NT) * T2, R0);
-      publish_flag(ready + tile_index(i, j, NT));
+      lds_to_global<T, TB>(tiles + (size_t)tv.slot(i, j) * T2, R0);
+      publish_flag(ready + tv.slot(i, j));
     } else {
       // pre-update with every k < j of C_{j+1,j} (CT_PRE_SUB, into the
       // scratch tile presub[j]) or of C_{j+1,j+1} (CT_PRE_DIAG, in place)
       const int jc = type == CT_PRE_SUB ? j : j + 1;
       CholAcc<T> acc;
-      acc_load<T, TB>(acc, tiles + (size_t)tile_index(i, jc, NT) * T2, lane, wr, wc);
-      acc_update<T, TB>(acc, tiles, ready, ctl, shv, i, jc, j, NT, lane, wr, wc);
+      acc_load<T, TB>(acc, tiles + (size_t)tv.slot(i, jc) * T2, lane, wr, wc);
+      acc_update<T, TB>(acc, tiles, ready, ctl, shv, i, jc, j, tv, lane, wr, wc);
       acc_store<T, T, TB>(acc, R0, lane, wr, wc);
       __syncthreads();
-      lds_to_global<T, TB>(type == CT_PRE_SUB ? presub + (size_t)j * T2 : tiles + (size_t)tile_index(i, jc, NT) * T2,
+      lds_to_global<T, TB>(type == CT_PRE_SUB ? presub + (size_t)j * T2 : tiles + (size_t)tv.slot(i, jc) * T2,
                            R0);
       publish_flag(type == CT_PRE_SUB ? pre_sub + j : pre_diag + j + 1);
     }
@@ -335,7 +337,9 @@ __global__ __launch_bounds__(256) void k_trsv(TileView tv, const T* __restrict__
   auto blk = [&](int s) { return BWD ? NR - 1 - s : s; };
   auto valid = [&](int b) { return b >= 0 && b < NR; };
   // the tile applied to x_k for destination i (L_ik forward, L_ki^T backward)
-  auto ntile = [&](int i, int k) { return tiles + (size_t)(BWD ? tile_index(k, i, NT) : tile_index(i, k, NT)) * T2; };
+  auto ntile = [&](int i, int k) { return tiles + (size_t)(BWD ? tv.slot(k, i) : tv.slot(i, k)) * T2; };
+  // the tile applied to x_k for destination i is inside the envelope
+  auto ntile_has = [&](int i, int k) { return BWD ? tv.has(k, i) : tv.has(i, k); };
   const bool own = Gemv::owner();
   const int i0 = Gemv::index(0), i1 = Gemv::index(1);
   if (blockIdx.x == 0) {
@@ -346,7 +350,7 @@ __global__ __launch_bounds__(256) void k_trsv(TileView tv, const T* __restrict__
     Gemv gd, gn[kNear];
     double r0 = 0.0, r1 = 0.0, f0 = 0.0, f1 = 0.0;
     auto rhs_at = [&](int row) {
-      return rhs ? rhs[row] : (double)tiles[(size_t)tile_index(n / TB, row / TB, NT) * T2 + (row % TB) * TB + (n % TB)];
+      return rhs ? rhs[row] : (double)tiles[(size_t)tv.slot(n / TB, row / TB) * T2 + (row % TB) * TB + (n % TB)];
     };
     auto load_near = [&](int s) {
       const int k = blk(s);
@@ -409,37 +413,53 @@ __global__ __launch_bounds__(256) void k_trsv(TileView tv, const T* __restrict__
   const int nown = (NR - 1 - first) / G + 1;  // <= kMaxOwned (launch)
   if (own)
     for (int m = 0; m < nown; m++) acc[m][i0] = acc[m][i1] = 0.0;
+  // the products (source s, destination m) in source order, destinations in
+  // order within a source: destination m takes the far sources s <= its
+  // solve index - kNear - 1 whose tile is inside the envelope; the tile of
+  // the next product is loaded while the current one runs
+  auto s_last = [&](int m) { return first + G * m - kNear - 1; };
+  auto find_next = [&](int& ps, int& pm) {  // the first product at or after (ps, pm)
+    for (; ps < NR; ps++, pm = 0) {
+      for (; pm < nown; pm++)
+        if (ps <= s_last(pm) && ntile_has(blk(first + G * pm), blk(ps))) return true;
+    }
+    return false;
+  };
   Gemv gg;
+  int cs = 0, cm = 0;
+  bool have = find_next(cs, cm);
+  if (have) gg.load(ntile(blk(first + G * cm), blk(cs)));
   int m0 = 0;  // first owned destination still open
-  gg.load(ntile(blk(first), blk(0)));
+  int pc = 0;  // sources polled so far (the parity of the LDS buffer)
   for (int s = 0; s < NR && m0 < nown; s++) {  // source steps
-    const int k = blk(s);
-    // wave 0 polls the source's solution into the LDS (double-buffered by
-    // source parity: the barrier of source s orders every read of the buffer
-    // of source s - 2 before this write)
-    if (threadIdx.x < TB) {
-      const int row = k * TB + threadIdx.x;
-      xs[s & 1][threadIdx.x] = row < n ? wait_pub(sol + row, load_pub(sol + row), ctl) : 0.0;
-    }
-    __syncthreads();
-    for (int m = m0; m < nown; m++) {
-      double v0 = 0.0, v1 = 0.0;
-      gg.partial(xs[s & 1], v0, v1);
-      Gemv::reduce(v0, v1);
-      if (own) {
-        acc[m][i0] += v0;
-        acc[m][i1] += v1;
+    if (have && cs == s) {
+      const int k = blk(s);
+      // wave 0 polls the source's solution into the LDS (double-buffered by
+      // the count of polled sources: the barrier of the previous one orders
+      // every read of this buffer, two polled sources back, before the write)
+      double* xb = xs[pc & 1];
+      if (threadIdx.x < TB) {
+        const int row = k * TB + threadIdx.x;
+        xb[threadIdx.x] = row < n ? wait_pub(sol + row, load_pub(sol + row), ctl) : 0.0;
       }
-      // next product: the following destination, or the first open one of the next source
-      int nm = m + 1, ns = s;
-      if (nm == nown) {
-        nm = m0 + (first + G * m0 - kNear - 1 == s ? 1 : 0);
-        ns = s + 1;
+      __syncthreads();
+      pc++;
+      while (have && cs == s) {
+        double v0 = 0.0, v1 = 0.0;
+        gg.partial(xb, v0, v1);
+        Gemv::reduce(v0, v1);
+        if (own) {
+          acc[cm][i0] += v0;
+          acc[cm][i1] += v1;
+        }
+        cm++;
+        have = find_next(cs, cm);
+        if (have) gg.load(ntile(blk(first + G * cm), blk(cs)));
       }
-      if (nm < nown && ns < NR) gg.load(ntile(blk(first + G * nm), blk(ns)));
     }
-    // destination m0 is complete after its last far source
-    if (first + G * m0 - kNear - 1 == s) {
+    // destination m0 is complete after its last far source (published even
+    // when no tile of its row is inside the envelope: then its sum is 0)
+    while (m0 < nown && s_last(m0) == s) {
       const int d = blk(first + G * m0);
       if (own) {
         if (d * TB + i0 < n) store_pub(far + d * TB + i0, acc[m0][i0]);

@@ -16,7 +16,11 @@ namespace nsn {
 // published.  Only such hoisted tasks can wait for tasks not yet taken, and
 // at most 3 * ahead + 3 of them do so at a time, so the queue cannot
 // deadlock while the launch has more workers than that (chol_lookahead).
-inline void chol_task_list(int NT, int ahead, int* out, int* ntasks) {
+// first (may be null: dense): the first stored tile column of each tile
+// row (an envelope with first[i] <= i - 2); tiles outside it are zero in S
+// and in the factor and get no task, and the updates of a tile (i, j) run
+// over k >= max(first[i], first[j]) only.
+inline void chol_task_list(int NT, int ahead, int* out, int* ntasks, const int* first = nullptr) {
   int t = 0;
   auto add = [&](int kind, int i, int j) {
     if (out) {
@@ -37,7 +41,8 @@ inline void chol_task_list(int NT, int ahead, int* out, int* ntasks) {
   for (int j = 0; j < NT; j++) {
     if (j >= 1 && j + ahead < NT) add_early(j + ahead);
     if (j + 1 < NT) add(3, j + 1, j);
-    for (int i = j + 3; i < NT; i++) add(0, i, j);
+    for (int i = j + 3; i < NT; i++)
+      if (!first || first[i] <= j) add(0, i, j);
   }
   *ntasks = t;
 }

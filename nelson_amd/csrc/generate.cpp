@@ -6,6 +6,7 @@
 // uniforms -> Box-Muller normals (cosine branch only).  Draw order: camera
 // poses, points, observations (per point: k, cameras, noise, outlier), then
 // initial-state perturbations (cameras, then points).
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -78,7 +79,9 @@ extern "C" int nsn_generate_ba(const nsn_gen_params* p, int edge_cap, double* ca
     if (p->layout == 0) {
       for (int i = 0; i < 3; i++) pts_gt[3 * q + i] = off[i];
     } else {
-      int a = rng.uniform_int(0, C - 1);
+      // layout 2: the camera window does not wrap round the ring
+      const int amax = (p->layout == 2) ? std::max(0, C - std::max(p->arc, 1)) : C - 1;
+      int a = rng.uniform_int(0, amax);
       arc_start[q] = a;
       int m = (a + p->arc / 2) % C;
       double phi = 2.0 * M_PI * (double)m / (double)C;

@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "chol_tasks.h"
 
 #include "internal.h"
@@ -61,18 +63,32 @@ struct BAView {
 };
 
 // Tiled lower storage of the reduced camera system S (plus the augmented
-// right-hand-side row at index n): tile (i, j), i >= j, of TB x TB values,
-// column-major inside, tiles packed column by column.
+// right-hand-side row at index n) over an envelope: tile row i stores the
+// tiles (i, j) for first[i] <= j <= i, each TB x TB values, column-major
+// inside, packed row by row (slot of (i, j) = row_off[i] + j - first[i]).
+// A Cholesky factor has no fill outside the envelope of its matrix, so the
+// factor takes the same storage.  A dense S has first[i] = 0.  Every
+// envelope here keeps the two sub-diagonal tiles (first[i] <= i - 2) and a
+// dense last tile row (the right-hand side).
 struct TileView {
   int n;        // order of S (6 NU)
   int TB;       // tile size
   int NT;       // tiles per dimension (covers rows 0..n inclusive)
-  int ntiles;   // NT (NT + 1) / 2
+  int ntiles;   // stored tiles
+  const int* first;    // [NT] first stored tile column of each tile row (device)
+  const int* row_off;  // [NT] slot of tile (i, first[i]) (device)
+  __device__ __forceinline__ long long slot(int i, int j) const { return (long long)row_off[i] + (j - first[i]); }
+  __device__ __forceinline__ bool has(int i, int j) const { return j >= first[i]; }  // for i >= j
 };
 
-__host__ __device__ inline long long tile_index(int i, int j, int NT) {
-  return (long long)j * NT - (long long)j * (j - 1) / 2 + (i - j);
-}
+// the envelope of a tiled S on the host (the device copy of first / row_off
+// is what TileView points to)
+struct TileProfile {
+  std::vector<int> first, row_off;
+  int ntiles = 0;
+  long long slot(int i, int j) const { return (long long)row_off[i] + (j - first[i]); }
+  bool has(int i, int j) const { return j >= first[i]; }
+};
 
 // reduction scratch layout (doubles), one slot per block of the producing kernel
 struct Scratch {

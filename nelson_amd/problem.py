@@ -36,6 +36,11 @@ CONFIGS = {
     # 1000 cameras on a ring, 200k points seen by 4-8 cameras of a 150-camera arc, 30 LM iterations
     "cfg2": dict(_BASE, n_cameras=1000, n_points=200000, k_min=4, k_max=8, layout=1, arc=150, seed=2,
                  iterations=30),
+    # 4000 cameras along an open path (a ring whose camera windows do not
+    # wrap), 400k points seen by 4-8 cameras of a 40-camera window: a banded S
+    # (n = 23 994) for the sparse-S path
+    "seq": dict(_BASE, n_cameras=4000, n_points=400000, k_min=4, k_max=8, layout=2, arc=40, seed=7,
+                iterations=20),
     # 200 cameras, 20k points, 6 observations per point (differentiable BA)
     "cfg5": dict(_BASE, n_cameras=200, n_points=20000, k_min=6, k_max=6, layout=1, arc=40, seed=5,
                  iterations=50),
@@ -203,6 +208,7 @@ class BAProblem:
         check(self._lib.nsn_problem_info_get(self._h, C.byref(info)))
         self.NU, self.NV, self.nW = info.n_var_cameras, info.n_var_points, info.n_wblocks
         self.n_s, self.n_s_blocks, self.n_s_pairs = info.n_s, info.n_s_blocks, info.n_s_pairs
+        self.n_tiles, self.n_tiles_dense = info.n_tiles, info.n_tiles_dense
         self._chi2 = None
 
     @classmethod

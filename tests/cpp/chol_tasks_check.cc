@@ -13,6 +13,7 @@
 //    which shows the simulation sees what it should.
 //
 // Output: one line per failure, then "OK" or "FAIL"; exit status 0 / 1.
+#include <algorithm>
 #include <cstdio>
 #include <map>
 #include <tuple>
@@ -29,11 +30,11 @@ struct Task {
   int kind, i, j;
 };
 
-std::vector<Task> tasks_of(int NT, int ahead) {
+std::vector<Task> tasks_of(int NT, int ahead, const int* first = nullptr) {
   int n = 0;
-  chol_task_list(NT, ahead, nullptr, &n);
+  chol_task_list(NT, ahead, nullptr, &n, first);
   std::vector<int> raw(3 * (size_t)(n > 0 ? n : 1));
-  chol_task_list(NT, ahead, raw.data(), &n);
+  chol_task_list(NT, ahead, raw.data(), &n, first);
   std::vector<Task> t(n);
   for (int q = 0; q < n; q++) t[q] = {raw[3 * q], raw[3 * q + 1], raw[3 * q + 2]};
   return t;
@@ -77,9 +78,23 @@ int check_order(int NT, int ahead) {
   return bad;
 }
 
+// an envelope: first[i] <= i - 2, the last row dense, otherwise a band of
+// random width (some rows reaching back further)
+std::vector<int> random_envelope(int NT, unsigned seed) {
+  std::vector<int> f(NT, 0);
+  unsigned x = seed * 2654435761u + 12345u;
+  for (int i = 0; i < NT; i++) {
+    x = x * 1103515245u + 12345u;
+    const int back = 2 + (int)((x >> 16) % 6) + (((x >> 8) % 7) == 0 ? (int)((x >> 20) % 20) : 0);
+    f[i] = i - back < 0 ? 0 : i - back;
+  }
+  if (NT > 0) f[NT - 1] = 0;
+  return f;
+}
+
 // true when every task of the simulated launch completes
-bool simulate(int NT, int ahead, int W) {
-  const std::vector<Task> t = tasks_of(NT, ahead);
+bool simulate(int NT, int ahead, int W, const int* first = nullptr) {
+  const std::vector<Task> t = tasks_of(NT, ahead, first);
   const int nt = (int)t.size();
   // nodes: tasks 0..nt-1, then chain steps nt..nt+NT-1
   std::map<std::tuple<int, int, int>, int> id;
@@ -90,13 +105,14 @@ bool simulate(int NT, int ahead, int W) {
   for (int q = 0; q < nt; q++) {
     const Task& x = t[q];
     std::vector<int>& d = deps[q];
+    auto k0 = [&](int i, int j) { return first ? std::max(first[i], first[j]) : 0; };
     if (x.kind == 0) {  // C_ij -= sum_k L_ik L_jk^T, then the inverse of step j
-      for (int k = 0; k < x.j; k++) d.push_back(tile(x.i, k)), d.push_back(tile(x.j, k));
+      for (int k = k0(x.i, x.j); k < x.j; k++) d.push_back(tile(x.i, k)), d.push_back(tile(x.j, k));
       d.push_back(chain(x.j));
     } else if (x.kind == 1) {  // pre-update of (j+1, j)
-      for (int k = 0; k < x.j; k++) d.push_back(tile(x.j + 1, k)), d.push_back(tile(x.j, k));
+      for (int k = k0(x.j + 1, x.j); k < x.j; k++) d.push_back(tile(x.j + 1, k)), d.push_back(tile(x.j, k));
     } else if (x.kind == 2) {  // pre-update of (j+1, j+1)
-      for (int k = 0; k < x.j; k++) d.push_back(tile(x.j + 1, k));
+      for (int k = k0(x.j + 1, x.j + 1); k < x.j; k++) d.push_back(tile(x.j + 1, k));
     } else {  // sub-diagonal tile: the pre-update and the inverse of step j
       d.push_back(id.at({1, x.j + 1, x.j}));
       d.push_back(chain(x.j));
@@ -159,6 +175,27 @@ int main() {
       if (!simulate(NT, chol_lookahead(workers), workers)) {
         std::printf("deadlock: NT=%d workers=%d ahead=%d\n", NT, workers, chol_lookahead(workers));
         bad++;
+      }
+    }
+  // envelopes (banded rows, a dense last row): only tiles inside get a task,
+  // every tile a task reads is inside, and the launch completes
+  for (int NT : {5, 13, 40, 94})
+    for (unsigned seed = 1; seed <= 4; seed++) {
+      const std::vector<int> f = random_envelope(NT, seed);
+      for (const Task& x : tasks_of(NT, 12, f.data()))
+        if (x.kind == 0 && f[x.i] > x.j) {
+          std::printf("NT=%d seed=%u: task for tile (%d, %d) outside the envelope\n", NT, seed, x.i, x.j);
+          bad++;
+        }
+      for (int W : {1, 2, 7, 20, 60, 255}) {
+        int n = 0;
+        chol_task_list(NT, 0, nullptr, &n, f.data());
+        const int workers = W < n ? W : n;
+        if (workers < 1) continue;
+        if (!simulate(NT, chol_lookahead(workers), workers, f.data())) {
+          std::printf("deadlock: NT=%d seed=%u workers=%d (envelope)\n", NT, seed, workers);
+          bad++;
+        }
       }
     }
   // the simulation does detect a queue that can deadlock

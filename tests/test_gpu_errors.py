@@ -14,8 +14,8 @@ from helpers import small_cfg
 pytestmark = pytest.mark.gpu
 
 
-# 64 allocations when the problem has no per-camera intrinsics (indices 0..63)
-@pytest.mark.parametrize("k", [0, 1, 7, 20, 33, 45, 60, 63])
+# 65 allocations when the problem has no per-camera intrinsics (indices 0..64)
+@pytest.mark.parametrize("k", [0, 1, 7, 20, 33, 45, 60, 63, 64])
 def test_each_allocation_failure_returns_enomem(k):
     g = generate_ba(small_cfg(n_cameras=6, n_points=80, k_min=3, k_max=4, seed=31))
     os.environ["NSN_TEST_FAIL_ALLOC"] = str(k)

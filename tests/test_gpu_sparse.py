@@ -1,0 +1,115 @@
+"""The sparse-S path: S stored and factorised over its tile envelope (tile
+row i from its first non-zero tile column; the factor has no fill outside
+it).
+  * The tiles outside the envelope are exact zeros of S and of the factor,
+    so the envelope layout and the dense one (NSN_DENSE_S=1) give the same
+    increments bit for bit (the dense layout is the one pinned against the
+    oracle at full size): on the 1000-camera ring (about half the tiles) and
+    on an open camera path.
+  * An open path of 4000 cameras / 400k points / 2.4M observations (n =
+    23 994, about 3 % of the dense tiles): the FP64 increment solves the
+    oracle's damped normal equations to a componentwise backward error of
+    1e-12, the mixed one to 1e-10 after its refinement (the size-independent
+    check of the full-size tests).
+  * A 300-camera open path against the oracle's own solve (its pivoted LDL^T
+    of the dense S): increments to 1e-9 and the LM log to 1e-9."""
+import os
+
+import numpy as np
+import pytest
+
+from helpers import oracle_blocks, relerr
+from nelson_amd import CONFIGS, BAProblem, SolverCholeskySchur, generate_ba
+from oracle.oracle import OracleBA
+from test_gpu_fullsize import _backward_error
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem(g, precision="fp64", dense=False):
+    old = os.environ.get("NSN_DENSE_S")
+    os.environ["NSN_DENSE_S"] = "1" if dense else "0"
+    try:
+        return BAProblem.from_generated(g, precision=precision)
+    finally:
+        if old is None:
+            del os.environ["NSN_DENSE_S"]
+        else:
+            os.environ["NSN_DENSE_S"] = old
+
+
+def _increments(p, dampings):
+    p.update(True)
+    s = SolverCholeskySchur()
+    s.init(p)
+    out = []
+    for rel, ab in dampings:
+        assert s.computeIncrement(rel, ab * s.maxAbsHDiag())
+        out.append(np.concatenate(s.incrementVector()))
+    return out
+
+
+@pytest.mark.parametrize("name,cfg", [
+    ("ring", dict(CONFIGS["cfg2"])),
+    ("path", dict(CONFIGS["seq"], n_cameras=600, n_points=60000, arc=30, seed=21)),
+])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_envelope_equals_dense_bitwise(name, cfg, precision):
+    g = generate_ba(cfg)
+    pe, pd = _problem(g, precision), _problem(g, precision, dense=True)
+    assert pd.n_tiles == pd.n_tiles_dense
+    assert pe.n_tiles < pd.n_tiles, (pe.n_tiles, pd.n_tiles)
+    print(f"\n{name}: {pe.n_tiles} of {pd.n_tiles} tiles")
+    damp = [(0.0, 1e-4), (1e-3, 0.0)]
+    for a, b in zip(_increments(pe, damp), _increments(pd, damp)):
+        assert np.array_equal(a, b)
+
+
+@pytest.fixture(scope="module")
+def seq():
+    g = generate_ba(CONFIGS["seq"])
+    o = OracleBA.from_generated(g)
+    o.update(True)
+    return g, oracle_blocks(o)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_open_path_4000_cameras_solves_normal_equations(seq, precision):
+    g, Ho = seq
+    p = BAProblem.from_generated(g, precision=precision)
+    assert p.n_tiles < 0.05 * p.n_tiles_dense, (p.n_tiles, p.n_tiles_dense)
+    p.update(True)
+    s = SolverCholeskySchur()
+    s.init(p)
+    mu = 1e-4 * s.maxAbsHDiag()
+    for rel, ab in ((0.0, mu), (1e-3, 0.0)):
+        assert s.computeIncrement(rel, ab)
+        dxu, dxv = s.incrementVector()
+        w = _backward_error(Ho, dxu, dxv, rel, ab)
+        print(f"\n{precision} n={p.n_s} tiles {p.n_tiles}/{p.n_tiles_dense} rel={rel}: backward error {w:.2e}")
+        assert w <= (1e-12 if precision == "fp64" else 1e-10), w
+
+
+def test_open_path_increment_and_lm_vs_oracle():
+    from nelson_amd import LevenbergMarquardt, LevenbergMarquardtSettings
+    g = generate_ba(dict(CONFIGS["seq"], n_cameras=300, n_points=30000, arc=20, seed=23))
+    o = OracleBA.from_generated(g)
+    ro = o.lm_solve(maxNumIt=3, dx_cap=3 * (6 * o.NU + 3 * o.NV))
+    p = BAProblem.from_generated(g)
+    assert p.n_tiles < 0.5 * p.n_tiles_dense
+    p.update(True)
+    s = SolverCholeskySchur()
+    s.init(p)
+    mu0 = 1e-3 * s.maxAbsHDiag()
+    assert abs(mu0 - ro["log"][1][3]) <= 1e-12 * mu0
+    assert s.computeIncrement(0.0, mu0)
+    got = np.concatenate(s.incrementVector())
+    assert relerr(got, ro["dx"][0]) <= 1e-9, relerr(got, ro["dx"][0])
+    p2 = BAProblem.from_generated(g)
+    lm = LevenbergMarquardt(LevenbergMarquardtSettings(maxNumIt=3))
+    lm.solve(p2)
+    log = lm.stats()
+    assert len(log) == len(ro["log"])
+    for a, b in zip(log, ro["log"]):
+        assert a[0] == b[0] and a[1] == b[1]
+        assert abs(a[2] - b[2]) <= 1e-9 * abs(b[2]) and abs(a[3] - b[3]) <= 1e-9 * abs(b[3])
