@@ -350,6 +350,9 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
   tv.n = n;
   tv.TB = kTB;
   tv.NT = (n + 1 + kTB - 1) / kTB;
+  if (tv.NT > kMaxTileRows)
+    return fail(NSN_EINVAL, "S of order %d has %d tile rows; the triangular solves keep at most %d", n, tv.NT,
+                kMaxTileRows);
   TileProfile& prof = h->prof;
   tile_envelope(tv.NT, sb_i, sb_j, h->uoff_i_host, h->uoff_j_host, prof);
   tv.ntiles = prof.ntiles;

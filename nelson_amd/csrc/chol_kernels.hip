@@ -292,6 +292,7 @@ __global__ __launch_bounds__(256) void k_chol(TileView tv, T* __restrict__ tiles
 // forward solve of bS that the factorisation leaves there).
 constexpr int kNear = 2;
 constexpr int kMaxOwned = 16;  // destination blocks per workgroup
+constexpr int kLdsFirst = kMaxTileRows;  // tile rows of the envelope kept in LDS
 
 __device__ __forceinline__ bool is_unset(double v) { return __builtin_bit_cast(unsigned long long, v) == kUnsetBits; }
 
@@ -330,16 +331,28 @@ __global__ __launch_bounds__(256) void k_trsv(TileView tv, const T* __restrict__
   __shared__ __align__(16) double xs[2][TB];
   __shared__ __align__(16) double xr[kNear][TB];
   __shared__ double acc[kMaxOwned][TB];
+  // the envelope (first tile column and slot offset per tile row) in LDS:
+  // every tile address and the destination loop's test of every (source,
+  // destination) pair read it, and a global read would put a memory round
+  // trip before each tile load
+  __shared__ int sfirst[kLdsFirst], soff[kLdsFirst];
   const int n = tv.n, NT = tv.NT;
+  for (int i = threadIdx.x; i < NT; i += blockDim.x) {
+    sfirst[i] = tv.first[i];
+    soff[i] = tv.row_off[i];
+  }
+  __syncthreads();
+  auto first_of = [&](int i) { return sfirst[i]; };
+  auto slot_of = [&](int i, int j) { return (long long)soff[i] + (j - sfirst[i]); };
   const int NR = (n + TB - 1) / TB;
   const size_t T2 = (size_t)TB * TB;
   const int step = BWD ? -1 : 1;  // solve order
   auto blk = [&](int s) { return BWD ? NR - 1 - s : s; };
   auto valid = [&](int b) { return b >= 0 && b < NR; };
   // the tile applied to x_k for destination i (L_ik forward, L_ki^T backward)
-  auto ntile = [&](int i, int k) { return tiles + (size_t)(BWD ? tv.slot(k, i) : tv.slot(i, k)) * T2; };
+  auto ntile = [&](int i, int k) { return tiles + (size_t)(BWD ? slot_of(k, i) : slot_of(i, k)) * T2; };
   // the tile applied to x_k for destination i is inside the envelope
-  auto ntile_has = [&](int i, int k) { return BWD ? tv.has(k, i) : tv.has(i, k); };
+  auto ntile_has = [&](int i, int k) { return BWD ? i >= first_of(k) : k >= first_of(i); };
   const bool own = Gemv::owner();
   const int i0 = Gemv::index(0), i1 = Gemv::index(1);
   if (blockIdx.x == 0) {
@@ -350,7 +363,7 @@ __global__ __launch_bounds__(256) void k_trsv(TileView tv, const T* __restrict__
     Gemv gd, gn[kNear];
     double r0 = 0.0, r1 = 0.0, f0 = 0.0, f1 = 0.0;
     auto rhs_at = [&](int row) {
-      return rhs ? rhs[row] : (double)tiles[(size_t)tv.slot(n / TB, row / TB) * T2 + (row % TB) * TB + (n % TB)];
+      return rhs ? rhs[row] : (double)tiles[(size_t)slot_of(n / TB, row / TB) * T2 + (row % TB) * TB + (n % TB)];
     };
     auto load_near = [&](int s) {
       const int k = blk(s);
@@ -418,8 +431,9 @@ __global__ __launch_bounds__(256) void k_trsv(TileView tv, const T* __restrict__
   // solve index - kNear - 1 whose tile is inside the envelope; the tile of
   // the next product is loaded while the current one runs
   auto s_last = [&](int m) { return first + G * m - kNear - 1; };
+  const int s_end = s_last(nown - 1);  // no product after the last destination's last source
   auto find_next = [&](int& ps, int& pm) {  // the first product at or after (ps, pm)
-    for (; ps < NR; ps++, pm = 0) {
+    for (; ps <= s_end; ps++, pm = 0) {
       for (; pm < nown; pm++)
         if (ps <= s_last(pm) && ntile_has(blk(first + G * pm), blk(ps))) return true;
     }
@@ -431,6 +445,18 @@ __global__ __launch_bounds__(256) void k_trsv(TileView tv, const T* __restrict__
   if (have) gg.load(ntile(blk(first + G * cm), blk(cs)));
   int m0 = 0;  // first owned destination still open
   int pc = 0;  // sources polled so far (the parity of the LDS buffer)
+  // publishes the destinations whose last far source is s (a destination
+  // with no tile of its row inside the envelope publishes 0)
+  auto publish_done = [&](int s) {
+    while (m0 < nown && s_last(m0) == s) {
+      const int d = blk(first + G * m0);
+      if (own) {
+        if (d * TB + i0 < n) store_pub(far + d * TB + i0, acc[m0][i0]);
+        if (d * TB + i1 < n) store_pub(far + d * TB + i1, acc[m0][i1]);
+      }
+      m0++;
+    }
+  };
   for (int s = 0; s < NR && m0 < nown; s++) {  // source steps
     if (have && cs == s) {
       const int k = blk(s);
@@ -444,7 +470,7 @@ __global__ __launch_bounds__(256) void k_trsv(TileView tv, const T* __restrict__
       }
       __syncthreads();
       pc++;
-      while (have && cs == s) {
+      for (;;) {
         double v0 = 0.0, v1 = 0.0;
         gg.partial(xb, v0, v1);
         Gemv::reduce(v0, v1);
@@ -452,20 +478,22 @@ __global__ __launch_bounds__(256) void k_trsv(TileView tv, const T* __restrict__
           acc[cm][i0] += v0;
           acc[cm][i1] += v1;
         }
-        cm++;
-        have = find_next(cs, cm);
-        if (have) gg.load(ntile(blk(first + G * cm), blk(cs)));
+        // the next destination with a product at this source
+        int nm = cm + 1;
+        while (nm < nown && !(s <= s_last(nm) && ntile_has(blk(first + G * nm), k))) nm++;
+        if (nm == nown) break;
+        cm = nm;
+        gg.load(ntile(blk(first + G * cm), k));
       }
-    }
-    // destination m0 is complete after its last far source (published even
-    // when no tile of its row is inside the envelope: then its sum is 0)
-    while (m0 < nown && s_last(m0) == s) {
-      const int d = blk(first + G * m0);
-      if (own) {
-        if (d * TB + i0 < n) store_pub(far + d * TB + i0, acc[m0][i0]);
-        if (d * TB + i1 < n) store_pub(far + d * TB + i1, acc[m0][i1]);
-      }
-      m0++;
+      // the sums complete at this source go out before the scan for the
+      // next product, which may pass many sources
+      publish_done(s);
+      cs = s + 1;
+      cm = 0;
+      have = find_next(cs, cm);
+      if (have) gg.load(ntile(blk(first + G * cm), blk(cs)));
+    } else {
+      publish_done(s);
     }
   }
 }
@@ -505,7 +533,7 @@ template <typename T>
 void launch_trsv(const TileView& tv, const T* tiles, const T* inv, bool backward, const double* rhs, double* sol,
                  double* far, DevCtl* ctl, int gate, int max_wg, hipStream_t st) {
   const int NR = (tv.n + 63) / 64;
-  if (NR == 0) return;
+  if (NR == 0 || tv.NT > kLdsFirst) return;  // (the structure stage refuses such an S)
   // the chain plus G workgroups owning the nd destinations with far sources
   const int nd = NR > kNear + 1 ? NR - kNear - 1 : 0;
   int G = nd < max_wg - 1 ? nd : max_wg - 1;

@@ -81,6 +81,10 @@ struct TileView {
   __device__ __forceinline__ bool has(int i, int j) const { return j >= first[i]; }  // for i >= j
 };
 
+// the largest number of tile rows (the triangular solves keep the envelope
+// of every tile row in LDS)
+constexpr int kMaxTileRows = 4096;
+
 // the envelope of a tiled S on the host (the device copy of first / row_off
 // is what TileView points to)
 struct TileProfile {
