@@ -68,7 +68,18 @@ typedef struct {
    * p = -(R X + t)_xy / (R X + t)_z, pixel = f (1 + k1 |p|^2 + k2 |p|^4) p;
    * focal / cx / cy are then unused.  The intrinsics stay constant. */
   const double* intrinsics;
+  /* order of the variable cameras inside the solver (the rows of S):
+   * NSN_ORDER_AS_GIVEN, or NSN_ORDER_RCM (reverse Cuthill-McKee on the camera
+   * graph: a narrow envelope of S for the sparse-S path, the role of the
+   * reference's AMD ordering, DoubleSection::permuteAMD_SchurU,
+   * nelson/DoubleSection.hpp:30-63).  Arrays indexed by variable camera at
+   * this interface (dxu, the exported / imported Hessian) stay in camera
+   * order either way. */
+  int camera_order;
 } nsn_problem_desc;
+
+#define NSN_ORDER_AS_GIVEN 0
+#define NSN_ORDER_RCM 1
 
 typedef struct {
   int n_var_cameras, n_var_points, n_wblocks;  /* NU, NV, non-zero W blocks */

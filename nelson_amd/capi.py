@@ -44,8 +44,12 @@ class ProblemDesc(C.Structure):
         ("cam_fixed", C.POINTER(C.c_ubyte)), ("pt_fixed", C.POINTER(C.c_ubyte)),
         ("focal", C.c_double), ("cx", C.c_double), ("cy", C.c_double),
         ("huber_delta", C.c_double), ("precision", C.c_int), ("device", C.c_int),
-        ("intrinsics", C.POINTER(C.c_double)),
+        ("intrinsics", C.POINTER(C.c_double)), ("camera_order", C.c_int),
     ]
+
+
+NSN_ORDER_AS_GIVEN = 0
+NSN_ORDER_RCM = 1
 
 
 class ProblemInfo(C.Structure):

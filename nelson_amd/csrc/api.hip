@@ -91,6 +91,9 @@ struct nsn_problem {
   long long npairs = 0;
   // host structure
   std::vector<int> cam_var, pt_var, var_cam_user, var_pt_user, w_cam, w_pt, w_ref;
+  // variable camera in solver order <-> interface index (camera order among
+  // the variable cameras); identities unless the cameras are reordered
+  std::vector<int> var_cam_api, api_cam_var;
   DevBuf mem;
   BAView view{};
   Scratch scr{};
@@ -322,12 +325,15 @@ static void tile_envelope(int NT, const std::vector<int>& sb_i, const std::vecto
 int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
   const int C = h->C, P = h->P, E = h->E;
   HostStructure hs;
-  int rc0 = host_structure(d, hs);
+  int rc0 = host_structure(d, hs, true);
   if (rc0) return rc0;
   h->cam_var = hs.cam_var;
   h->pt_var = hs.pt_var;
   h->var_cam_user = hs.var_cam_user;
   h->var_pt_user = hs.var_pt_user;
+  h->var_cam_api = hs.var_cam_api;
+  h->api_cam_var.assign(hs.NU, 0);
+  for (int u = 0; u < hs.NU; u++) h->api_cam_var[hs.var_cam_api[u]] = u;
   h->w_cam = hs.w_cam;
   h->w_pt = hs.w_pt;
   h->w_ref = hs.w_ref;
@@ -841,12 +847,13 @@ static int export_hessian_t(nsn_problem* h, double* U, double* V, double* W, int
   if (W) HIP_TRY(hipMemcpy(hw.data(), h->W, hw.size() * sizeof(T), hipMemcpyDeviceToHost));
   if (bU) HIP_TRY(hipMemcpy(hbu.data(), h->bU, hbu.size() * sizeof(T), hipMemcpyDeviceToHost));
   if (bV) HIP_TRY(hipMemcpy(hbv.data(), h->bV, hbv.size() * sizeof(T), hipMemcpyDeviceToHost));
+  // per-camera arrays in interface order (camera order)
   if (U)
     for (int u = 0; u < NU; u++)
       for (int a = 0; a < 6; a++)
         for (int b = 0; b < 6; b++) {
           const int i = std::min(a, b), j = std::max(a, b);
-          U[36 * (size_t)u + 6 * a + b] = (double)hu[21 * (size_t)u + i * 6 - i * (i - 1) / 2 + (j - i)];
+          U[36 * (size_t)h->var_cam_api[u] + 6 * a + b] = (double)hu[21 * (size_t)u + i * 6 - i * (i - 1) / 2 + (j - i)];
         }
   if (V)
     for (int q = 0; q < NV; q++) {
@@ -859,10 +866,12 @@ static int export_hessian_t(nsn_problem* h, double* U, double* V, double* W, int
     const int k = h->w_ref[r];
     if (W)
       for (int i = 0; i < 18; i++) W[18 * (size_t)r + i] = (double)hw[18 * (size_t)k + i];
-    if (Wi) Wi[r] = h->w_cam[k];
+    if (Wi) Wi[r] = h->var_cam_api[h->w_cam[k]];
     if (Wj) Wj[r] = h->w_pt[k];
   }
-  if (bU) for (size_t i = 0; i < hbu.size(); i++) bU[i] = (double)hbu[i];
+  if (bU)
+    for (int u = 0; u < NU; u++)
+      for (int i = 0; i < 6; i++) bU[6 * (size_t)h->var_cam_api[u] + i] = (double)hbu[6 * (size_t)u + i];
   if (bV) for (size_t i = 0; i < hbv.size(); i++) bV[i] = (double)hbv[i];
   return NSN_OK;
 }
@@ -887,9 +896,12 @@ static int import_hessian_t(nsn_problem* h, const double* U, const double* V, co
   const int NU = h->NU, NV = h->NV, nW = h->nW;
   std::vector<T> hu(21 * (size_t)NU), hv(6 * (size_t)NV), hw(18 * (size_t)nW), hbu(6 * (size_t)NU),
       hbv(3 * (size_t)NV);
-  for (int u = 0; u < NU; u++)
+  // per-camera arrays in interface order (camera order)
+  for (int u = 0; u < NU; u++) {
+    const size_t a = h->var_cam_api[u];
     for (int i = 0; i < 6; i++)
-      for (int j = i; j < 6; j++) hu[21 * (size_t)u + i * 6 - i * (i - 1) / 2 + (j - i)] = (T)U[36 * (size_t)u + 6 * i + j];
+      for (int j = i; j < 6; j++) hu[21 * (size_t)u + i * 6 - i * (i - 1) / 2 + (j - i)] = (T)U[36 * a + 6 * i + j];
+  }
   for (int q = 0; q < NV; q++) {
     const double* s = V + 9 * (size_t)q;
     const double up[6] = {s[0], s[1], s[2], s[4], s[5], s[8]};
@@ -899,7 +911,8 @@ static int import_hessian_t(nsn_problem* h, const double* U, const double* V, co
     const int k = h->w_ref[r];
     for (int i = 0; i < 18; i++) hw[18 * (size_t)k + i] = (T)W[18 * (size_t)r + i];
   }
-  for (size_t i = 0; i < hbu.size(); i++) hbu[i] = (T)bU[i];
+  for (int u = 0; u < NU; u++)
+    for (int i = 0; i < 6; i++) hbu[6 * (size_t)u + i] = (T)bU[6 * (size_t)h->var_cam_api[u] + i];
   for (size_t i = 0; i < hbv.size(); i++) hbv[i] = (T)bV[i];
   int rc;
   if ((rc = upload((T*)h->U, hu)) || (rc = upload((T*)h->V, hv)) || (rc = upload((T*)h->W, hw)) ||
@@ -977,8 +990,14 @@ extern "C" int nsn_schur_step(nsn_problem* h, double rel, double abs_, double* d
   h->last_abs = abs_;
   const bool ok = h->hctl->solve_ok != 0;
   if (dxu) {
-    if (ok) HIP_TRY(hipMemcpy(dxu, h->xU, sizeof(double) * 6 * (size_t)h->NU, hipMemcpyDeviceToHost));
-    else std::memset(dxu, 0, sizeof(double) * 6 * (size_t)h->NU);
+    if (ok) {
+      std::vector<double> x(6 * (size_t)h->NU);
+      HIP_TRY(hipMemcpy(x.data(), h->xU, sizeof(double) * x.size(), hipMemcpyDeviceToHost));
+      for (int u = 0; u < h->NU; u++)  // interface order
+        std::memcpy(dxu + 6 * (size_t)h->var_cam_api[u], x.data() + 6 * (size_t)u, 6 * sizeof(double));
+    } else {
+      std::memset(dxu, 0, sizeof(double) * 6 * (size_t)h->NU);
+    }
   }
   if (dxv) {
     if (ok) HIP_TRY(hipMemcpy(dxv, h->xV, sizeof(double) * 3 * (size_t)h->NV, hipMemcpyDeviceToHost));

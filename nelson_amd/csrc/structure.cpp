@@ -8,7 +8,68 @@
 
 namespace nsn {
 
-int host_structure(const nsn_problem_desc* d, HostStructure& s) {
+std::vector<int> rcm_order(const std::vector<std::vector<int>>& adj) {
+  const int n = (int)adj.size();
+  std::vector<int> order;
+  order.reserve(n);
+  std::vector<char> seen(n, 0);
+  std::vector<int> level(n, -1);
+  auto deg = [&](int v) { return (int)adj[v].size(); };
+  // breadth-first levels from s inside its component (over unplaced nodes);
+  // returns the last level's node of least degree and the eccentricity
+  auto far_node = [&](int st, int& ecc) {
+    std::vector<int> q{st};
+    std::vector<int> touched{st};
+    level[st] = 0;
+    for (size_t h = 0; h < q.size(); h++)
+      for (int w : adj[q[h]])
+        if (!seen[w] && level[w] < 0) {
+          level[w] = level[q[h]] + 1;
+          q.push_back(w);
+          touched.push_back(w);
+        }
+    ecc = level[q.back()];
+    int best = q.back();
+    for (int v : q)
+      if (level[v] == ecc && deg(v) < deg(best)) best = v;
+    for (int v : touched) level[v] = -1;
+    return best;
+  };
+  std::vector<int> by_deg(n);
+  for (int i = 0; i < n; i++) by_deg[i] = i;
+  std::stable_sort(by_deg.begin(), by_deg.end(), [&](int a, int b) { return deg(a) < deg(b); });
+  for (int cand : by_deg) {
+    if (seen[cand]) continue;
+    // pseudo-peripheral start: move to a far node while the eccentricity grows
+    int st = cand, ecc = 0;
+    int far = far_node(st, ecc);
+    for (int rep = 0; rep < 4; rep++) {
+      int e2 = 0;
+      const int f2 = far_node(far, e2);
+      if (e2 <= ecc) break;
+      st = far;
+      ecc = e2;
+      far = f2;
+    }
+    const size_t base = order.size();
+    order.push_back(st);
+    seen[st] = 1;
+    for (size_t h = base; h < order.size(); h++) {
+      std::vector<int> nb;
+      for (int w : adj[order[h]])
+        if (!seen[w]) nb.push_back(w);
+      std::stable_sort(nb.begin(), nb.end(), [&](int a, int b) { return deg(a) < deg(b); });
+      for (int w : nb) {
+        seen[w] = 1;
+        order.push_back(w);
+      }
+    }
+  }
+  std::reverse(order.begin(), order.end());
+  return order;
+}
+
+int host_structure(const nsn_problem_desc* d, HostStructure& s, bool allow_order) {
   const int C = d->n_cameras, P = d->n_points, E = d->n_edges;
   s.C = C;
   s.P = P;
@@ -33,6 +94,39 @@ int host_structure(const nsn_problem_desc* d, HostStructure& s) {
   for (int e = 0; e < E; e++) {
     if (d->cam_idx[e] < 0 || d->cam_idx[e] >= C || d->pt_idx[e] < 0 || d->pt_idx[e] >= P)
       return fail(NSN_EINVAL, "edge %d references camera %d / point %d out of range", e, d->cam_idx[e], d->pt_idx[e]);
+  }
+  s.var_cam_api.resize(NU);
+  for (int u = 0; u < NU; u++) s.var_cam_api[u] = u;
+  if (allow_order && d->camera_order == NSN_ORDER_RCM && NU > 2) {
+    // the camera graph: variable cameras sharing a variable point
+    std::vector<std::vector<int>> cams_of(P), adj(NU);
+    for (int e = 0; e < E; e++) {
+      const int cv = s.cam_var[d->cam_idx[e]];
+      if (cv >= 0 && s.pt_var[d->pt_idx[e]] >= 0) cams_of[d->pt_idx[e]].push_back(cv);
+    }
+    for (auto& l : cams_of) {
+      std::sort(l.begin(), l.end());
+      l.erase(std::unique(l.begin(), l.end()), l.end());
+      for (size_t a = 0; a < l.size(); a++)
+        for (size_t b = a + 1; b < l.size(); b++) {
+          adj[l[a]].push_back(l[b]);
+          adj[l[b]].push_back(l[a]);
+        }
+      std::vector<int>().swap(l);
+    }
+    for (auto& l : adj) {
+      std::sort(l.begin(), l.end());
+      l.erase(std::unique(l.begin(), l.end()), l.end());
+    }
+    const std::vector<int> ord = rcm_order(adj);  // solver position -> interface index
+    const std::vector<int> api_user = s.var_cam_user;
+    for (int v = 0; v < NU; v++) {
+      s.var_cam_api[v] = ord[v];
+      s.var_cam_user[v] = api_user[ord[v]];
+      s.cam_var[api_user[ord[v]]] = v;
+    }
+  } else if (d->camera_order != NSN_ORDER_AS_GIVEN && d->camera_order != NSN_ORDER_RCM) {
+    return fail(NSN_EINVAL, "unknown camera_order %d", d->camera_order);
   }
   // point-major order: (point, camera, edge index)
   s.pe.resize(E);
@@ -78,8 +172,10 @@ int host_structure(const nsn_problem_desc* d, HostStructure& s) {
   s.ce.reserve(E);
   for (int e = 0; e < E; e++)
     if (s.cam_var[d->cam_idx[e]] >= 0) s.ce.push_back(e);
+  // grouped by variable camera in solver order
   std::stable_sort(s.ce.begin(), s.ce.end(), [&](int a, int b) {
-    if (d->cam_idx[a] != d->cam_idx[b]) return d->cam_idx[a] < d->cam_idx[b];
+    const int ca = s.cam_var[d->cam_idx[a]], cb = s.cam_var[d->cam_idx[b]];
+    if (ca != cb) return ca < cb;
     return d->pt_idx[a] < d->pt_idx[b];
   });
   s.Ec = (int)s.ce.size();
@@ -105,8 +201,18 @@ int host_structure(const nsn_problem_desc* d, HostStructure& s) {
     std::vector<int> fill(s.cw_ptr.begin(), s.cw_ptr.end() - 1);
     for (int k = 0; k < nW; k++) s.cw_idx[fill[s.w_cam[k]]++] = k;
   }
-  // reference order of W blocks: row-major (camera, point)
-  s.w_ref = s.cw_idx;
+  // reference order of W blocks: row-major (camera, point), cameras in
+  // interface order
+  {
+    std::vector<int> api_var(NU);
+    for (int u = 0; u < NU; u++) api_var[s.var_cam_api[u]] = u;
+    s.w_ref.clear();
+    s.w_ref.reserve(nW);
+    for (int a = 0; a < NU; a++) {
+      const int u = api_var[a];
+      s.w_ref.insert(s.w_ref.end(), s.cw_idx.begin() + s.cw_ptr[u], s.cw_idx.begin() + s.cw_ptr[u + 1]);
+    }
+  }
   // S structure: pairs of blocks sharing a point, grouped by camera pair
   // (MatrixWWtMultiplier::prepare, nelson/MatrixWWtMultiplier.hpp:18-120)
   struct PairRec {
@@ -115,7 +221,12 @@ int host_structure(const nsn_problem_desc* d, HostStructure& s) {
   std::vector<PairRec> pr;
   for (int v = 0; v < NV; v++)
     for (int a = s.wptr[v]; a < s.wptr[v + 1]; a++)
-      for (int b = a + 1; b < s.wptr[v + 1]; b++) pr.push_back({s.w_cam[a], s.w_cam[b], a, b});
+      for (int b = a + 1; b < s.wptr[v + 1]; b++) {
+        // the lower camera (solver order) first: the block of S it gives is
+        // (ci, cj), ci < cj, the sum of Z_a Z_b^T
+        if (s.w_cam[a] < s.w_cam[b]) pr.push_back({s.w_cam[a], s.w_cam[b], a, b});
+        else pr.push_back({s.w_cam[b], s.w_cam[a], b, a});
+      }
   std::stable_sort(pr.begin(), pr.end(),
                    [](const PairRec& x, const PairRec& y) { return x.ci != y.ci ? x.ci < y.ci : x.cj < y.cj; });
   s.npairs = (long long)pr.size();

@@ -15,6 +15,10 @@ struct HostStructure {
   long long npairs = 0;
   std::vector<int> cam_var, pt_var;            // user -> variable index or -1
   std::vector<int> var_cam_user, var_pt_user;  // variable -> user index
+  // variable camera (solver order) -> its index in camera order among the
+  // variable cameras (the order of the interface); the identity unless the
+  // cameras are reordered
+  std::vector<int> var_cam_api;
   // edges in point-major order (user point, then camera)
   std::vector<int> pe;  // user edge index of each slot
   std::vector<int> pt_ptr, pe_cam, pe_wblk;
@@ -26,6 +30,7 @@ struct HostStructure {
   std::vector<int> cam_ptr, ce_pt;
   std::vector<double> ce_obs, ce_w;
   // W blocks grouped by camera; w_ref = the reference's row-major W order
+  // (camera in interface order, then point)
   std::vector<int> cw_ptr, cw_idx, w_ref;
   // S blocks (i <= j; the diagonal ones with empty pair lists) and, for the
   // off-diagonal ones, the (W_i, W_j) block pairs sharing a point
@@ -33,7 +38,13 @@ struct HostStructure {
 };
 
 // fills s from the problem description; NSN_OK or NSN_EINVAL (with the
-// library's error message) for edges out of range
-int host_structure(const nsn_problem_desc* d, HostStructure& s);
+// library's error message) for edges out of range.  allow_order: honour
+// d->camera_order (otherwise the variable cameras keep camera order)
+int host_structure(const nsn_problem_desc* d, HostStructure& s, bool allow_order = false);
+
+// reverse Cuthill-McKee order of a graph given by sorted adjacency lists:
+// order[k] = the node placed k-th (each connected component from a
+// pseudo-peripheral node of least degree, neighbours by increasing degree)
+std::vector<int> rcm_order(const std::vector<std::vector<int>>& adj);
 
 }  // namespace nsn

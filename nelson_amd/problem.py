@@ -171,7 +171,7 @@ class BAProblem:
     """
 
     def __init__(self, n_cameras, n_points, cam_idx, pt_idx, obs, focal, cx, cy, weight=None, cam_fixed=None,
-                 pt_fixed=None, huber_delta=0.0, precision="fp64", device=0, intrinsics=None):
+                 pt_fixed=None, huber_delta=0.0, precision="fp64", device=0, intrinsics=None, camera_order="given"):
         self._lib = capi.lib()
         self.n_cameras, self.n_points = int(n_cameras), int(n_points)
         self.cam_idx = np.ascontiguousarray(cam_idx, np.int32)
@@ -198,6 +198,12 @@ class BAProblem:
         d.cam_fixed = self.cam_fixed.ctypes.data_as(C.POINTER(C.c_ubyte))
         d.pt_fixed = self.pt_fixed.ctypes.data_as(C.POINTER(C.c_ubyte))
         d.focal, d.cx, d.cy, d.huber_delta = focal, cx, cy, huber_delta
+        # "rcm": reverse Cuthill-McKee order of the cameras inside the solver
+        # (a narrow envelope of S); arrays per camera at the interface keep
+        # camera order either way
+        if camera_order not in ("given", "rcm"):
+            raise ValueError("camera_order must be 'given' or 'rcm'")
+        d.camera_order = capi.NSN_ORDER_RCM if camera_order == "rcm" else capi.NSN_ORDER_AS_GIVEN
         d.precision = capi.NSN_MIXED if precision == "mixed" else capi.NSN_FP64
         d.device = device
         d.intrinsics = _dp(self.intrinsics)

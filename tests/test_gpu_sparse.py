@@ -113,3 +113,68 @@ def test_open_path_increment_and_lm_vs_oracle():
     for a, b in zip(log, ro["log"]):
         assert a[0] == b[0] and a[1] == b[1]
         assert abs(a[2] - b[2]) <= 1e-9 * abs(b[2]) and abs(a[3] - b[3]) <= 1e-9 * abs(b[3])
+
+
+def _shuffled(g, seed):
+    """The same problem with the cameras renumbered at random (camera 0, the
+    fixed gauge, keeps its number): the open path's banded S becomes a
+    scattered one in the given order."""
+    rng = np.random.default_rng(seed)
+    C = g["n_cameras"]
+    perm = np.r_[0, 1 + rng.permutation(C - 1)]  # new index of each camera
+    h = dict(g)
+    h["cam_idx"] = perm[g["cam_idx"]].astype(np.int32)
+    for k in ("cams_gt", "cams0"):
+        c = np.zeros_like(g[k].reshape(C, 6))
+        c[perm] = g[k].reshape(C, 6)
+        h[k] = c.reshape(-1)
+    return h
+
+
+def test_rcm_order_recovers_the_envelope_and_keeps_the_interface_order():
+    """Reverse Cuthill-McKee on a camera-shuffled open path: the envelope of S
+    shrinks to that of the unshuffled problem (within 2x) from near dense; the
+    Hessian exported in camera order equals the oracle's; the increments (in
+    camera order) solve the oracle's damped normal equations and equal the
+    given-order solve's to rounding; the LM log follows the oracle's."""
+    from nelson_amd import LevenbergMarquardt, LevenbergMarquardtSettings
+    g0 = generate_ba(dict(CONFIGS["seq"], n_cameras=600, n_points=60000, arc=30, seed=25))
+    g = _shuffled(g0, 5)
+    base = BAProblem.from_generated(g0)
+    given = BAProblem.from_generated(g)
+    rcm = BAProblem.from_generated(g, camera_order="rcm")
+    print(f"\ntiles: open path {base.n_tiles}, shuffled {given.n_tiles}, shuffled + RCM {rcm.n_tiles} "
+          f"(dense {rcm.n_tiles_dense})")
+    assert given.n_tiles > 0.5 * given.n_tiles_dense
+    assert rcm.n_tiles <= 2 * base.n_tiles
+    o = OracleBA.from_generated(g)
+    o.update(True)
+    Ho = oracle_blocks(o)
+    rcm.update(True)
+    Hg = rcm.export_hessian()
+    for k in ("U", "V", "W", "bU", "bV"):
+        assert relerr(Hg[k], Ho[k]) <= 1e-11, k
+    assert np.array_equal(Hg["Wi"], Ho["Wi"]) and np.array_equal(Hg["Wj"], Ho["Wj"])
+    given.update(True)
+    s1, s2 = SolverCholeskySchur(), SolverCholeskySchur()
+    s1.init(given)
+    s2.init(rcm)
+    mu = 1e-4 * s1.maxAbsHDiag()
+    assert s1.computeIncrement(0.0, mu) and s2.computeIncrement(0.0, mu)
+    a, b = s1.incrementVector(), s2.incrementVector()
+    assert relerr(np.concatenate(b), np.concatenate(a)) <= 1e-10
+    w = _backward_error(Ho, b[0], b[1], 0.0, mu)
+    assert w <= 1e-12, w
+    ro = o.lm_solve(maxNumIt=3, dx_cap=3 * (6 * o.NU + 3 * o.NV))
+    p = BAProblem.from_generated(g, camera_order="rcm")
+    lm = LevenbergMarquardt(LevenbergMarquardtSettings(maxNumIt=3))
+    lm.solve(p)
+    log = lm.stats()
+    assert len(log) == len(ro["log"])
+    for x, y in zip(log, ro["log"]):
+        assert x[0] == y[0] and x[1] == y[1]
+        assert abs(x[2] - y[2]) <= 1e-9 * abs(y[2]) and abs(x[3] - y[3]) <= 1e-9 * abs(y[3])
+    # the state comes back in camera order (the oracle's after the same LM)
+    cg, pg = p.get_state_rt()
+    co, po = o.get_state_rt()
+    assert relerr(cg, co) <= 1e-8 and relerr(pg, po) <= 1e-8, (relerr(cg, co), relerr(pg, po))
