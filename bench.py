@@ -46,8 +46,8 @@ def lm_settings(nsn, iters):
                                           epsChi2=-1.0)
 
 
-def run_lm(nsn, g, precision, iters, warmup, refine=None):
-    p = nsn.BAProblem.from_generated(g, precision=precision)
+def run_lm(nsn, g, precision, iters, warmup, refine=None, camera_order="given"):
+    p = nsn.BAProblem.from_generated(g, precision=precision, camera_order=camera_order)
     if refine is not None:
         p.set_refinement(refine)
     if warmup > 0:
@@ -76,6 +76,48 @@ def chol_timing(nsn, p, reps=3):
         t = s.phase_times_ms()
         acc = t if acc is None else {k: acc[k] + t[k] for k in t}
     return {k: v / reps for k, v in acc.items()}
+
+
+def sparse_timing(nsn, iters=10):
+    """The sparse-S path on a banded problem: 4000 cameras along an open path,
+    400k points, 2.4M observations (S of order 23 994 with about 3 % of its
+    64x64 tiles inside the envelope): LM iterations/s in FP64 and mixed
+    precision with the cameras shuffled and reordered by reverse
+    Cuthill-McKee, the Schur phases, and the factorisation with the dense
+    tile layout for comparison (NSN_DENSE_S=1)."""
+    import numpy as np
+    g = nsn.generate_ba(dict(nsn.CONFIGS["seq"]))
+    # cameras renumbered at random (camera 0, the gauge, stays): the solver's
+    # own ordering has to find the band
+    rng = np.random.default_rng(3)
+    C = g["n_cameras"]
+    perm = np.r_[0, 1 + rng.permutation(C - 1)]
+    g["cam_idx"] = perm[g["cam_idx"]].astype(np.int32)
+    for k in ("cams_gt", "cams0"):
+        c = np.zeros_like(g[k].reshape(C, 6))
+        c[perm] = g[k].reshape(C, 6)
+        g[k] = c.reshape(-1)
+    out = {"workload": f"BA {C} cameras along an open path (shuffled, reverse Cuthill-McKee order) / "
+                       f"{g['n_points']} points / {g['n_edges']} observations, S {6 * (C - 1)}x{6 * (C - 1)}"}
+    for prec in ("fp64", "mixed"):
+        p = nsn.BAProblem.from_generated(g, precision=prec, camera_order="rcm")
+        out["tiles"], out["tiles_dense"] = p.n_tiles, p.n_tiles_dense
+        _, r = run_lm(nsn, g, prec, iters, 1, camera_order="rcm")
+        out[prec] = {k: r[k] for k in ("it_per_s", "ms_per_solve", "iterations", "final_chi2")}
+        out[prec]["phases_ms"] = chol_timing(nsn, p)
+    old = os.environ.get("NSN_DENSE_S")
+    os.environ["NSN_DENSE_S"] = "1"
+    try:
+        pd = nsn.BAProblem.from_generated(g, camera_order="rcm")
+    finally:
+        if old is None:
+            del os.environ["NSN_DENSE_S"]
+        else:
+            os.environ["NSN_DENSE_S"] = old
+    out["fp64_dense_layout_phases_ms"] = chol_timing(nsn, pd, reps=2)
+    out["fp64_factorise_speedup_vs_dense_layout"] = (out["fp64_dense_layout_phases_ms"]["factorize"] /
+                                                    out["fp64"]["phases_ms"]["factorize"])
+    return out
 
 
 def adjoint_timing(nsn):
@@ -260,6 +302,7 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-sparse", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--only-ladybug", action="store_true",
                     help="only the BA sample's workload on the ladybug graph (its own JSON line)")
@@ -295,7 +338,9 @@ def main():
     ph = chol_timing(nsn, p64)
     log(f"bench: fp64 phases (ms) {ph}")
     n = p64.n_s
-    flops = n ** 3 / 3.0
+    # the factorisation's operations over the tile envelope of S (fewer than
+    # the dense n^3/3 where tiles are zero; the library counts them)
+    flops = p64.factor_flops
     chol_s = ph["factorize"] / 1e3
     achieved = flops / chol_s / 1e12
     traffic = None
@@ -310,8 +355,10 @@ def main():
                 "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
                 "traffic_note": "HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, copy-calibrated), from "
                                 "profiles/traffic_k_chol_cfg2_fp64.json",
-                "kernel": "k_chol<double,64> (tile Cholesky of S, n=%d, n^3/3 flop)" % n,
-                "kernel_ms": ph["factorize"]}
+                "kernel": "k_chol<double,64> (tile Cholesky of S over its tile envelope, n=%d, %d of %d tiles, "
+                          "%.3g flop)" % (n, p64.n_tiles, p64.n_tiles_dense, flops),
+                "kernel_ms": ph["factorize"],
+                "dense_equivalent_tflops": n ** 3 / 3.0 / chol_s / 1e12}
     mixed = None
     mixed_small = None
     mixed_ir1 = None
@@ -346,6 +393,10 @@ def main():
     if not args.no_batch:
         batch = batch_timing(nsn)
         log(f"bench: batch {batch}")
+    sparse = None
+    if not args.no_sparse:
+        sparse = sparse_timing(nsn)
+        log(f"bench: sparse {sparse}")
     lb = ladybug_problem(nsn)
     ladybug = ladybug_timing(nsn, lb)
     log(f"bench: ladybug {ladybug}")
@@ -395,6 +446,7 @@ def main():
         "mixed_ir1": mixed_ir1,
         "adjoint": adjoint,
         "batch": batch,
+        "sparse": sparse,
         "ladybug": ladybug,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -89,6 +89,9 @@ typedef struct {
   int n_tiles;                                 /* stored 64x64 tiles of [S; bS^T]: its envelope (the
                                                   sparse S path; fill stays inside it) */
   int n_tiles_dense;                           /* NT (NT + 1) / 2, the dense count */
+  double factor_flops;                         /* floating-point operations of the tile Cholesky over
+                                                  the envelope (tile updates 2*64^3 each, triangular
+                                                  solves 64^3, tile factorisations 64^3/3) */
 } nsn_problem_info;
 
 /* LevenbergMarquardtSettings (nelson/LevenbergMarquardt.h:18-34) */

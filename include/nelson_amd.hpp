@@ -280,6 +280,15 @@ class BASection {
   void setHuber(double delta) { huber_ = delta; }
 
   void parametersReady() { ready_ = true; }
+  // the counterpart of permuteAMD_SchurU (nelson/DoubleSection.hpp:30-63) for
+  // this section: the solver orders the cameras by reverse Cuthill-McKee on
+  // the camera graph, so that S is stored and factorised over a narrow tile
+  // envelope.  Unlike the reference, the increment, b and the Hessian blocks
+  // stay in camera order at the interface; only the work changes.
+  void permuteRCM_SchurU() {
+    if (!ready_ || h_) throw Error(NSN_ESTATE, "permuteRCM_SchurU outside parametersReady .. structureReady");
+    order_ = NSN_ORDER_RCM;
+  }
   // one reprojection edge (EdgeBinaryUV) between camera i and point j
   void addEdge(int i, int j, double u, double v, double weight = 1.0) {
     if (!ready_) throw Error(NSN_ESTATE, "addEdge before parametersReady");
@@ -327,6 +336,7 @@ class BASection {
     d.huber_delta = huber_;
     d.precision = (int)p;
     d.device = device;
+    d.camera_order = order_;
     check(nsn_problem_create(&d, &h_), "nsn_problem_create");
     if (!priCam_.empty() || !priPt_.empty())
       check(nsn_set_priors(h_, (int)priCam_.size(), priCam_.data(), priCamP_.data(), priCamW_.data(),
@@ -403,6 +413,7 @@ class BASection {
  private:
   double f_, cx_, cy_, huber_ = 0.0, chi2_ = 0.0;
   bool ready_ = false;
+  int order_ = NSN_ORDER_AS_GIVEN;
   std::vector<Camera> cams_;
   std::vector<std::array<double, 3>> pts_;
   std::vector<unsigned char> camFixed_, ptFixed_;

@@ -55,7 +55,7 @@ NSN_ORDER_RCM = 1
 class ProblemInfo(C.Structure):
     _fields_ = [("n_var_cameras", C.c_int), ("n_var_points", C.c_int), ("n_wblocks", C.c_int),
                 ("n_s", C.c_int), ("n_s_blocks", C.c_int), ("n_s_pairs", C.c_longlong), ("n_tiles", C.c_int),
-                ("n_tiles_dense", C.c_int)]
+                ("n_tiles_dense", C.c_int), ("factor_flops", C.c_double)]
 
 
 class LMSettings(C.Structure):

@@ -87,6 +87,7 @@ struct nsn_problem {
   double f = 0, cx = 0, cy = 0, huber = 0;
   TileView tv{};
   TileProfile prof;  // the host copy of tv's envelope
+  double factor_flops = 0.0;
   int nsb = 0;
   long long npairs = 0;
   // host structure
@@ -363,6 +364,20 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
   tile_envelope(tv.NT, sb_i, sb_j, h->uoff_i_host, h->uoff_j_host, prof);
   tv.ntiles = prof.ntiles;
   h->tv = tv;
+  {
+    // the tile Cholesky's operations over the envelope: each tile update
+    // C -= A B^T is 2 TB^3 (TB^3 on a diagonal tile), each off-diagonal
+    // tile's triangular solve TB^3, each diagonal tile's factorisation TB^3/3
+    const double t3 = (double)kTB * kTB * kTB;
+    double f = 0.0;
+    for (int i = 0; i < tv.NT; i++)
+      for (int j = prof.first[i]; j <= i; j++) {
+        const int k0 = std::max(prof.first[i], prof.first[j]);
+        const double terms = std::max(0, j - k0);
+        f += i == j ? terms * t3 + t3 / 3.0 : terms * 2.0 * t3 + t3;
+      }
+    h->factor_flops = f;
+  }
   std::vector<int> tile_order(2 * (size_t)tv.ntiles);
   for (int i = 0; i < tv.NT; i++)
     for (int j = prof.first[i]; j <= i; j++) {
@@ -722,6 +737,7 @@ extern "C" int nsn_problem_info_get(nsn_problem* h, nsn_problem_info* info) {
   info->n_s_pairs = h->npairs;
   info->n_tiles = h->tv.ntiles;
   info->n_tiles_dense = h->tv.NT * (h->tv.NT + 1) / 2;
+  info->factor_flops = h->factor_flops;
   return NSN_OK;
 }
 

@@ -215,6 +215,7 @@ class BAProblem:
         self.NU, self.NV, self.nW = info.n_var_cameras, info.n_var_points, info.n_wblocks
         self.n_s, self.n_s_blocks, self.n_s_pairs = info.n_s, info.n_s_blocks, info.n_s_pairs
         self.n_tiles, self.n_tiles_dense = info.n_tiles, info.n_tiles_dense
+        self.factor_flops = info.factor_flops
         self._chi2 = None
 
     @classmethod

@@ -3,7 +3,8 @@
 // (nelson/tests/testDoubleSectionCommon.hpp:55-130): Config 1's BA problem
 // (10 cameras, 500 points, 2000 observations) built through parameterU /
 // parameterV / addEdge / structureReady, one Schur step, an LM solve and a
-// GN solve.  Prints one JSON object per solve on stdout; the GPU test
+// GN solve, then a step and an LM solve with the cameras ordered by
+// permuteRCM_SchurU.  Prints one JSON object per solve on stdout; the GPU test
 // compares them with the oracle.
 #include <cmath>
 #include <cstdio>
@@ -15,7 +16,7 @@ using namespace nelson_amd;
 
 static void build(BASection& pss, const std::vector<double>& cams, const std::vector<double>& pts,
                   const std::vector<int>& ci, const std::vector<int>& pi, const std::vector<double>& obs, int C,
-                  int P, bool priors = false) {
+                  int P, bool priors = false, bool rcm = false) {
   pss.setNumParameters(C, P);
   for (int i = 0; i < C; i++)
     for (int k = 0; k < 3; k++) {
@@ -27,6 +28,7 @@ static void build(BASection& pss, const std::vector<double>& cams, const std::ve
   pss.setFixedU(0);  // gauge of the BA sample (BASchurPerfomances.cpp:245-246)
   pss.setFixedV(0);
   pss.parametersReady();
+  if (rcm) pss.permuteRCM_SchurU();
   for (size_t e = 0; e < ci.size(); e++) pss.addEdge(ci[e], pi[e], obs[2 * e], obs[2 * e + 1]);
   if (priors) {
     for (int c : {0, 2, 5, 7}) {
@@ -110,6 +112,30 @@ int main() {
       lm.settings().maxNumIt = 20;
       const TerminationReason tc = lm.solve(pss);
       print_log("lm_priors", tc, lm.numIterations(), pss.update(false), lm.stats());
+    }
+    {
+      // the cameras ordered by reverse Cuthill-McKee inside the solver: the
+      // increment stays in camera order, so the same values as "step"
+      BASection pss(g.focal, g.cx, g.cy);
+      build(pss, c0, p0, ci, pi, obs, C, P, false, true);
+      pss.update(true);
+      SolverCholeskySchur solver;
+      solver.init(pss.hessian().H(), pss.hessian().b());
+      const bool ok = solver.computeIncrement(pss.hessian().H(), pss.hessian().b(), 0.0, 62.0);
+      std::printf("{\"solve\": \"step_rcm\", \"ok\": %s, \"sq_norm\": %.17g, \"du0\": %.17g, \"du\": [",
+                  ok ? "true" : "false", solver.incrementVectorSquaredNorm(), solver.incrementVector().bU()[0]);
+      for (int i = 0; i < C - 1; i++)
+        for (int k = 0; k < 6; k++)
+          std::printf("%s%.17g", i + k ? ", " : "", solver.incrementVector().bU().segment(i)[k]);
+      std::printf("]}\n");
+    }
+    {
+      BASection pss(g.focal, g.cx, g.cy);
+      build(pss, c0, p0, ci, pi, obs, C, P, false, true);
+      LevenbergMarquardt<SolverCholeskySchur> lm;
+      lm.settings().maxNumIt = 20;
+      const TerminationReason tc = lm.solve(pss);
+      print_log("lm_rcm", tc, lm.numIterations(), pss.update(false), lm.stats());
     }
   } catch (const Error& e) {
     std::fprintf(stderr, "nelson_amd error %d: %s\n", e.code(), e.what());

@@ -78,3 +78,26 @@ def test_cpp_lm_with_unary_edges(results):
     d = results["lm_priors"]
     detail = _compare_logs([tuple(e) for e in d["log"]], ro["log"], "cpp lm priors")
     assert abs(d["final_chi2"] - ro["final_chi2"]) <= 1e-9 * ro["final_chi2"], detail
+
+
+def test_cpp_rcm_order(results):
+    """permuteRCM_SchurU: the solver orders the cameras by reverse
+    Cuthill-McKee; the increment comes back in camera order, so the step equals
+    the oracle's (which keeps the given order) entry by entry to 1e-9, and the
+    LM log matches it like the unordered solve."""
+    import numpy as np
+    g = generate_ba(CONFIGS["cfg1"])
+    o = OracleBA.from_generated(g)
+    o.update(True)
+    ro = o.schur_step(0.0, 62.0)
+    d = results["step_rcm"]
+    assert d["ok"] and ro["ok"]
+    du = np.array(d["du"])
+    assert du.shape == ro["xU"].shape
+    assert np.abs(du - ro["xU"]).max() <= 1e-9 * np.abs(ro["xU"]).max()
+    sq = float((ro["xU"] ** 2).sum() + (ro["xV"] ** 2).sum())
+    assert abs(d["sq_norm"] - sq) <= 1e-9 * sq
+    rl = OracleBA.from_generated(g).lm_solve(maxNumIt=20)
+    dl = results["lm_rcm"]
+    detail = _compare_logs([tuple(e) for e in dl["log"]], rl["log"], "cpp lm rcm")
+    assert abs(dl["final_chi2"] - rl["final_chi2"]) <= 1e-9 * rl["final_chi2"], detail
