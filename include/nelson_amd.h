@@ -69,10 +69,13 @@ typedef struct {
    * focal / cx / cy are then unused.  The intrinsics stay constant. */
   const double* intrinsics;
   /* order of the variable cameras inside the solver (the rows of S):
-   * NSN_ORDER_AS_GIVEN, or NSN_ORDER_RCM (reverse Cuthill-McKee on the camera
+   * NSN_ORDER_AS_GIVEN, NSN_ORDER_RCM (reverse Cuthill-McKee on the camera
    * graph: a narrow envelope of S for the sparse-S path, the role of the
    * reference's AMD ordering, DoubleSection::permuteAMD_SchurU,
-   * nelson/DoubleSection.hpp:30-63).  Arrays indexed by variable camera at
+   * nelson/DoubleSection.hpp:30-63), or NSN_ORDER_ND (nested dissection of
+   * the camera graph into independent parts and separators, aligned to the
+   * 64-row tiles of S: the factorisation runs the independent parts'
+   * diagonal chains at the same time).  Arrays indexed by variable camera at
    * this interface (dxu, the exported / imported Hessian) stay in camera
    * order either way. */
   int camera_order;
@@ -80,6 +83,7 @@ typedef struct {
 
 #define NSN_ORDER_AS_GIVEN 0
 #define NSN_ORDER_RCM 1
+#define NSN_ORDER_ND 2
 
 typedef struct {
   int n_var_cameras, n_var_points, n_wblocks;  /* NU, NV, non-zero W blocks */
@@ -92,6 +96,10 @@ typedef struct {
   double factor_flops;                         /* floating-point operations of the tile Cholesky over
                                                   the envelope (tile updates 2*64^3 each, triangular
                                                   solves 64^3, tile factorisations 64^3/3) */
+  int n_chains;                                /* diagonal chains the factorisation runs at the same
+                                                  time (independent parts of the camera order) */
+  int chain_steps;                             /* its critical path in 64-column diagonal steps (the
+                                                  number of tile rows for a single chain) */
 } nsn_problem_info;
 
 /* LevenbergMarquardtSettings (nelson/LevenbergMarquardt.h:18-34) */
@@ -198,6 +206,13 @@ int nsn_bal_load(const char* path, int capacity, int* n_cameras, int* n_points, 
 int nsn_problem_create(const nsn_problem_desc* d, nsn_problem** out);
 int nsn_problem_destroy(nsn_problem* h);
 int nsn_problem_info_get(nsn_problem* h, nsn_problem_info* info);
+/* The structure stage on the host alone (no device is touched): the figures
+ * nsn_problem_info_get would report for a problem created from d (camera
+ * order, tile envelope, chains, operations), and, when cam_order is not
+ * NULL, the order of the variable cameras inside the solver
+ * (cam_order[k] = the camera placed k-th, [number of variable cameras]).
+ * Lets a caller compare camera orders before allocating anything. */
+int nsn_problem_plan(const nsn_problem_desc* d, nsn_problem_info* info, int* cam_order);
 
 /* state: cameras 6 per camera (angle-axis, translation), points 3 per point;
  * replaces the user's parameterU/parameterV (DoubleSection.h:226-229) */

@@ -50,12 +50,14 @@ class ProblemDesc(C.Structure):
 
 NSN_ORDER_AS_GIVEN = 0
 NSN_ORDER_RCM = 1
+NSN_ORDER_ND = 2
 
 
 class ProblemInfo(C.Structure):
     _fields_ = [("n_var_cameras", C.c_int), ("n_var_points", C.c_int), ("n_wblocks", C.c_int),
                 ("n_s", C.c_int), ("n_s_blocks", C.c_int), ("n_s_pairs", C.c_longlong), ("n_tiles", C.c_int),
-                ("n_tiles_dense", C.c_int), ("factor_flops", C.c_double)]
+                ("n_tiles_dense", C.c_int), ("factor_flops", C.c_double),
+                ("n_chains", C.c_int), ("chain_steps", C.c_int)]
 
 
 class LMSettings(C.Structure):
@@ -113,6 +115,7 @@ SIGNATURES = {
     "nsn_problem_create": (C.c_int, [C.POINTER(ProblemDesc), C.POINTER(_P)]),
     "nsn_problem_destroy": (C.c_int, [_P]),
     "nsn_problem_info_get": (C.c_int, [_P, C.POINTER(ProblemInfo)]),
+    "nsn_problem_plan": (C.c_int, [C.POINTER(ProblemDesc), C.POINTER(ProblemInfo), _I]),
     "nsn_set_state": (C.c_int, [_P, _D, _D]),
     "nsn_get_state": (C.c_int, [_P, _D, _D]),
     "nsn_get_state_rt": (C.c_int, [_P, _D, _D]),

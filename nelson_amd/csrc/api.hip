@@ -31,6 +31,7 @@ namespace {
 
 constexpr int kTB = 64;          // Cholesky tile
 constexpr int kCholMaxWG = 256;  // persistent workgroups of the factorisation
+constexpr int kCholMaxChains = 16;  // of them, at most this many run diagonal chains
 // workgroups of a triangular solve: the chain and at most 255 destination
 // owners, one per CU, so that all of them are resident together whenever one
 // of them owns several destinations
@@ -130,6 +131,7 @@ struct nsn_problem {
   int log_cap = 0;
   bool linearized = false, stepped = false;
   double last_abs = 0.0;
+  int chain_steps = 0;  // the factorisation's critical path in diagonal steps
   hipEvent_t ev[8] = {};
   // the LM sub-iteration captured once per problem (a GN iteration is
   // launched directly: measured on the ladybug workload, replaying it as a
@@ -293,27 +295,37 @@ int check_device_errors(nsn_problem* h) {
 // the tile envelope of [S; bS^T]: tile row i starts at the first tile
 // column that holds a non-zero block of its rows (the S blocks (ci, cj),
 // ci < cj, of variable cameras sharing a point, at rows 6 cj, columns 6 ci;
-// the extra U-U blocks), at least two tiles before the diagonal (the
-// factorisation's chain reads the two sub-diagonal tiles); the last tile
-// row, which holds the right-hand side, is dense.  NSN_DENSE_S=1 in the
-// environment when a problem is created forces a dense layout (for
-// comparisons: the tiles outside the envelope are exact zeros, so both
-// layouts give the same results bit for bit).
+// the extra U-U blocks); the last tile row, which holds the right-hand side,
+// is dense.  A tile row with no block left of its diagonal tile starts a
+// diagonal chain of the factorisation (its step depends on no earlier
+// column: independent parts of a nested-dissection order); every other row
+// keeps the two sub-diagonal tiles of its chain (the chain and the
+// triangular solves read them).  NSN_DENSE_S=1 in the environment when a
+// problem is created forces a dense layout (for comparisons: the tiles
+// outside the envelope are exact zeros, so both layouts give the same
+// results bit for bit).
 static void tile_envelope(int NT, const std::vector<int>& sb_i, const std::vector<int>& sb_j,
                           const std::vector<int>& xi, const std::vector<int>& xj, TileProfile& p) {
   const char* de = std::getenv("NSN_DENSE_S");
   const bool dense = de != nullptr && de[0] != '\0' && de[0] != '0';
-  p.first.assign(NT, 0);
-  for (int i = 0; i < NT; i++) p.first[i] = std::max(0, i - 2);
+  std::vector<int> nz(NT);  // first tile column with a non-zero block
+  for (int i = 0; i < NT; i++) nz[i] = i;
   auto add = [&](int ci, int cj) {
     const int tc = (6 * ci) / kTB;
-    for (int ti = (6 * cj) / kTB; ti <= (6 * cj + 5) / kTB && ti < NT; ti++) p.first[ti] = std::min(p.first[ti], tc);
+    for (int ti = (6 * cj) / kTB; ti <= (6 * cj + 5) / kTB && ti < NT; ti++) nz[ti] = std::min(nz[ti], tc);
   };
   for (size_t b = 0; b < sb_i.size(); b++)
     if (sb_i[b] != sb_j[b]) add(std::min(sb_i[b], sb_j[b]), std::max(sb_i[b], sb_j[b]));
   for (size_t b = 0; b < xi.size(); b++) add(std::min(xi[b], xj[b]), std::max(xi[b], xj[b]));
-  if (NT > 0) p.first[NT - 1] = 0;
-  if (dense) std::fill(p.first.begin(), p.first.end(), 0);
+  if (NT > 0) nz[NT - 1] = 0;
+  if (dense) std::fill(nz.begin(), nz.end(), 0);
+  p.first.assign(NT, 0);
+  int cs = 0;  // start of the current chain
+  for (int i = 0; i < NT; i++) {
+    if (nz[i] == i) cs = i;
+    p.first[i] = std::min(nz[i], std::max(i - 2, cs));
+  }
+  p.chain_start = chol_chain_groups(NT, p.first.data(), kCholMaxChains);
   p.row_off.assign(NT, 0);
   long long off = 0;
   for (int i = 0; i < NT; i++) {
@@ -321,6 +333,31 @@ static void tile_envelope(int NT, const std::vector<int>& sb_i, const std::vecto
     off += i - p.first[i] + 1;
   }
   p.ntiles = (int)off;
+}
+
+// the tile Cholesky's operations over the envelope: each tile update
+// C -= A B^T is 2 TB^3 (TB^3 on a diagonal tile), each off-diagonal tile's
+// triangular solve TB^3, each diagonal tile's factorisation TB^3/3
+static double tile_factor_flops(const TileProfile& prof) {
+  const int NT = (int)prof.first.size();
+  const double t3 = (double)kTB * kTB * kTB;
+  double f = 0.0;
+  for (int i = 0; i < NT; i++)
+    for (int j = prof.first[i]; j <= i; j++) {
+      const int k0 = std::max(prof.first[i], prof.first[j]);
+      const double terms = std::max(0, j - k0);
+      f += i == j ? terms * t3 + t3 / 3.0 : terms * 2.0 * t3 + t3;
+    }
+  return f;
+}
+
+// the factorisation's critical path in diagonal steps: the largest column
+// key of the queue (chol_keys) plus one; NT for a single chain
+static int tile_chain_steps(const TileProfile& prof) {
+  const int NT = (int)prof.first.size();
+  if (NT == 0) return 0;
+  const std::vector<int> key = chol_keys(NT, prof.first.data(), prof.chain_start);
+  return *std::max_element(key.begin(), key.end()) + 1;
 }
 
 int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
@@ -364,20 +401,8 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
   tile_envelope(tv.NT, sb_i, sb_j, h->uoff_i_host, h->uoff_j_host, prof);
   tv.ntiles = prof.ntiles;
   h->tv = tv;
-  {
-    // the tile Cholesky's operations over the envelope: each tile update
-    // C -= A B^T is 2 TB^3 (TB^3 on a diagonal tile), each off-diagonal
-    // tile's triangular solve TB^3, each diagonal tile's factorisation TB^3/3
-    const double t3 = (double)kTB * kTB * kTB;
-    double f = 0.0;
-    for (int i = 0; i < tv.NT; i++)
-      for (int j = prof.first[i]; j <= i; j++) {
-        const int k0 = std::max(prof.first[i], prof.first[j]);
-        const double terms = std::max(0, j - k0);
-        f += i == j ? terms * t3 + t3 / 3.0 : terms * 2.0 * t3 + t3;
-      }
-    h->factor_flops = f;
-  }
+  h->factor_flops = tile_factor_flops(prof);
+  h->chain_steps = tile_chain_steps(prof);
   std::vector<int> tile_order(2 * (size_t)tv.ntiles);
   for (int i = 0; i < tv.NT; i++)
     for (int j = prof.first[i]; j <= i; j++) {
@@ -443,11 +468,14 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
   // flags of the factorisation: ready (ntiles), pre_sub / pre_diag (NT each)
   h->nflags = tv.ntiles + 2 * tv.NT;
   int n_tasks = 0;
-  chol_task_list(tv.NT, 0, nullptr, &n_tasks, prof.first.data());  // the count does not depend on the look-ahead
-  const int ahead = chol_lookahead(std::min(n_tasks, kCholMaxWG - 1));
+  const int nchain = (int)prof.chain_start.size() - 1;
+  // the count does not depend on the look-ahead
+  chol_task_list(tv.NT, 0, nullptr, &n_tasks, prof.first.data(), kCholMaxChains);
+  const int ahead = chol_lookahead(std::min(n_tasks, kCholMaxWG - nchain), nchain);
   std::vector<int> chol_tasks(3 * (size_t)std::max(n_tasks, 1));
-  chol_task_list(tv.NT, ahead, chol_tasks.data(), &n_tasks, prof.first.data());
+  chol_task_list(tv.NT, ahead, chol_tasks.data(), &n_tasks, prof.first.data(), kCholMaxChains);
   h->n_chol_tasks = n_tasks;
+  h->tv.nchain = nchain;
 
   // ---- device allocation ----
   DevBuf& m = h->mem;
@@ -489,7 +517,7 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
   h->Sd = a_T(21 * (size_t)NU);
   h->bS = a_T(6 * (size_t)NU);
   h->tiles = a_T((size_t)tv.ntiles * kTB * kTB);
-  h->tprof = a_int(2 * (size_t)tv.NT);
+  h->tprof = a_int(2 * (size_t)tv.NT + prof.chain_start.size());
   h->inv = a_T((size_t)tv.NT * kTB * kTB);
   h->presub = a_T((size_t)tv.NT * kTB * kTB);
   h->xU = a_dbl(6 * (size_t)NU);
@@ -545,9 +573,11 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
   {
     std::vector<int> tp(prof.first);
     tp.insert(tp.end(), prof.row_off.begin(), prof.row_off.end());
+    tp.insert(tp.end(), prof.chain_start.begin(), prof.chain_start.end());
     if ((rc = upload(h->tprof, tp))) return rc;
     h->tv.first = h->tprof;
     h->tv.row_off = h->tprof + tv.NT;
+    h->tv.chain_start = h->tprof + 2 * (size_t)tv.NT;
   }
   BAView& v = h->view;
   v.C = C; v.P = P; v.E = E; v.NU = NU; v.NV = NV; v.nW = nW;
@@ -738,6 +768,37 @@ extern "C" int nsn_problem_info_get(nsn_problem* h, nsn_problem_info* info) {
   info->n_tiles = h->tv.ntiles;
   info->n_tiles_dense = h->tv.NT * (h->tv.NT + 1) / 2;
   info->factor_flops = h->factor_flops;
+  info->n_chains = h->tv.nchain;
+  info->chain_steps = h->chain_steps;
+  return NSN_OK;
+}
+
+extern "C" int nsn_problem_plan(const nsn_problem_desc* d, nsn_problem_info* info, int* cam_order) {
+  if (!d || !info) return fail(NSN_EINVAL, "null argument");
+  HostStructure hs;
+  int rc = host_structure(d, hs, true);
+  if (rc) return rc;
+  const int n = 6 * hs.NU;
+  const int NT = (n + 1 + kTB - 1) / kTB;
+  if (NT > kMaxTileRows)
+    return fail(NSN_EINVAL, "S of order %d has %d tile rows; the triangular solves keep at most %d", n, NT,
+                kMaxTileRows);
+  TileProfile prof;
+  tile_envelope(NT, hs.sb_i, hs.sb_j, {}, {}, prof);
+  std::memset(info, 0, sizeof(*info));
+  info->n_var_cameras = hs.NU;
+  info->n_var_points = hs.NV;
+  info->n_wblocks = hs.nW;
+  info->n_s = n;
+  info->n_s_blocks = hs.nsb;
+  info->n_s_pairs = hs.npairs;
+  info->n_tiles = prof.ntiles;
+  info->n_tiles_dense = NT * (NT + 1) / 2;
+  info->factor_flops = tile_factor_flops(prof);
+  info->n_chains = (int)prof.chain_start.size() - 1;
+  info->chain_steps = tile_chain_steps(prof);
+  if (cam_order)
+    for (int u = 0; u < hs.NU; u++) cam_order[u] = hs.var_cam_user[u];
   return NSN_OK;
 }
 

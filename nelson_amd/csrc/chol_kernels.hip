@@ -163,13 +163,15 @@ __global__ __launch_bounds__(256) void k_chol(TileView tv, T* __restrict__ tiles
   const int wr = wv & 1, wc = wv >> 1;
   const int NT = tv.NT;
   const size_t T2 = (size_t)TB * TB;
-  if (blockIdx.x == 0) {
-    // ---- the diagonal chain ----
+  if (blockIdx.x < tv.nchain) {
+    // ---- a diagonal chain: the steps chain_start[c] .. chain_start[c+1]-1 ----
+    const int j0 = tv.chain_start[blockIdx.x], j1 = tv.chain_start[blockIdx.x + 1];
     CholAcc<T> acc;  // the next diagonal tile, loaded one step ahead
-    acc_load<T, TB>(acc, tiles, lane, wr, wc);
-    for (int j = 0; j < NT; j++) {
+    acc_load<T, TB>(acc, tiles + (size_t)tv.slot(j0, j0) * T2, lane, wr, wc);
+    bool have_sub = false;  // L_{j,j-1} of the previous step in R0
+    for (int j = j0; j < j1; j++) {
       T* Cg = tiles + (size_t)tv.slot(j, j) * T2;
-      if (j >= 1) acc_gemm_nt<T, T, TB>(acc, R0, R0, true, lane, wr, wc);  // - L_{j,j-1} L_{j,j-1}^T
+      if (have_sub) acc_gemm_nt<T, T, TB>(acc, R0, R0, true, lane, wr, wc);  // - L_{j,j-1} L_{j,j-1}^T
       __syncthreads();
       acc_store<T, double, TB>(acc, R0d, lane, wr, wc);
       __syncthreads();
@@ -184,24 +186,34 @@ __global__ __launch_bounds__(256) void k_chol(TileView tv, T* __restrict__ tiles
       // the inverse (R0d is overwritten only after the next barrier)
 #pragma unroll 4
       for (int e = threadIdx.x; e < TB * TB; e += 256) Cg[e] = (T)R0d[e];
-      if (j + 1 < NT) {
-        // the pre-updated C_{j+1,j+1} (in flight during the product below)
-        if (threadIdx.x == 0) {
-          if (j + 1 >= 2) wait_flag(pre_diag + j + 1, ctl);
-          wait_flag(pre_sub + j, ctl);
+      have_sub = false;
+      if (j + 1 < j1) {
+        const int f1 = tv.first[j + 1];
+        if (f1 <= j) {
+          // the pre-updated C_{j+1,j+1} (in flight during the product below);
+          // its pre-update exists when row j+1 reaches a column before j
+          if (threadIdx.x == 0) {
+            if (f1 < j) wait_flag(pre_diag + j + 1, ctl);
+            wait_flag(pre_sub + j, ctl);
+          }
+          __syncthreads();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+          acc_load<T, TB>(acc, tiles + (size_t)tv.slot(j + 1, j + 1) * T2, lane, wr, wc);
+          // L_{j+1,j} for the next step, from the pre-updated C_{j+1,j}; a
+          // worker forms the same product for the tile storage (CT_SUB)
+          global_to_lds<T, TB>(R0, presub + (size_t)j * T2);
+          __syncthreads();
+          TriProd<T> x;
+          tri_gemm_nt<T, double, TB>(x, R0, R1d, lane, wv);  // C L_jj^-T
+          __syncthreads();
+          tri_store<T, TB>(x, R0, lane, wv);  // kept for step j + 1
+          __syncthreads();
+          have_sub = true;
+        } else {
+          // step j+1 starts another chain of this group: no tile left of its
+          // diagonal, so its diagonal tile is final as assembled
+          acc_load<T, TB>(acc, tiles + (size_t)tv.slot(j + 1, j + 1) * T2, lane, wr, wc);
         }
-        __syncthreads();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        acc_load<T, TB>(acc, tiles + (size_t)tv.slot(j + 1, j + 1) * T2, lane, wr, wc);
-        // L_{j+1,j} for the next step, from the pre-updated C_{j+1,j}; a
-        // worker forms the same product for the tile storage (CT_SUB)
-        global_to_lds<T, TB>(R0, presub + (size_t)j * T2);
-        __syncthreads();
-        TriProd<T> x;
-        tri_gemm_nt<T, double, TB>(x, R0, R1d, lane, wv);  // C L_jj^-T
-        __syncthreads();
-        tri_store<T, TB>(x, R0, lane, wv);  // kept for step j + 1
-        __syncthreads();
       }
     }
     return;
@@ -348,7 +360,6 @@ __global__ __launch_bounds__(256) void k_trsv(TileView tv, const T* __restrict__
   const size_t T2 = (size_t)TB * TB;
   const int step = BWD ? -1 : 1;  // solve order
   auto blk = [&](int s) { return BWD ? NR - 1 - s : s; };
-  auto valid = [&](int b) { return b >= 0 && b < NR; };
   // the tile applied to x_k for destination i (L_ik forward, L_ki^T backward)
   auto ntile = [&](int i, int k) { return tiles + (size_t)(BWD ? slot_of(k, i) : slot_of(i, k)) * T2; };
   // the tile applied to x_k for destination i is inside the envelope
@@ -365,11 +376,18 @@ __global__ __launch_bounds__(256) void k_trsv(TileView tv, const T* __restrict__
     auto rhs_at = [&](int row) {
       return rhs ? rhs[row] : (double)tiles[(size_t)slot_of(n / TB, row / TB) * T2 + (row % TB) * TB + (n % TB)];
     };
+    // the near tile d of solve step s exists (inside the matrix and the
+    // envelope: a tile row that starts a chain has no tile left of its
+    // diagonal, and the one after it none before that chain)
+    auto near_has = [&](int s, int d) {
+      const int k = blk(s);
+      return d <= s && ntile_has(k, k - d * step);
+    };
     auto load_near = [&](int s) {
       const int k = blk(s);
 #pragma unroll
       for (int d = 1; d <= kNear; d++)
-        if (valid(k - d * step)) gn[d - 1].load(ntile(k, k - d * step));
+        if (near_has(s, d)) gn[d - 1].load(ntile(k, k - d * step));
     };
     auto load_vec = [&](int s) {
       const int k = blk(s);
@@ -390,7 +408,7 @@ __global__ __launch_bounds__(256) void k_trsv(TileView tv, const T* __restrict__
       double h0 = 0.0, h1 = 0.0;
 #pragma unroll
       for (int d = 1; d <= kNear; d++)
-        if (d <= s) gn[d - 1].partial(xr[(s - d) % kNear], h0, h1);
+        if (near_has(s, d)) gn[d - 1].partial(xr[(s - d) % kNear], h0, h1);
       if (s + 1 < NR) load_near(s + 1);
       Gemv::reduce(h0, h1);
       if (own) {
@@ -524,7 +542,10 @@ void launch_chol_reset(int* flags, int nflags, int* counters, double* unset0, do
 template <typename T>
 void launch_chol(const TileView& tv, T* tiles, T* inv, T* presub, const int* tasks, int ntasks, int* ready,
                  int* pre_sub, int* pre_diag, int* counter, DevCtl* ctl, int gate, int max_wg, hipStream_t st) {
-  int g = 1 + (ntasks < max_wg - 1 ? ntasks : max_wg - 1);
+  // the chain workgroups, then the workers (the structure stage keeps
+  // nchain well below max_wg)
+  const int nc = tv.nchain;
+  int g = nc + (ntasks < max_wg - nc ? ntasks : max_wg - nc);
   hipLaunchKernelGGL((k_chol<T, 64>), dim3(g), dim3(256), 0, st, tv, tiles, inv, presub, tasks, ntasks, ready,
                      pre_sub, pre_diag, counter, ctl, gate);
 }

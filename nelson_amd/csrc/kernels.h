@@ -67,9 +67,11 @@ struct BAView {
 // tiles (i, j) for first[i] <= j <= i, each TB x TB values, column-major
 // inside, packed row by row (slot of (i, j) = row_off[i] + j - first[i]).
 // A Cholesky factor has no fill outside the envelope of its matrix, so the
-// factor takes the same storage.  A dense S has first[i] = 0.  Every
-// envelope here keeps the two sub-diagonal tiles (first[i] <= i - 2) and a
-// dense last tile row (the right-hand side).
+// factor takes the same storage.  A dense S has first[i] = 0.  A tile row
+// with no tile left of the diagonal (first[i] == i) starts an independent
+// diagonal chain of the factorisation; every other tile row keeps the two
+// sub-diagonal tiles of its chain (first[i] <= max(i - 2, start of its
+// chain)).  The last tile row (the right-hand side) is dense.
 struct TileView {
   int n;        // order of S (6 NU)
   int TB;       // tile size
@@ -77,6 +79,8 @@ struct TileView {
   int ntiles;   // stored tiles
   const int* first;    // [NT] first stored tile column of each tile row (device)
   const int* row_off;  // [NT] slot of tile (i, first[i]) (device)
+  int nchain;          // chain workgroups of the factorisation
+  const int* chain_start;  // [nchain + 1] first diagonal step of each (device); chain_start[nchain] = NT
   __device__ __forceinline__ long long slot(int i, int j) const { return (long long)row_off[i] + (j - first[i]); }
   __device__ __forceinline__ bool has(int i, int j) const { return j >= first[i]; }  // for i >= j
 };
@@ -89,6 +93,7 @@ constexpr int kMaxTileRows = 4096;
 // is what TileView points to)
 struct TileProfile {
   std::vector<int> first, row_off;
+  std::vector<int> chain_start;  // chain groups of the factorisation (chol_chain_groups)
   int ntiles = 0;
   long long slot(int i, int j) const { return (long long)row_off[i] + (j - first[i]); }
   bool has(int i, int j) const { return j >= first[i]; }

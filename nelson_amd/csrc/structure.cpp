@@ -2,6 +2,7 @@
 #include "structure.h"
 
 #include <algorithm>
+#include <functional>
 #include <numeric>
 
 #include "internal.h"
@@ -69,6 +70,101 @@ std::vector<int> rcm_order(const std::vector<std::vector<int>>& adj) {
   return order;
 }
 
+std::vector<int> nd_order(const std::vector<std::vector<int>>& adj, int align, int depth) {
+  const int n = (int)adj.size();
+  std::vector<int> out;
+  out.reserve(n);
+  std::vector<int> loc(n, -1);  // node -> position in the current part's order
+  if (align < 1) align = 1;
+  // reverse Cuthill-McKee order of the subgraph induced by `nodes`
+  auto sub_rcm = [&](const std::vector<int>& nodes) {
+    const int N = (int)nodes.size();
+    for (int a = 0; a < N; a++) loc[nodes[a]] = a;
+    std::vector<std::vector<int>> sa(N);
+    for (int a = 0; a < N; a++)
+      for (int w : adj[nodes[a]])
+        if (loc[w] >= 0) sa[a].push_back(loc[w]);
+    for (int v : nodes) loc[v] = -1;
+    std::vector<int> o = rcm_order(sa);
+    for (int& v : o) v = nodes[v];
+    return o;
+  };
+  std::vector<char> inpart(n, 0);
+  std::function<void(const std::vector<int>&, int)> part = [&](const std::vector<int>& nodes, int lev) {
+    const int N = (int)nodes.size();
+    std::vector<int> ord = sub_rcm(nodes);
+    {
+      // orientation: the part's nodes adjacent to nodes outside it (the
+      // separators, which come later) go to the end of its order, so that
+      // the separator rows of S start late in the part (the envelope of a
+      // row runs from its first non-zero block to the diagonal)
+      for (int v : nodes) inpart[v] = 1;
+      double pos = 0.0;
+      int nbd = 0;
+      for (int a = 0; a < N; a++)
+        for (int w : adj[ord[a]])
+          if (!inpart[w]) {
+            pos += a;
+            nbd++;
+            break;
+          }
+      for (int v : nodes) inpart[v] = 0;
+      if (nbd > 0 && pos / nbd < 0.5 * (N - 1)) std::reverse(ord.begin(), ord.end());
+    }
+    if (lev >= depth || N < 4 * align) {
+      out.insert(out.end(), ord.begin(), ord.end());
+      return;
+    }
+    // sweep the prefixes A = ord[0, m): S = the nodes outside A with a
+    // neighbour in A (cnt[v] = neighbours of v in A), B = the rest
+    for (int a = 0; a < N; a++) loc[ord[a]] = a;
+    std::vector<int> cnt(N, 0);
+    std::vector<char> inA(N, 0);
+    int sizeS = 0, best_m = -1;
+    long long best = N;
+    for (int m = 1; m <= N; m++) {
+      const int u = m - 1;
+      if (cnt[u] > 0) sizeS--;
+      inA[u] = 1;
+      for (int w : adj[ord[u]]) {
+        const int lw = loc[w];
+        if (lw < 0 || inA[lw]) continue;
+        if (cnt[lw]++ == 0) sizeS++;
+      }
+      if (m % align) continue;
+      const int nb = N - m - sizeS;
+      if (nb < align) break;
+      const long long cost = std::max(m, nb) + sizeS;
+      if (cost < best) {
+        best = cost;
+        best_m = m;
+      }
+    }
+    std::vector<int> A, B, S;
+    if (best_m > 0 && best * 100 <= 85LL * N) {
+      std::vector<char> inS(N, 0);
+      for (int a = 0; a < best_m; a++)
+        for (int w : adj[ord[a]]) {
+          const int lw = loc[w];
+          if (lw >= best_m) inS[lw] = 1;
+        }
+      for (int a = 0; a < N; a++) (a < best_m ? A : inS[a] ? S : B).push_back(ord[a]);
+    }
+    for (int v : nodes) loc[v] = -1;
+    if (A.empty()) {
+      out.insert(out.end(), ord.begin(), ord.end());
+      return;
+    }
+    part(A, lev + 1);
+    part(B, lev + 1);
+    out.insert(out.end(), S.begin(), S.end());
+  };
+  std::vector<int> all(n);
+  std::iota(all.begin(), all.end(), 0);
+  part(all, 0);
+  return out;
+}
+
 int host_structure(const nsn_problem_desc* d, HostStructure& s, bool allow_order) {
   const int C = d->n_cameras, P = d->n_points, E = d->n_edges;
   s.C = C;
@@ -97,7 +193,7 @@ int host_structure(const nsn_problem_desc* d, HostStructure& s, bool allow_order
   }
   s.var_cam_api.resize(NU);
   for (int u = 0; u < NU; u++) s.var_cam_api[u] = u;
-  if (allow_order && d->camera_order == NSN_ORDER_RCM && NU > 2) {
+  if (allow_order && (d->camera_order == NSN_ORDER_RCM || d->camera_order == NSN_ORDER_ND) && NU > 2) {
     // the camera graph: variable cameras sharing a variable point
     std::vector<std::vector<int>> cams_of(P), adj(NU);
     for (int e = 0; e < E; e++) {
@@ -118,14 +214,18 @@ int host_structure(const nsn_problem_desc* d, HostStructure& s, bool allow_order
       std::sort(l.begin(), l.end());
       l.erase(std::unique(l.begin(), l.end()), l.end());
     }
-    const std::vector<int> ord = rcm_order(adj);  // solver position -> interface index
+    // solver position -> interface index; nested dissection in parts of
+    // multiples of 32 cameras (192 rows: three 64-row tiles)
+    const std::vector<int> ord =
+        d->camera_order == NSN_ORDER_ND ? nd_order(adj, kNdAlign, kNdDepth) : rcm_order(adj);
     const std::vector<int> api_user = s.var_cam_user;
     for (int v = 0; v < NU; v++) {
       s.var_cam_api[v] = ord[v];
       s.var_cam_user[v] = api_user[ord[v]];
       s.cam_var[api_user[ord[v]]] = v;
     }
-  } else if (d->camera_order != NSN_ORDER_AS_GIVEN && d->camera_order != NSN_ORDER_RCM) {
+  } else if (d->camera_order != NSN_ORDER_AS_GIVEN && d->camera_order != NSN_ORDER_RCM &&
+             d->camera_order != NSN_ORDER_ND) {
     return fail(NSN_EINVAL, "unknown camera_order %d", d->camera_order);
   }
   // point-major order: (point, camera, edge index)

@@ -47,4 +47,21 @@ int host_structure(const nsn_problem_desc* d, HostStructure& s, bool allow_order
 // pseudo-peripheral node of least degree, neighbours by increasing degree)
 std::vector<int> rcm_order(const std::vector<std::vector<int>>& adj);
 
+// nested-dissection order of a graph given by sorted adjacency lists, for
+// the tile Cholesky's independent diagonal chains: a part is laid out as
+// [A, B, S] with S the nodes outside A adjacent to A (so no edge joins A and
+// B), A a prefix of the part's reverse Cuthill-McKee order whose size is a
+// multiple of `align` (so B starts on a tile boundary when the part does),
+// chosen to minimise max(|A|, |B|) + |S|; A and B are split again the same
+// way, at most `depth` levels, while a split shortens that critical length
+// to at most 85 % of the part's.  Unsplit parts keep their reverse
+// Cuthill-McKee order.  order[k] = the node placed k-th.
+std::vector<int> nd_order(const std::vector<std::vector<int>>& adj, int align, int depth);
+
+// the nested-dissection parameters of NSN_ORDER_ND: parts of multiples of
+// 32 cameras (6 * 32 = 192 rows, three 64-row tiles), at most four levels
+// (16 independent chains)
+constexpr int kNdAlign = 32;
+constexpr int kNdDepth = 4;
+
 }  // namespace nsn

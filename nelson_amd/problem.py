@@ -199,11 +199,13 @@ class BAProblem:
         d.pt_fixed = self.pt_fixed.ctypes.data_as(C.POINTER(C.c_ubyte))
         d.focal, d.cx, d.cy, d.huber_delta = focal, cx, cy, huber_delta
         # "rcm": reverse Cuthill-McKee order of the cameras inside the solver
-        # (a narrow envelope of S); arrays per camera at the interface keep
-        # camera order either way
-        if camera_order not in ("given", "rcm"):
-            raise ValueError("camera_order must be 'given' or 'rcm'")
-        d.camera_order = capi.NSN_ORDER_RCM if camera_order == "rcm" else capi.NSN_ORDER_AS_GIVEN
+        # (a narrow envelope of S); "nd": nested dissection (independent
+        # diagonal chains of the factorisation); arrays per camera at the
+        # interface keep camera order either way
+        orders = {"given": capi.NSN_ORDER_AS_GIVEN, "rcm": capi.NSN_ORDER_RCM, "nd": capi.NSN_ORDER_ND}
+        if camera_order not in orders:
+            raise ValueError("camera_order must be 'given', 'rcm' or 'nd'")
+        d.camera_order = orders[camera_order]
         d.precision = capi.NSN_MIXED if precision == "mixed" else capi.NSN_FP64
         d.device = device
         d.intrinsics = _dp(self.intrinsics)
@@ -216,7 +218,41 @@ class BAProblem:
         self.n_s, self.n_s_blocks, self.n_s_pairs = info.n_s, info.n_s_blocks, info.n_s_pairs
         self.n_tiles, self.n_tiles_dense = info.n_tiles, info.n_tiles_dense
         self.factor_flops = info.factor_flops
+        self.n_chains, self.chain_steps = info.n_chains, info.chain_steps
         self._chi2 = None
+
+    @staticmethod
+    def plan(g, camera_order="given", cam_fixed=None, pt_fixed=None):
+        """The structure stage of a generated problem on the host alone
+        (nsn_problem_plan, no device): the problem figures (tile envelope,
+        diagonal chains, critical path in steps, operations) and the order
+        of the variable cameras inside the solver."""
+        lib = capi.lib()
+        C_, P_ = int(g["n_cameras"]), int(g["n_points"])
+        cam_idx = np.ascontiguousarray(g["cam_idx"], np.int32)
+        pt_idx = np.ascontiguousarray(g["pt_idx"], np.int32)
+        obs = np.ascontiguousarray(g["obs"], np.float64).reshape(-1)
+        if cam_fixed is None:
+            cam_fixed = np.zeros(C_, np.uint8)
+            cam_fixed[0] = 1
+        if pt_fixed is None:
+            pt_fixed = np.zeros(P_, np.uint8)
+            pt_fixed[0] = 1
+        cam_fixed = np.ascontiguousarray(cam_fixed, np.uint8)
+        pt_fixed = np.ascontiguousarray(pt_fixed, np.uint8)
+        orders = {"given": capi.NSN_ORDER_AS_GIVEN, "rcm": capi.NSN_ORDER_RCM, "nd": capi.NSN_ORDER_ND}
+        d = capi.ProblemDesc()
+        d.n_cameras, d.n_points, d.n_edges = C_, P_, len(cam_idx)
+        d.cam_idx, d.pt_idx, d.obs = _ip(cam_idx), _ip(pt_idx), _dp(obs)
+        d.weight = None
+        d.cam_fixed = cam_fixed.ctypes.data_as(C.POINTER(C.c_ubyte))
+        d.pt_fixed = pt_fixed.ctypes.data_as(C.POINTER(C.c_ubyte))
+        d.focal, d.cx, d.cy = g["focal"], g["cx"], g["cy"]
+        d.camera_order = orders[camera_order]
+        info = capi.ProblemInfo()
+        order = np.zeros(int((cam_fixed == 0).sum()), np.int32)
+        check(lib.nsn_problem_plan(C.byref(d), C.byref(info), _ip(order)))
+        return {k: getattr(info, k) for k, _ in capi.ProblemInfo._fields_}, order
 
     @classmethod
     def from_generated(cls, g, **kw):

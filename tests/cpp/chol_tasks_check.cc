@@ -1,16 +1,22 @@
-// Host check of the tile Cholesky's worker queue (nelson_amd/csrc/chol_tasks.h).
+// Host check of the tile Cholesky's worker queue and diagonal chains
+// (nelson_amd/csrc/chol_tasks.h).
 //
-// 1. Every look-ahead gives the same tasks as the plain column order, the
-//    sub-diagonal tile of column j follows the pre-update it reads, the tasks
-//    that are not queued early keep the column order, and the early ones come
-//    at most `ahead` columns early.
-// 2. A simulation of the persistent launch: W workgroups take tasks in queue
+// 1. Dense S: every look-ahead gives the same tasks as the plain column
+//    order, the sub-diagonal tile of column j follows the pre-update it
+//    reads, the tasks that are not queued early keep the column order, and
+//    the early ones come at most `ahead` columns early.
+// 2. Any envelope: every task that is not queued early comes after every
+//    task it reads, and every task reads only tiles inside the envelope.
+// 3. A simulation of the persistent launch: W workgroups take tasks in queue
 //    order and each holds its task until every tile, pre-update and chain
-//    step the task reads is complete; the diagonal chain (workgroup 0) runs
-//    beside them.  With the look-ahead the library uses for W workers
-//    (chol_lookahead) every task completes for every W; a look-ahead far
-//    beyond that bound deadlocks the simulated launch with few workers,
-//    which shows the simulation sees what it should.
+//    step the task reads is complete; the chain workgroups run their steps
+//    in order beside them.  With the look-ahead the library uses for W
+//    workers (chol_lookahead) every task completes for every W, on dense S,
+//    on banded envelopes and on envelopes of independent chains (the
+//    nested-dissection layout: leaves, then separators whose rows reach over
+//    their subtree); a look-ahead far beyond that bound deadlocks the
+//    simulated launch with few workers, which shows the simulation sees what
+//    it should.
 //
 // Output: one line per failure, then "OK" or "FAIL"; exit status 0 / 1.
 #include <algorithm>
@@ -21,6 +27,7 @@
 
 #include "chol_tasks.h"
 
+using nsn::chol_chain_groups;
 using nsn::chol_lookahead;
 using nsn::chol_task_list;
 
@@ -30,17 +37,19 @@ struct Task {
   int kind, i, j;
 };
 
-std::vector<Task> tasks_of(int NT, int ahead, const int* first = nullptr) {
+std::vector<Task> tasks_of(int NT, int ahead, const int* first = nullptr, int groups = 1) {
   int n = 0;
-  chol_task_list(NT, ahead, nullptr, &n, first);
+  chol_task_list(NT, ahead, nullptr, &n, first, groups);
   std::vector<int> raw(3 * (size_t)(n > 0 ? n : 1));
-  chol_task_list(NT, ahead, raw.data(), &n, first);
+  chol_task_list(NT, ahead, raw.data(), &n, first, groups);
   std::vector<Task> t(n);
   for (int q = 0; q < n; q++) t[q] = {raw[3 * q], raw[3 * q + 1], raw[3 * q + 2]};
   return t;
 }
 
 bool early(const Task& t) { return t.kind == 1 || t.kind == 2 || (t.kind == 0 && t.i == t.j + 2); }
+
+int F(const int* first, int i) { return first ? first[i] : 0; }
 
 int check_order(int NT, int ahead) {
   int bad = 0;
@@ -78,6 +87,98 @@ int check_order(int NT, int ahead) {
   return bad;
 }
 
+// the task ids and chain steps a task reads (nodes: tasks 0..nt-1, then
+// chain steps nt..nt+NT-1); -1 where a read tile has no task
+struct Graph {
+  std::vector<std::vector<int>> deps;
+  int nt = 0, missing = 0;
+};
+
+Graph build_graph(int NT, const std::vector<Task>& t, const int* first, const std::vector<int>& starts) {
+  Graph gr;
+  const int nt = (int)t.size();
+  gr.nt = nt;
+  std::map<std::tuple<int, int, int>, int> id;
+  for (int q = 0; q < nt; q++) id[{t[q].kind, t[q].i, t[q].j}] = q;
+  auto get = [&](int kind, int i, int j) {
+    auto it = id.find({kind, i, j});
+    if (it == id.end()) {
+      gr.missing++;
+      return -1;
+    }
+    return it->second;
+  };
+  auto tile = [&](int i, int k) {
+    if (k < F(first, i) || k < F(first, k)) gr.missing++;  // outside the envelope
+    return i == k + 1 ? get(3, i, k) : get(0, i, k);
+  };
+  auto chain = [&](int j) { return nt + j; };
+  gr.deps.assign(nt + NT, {});
+  for (int q = 0; q < nt; q++) {
+    const Task& x = t[q];
+    std::vector<int>& d = gr.deps[q];
+    auto k0 = [&](int i, int j) { return std::max(F(first, i), F(first, j)); };
+    if (x.kind == 0) {  // C_ij -= sum_k L_ik L_jk^T, then the inverse of step j
+      for (int k = k0(x.i, x.j); k < x.j; k++) d.push_back(tile(x.i, k)), d.push_back(tile(x.j, k));
+      d.push_back(chain(x.j));
+    } else if (x.kind == 1) {  // pre-update of (j+1, j)
+      for (int k = k0(x.j + 1, x.j); k < x.j; k++) d.push_back(tile(x.j + 1, k)), d.push_back(tile(x.j, k));
+    } else if (x.kind == 2) {  // pre-update of (j+1, j+1)
+      for (int k = F(first, x.j + 1); k < x.j; k++) d.push_back(tile(x.j + 1, k));
+    } else {  // sub-diagonal tile: the pre-update and the inverse of step j
+      d.push_back(get(1, x.j + 1, x.j));
+      d.push_back(chain(x.j));
+    }
+  }
+  // chain step j: the previous step of its group (the workgroup runs them in
+  // order) and, unless j starts a chain, the pre-updates of column j - 1
+  size_t g = 0;
+  for (int j = 0; j < NT; j++) {
+    while (g + 1 < starts.size() && starts[g + 1] <= j) g++;
+    if (j > starts[g]) gr.deps[chain(j)].push_back(chain(j - 1));
+    if (F(first, j) <= j - 1) {
+      gr.deps[chain(j)].push_back(get(1, j, j - 1));
+      if (F(first, j) < j - 1) gr.deps[chain(j)].push_back(get(2, j, j - 1));
+    }
+  }
+  for (auto& d : gr.deps) d.erase(std::remove(d.begin(), d.end(), -1), d.end());
+  return gr;
+}
+
+// every task that is not queued early comes after every task it reads, and
+// no task reads a tile outside the envelope or without a task
+int check_deps(int NT, const int* first, int groups, int ahead, const char* what) {
+  const std::vector<Task> t = tasks_of(NT, ahead, first, groups);
+  const std::vector<int> starts = chol_chain_groups(NT, first, groups);
+  Graph gr = build_graph(NT, t, first, starts);
+  int bad = 0;
+  if (gr.missing) {
+    std::printf("%s NT=%d: %d reads of tiles outside the envelope or without a task\n", what, NT, gr.missing);
+    bad++;
+  }
+  for (int q = 0; q < gr.nt; q++) {
+    if (early(t[q])) continue;
+    for (int u : gr.deps[q])
+      if (u < gr.nt && u > q) {
+        std::printf("%s NT=%d: task %d (%d, %d, %d) reads task %d queued after it\n", what, NT, q, t[q].kind, t[q].i,
+                    t[q].j, u);
+        bad++;
+        break;
+      }
+  }
+  // tiles inside the envelope all have a task (or are diagonal)
+  std::map<std::pair<int, int>, int> have;
+  for (const Task& x : t)
+    if (x.kind == 0 || x.kind == 3) have[{x.i, x.j}]++;
+  for (int i = 0; i < NT; i++)
+    for (int j = F(first, i); j < i; j++)
+      if (have[{i, j}] != 1) {
+        std::printf("%s NT=%d: tile (%d, %d) has %d tasks\n", what, NT, i, j, have[{i, j}]);
+        bad++;
+      }
+  return bad;
+}
+
 // an envelope: first[i] <= i - 2, the last row dense, otherwise a band of
 // random width (some rows reaching back further)
 std::vector<int> random_envelope(int NT, unsigned seed) {
@@ -92,51 +193,60 @@ std::vector<int> random_envelope(int NT, unsigned seed) {
   return f;
 }
 
+// the nested-dissection layout: a part [lo, hi) is a banded leaf (a chain:
+// first[lo] = lo, later rows reach back at most a random band, never before
+// lo) or [A, B, S] with A and B parts built the same way and the separator
+// rows S reaching back to lo (fill over the whole part); the last row is dense
+struct Rng {
+  unsigned x;
+  unsigned next() { return x = x * 1103515245u + 12345u, x >> 8; }
+};
+
+void nd_part(std::vector<int>& f, int lo, int hi, int depth, Rng& r) {
+  const int len = hi - lo;
+  if (depth == 0 || len < 8) {
+    const int band = 2 + (int)(r.next() % 5);
+    for (int i = lo; i < hi; i++) f[i] = std::max(lo, i - band);
+    return;
+  }
+  const int sep = 1 + (int)(r.next() % std::max(1, len / 5));
+  const int m = lo + std::max(2, (len - sep) / 2 + (int)(r.next() % 3) - 1);
+  const int s = hi - sep;
+  if (m >= s) {
+    nd_part(f, lo, hi, 0, r);
+    return;
+  }
+  nd_part(f, lo, m, depth - 1, r);
+  nd_part(f, m, s, depth - 1, r);
+  for (int i = s; i < hi; i++) f[i] = lo;
+}
+
+std::vector<int> nd_envelope(int NT, unsigned seed, int depth) {
+  std::vector<int> f(NT, 0);
+  Rng r{seed * 2654435761u + 77u};
+  nd_part(f, 0, NT, depth, r);
+  if (NT > 0) f[NT - 1] = 0;
+  return f;
+}
+
 // true when every task of the simulated launch completes
-bool simulate(int NT, int ahead, int W, const int* first = nullptr) {
-  const std::vector<Task> t = tasks_of(NT, ahead, first);
-  const int nt = (int)t.size();
-  // nodes: tasks 0..nt-1, then chain steps nt..nt+NT-1
-  std::map<std::tuple<int, int, int>, int> id;
-  for (int q = 0; q < nt; q++) id[{t[q].kind, t[q].i, t[q].j}] = q;
-  auto tile = [&](int i, int k) { return i == k + 1 ? id.at({3, i, k}) : id.at({0, i, k}); };
-  auto chain = [&](int j) { return nt + j; };
-  std::vector<std::vector<int>> deps(nt + NT);
-  for (int q = 0; q < nt; q++) {
-    const Task& x = t[q];
-    std::vector<int>& d = deps[q];
-    auto k0 = [&](int i, int j) { return first ? std::max(first[i], first[j]) : 0; };
-    if (x.kind == 0) {  // C_ij -= sum_k L_ik L_jk^T, then the inverse of step j
-      for (int k = k0(x.i, x.j); k < x.j; k++) d.push_back(tile(x.i, k)), d.push_back(tile(x.j, k));
-      d.push_back(chain(x.j));
-    } else if (x.kind == 1) {  // pre-update of (j+1, j)
-      for (int k = k0(x.j + 1, x.j); k < x.j; k++) d.push_back(tile(x.j + 1, k)), d.push_back(tile(x.j, k));
-    } else if (x.kind == 2) {  // pre-update of (j+1, j+1)
-      for (int k = k0(x.j + 1, x.j + 1); k < x.j; k++) d.push_back(tile(x.j + 1, k));
-    } else {  // sub-diagonal tile: the pre-update and the inverse of step j
-      d.push_back(id.at({1, x.j + 1, x.j}));
-      d.push_back(chain(x.j));
-    }
-  }
-  for (int j = 0; j < NT; j++) {  // step j: step j-1 and the pre-updates of column j-1
-    if (j >= 1) {
-      deps[chain(j)].push_back(chain(j - 1));
-      deps[chain(j)].push_back(id.at({1, j, j - 1}));
-      if (j >= 2) deps[chain(j)].push_back(id.at({2, j, j - 1}));
-    }
-  }
-  const int nn = nt + NT;
+bool simulate(int NT, int ahead, int W, const int* first = nullptr, int groups = 1) {
+  const std::vector<Task> t = tasks_of(NT, ahead, first, groups);
+  const std::vector<int> starts = chol_chain_groups(NT, first, groups);
+  Graph gr = build_graph(NT, t, first, starts);
+  if (gr.missing) return false;
+  const int nt = gr.nt, nn = nt + NT;
   std::vector<std::vector<int>> users(nn);
   std::vector<int> rem(nn);
   for (int v = 0; v < nn; v++) {
-    rem[v] = (int)deps[v].size();
-    for (int u : deps[v]) users[u].push_back(v);
+    rem[v] = (int)gr.deps[v].size();
+    for (int u : gr.deps[v]) users[u].push_back(v);
   }
-  std::vector<char> taken(nn, 0), done(nn, 0);
+  std::vector<char> taken(nn, 0);
   std::vector<int> ready;  // taken tasks and chain steps whose dependencies are complete
   for (int j = 0; j < NT; j++) {
-    taken[chain(j)] = 1;
-    if (rem[chain(j)] == 0) ready.push_back(chain(j));
+    taken[nt + j] = 1;
+    if (rem[nt + j] == 0) ready.push_back(nt + j);
   }
   int next = 0, free_wg = W, ndone = 0;
   while (true) {
@@ -149,13 +259,30 @@ bool simulate(int NT, int ahead, int W, const int* first = nullptr) {
     if (ready.empty()) break;
     const int v = ready.back();
     ready.pop_back();
-    done[v] = 1;
     ndone++;
     if (v < nt) free_wg++;
     for (int u : users[v])
       if (--rem[u] == 0 && taken[u]) ready.push_back(u);
   }
   return ndone == nn;
+}
+
+int check_launches(int NT, const int* first, int groups, const char* what) {
+  int bad = 0;
+  const int ng = (int)chol_chain_groups(NT, first, groups).size() - 1;
+  int n = 0;
+  chol_task_list(NT, 0, nullptr, &n, first, groups);
+  for (int W = 1; W <= 256 - ng; W++) {
+    const int workers = W < n ? W : n;  // the launch never has more workers than tasks
+    if (workers < 1) continue;
+    if (!simulate(NT, chol_lookahead(workers, ng), workers, first, groups)) {
+      std::printf("deadlock: %s NT=%d groups=%d workers=%d ahead=%d\n", what, NT, ng, workers,
+                  chol_lookahead(workers, ng));
+      bad++;
+    }
+    if (W > 64) W += 6;  // every width up to 64, then a sample
+  }
+  return bad;
 }
 
 }  // namespace
@@ -166,38 +293,40 @@ int main() {
     for (int ahead : {0, 1, 2, 5, 12}) bad += check_order(NT, ahead);
   for (int NT : {94, 120, 282}) bad += check_order(NT, 12);
   // the look-ahead the library uses, for every launch width
-  for (int NT : {2, 3, 5, 8, 13, 21, 40, 94})
-    for (int W = 1; W <= 255; W++) {
-      int n = 0;
-      chol_task_list(NT, 0, nullptr, &n);
-      const int workers = W < n ? W : n;  // the launch never has more workers than tasks
-      if (workers < 1) continue;
-      if (!simulate(NT, chol_lookahead(workers), workers)) {
-        std::printf("deadlock: NT=%d workers=%d ahead=%d\n", NT, workers, chol_lookahead(workers));
-        bad++;
-      }
-    }
-  // envelopes (banded rows, a dense last row): only tiles inside get a task,
-  // every tile a task reads is inside, and the launch completes
+  for (int NT : {2, 3, 5, 8, 13, 21, 40, 94}) bad += check_launches(NT, nullptr, 1, "dense");
+  // banded envelopes with a dense last row
   for (int NT : {5, 13, 40, 94})
     for (unsigned seed = 1; seed <= 4; seed++) {
       const std::vector<int> f = random_envelope(NT, seed);
-      for (const Task& x : tasks_of(NT, 12, f.data()))
-        if (x.kind == 0 && f[x.i] > x.j) {
-          std::printf("NT=%d seed=%u: task for tile (%d, %d) outside the envelope\n", NT, seed, x.i, x.j);
-          bad++;
-        }
-      for (int W : {1, 2, 7, 20, 60, 255}) {
-        int n = 0;
-        chol_task_list(NT, 0, nullptr, &n, f.data());
-        const int workers = W < n ? W : n;
-        if (workers < 1) continue;
-        if (!simulate(NT, chol_lookahead(workers), workers, f.data())) {
-          std::printf("deadlock: NT=%d seed=%u workers=%d (envelope)\n", NT, seed, workers);
-          bad++;
+      bad += check_deps(NT, f.data(), 1, 12, "band");
+      bad += check_launches(NT, f.data(), 1, "band");
+    }
+  // independent chains (nested dissection), several chain workgroups, and
+  // more chains than chain workgroups (merged groups)
+  for (int NT : {6, 19, 40, 94, 160})
+    for (unsigned seed = 1; seed <= 4; seed++)
+      for (int depth : {1, 2, 3}) {
+        const std::vector<int> f = nd_envelope(NT, seed, depth);
+        for (int groups : {1, 2, 3, 16}) {
+          for (int ahead : {0, 3, 12}) bad += check_deps(NT, f.data(), groups, ahead, "nd");
+          bad += check_launches(NT, f.data(), groups, "nd");
         }
       }
+  {
+    // chains are found where the envelope has no tile left of the diagonal
+    const std::vector<int> f = nd_envelope(94, 3, 2);
+    const std::vector<int> st = chol_chain_groups(94, f.data(), 16);
+    int starts = 0;
+    for (int j = 0; j < 94; j++) starts += f[j] == j;
+    if ((int)st.size() - 1 != starts) {
+      std::printf("nd: %d chain starts, %d groups\n", starts, (int)st.size() - 1);
+      bad++;
     }
+    if (starts < 3) {
+      std::printf("nd: the test envelope has only %d chains\n", starts);
+      bad++;
+    }
+  }
   // the simulation does detect a queue that can deadlock
   if (simulate(40, 12, 4)) {
     std::printf("the simulation found no deadlock with 4 workers and ahead=12\n");

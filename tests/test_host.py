@@ -161,3 +161,52 @@ def test_block_problem_structure_is_validated(n_u, n_v, w, uo, what):
     U blocks that are not strictly upper or repeated."""
     rc, msg = _block_desc(n_u, n_v, w, uo)
     assert rc == capi.NSN_EINVAL and what in msg, (rc, msg)
+
+
+def _chains_from_order(g, order):
+    """Independent restatement: the tile rows of S (64 rows, 6 per camera in
+    solver order) whose first non-zero block lies in the row's own tile
+    column, the last row (the right-hand side) dense."""
+    pos = np.full(g["n_cameras"], -1)
+    pos[order] = np.arange(len(order))
+    NT = (6 * len(order) + 1 + 63) // 64
+    nz = np.arange(NT)
+    cam, pt = np.asarray(g["cam_idx"]), np.asarray(g["pt_idx"])
+    keep = (pos[cam] >= 0) & (pt != 0)  # variable cameras, variable points (point 0 is fixed)
+    cam, pt = pos[cam[keep]], pt[keep]
+    srt = np.lexsort((cam, pt))
+    cam, pt = cam[srt], pt[srt]
+    starts = np.r_[0, np.flatnonzero(np.diff(pt)) + 1, len(pt)]
+    for a, b in zip(starts[:-1], starts[1:]):
+        c = np.unique(cam[a:b])
+        if len(c) < 2:
+            continue
+        lo = c.min()
+        for h in c[1:]:
+            for ti in range(6 * h // 64, (6 * h + 5) // 64 + 1):
+                nz[ti] = min(nz[ti], 6 * lo // 64)
+    nz[NT - 1] = 0
+    return int(np.sum(nz == np.arange(NT))), NT
+
+
+@pytest.mark.parametrize("name,cfg,min_chains", [
+    ("ring", dict(nsn.CONFIGS["cfg2"], n_cameras=300, arc=40, n_points=30000, seed=31), 2),
+    ("path", dict(nsn.CONFIGS["seq"], n_cameras=600, arc=30, n_points=60000, seed=33), 4),
+])
+def test_nested_dissection_plan(name, cfg, min_chains):
+    """nsn_problem_plan on the host: the nested-dissection order is a
+    permutation of the variable cameras whose independent parts start on
+    tile boundaries (chain count equal to a numpy restatement of the tile
+    envelope), with a shorter critical path than the given order; the given
+    order is one chain of NT steps."""
+    g = nsn.generate_ba(cfg)
+    gi, go = nsn.BAProblem.plan(g, "given")
+    ni, no = nsn.BAProblem.plan(g, "nd")
+    assert sorted(go.tolist()) == list(range(1, g["n_cameras"])) == go.tolist()
+    assert sorted(no.tolist()) == list(range(1, g["n_cameras"]))
+    nc, NT = _chains_from_order(g, go)
+    assert (gi["n_chains"], gi["chain_steps"], nc) == (1, NT, 1)
+    nc, NT = _chains_from_order(g, no)
+    assert ni["n_chains"] == nc >= min_chains, (ni["n_chains"], nc)
+    assert ni["chain_steps"] < 0.8 * NT, (ni["chain_steps"], NT)
+    assert ni["factor_flops"] > 0
