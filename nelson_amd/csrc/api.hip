@@ -31,7 +31,6 @@ namespace {
 
 constexpr int kTB = 64;          // Cholesky tile
 constexpr int kCholMaxWG = 256;  // persistent workgroups of the factorisation
-constexpr int kCholMaxChains = 16;  // of them, at most this many run diagonal chains
 // workgroups of a triangular solve: the chain and at most 255 destination
 // owners, one per CU, so that all of them are resident together whenever one
 // of them owns several destinations

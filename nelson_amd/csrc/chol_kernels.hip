@@ -280,19 +280,26 @@ __global__ __launch_bounds__(256) void k_chol(TileView tv, T* __restrict__ tiles
 // Triangular solve with the tile factor and the inverses of its diagonal
 // tiles (inv), over the NR = ceil(n / TB) tile rows of S, in solve order
 // (backward, L^T x = y: blocks NR-1 .. 0; forward, L z = r: blocks 0 .. NR-1).
-// Workgroup 0 is the chain: for the block k of each solve step it forms
+// The chain workgroups are the factorisation's chain groups: group c owns
+// the blocks chain_start[c] .. chain_start[c+1]-1, a contiguous run of solve
+// indices (independent parts of a nested-dissection order solve at the same
+// time: forward the parts before their separators, backward the separators
+// before their parts).  For the block k of each of its solve steps a chain
+// forms
 //   v = rhs_k - (far_k + sum_{d=1..kNear} N_{k,d} x_{k-d})
 //   x_k = Linv_kk^(T) v
-// with the kNear previous solutions kept in its LDS (N_{k,d} = L_{k,k-d}
-// forward, L_{k+d,k}^T backward) and the tiles, rhs_k and far_k of the next
-// step loaded into registers during the current one (far_k speculatively:
-// an entry still unset is polled again when it is needed).  Every other workgroup
-// owns destination blocks (solve index kNear+1+w, +G, ...): it takes the
-// solutions in solve order and accumulates, in its LDS and in source order
-// (deterministic), far_i = sum of N x_k over the sources more than kNear
-// steps before i, and publishes far_i after its last far source.  The chain
-// waits only for destination sums whose sources it has published, and the
-// destination workgroups only for published solutions: no cycle.
+// with its kNear previous solutions kept in its LDS (N_{k,d} = L_{k,k-d}
+// forward, L_{k+d,k}^T backward; only sources of its own run: near_lo) and
+// the tiles, rhs_k and far_k of the next step loaded into registers during
+// the current one (far_k speculatively: an entry still unset is polled again
+// when it is needed).  Every other workgroup owns destination blocks (the
+// solve indices t >= tf with a possible far source, round robin): it takes
+// the solutions in solve order and accumulates, in its LDS and in source
+// order (deterministic), far_t = sum of N x_s over the sources s <
+// near_lo(t), and publishes far_t after its last far source.  Deadlock
+// freedom: the unsolved block of least solve index has every earlier block
+// solved, so its far sum gets published, and its chain has solved every
+// earlier block of its run.
 //
 // Publication carries no flags or fences: sol and far hold kUnsetBits (a
 // signalling NaN, which arithmetic never produces) before the launch, every
@@ -348,11 +355,14 @@ __global__ __launch_bounds__(256) void k_trsv(TileView tv, const T* __restrict__
   // destination) pair read it, and a global read would put a memory round
   // trip before each tile load
   __shared__ int sfirst[kLdsFirst], soff[kLdsFirst];
+  __shared__ int scs[kCholMaxChains + 1];
   const int n = tv.n, NT = tv.NT;
   for (int i = threadIdx.x; i < NT; i += blockDim.x) {
     sfirst[i] = tv.first[i];
     soff[i] = tv.row_off[i];
   }
+  const int nch = tv.nchain;
+  for (int i = threadIdx.x; i <= nch; i += blockDim.x) scs[i] = tv.chain_start[i];
   __syncthreads();
   auto first_of = [&](int i) { return sfirst[i]; };
   auto slot_of = [&](int i, int j) { return (long long)soff[i] + (j - sfirst[i]); };
@@ -366,8 +376,24 @@ __global__ __launch_bounds__(256) void k_trsv(TileView tv, const T* __restrict__
   auto ntile_has = [&](int i, int k) { return BWD ? i >= first_of(k) : k >= first_of(i); };
   const bool own = Gemv::owner();
   const int i0 = Gemv::index(0), i1 = Gemv::index(1);
-  if (blockIdx.x == 0) {
-    // ---- the chain ----
+  // the solve-index run [run_lo(c), run_hi(c)) of chain group c
+  auto run_lo = [&](int c) { return BWD ? NR - min(scs[c + 1], NR) : min(scs[c], NR); };
+  auto run_hi = [&](int c) { return BWD ? NR - min(scs[c], NR) : min(scs[c + 1], NR); };
+  // the first near source of solve index t: kNear back, not before its run
+  auto near_lo = [&](int t) {
+    const int k = blk(t);
+    int c = 0;
+    while (c + 1 < nch && scs[c + 1] <= k) c++;
+    return max(t - kNear, run_lo(c));
+  };
+  // the first solve index with a possible far source (near_lo >= 1); every
+  // later one has one too (near_lo does not decrease)
+  int tf = 1;
+  while (tf < NR && near_lo(tf) < 1) tf++;
+  if (blockIdx.x < nch) {
+    // ---- a chain: the solve indices [sb, se) ----
+    const int sb = run_lo(blockIdx.x), se = run_hi(blockIdx.x);
+    if (sb >= se) return;
     // the loads of a step's tiles are issued as soon as the registers they
     // replace are consumed (issuing 96 KB takes about a microsecond on one
     // CU: spread over the step it overlaps the rest of the step's work)
@@ -381,7 +407,7 @@ __global__ __launch_bounds__(256) void k_trsv(TileView tv, const T* __restrict__
     // diagonal, and the one after it none before that chain)
     auto near_has = [&](int s, int d) {
       const int k = blk(s);
-      return d <= s && ntile_has(k, k - d * step);
+      return s - d >= sb && ntile_has(k, k - d * step);
     };
     auto load_near = [&](int s) {
       const int k = blk(s);
@@ -393,23 +419,23 @@ __global__ __launch_bounds__(256) void k_trsv(TileView tv, const T* __restrict__
       const int k = blk(s);
       if (own) {
         const int a = k * TB + i0, b = k * TB + i1;
-        f0 = (s > kNear && a < n) ? load_pub(far + a) : 0.0;
-        f1 = (s > kNear && b < n) ? load_pub(far + b) : 0.0;
+        f0 = (s >= tf && a < n) ? load_pub(far + a) : 0.0;
+        f1 = (s >= tf && b < n) ? load_pub(far + b) : 0.0;
         r0 = a < n ? rhs_at(a) : 0.0;
         r1 = b < n ? rhs_at(b) : 0.0;
       }
     };
-    load_vec(0);
-    gd.load(inv + (size_t)blk(0) * T2);
-    load_near(0);
-    for (int s = 0; s < NR; s++) {
+    load_vec(sb);
+    gd.load(inv + (size_t)blk(sb) * T2);
+    load_near(sb);
+    for (int s = sb; s < se; s++) {
       const int k = blk(s);
       // the near blocks' contributions, one reduction for all of them
       double h0 = 0.0, h1 = 0.0;
 #pragma unroll
       for (int d = 1; d <= kNear; d++)
         if (near_has(s, d)) gn[d - 1].partial(xr[(s - d) % kNear], h0, h1);
-      if (s + 1 < NR) load_near(s + 1);
+      if (s + 1 < se) load_near(s + 1);
       Gemv::reduce(h0, h1);
       if (own) {
         const int a = k * TB + i0, b = k * TB + i1;
@@ -421,7 +447,7 @@ __global__ __launch_bounds__(256) void k_trsv(TileView tv, const T* __restrict__
       __syncthreads();
       double x0 = 0.0, x1 = 0.0;
       gd.partial(xs[0], x0, x1);
-      if (s + 1 < NR) gd.load(inv + (size_t)blk(s + 1) * T2);
+      if (s + 1 < se) gd.load(inv + (size_t)blk(s + 1) * T2);
       Gemv::reduce(x0, x1);
       if (own) {
         const int a = k * TB + i0, b = k * TB + i1;
@@ -433,22 +459,25 @@ __global__ __launch_bounds__(256) void k_trsv(TileView tv, const T* __restrict__
       // x_k complete in the LDS before the next step reads it (and xs is
       // read by every wave before it is written again)
       __syncthreads();
-      if (s + 1 < NR) load_vec(s + 1);
+      if (s + 1 < se) load_vec(s + 1);
     }
     return;
   }
   // ---- destination sums ----
-  const int G = gridDim.x - 1, w = blockIdx.x - 1;
-  const int first = kNear + 1 + w;  // solve index of the first owned destination
+  const int G = gridDim.x - nch, w = blockIdx.x - nch;
+  const int first = tf + w;  // solve index of the first owned destination
   if (first >= NR) return;
   const int nown = (NR - 1 - first) / G + 1;  // <= kMaxOwned (launch)
   if (own)
     for (int m = 0; m < nown; m++) acc[m][i0] = acc[m][i1] = 0.0;
   // the products (source s, destination m) in source order, destinations in
-  // order within a source: destination m takes the far sources s <= its
-  // solve index - kNear - 1 whose tile is inside the envelope; the tile of
+  // order within a source: destination m takes the far sources s <
+  // near_lo(its solve index) whose tile is inside the envelope; the tile of
   // the next product is loaded while the current one runs
-  auto s_last = [&](int m) { return first + G * m - kNear - 1; };
+  __shared__ int slast[kMaxOwned];
+  if (threadIdx.x < nown) slast[threadIdx.x] = near_lo(first + G * threadIdx.x) - 1;
+  __syncthreads();
+  auto s_last = [&](int m) { return slast[m]; };
   const int s_end = s_last(nown - 1);  // no product after the last destination's last source
   auto find_next = [&](int& ps, int& pm) {  // the first product at or after (ps, pm)
     for (; ps <= s_end; ps++, pm = 0) {
@@ -466,7 +495,7 @@ __global__ __launch_bounds__(256) void k_trsv(TileView tv, const T* __restrict__
   // publishes the destinations whose last far source is s (a destination
   // with no tile of its row inside the envelope publishes 0)
   auto publish_done = [&](int s) {
-    while (m0 < nown && s_last(m0) == s) {
+    while (m0 < nown && s_last(m0) <= s) {
       const int d = blk(first + G * m0);
       if (own) {
         if (d * TB + i0 < n) store_pub(far + d * TB + i0, acc[m0][i0]);
@@ -554,12 +583,14 @@ template <typename T>
 void launch_trsv(const TileView& tv, const T* tiles, const T* inv, bool backward, const double* rhs, double* sol,
                  double* far, DevCtl* ctl, int gate, int max_wg, hipStream_t st) {
   const int NR = (tv.n + 63) / 64;
-  if (NR == 0 || tv.NT > kLdsFirst) return;  // (the structure stage refuses such an S)
-  // the chain plus G workgroups owning the nd destinations with far sources
-  const int nd = NR > kNear + 1 ? NR - kNear - 1 : 0;
-  int G = nd < max_wg - 1 ? nd : max_wg - 1;
+  if (NR == 0 || tv.NT > kLdsFirst || tv.nchain > kCholMaxChains) return;  // (the structure stage refuses these)
+  // the chain groups plus G workgroups owning the destinations with far
+  // sources (at most NR - 1 of them: solve index 0 has none)
+  const int nc = tv.nchain;
+  const int nd = NR - 1;
+  int G = nd < max_wg - nc ? nd : max_wg - nc;
   if (G > 0 && (nd + G - 1) / G > kMaxOwned) G = (nd + kMaxOwned - 1) / kMaxOwned;
-  const int g = 1 + G;
+  const int g = nc + G;
   if (backward)
     hipLaunchKernelGGL((k_trsv<T, 64, true>), dim3(g), dim3(256), 0, st, tv, tiles, inv, rhs, sol, far, ctl, gate);
   else

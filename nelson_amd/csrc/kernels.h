@@ -88,6 +88,9 @@ struct TileView {
 // the largest number of tile rows (the triangular solves keep the envelope
 // of every tile row in LDS)
 constexpr int kMaxTileRows = 4096;
+// the most diagonal chain groups of the factorisation and the triangular
+// solves (workgroups of their launches that run chains)
+constexpr int kCholMaxChains = 16;
 
 // the envelope of a tiled S on the host (the device copy of first / row_off
 // is what TileView points to)
