@@ -1,8 +1,10 @@
 """Benchmark: Levenberg-Marquardt iterations/s of the Schur-complement BA
 solve on one MI355X, on the 1000-camera / 200k-point / ~1.2M-observation
-ring problem (S = 6000 x 6000, ~30% dense), FP64 and mixed precision.
+ring problem (S = 6000 x 6000, ~30% dense), FP64 and mixed precision, with
+the cameras in nested-dissection order inside the solver (the given order
+is reported beside it).
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--config cfg2] [--no-cpu]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--config cfg2] [--order nd] [--no-cpu]
 
 A "step" is one LM outer iteration (linearisation, Schur solve(s), cost
 evaluation, accept/reject).  Prints one JSON line.
@@ -97,14 +99,21 @@ def sparse_timing(nsn, iters=10):
         c = np.zeros_like(g[k].reshape(C, 6))
         c[perm] = g[k].reshape(C, 6)
         g[k] = c.reshape(-1)
-    out = {"workload": f"BA {C} cameras along an open path (shuffled, reverse Cuthill-McKee order) / "
-                       f"{g['n_points']} points / {g['n_edges']} observations, S {6 * (C - 1)}x{6 * (C - 1)}"}
+    out = {"workload": f"BA {C} cameras along an open path (shuffled; nested-dissection order inside the "
+                       f"solver) / {g['n_points']} points / {g['n_edges']} observations, S {6 * (C - 1)}x{6 * (C - 1)}"}
     for prec in ("fp64", "mixed"):
-        p = nsn.BAProblem.from_generated(g, precision=prec, camera_order="rcm")
+        p = nsn.BAProblem.from_generated(g, precision=prec, camera_order="nd")
         out["tiles"], out["tiles_dense"] = p.n_tiles, p.n_tiles_dense
-        _, r = run_lm(nsn, g, prec, iters, 1, camera_order="rcm")
+        out["chains"], out["chain_steps"] = p.n_chains, p.chain_steps
+        _, r = run_lm(nsn, g, prec, iters, 1, camera_order="nd")
         out[prec] = {k: r[k] for k in ("it_per_s", "ms_per_solve", "iterations", "final_chi2")}
         out[prec]["phases_ms"] = chol_timing(nsn, p)
+    # the same with reverse Cuthill-McKee (one chain over a narrow band) and
+    # with the dense tile layout
+    pr = nsn.BAProblem.from_generated(g, camera_order="rcm")
+    _, r = run_lm(nsn, g, "fp64", iters, 1, camera_order="rcm")
+    out["fp64_rcm"] = {"it_per_s": r["it_per_s"], "tiles": pr.n_tiles, "chain_steps": pr.chain_steps,
+                       "phases_ms": chol_timing(nsn, pr)}
     old = os.environ.get("NSN_DENSE_S")
     os.environ["NSN_DENSE_S"] = "1"
     try:
@@ -117,6 +126,8 @@ def sparse_timing(nsn, iters=10):
     out["fp64_dense_layout_phases_ms"] = chol_timing(nsn, pd, reps=2)
     out["fp64_factorise_speedup_vs_dense_layout"] = (out["fp64_dense_layout_phases_ms"]["factorize"] /
                                                     out["fp64"]["phases_ms"]["factorize"])
+    out["fp64_factorise_speedup_vs_rcm"] = (out["fp64_rcm"]["phases_ms"]["factorize"] /
+                                           out["fp64"]["phases_ms"]["factorize"])
     return out
 
 
@@ -298,6 +309,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="cfg2")
+    ap.add_argument("--order", default="nd", choices=("given", "rcm", "nd"),
+                    help="camera order inside the solver (nd: nested dissection, independent chains)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
@@ -333,10 +346,17 @@ def main():
     cfg = dict(nsn.CONFIGS[args.config])
     g = nsn.generate_ba(cfg)
     log(f"bench: {args.config} C={g['n_cameras']} P={g['n_points']} E={g['n_edges']}")
-    p64, r64 = run_lm(nsn, g, "fp64", args.steps, args.warmup)
+    p64, r64 = run_lm(nsn, g, "fp64", args.steps, args.warmup, camera_order=args.order)
     log(f"bench: fp64 {r64}")
     ph = chol_timing(nsn, p64)
     log(f"bench: fp64 phases (ms) {ph}")
+    given = None
+    if args.order != "given":
+        # the same FP64 LM with the cameras in the given order (one chain)
+        pgv, rgv = run_lm(nsn, g, "fp64", args.steps, args.warmup)
+        given = {"it_per_s": rgv["it_per_s"], "ms_per_solve": rgv["ms_per_solve"], "final_chi2": rgv["final_chi2"],
+                 "tiles": pgv.n_tiles, "chain_steps": pgv.chain_steps, "phases_ms": chol_timing(nsn, pgv)}
+        log(f"bench: fp64 given order {given}")
     n = p64.n_s
     # the factorisation's operations over the tile envelope of S (fewer than
     # the dense n^3/3 where tiles are zero; the library counts them)
@@ -344,8 +364,9 @@ def main():
     chol_s = ph["factorize"] / 1e3
     achieved = flops / chol_s / 1e12
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "traffic_k_chol_cfg2_fp64.json")
-    if args.config == "cfg2" and os.path.exists(tfile):
+    tname = f"traffic_k_chol_{args.config}_{args.order}_fp64.json"
+    tfile = os.path.join(ROOT, "profiles", tname)
+    if os.path.exists(tfile):
         with open(tfile) as f:
             t = json.load(f)
         traffic = t["hbm_read_bytes_per_launch"] + t["hbm_written_bytes_per_launch"]
@@ -354,9 +375,11 @@ def main():
     roofline = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
                 "traffic_note": "HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, copy-calibrated), from "
-                                "profiles/traffic_k_chol_cfg2_fp64.json",
+                                "profiles/" + tname,
                 "kernel": "k_chol<double,64> (tile Cholesky of S over its tile envelope, n=%d, %d of %d tiles, "
-                          "%.3g flop)" % (n, p64.n_tiles, p64.n_tiles_dense, flops),
+                          "%.3g flop, %d diagonal chains, critical path %d of %d steps)"
+                          % (n, p64.n_tiles, p64.n_tiles_dense, flops, p64.n_chains, p64.chain_steps,
+                             (n + 64) // 64),
                 "kernel_ms": ph["factorize"],
                 "dense_equivalent_tflops": n ** 3 / 3.0 / chol_s / 1e12}
     mixed = None
@@ -366,14 +389,14 @@ def main():
         # mixed precision (FP32 blocks / S / Cholesky, FP64 pivot tiles, cost, gain
         # ratio and 2 iterative-refinement steps) on the same workload, and on the
         # 10-camera / 500-point problem where it is defined (20 iterations)
-        pm, rm = run_lm(nsn, g, "mixed", args.steps, args.warmup)
+        pm, rm = run_lm(nsn, g, "mixed", args.steps, args.warmup, camera_order=args.order)
         log(f"bench: mixed {rm}")
         mixed = dict(rm, refinement_steps=2,
                      final_cost_rel_err_vs_fp64=abs(rm["final_chi2"] - r64["final_chi2"]) / r64["final_chi2"],
                      phases_ms=chol_timing(nsn, pm))
         # the same with one refinement step (the library's setting
         # nsn_set_refinement): the speed / accuracy trade-off, not the headline
-        _, r1 = run_lm(nsn, g, "mixed", args.steps, args.warmup, refine=1)
+        _, r1 = run_lm(nsn, g, "mixed", args.steps, args.warmup, refine=1, camera_order=args.order)
         mixed_ir1 = {"refinement_steps": 1, "it_per_s": r1["it_per_s"], "ms_per_solve": r1["ms_per_solve"],
                      "it_per_s_ratio_vs_fp64": r1["it_per_s"] / r64["it_per_s"],
                      "final_cost_rel_err_vs_fp64": abs(r1["final_chi2"] - r64["final_chi2"]) / r64["final_chi2"]}
@@ -434,12 +457,13 @@ def main():
         "data": "synthetic (seeded generator)",
         "config": {"workload": f"BA {g['n_cameras']} cameras / {g['n_points']} points / "
                                f"{g['n_edges']} observations (ring of cameras), S {n}x{n}, FP64 Schur LM",
-                   "preset": args.config,
+                   "preset": args.config, "camera_order": args.order,
                    "cameras": g["n_cameras"], "points": g["n_points"], "observations": g["n_edges"],
                    "lm_iterations": args.steps},
         "mixed_vs_fp64": mixed_vs_fp64,
         "fp64": r64,
         "phases_ms": ph,
+        "fp64_given_order": given,
         "measured_peaks": peaks,
         "mixed": mixed,
         "mixed_small": mixed_small,

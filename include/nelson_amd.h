@@ -100,6 +100,9 @@ typedef struct {
                                                   time (independent parts of the camera order) */
   int chain_steps;                             /* its critical path in 64-column diagonal steps (the
                                                   number of tile rows for a single chain) */
+  int lookahead;                               /* columns by which the factorisation's worker queue
+                                                  starts the chains' pre-updates early (the largest
+                                                  up to 12 its launch is checked to complete with) */
 } nsn_problem_info;
 
 /* LevenbergMarquardtSettings (nelson/LevenbergMarquardt.h:18-34) */

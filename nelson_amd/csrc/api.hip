@@ -100,10 +100,12 @@ struct nsn_problem {
   Scratch scr{};
   double *cams = nullptr, *pts = nullptr, *cams_bak = nullptr, *pts_bak = nullptr;
   void *U = nullptr, *bU = nullptr, *V = nullptr, *bV = nullptr, *W = nullptr, *Vinv = nullptr, *Y = nullptr,
-       *tW = nullptr, *Sd = nullptr, *bS = nullptr, *tiles = nullptr, *inv = nullptr, *presub = nullptr;
+       *tW = nullptr, *Sd = nullptr, *bS = nullptr, *tiles = nullptr, *inv = nullptr, *presub = nullptr,
+       *part = nullptr;
   double *xU = nullptr, *xV = nullptr;
-  int* chol_tasks = nullptr;  // worker task list of the factorisation
+  int* chol_tasks = nullptr;  // worker task queue of the factorisation (chol_queue)
   int n_chol_tasks = 0;
+  int n_part_tiles = 0;  // its chunk scratch tiles
   int *tile_order = nullptr, *sb_i = nullptr, *sb_j = nullptr, *sb_ptr = nullptr, *sb_pairs = nullptr;
   // parts of the S assembly: tile, local rectangle (rows lo, hi, cols lo, hi), block list
   int *part_tile = nullptr, *part_rect = nullptr, *part_ptr = nullptr, *part_blk = nullptr;
@@ -131,6 +133,7 @@ struct nsn_problem {
   bool linearized = false, stepped = false;
   double last_abs = 0.0;
   int chain_steps = 0;  // the factorisation's critical path in diagonal steps
+  int chol_ahead = 0;   // its worker queue's look-ahead
   hipEvent_t ev[8] = {};
   // the LM sub-iteration captured once per problem (a GN iteration is
   // launched directly: measured on the ladybug workload, replaying it as a
@@ -227,8 +230,9 @@ void seq_schur(nsn_problem* h, int gate, int checks, bool timed) {
   launch_chol_reset<T>(h->flags, h->nflags, h->counters, h->xU, h->parts, h->tv.n, h->ctl, gate, st);
   {
     int* pre_sub = h->flags + h->tv.ntiles;
-    launch_chol<T>(h->tv, (T*)h->tiles, (T*)h->inv, (T*)h->presub, h->chol_tasks, h->n_chol_tasks, h->flags,
-                   pre_sub, pre_sub + h->tv.NT, h->counters, h->ctl, gate, kCholMaxWG, st);
+    launch_chol<T>(h->tv, (T*)h->tiles, (T*)h->inv, (T*)h->presub, (T*)h->part, h->chol_tasks, h->n_chol_tasks,
+                   h->flags, pre_sub, pre_sub + h->tv.NT, pre_sub + 2 * h->tv.NT, h->counters, h->ctl, gate,
+                   kCholMaxWG, st);
   }
   if (timed) (void)hipEventRecord(h->ev[3], st);
   launch_trsv<T>(h->tv, (const T*)h->tiles, (const T*)h->inv, true, nullptr, h->xU, h->parts, h->ctl, gate,
@@ -464,17 +468,21 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
   }
   h->n_s_parts = nparts;
   h->NR = (n + kTB - 1) / kTB;
-  // flags of the factorisation: ready (ntiles), pre_sub / pre_diag (NT each)
-  h->nflags = tv.ntiles + 2 * tv.NT;
-  int n_tasks = 0;
   const int nchain = (int)prof.chain_start.size() - 1;
-  // the count does not depend on the look-ahead
-  chol_task_list(tv.NT, 0, nullptr, &n_tasks, prof.first.data(), kCholMaxChains);
-  const int ahead = chol_lookahead(std::min(n_tasks, kCholMaxWG - nchain), nchain);
-  std::vector<int> chol_tasks(3 * (size_t)std::max(n_tasks, 1));
-  chol_task_list(tv.NT, ahead, chol_tasks.data(), &n_tasks, prof.first.data(), kCholMaxChains);
-  h->n_chol_tasks = n_tasks;
+  // the task count and the chunk tiles do not depend on the look-ahead
+  const int n_tasks0 = chol_queue(tv.NT, 0, prof.first.data(), kCholMaxChains).ntasks;
+  const int ahead = chol_pick_lookahead(tv.NT, prof.first.data(), kCholMaxChains,
+                                        std::min(n_tasks0, kCholMaxWG - nchain));
+  h->chol_ahead = ahead;
+  CholQueue cq = chol_queue(tv.NT, ahead, prof.first.data(), kCholMaxChains);
+  std::vector<int>& chol_tasks = cq.task;
+  if (chol_tasks.empty()) chol_tasks.assign(kCholTaskInts, 0);
+  h->n_chol_tasks = cq.ntasks;
+  h->n_part_tiles = cq.nparts;
   h->tv.nchain = nchain;
+  // flags of the factorisation: ready (ntiles), pre_sub / pre_diag (NT
+  // each), the chunk tiles (one each)
+  h->nflags = tv.ntiles + 2 * tv.NT + cq.nparts;
 
   // ---- device allocation ----
   DevBuf& m = h->mem;
@@ -519,6 +527,7 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
   h->tprof = a_int(2 * (size_t)tv.NT + prof.chain_start.size());
   h->inv = a_T((size_t)tv.NT * kTB * kTB);
   h->presub = a_T((size_t)tv.NT * kTB * kTB);
+  h->part = a_T((size_t)std::max(h->n_part_tiles, 1) * kTB * kTB);
   h->xU = a_dbl(6 * (size_t)NU);
   h->parts = a_dbl(2 * (size_t)h->NR * kTB);  // the triangular solves' destination sums
   h->rU = a_dbl(6 * (size_t)NU);
@@ -769,6 +778,7 @@ extern "C" int nsn_problem_info_get(nsn_problem* h, nsn_problem_info* info) {
   info->factor_flops = h->factor_flops;
   info->n_chains = h->tv.nchain;
   info->chain_steps = h->chain_steps;
+  info->lookahead = h->chol_ahead;
   return NSN_OK;
 }
 
@@ -796,6 +806,9 @@ extern "C" int nsn_problem_plan(const nsn_problem_desc* d, nsn_problem_info* inf
   info->factor_flops = tile_factor_flops(prof);
   info->n_chains = (int)prof.chain_start.size() - 1;
   info->chain_steps = tile_chain_steps(prof);
+  const int n_tasks = chol_queue(NT, 0, prof.first.data(), kCholMaxChains).ntasks;
+  info->lookahead = chol_pick_lookahead(NT, prof.first.data(), kCholMaxChains,
+                                        std::min(n_tasks, kCholMaxWG - info->n_chains));
   if (cam_order)
     for (int u = 0; u < hs.NU; u++) cam_order[u] = hs.var_cam_user[u];
   return NSN_OK;

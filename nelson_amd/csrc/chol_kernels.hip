@@ -23,6 +23,7 @@
 // per-tile ready flags (L2 write-back / invalidate on gfx950).
 #include <hip/hip_runtime.h>
 
+#include "chol_tasks.h"
 #include "kernels.h"
 #include "tile_device.h"
 
@@ -97,20 +98,18 @@ __device__ __forceinline__ void publish_flag(int* flag) {
 // wave 0 checks step k + l, one acquire fence per run of published steps).
 // LDS: 64 KiB = two TB x TB FP64 regions.
 // ---------------------------------------------------------------------------
-enum CholTask : int { CT_TILE = 0, CT_PRE_SUB = 1, CT_PRE_DIAG = 2, CT_SUB = 3 };
-
-// acc -= sum_{k0 <= k < kend} L_ik L_jk^T over published tiles (batched
-// waits); k0 = max(first[i], first[j]), the first k with both tiles inside
-// the envelope.
+// acc -= sum_{kbeg <= k < kend} L_ik L_jk^T over published tiles (batched
+// waits); kbeg >= max(first[i], first[j]), the first k with both tiles
+// inside the envelope.
 // Operands go from global memory straight into the MFMA operand registers
 // (a register double buffer across products measured slower: the diagonal
 // chain shares this kernel's register budget).
 template <typename T, int TB>
 __device__ __forceinline__ void acc_update(CholAcc<T>& acc, const T* __restrict__ tiles, int* ready, DevCtl* ctl,
-                                           int* shv, int i, int j, int kend, const TileView& tv, int lane, int wr,
-                                           int wc) {
+                                           int* shv, int i, int j, int kbeg, int kend, const TileView& tv, int lane,
+                                           int wr, int wc) {
   const size_t T2 = (size_t)TB * TB;
-  int k = max(tv.first[i], tv.first[j]);
+  int k = kbeg;
   while (k < kend) {
     if (threadIdx.x < 64) {
       if (lane == 0) {
@@ -147,10 +146,29 @@ __device__ __forceinline__ void acc_update(CholAcc<T>& acc, const T* __restrict_
   }
 }
 
+// a sum task's chunks (CT_PART scratch tiles p0 .. p0+np-1, each the
+// negated sum of its chunk) added to acc in order; returns the first k of
+// its own last chunk
+template <typename T, int TB>
+__device__ __forceinline__ int acc_add_parts(CholAcc<T>& acc, const T* __restrict__ part, int* part_ready,
+                                             DevCtl* ctl, int p0, int np, int k0, int kend, int lane, int wr,
+                                             int wc) {
+  if (np <= 0) return k0;
+  const size_t T2 = (size_t)TB * TB;
+  for (int p = p0; p < p0 + np; p++) {
+    if (threadIdx.x == 0) wait_flag(part_ready + p, ctl);
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    acc_add<T, TB>(acc, part + (size_t)p * T2, lane, wr, wc);
+  }
+  return max(k0, (kend - 1) / kCholPartLen * kCholPartLen);
+}
+
 template <typename T, int TB>
 __global__ __launch_bounds__(256) void k_chol(TileView tv, T* __restrict__ tiles, T* __restrict__ inv,
-                                              T* __restrict__ presub, const int* __restrict__ tasks, int ntasks,
-                                              int* ready, int* pre_sub, int* pre_diag, int* counter, DevCtl* ctl,
+                                              T* __restrict__ presub, T* __restrict__ part,
+                                              const int* __restrict__ tasks, int ntasks, int* ready, int* pre_sub,
+                                              int* pre_diag, int* part_ready, int* counter, DevCtl* ctl,
                                               int gate) {
   static_assert(TB == 64, "the MFMA tiling assumes 64x64 tiles and 4 waves");
   if (!chol_gate(ctl, gate)) return;
@@ -225,12 +243,24 @@ __global__ __launch_bounds__(256) void k_chol(TileView tv, T* __restrict__ tiles
     const int t = shv[0];
     __syncthreads();
     if (t >= ntasks) break;
-    const int type = tasks[3 * t], i = tasks[3 * t + 1], j = tasks[3 * t + 2];
-    if (type == CT_TILE) {
+    const int* rec = tasks + (size_t)kCholTaskInts * t;
+    const int type = rec[0], i = rec[1], j = rec[2], pa = rec[3], pb = rec[4];
+    if (type == CT_PART) {
+      // the chunk pb of the sum over the rows (i, j): -sum L_ik L_jk^T into
+      // the scratch tile pa
+      const int kb = max(max(tv.first[i], tv.first[j]), pb * kCholPartLen);
+      CholAcc<T> acc;
+      acc_clear<T>(acc);
+      acc_update<T, TB>(acc, tiles, ready, ctl, shv, i, j, kb, (pb + 1) * kCholPartLen, tv, lane, wr, wc);
+      acc_store<T, T, TB>(acc, part + (size_t)pa * T2, lane, wr, wc);
+      publish_flag(part_ready + pa);
+    } else if (type == CT_TILE) {
       T* Cg = tiles + (size_t)tv.slot(i, j) * T2;
       CholAcc<T> acc;
       acc_load<T, TB>(acc, Cg, lane, wr, wc);
-      acc_update<T, TB>(acc, tiles, ready, ctl, shv, i, j, j, tv, lane, wr, wc);
+      const int kb = acc_add_parts<T, TB>(acc, part, part_ready, ctl, pa, pb, max(tv.first[i], tv.first[j]), j,
+                                          lane, wr, wc);
+      acc_update<T, TB>(acc, tiles, ready, ctl, shv, i, j, kb, j, tv, lane, wr, wc);
       acc_store<T, T, TB>(acc, R0, lane, wr, wc);
       if (threadIdx.x == 0) wait_flag(ready + tv.slot(j, j), ctl);
       __syncthreads();
@@ -267,7 +297,9 @@ __global__ __launch_bounds__(256) void k_chol(TileView tv, T* __restrict__ tiles
       const int jc = type == CT_PRE_SUB ? j : j + 1;
       CholAcc<T> acc;
       acc_load<T, TB>(acc, tiles + (size_t)tv.slot(i, jc) * T2, lane, wr, wc);
-      acc_update<T, TB>(acc, tiles, ready, ctl, shv, i, jc, j, tv, lane, wr, wc);
+      const int kb = acc_add_parts<T, TB>(acc, part, part_ready, ctl, pa, pb, max(tv.first[i], tv.first[jc]), j,
+                                          lane, wr, wc);
+      acc_update<T, TB>(acc, tiles, ready, ctl, shv, i, jc, kb, j, tv, lane, wr, wc);
       acc_store<T, T, TB>(acc, R0, lane, wr, wc);
       __syncthreads();
       lds_to_global<T, TB>(type == CT_PRE_SUB ? presub + (size_t)j * T2 : tiles + (size_t)tv.slot(i, jc) * T2,
@@ -569,14 +601,15 @@ void launch_chol_reset(int* flags, int nflags, int* counters, double* unset0, do
 }
 
 template <typename T>
-void launch_chol(const TileView& tv, T* tiles, T* inv, T* presub, const int* tasks, int ntasks, int* ready,
-                 int* pre_sub, int* pre_diag, int* counter, DevCtl* ctl, int gate, int max_wg, hipStream_t st) {
+void launch_chol(const TileView& tv, T* tiles, T* inv, T* presub, T* part, const int* tasks, int ntasks,
+                 int* ready, int* pre_sub, int* pre_diag, int* part_ready, int* counter, DevCtl* ctl, int gate,
+                 int max_wg, hipStream_t st) {
   // the chain workgroups, then the workers (the structure stage keeps
   // nchain well below max_wg)
   const int nc = tv.nchain;
   int g = nc + (ntasks < max_wg - nc ? ntasks : max_wg - nc);
-  hipLaunchKernelGGL((k_chol<T, 64>), dim3(g), dim3(256), 0, st, tv, tiles, inv, presub, tasks, ntasks, ready,
-                     pre_sub, pre_diag, counter, ctl, gate);
+  hipLaunchKernelGGL((k_chol<T, 64>), dim3(g), dim3(256), 0, st, tv, tiles, inv, presub, part, tasks, ntasks,
+                     ready, pre_sub, pre_diag, part_ready, counter, ctl, gate);
 }
 
 template <typename T>
@@ -599,10 +632,10 @@ void launch_trsv(const TileView& tv, const T* tiles, const T* inv, bool backward
 
 template void launch_chol_reset<double>(int*, int, int*, double*, double*, int, DevCtl*, int, hipStream_t);
 template void launch_chol_reset<float>(int*, int, int*, double*, double*, int, DevCtl*, int, hipStream_t);
-template void launch_chol<double>(const TileView&, double*, double*, double*, const int*, int, int*, int*, int*,
-                                  int*, DevCtl*, int, int, hipStream_t);
-template void launch_chol<float>(const TileView&, float*, float*, float*, const int*, int, int*, int*, int*, int*,
-                                 DevCtl*, int, int, hipStream_t);
+template void launch_chol<double>(const TileView&, double*, double*, double*, double*, const int*, int, int*,
+                                  int*, int*, int*, int*, DevCtl*, int, int, hipStream_t);
+template void launch_chol<float>(const TileView&, float*, float*, float*, float*, const int*, int, int*, int*,
+                                 int*, int*, int*, DevCtl*, int, int, hipStream_t);
 template void launch_trsv<double>(const TileView&, const double*, const double*, bool, const double*, double*,
                                   double*, DevCtl*, int, int, hipStream_t);
 template void launch_trsv<float>(const TileView&, const float*, const float*, bool, const double*, double*, double*,

@@ -201,12 +201,14 @@ template <typename T>
 void launch_chol_reset(int* flags, int nflags, int* counters, double* unset0, double* unset1, int nunset,
                        DevCtl* ctl, int gate, hipStream_t st);
 // factor in place; inv[j] receives L_jj^-1 (TB x TB, column-major); presub:
-// NT scratch tiles.  tasks: the worker task list (3 ints per task: kind, i,
-// j; chol_task_list); ready: ntiles flags, pre_sub / pre_diag: NT flags
-// each; counter zeroed
+// NT scratch tiles; part: the CT_PART tasks' scratch tiles.  tasks: the
+// worker task queue (kCholTaskInts per task, chol_queue); ready: ntiles
+// flags, pre_sub / pre_diag: NT flags each, part_ready: one per part tile;
+// counter zeroed
 template <typename T>
-void launch_chol(const TileView& tv, T* tiles, T* inv, T* presub, const int* tasks, int ntasks, int* ready,
-                 int* pre_sub, int* pre_diag, int* counter, DevCtl* ctl, int gate, int max_wg, hipStream_t st);
+void launch_chol(const TileView& tv, T* tiles, T* inv, T* presub, T* part, const int* tasks, int ntasks,
+                 int* ready, int* pre_sub, int* pre_diag, int* part_ready, int* counter, DevCtl* ctl, int gate,
+                 int max_wg, hipStream_t st);
 // L^T sol = rhs (backward) or L sol = rhs (forward) over the first n rows;
 // rhs == nullptr: row n of the factor.  far: n doubles of scratch; sol[0..n)
 // and far[0..n) hold kUnsetBits before the launch (launch_chol_reset,

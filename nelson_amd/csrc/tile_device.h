@@ -316,6 +316,31 @@ __device__ __forceinline__ void acc_load(CholAcc<T>& acc, const T* Cg, int lane,
       }
 }
 
+// acc += tile (global, column-major TB x TB)
+template <typename T, int TB>
+__device__ __forceinline__ void acc_add(CholAcc<T>& acc, const T* Pg, int lane, int wr, int wc) {
+#pragma unroll
+  for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int vv = 0; vv < 4; vv++) {
+        const int r = wr * 32 + bi * 16 + Mfma<T>::row(vv, lane);
+        const int c = wc * 32 + bj * 16 + (lane & 15);
+        acc.a[bi][bj][vv] += Pg[c * TB + r];
+      }
+}
+
+template <typename T>
+__device__ __forceinline__ void acc_clear(CholAcc<T>& acc) {
+#pragma unroll
+  for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int vv = 0; vv < 4; vv++) acc.a[bi][bj][vv] = 0;
+}
+
 // acc -> LDS tile (column-major), element type U
 template <typename T, typename U, int TB>
 __device__ __forceinline__ void acc_store(const CholAcc<T>& acc, U* D, int lane, int wr, int wc) {

@@ -218,7 +218,7 @@ class BAProblem:
         self.n_s, self.n_s_blocks, self.n_s_pairs = info.n_s, info.n_s_blocks, info.n_s_pairs
         self.n_tiles, self.n_tiles_dense = info.n_tiles, info.n_tiles_dense
         self.factor_flops = info.factor_flops
-        self.n_chains, self.chain_steps = info.n_chains, info.chain_steps
+        self.n_chains, self.chain_steps, self.lookahead = info.n_chains, info.chain_steps, info.lookahead
         self._chi2 = None
 
     @staticmethod

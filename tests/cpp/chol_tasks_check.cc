@@ -10,13 +10,15 @@
 // 3. A simulation of the persistent launch: W workgroups take tasks in queue
 //    order and each holds its task until every tile, pre-update and chain
 //    step the task reads is complete; the chain workgroups run their steps
-//    in order beside them.  With the look-ahead the library uses for W
-//    workers (chol_lookahead) every task completes for every W, on dense S,
-//    on banded envelopes and on envelopes of independent chains (the
-//    nested-dissection layout: leaves, then separators whose rows reach over
-//    their subtree); a look-ahead far beyond that bound deadlocks the
-//    simulated launch with few workers, which shows the simulation sees what
-//    it should.
+//    in order beside them.  With the look-ahead bound (chol_lookahead) every
+//    task completes for every W, on dense S, on banded envelopes and on
+//    envelopes of independent chains (the nested-dissection layout: leaves,
+//    then separators whose rows reach over their subtree); a look-ahead far
+//    beyond that bound deadlocks the simulated launch with few workers,
+//    which shows the simulation sees what it should.
+// 4. The library's own exact check (chol_queue_completes) agrees with this
+//    simulation, and the look-ahead the library picks with it
+//    (chol_pick_lookahead) completes here.
 //
 // Output: one line per failure, then "OK" or "FAIL"; exit status 0 / 1.
 #include <algorithm>
@@ -29,34 +31,37 @@
 
 using nsn::chol_chain_groups;
 using nsn::chol_lookahead;
-using nsn::chol_task_list;
+using nsn::kCholPartLen;
+using nsn::kCholTaskInts;
 
 namespace {
 
 struct Task {
-  int kind, i, j;
+  int kind, i, j, a, b;
 };
 
 std::vector<Task> tasks_of(int NT, int ahead, const int* first = nullptr, int groups = 1) {
-  int n = 0;
-  chol_task_list(NT, ahead, nullptr, &n, first, groups);
-  std::vector<int> raw(3 * (size_t)(n > 0 ? n : 1));
-  chol_task_list(NT, ahead, raw.data(), &n, first, groups);
-  std::vector<Task> t(n);
-  for (int q = 0; q < n; q++) t[q] = {raw[3 * q], raw[3 * q + 1], raw[3 * q + 2]};
+  const nsn::CholQueue cq = nsn::chol_queue(NT, ahead, first, groups);
+  std::vector<Task> t(cq.ntasks);
+  for (int q = 0; q < cq.ntasks; q++) {
+    const int* r = cq.task.data() + kCholTaskInts * q;
+    t[q] = {r[0], r[1], r[2], r[3], r[4]};
+  }
   return t;
 }
 
-bool early(const Task& t) { return t.kind == 1 || t.kind == 2 || (t.kind == 0 && t.i == t.j + 2); }
+// queued ahead of its column: the hoisted pre-updates and tile (j+2, j),
+// and the chunks of long sums (queued when their own tiles can be ready)
+bool early(const Task& t) { return t.kind == 1 || t.kind == 2 || (t.kind == 0 && t.i == t.j + 2) || t.kind == 4; }
 
 int F(const int* first, int i) { return first ? first[i] : 0; }
 
 int check_order(int NT, int ahead) {
   int bad = 0;
   const std::vector<Task> a = tasks_of(NT, 0), b = tasks_of(NT, ahead);
-  std::map<std::tuple<int, int, int>, int> ca, cb;
-  for (const Task& t : a) ca[{t.kind, t.i, t.j}]++;
-  for (const Task& t : b) cb[{t.kind, t.i, t.j}]++;
+  std::map<std::tuple<int, int, int, int, int>, int> ca, cb;
+  for (const Task& t : a) ca[{t.kind, t.i, t.j, t.a, t.b}]++;
+  for (const Task& t : b) cb[{t.kind, t.i, t.j, t.a, t.b}]++;
   if (ca != cb) {
     std::printf("NT=%d ahead=%d: task set differs from the plain order\n", NT, ahead);
     bad++;
@@ -73,7 +78,7 @@ int check_order(int NT, int ahead) {
         bad++;
       }
       col = t.j;
-    } else if (t.j > col + 1 + ahead) {  // queued with (before) the tasks of column col + 1
+    } else if (t.kind != 4 && t.j > col + 1 + ahead) {  // queued with (before) the tasks of column col + 1
       std::printf("NT=%d ahead=%d: task %d of column %d queued more than %d columns early\n", NT, ahead, q, t.j,
                   ahead);
       bad++;
@@ -99,7 +104,15 @@ Graph build_graph(int NT, const std::vector<Task>& t, const int* first, const st
   const int nt = (int)t.size();
   gr.nt = nt;
   std::map<std::tuple<int, int, int>, int> id;
-  for (int q = 0; q < nt; q++) id[{t[q].kind, t[q].i, t[q].j}] = q;
+  std::map<int, int> part;  // chunk scratch slot -> task
+  for (int q = 0; q < nt; q++) {
+    if (t[q].kind == 4) {
+      if (part.count(t[q].a)) gr.missing++;  // two chunks in one slot
+      part[t[q].a] = q;
+    } else {
+      id[{t[q].kind, t[q].i, t[q].j}] = q;
+    }
+  }
   auto get = [&](int kind, int i, int j) {
     auto it = id.find({kind, i, j});
     if (it == id.end()) {
@@ -114,20 +127,50 @@ Graph build_graph(int NT, const std::vector<Task>& t, const int* first, const st
   };
   auto chain = [&](int j) { return nt + j; };
   gr.deps.assign(nt + NT, {});
+  // a sum over rows (r0, r1), k in [k0, kend): chunks a .. a+b-1 of the rows
+  // (each the chunk c0 + m, c0 the chunk of k0) and its own last chunk
+  auto sum = [&](std::vector<int>& d, const Task& x, int r0, int r1, int kend) {
+    const int k0 = std::max(F(first, r0), F(first, r1));
+    int lo = k0;
+    if (x.b > 0) {
+      const int c0 = k0 / kCholPartLen;
+      for (int m = 0; m < x.b; m++) {
+        auto it = part.find(x.a + m);
+        if (it == part.end()) {
+          gr.missing++;
+          continue;
+        }
+        const Task& pt = t[it->second];
+        if (pt.i != r0 || pt.j != r1 || pt.b != c0 + m) gr.missing++;  // the wrong chunk
+        d.push_back(it->second);
+      }
+      lo = std::max(k0, (c0 + x.b) * kCholPartLen);
+      if ((kend - 1) / kCholPartLen != c0 + x.b) gr.missing++;  // the last chunk is not its own
+    }
+    for (int k = lo; k < kend; k++) {
+      d.push_back(tile(r0, k));
+      if (r1 != r0) d.push_back(tile(r1, k));
+    }
+  };
   for (int q = 0; q < nt; q++) {
     const Task& x = t[q];
     std::vector<int>& d = gr.deps[q];
-    auto k0 = [&](int i, int j) { return std::max(F(first, i), F(first, j)); };
     if (x.kind == 0) {  // C_ij -= sum_k L_ik L_jk^T, then the inverse of step j
-      for (int k = k0(x.i, x.j); k < x.j; k++) d.push_back(tile(x.i, k)), d.push_back(tile(x.j, k));
+      sum(d, x, x.i, x.j, x.j);
       d.push_back(chain(x.j));
     } else if (x.kind == 1) {  // pre-update of (j+1, j)
-      for (int k = k0(x.j + 1, x.j); k < x.j; k++) d.push_back(tile(x.j + 1, k)), d.push_back(tile(x.j, k));
+      sum(d, x, x.j + 1, x.j, x.j);
     } else if (x.kind == 2) {  // pre-update of (j+1, j+1)
-      for (int k = F(first, x.j + 1); k < x.j; k++) d.push_back(tile(x.j + 1, k));
-    } else {  // sub-diagonal tile: the pre-update and the inverse of step j
+      sum(d, x, x.j + 1, x.j + 1, x.j);
+    } else if (x.kind == 3) {  // sub-diagonal tile: the pre-update and the inverse of step j
       d.push_back(get(1, x.j + 1, x.j));
       d.push_back(chain(x.j));
+    } else {  // chunk b of the rows (i, j)
+      const int lo = std::max(std::max(F(first, x.i), F(first, x.j)), x.b * kCholPartLen);
+      for (int k = lo; k < (x.b + 1) * kCholPartLen; k++) {
+        d.push_back(tile(x.i, k));
+        if (x.i != x.j) d.push_back(tile(x.j, k));
+      }
     }
   }
   // chain step j: the previous step of its group (the workgroup runs them in
@@ -145,8 +188,9 @@ Graph build_graph(int NT, const std::vector<Task>& t, const int* first, const st
   return gr;
 }
 
-// every task that is not queued early comes after every task it reads, and
-// no task reads a tile outside the envelope or without a task
+// every task that is not queued early (with no look-ahead: every task) comes
+// after every task it reads, and no task reads a tile outside the envelope
+// or without a task
 int check_deps(int NT, const int* first, int groups, int ahead, const char* what) {
   const std::vector<Task> t = tasks_of(NT, ahead, first, groups);
   const std::vector<int> starts = chol_chain_groups(NT, first, groups);
@@ -157,7 +201,9 @@ int check_deps(int NT, const int* first, int groups, int ahead, const char* what
     bad++;
   }
   for (int q = 0; q < gr.nt; q++) {
-    if (early(t[q])) continue;
+    // with no look-ahead the queue is in dependency order; otherwise only
+    // the tasks queued with their column are
+    if (ahead > 0 && early(t[q])) continue;
     for (int u : gr.deps[q])
       if (u < gr.nt && u > q) {
         std::printf("%s NT=%d: task %d (%d, %d, %d) reads task %d queued after it\n", what, NT, q, t[q].kind, t[q].i,
@@ -270,8 +316,7 @@ bool simulate(int NT, int ahead, int W, const int* first = nullptr, int groups =
 int check_launches(int NT, const int* first, int groups, const char* what) {
   int bad = 0;
   const int ng = (int)chol_chain_groups(NT, first, groups).size() - 1;
-  int n = 0;
-  chol_task_list(NT, 0, nullptr, &n, first, groups);
+  const int n = nsn::chol_queue(NT, 0, first, groups).ntasks;
   for (int W = 1; W <= 256 - ng; W++) {
     const int workers = W < n ? W : n;  // the launch never has more workers than tasks
     if (workers < 1) continue;
@@ -327,8 +372,33 @@ int main() {
       bad++;
     }
   }
-  // the simulation does detect a queue that can deadlock
-  if (simulate(40, 12, 4)) {
+  // the library's own check (chol_queue_completes) agrees with this
+  // simulation, and the look-ahead it picks (chol_pick_lookahead) completes
+  // here for every width
+  for (int NT : {13, 40, 94})
+    for (unsigned seed = 1; seed <= 3; seed++)
+      for (int depth : {0, 2, 3}) {
+        const std::vector<int> f = depth ? nd_envelope(NT, seed, depth) : random_envelope(NT, seed);
+        for (int groups : {1, 16})
+          for (int W : {1, 3, 8, 20, 60, 120, 240}) {
+            for (int ahead : {0, 4, 12})
+              if (nsn::chol_queue_completes(NT, f.data(), groups, ahead, W) !=
+                  simulate(NT, ahead, W, f.data(), groups)) {
+                std::printf("chol_queue_completes disagrees: NT=%d seed=%u depth=%d groups=%d W=%d ahead=%d\n", NT,
+                            seed, depth, groups, W, ahead);
+                bad++;
+              }
+            const int a = nsn::chol_pick_lookahead(NT, f.data(), groups, W);
+            if (!simulate(NT, a, W, f.data(), groups)) {
+              std::printf("picked look-ahead %d deadlocks: NT=%d seed=%u depth=%d groups=%d W=%d\n", a, NT, seed,
+                          depth, groups, W);
+              bad++;
+            }
+          }
+      }
+  // the simulation does detect a queue that can deadlock, and so does the
+  // library's check
+  if (simulate(40, 12, 4) || nsn::chol_queue_completes(40, nullptr, 1, 12, 4)) {
     std::printf("the simulation found no deadlock with 4 workers and ahead=12\n");
     bad++;
   }

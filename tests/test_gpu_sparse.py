@@ -3,9 +3,12 @@ row i from its first non-zero tile column; the factor has no fill outside
 it).
   * The tiles outside the envelope are exact zeros of S and of the factor,
     so the envelope layout and the dense one (NSN_DENSE_S=1) give the same
-    increments bit for bit (the dense layout is the one pinned against the
-    oracle at full size): on the 1000-camera ring (about half the tiles) and
-    on an open camera path.
+    increments to rounding (1e-12 normwise; the dense layout is the one
+    pinned against the oracle at full size): on the 1000-camera ring (about
+    half the tiles) and on an open camera path.  Not bit for bit: a sum of
+    tile products longer than 24 terms is cut into chunks added in order,
+    and the dense layout's sums (from column 0) are cut where the
+    envelope's (from their first non-zero tile) are not.
   * An open path of 4000 cameras / 400k points / 2.4M observations (n =
     23 994, about 3 % of the dense tiles): the FP64 increment solves the
     oracle's damped normal equations to a componentwise backward error of
@@ -54,15 +57,16 @@ def _increments(p, dampings):
     ("path", dict(CONFIGS["seq"], n_cameras=600, n_points=60000, arc=30, seed=21)),
 ])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_envelope_equals_dense_bitwise(name, cfg, precision):
+def test_envelope_equals_dense(name, cfg, precision):
     g = generate_ba(cfg)
     pe, pd = _problem(g, precision), _problem(g, precision, dense=True)
     assert pd.n_tiles == pd.n_tiles_dense
     assert pe.n_tiles < pd.n_tiles, (pe.n_tiles, pd.n_tiles)
-    print(f"\n{name}: {pe.n_tiles} of {pd.n_tiles} tiles")
     damp = [(0.0, 1e-4), (1e-3, 0.0)]
     for a, b in zip(_increments(pe, damp), _increments(pd, damp)):
-        assert np.array_equal(a, b)
+        e = relerr(a, b)
+        print(f"\n{name} {precision}: {pe.n_tiles} of {pd.n_tiles} tiles, envelope vs dense {e:.2e}")
+        assert e <= (1e-12 if precision == "fp64" else 1e-6), e
 
 
 @pytest.fixture(scope="module")
