@@ -56,14 +56,18 @@ def run_lm(nsn, g, precision, iters, warmup, refine=None, camera_order="given"):
         nsn.LevenbergMarquardt(lm_settings(nsn, warmup)).solve(p)
         p.set_state(g["cams0"], g["pts0"])
     lm = nsn.LevenbergMarquardt(lm_settings(nsn, iters))
+    c0 = p.refinement_count() if precision == "mixed" else 0
     t0 = time.perf_counter()
     reason = lm.solve(p)
     t1 = time.perf_counter()
     r = lm.result()
-    return p, dict(reason=reason, iterations=r["iterations"], sub_iterations=r["sub_iterations"],
-                   final_chi2=r["final_chi2"], wall_s=t1 - t0, device_loop_ms=r["elapsed_ms"],
-                   it_per_s=r["iterations"] / (t1 - t0), ms_per_solve=1e3 * (t1 - t0) / max(r["sub_iterations"], 1),
-                   log=[(e[0], e[1], e[2], e[3]) for e in lm.stats()])
+    out = dict(reason=reason, iterations=r["iterations"], sub_iterations=r["sub_iterations"],
+               final_chi2=r["final_chi2"], wall_s=t1 - t0, device_loop_ms=r["elapsed_ms"],
+               it_per_s=r["iterations"] / (t1 - t0), ms_per_solve=1e3 * (t1 - t0) / max(r["sub_iterations"], 1),
+               log=[(e[0], e[1], e[2], e[3]) for e in lm.stats()])
+    if precision == "mixed":
+        out["refinement_solves_per_increment"] = (p.refinement_count() - c0) / max(r["sub_iterations"], 1)
+    return p, out
 
 
 def chol_timing(nsn, p, reps=3):
@@ -385,22 +389,30 @@ def main():
     mixed = None
     mixed_small = None
     mixed_ir1 = None
+    mixed_ir0 = None
     if not args.no_mixed:
         # mixed precision (FP32 blocks / S / Cholesky, FP64 pivot tiles, cost, gain
-        # ratio and 2 iterative-refinement steps) on the same workload, and on the
-        # 10-camera / 500-point problem where it is defined (20 iterations)
+        # ratio and the default iterative refinement: at most 2 steps, each
+        # stopping on the device once the FP64 residual is at 1e-10 of the
+        # gradient) on the same workload, and on the 10-camera / 500-point
+        # problem where it is defined (20 iterations)
         pm, rm = run_lm(nsn, g, "mixed", args.steps, args.warmup, camera_order=args.order)
         log(f"bench: mixed {rm}")
-        mixed = dict(rm, refinement_steps=2,
+        mixed = dict(rm, refinement="at most 2 steps, stopping once max|r| <= 1e-10 max|b| (FP64 residual)",
                      final_cost_rel_err_vs_fp64=abs(rm["final_chi2"] - r64["final_chi2"]) / r64["final_chi2"],
                      phases_ms=chol_timing(nsn, pm))
-        # the same with one refinement step (the library's setting
-        # nsn_set_refinement): the speed / accuracy trade-off, not the headline
-        _, r1 = run_lm(nsn, g, "mixed", args.steps, args.warmup, refine=1, camera_order=args.order)
-        mixed_ir1 = {"refinement_steps": 1, "it_per_s": r1["it_per_s"], "ms_per_solve": r1["ms_per_solve"],
-                     "it_per_s_ratio_vs_fp64": r1["it_per_s"] / r64["it_per_s"],
-                     "final_cost_rel_err_vs_fp64": abs(r1["final_chi2"] - r64["final_chi2"]) / r64["final_chi2"]}
+        # the same with one and with no refinement step (the library's
+        # setting nsn_set_refinement): the speed / accuracy trade-off; every
+        # one of them matches the FP64 final cost far below 1e-6
+        def variant(k):
+            _, rk = run_lm(nsn, g, "mixed", args.steps, args.warmup, refine=k, camera_order=args.order)
+            return {"refinement_steps": k, "it_per_s": rk["it_per_s"], "ms_per_solve": rk["ms_per_solve"],
+                    "it_per_s_ratio_vs_fp64": rk["it_per_s"] / r64["it_per_s"],
+                    "final_cost_rel_err_vs_fp64": abs(rk["final_chi2"] - r64["final_chi2"]) / r64["final_chi2"]}
+        mixed_ir1 = variant(1)
         log(f"bench: mixed, one refinement step {mixed_ir1}")
+        mixed_ir0 = variant(0)
+        log(f"bench: mixed, no refinement {mixed_ir0}")
         # the 10-camera / 500-point problem of the mixed-precision
         # configuration (20 iterations)
         gs = nsn.generate_ba(dict(nsn.CONFIGS["cfg1"]))
@@ -440,6 +452,11 @@ def main():
         mixed_vs_fp64 = {"it_per_s_ratio": mixed["it_per_s"] / value,
                          "final_cost_rel_err": mixed["final_cost_rel_err_vs_fp64"],
                          "fp64_ms_per_solve": r64["ms_per_solve"], "mixed_ms_per_solve": mixed["ms_per_solve"],
+                         "refinement_solves_per_increment": mixed["refinement_solves_per_increment"],
+                         "one_step_ratio": mixed_ir1["it_per_s_ratio_vs_fp64"],
+                         "one_step_final_cost_rel_err": mixed_ir1["final_cost_rel_err_vs_fp64"],
+                         "no_refinement_ratio": mixed_ir0["it_per_s_ratio_vs_fp64"],
+                         "no_refinement_final_cost_rel_err": mixed_ir0["final_cost_rel_err_vs_fp64"],
                          "small_problem_ratio": mixed_small["ratio"],
                          "small_problem_final_cost_rel_err": mixed_small["final_cost_rel_err"]}
     out = {
@@ -468,6 +485,7 @@ def main():
         "mixed": mixed,
         "mixed_small": mixed_small,
         "mixed_ir1": mixed_ir1,
+        "mixed_ir0": mixed_ir0,
         "adjoint": adjoint,
         "batch": batch,
         "sparse": sparse,

@@ -307,11 +307,20 @@ int nsn_gn_solve(nsn_problem* h, const nsn_gn_settings* s, nsn_lm_result* res,
 int nsn_phase_times(nsn_problem* h, double* ms, int n);
 
 /* ---- mixed precision (no reference counterpart) ---------------------------
- * Number of iterative-refinement steps applied to every NSN_MIXED increment
- * (0..8; default 2): each step evaluates the FP64 residual of the damped
- * normal equations from the edges and adds the FP32 Schur solve of it.  The
- * gain-ratio denominator then uses the FP64 gradient.  Ignored for NSN_FP64. */
+ * Iterative refinement of every NSN_MIXED increment: each step evaluates the
+ * FP64 residual r = -b - (H + D) dx of the damped normal equations from the
+ * edges and adds the FP32 Schur solve of it.  The gain-ratio denominator
+ * uses the FP64 gradient.  Ignored for NSN_FP64.
+ * nsn_set_refinement_tol: at most max_steps steps (0..8); a step stops after
+ * its residual when max |r| <= tol * max |b| (tol = 0: every step runs), the
+ * test made on the device.  Default: 2 steps, tol 1e-10 (the work of
+ * two fixed steps, less the second solve when the first reaches the
+ * tolerance).
+ * nsn_set_refinement: exactly `steps` steps (tol 0).
+ * nsn_refinement_count: refinement solves run since the problem was created. */
 int nsn_set_refinement(nsn_problem* h, int steps);
+int nsn_set_refinement_tol(nsn_problem* h, int max_steps, double tol);
+int nsn_refinement_count(nsn_problem* h, int* steps);
 
 /* replaces the observations [2*n_edges] and / or the per-edge weights
  * [n_edges] (either may be NULL) of an existing problem, in the user's edge

@@ -135,6 +135,8 @@ SIGNATURES = {
     "nsn_gn_solve": (C.c_int, [_P, C.POINTER(GNSettings), C.POINTER(LMResult), C.POINTER(LogEntry), C.c_int]),
     "nsn_phase_times": (C.c_int, [_P, _D, C.c_int]),
     "nsn_set_refinement": (C.c_int, [_P, C.c_int]),
+    "nsn_set_refinement_tol": (C.c_int, [_P, C.c_int, C.c_double]),
+    "nsn_refinement_count": (C.c_int, [_P, _I]),
     "nsn_problem_create_blocks": (C.c_int, [C.POINTER(BlockDesc), C.POINTER(_P)]),
     "nsn_import_hessian_blocks": (C.c_int, [_P, _D, _D, _D, _D, _D, _D, C.c_double]),
     "nsn_adjoint": (C.c_int, [_P, _D, _D, C.c_int, _D, _D]),

@@ -31,6 +31,11 @@ namespace {
 
 constexpr int kTB = 64;          // Cholesky tile
 constexpr int kCholMaxWG = 256;  // persistent workgroups of the factorisation
+// mixed precision: at most this many refinement steps per increment, each
+// run while the FP64 residual's largest entry is above kIrTol times the
+// gradient's
+constexpr int kIrMaxSteps = 2;
+constexpr double kIrTol = 1e-10;
 // workgroups of a triangular solve: the chain and at most 255 destination
 // owners, one per CU, so that all of them are resident together whenever one
 // of them owns several destinations
@@ -124,6 +129,7 @@ struct nsn_problem {
          *zeroU = nullptr, *zeroV = nullptr, *dobs = nullptr, *dw = nullptr, *adj_eps = nullptr;
   // iterative refinement (mixed precision)
   int ir_steps = 0;
+  double ir_tol = 0.0;  // refinement stops once max |r| <= ir_tol * max |b| (0: never early)
   double *rU = nullptr, *rV = nullptr, *bU64 = nullptr, *bV64 = nullptr, *rS = nullptr, *zS = nullptr,
          *zU = nullptr;
   DevCtl* ctl = nullptr;
@@ -244,10 +250,15 @@ void seq_schur(nsn_problem* h, int gate, int checks, bool timed) {
   // the FP64 refinement residual is evaluated from the edges, which an
   // imported Hessian does not come from: imported systems are not refined
   const int ir = (h->prec == NSN_MIXED && !h->imported) ? h->ir_steps : 0;
+  // a refinement step runs its solve while the residual it starts with is
+  // above the tolerance (k_ir_decide, on the device); the first residual
+  // also gives the FP64 gradient of the gain ratio
+  const int gir = gate == GATE_ALWAYS ? GATE_IR_ANY : GATE_IR;
   for (int it = 0; it < ir; it++) {
     // x += M^-1 (-b - (H + D) x), M the FP32 Schur solve
-    launch_ir_residual(h->view, h->xU, h->xV, h->rU, h->rV, h->bU64, h->bV64, h->ctl, gate, st);
-    seq_schur_solve<T>(h, gate, h->rU, h->rV, h->zU, h->xV, 1, h->bV64, h->xU);
+    launch_ir_residual(h->view, h->xU, h->xV, h->rU, h->rV, h->bU64, h->bV64, h->ctl, gir, st);
+    launch_ir_decide(h->ctl, gir, h->ir_tol, st);
+    seq_schur_solve<T>(h, gir, h->rU, h->rV, h->zU, h->xV, 1, h->bV64, h->xU);
   }
   if (ir > 0) launch_xu_stats<double>(h->NU, h->xU, h->bU64, h->scr, h->ctl, gate, st);
   else launch_xu_stats<T>(h->NU, h->xU, (const T*)h->bU, h->scr, h->ctl, gate, st);
@@ -660,7 +671,8 @@ static int problem_create_impl(const nsn_problem_desc* d, nsn_problem** out, lon
   std::unique_ptr<nsn_problem> h(new nsn_problem());
   h->device = d->device;
   h->C = d->n_cameras; h->P = d->n_points; h->E = d->n_edges; h->prec = d->precision;
-  h->ir_steps = d->precision == NSN_MIXED ? 2 : 0;
+  h->ir_steps = d->precision == NSN_MIXED ? kIrMaxSteps : 0;
+  h->ir_tol = kIrTol;
   h->f = d->focal; h->cx = d->cx; h->cy = d->cy; h->huber = d->huber_delta;
   h->mem.fail_after = fail_after;
   HIP_TRY(hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking));
@@ -1177,10 +1189,27 @@ extern "C" int nsn_set_priors(nsn_problem* h, int n_cam, const int* cam_ids, con
 }
 
 extern "C" int nsn_set_refinement(nsn_problem* h, int steps) {
+  return nsn_set_refinement_tol(h, steps, 0.0);
+}
+
+extern "C" int nsn_set_refinement_tol(nsn_problem* h, int max_steps, double tol) {
   if (!h) return fail(NSN_EINVAL, "null argument");
-  if (steps < 0 || steps > 8) return fail(NSN_EINVAL, "nsn_set_refinement: steps must be in 0..8, got %d", steps);
-  if (steps != h->ir_steps) h->drop_graphs();  // the captured iterations hold the old sequence
-  h->ir_steps = steps;
+  if (max_steps < 0 || max_steps > 8)
+    return fail(NSN_EINVAL, "%s: steps must be in 0..8, got %d", __func__, max_steps);
+  if (!(tol >= 0.0) || !std::isfinite(tol)) return fail(NSN_EINVAL, "%s: tol must be finite and >= 0", __func__);
+  // the captured iterations hold the old sequence and tolerance
+  if (max_steps != h->ir_steps || tol != h->ir_tol) h->drop_graphs();
+  h->ir_steps = max_steps;
+  h->ir_tol = tol;
+  return NSN_OK;
+}
+
+extern "C" int nsn_refinement_count(nsn_problem* h, int* steps) {
+  if (!h || !steps) return fail(NSN_EINVAL, "null argument");
+  HIP_TRY(hipSetDevice(h->device));
+  HIP_TRY(hipMemcpyAsync(h->hctl, h->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, h->st));
+  HIP_TRY(hipStreamSynchronize(h->st));
+  *steps = h->hctl->ir_count;
   return NSN_OK;
 }
 

@@ -364,7 +364,8 @@ __global__ __launch_bounds__(256) void k_backsub(BAView v, const T* __restrict__
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void ir_res_point(const BAView& v, const double* __restrict__ xU,
                                              const double* __restrict__ xV, double* __restrict__ rV,
-                                             double* __restrict__ bV64, const DevCtl* ctl, int p) {
+                                             double* __restrict__ bV64, const DevCtl* ctl, int p, double& mr,
+                                             double& mb) {
   const int pv = v.pt_var[p];
   if (pv < 0) return;
   const double X0 = v.pts[3 * p], X1 = v.pts[3 * p + 1], X2 = v.pts[3 * p + 2];
@@ -416,8 +417,11 @@ __device__ __forceinline__ void ir_res_point(const BAView& v, const double* __re
   const double rel = ctl->rel, ab = ctl->abs_;
 #pragma unroll
   for (int k = 0; k < 3; k++) {
-    rV[3 * (size_t)pv + k] = -b[k] - hq[k] - (rel * dg[k] + ab) * d[k];
+    const double r = -b[k] - hq[k] - (rel * dg[k] + ab) * d[k];
+    rV[3 * (size_t)pv + k] = r;
     bV64[3 * (size_t)pv + k] = b[k];
+    mr = fmax(mr, fabs(r));
+    mb = fmax(mb, fabs(b[k]));
   }
 }
 
@@ -427,7 +431,7 @@ __device__ __forceinline__ void ir_res_point(const BAView& v, const double* __re
 __device__ __forceinline__ void ir_res_cam(const BAView& v, const double* __restrict__ xU,
                                            const double* __restrict__ xV, double* __restrict__ rU,
                                            double* __restrict__ bU64, const DevCtl* ctl, int u, double (*red)[18],
-                                           double* pri) {
+                                           double* pri, double& mr, double& mb) {
   const int c = v.var_cam_user[u];
   double R[kCamRec];
   load_cam(v, c, R);
@@ -505,8 +509,11 @@ __device__ __forceinline__ void ir_res_cam(const BAView& v, const double* __rest
     t[1] += pri[6 + i];
     t[2] += pri[12 + i];
     const double rel = ctl->rel, ab = ctl->abs_;
-    rU[6 * (size_t)u + i] = -t[0] - t[1] - (rel * t[2] + ab) * xc[i];
+    const double r = -t[0] - t[1] - (rel * t[2] + ab) * xc[i];
+    rU[6 * (size_t)u + i] = r;
     bU64[6 * (size_t)u + i] = t[0];
+    mr = fabs(r);
+    mb = fabs(t[0]);
   }
 }
 
@@ -520,11 +527,33 @@ __global__ __launch_bounds__(256) void k_ir_res(BAView v, const double* __restri
   if (!gate_open(ctl, gate)) return;
   __shared__ double red[4][18];
   __shared__ double pri[18];
+  double mr = 0.0, mb = 0.0;  // max |r|, |b| of this thread's entries
   if ((int)blockIdx.x < nbp) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < v.P) ir_res_point(v, xU, xV, rV, bV64, ctl, p);
+    if (p < v.P) ir_res_point(v, xU, xV, rV, bV64, ctl, p, mr, mb);
   } else {
-    ir_res_cam(v, xU, xV, rU, bU64, ctl, (int)blockIdx.x - nbp, red, pri);
+    ir_res_cam(v, xU, xV, rU, bU64, ctl, (int)blockIdx.x - nbp, red, pri, mr, mb);
+  }
+  // the residual's and the gradient's largest entries (the refinement's
+  // stopping test), one atomic per wave
+  mr = wave_max(mr);
+  mb = wave_max(mb);
+  if ((threadIdx.x & 63) == 0) {
+    atomic_max_nonneg(&ctl->ir_maxr, mr);
+    atomic_max_nonneg(&ctl->ir_maxb, mb);
+  }
+}
+
+__global__ void k_ir_decide(DevCtl* c, int gate, double tol) {
+  if (!gate_open(c, gate)) return;
+  if (threadIdx.x == 0) {
+    const double mr = __longlong_as_double((long long)c->ir_maxr);
+    const double mb = __longlong_as_double((long long)c->ir_maxb);
+    const int more = tol <= 0.0 || mr > tol * mb;
+    c->ir_more = more;
+    c->ir_count += more;
+    c->ir_maxr = 0ull;
+    c->ir_maxb = 0ull;
   }
 }
 
@@ -820,6 +849,9 @@ __global__ void k_begin_step(DevCtl* c, int gate) {
     c->fail_v = 0;
     c->fail_s = 0;
     c->wait_timeout = 0;
+    c->ir_more = 1;
+    c->ir_maxr = 0ull;
+    c->ir_maxb = 0ull;
   }
 }
 
@@ -995,6 +1027,10 @@ void launch_set_damping(DevCtl* ctl, double rel, double abs_, hipStream_t st) {
 }
 void launch_begin_step(DevCtl* ctl, int gate, hipStream_t st) {
   hipLaunchKernelGGL(k_begin_step, dim3(1), dim3(64), 0, st, ctl, gate);
+}
+
+void launch_ir_decide(DevCtl* ctl, int gate, double tol, hipStream_t st) {
+  hipLaunchKernelGGL(k_ir_decide, dim3(1), dim3(64), 0, st, ctl, gate, tol);
 }
 
 // ---------------------------------------------------------------------------

@@ -36,6 +36,8 @@ __device__ __forceinline__ bool chol_gate(const DevCtl* c, int gate) {
     case GATE_NEED_LIN: return !c->done && c->need_lin;
     case GATE_SOLVE_OK: return !c->done && c->solve_ok;
     case GATE_ROLLBACK: return c->rollback_pending != 0;
+    case GATE_IR: return !c->done && c->ir_more;
+    case GATE_IR_ANY: return c->ir_more != 0;
   }
   return true;
 }

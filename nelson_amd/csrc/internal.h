@@ -39,6 +39,11 @@ struct DevCtl {
   int wait_timeout;       // a bounded device wait expired
   int total_solves;
   int rollback_pending;   // rejected LM step: restore the backup
+  // iterative refinement (mixed precision): max |r| and max |b| of the last
+  // FP64 residual (bit patterns of non-negative doubles), whether the step
+  // goes on, and the refinement steps run so far
+  unsigned long long ir_maxr, ir_maxb;
+  int ir_more, ir_count;
   int accepted;           // last LM step accepted
   int n_log, log_cap;
   int mode;               // 0 = primitives, 1 = LM loop, 2 = GN loop

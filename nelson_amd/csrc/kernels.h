@@ -20,6 +20,8 @@ enum Gate : int {
   GATE_NEED_LIN = 2,        // !done && need_lin
   GATE_SOLVE_OK = 3,        // !done && solve_ok
   GATE_ROLLBACK = 4,        // rollback_pending
+  GATE_IR = 5,              // !done && ir_more (a refinement step goes on)
+  GATE_IR_ANY = 6,          // ir_more
 };
 
 // Read-only structure and state pointers of one BA problem on the device.
@@ -182,6 +184,9 @@ void launch_gn_after_step(DevCtl* ctl, DevLog* log, hipStream_t st);
 void launch_gn_after_lin(DevCtl* ctl, DevLog* log, hipStream_t st);
 void launch_set_damping(DevCtl* ctl, double rel, double abs_, hipStream_t st);
 void launch_begin_step(DevCtl* ctl, int gate, hipStream_t st);
+// after a refinement residual: the step goes on (ctl->ir_more) while
+// max |r| > tol * max |b| (tol <= 0: always); clears the maxima
+void launch_ir_decide(DevCtl* ctl, int gate, double tol, hipStream_t st);
 // adds the off-diagonal U blocks (i < j, 6x6 row-major, nb of them) to the
 // assembled S tiles
 template <typename T>

@@ -277,10 +277,21 @@ class BAProblem:
     def handle(self):
         return self._h
 
-    def set_refinement(self, steps):
-        """Iterative-refinement steps of every mixed-precision increment
-        (0..8, default 2; ignored in FP64)."""
-        check(self._lib.nsn_set_refinement(self._h, int(steps)))
+    def set_refinement(self, steps, tol=None):
+        """Iterative refinement of every mixed-precision increment (ignored
+        in FP64): exactly `steps` steps (0..8), or with `tol` at most `steps`
+        steps, stopping once the FP64 residual's largest entry is at most
+        tol times the gradient's.  Default: at most 2 steps, tol 1e-10."""
+        if tol is None:
+            check(self._lib.nsn_set_refinement(self._h, int(steps)))
+        else:
+            check(self._lib.nsn_set_refinement_tol(self._h, int(steps), float(tol)))
+
+    def refinement_count(self):
+        """Refinement solves run since the problem was created."""
+        n = C.c_int(0)
+        check(self._lib.nsn_refinement_count(self._h, C.byref(n)))
+        return n.value
 
     # parameters ------------------------------------------------------------
     def set_state(self, cams, pts):

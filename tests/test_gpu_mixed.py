@@ -4,8 +4,10 @@ residual) on Config 1's problem, against the FP64 oracle.
 
 Reported figures (printed with -s): the increment error after 0..3
 refinement steps, the increment error along an LM trajectory and the final
-cost error.  Bars: with the default 2 refinement steps the increment meets
-the FP64 parity bar (1e-9); the mixed LM reaches the FP64 final cost to 1e-6.
+cost error.  Bars: with 2 refinement steps, and with the default adaptive
+refinement (at most 2 steps, stopping on the device once the residual is at
+1e-10 of the gradient), the increment meets the FP64 parity bar (1e-9); the
+mixed LM reaches the FP64 final cost to 1e-6.
 """
 import numpy as np
 import pytest
@@ -51,8 +53,10 @@ def test_mixed_increment_along_trajectory(cfg1):
     increment and the FP64 GPU increment vs the oracle's FP64 increment at
     the same state and damping.  Near the minimum b is at rounding level and
     any two FP64 implementations differ by more than 1e-9 relative, so the
-    bar is: mixed error <= max(1e-9, 10 x the FP64 GPU path's error)."""
+    bar is: mixed error <= max(1e-9, 10 x the FP64 GPU path's error), with
+    two refinement steps."""
     pm = BAProblem.from_generated(cfg1, precision="mixed")
+    pm.set_refinement(2)
     p64 = BAProblem.from_generated(cfg1)
     o = OracleBA.from_generated(cfg1)
     mu = 1.0
@@ -116,3 +120,36 @@ def test_refinement_without_variable_points(cfg1):
     rl = OracleBA.from_generated(cfg1, pt_fixed=pf).lm_solve(maxNumIt=30)
     cost = lm.result()["final_chi2"]
     assert abs(cost - rl["final_chi2"]) <= 1e-6 * rl["final_chi2"], (cost, rl["final_chi2"])
+
+
+@pytest.mark.parametrize("ab", [1e-3, 1.0])
+def test_adaptive_refinement(cfg1, ab):
+    """The default refinement stops on the device once the FP64 residual's
+    largest entry is at most 1e-10 of the gradient's (at most 2 steps): it
+    runs 1..2 solves per increment and meets the FP64 parity bar (1e-9)
+    against the oracle.  A tolerance above the FP32 solve's residual runs no
+    solve and gives the unrefined increment bit for bit; tolerance 0 gives
+    the fixed-step increment bit for bit."""
+    o = OracleBA.from_generated(cfg1)
+    o.update(True)
+    ro = o.schur_step(0.0, ab)
+
+    def run(setup):
+        p = BAProblem.from_generated(cfg1, precision="mixed")
+        setup(p)
+        p.update(True)
+        c0 = p.refinement_count()
+        ok, (dxu, dxv) = _step(p, 0.0, ab)
+        assert ok
+        return np.concatenate([dxu, dxv]), p.refinement_count() - c0
+
+    x, n = run(lambda p: None)
+    e = relerr(x, np.concatenate([ro["xU"], ro["xV"]]))
+    print(f"\nadaptive refinement (abs damping {ab:g}): {n} solves, increment error vs oracle {e:.2e}")
+    assert 1 <= n <= 2 and e <= 1e-9, (n, e)
+    x0, n0 = run(lambda p: p.set_refinement(0))
+    xl, nl = run(lambda p: p.set_refinement(3, tol=1e-2))
+    assert (n0, nl) == (0, 0) and np.array_equal(x0, xl)
+    x2, n2 = run(lambda p: p.set_refinement(2))
+    x2t, n2t = run(lambda p: p.set_refinement(2, tol=0.0))
+    assert (n2, n2t) == (2, 2) and np.array_equal(x2, x2t)

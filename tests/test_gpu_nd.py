@@ -8,11 +8,13 @@ the elimination order, so results equal the given order's to rounding:
     LM log to 1e-9, the state after LM in camera order;
   * a 600-camera open path (several chains, separators reaching over their
     subtrees): increments solve the oracle's damped normal equations to a
-    componentwise backward error of 1e-12 (FP64) / 1e-10 (mixed, refined)
+    componentwise backward error of 1e-12 (FP64) / 1e-10 (mixed, two
+    refinement steps)
     and equal the given order's to 1e-10;
   * the 1000-camera ring and the 4000-camera path at full size: the chains
     the plan reports, the backward error of the increment (1e-12 FP64,
-    1e-10 mixed), and the increment against the given order's;
+    1e-10 mixed with two refinement steps), and the increment against the
+    given order's;
   * mixed-precision LM reaches the FP64 final cost (1e-6), the Gauss-Newton
     adjoint equals the given order's (1e-9)."""
 import numpy as np
@@ -86,6 +88,9 @@ def test_nd_open_path_solves_normal_equations(precision):
     g = _path(600, 30, 60000, 33)
     given = BAProblem.from_generated(g, precision=precision)
     nd = BAProblem.from_generated(g, precision=precision, camera_order="nd")
+    if precision == "mixed":
+        given.set_refinement(2)
+        nd.set_refinement(2)
     print(f"\npath 600: {nd.n_chains} chains, {nd.chain_steps} of {given.chain_steps} steps, "
           f"tiles {nd.n_tiles} (given {given.n_tiles})")
     assert nd.n_chains >= 4 and nd.chain_steps < given.chain_steps // 2
@@ -115,6 +120,8 @@ def test_nd_full_size(name, expect_chains):
     ref = _increments(given, [(0.0, 1e-4)])[0]
     for precision in ("fp64", "mixed"):
         p = BAProblem.from_generated(g, precision=precision, camera_order="nd")
+        if precision == "mixed":
+            p.set_refinement(2)
         print(f"\n{name} {precision}: {p.n_chains} chains, {p.chain_steps} of {given.chain_steps} steps, "
               f"{p.n_tiles} tiles (given {given.n_tiles})")
         assert p.n_chains == expect_chains
