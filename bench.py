@@ -320,6 +320,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-sparse", action="store_true")
+    ap.add_argument("--no-given", action="store_true",
+                    help="skip the FP64 LM with the cameras in the given order (reported beside --order)")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--only-ladybug", action="store_true",
                     help="only the BA sample's workload on the ladybug graph (its own JSON line)")
@@ -355,7 +357,7 @@ def main():
     ph = chol_timing(nsn, p64)
     log(f"bench: fp64 phases (ms) {ph}")
     given = None
-    if args.order != "given":
+    if args.order != "given" and not args.no_given:
         # the same FP64 LM with the cameras in the given order (one chain)
         pgv, rgv = run_lm(nsn, g, "fp64", args.steps, args.warmup)
         given = {"it_per_s": rgv["it_per_s"], "ms_per_solve": rgv["ms_per_solve"], "final_chi2": rgv["final_chi2"],
