@@ -103,6 +103,8 @@ typedef struct {
   int lookahead;                               /* columns by which the factorisation's worker queue
                                                   starts the chains' pre-updates early (the largest
                                                   up to 12 its launch is checked to complete with) */
+  int chunk_tiles;                             /* scratch tiles of the sums split into chunks (sums
+                                                  of more than 128 tile products) */
 } nsn_problem_info;
 
 /* LevenbergMarquardtSettings (nelson/LevenbergMarquardt.h:18-34) */

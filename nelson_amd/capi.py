@@ -57,7 +57,8 @@ class ProblemInfo(C.Structure):
     _fields_ = [("n_var_cameras", C.c_int), ("n_var_points", C.c_int), ("n_wblocks", C.c_int),
                 ("n_s", C.c_int), ("n_s_blocks", C.c_int), ("n_s_pairs", C.c_longlong), ("n_tiles", C.c_int),
                 ("n_tiles_dense", C.c_int), ("factor_flops", C.c_double),
-                ("n_chains", C.c_int), ("chain_steps", C.c_int), ("lookahead", C.c_int)]
+                ("n_chains", C.c_int), ("chain_steps", C.c_int), ("lookahead", C.c_int),
+                ("chunk_tiles", C.c_int)]
 
 
 class LMSettings(C.Structure):

@@ -791,6 +791,7 @@ extern "C" int nsn_problem_info_get(nsn_problem* h, nsn_problem_info* info) {
   info->n_chains = h->tv.nchain;
   info->chain_steps = h->chain_steps;
   info->lookahead = h->chol_ahead;
+  info->chunk_tiles = h->n_part_tiles;
   return NSN_OK;
 }
 
@@ -818,9 +819,10 @@ extern "C" int nsn_problem_plan(const nsn_problem_desc* d, nsn_problem_info* inf
   info->factor_flops = tile_factor_flops(prof);
   info->n_chains = (int)prof.chain_start.size() - 1;
   info->chain_steps = tile_chain_steps(prof);
-  const int n_tasks = chol_queue(NT, 0, prof.first.data(), kCholMaxChains).ntasks;
+  const CholQueue cq0 = chol_queue(NT, 0, prof.first.data(), kCholMaxChains);
   info->lookahead = chol_pick_lookahead(NT, prof.first.data(), kCholMaxChains,
-                                        std::min(n_tasks, kCholMaxWG - info->n_chains));
+                                        std::min(cq0.ntasks, kCholMaxWG - info->n_chains));
+  info->chunk_tiles = cq0.nparts;
   if (cam_order)
     for (int u = 0; u < hs.NU; u++) cam_order[u] = hs.var_cam_user[u];
   return NSN_OK;
