@@ -73,7 +73,9 @@ def build(verbose=False, jobs=8):
         for c, o in zip(cmds, outs):
             print(" ".join(c))
     tmp = LIB + ".tmp"
-    _run(["hipcc", "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs)
+    # roctx ranges around the C ABI calls (rocprofiler-sdk-roctx)
+    _run(["hipcc", "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs +
+         ["-L/opt/rocm/lib", "-lrocprofiler-sdk-roctx"])
     os.replace(tmp, LIB)
     build_cpp_tests()
     return LIB

@@ -1,6 +1,7 @@
 // C ABI of the MI355X Schur LM step: problem handle, structure stage
 // (DoubleSection::structureReady), kernel sequences, LM / GN drivers.
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <algorithm>
 #include <chrono>
@@ -177,6 +178,9 @@ struct nsn_problem {
     lm_graph_tried = false;
   }
 };
+
+TraceRange::TraceRange(const char* name) { roctxRangePushA(name); }
+TraceRange::~TraceRange() { roctxRangePop(); }
 
 namespace {
 
@@ -644,6 +648,7 @@ static int problem_create_impl(const nsn_problem_desc* d, nsn_problem** out, lon
                                const std::vector<int>* uoff_i = nullptr, const std::vector<int>* uoff_j = nullptr);
 
 extern "C" int nsn_problem_create(const nsn_problem_desc* d, nsn_problem** out) {
+  TraceRange trace_range("nsn_problem_create");
   clear_error();
   long fa = -1;
   if (const char* e = std::getenv("NSN_TEST_FAIL_ALLOC")) fa = std::strtol(e, nullptr, 10);
@@ -691,6 +696,7 @@ static int problem_create_impl(const nsn_problem_desc* d, nsn_problem** out, lon
 }
 
 extern "C" int nsn_problem_create_blocks(const nsn_block_desc* d, nsn_problem** out) {
+  TraceRange trace_range("nsn_problem_create_blocks");
   clear_error();
   if (!d || !out) return fail(NSN_EINVAL, "nsn_problem_create_blocks: null argument");
   *out = nullptr;
@@ -902,6 +908,7 @@ static int reset_ctl_primitive(nsn_problem* h) {
 }
 
 extern "C" int nsn_linearize(nsn_problem* h, int with_hessian, double* chi2) {
+  TraceRange trace_range("nsn_linearize");
   if (h && h->blocks_only) return fail(NSN_ESTATE, "%s: a block problem has no edges or state (import its Hessian)", __func__);
   if (!h) return fail(NSN_EINVAL, "null argument");
   h->imported = false;
@@ -1034,6 +1041,7 @@ static int import_uoff_t(nsn_problem* h, const double* Uoff) {
 
 extern "C" int nsn_import_hessian_blocks(nsn_problem* h, const double* U, const double* Uoff, const double* V,
                                          const double* W, const double* bU, const double* bV, double chi2) {
+  TraceRange trace_range("nsn_import_hessian_blocks");
   if (!h) return fail(NSN_EINVAL, "null argument");
   if (h->n_uoff > 0) {
     if (!Uoff) return fail(NSN_EINVAL, "nsn_import_hessian_blocks: the problem has off-diagonal U blocks, Uoff is NULL");
@@ -1046,6 +1054,7 @@ extern "C" int nsn_import_hessian_blocks(nsn_problem* h, const double* U, const 
 
 extern "C" int nsn_import_hessian(nsn_problem* h, const double* U, const double* V, const double* W, const double* bU,
                                   const double* bV, double chi2) {
+  TraceRange trace_range("nsn_import_hessian");
   if (!h || !U || !V || !W || !bU || !bV) return fail(NSN_EINVAL, "null argument");
   HIP_TRY(hipSetDevice(h->device));
   int rc = h->prec == NSN_MIXED ? import_hessian_t<float>(h, U, V, W, bU, bV)
@@ -1078,6 +1087,7 @@ extern "C" int nsn_schur_init(nsn_problem* h) {
 }
 
 extern "C" int nsn_schur_step(nsn_problem* h, double rel, double abs_, double* dxu, double* dxv, double* sq_norm) {
+  TraceRange trace_range("nsn_schur_step");
   if (!h) return fail(NSN_EINVAL, "null argument");
   if (!h->linearized) return fail(NSN_ESTATE, "nsn_schur_step before nsn_linearize(with_hessian=1)");
   HIP_TRY(hipSetDevice(h->device));
@@ -1389,6 +1399,7 @@ int adjoint_t(nsn_problem* h, const AdjBufs& b, const double* g_cams, const doub
 
 extern "C" int nsn_adjoint(nsn_problem* h, const double* g_cams, const double* g_pts, int hessian_steps,
                            double* dL_dobs, double* dL_dw) {
+  TraceRange trace_range("nsn_adjoint");
   if (h && h->blocks_only) return fail(NSN_ESTATE, "%s: a block problem has no edges or state (import its Hessian)", __func__);
   if (!h) return fail(NSN_EINVAL, "null argument");
   if (hessian_steps < 0 || hessian_steps > 16)
@@ -1448,6 +1459,7 @@ size_t adj_scratch_need(const nsn_problem* h) {
 
 extern "C" int nsn_adjoint_noalloc(nsn_problem* h, const double* g_cams, const double* g_pts, int hessian_steps,
                                    double* dL_dobs, double* dL_dw) {
+  TraceRange trace_range("nsn_adjoint_noalloc");
   if (h && h->blocks_only) return fail(NSN_ESTATE, "%s: a block problem has no edges or state (import its Hessian)", __func__);
   if (!h) return fail(NSN_EINVAL, "null argument");
   if (hessian_steps < 0 || hessian_steps > 16)
@@ -1555,6 +1567,7 @@ static int run_lm(nsn_problem* h, const nsn_lm_settings* s, nsn_lm_result* res, 
 
 extern "C" int nsn_lm_solve(nsn_problem* h, const nsn_lm_settings* s, nsn_lm_result* res, nsn_log_entry* log,
                             int log_cap) {
+  TraceRange trace_range("nsn_lm_solve");
   if (h && h->blocks_only) return fail(NSN_ESTATE, "%s: a block problem has no edges or state (import its Hessian)", __func__);
   if (!h || !s || !res) return fail(NSN_EINVAL, "null argument");
   if (s->maxNumSubIt < 1) return fail(NSN_EINVAL, "maxNumSubIt must be >= 1");
@@ -1608,6 +1621,7 @@ static int run_gn(nsn_problem* h, const nsn_gn_settings* s, nsn_lm_result* res, 
 
 extern "C" int nsn_gn_solve(nsn_problem* h, const nsn_gn_settings* s, nsn_lm_result* res, nsn_log_entry* log,
                             int log_cap) {
+  TraceRange trace_range("nsn_gn_solve");
   if (h && h->blocks_only) return fail(NSN_ESTATE, "%s: a block problem has no edges or state (import its Hessian)", __func__);
   if (!h || !s || !res) return fail(NSN_EINVAL, "null argument");
   HIP_TRY(hipSetDevice(h->device));

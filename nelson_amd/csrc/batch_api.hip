@@ -219,6 +219,7 @@ int autotune(nsn_batch* b, const nsn_lm_settings* s) {
 }  // namespace
 
 extern "C" int nsn_batch_create(const nsn_problem_desc* wd, int n_windows, nsn_batch** out) {
+  TraceRange trace_range("nsn_batch_create");
   clear_error();
   if (!out) return fail(NSN_EINVAL, "nsn_batch_create: null argument");
   *out = nullptr;
@@ -424,6 +425,7 @@ extern "C" int nsn_batch_get_state(nsn_batch* b, double* cams, double* pts) {
 }
 
 extern "C" int nsn_batch_tune(nsn_batch* b, const nsn_lm_settings* s, int* nb, int* grid) {
+  TraceRange trace_range("nsn_batch_tune");
   if (!b || !s) return fail(NSN_EINVAL, "null argument");
   if (s->maxNumSubIt < 1) return fail(NSN_EINVAL, "maxNumSubIt must be >= 1");
   HIP_TRY(hipSetDevice(b->device));
@@ -445,6 +447,7 @@ extern "C" int nsn_batch_set_config(nsn_batch* b, int nb, int grid) {
 }
 
 extern "C" int nsn_batch_lm_solve(nsn_batch* b, const nsn_lm_settings* s, double* chi2_out, double* elapsed_ms) {
+  TraceRange trace_range("nsn_batch_lm_solve");
   if (!b || !s) return fail(NSN_EINVAL, "null argument");
   if (s->maxNumSubIt < 1) return fail(NSN_EINVAL, "maxNumSubIt must be >= 1");
   HIP_TRY(hipSetDevice(b->device));

@@ -54,6 +54,16 @@ struct DevCtl {
   double gn_rel, gn_abs;
 };
 
+// A roctx range (rocprofiler-sdk-roctx) around a library call: a rocprofv3
+// run with --marker-trace shows the C ABI calls beside the kernels they
+// launch.  Host-side markers only; no device work.
+struct TraceRange {
+  explicit TraceRange(const char* name);
+  ~TraceRange();
+  TraceRange(const TraceRange&) = delete;
+  TraceRange& operator=(const TraceRange&) = delete;
+};
+
 struct DevLog {
   int it, subit;
   double chi2, mu;

@@ -4,6 +4,8 @@ settings defaults are the reference's, and the seeded generator."""
 import ctypes as C
 import os
 import re
+import shutil
+import subprocess
 
 import numpy as np
 import pytest
@@ -210,3 +212,12 @@ def test_nested_dissection_plan(name, cfg, min_chains):
     assert ni["n_chains"] == nc >= min_chains, (ni["n_chains"], nc)
     assert ni["chain_steps"] < 0.8 * NT, (ni["chain_steps"], NT)
     assert ni["factor_flops"] > 0
+
+
+@pytest.mark.skipif(shutil.which("nm") is None, reason="nm not found")
+def test_library_marks_its_calls_with_roctx_ranges():
+    """The C ABI's long calls push and pop roctx ranges (rocprofiler-sdk-roctx),
+    so a rocprofv3 --marker-trace run shows them beside their kernels."""
+    out = subprocess.run(["nm", "-D", "--undefined-only", capi.library_path], stdout=subprocess.PIPE, text=True,
+                         check=True).stdout
+    assert "roctxRangePushA" in out and "roctxRangePop" in out
