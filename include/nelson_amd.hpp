@@ -289,6 +289,14 @@ class BASection {
     if (!ready_ || h_) throw Error(NSN_ESTATE, "permuteRCM_SchurU outside parametersReady .. structureReady");
     order_ = NSN_ORDER_RCM;
   }
+  // nested dissection of the camera graph: independent parts (whose
+  // diagonal chains the factorisation and the triangular solves run at the
+  // same time) and their separators, aligned to the 64-row tiles of S; the
+  // interface stays in camera order, as with permuteRCM_SchurU
+  void permuteND_SchurU() {
+    if (!ready_ || h_) throw Error(NSN_ESTATE, "permuteND_SchurU outside parametersReady .. structureReady");
+    order_ = NSN_ORDER_ND;
+  }
   // one reprojection edge (EdgeBinaryUV) between camera i and point j
   void addEdge(int i, int j, double u, double v, double weight = 1.0) {
     if (!ready_) throw Error(NSN_ESTATE, "addEdge before parametersReady");

@@ -4,8 +4,10 @@
 // (10 cameras, 500 points, 2000 observations) built through parameterU /
 // parameterV / addEdge / structureReady, one Schur step, an LM solve and a
 // GN solve, then a step and an LM solve with the cameras ordered by
-// permuteRCM_SchurU.  Prints one JSON object per solve on stdout; the GPU test
-// compares them with the oracle.
+// permuteRCM_SchurU, and a step of a 300-camera ring problem with the cameras
+// in nested-dissection order (permuteND_SchurU: independent diagonal chains).
+// Prints one JSON object per solve on stdout; the GPU test compares them with
+// the oracle.
 #include <cmath>
 #include <cstdio>
 #include <vector>
@@ -16,7 +18,7 @@ using namespace nelson_amd;
 
 static void build(BASection& pss, const std::vector<double>& cams, const std::vector<double>& pts,
                   const std::vector<int>& ci, const std::vector<int>& pi, const std::vector<double>& obs, int C,
-                  int P, bool priors = false, bool rcm = false) {
+                  int P, bool priors = false, bool rcm = false, bool nd = false) {
   pss.setNumParameters(C, P);
   for (int i = 0; i < C; i++)
     for (int k = 0; k < 3; k++) {
@@ -29,6 +31,7 @@ static void build(BASection& pss, const std::vector<double>& cams, const std::ve
   pss.setFixedV(0);
   pss.parametersReady();
   if (rcm) pss.permuteRCM_SchurU();
+  if (nd) pss.permuteND_SchurU();
   for (size_t e = 0; e < ci.size(); e++) pss.addEdge(ci[e], pi[e], obs[2 * e], obs[2 * e + 1]);
   if (priors) {
     for (int c : {0, 2, 5, 7}) {
@@ -136,6 +139,34 @@ int main() {
       lm.settings().maxNumIt = 20;
       const TerminationReason tc = lm.solve(pss);
       print_log("lm_rcm", tc, lm.numIterations(), pss.update(false), lm.stats());
+    }
+    {
+      // a 300-camera ring (each point seen by 4-8 cameras of a 40-camera arc)
+      // in nested-dissection order: one step, the increment in camera order
+      nsn_gen_params r = g;
+      r.n_cameras = 300; r.n_points = 30000; r.k_min = 4; r.k_max = 8; r.layout = 1; r.arc = 40; r.seed = 31;
+      const int RC = r.n_cameras, RP = r.n_points, rcap = RP * r.k_max;
+      std::vector<double> rcgt(6 * RC), rpgt(3 * RP), rc0(6 * RC), rp0(3 * RP), robs(2 * rcap);
+      std::vector<int> rci(rcap), rpi(rcap);
+      int rne = 0;
+      check(nsn_generate_ba(&r, rcap, rcgt.data(), rpgt.data(), rc0.data(), rp0.data(), rci.data(), rpi.data(),
+                            robs.data(), &rne),
+            "nsn_generate_ba");
+      rci.resize(rne);
+      rpi.resize(rne);
+      robs.resize(2 * rne);
+      BASection pss(r.focal, r.cx, r.cy);
+      build(pss, rc0, rp0, rci, rpi, robs, RC, RP, false, false, true);
+      pss.update(true);
+      SolverCholeskySchur solver;
+      solver.init(pss.hessian().H(), pss.hessian().b());
+      const double mu = 1e-4 * solver.maxAbsHDiag();
+      const bool ok = solver.computeIncrement(pss.hessian().H(), pss.hessian().b(), 0.0, mu);
+      std::printf("{\"solve\": \"step_nd\", \"ok\": %s, \"mu\": %.17g, \"du\": [", ok ? "true" : "false", mu);
+      for (int i = 0; i < RC - 1; i++)
+        for (int k = 0; k < 6; k++)
+          std::printf("%s%.17g", i + k ? ", " : "", solver.incrementVector().bU().segment(i)[k]);
+      std::printf("]}\n");
     }
   } catch (const Error& e) {
     std::fprintf(stderr, "nelson_amd error %d: %s\n", e.code(), e.what());

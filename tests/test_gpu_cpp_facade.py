@@ -9,7 +9,7 @@ import subprocess
 
 import pytest
 
-from nelson_amd import CONFIGS, generate_ba
+from nelson_amd import CONFIGS, BAProblem, generate_ba
 from oracle.oracle import OracleBA
 
 from test_gpu_parity import _compare_logs
@@ -101,3 +101,22 @@ def test_cpp_rcm_order(results):
     dl = results["lm_rcm"]
     detail = _compare_logs([tuple(e) for e in dl["log"]], rl["log"], "cpp lm rcm")
     assert abs(dl["final_chi2"] - rl["final_chi2"]) <= 1e-9 * rl["final_chi2"], detail
+
+
+def test_cpp_nested_dissection_order(results):
+    """permuteND_SchurU on a 300-camera ring (the solver runs two independent
+    diagonal chains): the camera increment of one damped step, in camera
+    order, equals the oracle's (its pivoted LDL^T of the dense S) to 1e-9."""
+    import numpy as np
+    g = generate_ba(dict(CONFIGS["cfg2"], n_cameras=300, n_points=30000, arc=40, seed=31))
+    assert BAProblem.plan(g, "nd")[0]["n_chains"] >= 2
+    d = results["step_nd"]
+    o = OracleBA.from_generated(g)
+    o.update(True)
+    ro = o.schur_step(0.0, d["mu"])
+    assert d["ok"] and ro["ok"]
+    du = np.array(d["du"])
+    assert du.shape == ro["xU"].shape
+    err = np.abs(du - ro["xU"]).max() / np.abs(ro["xU"]).max()
+    print(f"\nC++ nested-dissection step vs oracle: {err:.2e}")
+    assert err <= 1e-9
