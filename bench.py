@@ -169,8 +169,11 @@ def adjoint_timing(nsn):
 def batch_timing(nsn, n_windows=512, iters=10):
     """Batched local BA: 512 independent windows (cameras U{5..40}, points
     U{200..3000}, 3-6 observations per point, 5% outliers at 20 px, Huber
-    delta 2 px), mixed precision, LM with `iters` iterations on every window
-    in one persistent launch; launch shape from the autotuner."""
+    delta 2 px), mixed precision (no refinement, the batch default), LM
+    with `iters` iterations on every window in one persistent launch; launch
+    shape from the autotuner.  The same solve with refinement (at most 2
+    steps per increment, tol 1e-10) and the FP64 batch's are timed beside
+    it, with each mixed variant's final costs against the FP64 batch's."""
     import numpy as np
     wins = [nsn.generate_ba(nsn.window_config(w)) for w in range(n_windows)]
     t0 = time.perf_counter()
@@ -179,18 +182,34 @@ def batch_timing(nsn, n_windows=512, iters=10):
     s = nsn.LevenbergMarquardtSettings(maxNumIt=iters)
     b.tune(s)
     cams0, pts0 = b.get_state()
-    times = []
-    for _ in range(3):
-        b.set_state(cams0, pts0)
-        chi2, ms = b.lm_solve(s)
-        times.append(ms)
+
+    def timed(bb):
+        times = []
+        for _ in range(3):
+            bb.set_state(cams0, pts0)
+            chi2, ms = bb.lm_solve(s)
+            times.append(ms)
+        return chi2, min(times)
+
+    chi2, ms = timed(b)
     reasons, its, subs = b.results()
-    ms = min(times)
+    b.set_refinement(2, 1e-10)
+    chi2_ir, ms_ir = timed(b)
+    nref = b.refinement_counts()
+    _, _, subs_ir = b.results()
+    b64 = nsn.BABatch(wins, precision="fp64")
+    b64.set_config(*[b.config()[k] for k in ("variant", "grid")])
+    chi2_64, ms_64 = timed(b64)
     return {"workload": f"{n_windows} windows, {sum(g['n_cameras'] for g in wins)} cameras / "
                         f"{sum(g['n_points'] for g in wins)} points / {sum(g['n_edges'] for g in wins)} observations, "
                         f"mixed precision, Huber 2 px, LM {iters} iterations",
             "batch_solves_per_s": 1e3 / ms, "windows_per_s": n_windows * 1e3 / ms, "ms_per_batch_solve": ms,
             "mean_iterations": float(np.mean(its)), "mean_sub_iterations": float(np.mean(subs)),
+            "final_cost_max_rel_err_vs_fp64": float(np.max(np.abs(chi2 - chi2_64) / chi2_64)),
+            "refined": {"ms_per_batch_solve": ms_ir,
+                        "refinement_solves_per_increment": float(np.sum(nref) / max(1, np.sum(subs_ir))),
+                        "final_cost_max_rel_err_vs_fp64": float(np.max(np.abs(chi2_ir - chi2_64) / chi2_64))},
+            "fp64_ms_per_batch_solve": ms_64,
             "config": b.config(), "create_s": t1 - t0}
 
 

@@ -388,6 +388,17 @@ int nsn_batch_tune(nsn_batch* b, const nsn_lm_settings* s, int* variant, int* gr
 int nsn_batch_set_config(nsn_batch* b, int variant, int grid);
 /* the launch shape in use and the tuner's timings (ms, in candidate order) */
 int nsn_batch_config(nsn_batch* b, int* variant, int* grid, double* tune_ms, int n_tune);
+/* mixed-precision windows: at most max_steps (0..8) refinement steps per
+ * increment, each adding the FP32 factor's solution for the window's FP64
+ * residual of the damped normal equations, stopping once max |r| <= tol *
+ * max |b| (as nsn_set_refinement_tol for one problem; tol 0: every step
+ * runs).  Default 0 steps (the unrefined kernel): a step evaluates the
+ * FP64 residual from every edge twice, which costs more than an FP64
+ * batch's whole iteration saves.  FP64 batches accept and ignore it.  The
+ * launch shape stays as tuned. */
+int nsn_batch_set_refinement(nsn_batch* b, int max_steps, double tol);
+/* refinement solves per window [n_windows] in the last nsn_batch_lm_solve */
+int nsn_batch_refinement_counts(nsn_batch* b, int* steps);
 
 /* ---- unary priors (nelson's EdgeUnary, EdgeUnary.h:76-93, for the BA
  * section) ----

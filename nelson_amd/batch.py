@@ -111,3 +111,16 @@ class BABatch:
         sub = np.zeros(self.n, np.int32)
         check(self._lib.nsn_batch_results(self._h, _ip(r), _ip(it), _ip(sub)))
         return [capi.TERMINATION_REASONS[int(x)] for x in r], it, sub
+
+    def set_refinement(self, max_steps, tol=1e-10):
+        """Mixed precision: at most max_steps iterative-refinement steps per
+        increment against each window's FP64 residual, stopping once
+        max|r| <= tol * max|b| (tol 0: every step runs); 0 (the default)
+        turns it off.  FP64 batches ignore it."""
+        check(self._lib.nsn_batch_set_refinement(self._h, int(max_steps), float(tol)))
+
+    def refinement_counts(self):
+        """Refinement solves per window in the last lm_solve."""
+        n = np.zeros(self.n, np.int32)
+        check(self._lib.nsn_batch_refinement_counts(self._h, _ip(n)))
+        return n

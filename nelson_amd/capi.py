@@ -152,6 +152,8 @@ SIGNATURES = {
     "nsn_batch_tune": (C.c_int, [_P, C.POINTER(LMSettings), _I, _I]),
     "nsn_batch_set_config": (C.c_int, [_P, C.c_int, C.c_int]),
     "nsn_batch_config": (C.c_int, [_P, _I, _I, _D, C.c_int]),
+    "nsn_batch_set_refinement": (C.c_int, [_P, C.c_int, C.c_double]),
+    "nsn_batch_refinement_counts": (C.c_int, [_P, _I]),
     "nsn_measure_peaks": (C.c_int, [C.c_int, _D, _D, _D]),
     "nsn_set_priors": (C.c_int, [_P, C.c_int, _I, _D, _D, C.c_int, _I, _D, _D]),
 }

@@ -350,173 +350,6 @@ __global__ __launch_bounds__(256) void k_backsub(BAView v, const T* __restrict__
   }
 }
 
-// ---------------------------------------------------------------------------
-// Iterative refinement of the mixed-precision increment.  The FP32 Schur
-// solve M approximates the damped normal equations (H + D) x = -b; each
-// refinement step takes the FP64 residual
-//   res = -b - (H + D) x,  H = sum_e J_e^T om_e J_e,  b = sum_e J_e^T om_e r_e,
-//   D = diag(rel H_ii + abs) (SolverCholeskySchur.hpp:121-124,
-//   MatrixDiagInv.hpp:36-40),
-// evaluated matrix-free from the edges with J, r and om in FP64 at the
-// current state (no FP32 block enters it), and adds M^-1 res to x.  The
-// FP64 gradient b is written as well: the gain-ratio denominator
-// x^T (mu x - b) uses it (DoubleSection.h:300-311).
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ void ir_res_point(const BAView& v, const double* __restrict__ xU,
-                                             const double* __restrict__ xV, double* __restrict__ rV,
-                                             double* __restrict__ bV64, const DevCtl* ctl, int p, double& mr,
-                                             double& mb) {
-  const int pv = v.pt_var[p];
-  if (pv < 0) return;
-  const double X0 = v.pts[3 * p], X1 = v.pts[3 * p + 1], X2 = v.pts[3 * p + 2];
-  const double d0 = xV[3 * (size_t)pv], d1 = xV[3 * (size_t)pv + 1], d2 = xV[3 * (size_t)pv + 2];
-  double b[3] = {0, 0, 0}, hq[3] = {0, 0, 0}, dg[3] = {0, 0, 0};
-  const int e1 = v.pt_ptr[p + 1];
-  for (int e = v.pt_ptr[p]; e < e1; e++) {
-    const int c = v.pe_cam[e];
-    double R[kCamRec];
-    load_cam(v, c, R);
-    const Proj pr = project(v, R, X0, X1, X2, v.pe_obs[2 * e], v.pe_obs[2 * e + 1], v.pe_w[e]);
-    double g0[3], g1[3];
-    jac_pt<double>(v, pr, R, g0, g1);
-    double q0 = g0[0] * d0 + g0[1] * d1 + g0[2] * d2;
-    double q1 = g1[0] * d0 + g1[1] * d1 + g1[2] * d2;
-    const int cv = v.cam_var[c];
-    if (cv >= 0) {
-      double j0[6], j1[6];
-      jac_cam<double>(v, pr, j0, j1);
-      const double* xc = xU + 6 * (size_t)cv;
-#pragma unroll
-      for (int i = 0; i < 6; i++) {
-        q0 += j0[i] * xc[i];
-        q1 += j1[i] * xc[i];
-      }
-    }
-    const double om = pr.om;
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-      b[k] += om * (g0[k] * pr.r0 + g1[k] * pr.r1);
-      hq[k] += om * (g0[k] * q0 + g1[k] * q1);
-      dg[k] += om * (g0[k] * g0[k] + g1[k] * g1[k]);
-    }
-  }
-  const double d[3] = {d0, d1, d2};
-  if (v.pt_prior) {  // the point's unary prior: J = diag(w)
-    const int k = v.pt_prior[p];
-    if (k >= 0) {
-      const double Xp[3] = {X0, X1, X2};
-#pragma unroll
-      for (int i = 0; i < 3; i++) {
-        const double w2 = v.pt_prior_w[3 * (size_t)k + i] * v.pt_prior_w[3 * (size_t)k + i];
-        b[i] += w2 * (Xp[i] - v.pt_prior_p[3 * (size_t)k + i]);
-        hq[i] += w2 * d[i];
-        dg[i] += w2;
-      }
-    }
-  }
-  const double rel = ctl->rel, ab = ctl->abs_;
-#pragma unroll
-  for (int k = 0; k < 3; k++) {
-    const double r = -b[k] - hq[k] - (rel * dg[k] + ab) * d[k];
-    rV[3 * (size_t)pv + k] = r;
-    bV64[3 * (size_t)pv + k] = b[k];
-    mr = fmax(mr, fabs(r));
-    mb = fmax(mb, fabs(b[k]));
-  }
-}
-
-// one workgroup per variable camera; deterministic reduction
-// red: 4 x 18 doubles of LDS; pri: 18 (the camera's unary prior: J^T r,
-// J^T J x, diag(J^T J)); the whole workgroup calls it for one camera
-__device__ __forceinline__ void ir_res_cam(const BAView& v, const double* __restrict__ xU,
-                                           const double* __restrict__ xV, double* __restrict__ rU,
-                                           double* __restrict__ bU64, const DevCtl* ctl, int u, double (*red)[18],
-                                           double* pri, double& mr, double& mb) {
-  const int c = v.var_cam_user[u];
-  double R[kCamRec];
-  load_cam(v, c, R);
-  double xc[6];
-#pragma unroll
-  for (int i = 0; i < 6; i++) xc[i] = xU[6 * (size_t)u + i];
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int i = 0; i < 18; i++) pri[i] = 0.0;
-    const int kp = v.cam_prior ? v.cam_prior[c] : -1;
-    if (kp >= 0) {
-      double r[6], J[36];
-      cam_prior_eval<true>(v, c, kp, r, J);
-#pragma unroll
-      for (int m = 0; m < 6; m++) {
-        double jx = 0.0;
-#pragma unroll
-        for (int n = 0; n < 6; n++) jx += J[m * 6 + n] * xc[n];
-#pragma unroll
-        for (int i = 0; i < 6; i++) {
-          pri[i] += J[m * 6 + i] * r[m];
-          pri[6 + i] += J[m * 6 + i] * jx;
-          pri[12 + i] += J[m * 6 + i] * J[m * 6 + i];
-        }
-      }
-    }
-  }
-  double acc[18];
-#pragma unroll
-  for (int i = 0; i < 18; i++) acc[i] = 0.0;
-  const int e1 = v.cam_ptr[u + 1];
-  for (int e = v.cam_ptr[u] + threadIdx.x; e < e1; e += 256) {
-    const int q = v.ce_pt[e];
-    const Proj pr = project(v, R, v.pts[3 * q], v.pts[3 * q + 1], v.pts[3 * q + 2], v.ce_obs[2 * e],
-                            v.ce_obs[2 * e + 1], v.ce_w[e]);
-    double j0[6], j1[6];
-    jac_cam<double>(v, pr, j0, j1);
-    double q0 = 0.0, q1 = 0.0;
-#pragma unroll
-    for (int i = 0; i < 6; i++) {
-      q0 += j0[i] * xc[i];
-      q1 += j1[i] * xc[i];
-    }
-    const int pv = v.pt_var[q];
-    if (pv >= 0) {
-      double g0[3], g1[3];
-      jac_pt<double>(v, pr, R, g0, g1);
-      const double* xp = xV + 3 * (size_t)pv;
-#pragma unroll
-      for (int k = 0; k < 3; k++) {
-        q0 += g0[k] * xp[k];
-        q1 += g1[k] * xp[k];
-      }
-    }
-    const double om = pr.om;
-#pragma unroll
-    for (int i = 0; i < 6; i++) {
-      acc[i] += om * (j0[i] * pr.r0 + j1[i] * pr.r1);
-      acc[6 + i] += om * (j0[i] * q0 + j1[i] * q1);
-      acc[12 + i] += om * (j0[i] * j0[i] + j1[i] * j1[i]);
-    }
-  }
-#pragma unroll
-  for (int i = 0; i < 18; i++) {
-    const double s = wave_sum(acc[i]);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][i] = s;
-  }
-  __syncthreads();
-  if (threadIdx.x < 6) {
-    const int i = threadIdx.x;
-    double t[3];
-#pragma unroll
-    for (int k = 0; k < 3; k++) t[k] = ((red[0][6 * k + i] + red[1][6 * k + i]) + red[2][6 * k + i]) + red[3][6 * k + i];
-    t[0] += pri[i];
-    t[1] += pri[6 + i];
-    t[2] += pri[12 + i];
-    const double rel = ctl->rel, ab = ctl->abs_;
-    const double r = -t[0] - t[1] - (rel * t[2] + ab) * xc[i];
-    rU[6 * (size_t)u + i] = r;
-    bU64[6 * (size_t)u + i] = t[0];
-    mr = fabs(r);
-    mb = fabs(t[0]);
-  }
-}
-
 // the FP64 residual of the damped normal equations, res = -b - (H + D) x,
 // matrix free from the edges, in one launch: the first nbp workgroups take
 // the points (one thread per point), the next NU one variable camera each

@@ -28,15 +28,23 @@ struct BWin {
   void* S;                                    // nP x nP, column-major, lower
   void* Linv;                                 // nP / 64 inverses of the diagonal factor tiles
   double *xU, *xV, *cams_bak, *pts_bak;       // xU: nP entries
+  // iterative refinement (mixed precision): the FP64 residual (rU 6 NU,
+  // rV 3 NV) and gradient (bU64, bV64), its reduction to the camera system
+  // (rS) and the camera correction (zU), nP entries each
+  double *rU, *rV, *bU64, *bV64, *rS, *zU;
+  int ir_steps;    // at most this many refinement steps per increment
+  double ir_tol;   // a step stops after its residual once max |r| <= ir_tol * max |b| (0: never)
 };
 
 // One persistent launch: `grid` workgroups of 256 threads take windows from
 // counter[0] (zeroed by the caller) in the order counter[1 .. nwin] (largest
 // windows first, so that the last windows taken are short) and run each window's LM solve with the
 // settings in its control block.  nb: the factorisation variant, 8 or 16
-// (panels of that width) or 64 (64x64 tiles).
+// (panels of that width) or 64 (64x64 tiles).  refine: the instance that
+// refines every increment (mixed precision only; -2 for FP64).
 template <typename T>
-int launch_batch_lm(const BWin* wins, int nwin, DevCtl* ctls, int* counter, int nb, int grid, hipStream_t st);
+int launch_batch_lm(const BWin* wins, int nwin, DevCtl* ctls, int* counter, int nb, int grid, bool refine,
+                    hipStream_t st);
 
 // zeroes the window counter of the persistent launch (a kernel, so that the
 // launch can be captured in a graph without a memset node)

@@ -51,7 +51,7 @@ __device__ __forceinline__ T* tp(void* p) {
 // update(true): V, bV, W (point-major), U, bU (one wave per camera), chi2,
 // max |b| and max |diag H|
 template <typename T>
-__device__ void win_linearize(const BWin& w, DevCtl* c, double* sh) {
+__device__ __forceinline__ void win_linearize(const BWin& w, DevCtl* c, double* sh) {
   const BAView v = w.v;  // a copy: the stores below cannot alias it
   T* V = tp<T>(w.V);
   T* bV = tp<T>(w.bV);
@@ -108,7 +108,7 @@ __device__ void win_linearize(const BWin& w, DevCtl* c, double* sh) {
 
 // update(false): chi2 only
 template <typename T>
-__device__ void win_cost(const BWin& w, DevCtl* c, double* sh) {
+__device__ __forceinline__ void win_cost(const BWin& w, DevCtl* c, double* sh) {
   double chi2 = 0.0, mb = 0.0, md = 0.0;
   const BAView v = w.v;
   for (int p = threadIdx.x; p < v.P; p += kBT)
@@ -117,7 +117,7 @@ __device__ void win_cost(const BWin& w, DevCtl* c, double* sh) {
   if (threadIdx.x == 0) c->chi2 = tot;
 }
 
-__device__ void win_copy(const BAView& v, double* cd, double* pd, const double* cs, const double* ps) {
+__device__ __forceinline__ void win_copy(const BAView& v, double* cd, double* pd, const double* cs, const double* ps) {
   for (int i = threadIdx.x; i < 12 * v.C; i += kBT) cd[i] = cs[i];
   for (int i = threadIdx.x; i < 3 * v.P; i += kBT) pd[i] = ps[i];
 }
@@ -127,7 +127,7 @@ __device__ void win_copy(const BAView& v, double* cd, double* pd, const double* 
 // (pan: NB columns of kMaxS rows), the trailing matrix is updated from the
 // panel; false if a pivot is not positive
 template <typename T, int NB>
-__device__ bool win_cholesky_panels(T* S, int n, int ld, double* pan) {
+__device__ __forceinline__ bool win_cholesky_panels(T* S, int n, int ld, double* pan) {
   const int tid = threadIdx.x;
   bool ok = true;
   for (int k0 = 0; k0 < n; k0 += NB) {
@@ -176,7 +176,7 @@ __device__ bool win_cholesky_panels(T* S, int n, int ld, double* pan) {
 // column sweeps L y = b, then L^T x = y, on the panel variant's factor; b in
 // yv (LDS), x to xU
 template <typename T>
-__device__ void win_solve_sweeps(const T* S, int n, int ld, double* yv, double* ys, double* xU) {
+__device__ __forceinline__ void win_solve_sweeps(const T* S, int n, int ld, double* yv, double* ys, double* xU) {
   const int tid = threadIdx.x;
   for (int cc = 0; cc < n; cc++) {
     const double yc = yv[cc] / (double)S[(size_t)cc * ld + cc];
@@ -256,7 +256,7 @@ __device__ __forceinline__ void acc_zero(CholAcc<T>& acc) {
 // Linv[j], the tiles below become C L_jj^-T and the trailing tiles are
 // updated, all products on MFMA.  Pivot failures are counted in c->fail_s.
 template <typename T>
-__device__ void win_cholesky_tiles(const BWin& w, DevCtl* c, double* lds) {
+__device__ __forceinline__ void win_cholesky_tiles(const BWin& w, DevCtl* c, double* lds) {
   double* R0d = lds;
   double* R1d = lds + 4096;
   T* R0 = (T*)lds;
@@ -338,17 +338,17 @@ __device__ __forceinline__ double tile_gemv(const T* M, int ld, const double* v,
 
 // L y = b then L^T x = y by 64-row blocks with the diagonal-tile inverses:
 // y_k = Linv_kk (b_k - sum_{i<k} L_ki y_i), x_k = Linv_kk^T (y_k - sum_{i>k}
-// L_ik^T x_i).  b: xU (global, nP entries, zero beyond n), overwritten by x.
-// ws: LDS workspace (nP + 64 + 256 doubles)
+// L_ik^T x_i).  b (global, nP entries, zero beyond n) and x (n entries
+// written) may be the same array.  ws: LDS workspace (nP + 64 + 256 doubles)
 template <typename T>
-__device__ void win_solve_tiles(const BWin& w, double* ws) {
+__device__ __forceinline__ void win_solve_tiles(const BWin& w, double* ws, const double* b, double* x) {
   const T* S = tp<T>(w.S);
   const T* Li = tp<T>(w.Linv);
   const int ld = w.nP, NTw = w.nP / 64, tid = threadIdx.x;
   double* yv = ws;                 // nP: y, then x
   double* tv = ws + w.nP;          // 64
   double* red = tv + 64;           // 256
-  for (int e = tid; e < w.nP; e += 256) yv[e] = w.xU[e];
+  for (int e = tid; e < w.nP; e += 256) yv[e] = b[e];
   __syncthreads();
   for (int k = 0; k < NTw; k++) {
     double acc = 0.0;
@@ -368,7 +368,7 @@ __device__ void win_solve_tiles(const BWin& w, double* ws) {
     if (tid < 64) yv[64 * k + tid] = x;
     __syncthreads();
   }
-  for (int e = tid; e < w.n; e += 256) w.xU[e] = yv[e];
+  for (int e = tid; e < w.n; e += 256) x[e] = yv[e];
   __syncthreads();
 }
 
@@ -408,7 +408,7 @@ __device__ __forceinline__ void load18(const double* p, double* o) {
 // their loads issued before the products), the last group padded with repeats
 // of its first term that are not added
 template <typename T>
-__device__ void win_s_chunks(const BWin& w) {
+__device__ __forceinline__ void win_s_chunks(const BWin& w) {
   constexpr int U = sizeof(T) == 4 ? 4 : 2;
   const T* Y = tp<T>(w.Y);  // Z
   const T* tW = tp<T>(w.tW);
@@ -458,8 +458,190 @@ __device__ void win_s_chunks(const BWin& w) {
   }
 }
 
+// the FP64 residual rows of variable camera u of a window (the camera part
+// of ir_res_cam) by one wave, lanes striding over the camera's edges and the
+// 18 sums closed by a butterfly: r_u = -b_u - (H x)_u - (rel diag(H)_u +
+// abs) x_u with b_u = sum_e J_e^T om_e r_e, (H x)_u = sum_e J_e^T om_e (J_e
+// x), J_e = [camera block, point block] of edge e.  Windows carry no unary
+// priors.  Lane 0 writes rU and bU64 and folds max |r|, max |b| into mr, mb.
+__device__ __forceinline__ void ir_res_cam_wave(const BAView& v, const double* __restrict__ xU,
+                                                const double* __restrict__ xV, double* __restrict__ rU,
+                                                double* __restrict__ bU64, const DevCtl* ctl, int u, int lane,
+                                                double& mr, double& mb) {
+  const int c = v.var_cam_user[u];
+  double R[kCamRec];
+  load_cam(v, c, R);
+  double xc[6];
+#pragma unroll
+  for (int i = 0; i < 6; i++) xc[i] = xU[6 * (size_t)u + i];
+  double acc[18];
+#pragma unroll
+  for (int i = 0; i < 18; i++) acc[i] = 0.0;
+  const int e1 = v.cam_ptr[u + 1];
+  for (int e = v.cam_ptr[u] + lane; e < e1; e += 64) {
+    const int q = v.ce_pt[e];
+    const Proj pr = project(v, R, v.pts[3 * q], v.pts[3 * q + 1], v.pts[3 * q + 2], v.ce_obs[2 * e],
+                            v.ce_obs[2 * e + 1], v.ce_w[e]);
+    double j0[6], j1[6];
+    jac_cam<double>(v, pr, j0, j1);
+    double q0 = 0.0, q1 = 0.0;
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+      q0 += j0[i] * xc[i];
+      q1 += j1[i] * xc[i];
+    }
+    const int pv = v.pt_var[q];
+    if (pv >= 0) {
+      double g0[3], g1[3];
+      jac_pt<double>(v, pr, R, g0, g1);
+      const double* xp = xV + 3 * (size_t)pv;
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        q0 += g0[k] * xp[k];
+        q1 += g1[k] * xp[k];
+      }
+    }
+    const double om = pr.om;
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+      acc[i] += om * (j0[i] * pr.r0 + j1[i] * pr.r1);
+      acc[6 + i] += om * (j0[i] * q0 + j1[i] * q1);
+      acc[12 + i] += om * (j0[i] * j0[i] + j1[i] * j1[i]);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 18; i++) acc[i] = wave_sum(acc[i]);
+  if (lane == 0) {
+    const double rel = ctl->rel, ab = ctl->abs_;
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+      const double r = -acc[i] - acc[6 + i] - (rel * acc[12 + i] + ab) * xc[i];
+      rU[6 * (size_t)u + i] = r;
+      bU64[6 * (size_t)u + i] = acc[i];
+      mr = fmax(mr, fabs(r));
+      mb = fmax(mb, fabs(acc[i]));
+    }
+  }
+}
+
+// Iterative refinement of a window's mixed-precision increment (as for a
+// single problem): each step evaluates the FP64 residual r = -b - (H + D) dx
+// of the damped normal equations from the edges (ir_res_point per point,
+// ir_res_cam_wave per camera, a wave each), stops once max |r| <= ir_tol *
+// max |b|, reduces r to the camera system (rS = rU - W V^-1 rV), solves it
+// with the window's FP32 factor and back-substitutes, adding the correction
+// to both parts of the increment.  Leaves the FP64 gradient in bU64 / bV64
+// for the step statistics.  Every thread of the workgroup calls it; lds: the
+// workgroup's 64 KiB (free after the solve, the control scalars at its end
+// untouched).
 template <typename T, int ALG>
-__device__ void win_schur(const BWin& w, DevCtl* c, double* lds, double* sh) {
+__device__ __forceinline__ void win_refine(const BWin& w, DevCtl* c, double* lds, double* sh) {
+  const BAView& v = w.v;
+  const int tid = threadIdx.x, n = w.n, ld = w.nP;
+  const double ab = c->abs_;
+  const T* W = tp<T>(w.W);
+  const T* Vinv = tp<T>(w.Vinv);
+  const T* bV = tp<T>(w.bV);
+  for (int it = 0; it < w.ir_steps; it++) {
+    double mr = 0.0, mb = 0.0;
+    for (int q = tid; q < v.P; q += kBT) ir_res_point(v, w.xU, w.xV, w.rV, w.bV64, c, q, mr, mb);
+    for (int u = tid >> 6; u < v.NU; u += kBT / 64)
+      ir_res_cam_wave(v, w.xU, w.xV, w.rU, w.bU64, c, u, tid & 63, mr, mb);
+    mr = bmax(mr, sh);
+    mb = bmax(mb, sh);
+    if (w.ir_tol > 0.0 && mr <= w.ir_tol * mb) break;
+    // rS_u = rU_u - sum_{W blocks k of u} W_k V^-1 rV_point(k)
+    for (int e = tid; e < ld; e += kBT) {
+      double val = 0.0;
+      if (e < n) {
+        const int u = e / 6, a = e - 6 * (e / 6);
+        double sacc = w.rU[e];
+        const int i1 = v.cw_ptr[u + 1];
+        for (int ii = v.cw_ptr[u]; ii < i1; ii++) {
+          const int k = v.cw_idx[ii], q = v.w_pt[k];
+          const double* r = w.rV + 3 * (size_t)q;
+          const T* m = Vinv + 6 * (size_t)q;
+          const double r0 = r[0], r1 = r[1], r2 = r[2];
+          const double g0 = (double)m[0] * r0 + (double)m[1] * r1 + (double)m[2] * r2;
+          const double g1 = (double)m[1] * r0 + (double)m[3] * r1 + (double)m[4] * r2;
+          const double g2 = (double)m[2] * r0 + (double)m[4] * r1 + (double)m[5] * r2;
+          const T* wk = W + 18 * (size_t)k + 3 * a;
+          sacc -= (double)wk[0] * g0 + (double)wk[1] * g1 + (double)wk[2] * g2;
+        }
+        val = sacc;
+      }
+      w.rS[e] = val;
+    }
+    __syncthreads();
+    // z = S^-1 rS with the FP32 factor
+    if constexpr (ALG == 64) {
+      win_solve_tiles<T>(w, lds, w.rS, w.zU);
+    } else {
+      double* yv = lds + 4096;
+      if (tid < ld) yv[tid] = w.rS[tid];
+      __syncthreads();
+      win_solve_sweeps<T>(tp<T>(w.S), n, ld, yv, yv + 256, w.zU);
+    }
+    __syncthreads();
+    // xU += zU; xV += V^-1 (rV - W^T zU)
+    for (int e = tid; e < n; e += kBT) w.xU[e] += w.zU[e];
+    for (int q = tid; q < v.NV; q += kBT) {
+      double i1, d1;
+      backsub_point<T>(v, q, W, Vinv, bV, w.zU, w.xV, ab, w.rV, 1, nullptr, i1, d1);
+    }
+    __syncthreads();
+    if (tid == 0) c->ir_count++;
+  }
+}
+
+// the step's statistics (the small-increment test of
+// LevenbergMarquardt.hpp:73-75 included) from per-thread sums of |x|^2 and
+// x^T (mu x - b) over the camera part (incu, denu) and the point part
+__device__ __forceinline__ void win_step_stats(DevCtl* c, double* sh, double incu, double denu, double inc, double den) {
+  const double a1 = bsum(incu, sh), b1 = bsum(denu, sh), c1 = bsum(inc, sh), d1 = bsum(den, sh);
+  if (threadIdx.x == 0) {
+    const int fails = __hip_atomic_load(&c->fail_v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) +
+                      __hip_atomic_load(&c->fail_s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int ok = fails == 0 ? 1 : 0;
+    c->solve_ok = ok;
+    c->incsq = a1 + c1;
+    c->den = b1 + d1;
+    c->total_solves++;
+    if (ok && c->iter >= c->minNumIt && c->incsq <= c->epsIncSquare) {
+      c->done = 1;
+      c->reason = 1;  // SuccessSmallIncrement
+    }
+  }
+}
+
+// mixed precision with refinement: refine the window's increment, then take
+// the step statistics with the FP64 gradient the refinement evaluated
+template <typename T, int ALG>
+__device__ __forceinline__ void win_refine_step(const BWin& w, DevCtl* c, double* lds, double* sh) {
+  win_refine<T, ALG>(w, c, lds, sh);
+  const int tid = threadIdx.x;
+  const double ab = c->abs_;
+  double inc = 0.0, den = 0.0;
+  for (int q = tid; q < w.v.NV; q += kBT)
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const double x = w.xV[3 * (size_t)q + k];
+      inc += x * x;
+      den += x * (ab * x - w.bV64[3 * (size_t)q + k]);
+    }
+  double incu = 0.0, denu = 0.0;
+  if (tid < w.n) {
+    const double x = w.xU[tid];
+    incu = x * x;
+    denu = x * (ab * x - w.bU64[tid]);
+  }
+  win_step_stats(c, sh, incu, denu, inc, den);
+}
+
+// IR: the mixed-precision instance with refinement, which leaves the step
+// statistics to win_refine_step
+template <typename T, int ALG, bool IR>
+__device__ __forceinline__ void win_schur(const BWin& w, DevCtl* c, double* lds, double* sh) {
   const BAView v = w.v;  // a copy: the stores below cannot alias it
   const int tid = threadIdx.x, n = w.n, ld = w.nP;
   const double rel = c->rel, ab = c->abs_;
@@ -537,7 +719,7 @@ __device__ void win_schur(const BWin& w, DevCtl* c, double* lds, double* sh) {
   __syncthreads();
   if constexpr (ALG == 64) {
     win_cholesky_tiles<T>(w, c, lds);
-    win_solve_tiles<T>(w, lds);
+    win_solve_tiles<T>(w, lds, w.xU, w.xU);
   } else {
     const bool ok = win_cholesky_panels<T, ALG>(S, n, ld, lds);
     if (!ok && tid == 0) atomicAdd(&c->fail_s, 1);
@@ -550,32 +732,20 @@ __device__ void win_schur(const BWin& w, DevCtl* c, double* lds, double* sh) {
     inc += i1;
     den += d1;
   }
+  if constexpr (IR) return;
   double incu = 0.0, denu = 0.0;
   if (tid < n) {
     const double x = w.xU[tid];
     incu = x * x;
     denu = x * (ab * x - (double)bU[tid]);
   }
-  const double a1 = bsum(incu, sh), b1 = bsum(denu, sh), c1 = bsum(inc, sh), d1 = bsum(den, sh);
-  if (tid == 0) {
-    const int fails = __hip_atomic_load(&c->fail_v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) +
-                      __hip_atomic_load(&c->fail_s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const int ok = fails == 0 ? 1 : 0;
-    c->solve_ok = ok;
-    c->incsq = a1 + c1;
-    c->den = b1 + d1;
-    c->total_solves++;
-    if (ok && c->iter >= c->minNumIt && c->incsq <= c->epsIncSquare) {
-      c->done = 1;
-      c->reason = 1;  // SuccessSmallIncrement
-    }
-  }
+  win_step_stats(c, sh, incu, denu, inc, den);
 }
 
 // the LM loop of one window (LevenbergMarquardt.hpp:35-146); the control
 // scalars live in c and are broadcast through LDS after every update
-template <typename T, int ALG>
-__device__ void win_lm(const BWin& w, DevCtl* c, double* lds, double* sh, int* shi) {
+template <typename T, int ALG, bool IR>
+__device__ __forceinline__ void win_lm(const BWin& w, DevCtl* c, double* lds, double* sh, int* shi) {
   const int tid = threadIdx.x;
   const long long maxsub = (long long)max(1, c->maxNumIt) * max(1, c->maxNumSubIt) + 4;
   for (long long it = 0; it < maxsub; it++) {
@@ -598,8 +768,12 @@ __device__ void win_lm(const BWin& w, DevCtl* c, double* lds, double* sh, int* s
       __syncthreads();
       if (shi[1]) break;
     }
-    win_schur<T, ALG>(w, c, lds, sh);
+    win_schur<T, ALG, IR>(w, c, lds, sh);
     __syncthreads();
+    if constexpr (IR) {
+      win_refine_step<T, ALG>(w, c, lds, sh);
+      __syncthreads();
+    }
     if (tid == 0) shi[3] = !c->done && c->solve_ok;
     __syncthreads();
     if (shi[3]) {
@@ -632,7 +806,7 @@ __device__ void win_lm(const BWin& w, DevCtl* c, double* lds, double* sh, int* s
 // panel variant's panel and vectors, with the reduction / control scalars in
 // its last 96 bytes, which only the tile factorisation (inside a Schur step)
 // overwrites
-template <typename T, int ALG>
+template <typename T, int ALG, bool IR>
 __global__ __launch_bounds__(256) void k_batch_lm(const BWin* __restrict__ wins, int nwin, DevCtl* ctls,
                                                   int* counter) {
   __shared__ double lds[8192];
@@ -651,10 +825,10 @@ __global__ __launch_bounds__(256) void k_batch_lm(const BWin* __restrict__ wins,
       // the tile variant reads the descriptor's fields where it uses them
       // (its factorisation needs the registers)
       const BWin& w = wins[wi];
-      win_lm<T, ALG>(w, ctls + wi, lds, sh, shi);
+      win_lm<T, ALG, IR>(w, ctls + wi, lds, sh, shi);
     } else {
       const BWin w = wins[wi];
-      win_lm<T, ALG>(w, ctls + wi, lds, sh, shi);
+      win_lm<T, ALG, IR>(w, ctls + wi, lds, sh, shi);
     }
     __syncthreads();
   }
@@ -670,20 +844,33 @@ void launch_batch_reset(int* counter, hipStream_t st) {
   hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, st, counter);
 }
 
-template <typename T>
-int launch_batch_lm(const BWin* wins, int nwin, DevCtl* ctls, int* counter, int nb, int grid, hipStream_t st) {
-  const int g = grid < nwin ? grid : nwin;
-  if (g <= 0) return 0;
+template <typename T, bool IR>
+static int launch_lm(const BWin* wins, int nwin, DevCtl* ctls, int* counter, int nb, int g, hipStream_t st) {
   switch (nb) {
-    case 8: hipLaunchKernelGGL((k_batch_lm<T, 8>), dim3(g), dim3(kBT), 0, st, wins, nwin, ctls, counter); break;
-    case 16: hipLaunchKernelGGL((k_batch_lm<T, 16>), dim3(g), dim3(kBT), 0, st, wins, nwin, ctls, counter); break;
-    case 64: hipLaunchKernelGGL((k_batch_lm<T, 64>), dim3(g), dim3(kBT), 0, st, wins, nwin, ctls, counter); break;
+    case 8: hipLaunchKernelGGL((k_batch_lm<T, 8, IR>), dim3(g), dim3(kBT), 0, st, wins, nwin, ctls, counter); break;
+    case 16: hipLaunchKernelGGL((k_batch_lm<T, 16, IR>), dim3(g), dim3(kBT), 0, st, wins, nwin, ctls, counter); break;
+    case 64: hipLaunchKernelGGL((k_batch_lm<T, 64, IR>), dim3(g), dim3(kBT), 0, st, wins, nwin, ctls, counter); break;
     default: return -1;
   }
   return 0;
 }
 
-template int launch_batch_lm<double>(const BWin*, int, DevCtl*, int*, int, int, hipStream_t);
-template int launch_batch_lm<float>(const BWin*, int, DevCtl*, int*, int, int, hipStream_t);
+template <>
+int launch_batch_lm<double>(const BWin* wins, int nwin, DevCtl* ctls, int* counter, int nb, int grid, bool refine,
+                            hipStream_t st) {
+  const int g = grid < nwin ? grid : nwin;
+  if (g <= 0) return 0;
+  if (refine) return -2;  // FP64 windows are not refined
+  return launch_lm<double, false>(wins, nwin, ctls, counter, nb, g, st);
+}
+
+template <>
+int launch_batch_lm<float>(const BWin* wins, int nwin, DevCtl* ctls, int* counter, int nb, int grid, bool refine,
+                           hipStream_t st) {
+  const int g = grid < nwin ? grid : nwin;
+  if (g <= 0) return 0;
+  return refine ? launch_lm<float, true>(wins, nwin, ctls, counter, nb, g, st)
+                : launch_lm<float, false>(wins, nwin, ctls, counter, nb, g, st);
+}
 
 }  // namespace nsn

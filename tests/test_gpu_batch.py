@@ -1,7 +1,8 @@
 """Batched independent windows (nsn_batch_*): one persistent launch runs the
 LM solve of every window.  FP64 batch results match the single-problem GPU
 path and the oracle (final cost 1e-9, same termination); the mixed-precision
-batch reaches the FP64 final cost to 1e-6; the autotuner's pick and the state
+batch reaches the FP64 final cost to 1e-6, and with two refinement steps
+follows the FP64 trajectory (1e-9); the autotuner's pick and the state
 round trip behave; solves replay one captured graph per launch shape.
 """
 import numpy as np
@@ -68,6 +69,55 @@ def test_batch_mixed_reaches_fp64_cost():
     print(f"\nmixed vs FP64 batch (8 windows of the local-BA workload): max final-cost rel err {err.max():.2e}, "
           f"config {bm.config()}")
     assert err.max() <= 1e-6
+
+
+@pytest.mark.parametrize("variant", [8, 16, 64])
+def test_batch_mixed_refinement(variant):
+    """Mixed-precision windows with iterative refinement: two fixed steps per
+    increment (tol 0) make a 6-iteration LM trajectory equal the FP64
+    batch's -- final costs to 1e-9, states to 1e-8, the same iteration and
+    solve counts -- where the unrefined FP32 increments differ visibly; every
+    solve runs exactly two refinement solves, none by default (refinement
+    off), and the adaptive form (2 steps, tol 1e-10) runs at most two."""
+    rng = np.random.default_rng(21)
+    wins = []
+    for w, nc in enumerate([5, 9, 14, 22, 31, 42]):
+        npt = int(rng.integers(150, 600))
+        wins.append(generate_ba(dict(CONFIGS["cfg1"], n_cameras=nc, n_points=npt, k_min=3, k_max=6, seed=1300 + w)))
+    s = LevenbergMarquardtSettings(maxNumIt=6)
+    b64 = BABatch(wins, precision="fp64")
+    b64.set_config(variant, 6)
+    c64, _ = b64.lm_solve(s)
+    _, it64, sub64 = b64.results()
+    cams64, pts64 = b64.get_state()
+    errs = {}
+    for mode in ("off", "fixed", "adaptive"):
+        bm = BABatch(wins, precision="mixed")
+        bm.set_config(variant, 6)
+        if mode == "fixed":
+            bm.set_refinement(2, 0.0)
+        elif mode == "adaptive":
+            bm.set_refinement(2, 1e-10)
+        cm, ms = bm.lm_solve(s)
+        _, itm, subm = bm.results()
+        nref = bm.refinement_counts()
+        cams, pts = bm.get_state()
+        errs[mode] = float(np.max(np.abs(cm - c64) / c64))
+        print(f"\nvariant {variant}, refinement {mode}: final cost vs FP64 {errs[mode]:.2e}, "
+              f"state {max(np.abs(cams - cams64).max(), np.abs(pts - pts64).max()):.2e}, "
+              f"refinement solves {list(nref)} for {list(subm)} solves, {ms:.2f} ms")
+        if mode == "off":
+            assert np.all(nref == 0)
+            continue
+        if mode == "fixed":
+            assert np.array_equal(nref, 2 * subm), (nref, subm)
+            assert errs[mode] <= 1e-9, errs[mode]
+            assert np.array_equal(itm, it64) and np.array_equal(subm, sub64)
+            scale = max(np.abs(cams64).max(), np.abs(pts64).max())
+            assert np.abs(cams - cams64).max() <= 1e-8 * scale and np.abs(pts - pts64).max() <= 1e-8 * scale
+        else:
+            assert np.all(nref <= 2 * subm) and errs[mode] <= 1e-8, (nref, subm, errs[mode])
+    assert errs["fixed"] <= errs["off"]
 
 
 def test_batch_tune_and_state_roundtrip():
