@@ -495,15 +495,23 @@ __device__ __forceinline__ void lin_point(const BAView& v, int p, T* __restrict_
   }
 }
 
-// damped V_q^-1 through its Cholesky factor L in FP64 (MatrixDiagInv.hpp:25-140),
-// Y = W V^-1 (or, with ZF, Y = W L^-T, so that W V^-1 W'^T = Y Y'^T) and
-// t = W V^-1 (-bV) for the W blocks of point q; false if the damped block is
-// not positive definite (V^-1 is then zero)
-template <typename T, bool ZF = false>
-__device__ __forceinline__ bool vinv_y_point(const BAView& v, int q, const T* __restrict__ V,
-                                             const T* __restrict__ bV, const T* __restrict__ W,
-                                             T* __restrict__ Vinv, T* __restrict__ Y, T* __restrict__ tW, double rel,
-                                             double ab) {
+// The damped 3x3 block of point q (V + rel diag V + abs I) through its
+// Cholesky factor L in FP64 (MatrixDiagInv.hpp:25-140): V^-1 (packed upper,
+// zero when the block is not positive definite), x = V^-1 (-bV), and the
+// right factor M of the W products: M = V^-1, or with ZF M = L^-T (zero when
+// the block failed), so that Y = W M gives W V^-1 W'^T = Y Y'^T.
+template <typename T, bool ZF>
+struct VFactor {
+  double vi[6];
+  double x0, x1, x2;
+  T M[9];
+  bool ok;
+};
+
+template <typename T, bool ZF>
+__device__ __forceinline__ VFactor<T, ZF> v_factor(const T* __restrict__ V, const T* __restrict__ bV, int q,
+                                                   double rel, double ab) {
+  VFactor<T, ZF> f;
   double a00 = V[6 * (size_t)q + 0], a01 = V[6 * (size_t)q + 1], a02 = V[6 * (size_t)q + 2];
   double a11 = V[6 * (size_t)q + 3], a12 = V[6 * (size_t)q + 4], a22 = V[6 * (size_t)q + 5];
   if (rel != 0.0 || ab != 0.0) {
@@ -511,7 +519,7 @@ __device__ __forceinline__ bool vinv_y_point(const BAView& v, int q, const T* __
     a11 = a11 * (1.0 + rel) + ab;
     a22 = a22 * (1.0 + rel) + ab;
   }
-  double vi[6];
+  double* vi = f.vi;
   bool ok = a00 > 0.0;
   const double l00 = sqrt(a00);
   const double l10 = a01 / l00, l20 = a02 / l00;
@@ -535,15 +543,13 @@ __device__ __forceinline__ bool vinv_y_point(const BAView& v, int q, const T* __
   ok = ok && isfinite(vi[0]) && isfinite(vi[3]) && isfinite(vi[5]);
   if (!ok)
     for (int i = 0; i < 6; i++) vi[i] = 0.0;
-#pragma unroll
-  for (int i = 0; i < 6; i++) Vinv[6 * (size_t)q + i] = (T)vi[i];
+  f.ok = ok;
   const double b0 = bV[3 * (size_t)q], b1 = bV[3 * (size_t)q + 1], b2 = bV[3 * (size_t)q + 2];
-  // x0 = V^-1 (-bV)
-  const double x0 = -(vi[0] * b0 + vi[1] * b1 + vi[2] * b2);
-  const double x1 = -(vi[1] * b0 + vi[3] * b1 + vi[4] * b2);
-  const double x2 = -(vi[2] * b0 + vi[4] * b1 + vi[5] * b2);
-  // y_i = w_i M: M = V^-1, or M = L^-T (zero when V failed)
-  T M[9];
+  // x = V^-1 (-bV)
+  f.x0 = -(vi[0] * b0 + vi[1] * b1 + vi[2] * b2);
+  f.x1 = -(vi[1] * b0 + vi[3] * b1 + vi[4] * b2);
+  f.x2 = -(vi[2] * b0 + vi[4] * b1 + vi[5] * b2);
+  T* M = f.M;
   if constexpr (ZF) {
     const double z = ok ? 1.0 : 0.0;
     M[0] = (T)(z * m00); M[1] = (T)(z * m10); M[2] = (T)(z * m20);
@@ -554,24 +560,43 @@ __device__ __forceinline__ bool vinv_y_point(const BAView& v, int q, const T* __
     M[3] = (T)vi[1]; M[4] = (T)vi[3]; M[5] = (T)vi[4];
     M[6] = (T)vi[2]; M[7] = (T)vi[4]; M[8] = (T)vi[5];
   }
-  const int k1 = v.wptr[q + 1];
-  for (int k = v.wptr[q]; k < k1; k++) {
-    const T* w = W + 18 * (size_t)k;
-    T y[18];
-    T t[6];
+  return f;
+}
+
+// Y_k = W_k M (6x3) and t_k = W_k x (6) for the W block k
+template <typename T, bool ZF>
+__device__ __forceinline__ void y_block(const VFactor<T, ZF>& f, const T* __restrict__ W, T* __restrict__ Y,
+                                        T* __restrict__ tW, int k) {
+  const T* w = W + 18 * (size_t)k;
+  const T* M = f.M;
+  T y[18];
+  T t[6];
 #pragma unroll
-    for (int i = 0; i < 6; i++) {
-      const T w0 = w[3 * i], w1 = w[3 * i + 1], w2 = w[3 * i + 2];
+  for (int i = 0; i < 6; i++) {
+    const T w0 = w[3 * i], w1 = w[3 * i + 1], w2 = w[3 * i + 2];
 #pragma unroll
-      for (int j = 0; j < 3; j++) y[3 * i + j] = w0 * M[j] + w1 * M[3 + j] + w2 * M[6 + j];
-      t[i] = (T)((double)w0 * x0 + (double)w1 * x1 + (double)w2 * x2);
-    }
-#pragma unroll
-    for (int i = 0; i < 18; i++) Y[18 * (size_t)k + i] = y[i];
-#pragma unroll
-    for (int i = 0; i < 6; i++) tW[6 * (size_t)k + i] = t[i];
+    for (int j = 0; j < 3; j++) y[3 * i + j] = w0 * M[j] + w1 * M[3 + j] + w2 * M[6 + j];
+    t[i] = (T)((double)w0 * f.x0 + (double)w1 * f.x1 + (double)w2 * f.x2);
   }
-  return ok;
+#pragma unroll
+  for (int i = 0; i < 18; i++) Y[18 * (size_t)k + i] = y[i];
+#pragma unroll
+  for (int i = 0; i < 6; i++) tW[6 * (size_t)k + i] = t[i];
+}
+
+// V_q^-1 into Vinv and Y, t for the W blocks of point q; false if the damped
+// block is not positive definite (V^-1 is then zero)
+template <typename T, bool ZF = false>
+__device__ __forceinline__ bool vinv_y_point(const BAView& v, int q, const T* __restrict__ V,
+                                             const T* __restrict__ bV, const T* __restrict__ W,
+                                             T* __restrict__ Vinv, T* __restrict__ Y, T* __restrict__ tW, double rel,
+                                             double ab) {
+  const VFactor<T, ZF> f = v_factor<T, ZF>(V, bV, q, rel, ab);
+#pragma unroll
+  for (int i = 0; i < 6; i++) Vinv[6 * (size_t)q + i] = (T)f.vi[i];
+  const int k1 = v.wptr[q + 1];
+  for (int k = v.wptr[q]; k < k1; k++) y_block<T, ZF>(f, W, Y, tW, k);
+  return f.ok;
 }
 
 // back-substitution of point q: xV_q = V_q^-1 (rhs_q - sum_k W_k^T xU_c(k)),

@@ -164,14 +164,32 @@ __global__ __launch_bounds__(256) void k_finalize_lin(const double* __restrict__
 // W_a V^-1 W_b^T = Z_a Z_b^T then read one array; and t = W V^-1 (-bV) (the
 // W * V^-1 bV term of bS, SolverCholeskySchur.hpp:114).
 // ---------------------------------------------------------------------------
+// One thread per W block k (threads 0 .. nW-1), then one per variable point
+// (nW .. nW+NV-1): a block's thread forms its point's factor itself (a few
+// dozen operations, the point's V and bV shared with its neighbours in the
+// wave) and writes Z_k, t_k, so a wave writes 64 consecutive blocks at once;
+// a point's thread writes V_q^-1 and counts a failed block.  (One thread per
+// point writing all of its blocks left each line partly written for the
+// length of its loop: the partial lines were filled from memory, about
+// 200 MB of extra reads at the 1000-camera problem.)
 template <typename T>
 __global__ __launch_bounds__(256) void k_vinv_y(BAView v, const T* __restrict__ V, const T* __restrict__ bV,
                                                 const T* __restrict__ W, T* __restrict__ Vinv,
                                                 T* __restrict__ Z, T* __restrict__ tW, DevCtl* ctl, int gate) {
   if (!gate_open(ctl, gate)) return;
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= v.NV) return;
-  if (!vinv_y_point<T, true>(v, q, V, bV, W, Vinv, Z, tW, ctl->rel, ctl->abs_)) atomicAdd(&ctl->fail_v, 1);
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const double rel = ctl->rel, ab = ctl->abs_;
+  if (g < v.nW) {
+    const int k = (int)g;
+    const VFactor<T, true> f = v_factor<T, true>(V, bV, v.w_pt[k], rel, ab);
+    y_block<T, true>(f, W, Z, tW, k);
+  } else if (g < (long long)v.nW + v.NV) {
+    const int q = (int)(g - v.nW);
+    const VFactor<T, true> f = v_factor<T, true>(V, bV, q, rel, ab);
+#pragma unroll
+    for (int i = 0; i < 6; i++) Vinv[6 * (size_t)q + i] = (T)f.vi[i];
+    if (!f.ok) atomicAdd(&ctl->fail_v, 1);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -730,8 +748,8 @@ template <typename T>
 void launch_vinv_y(const BAView& v, const T* V, const T* bV, const T* W, T* Vinv, T* Z, T* tW, DevCtl* ctl,
                    int gate, hipStream_t st) {
   if (v.NV == 0) return;
-  hipLaunchKernelGGL((k_vinv_y<T>), dim3(nblocks(v.NV, 256)), dim3(256), 0, st, v, V, bV, W, Vinv, Z, tW, ctl,
-                     gate);
+  hipLaunchKernelGGL((k_vinv_y<T>), dim3(nblocks(v.nW + v.NV, 256)), dim3(256), 0, st, v, V, bV, W, Vinv, Z, tW,
+                     ctl, gate);
 }
 
 template <typename T>
