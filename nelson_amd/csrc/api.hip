@@ -106,9 +106,10 @@ struct nsn_problem {
   Scratch scr{};
   double *cams = nullptr, *pts = nullptr, *cams_bak = nullptr, *pts_bak = nullptr;
   void *U = nullptr, *bU = nullptr, *V = nullptr, *bV = nullptr, *W = nullptr, *Vinv = nullptr, *Y = nullptr,
-       *tW = nullptr, *Sd = nullptr, *bS = nullptr, *tiles = nullptr, *inv = nullptr, *presub = nullptr,
+       *Sd = nullptr, *bS = nullptr, *tiles = nullptr, *inv = nullptr, *presub = nullptr,
        *part = nullptr;
   double *xU = nullptr, *xV = nullptr;
+  double* yV = nullptr;  // y_q = L_q^-1 (-bV_q) per variable point (k_vinv_y)
   int* chol_tasks = nullptr;  // worker task queue of the factorisation (chol_queue)
   int n_chol_tasks = 0;
   int n_part_tiles = 0;  // its chunk scratch tiles
@@ -226,11 +227,11 @@ void seq_schur(nsn_problem* h, int gate, int checks, bool timed) {
   hipStream_t st = h->st;
   launch_begin_step(h->ctl, gate, st);
   if (timed) (void)hipEventRecord(h->ev[0], st);
-  launch_vinv_y<T>(h->view, (const T*)h->V, (const T*)h->bV, (const T*)h->W, (T*)h->Vinv, (T*)h->Y, (T*)h->tW,
-                   h->ctl, gate, st);
+  launch_vinv_y<T>(h->view, (const T*)h->V, (const T*)h->bV, (const T*)h->W, (T*)h->Vinv, (T*)h->Y, h->yV, h->ctl,
+                   gate, st);
   if (timed) (void)hipEventRecord(h->ev[1], st);
-  launch_s_diag<T>(h->view, (const T*)h->U, (const T*)h->bU, (const T*)h->Y, (const T*)h->tW, (T*)h->Sd,
-                   (T*)h->bS, h->ctl, gate, st);
+  launch_s_diag<T>(h->view, (const T*)h->U, (const T*)h->bU, (const T*)h->Y, h->yV, (T*)h->Sd, (T*)h->bS, h->ctl,
+                   gate, st);
   launch_s_tiles<T>(h->tv, h->n_s_parts, h->tile_order, h->part_tile, h->part_rect, h->part_ptr, h->part_blk, h->sb_i,
                     h->sb_j, h->sb_ptr, h->sb_pairs, (const T*)h->Sd, (const T*)h->bS, (const T*)h->Y,
                     (T*)h->tiles, h->ctl, gate, st);
@@ -406,6 +407,8 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
   &cam_ptr = hs.cam_ptr, &ce_pt = hs.ce_pt, &cw_ptr = hs.cw_ptr, &cw_idx = hs.cw_idx, &sb_i = hs.sb_i,
   &sb_j = hs.sb_j, &sb_ptr = hs.sb_ptr, &sb_pairs = hs.sb_pairs;
   const std::vector<double>&pe_obs = hs.pe_obs, &pe_w = hs.pe_w, &ce_obs = hs.ce_obs, &ce_w = hs.ce_w;
+  std::vector<int> cw_pt(nW);  // the point of each camera-major block entry
+  for (int i = 0; i < nW; i++) cw_pt[i] = hs.w_pt[cw_idx[i]];
   // tiles of [S; bS^T] over the envelope of S (tile_envelope)
   const int n = 6 * NU;
   TileView tv{};
@@ -524,6 +527,7 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
   int* d_w_pt = a_int(nW);
   int* d_cw_ptr = a_int(NU + 1);
   int* d_cw_idx = a_int(nW);
+  int* d_cw_pt = a_int(nW);
   h->cams = a_dbl(12 * (size_t)C);
   h->pts = a_dbl(3 * (size_t)P);
   h->cams_bak = a_dbl(12 * (size_t)C);
@@ -535,7 +539,7 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
   h->W = a_T(18 * (size_t)nW);
   h->Vinv = a_T(6 * (size_t)NV);
   h->Y = a_T(18 * (size_t)nW);
-  h->tW = a_T(6 * (size_t)nW);
+  h->yV = a_dbl(3 * (size_t)NV);
   h->Sd = a_T(21 * (size_t)NU);
   h->bS = a_T(6 * (size_t)NU);
   h->tiles = a_T((size_t)tv.ntiles * kTB * kTB);
@@ -587,7 +591,7 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
       (rc = upload(d_pe_w, pe_w)) || (rc = upload(d_pe_wblk, pe_wblk)) || (rc = upload(d_pe_edge, pe)) || (rc = upload(d_cam_ptr, cam_ptr)) ||
       (rc = upload(d_ce_pt, ce_pt)) || (rc = upload(d_ce_obs, ce_obs)) || (rc = upload(d_ce_w, ce_w)) ||
       (rc = upload(d_wptr, wptr)) || (rc = upload(d_w_cam, h->w_cam)) || (rc = upload(d_w_pt, h->w_pt)) || (rc = upload(d_cw_ptr, cw_ptr)) ||
-      (rc = upload(d_cw_idx, cw_idx)) || (rc = upload(h->tile_order, tile_order)) || (rc = upload(h->chol_tasks, chol_tasks)) ||
+      (rc = upload(d_cw_idx, cw_idx)) || (rc = upload(d_cw_pt, cw_pt)) || (rc = upload(h->tile_order, tile_order)) || (rc = upload(h->chol_tasks, chol_tasks)) ||
       (rc = upload(h->part_tile, part_tile)) || (rc = upload(h->part_rect, part_rect)) ||
       (rc = upload(h->part_ptr, part_ptr)) || (rc = upload(h->part_blk, part_blk)) ||
       (rc = upload(h->sb_i, sb_i)) || (rc = upload(h->sb_j, sb_j)) || (rc = upload(h->sb_ptr, sb_ptr)) ||
@@ -608,7 +612,7 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
   v.cam_var = d_cam_var; v.pt_var = d_pt_var; v.var_cam_user = d_vcu; v.var_pt_user = d_vpu;
   v.pt_ptr = d_pt_ptr; v.pe_cam = d_pe_cam; v.pe_obs = d_pe_obs; v.pe_w = d_pe_w; v.pe_wblk = d_pe_wblk; v.pe_edge = d_pe_edge;
   v.cam_ptr = d_cam_ptr; v.ce_pt = d_ce_pt; v.ce_obs = d_ce_obs; v.ce_w = d_ce_w;
-  v.wptr = d_wptr; v.w_cam = d_w_cam; v.w_pt = d_w_pt; v.cw_ptr = d_cw_ptr; v.cw_idx = d_cw_idx;
+  v.wptr = d_wptr; v.w_cam = d_w_cam; v.w_pt = d_w_pt; v.cw_ptr = d_cw_ptr; v.cw_idx = d_cw_idx; v.cw_pt = d_cw_pt;
   v.cams = h->cams; v.pts = h->pts;
   v.intr = nullptr;
   if (d->intrinsics) {

@@ -499,11 +499,14 @@ __device__ __forceinline__ void lin_point(const BAView& v, int p, T* __restrict_
 // Cholesky factor L in FP64 (MatrixDiagInv.hpp:25-140): V^-1 (packed upper,
 // zero when the block is not positive definite), x = V^-1 (-bV), and the
 // right factor M of the W products: M = V^-1, or with ZF M = L^-T (zero when
-// the block failed), so that Y = W M gives W V^-1 W'^T = Y Y'^T.
+// the block failed), so that Y = W M gives W V^-1 W'^T = Y Y'^T; with ZF
+// also y = L^-1 (-bV) (zero when the block failed), so that t = W V^-1 (-bV)
+// = Z y.
 template <typename T, bool ZF>
 struct VFactor {
   double vi[6];
   double x0, x1, x2;
+  double y0, y1, y2;
   T M[9];
   bool ok;
 };
@@ -550,8 +553,14 @@ __device__ __forceinline__ VFactor<T, ZF> v_factor(const T* __restrict__ V, cons
   f.x1 = -(vi[1] * b0 + vi[3] * b1 + vi[4] * b2);
   f.x2 = -(vi[2] * b0 + vi[4] * b1 + vi[5] * b2);
   T* M = f.M;
+  f.y0 = f.y1 = f.y2 = 0.0;
   if constexpr (ZF) {
     const double z = ok ? 1.0 : 0.0;
+    if (ok) {
+      f.y0 = -(m00 * b0);
+      f.y1 = -(m10 * b0 + m11 * b1);
+      f.y2 = -(m20 * b0 + m21 * b1 + m22 * b2);
+    }
     M[0] = (T)(z * m00); M[1] = (T)(z * m10); M[2] = (T)(z * m20);
     M[3] = (T)0;         M[4] = (T)(z * m11); M[5] = (T)(z * m21);
     M[6] = (T)0;         M[7] = (T)0;         M[8] = (T)(z * m22);
@@ -561,6 +570,22 @@ __device__ __forceinline__ VFactor<T, ZF> v_factor(const T* __restrict__ V, cons
     M[6] = (T)vi[2]; M[7] = (T)vi[4]; M[8] = (T)vi[5];
   }
   return f;
+}
+
+// Y_k = W_k M (6x3) for the W block k
+template <typename T, bool ZF>
+__device__ __forceinline__ void z_block(const VFactor<T, ZF>& f, const T* __restrict__ W, T* __restrict__ Y, int k) {
+  const T* w = W + 18 * (size_t)k;
+  const T* M = f.M;
+  T y[18];
+#pragma unroll
+  for (int i = 0; i < 6; i++) {
+    const T w0 = w[3 * i], w1 = w[3 * i + 1], w2 = w[3 * i + 2];
+#pragma unroll
+    for (int j = 0; j < 3; j++) y[3 * i + j] = w0 * M[j] + w1 * M[3 + j] + w2 * M[6 + j];
+  }
+#pragma unroll
+  for (int i = 0; i < 18; i++) Y[18 * (size_t)k + i] = y[i];
 }
 
 // Y_k = W_k M (6x3) and t_k = W_k x (6) for the W block k

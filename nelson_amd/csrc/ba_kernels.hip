@@ -167,27 +167,32 @@ __global__ __launch_bounds__(256) void k_finalize_lin(const double* __restrict__
 // One thread per W block k (threads 0 .. nW-1), then one per variable point
 // (nW .. nW+NV-1): a block's thread forms its point's factor itself (a few
 // dozen operations, the point's V and bV shared with its neighbours in the
-// wave) and writes Z_k, t_k, so a wave writes 64 consecutive blocks at once;
-// a point's thread writes V_q^-1 and counts a failed block.  (One thread per
+// wave) and writes Z_k, so a wave writes 64 consecutive blocks at once; a
+// point's thread writes V_q^-1 and y_q = L_q^-1 (-bV_q) and counts a failed
+// block.  The bS terms t_k = W_k V^-1 (-bV) = Z_k y_q are formed where they
+// are summed (k_s_diag): 24 bytes per point instead of 48 per block.  (One thread per
 // point writing all of its blocks left each line partly written for the
 // length of its loop: the partial lines were filled from memory, about
 // 200 MB of extra reads at the 1000-camera problem.)
 template <typename T>
 __global__ __launch_bounds__(256) void k_vinv_y(BAView v, const T* __restrict__ V, const T* __restrict__ bV,
                                                 const T* __restrict__ W, T* __restrict__ Vinv,
-                                                T* __restrict__ Z, T* __restrict__ tW, DevCtl* ctl, int gate) {
+                                                T* __restrict__ Z, double* __restrict__ yV, DevCtl* ctl, int gate) {
   if (!gate_open(ctl, gate)) return;
   const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const double rel = ctl->rel, ab = ctl->abs_;
   if (g < v.nW) {
     const int k = (int)g;
     const VFactor<T, true> f = v_factor<T, true>(V, bV, v.w_pt[k], rel, ab);
-    y_block<T, true>(f, W, Z, tW, k);
+    z_block<T, true>(f, W, Z, k);
   } else if (g < (long long)v.nW + v.NV) {
     const int q = (int)(g - v.nW);
     const VFactor<T, true> f = v_factor<T, true>(V, bV, q, rel, ab);
 #pragma unroll
     for (int i = 0; i < 6; i++) Vinv[6 * (size_t)q + i] = (T)f.vi[i];
+    yV[3 * (size_t)q] = f.y0;
+    yV[3 * (size_t)q + 1] = f.y1;
+    yV[3 * (size_t)q + 2] = f.y2;
     if (!f.ok) atomicAdd(&ctl->fail_v, 1);
   }
 }
@@ -201,7 +206,7 @@ __global__ __launch_bounds__(256) void k_vinv_y(BAView v, const T* __restrict__ 
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void k_s_diag(BAView v, const T* __restrict__ U, const T* __restrict__ bU,
-                                                const T* __restrict__ Z, const T* __restrict__ tW,
+                                                const T* __restrict__ Z, const double* __restrict__ yV,
                                                 T* __restrict__ Sd,
                                                 T* __restrict__ bS, DevCtl* ctl, int gate) {
   if (!gate_open(ctl, gate)) return;
@@ -212,17 +217,19 @@ __global__ __launch_bounds__(256) void k_s_diag(BAView v, const T* __restrict__ 
   for (int i = 0; i < 27; i++) acc[i] = (T)0;
   const int i1 = v.cw_ptr[u + 1];
   for (int ii = v.cw_ptr[u] + threadIdx.x; ii < i1; ii += 256) {
-    const int k = v.cw_idx[ii];
+    const int k = v.cw_idx[ii], q = v.cw_pt[ii];
     T z[18];
 #pragma unroll
     for (int i = 0; i < 18; i++) z[i] = Z[18 * (size_t)k + i];
+    const double y0 = yV[3 * (size_t)q], y1 = yV[3 * (size_t)q + 1], y2 = yV[3 * (size_t)q + 2];
 #pragma unroll
     for (int a = 0; a < 6; a++)
 #pragma unroll
       for (int b = a; b < 6; b++)
         acc[up6(a, b)] += z[3 * a] * z[3 * b] + z[3 * a + 1] * z[3 * b + 1] + z[3 * a + 2] * z[3 * b + 2];
 #pragma unroll
-    for (int a = 0; a < 6; a++) acc[21 + a] += tW[6 * (size_t)k + a];
+    for (int a = 0; a < 6; a++)  // t_k = Z_k y_q
+      acc[21 + a] += (T)((double)z[3 * a] * y0 + (double)z[3 * a + 1] * y1 + (double)z[3 * a + 2] * y2);
   }
 #pragma unroll
   for (int i = 0; i < 27; i++) {
@@ -745,18 +752,18 @@ void launch_finalize_lin(const Scratch& s, DevCtl* ctl, int gate, int full, hipS
 }
 
 template <typename T>
-void launch_vinv_y(const BAView& v, const T* V, const T* bV, const T* W, T* Vinv, T* Z, T* tW, DevCtl* ctl,
+void launch_vinv_y(const BAView& v, const T* V, const T* bV, const T* W, T* Vinv, T* Z, double* yV, DevCtl* ctl,
                    int gate, hipStream_t st) {
   if (v.NV == 0) return;
-  hipLaunchKernelGGL((k_vinv_y<T>), dim3(nblocks(v.nW + v.NV, 256)), dim3(256), 0, st, v, V, bV, W, Vinv, Z, tW,
+  hipLaunchKernelGGL((k_vinv_y<T>), dim3(nblocks(v.nW + v.NV, 256)), dim3(256), 0, st, v, V, bV, W, Vinv, Z, yV,
                      ctl, gate);
 }
 
 template <typename T>
-void launch_s_diag(const BAView& v, const T* U, const T* bU, const T* Z, const T* tW, T* Sd, T* bS, DevCtl* ctl,
+void launch_s_diag(const BAView& v, const T* U, const T* bU, const T* Z, const double* yV, T* Sd, T* bS, DevCtl* ctl,
                    int gate, hipStream_t st) {
   if (v.NU == 0) return;
-  hipLaunchKernelGGL((k_s_diag<T>), dim3(v.NU), dim3(256), 0, st, v, U, bU, Z, tW, Sd, bS, ctl, gate);
+  hipLaunchKernelGGL((k_s_diag<T>), dim3(v.NU), dim3(256), 0, st, v, U, bU, Z, yV, Sd, bS, ctl, gate);
 }
 
 template <typename T>
@@ -922,9 +929,9 @@ void launch_zero(T* p, long long n, hipStream_t st) {
   template void launch_lin_points<T>(const BAView&, bool, T*, T*, T*, const Scratch&, DevCtl*, int,          \
                                      hipStream_t);                                                           \
   template void launch_lin_cams<T>(const BAView&, T*, T*, const Scratch&, DevCtl*, int, hipStream_t);       \
-  template void launch_vinv_y<T>(const BAView&, const T*, const T*, const T*, T*, T*, T*, DevCtl*, int,      \
+  template void launch_vinv_y<T>(const BAView&, const T*, const T*, const T*, T*, T*, double*, DevCtl*, int, \
                                  hipStream_t);                                                               \
-  template void launch_s_diag<T>(const BAView&, const T*, const T*, const T*, const T*, T*, T*, DevCtl*,    \
+  template void launch_s_diag<T>(const BAView&, const T*, const T*, const T*, const double*, T*, T*, DevCtl*, \
                                  int, hipStream_t);                                                          \
   template void launch_s_tiles<T>(const TileView&, int, const int*, const int*, const int*, const int*,      \
                                   const int*, const int*, const int*, const int*, const int*, const T*,      \

@@ -50,6 +50,7 @@ struct BAView {
   const int* w_pt;          // [nW] variable point of the block
   const int* cw_ptr;        // [NU+1] camera-major list of W blocks
   const int* cw_idx;        // [nW]
+  const int* cw_pt;         // [nW] variable point of the block cw_idx[i]
   double* cams;             // [12C] R (row-major) and t per camera
   double* pts;              // [3P]
   const double* intr;       // [3C] (f, k1, k2) per camera of the BAL model, or null: shared pinhole
@@ -124,11 +125,11 @@ void launch_finalize_lin(const Scratch& s, DevCtl* ctl, int gate, int full, hipS
 // Vinv = damped V^-1, Z = W L^-T (V = L L^T, so W_a V^-1 W_b^T = Z_a Z_b^T)
 // and t = W V^-1 (-bV)
 template <typename T>
-void launch_vinv_y(const BAView& v, const T* V, const T* bV, const T* W, T* Vinv, T* Z, T* tW,
-                   DevCtl* ctl, int gate, hipStream_t st);
+void launch_vinv_y(const BAView& v, const T* V, const T* bV, const T* W, T* Vinv, T* Z, double* yV, DevCtl* ctl,
+                   int gate, hipStream_t st);
 template <typename T>
-void launch_s_diag(const BAView& v, const T* U, const T* bU, const T* Z, const T* tW,
-                   T* Sd, T* bS, DevCtl* ctl, int gate, hipStream_t st);
+void launch_s_diag(const BAView& v, const T* U, const T* bU, const T* Z, const double* yV, T* Sd, T* bS,
+                   DevCtl* ctl, int gate, hipStream_t st);
 // one workgroup per part of a tile (part_tile, part_rect = local rows lo, hi,
 // cols lo, hi, part_ptr / part_blk = the camera blocks inside)
 template <typename T>
