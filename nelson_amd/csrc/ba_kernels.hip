@@ -32,7 +32,7 @@ __global__ __launch_bounds__(256) void k_lin_points(BAView v, T* __restrict__ V,
                                                     unsigned long long* __restrict__ maxes, DevCtl* ctl,
                                                     int gate) {
   if (!gate_open(ctl, gate)) return;
-  __shared__ double sh[4];
+  __shared__ double sh[8];
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   double chi2 = 0.0, mb = 0.0, md = 0.0;
   if (p < v.P) lin_point<T, HESS>(v, p, V, bV, W, chi2, mb, md);
@@ -45,14 +45,7 @@ __global__ __launch_bounds__(256) void k_lin_points(BAView v, T* __restrict__ V,
   }
   double s = block_sum256(chi2, sh);
   if (threadIdx.x == 0) part_chi2[blockIdx.x] = s;
-  if (HESS) {
-    mb = wave_max(mb);
-    md = wave_max(md);
-    if ((threadIdx.x & 63) == 0) {
-      atomic_max_nonneg(maxes + 1, mb);
-      atomic_max_nonneg(maxes + 3, md);
-    }
-  }
+  if (HESS) block_max2_atomic(mb, md, sh, maxes + 1, maxes + 3);
 }
 
 // ---------------------------------------------------------------------------
@@ -385,6 +378,7 @@ __global__ __launch_bounds__(256) void k_ir_res(BAView v, const double* __restri
   if (!gate_open(ctl, gate)) return;
   __shared__ double red[4][18];
   __shared__ double pri[18];
+  __shared__ double shm[8];
   double mr = 0.0, mb = 0.0;  // max |r|, |b| of this thread's entries
   if ((int)blockIdx.x < nbp) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -393,13 +387,8 @@ __global__ __launch_bounds__(256) void k_ir_res(BAView v, const double* __restri
     ir_res_cam(v, xU, xV, rU, bU64, ctl, (int)blockIdx.x - nbp, red, pri, mr, mb);
   }
   // the residual's and the gradient's largest entries (the refinement's
-  // stopping test), one atomic per wave
-  mr = wave_max(mr);
-  mb = wave_max(mb);
-  if ((threadIdx.x & 63) == 0) {
-    atomic_max_nonneg(&ctl->ir_maxr, mr);
-    atomic_max_nonneg(&ctl->ir_maxb, mb);
-  }
+  // stopping test), one atomic each per workgroup
+  block_max2_atomic(mr, mb, shm, &ctl->ir_maxr, &ctl->ir_maxb);
 }
 
 __global__ void k_ir_decide(DevCtl* c, int gate, double tol) {
