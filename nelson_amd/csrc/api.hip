@@ -569,13 +569,13 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
   h->sb_pairs = a_int(sb_pairs.size());
   h->flags = a_int(h->nflags);
   h->counters = a_int(8);
-  const int nblk_pts = std::max(1, (std::max(P, NV) + 255) / 256);
+  const int nblk_pts = std::max(1, (int)((std::max((long long)P * kLinLanes, (long long)NV) + 255) / 256));
   const int nblk_cams = std::max(1, (6 * NU + 255) / 256);
   h->scr.part_chi2 = a_dbl(nblk_pts);
   h->scr.part_xv = a_dbl(2 * (size_t)nblk_pts);
   h->scr.part_xu = a_dbl(2 * (size_t)nblk_cams);
   h->scr.maxes = m.alloc<unsigned long long>(4);
-  h->scr.nblk_pts = (P + 255) / 256;
+  h->scr.nblk_pts = (int)(((long long)P * kLinLanes + 255) / 256);
   h->scr.nblk_cams = (6 * NU + 255) / 256;
   h->scr.nblk_xv = (NV + 255) / 256;
   h->ctl = m.alloc<DevCtl>(1);
@@ -607,7 +607,7 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
     h->tv.chain_start = h->tprof + 2 * (size_t)tv.NT;
   }
   BAView& v = h->view;
-  v.C = C; v.P = P; v.E = E; v.NU = NU; v.NV = NV; v.nW = nW;
+  v.C = C; v.P = P; v.E = E; v.NU = NU; v.NV = NV; v.nW = nW; v.w_dup = hs.w_dup ? 1 : 0;
   v.f = h->f; v.cx = h->cx; v.cy = h->cy; v.huber = h->huber;
   v.cam_var = d_cam_var; v.pt_var = d_pt_var; v.var_cam_user = d_vcu; v.var_pt_user = d_vpu;
   v.pt_ptr = d_pt_ptr; v.pe_cam = d_pe_cam; v.pe_obs = d_pe_obs; v.pe_w = d_pe_w; v.pe_wblk = d_pe_wblk; v.pe_edge = d_pe_edge;

@@ -514,6 +514,120 @@ __device__ __forceinline__ void lin_point(const BAView& v, int p, T* __restrict_
   }
 }
 
+// The point-major linearisation pass with L lanes per point (a group of L
+// consecutive lanes of one wave; L divides 64): lane l takes the edges
+// e0 + l, e0 + l + L, ... of point p (p >= v.P: a padding group with no
+// edges, which still joins the shuffles).  Each lane sums its edges' V and bV
+// terms; the group closes them with xor shuffles (every lane ends with the
+// same sum), then lane 0 adds the point's unary prior and writes V_p, bV_p.
+// The lane of a W block's first edge forms the block: its own edge, then
+// (DUP: the problem has them) the camera's repeated observations of the
+// point, which are adjacent; without DUP the kernel needs no registers for
+// them (124 instead of 204 VGPRs in FP64).  The chi2 of each lane's edges
+// goes into that lane's chi2.  Same terms as lin_point, with the sum over a
+// point's edges grouped by lane.
+template <typename T, bool HESS, int L, bool DUP>
+__device__ __forceinline__ void lin_point_lanes(const BAView& v, int p, int l, T* __restrict__ V,
+                                                T* __restrict__ bV, T* __restrict__ W, double& chi2, double& mb,
+                                                double& md) {
+  const bool valid = p < v.P;
+  double X0 = 0.0, X1 = 0.0, X2 = 0.0;
+  int pv = -1, e0 = 0, e1 = 0;
+  if (valid) {
+    X0 = v.pts[3 * p];
+    X1 = v.pts[3 * p + 1];
+    X2 = v.pts[3 * p + 2];
+    pv = v.pt_var[p];
+    e0 = v.pt_ptr[p];
+    e1 = v.pt_ptr[p + 1];
+  }
+  T vv[6] = {0, 0, 0, 0, 0, 0};
+  T bv[3] = {0, 0, 0};
+  for (int e = e0 + l; e < e1; e += L) {
+    const int c = v.pe_cam[e];
+    double R[kCamRec];
+    load_cam(v, c, R);
+    const Proj pr = project(v, R, X0, X1, X2, v.pe_obs[2 * e], v.pe_obs[2 * e + 1], v.pe_w[e]);
+    chi2 += pr.cost;
+    if (HESS && pv >= 0) {
+      T g0[3], g1[3];
+      jac_pt<T>(v, pr, R, g0, g1);
+      const T om = (T)pr.om, r0 = (T)pr.r0, r1 = (T)pr.r1;
+      vv[0] += om * (g0[0] * g0[0] + g1[0] * g1[0]);
+      vv[1] += om * (g0[0] * g0[1] + g1[0] * g1[1]);
+      vv[2] += om * (g0[0] * g0[2] + g1[0] * g1[2]);
+      vv[3] += om * (g0[1] * g0[1] + g1[1] * g1[1]);
+      vv[4] += om * (g0[1] * g0[2] + g1[1] * g1[2]);
+      vv[5] += om * (g0[2] * g0[2] + g1[2] * g1[2]);
+#pragma unroll
+      for (int k = 0; k < 3; k++) bv[k] += om * (g0[k] * r0 + g1[k] * r1);
+      const int blk = v.pe_wblk[e];
+      if (blk >= 0 && (e == e0 || v.pe_wblk[e - 1] != blk)) {
+        T w[18];
+        {
+          T j0[6], j1[6];
+          jac_cam<T>(v, pr, j0, j1);
+#pragma unroll
+          for (int i = 0; i < 6; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) w[i * 3 + j] = (T)0 + om * (j0[i] * g0[j] + j1[i] * g1[j]);
+        }
+        if constexpr (DUP) {
+          for (int f = e + 1; f < e1 && v.pe_wblk[f] == blk; f++) {
+            const Proj pf = project(v, R, X0, X1, X2, v.pe_obs[2 * f], v.pe_obs[2 * f + 1], v.pe_w[f]);
+            T h0[3], h1[3], j0[6], j1[6];
+            jac_pt<T>(v, pf, R, h0, h1);
+            jac_cam<T>(v, pf, j0, j1);
+            const T of = (T)pf.om;
+#pragma unroll
+            for (int i = 0; i < 6; i++)
+#pragma unroll
+              for (int j = 0; j < 3; j++) w[i * 3 + j] += of * (j0[i] * h0[j] + j1[i] * h1[j]);
+          }
+        }
+#pragma unroll
+        for (int i = 0; i < 18; i++) W[18 * (size_t)blk + i] = w[i];
+      }
+    }
+  }
+  if (HESS) {
+#pragma unroll
+    for (int o = 1; o < L; o <<= 1) {
+#pragma unroll
+      for (int i = 0; i < 6; i++) vv[i] += (T)__shfl_xor((double)vv[i], o);
+#pragma unroll
+      for (int i = 0; i < 3; i++) bv[i] += (T)__shfl_xor((double)bv[i], o);
+    }
+  }
+  if (l != 0 || !valid) return;
+  // the point's unary prior (after its edges): r = w .* (X - X0), J = diag(w)
+  if (v.pt_prior) {
+    const int k = v.pt_prior[p];
+    if (k >= 0) {
+      const double* x0 = v.pt_prior_p + 3 * (size_t)k;
+      const double* wp = v.pt_prior_w + 3 * (size_t)k;
+      const double Xp[3] = {X0, X1, X2};
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        const double e = Xp[i] - x0[i], w2 = wp[i] * wp[i];
+        chi2 += w2 * e * e;
+        if (HESS && pv >= 0) {
+          vv[i == 0 ? 0 : (i == 1 ? 3 : 5)] += (T)w2;
+          bv[i] += (T)(w2 * e);
+        }
+      }
+    }
+  }
+  if (HESS && pv >= 0) {
+#pragma unroll
+    for (int i = 0; i < 6; i++) V[6 * (size_t)pv + i] = vv[i];
+#pragma unroll
+    for (int i = 0; i < 3; i++) bV[3 * (size_t)pv + i] = bv[i];
+    mb = fmax(fabs((double)bv[0]), fmax(fabs((double)bv[1]), fabs((double)bv[2])));
+    md = fmax(fabs((double)vv[0]), fmax(fabs((double)vv[3]), fabs((double)vv[5])));
+  }
+}
+
 // The damped 3x3 block of point q (V + rel diag V + abs I) through its
 // Cholesky factor L in FP64 (MatrixDiagInv.hpp:25-140): V^-1 (packed upper,
 // zero when the block is not positive definite), x = V^-1 (-bV), and the

@@ -24,20 +24,24 @@ namespace nsn {
 
 // ---------------------------------------------------------------------------
 // linearisation, point-major pass: V, bV, W and the chi2 of every edge
-// (nelson/BaseSection.cpp:88-169 + updateH22Block/updateH12Block)
+// (nelson/BaseSection.cpp:88-169 + updateH22Block/updateH12Block), with
+// kLinLanes lanes per point (one lane per point left lines of the W blocks
+// partly written for the length of its edge loop, and they were filled from
+// memory: 200 MB of extra reads at 200 000 points); DUP: some camera
+// observes some point more than once
 // ---------------------------------------------------------------------------
-template <typename T, bool HESS>
+template <typename T, bool HESS, bool DUP>
 __global__ __launch_bounds__(256) void k_lin_points(BAView v, T* __restrict__ V, T* __restrict__ bV,
                                                     T* __restrict__ W, double* __restrict__ part_chi2,
                                                     unsigned long long* __restrict__ maxes, DevCtl* ctl,
                                                     int gate) {
   if (!gate_open(ctl, gate)) return;
   __shared__ double sh[8];
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
   double chi2 = 0.0, mb = 0.0, md = 0.0;
-  if (p < v.P) lin_point<T, HESS>(v, p, V, bV, W, chi2, mb, md);
+  lin_point_lanes<T, HESS, kLinLanes, DUP>(v, g / kLinLanes, g % kLinLanes, V, bV, W, chi2, mb, md);
   if (v.cam_prior) {  // the cameras' unary priors (cost)
-    for (int c = p; c < v.C; c += gridDim.x * blockDim.x) {
+    for (int c = g; c < v.C; c += gridDim.x * blockDim.x) {
       const int k = v.cam_prior[c];
       double r[6];
       if (k >= 0) chi2 += cam_prior_eval<false>(v, c, k, r, nullptr);
@@ -718,12 +722,16 @@ void launch_lin_points(const BAView& v, bool hess, T* V, T* bV, T* W, const Scra
                        hipStream_t st) {
   // always launched (nsn_problem_create requires n_points >= 1): the camera
   // priors' cost is added here too, so it is counted with every point fixed
-  if (hess)
-    hipLaunchKernelGGL((k_lin_points<T, true>), dim3(nblocks(v.P, 256)), dim3(256), 0, st, v, V, bV, W,
-                       s.part_chi2, s.maxes, ctl, gate);
+  const unsigned g = nblocks((long long)v.P * kLinLanes, 256);  // = s.nblk_pts
+  if (hess && v.w_dup)
+    hipLaunchKernelGGL((k_lin_points<T, true, true>), dim3(g), dim3(256), 0, st, v, V, bV, W, s.part_chi2, s.maxes,
+                       ctl, gate);
+  else if (hess)
+    hipLaunchKernelGGL((k_lin_points<T, true, false>), dim3(g), dim3(256), 0, st, v, V, bV, W, s.part_chi2, s.maxes,
+                       ctl, gate);
   else
-    hipLaunchKernelGGL((k_lin_points<T, false>), dim3(nblocks(v.P, 256)), dim3(256), 0, st, v, V, bV, W,
-                       s.part_chi2, s.maxes, ctl, gate);
+    hipLaunchKernelGGL((k_lin_points<T, false, false>), dim3(g), dim3(256), 0, st, v, V, bV, W, s.part_chi2,
+                       s.maxes, ctl, gate);
 }
 
 template <typename T>

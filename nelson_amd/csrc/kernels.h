@@ -27,6 +27,7 @@ enum Gate : int {
 // Read-only structure and state pointers of one BA problem on the device.
 struct BAView {
   int C, P, E, NU, NV, nW;
+  int w_dup;                // some W block has more than one edge (repeated observations)
   double f, cx, cy, huber;
   const int* cam_var;       // [C]  user camera -> variable index or -1
   const int* pt_var;        // [P]  user point  -> variable index or -1
@@ -106,12 +107,15 @@ struct TileProfile {
 };
 
 // reduction scratch layout (doubles), one slot per block of the producing kernel
+// lanes per point in the point-major linearisation pass (divides 64)
+constexpr int kLinLanes = 4;
+
 struct Scratch {
   double* part_chi2;    // [nblk_pts]
   double* part_xv;      // [2 * nblk_pts]  (incsq, den) of the points
   double* part_xu;      // [2 * nblk_cams] (incsq, den) of the cameras
   unsigned long long* maxes;  // [4] max|bU|, max|bV|, max|diag U|, max|diag V| as bit patterns
-  int nblk_pts, nblk_cams, nblk_xv;  // blocks of the point pass (P), xU stats (6 NU), back-substitution (NV)
+  int nblk_pts, nblk_cams, nblk_xv;  // blocks of the point pass (kLinLanes P), xU stats (6 NU), back-substitution (NV)
 };
 
 // ---- BA kernels (ba_kernels.hip), T = double (FP64) or float (mixed) ----

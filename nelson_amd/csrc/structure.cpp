@@ -245,6 +245,7 @@ int host_structure(const nsn_problem_desc* d, HostStructure& s, bool allow_order
   s.wptr.assign(NV + 1, 0);
   s.w_cam.clear();
   s.w_pt.clear();
+  s.w_dup = false;
   for (int k = 0; k < E; k++) {
     const int e = s.pe[k];
     s.pe_cam[k] = d->cam_idx[e];
@@ -256,6 +257,7 @@ int host_structure(const nsn_problem_desc* d, HostStructure& s, bool allow_order
       // repeated observations of one (camera, point) pair are adjacent and
       // share one W block
       const bool same = !s.w_cam.empty() && s.w_pt.back() == pv && s.w_cam.back() == cv;
+      if (same) s.w_dup = true;
       if (!same) {
         s.w_cam.push_back(cv);
         s.w_pt.push_back(pv);

@@ -25,6 +25,7 @@ struct HostStructure {
   std::vector<double> pe_obs, pe_w;
   // W blocks (one per variable (camera, point) pair), point-major
   std::vector<int> wptr, w_cam, w_pt;
+  bool w_dup = false;  // some block has more than one edge
   // edges of the variable cameras in camera-major order (camera, then point)
   std::vector<int> ce;  // user edge index of each slot
   std::vector<int> cam_ptr, ce_pt;
