@@ -577,7 +577,7 @@ int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
   h->scr.maxes = m.alloc<unsigned long long>(4);
   h->scr.nblk_pts = (int)(((long long)P * kLinLanes + 255) / 256);
   h->scr.nblk_cams = (6 * NU + 255) / 256;
-  h->scr.nblk_xv = (NV + 255) / 256;
+  h->scr.nblk_xv = (int)(((long long)NV * kLinLanes + 255) / 256);
   h->ctl = m.alloc<DevCtl>(1);
   h->log_cap = 4096;
   h->log = m.alloc<DevLog>(h->log_cap);

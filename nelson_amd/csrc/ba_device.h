@@ -761,14 +761,18 @@ __device__ __forceinline__ bool vinv_y_point(const BAView& v, int q, const T* __
 // rhs = -bV or rhs_v, added to xV with `add`; inc = |xV_q|^2 and
 // den = xV_q^T (mu xV_q - b_q) with b = bden (FP64) or bV
 template <typename T>
-__device__ __forceinline__ void backsub_point(const BAView& v, int q, const T* __restrict__ W,
-                                              const T* __restrict__ Vinv, const T* __restrict__ bV,
-                                              const double* __restrict__ xU, double* __restrict__ xV, double mu,
-                                              const double* __restrict__ rhs_v, int add,
-                                              const double* __restrict__ bden, double& inc, double& den) {
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+__device__ __forceinline__ void backsub_finish(const BAView& v, int q, const T* __restrict__ Vinv,
+                                               const T* __restrict__ bV, double* __restrict__ xV, double mu,
+                                               const double* __restrict__ rhs_v, int add,
+                                               const double* __restrict__ bden, double s0, double s1, double s2,
+                                               double& inc, double& den);
+
+// (W^T xU)_q over the W blocks wptr[q] + l, wptr[q] + l + L, ... of point q
+template <typename T, int L>
+__device__ __forceinline__ void wt_xu(const BAView& v, int q, int l, const T* __restrict__ W,
+                                      const double* __restrict__ xU, double& s0, double& s1, double& s2) {
   const int k1 = v.wptr[q + 1];
-  for (int k = v.wptr[q]; k < k1; k++) {
+  for (int k = v.wptr[q] + l; k < k1; k += L) {
     const T* w = W + 18 * (size_t)k;
     const double* x = xU + 6 * (size_t)v.w_cam[k];
 #pragma unroll
@@ -779,6 +783,26 @@ __device__ __forceinline__ void backsub_point(const BAView& v, int q, const T* _
       s2 += (double)w[3 * i + 2] * xi;
     }
   }
+}
+
+template <typename T>
+__device__ __forceinline__ void backsub_point(const BAView& v, int q, const T* __restrict__ W,
+                                              const T* __restrict__ Vinv, const T* __restrict__ bV,
+                                              const double* __restrict__ xU, double* __restrict__ xV, double mu,
+                                              const double* __restrict__ rhs_v, int add,
+                                              const double* __restrict__ bden, double& inc, double& den) {
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  wt_xu<T, 1>(v, q, 0, W, xU, s0, s1, s2);
+  backsub_finish<T>(v, q, Vinv, bV, xV, mu, rhs_v, add, bden, s0, s1, s2, inc, den);
+}
+
+// the rest of point q's back-substitution from s = (W^T xU)_q
+template <typename T>
+__device__ __forceinline__ void backsub_finish(const BAView& v, int q, const T* __restrict__ Vinv,
+                                               const T* __restrict__ bV, double* __restrict__ xV, double mu,
+                                               const double* __restrict__ rhs_v, int add,
+                                               const double* __restrict__ bden, double s0, double s1, double s2,
+                                               double& inc, double& den) {
   double bv0, bv1, bv2;
   if (bden) {
     bv0 = bden[3 * (size_t)q]; bv1 = bden[3 * (size_t)q + 1]; bv2 = bden[3 * (size_t)q + 2];

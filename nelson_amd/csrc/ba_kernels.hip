@@ -358,12 +358,23 @@ __global__ __launch_bounds__(256) void k_backsub(BAView v, const T* __restrict__
                                                  double* __restrict__ accU, int nacc) {
   if (!gate_open(ctl, gate)) return;
   __shared__ double sh[4];
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
   // refinement: the camera correction is added to the solution here
   // (accU += xU, grid-stride), saving a launch
-  for (int g = q; g < nacc; g += gridDim.x * blockDim.x) accU[g] += xU[g];
+  for (int g = t; g < nacc; g += gridDim.x * blockDim.x) accU[g] += xU[g];
+  // kLinLanes lanes per point share its W blocks (a wave reads 16 points'
+  // blocks about 4 at a time each), the sums closed by shuffles
+  const int q = t / kLinLanes, l = t % kLinLanes;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  if (q < v.NV) wt_xu<T, kLinLanes>(v, q, l, W, xU, s0, s1, s2);
+#pragma unroll
+  for (int o = 1; o < kLinLanes; o <<= 1) {
+    s0 += __shfl_xor(s0, o);
+    s1 += __shfl_xor(s1, o);
+    s2 += __shfl_xor(s2, o);
+  }
   double inc = 0.0, den = 0.0;
-  if (q < v.NV) backsub_point<T>(v, q, W, Vinv, bV, xU, xV, ctl->abs_, rhs_v, add, bden, inc, den);
+  if (q < v.NV && l == 0) backsub_finish<T>(v, q, Vinv, bV, xV, ctl->abs_, rhs_v, add, bden, s0, s1, s2, inc, den);
   const double a = block_sum256(inc, sh);
   const double b = block_sum256(den, sh);
   if (threadIdx.x == 0) {
@@ -784,8 +795,8 @@ void launch_backsub(const BAView& v, const T* W, const T* Vinv, const T* bV, con
     if (accU && nacc > 0) launch_add(accU, xU, nacc, ctl, gate, st);
     return;
   }
-  hipLaunchKernelGGL((k_backsub<T>), dim3(nblocks(v.NV, 256)), dim3(256), 0, st, v, W, Vinv, bV, xU, xV,
-                     s.part_xv, ctl, gate, rhs_v, add, bden, accU, accU ? nacc : 0);
+  hipLaunchKernelGGL((k_backsub<T>), dim3(nblocks((long long)v.NV * kLinLanes, 256)), dim3(256), 0, st, v, W, Vinv,
+                     bV, xU, xV, s.part_xv, ctl, gate, rhs_v, add, bden, accU, accU ? nacc : 0);  // s.nblk_xv
 }
 
 void launch_ir_residual(const BAView& v, const double* xU, const double* xV, double* rU, double* rV, double* bU64,

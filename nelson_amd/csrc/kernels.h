@@ -112,10 +112,11 @@ constexpr int kLinLanes = 4;
 
 struct Scratch {
   double* part_chi2;    // [nblk_pts]
-  double* part_xv;      // [2 * nblk_pts]  (incsq, den) of the points
+  double* part_xv;      // [2 * nblk_pts] (nblk_pts >= nblk_xv)  (incsq, den) of the points
   double* part_xu;      // [2 * nblk_cams] (incsq, den) of the cameras
   unsigned long long* maxes;  // [4] max|bU|, max|bV|, max|diag U|, max|diag V| as bit patterns
-  int nblk_pts, nblk_cams, nblk_xv;  // blocks of the point pass (kLinLanes P), xU stats (6 NU), back-substitution (NV)
+  // blocks of the point pass (kLinLanes P), xU stats (6 NU), back-substitution (kLinLanes NV)
+  int nblk_pts, nblk_cams, nblk_xv;
 };
 
 // ---- BA kernels (ba_kernels.hip), T = double (FP64) or float (mixed) ----
