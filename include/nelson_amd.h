@@ -251,9 +251,14 @@ int nsn_import_hessian(nsn_problem* h, const double* U, const double* V, const d
  * whose edges the caller evaluates on the host (the reference's generic
  * DoubleSection with user edges, nelson/DoubleSection.h:181-545): n_u
  * parameters of the first section (6x6 diagonal blocks of U), n_v of the
- * second (3x3 diagonal blocks of V, which must be block diagonal: no V-V
- * edges), the non-zero 6x3 W blocks (u, v) and the non-zero off-diagonal 6x6
- * U blocks (i, j), i < j, of U-U edges.  Every parameter is variable (the
+ * second (3x3 diagonal blocks of V, which must be block diagonal), the
+ * non-zero 6x3 W blocks (u, v) and the non-zero off-diagonal 6x6 U blocks
+ * (i, j), i < j, of U-U edges.  A section with V-V edges (V not block
+ * diagonal) is given as one first section of n_v + n_u parameters, the V
+ * parameters first, with n_v = 0: its V-V blocks, W blocks (transposed) and
+ * U-U blocks are all off-diagonal blocks; the factorisation then eliminates
+ * V first and S = U - W V^-1 W^T is its trailing part (include/nelson_amd.hpp
+ * does this for EdgeBinaryVV).  Every parameter is variable (the
  * caller leaves fixed ones out).  Smaller blocks are embedded by the caller
  * (include/nelson_amd.hpp pads them with an identity diagonal and zero
  * gradient, which leaves the solution of the real part unchanged).  The

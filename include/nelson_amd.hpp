@@ -9,11 +9,13 @@
 //                              implements parameterU / parameterV (NodeId),
 //                              oplus(inc), backupSolution, rollbackSolution;
 //                              edges derive from EdgeUnaryU / EdgeUnaryV /
-//                              EdgeBinaryUV / EdgeBinaryUU and implement
-//                              update(bool) (calling setChi2) and the block
-//                              updaters (nelson/EdgeBinary.h:121-158).  Edges
-//                              are evaluated on the host; the Schur step runs
-//                              on the GPU (V block diagonal, BU <= 6, BV <= 3).
+//                              EdgeBinaryUV / EdgeBinaryUU / EdgeBinaryVV /
+//                              EdgeNaryU and implement update(bool) (calling
+//                              setChi2) and the block updaters
+//                              (nelson/EdgeBinary.h:121-158).  Edges are
+//                              evaluated on the host; the Schur step runs on
+//                              the GPU (BU <= 6, BV <= 3; with V-V edges the
+//                              damped V is factorised together with S).
 //   BASection                  the BA sample's section (nelson/samples/
 //                              BASchurPerfomances.cpp:163-246) with its
 //                              reprojection edges evaluated on the GPU
@@ -211,6 +213,10 @@ class SchurSystem {
   virtual int numU() const = 0;  // variable parameters
   virtual int numV() const = 0;
   virtual const void* owner() const = 0;
+  // true when the device system holds the V parameters as its first numV()
+  // parameters and the U parameters after them (a V with off-diagonal
+  // blocks, factorised together with S); false: separate U and V parts
+  virtual bool vFirstInU() const { return false; }
 };
 
 // ---------------------------------------------------------------------------
@@ -581,6 +587,34 @@ class EdgeBinaryUU : public detail::EdgeBase {
   NodeId i_, j_;
 };
 
+// an edge between two V parameters (nelson/DoubleSection.h:498-529,
+// EdgeBinaryAdapterVV): updateH11Block / updateH22Block (BV x BV, BV),
+// updateH12Block(H) (BV x BV, the block of rows par_1, columns par_2).  V is
+// then not block diagonal: the solver factorises the damped V together with
+// S (the V parameters eliminated first), which gives the Schur complement
+// S = U - W V^-1 W^T as the trailing part of one factorisation.
+template <class Section, class Derived>
+class EdgeBinaryVV : public detail::EdgeBase {
+ public:
+  NodeId par_1_Id() const { return i_; }
+  NodeId par_2_Id() const { return j_; }
+  const auto& parameter_1() const { return section_->parameterV(i_); }
+  const auto& parameter_2() const { return section_->parameterV(j_); }
+  Section& section() const { return *section_; }
+  void bind_(Section* s, NodeId i, NodeId j) { section_ = s, i_ = i, j_ = j; }
+  void applyUpdaters_() override {
+    auto& H = section_->hessianStorage_();
+    Derived* d = static_cast<Derived*>(this);
+    if (i_.isVariable()) d->updateH11Block(H.Vblock(i_.id()), H.bVseg(i_.id()));
+    if (j_.isVariable()) d->updateH22Block(H.Vblock(j_.id()), H.bVseg(j_.id()));
+    if (i_.isVariable() && j_.isVariable()) detail::updateH12(d, H.Voffblock(uid_, uidTrans_));
+  }
+
+ private:
+  Section* section_ = nullptr;
+  NodeId i_, j_;
+};
+
 // an edge on N parameters of the first section (nelson/EdgeNary.h:125-150;
 // N = kDynamic: the count is set by addEdge): parameter(i) / parId(i) /
 // numParams(); the derived edge implements update(bool) and
@@ -653,6 +687,8 @@ class DoubleSectionHessian : public SchurSystem {
   MatRef Uoffblock(int uid, bool trans) { return MatRef(&Uo_[(size_t)BU * BU * uid], BU, BU, BU, trans); }
   MatRef Vblock(int j) { return MatRef(&V_[(size_t)BV * BV * j], BV, BV, BV); }
   MatRef Wblock(int uid) { return MatRef(&W_[(size_t)BU * BV * uid], BU, BV, BV); }
+  MatRef Voffblock(int uid, bool trans) { return MatRef(&Vo_[(size_t)BV * BV * uid], BV, BV, BV, trans); }
+  int numVoff() const { return (int)voI_.size(); }
   VecRef bUseg(int i) { return VecRef(&bU_[(size_t)BU * i], BU); }
   VecRef bVseg(int j) { return VecRef(&bV_[(size_t)BV * j], BV); }
   const std::vector<double>& bUvalues() const { return bU_; }
@@ -665,6 +701,11 @@ class DoubleSectionHessian : public SchurSystem {
   }
   void prepare() const override {
     if (!dirty_) return;
+    if (merged_) {
+      prepareMerged();
+      dirty_ = false;
+      return;
+    }
     const int NU = numU(), NV = numV(), nW = numW(), nO = numUoff();
     std::vector<double> U(36 * (size_t)NU, 0.0), Uo(36 * (size_t)nO, 0.0), V(9 * (size_t)NV, 0.0),
         W(18 * (size_t)nW, 0.0), bU(6 * (size_t)NU, 0.0), bV(3 * (size_t)NV, 0.0);
@@ -708,30 +749,56 @@ class DoubleSectionHessian : public SchurSystem {
   int numU() const override { return nU_; }
   int numV() const override { return nV_; }
   const void* owner() const override { return owner_; }
+  bool vFirstInU() const override { return merged_; }
 
   // section internals
   void setup_(const void* owner, int nU, int nV, std::vector<std::pair<int, int>> wPairs,
-              std::vector<std::pair<int, int>> uoPairs, Precision p, int device) {
+              std::vector<std::pair<int, int>> uoPairs, std::vector<std::pair<int, int>> voPairs, Precision p,
+              int device) {
     owner_ = owner;
     nU_ = nU;
     nV_ = nV;
     for (auto& w : wPairs) wU_.push_back(w.first), wV_.push_back(w.second);
     for (auto& o : uoPairs) uoI_.push_back(o.first), uoJ_.push_back(o.second);
+    for (auto& o : voPairs) voI_.push_back(o.first), voJ_.push_back(o.second);
     U_.assign((size_t)BU * BU * nU, 0.0);
     Uo_.assign((size_t)BU * BU * uoI_.size(), 0.0);
     V_.assign((size_t)BV * BV * nV, 0.0);
+    Vo_.assign((size_t)BV * BV * voI_.size(), 0.0);
     W_.assign((size_t)BU * BV * wU_.size(), 0.0);
     bU_.assign((size_t)BU * nU, 0.0);
     bV_.assign((size_t)BV * nV, 0.0);
+    merged_ = !voI_.empty();
     nsn_block_desc d{};
-    d.n_u = nU;
-    d.n_v = nV;
-    d.n_w = (int)wU_.size();
-    d.w_u = wU_.data();
-    d.w_v = wV_.data();
-    d.n_uoff = (int)uoI_.size();
-    d.uoff_i = uoI_.data();
-    d.uoff_j = uoJ_.data();
+    std::vector<int> mi, mj;
+    if (merged_) {
+      // V has off-diagonal blocks: one system of nV + nU parameters, the V
+      // parameters first (device index q), then the U parameters (nV + u);
+      // its off-diagonal blocks are the V-V blocks, the W blocks transposed
+      // (rows V q, columns U u) and the U-U blocks, in row-major order
+      for (size_t k = 0; k < voI_.size(); k++) mp_.push_back({voI_[k], voJ_[k], 0, (int)k});
+      for (size_t k = 0; k < wU_.size(); k++) mp_.push_back({wV_[k], nV + wU_[k], 1, (int)k});
+      for (size_t k = 0; k < uoI_.size(); k++) mp_.push_back({nV + uoI_[k], nV + uoJ_[k], 2, (int)k});
+      std::sort(mp_.begin(), mp_.end(), [](const MergedBlock& x, const MergedBlock& y) {
+        return x.i != y.i ? x.i < y.i : x.j < y.j;
+      });
+      for (auto& m : mp_) mi.push_back(m.i), mj.push_back(m.j);
+      d.n_u = nV + nU;
+      d.n_v = 0;
+      d.n_w = 0;
+      d.n_uoff = (int)mp_.size();
+      d.uoff_i = mi.data();
+      d.uoff_j = mj.data();
+    } else {
+      d.n_u = nU;
+      d.n_v = nV;
+      d.n_w = (int)wU_.size();
+      d.w_u = wU_.data();
+      d.w_v = wV_.data();
+      d.n_uoff = (int)uoI_.size();
+      d.uoff_i = uoI_.data();
+      d.uoff_j = uoJ_.data();
+    }
     d.precision = (int)p;
     d.device = device;
     check(nsn_problem_create_blocks(&d, &h_), "nsn_problem_create_blocks");
@@ -740,6 +807,7 @@ class DoubleSectionHessian : public SchurSystem {
     std::fill(U_.begin(), U_.end(), 0.0);
     std::fill(Uo_.begin(), Uo_.end(), 0.0);
     std::fill(V_.begin(), V_.end(), 0.0);
+    std::fill(Vo_.begin(), Vo_.end(), 0.0);
     std::fill(W_.begin(), W_.end(), 0.0);
     std::fill(bU_.begin(), bU_.end(), 0.0);
     std::fill(bV_.begin(), bV_.end(), 0.0);
@@ -748,10 +816,55 @@ class DoubleSectionHessian : public SchurSystem {
   void touch_() { dirty_ = true; }
 
  private:
+  struct MergedBlock {
+    int i, j;  // device parameters, i < j
+    int kind;  // 0: V-V block, 1: W block (transposed), 2: U-U block
+    int k;     // its index in Vo_ / W_ / Uo_
+  };
+  // the merged system (V with off-diagonal blocks): every block embedded in
+  // the device's 6 x 6 layout, V parameters first
+  void prepareMerged() const {
+    const int NU = numU(), NV = numV(), ND = NU + NV;
+    std::vector<double> U(36 * (size_t)ND, 0.0), Uo(36 * mp_.size() + 1, 0.0), bU(6 * (size_t)ND, 0.0);
+    for (int q = 0; q < NV; q++) {
+      for (int a = 0; a < 6; a++)
+        for (int c = 0; c < 6; c++)
+          U[36 * (size_t)q + 6 * a + c] = (a < BV && c < BV) ? V_[(size_t)BV * BV * q + BV * a + c] : (a == c ? 1.0 : 0.0);
+      for (int a = 0; a < BV; a++) bU[6 * (size_t)q + a] = bV_[(size_t)BV * q + a];
+    }
+    for (int u = 0; u < NU; u++) {
+      double* dst = &U[36 * (size_t)(NV + u)];
+      for (int a = 0; a < 6; a++)
+        for (int c = 0; c < 6; c++)
+          dst[6 * a + c] = (a < BU && c < BU) ? U_[(size_t)BU * BU * u + BU * a + c] : (a == c ? 1.0 : 0.0);
+      for (int a = 0; a < BU; a++) bU[6 * (size_t)(NV + u) + a] = bU_[(size_t)BU * u + a];
+    }
+    for (size_t m = 0; m < mp_.size(); m++) {
+      double* dst = &Uo[36 * m];
+      const int k = mp_[m].k;
+      if (mp_[m].kind == 0) {
+        for (int a = 0; a < BV; a++)
+          for (int c = 0; c < BV; c++) dst[6 * a + c] = Vo_[(size_t)BV * BV * k + BV * a + c];
+      } else if (mp_[m].kind == 1) {
+        for (int a = 0; a < BV; a++)
+          for (int c = 0; c < BU; c++) dst[6 * a + c] = W_[(size_t)BU * BV * k + BV * c + a];
+      } else {
+        for (int a = 0; a < BU; a++)
+          for (int c = 0; c < BU; c++) dst[6 * a + c] = Uo_[(size_t)BU * BU * k + BU * a + c];
+      }
+    }
+    double none = 0.0;
+    check(nsn_import_hessian_blocks(problem(), U.data(), mp_.empty() ? nullptr : Uo.data(), &none, &none,
+                                    bU.data(), &none, chi2_),
+          "nsn_import_hessian_blocks");
+  }
+
   const void* owner_ = nullptr;
   int nU_ = 0, nV_ = 0;
-  std::vector<int> wU_, wV_, uoI_, uoJ_;
-  std::vector<double> U_, Uo_, V_, W_, bU_, bV_;
+  std::vector<int> wU_, wV_, uoI_, uoJ_, voI_, voJ_;
+  std::vector<double> U_, Uo_, V_, Vo_, W_, bU_, bV_;
+  bool merged_ = false;
+  std::vector<MergedBlock> mp_;
   double chi2_ = 0.0;
   mutable bool dirty_ = true;
   nsn_problem* h_ = nullptr;
@@ -773,6 +886,8 @@ class DoubleSection {
   using EdgeBinaryUV = nelson_amd::EdgeBinaryUV<Derived, E>;
   template <class E>
   using EdgeBinaryUU = nelson_amd::EdgeBinaryUU<Derived, E>;
+  template <class E>
+  using EdgeBinaryVV = nelson_amd::EdgeBinaryVV<Derived, E>;
   template <class E, int N = kDynamic>
   using EdgeNaryU = nelson_amd::EdgeNaryU<Derived, E, N>;
 
@@ -826,6 +941,17 @@ class DoubleSection {
       throw Error(NSN_EINVAL, "addEdge: a binary U edge joins a parameter to itself");
     e->bind_(&self(), i, j);
     if (i.isVariable() && j.isVariable()) uu_.push_back({edges_.size(), {i.id(), j.id()}});
+    edges_.push_back(std::move(own));
+  }
+  template <class E>
+  void addEdge(NodeId i, NodeId j, nelson_amd::EdgeBinaryVV<Derived, E>* e) {
+    std::unique_ptr<detail::EdgeBase> own(e);
+    checkAdd(i, false);
+    checkAdd(j, false);
+    if (i.isVariable() && j.isVariable() && i.id() == j.id())
+      throw Error(NSN_EINVAL, "addEdge: a binary V edge joins a parameter to itself");
+    e->bind_(&self(), i, j);
+    if (i.isVariable() && j.isVariable()) vv_.push_back({edges_.size(), {i.id(), j.id()}});
     edges_.push_back(std::move(own));
   }
 
@@ -899,8 +1025,17 @@ class DoubleSection {
       std::sort(v.begin(), v.end());
       v.erase(std::unique(v.begin(), v.end()), v.end());
     };
+    std::vector<std::pair<int, int>> vo;
+    for (auto& e : vv_) vo.push_back({std::min(e.second.first, e.second.second), std::max(e.second.first, e.second.second)});
     uniq(w);
     uniq(o);
+    uniq(vo);
+    for (auto& e : vv_) {
+      const int a = e.second.first, b = e.second.second;
+      edges_[e.first]->uid_ =
+          (int)(std::lower_bound(vo.begin(), vo.end(), std::make_pair(std::min(a, b), std::max(a, b))) - vo.begin());
+      edges_[e.first]->uidTrans_ = a > b;
+    }
     for (auto& e : uv_)
       edges_[e.first]->uid_ =
           (int)(std::lower_bound(w.begin(), w.end(), std::make_pair(u2i_[e.second.first], e.second.second)) - w.begin());
@@ -917,7 +1052,7 @@ class DoubleSection {
           (int)(std::lower_bound(o.begin(), o.end(), std::make_pair(std::min(a, b), std::max(a, b))) - o.begin()));
       ed.pairTrans_.push_back(a > b ? 1 : 0);
     }
-    H_.setup_(this, numParametersU(), numParametersV(), w, o, p, device);
+    H_.setup_(this, numParametersU(), numParametersV(), w, o, vo, p, device);
     structured_ = true;
   }
 
@@ -989,6 +1124,7 @@ class DoubleSection {
   std::vector<int> u2i_;
   std::vector<std::unique_ptr<detail::EdgeBase>> edges_;
   std::vector<std::pair<size_t, std::pair<int, int>>> uv_, uu_;  // (edge, (user U id, V / U id))
+  std::vector<std::pair<size_t, std::pair<int, int>>> vv_;       // (edge, (V id, V id))
   std::vector<std::pair<size_t, std::pair<int, int>>> nary_;    // (edge, variable pair), in pair order
   HessianT H_;
 };
@@ -1079,7 +1215,11 @@ class SolverCholeskySchur {
     sys_ = &H;
     H.prepare();
     const int NU = H.numU(), NV = H.numV(), BU = H.blockU(), BV = H.blockV();
-    std::vector<double> du(6 * (size_t)NU), dv(3 * (size_t)NV);
+    // with V factorised together with S the device's U part holds the V
+    // parameters first (6 values each), then the U parameters
+    const bool vu = H.vFirstInU();
+    const int offU = vu ? NV : 0;
+    std::vector<double> du(6 * (size_t)(NU + offU) + 1), dv(3 * (size_t)(vu ? 0 : NV) + 1);
     double sq = 0;
     const int rc = nsn_schur_step(H.problem(), relLambda, absLambda, du.data(), dv.data(), &sq);
     check(rc, "nsn_schur_step");
@@ -1089,9 +1229,9 @@ class SolverCholeskySchur {
     inc_.u_.v_.resize((size_t)BU * NU);
     inc_.v_.v_.resize((size_t)BV * NV);
     for (int u = 0; u < NU; u++)
-      for (int a = 0; a < BU; a++) inc_.u_.v_[(size_t)BU * u + a] = du[6 * (size_t)u + a];
+      for (int a = 0; a < BU; a++) inc_.u_.v_[(size_t)BU * u + a] = du[6 * (size_t)(offU + u) + a];
     for (int q = 0; q < NV; q++)
-      for (int a = 0; a < BV; a++) inc_.v_.v_[(size_t)BV * q + a] = dv[3 * (size_t)q + a];
+      for (int a = 0; a < BV; a++) inc_.v_.v_[(size_t)BV * q + a] = vu ? du[6 * (size_t)q + a] : dv[3 * (size_t)q + a];
     // the padding's increment is exactly zero, so the norm is the device's
     sq_ = sq;
     return rc == NSN_OK;

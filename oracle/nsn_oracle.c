@@ -49,8 +49,17 @@ int or_hessian_alloc(or_hessian* h, int BU, int BV, int NU, int NV, int nW) {
   return h->U && h->V && h->W && h->bU && h->bV && h->Wi && h->Wj;
 }
 
+int or_hessian_alloc_voff(or_hessian* h, int nVo) {
+  h->nVo = nVo;
+  h->Vo = (double*)calloc((size_t)nVo * h->BV * h->BV + 1, sizeof(double));
+  h->Voi = (int*)calloc((size_t)nVo + 1, sizeof(int));
+  h->Voj = (int*)calloc((size_t)nVo + 1, sizeof(int));
+  return h->Vo && h->Voi && h->Voj;
+}
+
 void or_hessian_free(or_hessian* h) {
   free(h->U); free(h->V); free(h->W); free(h->bU); free(h->bV); free(h->Wi); free(h->Wj);
+  free(h->Vo); free(h->Voi); free(h->Voj);
   memset(h, 0, sizeof(*h));
 }
 
@@ -63,6 +72,7 @@ void or_hessian_clear(or_hessian* h) {
   memset(h->W, 0, (size_t)h->nW * h->BU * h->BV * sizeof(double));
   memset(h->bU, 0, nu * sizeof(double));
   memset(h->bV, 0, (size_t)h->NV * h->BV * sizeof(double));
+  if (h->nVo) memset(h->Vo, 0, (size_t)h->nVo * h->BV * h->BV * sizeof(double));
 }
 
 /* nelson/DoubleSectionHessian.hpp:60-63 */
@@ -368,6 +378,77 @@ double or_max_abs_h_diag(const or_hessian* h) {  /* SolverCholeskySchur.hpp:75-7
   return mv > mu ? mv : mu;
 }
 
+/* V with off-diagonal blocks (V-V edges, nelson/DoubleSection.h:498-510):
+ * SolverCholeskySchur's V solver is then a sparse LDL^T of the whole damped
+ * V (solverVType, SolverCholeskySchur.h:82; SolverCholeskySparse.hpp:41-69),
+ * restated as the simplicial LDL^T without pivoting in natural order over
+ * the nv x nv matrix (column-major lower, upper blocks read as symmetric).
+ * The reference orders V by AMD first; an ordering changes the rounding
+ * only.  Returns the factor (NULL if a pivot is exactly zero). */
+static double* v_full_factor(const or_hessian* h, double rel, double abs_, int* ok) {
+  const int BV = h->BV, NV = h->NV;
+  const size_t nv = (size_t)BV * NV;
+  double* a = (double*)calloc(nv * nv + 1, sizeof(double));
+  for (int p = 0; p < NV; p++) {
+    const double* v = h->V + (size_t)p * BV * BV;
+    for (int i = 0; i < BV; i++)
+      for (int j = 0; j <= i; j++) a[((size_t)p * BV + j) * nv + (size_t)p * BV + i] = v[j * BV + i];
+  }
+  for (int k = 0; k < h->nVo; k++) {   /* block (Voi, Voj), Voi < Voj: lower entry (Voj row, Voi col) */
+    const double* v = h->Vo + (size_t)k * BV * BV;
+    const size_t r0 = (size_t)h->Voj[k] * BV, c0 = (size_t)h->Voi[k] * BV;
+    for (int i = 0; i < BV; i++)
+      for (int j = 0; j < BV; j++) a[(c0 + i) * nv + r0 + j] = v[i * BV + j];
+  }
+  if (rel != 0 || abs_ != 0)    /* the V solver's damping, MatrixDiagInv.hpp:25-140 */
+    for (size_t i = 0; i < nv; i++) a[i * nv + i] = a[i * nv + i] * (1 + rel) + abs_;
+  *ok = ldlt_nopiv_factor(a, (int)nv);
+  return a;
+}
+
+/* the reduction with a V factored whole: bS = -bU - W x0, S = (U - W V^-1
+ * W^T).upper + damping, Y = V^-1 W^T formed column by column
+ * (SolverCholeskySchur.hpp:114-124) */
+static void schur_reduce_full_v(const or_hessian* h, double rel, double abs_, const double* vf,
+                                const double* xv0, double* S, double* bS) {
+  const int BU = h->BU, BV = h->BV, nW = h->nW;
+  const size_t nu = (size_t)BU * h->NU, nv = (size_t)BV * h->NV;
+  for (size_t i = 0; i < nu; i++) {
+    double t = 0;
+    for (int k = 0; k < nW; k++) {
+      if ((size_t)h->Wi[k] != i / BU) continue;
+      const double* w = h->W + (size_t)k * BU * BV + (i % BU) * BV;
+      for (int c = 0; c < BV; c++) t += w[c] * xv0[(size_t)h->Wj[k] * BV + c];
+    }
+    bS[i] = -h->bU[i] - t;
+  }
+  for (size_t i = 0; i < nu; i++)
+    for (size_t j = 0; j < nu; j++) S[i * nu + j] = (j >= i) ? h->U[i * nu + j] : 0.0;
+  double* y = (double*)malloc(nv * sizeof(double) + 8);
+  for (size_t j = 0; j < nu; j++) {     /* column j of V^-1 W^T */
+    memset(y, 0, nv * sizeof(double));
+    for (int k = 0; k < nW; k++) {
+      if ((size_t)h->Wi[k] != j / BU) continue;
+      const double* w = h->W + (size_t)k * BU * BV + (j % BU) * BV;
+      for (int c = 0; c < BV; c++) y[(size_t)h->Wj[k] * BV + c] += w[c];
+    }
+    ldlt_nopiv_solve(vf, (int)nv, y);
+    for (int k = 0; k < nW; k++) {      /* S_ij -= (W y)_i for i <= j */
+      const size_t i0 = (size_t)h->Wi[k] * BU;
+      const double* w = h->W + (size_t)k * BU * BV;
+      for (int r = 0; r < BU; r++) {
+        if (i0 + r > j) continue;
+        double t = 0;
+        for (int c = 0; c < BV; c++) t += w[r * BV + c] * y[(size_t)h->Wj[k] * BV + c];
+        S[(i0 + r) * nu + j] -= t;
+      }
+    }
+  }
+  free(y);
+  if (rel != 0 || abs_ != 0)
+    for (size_t i = 0; i < nu; i++) { S[i * nu + i] += rel * h->U[i * nu + i]; S[i * nu + i] += abs_; }
+}
+
 /* The reduction half of the Schur step (nelson/SolverCholeskySchur.hpp:91-124):
  * the damped V factor per point block (vf, BV x BV column-major lower LDL^T),
  * xv0 = V^-1 (-bV), bS = -bU - W xv0 and the damped S = (U - W V^-1 W^T).upper,
@@ -380,6 +461,20 @@ static int schur_reduce(const or_hessian* h, double rel, double abs_, double** v
   const int BU = h->BU, BV = h->BV, NV = h->NV, nW = h->nW;
   const size_t nu = (size_t)BU * h->NU, nv = (size_t)BV * NV;
   int ok = 1;
+  if (h->nVo > 0) {
+    double* vf = v_full_factor(h, rel, abs_, &ok);
+    double* xv0 = (double*)malloc(nv * sizeof(double) + 8);
+    double* S = NULL; double* bS = NULL;
+    if (ok) {
+      for (size_t i = 0; i < nv; i++) xv0[i] = -h->bV[i];
+      ldlt_nopiv_solve(vf, (int)nv, xv0);
+      S = (double*)malloc(nu * nu * sizeof(double) + 8);
+      bS = (double*)malloc(nu * sizeof(double) + 8);
+      schur_reduce_full_v(h, rel, abs_, vf, xv0, S, bS);
+    }
+    *vf_out = vf; *xv0_out = xv0; *S_out = S; *bS_out = bS;
+    return ok;
+  }
   /* V solver: per diagonal block, damped LDL^T without pivoting in natural
    * order, failing on an exactly zero pivot: the simplicial LDL^T that
    * SolverCholeskySchur uses as its block-diagonal V solver
@@ -475,8 +570,11 @@ int or_schur_step(const or_hessian* h, double rel, double abs_, double* xU, doub
     for (size_t i = 0; i < nv; i++) xV[i] = 0.0;
     or_wtx_sub(nW, BU, BV, h->Wi, h->Wj, h->W, xU, xV);
     for (size_t i = 0; i < nv; i++) xV[i] = -h->bV[i] + xV[i];
-    for (int p = 0; p < NV; p++)
-      ldlt_nopiv_solve(vf + (size_t)p * BV * BV, BV, xV + (size_t)p * BV);
+    if (h->nVo > 0)
+      ldlt_nopiv_solve(vf, (int)nv, xV);
+    else
+      for (int p = 0; p < NV; p++)
+        ldlt_nopiv_solve(vf + (size_t)p * BV * BV, BV, xV + (size_t)p * BV);
   } else {
     memset(xU, 0, nu * sizeof(double));        /* :151-154 */
     memset(xV, 0, nv * sizeof(double));

@@ -30,10 +30,17 @@ typedef struct {
   double *U, *V, *W, *bU, *bV;
   int *Wi, *Wj;
   double chi2;
+  /* off-diagonal V blocks of V-V edges (nelson/DoubleSection.h:498-510):
+   * Vo holds BV x BV row-major blocks V_(Voi[k], Voj[k]), Voi[k] < Voj[k];
+   * nVo = 0 (V block diagonal) unless or_hessian_alloc_voff was called */
+  int nVo;
+  double* Vo;
+  int *Voi, *Voj;
 } or_hessian;
 
 int  or_hessian_alloc(or_hessian* h, int BU, int BV, int NU, int NV, int nW);
 void or_hessian_free(or_hessian* h);
+int  or_hessian_alloc_voff(or_hessian* h, int nVo);          /* after or_hessian_alloc */
 void or_hessian_clear(or_hessian* h);                     /* DoubleSectionHessian.hpp:51-58 */
 double or_hessian_max_abs_b(const or_hessian* h);         /* DoubleSectionHessian.hpp:60-63 */
 

@@ -20,7 +20,7 @@ _U8 = C.POINTER(C.c_ubyte)
 class Hessian(C.Structure):
     _fields_ = [("BU", C.c_int), ("BV", C.c_int), ("NU", C.c_int), ("NV", C.c_int), ("nW", C.c_int),
                 ("U", _D), ("V", _D), ("W", _D), ("bU", _D), ("bV", _D), ("Wi", _I), ("Wj", _I),
-                ("chi2", C.c_double)]
+                ("chi2", C.c_double), ("nVo", C.c_int), ("Vo", _D), ("Voi", _I), ("Voj", _I)]
 
 
 UPDATE_CB = C.CFUNCTYPE(None, C.c_void_p, C.c_int)
@@ -76,6 +76,7 @@ def lib():
         sig = {
             "or_hessian_alloc": (C.c_int, [C.POINTER(Hessian), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
             "or_hessian_free": (None, [C.POINTER(Hessian)]),
+            "or_hessian_alloc_voff": (C.c_int, [C.POINTER(Hessian), C.c_int]),
             "or_hessian_max_abs_b": (C.c_double, [C.POINTER(Hessian)]),
             "or_ldlt_factor": (C.c_int, [_D, C.c_int, _I]),
             "or_ldlt_solve": (None, [_D, C.c_int, _I, _D]),
@@ -149,7 +150,7 @@ def hessian_views(h):
     """numpy views of an or_hessian's arrays."""
     BU, BV, NU, NV, nW = h.BU, h.BV, h.NU, h.NV, h.nW
     nu = BU * NU
-    return dict(
+    d = dict(
         U=np.ctypeslib.as_array(h.U, shape=(nu * nu,)).reshape(nu, nu),
         V=np.ctypeslib.as_array(h.V, shape=(max(NV * BV * BV, 1),))[:NV * BV * BV].reshape(NV, BV, BV),
         W=np.ctypeslib.as_array(h.W, shape=(max(nW * BU * BV, 1),))[:nW * BU * BV].reshape(nW, BU, BV),
@@ -158,6 +159,12 @@ def hessian_views(h):
         Wi=np.ctypeslib.as_array(h.Wi, shape=(max(nW, 1),))[:nW],
         Wj=np.ctypeslib.as_array(h.Wj, shape=(max(nW, 1),))[:nW],
     )
+    if h.nVo > 0:  # off-diagonal V blocks (V-V edges)
+        n = h.nVo
+        d["Vo"] = np.ctypeslib.as_array(h.Vo, shape=(n * BV * BV,)).reshape(n, BV, BV)
+        d["Voi"] = np.ctypeslib.as_array(h.Voi, shape=(n,))
+        d["Voj"] = np.ctypeslib.as_array(h.Voj, shape=(n,))
+    return d
 
 
 def ldlt(a):
@@ -332,12 +339,17 @@ class PyProblem:
     generic block Hessian; used for the linear toy problem of
     nelson/tests/testDoubleSectionCommon.h."""
 
-    def __init__(self, BU, BV, NU, NV, Wi, Wj):
+    def __init__(self, BU, BV, NU, NV, Wi, Wj, Vpairs=()):
         self.h = Hessian()
         lib().or_hessian_alloc(C.byref(self.h), BU, BV, NU, NV, len(Wi))
+        if len(Vpairs):  # off-diagonal V blocks (i, j), i < j, of V-V edges
+            lib().or_hessian_alloc_voff(C.byref(self.h), len(Vpairs))
         v = hessian_views(self.h)
         v["Wi"][:] = Wi
         v["Wj"][:] = Wj
+        if len(Vpairs):
+            v["Voi"][:] = [p[0] for p in Vpairs]
+            v["Voj"][:] = [p[1] for p in Vpairs]
         self._cbs = None
 
     def __del__(self):

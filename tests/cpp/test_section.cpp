@@ -3,12 +3,16 @@
 // 2-D points in the first section, 3-D points in the second; unary edges on
 // both, U-U difference edges, U-V edges on the first two coordinates; the
 // measurements are exact for the ground truth, so the optimum has chi2 = 0).
-// The second section is block diagonal (no V-V edges), the configuration of
-// the reference's Schur tests (nelson/tests/testDoubleSectionSchurCommon.hpp).
+// With VV = 0 the second section is block diagonal (no V-V edges), the
+// configuration of the reference's Schur tests
+// (nelson/tests/testDoubleSectionSchurCommon.hpp); with VV = 1 every pair of
+// 3-D points has a V-V difference edge, as in the reference's DoubleSection
+// tests with a non-block-diagonal V (testDoubleSectionCommon.hpp:31-38).
 //
 // Reads the problem from the file given as argv[1]:
-//   N2 N3 UU WALL          (UU: 1 = U-U edges on every pair; WALL: 1 = a U-V
-//                           edge on every pair, 0 = on (i, i) only)
+//   N2 N3 UU WALL VV       (UU: 1 = U-U edges on every pair; WALL: 1 = a U-V
+//                           edge on every pair, 0 = on (i, i) only; VV: 1 =
+//                           V-V edges on every pair)
 //   N2 lines gt2, N3 lines gt3, N2 lines p2, N3 lines p3 (the start)
 // and prints one JSON object per solve: Gauss-Newton and Levenberg-Marquardt
 // with the default settings, LM with the stopping tests off, LM with a
@@ -89,6 +93,31 @@ class EdgeBinaryPoint2d : public EdgeBinaryUU<Section, EdgeBinaryPoint2d<Section
   void updateH12Block(MatRef H) { H.addIdentity(-1.0); }
 };
 
+// (p_i - p_j) - meas between two 3-D points
+template <class Section>
+class EdgeBinaryPoint3d : public EdgeBinaryVV<Section, EdgeBinaryPoint3d<Section>> {
+  double meas_[3], err_[3] = {0, 0, 0};
+
+ public:
+  explicit EdgeBinaryPoint3d(const double* m) : meas_{m[0], m[1], m[2]} {}
+  void update(bool) override {
+    const auto& a = this->parameter_1();
+    const auto& b = this->parameter_2();
+    double c = 0;
+    for (int k = 0; k < 3; k++) err_[k] = (a.p[k] - b.p[k]) - meas_[k], c += err_[k] * err_[k];
+    this->setChi2(c);
+  }
+  void updateH11Block(MatRef H, VecRef v) {
+    H.addIdentity();
+    for (int k = 0; k < 3; k++) v(k) += err_[k];
+  }
+  void updateH22Block(MatRef H, VecRef v) {
+    H.addIdentity();
+    for (int k = 0; k < 3; k++) v(k) -= err_[k];
+  }
+  void updateH12Block(MatRef H) { H.addIdentity(-1.0); }
+};
+
 // (p2_i - p3_j.head(2)) - meas between a 2-D and a 3-D point
 template <class Section>
 class EdgeBinaryPoint2d3d : public EdgeBinaryUV<Section, EdgeBinaryPoint2d3d<Section>> {
@@ -147,7 +176,7 @@ class Points2d3d : public DoubleSection<Points2d3d, Point2d, Point3d, 2, 3> {
 };
 
 struct ToyData {
-  int N2 = 0, N3 = 0, uu = 1, wall = 1;
+  int N2 = 0, N3 = 0, uu = 1, wall = 1, vv = 0;
   std::vector<Point2d> gt2, p2;
   std::vector<Point3d> gt3, p3;
 };
@@ -155,7 +184,7 @@ struct ToyData {
 static ToyData read(const char* path) {
   std::ifstream f(path);
   ToyData d;
-  f >> d.N2 >> d.N3 >> d.uu >> d.wall;
+  f >> d.N2 >> d.N3 >> d.uu >> d.wall >> d.vv;
   d.gt2.resize(d.N2), d.p2.resize(d.N2), d.gt3.resize(d.N3), d.p3.resize(d.N3);
   for (auto& q : d.gt2) f >> q.p[0] >> q.p[1];
   for (auto& q : d.gt3) f >> q.p[0] >> q.p[1] >> q.p[2];
@@ -175,6 +204,13 @@ static void build(Points2d3d& pss, const ToyData& d, int order) {
       for (int j = i + 1; j < d.N2; j++) {
         const double m[2] = {d.gt2[i].p[0] - d.gt2[j].p[0], d.gt2[i].p[1] - d.gt2[j].p[1]};
         pss.addEdge(i, j, new EdgeBinaryPoint2d<Points2d3d>(m));
+      }
+  if (d.vv)
+    for (int i = 0; i < d.N3; i++)
+      for (int j = i + 1; j < d.N3; j++) {
+        const double m[3] = {d.gt3[i].p[0] - d.gt3[j].p[0], d.gt3[i].p[1] - d.gt3[j].p[1],
+                             d.gt3[i].p[2] - d.gt3[j].p[2]};
+        pss.addEdge(i, j, new EdgeBinaryPoint3d<Points2d3d>(m));
       }
   for (int i = 0; i < d.N2; i++)
     for (int j = 0; j < d.N3; j++) {

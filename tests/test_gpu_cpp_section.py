@@ -25,13 +25,16 @@ EXE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "test_sect
 
 class Toy(PyProblem):
     """The oracle's copy of the problem: 2-D points (U), 3-D points (V),
-    unary edges on both, U-U difference edges on every pair (uu), U-V edges
-    on every pair (wall) or on (i, i) only; edges in the reference's order."""
+    unary edges on both, U-U difference edges on every pair (uu), V-V
+    difference edges on every pair (vv: V is then not block diagonal), U-V
+    edges on every pair (wall) or on (i, i) only; edges in the reference's
+    order."""
 
-    def __init__(self, N2, N3, uu, wall, seed, sigma=0.5):
-        self.N2, self.N3, self.uu, self.wall = N2, N3, uu, wall
+    def __init__(self, N2, N3, uu, wall, seed, vv=False, sigma=0.5):
+        self.N2, self.N3, self.uu, self.wall, self.vv = N2, N3, uu, wall, vv
         self.pairs = [(i, j) for i in range(N2) for j in range(N3) if wall or i == j]
-        super().__init__(2, 3, N2, N3, [p[0] for p in self.pairs], [p[1] for p in self.pairs])
+        self.vpairs = [(i, j) for i in range(N3) for j in range(i + 1, N3)] if vv else []
+        super().__init__(2, 3, N2, N3, [p[0] for p in self.pairs], [p[1] for p in self.pairs], self.vpairs)
         rng = np.random.default_rng(seed)
         self.gt2 = rng.uniform(-1, 1, (N2, 2))
         self.gt3 = rng.uniform(-1, 1, (N3, 3))
@@ -40,7 +43,7 @@ class Toy(PyProblem):
 
     def write(self, path):
         with open(path, "w") as f:
-            f.write(f"{self.N2} {self.N3} {int(self.uu)} {int(self.wall)}\n")
+            f.write(f"{self.N2} {self.N3} {int(self.uu)} {int(self.wall)} {int(self.vv)}\n")
             for a in (self.gt2, self.gt3, self.p2, self.p3):
                 for row in a:
                     f.write(" ".join(f"{v:.17g}" for v in row) + "\n")
@@ -49,7 +52,7 @@ class Toy(PyProblem):
         v = self.views()
         chi2 = 0.0
         if hess:
-            for k in ("U", "V", "W", "bU", "bV"):
+            for k in ("U", "V", "W", "bU", "bV") + (("Vo",) if self.vv else ()):
                 v[k][...] = 0
         U, V, W, bU, bV = v["U"], v["V"], v["W"], v["bU"], v["bV"]
         for i in range(self.N2):
@@ -75,6 +78,15 @@ class Toy(PyProblem):
                         U[2 * i:2 * i + 2, 2 * j:2 * j + 2] -= np.eye(2)
                         bU[2 * i:2 * i + 2] += e
                         bU[2 * j:2 * j + 2] -= e
+        for k, (i, j) in enumerate(self.vpairs):
+            e = (self.p3[i] - self.p3[j]) - (self.gt3[i] - self.gt3[j])
+            chi2 += e @ e
+            if hess:
+                V[i] += np.eye(3)
+                V[j] += np.eye(3)
+                v["Vo"][k] -= np.eye(3)
+                bV[3 * i:3 * i + 3] += e
+                bV[3 * j:3 * j + 3] -= e
         for k, (i, j) in enumerate(self.pairs):
             e = (self.p2[i] - self.p3[j][:2]) - (self.gt2[i] - self.gt3[j][:2])
             chi2 += e @ e
@@ -108,11 +120,17 @@ class Toy(PyProblem):
         for k, (i, j) in enumerate(self.pairs):
             H[2 * i:2 * i + 2, nu + 3 * j:nu + 3 * j + 3] = v["W"][k]
             H[nu + 3 * j:nu + 3 * j + 3, 2 * i:2 * i + 2] = v["W"][k].T
+        for k, (i, j) in enumerate(self.vpairs):
+            H[nu + 3 * i:nu + 3 * i + 3, nu + 3 * j:nu + 3 * j + 3] = v["Vo"][k]
+            H[nu + 3 * j:nu + 3 * j + 3, nu + 3 * i:nu + 3 * i + 3] = v["Vo"][k].T
         return H, np.concatenate([v["bU"], v["bV"]])
 
 
-CASES = [(5, 3, True, True, 0), (5, 3, False, True, 1), (5, 3, True, False, 2), (7, 4, True, True, 3),
-         (5, 3, False, False, 4)]
+# (N2, N3, U-U edges, U-V on every pair, seed, V-V edges)
+CASES = [(5, 3, True, True, 0, False), (5, 3, False, True, 1, False), (5, 3, True, False, 2, False),
+         (7, 4, True, True, 3, False), (5, 3, False, False, 4, False),
+         (5, 3, True, True, 5, True), (4, 6, False, True, 6, True), (5, 4, True, False, 7, True),
+         (3, 7, False, False, 8, True)]
 
 
 def _run(toy, tmp_path):
@@ -145,11 +163,11 @@ def _same_logs(a, b, what, exact_length=False):
     assert len(a) == len(b), (what, a, b)
 
 
-@pytest.mark.parametrize("N2,N3,uu,wall,seed", CASES)
-def test_user_section_matches_oracle(N2, N3, uu, wall, seed, tmp_path):
-    res = _run(Toy(N2, N3, uu, wall, seed), tmp_path)
+@pytest.mark.parametrize("N2,N3,uu,wall,seed,vv", CASES)
+def test_user_section_matches_oracle(N2, N3, uu, wall, seed, vv, tmp_path):
+    res = _run(Toy(N2, N3, uu, wall, seed, vv), tmp_path)
     for name, solve in (("gn", "gn_solve"), ("lm", "lm_solve")):
-        t = Toy(N2, N3, uu, wall, seed)
+        t = Toy(N2, N3, uu, wall, seed, vv)
         ro = getattr(t, solve)()
         d = res[name]
         assert d["reason"] == ro["reason"] or {d["reason"], ro["reason"]} <= SUCCESS, (name, d["reason"], ro["reason"])
@@ -159,7 +177,7 @@ def test_user_section_matches_oracle(N2, N3, uu, wall, seed, tmp_path):
         assert np.allclose(d["p3"], t.p3.ravel(), rtol=0, atol=1e-12)
     # stopping tests off: the damping schedule of a linear problem (rho = 1,
     # mu x 1/3 per accepted step, LevenbergMarquardt.hpp:109-112)
-    t = Toy(N2, N3, uu, wall, seed)
+    t = Toy(N2, N3, uu, wall, seed, vv)
     ro = t.lm_solve(maxNumIt=3, epsIncSquare=-1.0, epsChi2=0.0, epsBVector=0.0)
     d = res["lm_nostop"]
     assert d["reason"] == ro["reason"] == "MaxIterationReached"
@@ -169,11 +187,11 @@ def test_user_section_matches_oracle(N2, N3, uu, wall, seed, tmp_path):
         assert abs(b - a / 3.0) <= 1e-12 * a
 
 
-@pytest.mark.parametrize("N2,N3,uu,wall,seed", CASES)
-def test_internal_order_of_u(N2, N3, uu, wall, seed, tmp_path):
+@pytest.mark.parametrize("N2,N3,uu,wall,seed,vv", CASES)
+def test_internal_order_of_u(N2, N3, uu, wall, seed, vv, tmp_path):
     """setUser2InternalIndexesU (reversed) and permuteAMD_SchurU change the
     internal order of bU and of the increment's U part, not the solution."""
-    res = _run(Toy(N2, N3, uu, wall, seed), tmp_path)
+    res = _run(Toy(N2, N3, uu, wall, seed, vv), tmp_path)
     base = res["lm"]
     rev = res["lm_reversed"]
     assert rev["u2i"] == list(range(N2 - 1, -1, -1))
@@ -186,7 +204,7 @@ def test_internal_order_of_u(N2, N3, uu, wall, seed, tmp_path):
         assert np.allclose(d["p3"], base["p3"], rtol=0, atol=1e-12)
     # one damped step in the reversed order against a dense solve of the
     # oracle's Hessian: (H + diag(rel H_ii + abs)) dx = -b
-    t = Toy(N2, N3, uu, wall, seed)
+    t = Toy(N2, N3, uu, wall, seed, vv)
     t.update(True)
     H, b = t.dense()
     x = np.linalg.solve(H + np.diag(0.05 * np.diag(H) + 0.3), -b)

@@ -674,3 +674,61 @@ def test_schur_assemble_is_the_steps_reduction(cfg1):
     s = o.schur_step(0.01, 3.0, want_s=True)
     assert a["ok"] and s["ok"]
     assert np.array_equal(a["S"], s["S"]) and np.array_equal(a["bS"], s["bS"])
+
+
+# ---------------------------------------------------------------------------
+# V with off-diagonal blocks (V-V edges, nelson/DoubleSection.h:498-510):
+# the Schur step with the whole V factorised (SolverCholeskySchur.hpp:97-147)
+# equals a dense solve of the damped full system
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("BU,BV,rel,ab", [(2, 3, 0.0, 0.0), (6, 3, 1e-3, 0.2), (3, 2, 0.5, 0.0)])
+def test_schur_step_with_vv_blocks_equals_dense_solve(BU, BV, rel, ab):
+    rng = np.random.default_rng(BU * 10 + BV)
+    NU, NV = 5, 7
+    wpairs = [(i, j) for i in range(NU) for j in range(NV) if (i + 2 * j) % 3 != 0]
+    vpairs = [(i, j) for i in range(NV) for j in range(i + 1, NV) if (i * j + i) % 4 != 1]
+    p = PyProblem(BU, BV, NU, NV, [a for a, _ in wpairs], [b for _, b in wpairs], vpairs)
+    nu, nv = BU * NU, BV * NV
+    n = nu + nv
+    H = np.zeros((n, n))
+    J = rng.standard_normal((3 * n, n))
+    dense = J.T @ J + n * np.eye(n)
+    mask = np.zeros((n, n), bool)  # the block pattern: U dense, V blocks, W pairs
+    mask[:nu, :nu] = True
+    for q in range(NV):
+        mask[nu + BV * q:nu + BV * q + BV, nu + BV * q:nu + BV * q + BV] = True
+    for i, j in wpairs:
+        mask[BU * i:BU * i + BU, nu + BV * j:nu + BV * j + BV] = True
+    for i, j in vpairs:
+        mask[nu + BV * i:nu + BV * i + BV, nu + BV * j:nu + BV * j + BV] = True
+    mask = mask | mask.T
+    H[mask] = dense[mask]
+    H += 4 * n * np.eye(n)  # diagonally dominant: SPD with every pattern
+    b = rng.standard_normal(n)
+    v = p.views()
+    v["U"][...] = np.triu(H[:nu, :nu])
+    for q in range(NV):
+        v["V"][q] = H[nu + BV * q:nu + BV * q + BV, nu + BV * q:nu + BV * q + BV]
+    for k, (i, j) in enumerate(wpairs):
+        v["W"][k] = H[BU * i:BU * i + BU, nu + BV * j:nu + BV * j + BV]
+    for k, (i, j) in enumerate(vpairs):
+        v["Vo"][k] = H[nu + BV * i:nu + BV * i + BV, nu + BV * j:nu + BV * j + BV]
+    v["bU"][:] = b[:nu]
+    v["bV"][:] = b[nu:]
+    xU, xV = np.zeros(nu), np.zeros(nv)
+    S, bS = np.zeros((nu, nu)), np.zeros(nu)
+    ok = orc.lib().or_schur_step(orc.C.byref(p.h), rel, ab, orc.dp(xU), orc.dp(xV), orc.dp(S), orc.dp(bS))
+    assert ok
+    Hd = H + np.diag(rel * np.diag(H) + ab)
+    x = np.linalg.solve(Hd, -b)
+    assert relerr(np.concatenate([xU, xV]), x) < 1e-12
+    # S and bS are the Schur complement of the damped V block and its reduced
+    # right-hand side
+    A, B, D = Hd[:nu, :nu], Hd[:nu, nu:], Hd[nu:, nu:]
+    Sref = A - B @ np.linalg.solve(D, B.T)
+    assert relerr(np.triu(S), np.triu(Sref)) < 1e-12
+    assert relerr(bS, -b[:nu] + B @ np.linalg.solve(D, b[nu:])) < 1e-12
+    # a V-V block left out changes the step: the blocks are used
+    v["Vo"][0] = 0.0
+    ok = orc.lib().or_schur_step(orc.C.byref(p.h), rel, ab, orc.dp(xU), orc.dp(xV), None, None)
+    assert ok and relerr(np.concatenate([xU, xV]), x) > 1e-6
