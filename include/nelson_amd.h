@@ -284,6 +284,14 @@ int nsn_problem_create_blocks(const nsn_block_desc* d, nsn_problem** out);
 int nsn_import_hessian_blocks(nsn_problem* h, const double* U, const double* Uoff, const double* V,
                               const double* W, const double* bU, const double* bV, double chi2);
 
+/* Test hook, no counterpart in the reference: builds the structure of d
+ * (structureReady: edge orders, W blocks, the S pattern and its (W, W)
+ * pairs, d->camera_order honoured) twice, on the host and on the GPU (the
+ * path nsn_problem_create takes), and sets *n_diff to the number of fields
+ * in which they differ (0: equal; nsn_last_error names the first).
+ * host_ms / device_ms (may be NULL) receive the two build times. */
+int nsn_structure_check(const nsn_problem_desc* d, long long* n_diff, double* host_ms, double* device_ms);
+
 /* ---- SolverCholeskySchur (nelson/SolverCholeskySchur.h:53-117) ---------- */
 int nsn_schur_init(nsn_problem* h);                                  /* init (.hpp:44-62) */
 int nsn_max_abs_h_diag(nsn_problem* h, double* out);                 /* maxAbsHDiag (.hpp:75-78) */

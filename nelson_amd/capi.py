@@ -114,6 +114,7 @@ SIGNATURES = {
     "nsn_visibility_load": (C.c_int, [C.c_char_p, C.c_int, _I, _I, _I, _I, _I]),
     "nsn_bal_load": (C.c_int, [C.c_char_p, C.c_int, _I, _I, _I, _I, _I, _D, _D, _D, _D]),
     "nsn_problem_create": (C.c_int, [C.POINTER(ProblemDesc), C.POINTER(_P)]),
+    "nsn_structure_check": (C.c_int, [C.POINTER(ProblemDesc), C.POINTER(C.c_longlong), _D, _D]),
     "nsn_problem_destroy": (C.c_int, [_P]),
     "nsn_problem_info_get": (C.c_int, [_P, C.POINTER(ProblemInfo)]),
     "nsn_problem_plan": (C.c_int, [C.POINTER(ProblemDesc), C.POINTER(ProblemInfo), _I]),

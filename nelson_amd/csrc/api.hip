@@ -17,6 +17,7 @@
 #include "kernels.h"
 #include "nelson_amd.h"
 #include "structure.h"
+#include "structure_dev.h"
 
 using namespace nsn;
 
@@ -382,7 +383,7 @@ static int tile_chain_steps(const TileProfile& prof) {
 int build_structure(nsn_problem* h, const nsn_problem_desc* d) {
   const int C = h->C, P = h->P, E = h->E;
   HostStructure hs;
-  int rc0 = host_structure(d, hs, true);
+  int rc0 = device_structure(d, hs, true, h->st);
   if (rc0) return rc0;
   h->cam_var = hs.cam_var;
   h->pt_var = hs.pt_var;
@@ -645,6 +646,35 @@ extern "C" int nsn_device_count(int* n) {
   hipError_t e = hipGetDeviceCount(&c);
   if (e != hipSuccess) { *n = 0; return fail(NSN_EHIP, "hipGetDeviceCount: %s", hipGetErrorString(e)); }
   *n = c;
+  return NSN_OK;
+}
+
+extern "C" int nsn_structure_check(const nsn_problem_desc* d, long long* n_diff, double* host_ms,
+                                   double* device_ms) {
+  TraceRange trace_range("nsn_structure_check");
+  clear_error();
+  if (!d || !n_diff) return fail(NSN_EINVAL, "nsn_structure_check: null argument");
+  if (d->n_cameras < 1 || d->n_points < 1 || d->n_edges < 0 || (d->n_edges > 0 && (!d->cam_idx || !d->pt_idx || !d->obs)))
+    return fail(NSN_EINVAL, "nsn_structure_check: bad sizes or missing edge arrays");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(NSN_EHIP, "nsn_structure_check: no HIP device");
+  if (d->device < 0 || d->device >= ndev) return fail(NSN_EINVAL, "nsn_structure_check: bad device %d", d->device);
+  HIP_TRY(hipSetDevice(d->device));
+  hipStream_t st = nullptr;
+  HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  HostStructure a, b;
+  auto t0 = std::chrono::steady_clock::now();
+  int rc = host_structure(d, a, true);
+  auto t1 = std::chrono::steady_clock::now();
+  if (!rc) rc = device_structure(d, b, true, st);
+  auto t2 = std::chrono::steady_clock::now();
+  (void)hipStreamDestroy(st);
+  if (rc) return rc;
+  const char* what = nullptr;
+  *n_diff = structure_diff(a, b, &what);
+  if (host_ms) *host_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  if (device_ms) *device_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
+  if (*n_diff) fail(NSN_OK, "device structure differs from the host's in %lld fields, first: %s", *n_diff, what);
   return NSN_OK;
 }
 

@@ -165,7 +165,7 @@ std::vector<int> nd_order(const std::vector<std::vector<int>>& adj, int align, i
   return out;
 }
 
-int host_structure(const nsn_problem_desc* d, HostStructure& s, bool allow_order) {
+int number_variables(const nsn_problem_desc* d, HostStructure& s) {
   const int C = d->n_cameras, P = d->n_points, E = d->n_edges;
   s.C = C;
   s.P = P;
@@ -186,14 +186,64 @@ int host_structure(const nsn_problem_desc* d, HostStructure& s, bool allow_order
     }
   s.NU = (int)s.var_cam_user.size();
   s.NV = (int)s.var_pt_user.size();
-  const int NU = s.NU, NV = s.NV;
+  const int NU = s.NU;
   for (int e = 0; e < E; e++) {
     if (d->cam_idx[e] < 0 || d->cam_idx[e] >= C || d->pt_idx[e] < 0 || d->pt_idx[e] >= P)
       return fail(NSN_EINVAL, "edge %d references camera %d / point %d out of range", e, d->cam_idx[e], d->pt_idx[e]);
   }
   s.var_cam_api.resize(NU);
   for (int u = 0; u < NU; u++) s.var_cam_api[u] = u;
-  if (allow_order && (d->camera_order == NSN_ORDER_RCM || d->camera_order == NSN_ORDER_ND) && NU > 2) {
+  if (d->camera_order != NSN_ORDER_AS_GIVEN && d->camera_order != NSN_ORDER_RCM && d->camera_order != NSN_ORDER_ND)
+    return fail(NSN_EINVAL, "unknown camera_order %d", d->camera_order);
+  return NSN_OK;
+}
+
+bool wants_camera_order(const nsn_problem_desc* d, const HostStructure& s) {
+  return (d->camera_order == NSN_ORDER_RCM || d->camera_order == NSN_ORDER_ND) && s.NU > 2;
+}
+
+void apply_camera_order(const nsn_problem_desc* d, HostStructure& s, const std::vector<std::vector<int>>& adj) {
+  // solver position -> interface index; nested dissection in parts of
+  // multiples of 32 cameras (192 rows: three 64-row tiles)
+  const std::vector<int> ord = d->camera_order == NSN_ORDER_ND ? nd_order(adj, kNdAlign, kNdDepth) : rcm_order(adj);
+  const std::vector<int> api_user = s.var_cam_user;
+  for (int v = 0; v < s.NU; v++) {
+    s.var_cam_api[v] = ord[v];
+    s.var_cam_user[v] = api_user[ord[v]];
+    s.cam_var[api_user[ord[v]]] = v;
+  }
+}
+
+std::vector<std::vector<int>> camera_graph_from_blocks(const HostStructure& s) {
+  // the off-diagonal S blocks (i, j), i < j, in row-major order: adj[x]
+  // receives its lower neighbours in ascending order before its upper ones
+  std::vector<std::vector<int>> adj(s.NU);
+  for (int b = 0; b < s.nsb; b++)
+    if (s.sb_i[b] != s.sb_j[b]) {
+      adj[s.sb_i[b]].push_back(s.sb_j[b]);
+      adj[s.sb_j[b]].push_back(s.sb_i[b]);
+    }
+  return adj;
+}
+
+void w_ref_order(HostStructure& s) {
+  // reference order of W blocks: row-major (camera, point), cameras in
+  // interface order
+  std::vector<int> api_var(s.NU);
+  for (int u = 0; u < s.NU; u++) api_var[s.var_cam_api[u]] = u;
+  s.w_ref.clear();
+  s.w_ref.reserve(s.nW);
+  for (int a = 0; a < s.NU; a++) {
+    const int u = api_var[a];
+    s.w_ref.insert(s.w_ref.end(), s.cw_idx.begin() + s.cw_ptr[u], s.cw_idx.begin() + s.cw_ptr[u + 1]);
+  }
+}
+
+int host_structure(const nsn_problem_desc* d, HostStructure& s, bool allow_order) {
+  const int rc = number_variables(d, s);
+  if (rc) return rc;
+  const int P = d->n_points, E = d->n_edges, NU = s.NU, NV = s.NV;
+  if (allow_order && wants_camera_order(d, s)) {
     // the camera graph: variable cameras sharing a variable point
     std::vector<std::vector<int>> cams_of(P), adj(NU);
     for (int e = 0; e < E; e++) {
@@ -214,19 +264,7 @@ int host_structure(const nsn_problem_desc* d, HostStructure& s, bool allow_order
       std::sort(l.begin(), l.end());
       l.erase(std::unique(l.begin(), l.end()), l.end());
     }
-    // solver position -> interface index; nested dissection in parts of
-    // multiples of 32 cameras (192 rows: three 64-row tiles)
-    const std::vector<int> ord =
-        d->camera_order == NSN_ORDER_ND ? nd_order(adj, kNdAlign, kNdDepth) : rcm_order(adj);
-    const std::vector<int> api_user = s.var_cam_user;
-    for (int v = 0; v < NU; v++) {
-      s.var_cam_api[v] = ord[v];
-      s.var_cam_user[v] = api_user[ord[v]];
-      s.cam_var[api_user[ord[v]]] = v;
-    }
-  } else if (d->camera_order != NSN_ORDER_AS_GIVEN && d->camera_order != NSN_ORDER_RCM &&
-             d->camera_order != NSN_ORDER_ND) {
-    return fail(NSN_EINVAL, "unknown camera_order %d", d->camera_order);
+    apply_camera_order(d, s, adj);
   }
   // point-major order: (point, camera, edge index)
   s.pe.resize(E);
@@ -303,18 +341,7 @@ int host_structure(const nsn_problem_desc* d, HostStructure& s, bool allow_order
     std::vector<int> fill(s.cw_ptr.begin(), s.cw_ptr.end() - 1);
     for (int k = 0; k < nW; k++) s.cw_idx[fill[s.w_cam[k]]++] = k;
   }
-  // reference order of W blocks: row-major (camera, point), cameras in
-  // interface order
-  {
-    std::vector<int> api_var(NU);
-    for (int u = 0; u < NU; u++) api_var[s.var_cam_api[u]] = u;
-    s.w_ref.clear();
-    s.w_ref.reserve(nW);
-    for (int a = 0; a < NU; a++) {
-      const int u = api_var[a];
-      s.w_ref.insert(s.w_ref.end(), s.cw_idx.begin() + s.cw_ptr[u], s.cw_idx.begin() + s.cw_ptr[u + 1]);
-    }
-  }
+  w_ref_order(s);
   // S structure: pairs of blocks sharing a point, grouped by camera pair
   // (MatrixWWtMultiplier::prepare, nelson/MatrixWWtMultiplier.hpp:18-120)
   struct PairRec {

@@ -43,6 +43,16 @@ struct HostStructure {
 // d->camera_order (otherwise the variable cameras keep camera order)
 int host_structure(const nsn_problem_desc* d, HostStructure& s, bool allow_order = false);
 
+// pieces shared by the two: variable numbering (cam_var, pt_var, their
+// inverses, var_cam_api = identity) with the edge range check; whether the
+// camera order applies; applying it on a camera graph (sorted adjacency
+// lists); the camera graph of the S blocks; the reference order of W
+int number_variables(const nsn_problem_desc* d, HostStructure& s);
+bool wants_camera_order(const nsn_problem_desc* d, const HostStructure& s);
+void apply_camera_order(const nsn_problem_desc* d, HostStructure& s, const std::vector<std::vector<int>>& adj);
+std::vector<std::vector<int>> camera_graph_from_blocks(const HostStructure& s);
+void w_ref_order(HostStructure& s);
+
 // reverse Cuthill-McKee order of a graph given by sorted adjacency lists:
 // order[k] = the node placed k-th (each connected component from a
 // pseudo-peripheral node of least degree, neighbours by increasing degree)

@@ -254,6 +254,40 @@ class BAProblem:
         check(lib.nsn_problem_plan(C.byref(d), C.byref(info), _ip(order)))
         return {k: getattr(info, k) for k, _ in capi.ProblemInfo._fields_}, order
 
+    @staticmethod
+    def structure_check(g, camera_order="given", cam_fixed=None, pt_fixed=None, weight=None, device=0):
+        """The structure stage of a generated problem built on the host and on
+        the GPU (nsn_structure_check): (number of differing fields, host ms,
+        device ms)."""
+        lib = capi.lib()
+        C_, P_ = int(g["n_cameras"]), int(g["n_points"])
+        cam_idx = np.ascontiguousarray(g["cam_idx"], np.int32)
+        pt_idx = np.ascontiguousarray(g["pt_idx"], np.int32)
+        obs = np.ascontiguousarray(g["obs"], np.float64).reshape(-1)
+        if cam_fixed is None:
+            cam_fixed = np.zeros(C_, np.uint8)
+            cam_fixed[0] = 1
+        if pt_fixed is None:
+            pt_fixed = np.zeros(P_, np.uint8)
+            pt_fixed[0] = 1
+        cam_fixed = np.ascontiguousarray(cam_fixed, np.uint8)
+        pt_fixed = np.ascontiguousarray(pt_fixed, np.uint8)
+        weight = None if weight is None else np.ascontiguousarray(weight, np.float64)
+        orders = {"given": capi.NSN_ORDER_AS_GIVEN, "rcm": capi.NSN_ORDER_RCM, "nd": capi.NSN_ORDER_ND}
+        d = capi.ProblemDesc()
+        d.n_cameras, d.n_points, d.n_edges = C_, P_, len(cam_idx)
+        d.cam_idx, d.pt_idx, d.obs = _ip(cam_idx), _ip(pt_idx), _dp(obs)
+        d.weight = _dp(weight)
+        d.cam_fixed = cam_fixed.ctypes.data_as(C.POINTER(C.c_ubyte))
+        d.pt_fixed = pt_fixed.ctypes.data_as(C.POINTER(C.c_ubyte))
+        d.focal, d.cx, d.cy = g["focal"], g["cx"], g["cy"]
+        d.camera_order = orders[camera_order]
+        d.device = device
+        nd = C.c_longlong(-1)
+        th, td = C.c_double(0), C.c_double(0)
+        check(lib.nsn_structure_check(C.byref(d), C.byref(nd), C.byref(th), C.byref(td)))
+        return nd.value, th.value, td.value
+
     @classmethod
     def from_generated(cls, g, **kw):
         p = cls(g["n_cameras"], g["n_points"], g["cam_idx"], g["pt_idx"], g["obs"], g["focal"], g["cx"],
