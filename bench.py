@@ -49,7 +49,9 @@ def lm_settings(nsn, iters):
 
 
 def run_lm(nsn, g, precision, iters, warmup, refine=None, camera_order="given"):
+    c0 = time.perf_counter()
     p = nsn.BAProblem.from_generated(g, precision=precision, camera_order=camera_order)
+    create_s = time.perf_counter() - c0
     if refine is not None:
         p.set_refinement(refine)
     if warmup > 0:
@@ -64,7 +66,8 @@ def run_lm(nsn, g, precision, iters, warmup, refine=None, camera_order="given"):
     out = dict(reason=reason, iterations=r["iterations"], sub_iterations=r["sub_iterations"],
                final_chi2=r["final_chi2"], wall_s=t1 - t0, device_loop_ms=r["elapsed_ms"],
                it_per_s=r["iterations"] / (t1 - t0), ms_per_solve=1e3 * (t1 - t0) / max(r["sub_iterations"], 1),
-               log=[(e[0], e[1], e[2], e[3]) for e in lm.stats()])
+               log=[(e[0], e[1], e[2], e[3]) for e in lm.stats()],
+               create_ms=1e3 * create_s)
     if precision == "mixed":
         out["refinement_solves_per_increment"] = (p.refinement_count() - c0) / max(r["sub_iterations"], 1)
     return p, out
