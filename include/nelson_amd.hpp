@@ -171,7 +171,11 @@ class HessianVectors {
  public:
   class Part {
    public:
-    ConstSeg segment(int i) const { return ConstSeg(v_.data() + (size_t)B_ * i, B_); }
+    // block i (B_ values each, or the sizes of a section with per-parameter
+    // sizes, off_[i] .. off_[i + 1])
+    ConstSeg segment(int i) const {
+      return off_.empty() ? ConstSeg(v_.data() + (size_t)B_ * i, B_) : ConstSeg(v_.data() + off_[i], off_[i + 1] - off_[i]);
+    }
     int blockSize() const { return B_; }
     const std::vector<double>& values() const { return v_; }
     double operator[](size_t i) const { return v_[i]; }
@@ -182,6 +186,7 @@ class HessianVectors {
     friend class SolverCholeskySchur;
     int B_ = 0;
     std::vector<double> v_;
+    std::vector<int> off_;
   };
   const Part& bU() const { return u_; }
   const Part& bV() const { return v_; }
@@ -217,6 +222,16 @@ class SchurSystem {
   // parameters and the U parameters after them (a V with off-diagonal
   // blocks, factorised together with S); false: separate U and V parts
   virtual bool vFirstInU() const { return false; }
+  // the size of variable parameter u / q (at most blockU() / blockV())
+  virtual int sizeU(int u) const {
+    (void)u;
+    return blockU();
+  }
+  virtual int sizeV(int q) const {
+    (void)q;
+    return blockV();
+  }
+  virtual bool variableSizes() const { return false; }
 };
 
 // ---------------------------------------------------------------------------
@@ -683,14 +698,24 @@ class DoubleSectionHessian : public SchurSystem {
   int numUoff() const { return (int)uoI_.size(); }
   int numW() const { return (int)wU_.size(); }
   // block access (internal order)
-  MatRef Ublock(int i) { return MatRef(&U_[(size_t)BU * BU * i], BU, BU, BU); }
-  MatRef Uoffblock(int uid, bool trans) { return MatRef(&Uo_[(size_t)BU * BU * uid], BU, BU, BU, trans); }
-  MatRef Vblock(int j) { return MatRef(&V_[(size_t)BV * BV * j], BV, BV, BV); }
-  MatRef Wblock(int uid) { return MatRef(&W_[(size_t)BU * BV * uid], BU, BV, BV); }
-  MatRef Voffblock(int uid, bool trans) { return MatRef(&Vo_[(size_t)BV * BV * uid], BV, BV, BV, trans); }
+  MatRef Ublock(int i) { return MatRef(&U_[(size_t)BU * BU * i], su(i), su(i), BU); }
+  MatRef Uoffblock(int uid, bool trans) {
+    return MatRef(&Uo_[(size_t)BU * BU * uid], su(uoI_[uid]), su(uoJ_[uid]), BU, trans);
+  }
+  MatRef Vblock(int j) { return MatRef(&V_[(size_t)BV * BV * j], sv(j), sv(j), BV); }
+  MatRef Wblock(int uid) { return MatRef(&W_[(size_t)BU * BV * uid], su(wU_[uid]), sv(wV_[uid]), BV); }
+  MatRef Voffblock(int uid, bool trans) {
+    return MatRef(&Vo_[(size_t)BV * BV * uid], sv(voI_[uid]), sv(voJ_[uid]), BV, trans);
+  }
   int numVoff() const { return (int)voI_.size(); }
-  VecRef bUseg(int i) { return VecRef(&bU_[(size_t)BU * i], BU); }
-  VecRef bVseg(int j) { return VecRef(&bV_[(size_t)BV * j], BV); }
+  VecRef bUseg(int i) { return VecRef(&bU_[(size_t)BU * i], su(i)); }
+  VecRef bVseg(int j) { return VecRef(&bV_[(size_t)BV * j], sv(j)); }
+  // per-parameter sizes (internal order; the block size when not given)
+  int su(int u) const { return suI_.empty() ? BU : suI_[u]; }
+  int sv(int q) const { return svI_.empty() ? BV : svI_[q]; }
+  int sizeU(int u) const override { return su(u); }
+  int sizeV(int q) const override { return sv(q); }
+  bool variableSizes() const override { return !suI_.empty() || !svI_.empty(); }
   const std::vector<double>& bUvalues() const { return bU_; }
   const std::vector<double>& bVvalues() const { return bV_; }
 
@@ -710,19 +735,21 @@ class DoubleSectionHessian : public SchurSystem {
     std::vector<double> U(36 * (size_t)NU, 0.0), Uo(36 * (size_t)nO, 0.0), V(9 * (size_t)NV, 0.0),
         W(18 * (size_t)nW, 0.0), bU(6 * (size_t)NU, 0.0), bV(3 * (size_t)NV, 0.0);
     for (int u = 0; u < NU; u++) {
+      const int n = su(u);
       for (int a = 0; a < 6; a++)
         for (int c = 0; c < 6; c++)
-          U[36 * (size_t)u + 6 * a + c] = (a < BU && c < BU) ? U_[(size_t)BU * BU * u + BU * a + c] : (a == c ? 1.0 : 0.0);
-      for (int a = 0; a < BU; a++) bU[6 * (size_t)u + a] = bU_[(size_t)BU * u + a];
+          U[36 * (size_t)u + 6 * a + c] = (a < n && c < n) ? U_[(size_t)BU * BU * u + BU * a + c] : (a == c ? 1.0 : 0.0);
+      for (int a = 0; a < n; a++) bU[6 * (size_t)u + a] = bU_[(size_t)BU * u + a];
     }
     for (int k = 0; k < nO; k++)
       for (int a = 0; a < BU; a++)
         for (int c = 0; c < BU; c++) Uo[36 * (size_t)k + 6 * a + c] = Uo_[(size_t)BU * BU * k + BU * a + c];
     for (int q = 0; q < NV; q++) {
+      const int n = sv(q);
       for (int a = 0; a < 3; a++)
         for (int c = 0; c < 3; c++)
-          V[9 * (size_t)q + 3 * a + c] = (a < BV && c < BV) ? V_[(size_t)BV * BV * q + BV * a + c] : (a == c ? 1.0 : 0.0);
-      for (int a = 0; a < BV; a++) bV[3 * (size_t)q + a] = bV_[(size_t)BV * q + a];
+          V[9 * (size_t)q + 3 * a + c] = (a < n && c < n) ? V_[(size_t)BV * BV * q + BV * a + c] : (a == c ? 1.0 : 0.0);
+      for (int a = 0; a < n; a++) bV[3 * (size_t)q + a] = bV_[(size_t)BV * q + a];
     }
     for (int k = 0; k < nW; k++)
       for (int a = 0; a < BU; a++)
@@ -739,9 +766,9 @@ class DoubleSectionHessian : public SchurSystem {
   double maxAbsHDiag() const override {
     double m = 0;
     for (int u = 0; u < numU(); u++)
-      for (int a = 0; a < BU; a++) m = std::max(m, std::fabs(U_[(size_t)BU * BU * u + (BU + 1) * a]));
+      for (int a = 0; a < su(u); a++) m = std::max(m, std::fabs(U_[(size_t)BU * BU * u + (BU + 1) * a]));
     for (int q = 0; q < numV(); q++)
-      for (int a = 0; a < BV; a++) m = std::max(m, std::fabs(V_[(size_t)BV * BV * q + (BV + 1) * a]));
+      for (int a = 0; a < sv(q); a++) m = std::max(m, std::fabs(V_[(size_t)BV * BV * q + (BV + 1) * a]));
     return m;
   }
   int blockU() const override { return BU; }
@@ -754,10 +781,12 @@ class DoubleSectionHessian : public SchurSystem {
   // section internals
   void setup_(const void* owner, int nU, int nV, std::vector<std::pair<int, int>> wPairs,
               std::vector<std::pair<int, int>> uoPairs, std::vector<std::pair<int, int>> voPairs, Precision p,
-              int device) {
+              int device, std::vector<int> suI = {}, std::vector<int> svI = {}) {
     owner_ = owner;
     nU_ = nU;
     nV_ = nV;
+    suI_ = std::move(suI);
+    svI_ = std::move(svI);
     for (auto& w : wPairs) wU_.push_back(w.first), wV_.push_back(w.second);
     for (auto& o : uoPairs) uoI_.push_back(o.first), uoJ_.push_back(o.second);
     for (auto& o : voPairs) voI_.push_back(o.first), voJ_.push_back(o.second);
@@ -827,17 +856,19 @@ class DoubleSectionHessian : public SchurSystem {
     const int NU = numU(), NV = numV(), ND = NU + NV;
     std::vector<double> U(36 * (size_t)ND, 0.0), Uo(36 * mp_.size() + 1, 0.0), bU(6 * (size_t)ND, 0.0);
     for (int q = 0; q < NV; q++) {
+      const int n = sv(q);
       for (int a = 0; a < 6; a++)
         for (int c = 0; c < 6; c++)
-          U[36 * (size_t)q + 6 * a + c] = (a < BV && c < BV) ? V_[(size_t)BV * BV * q + BV * a + c] : (a == c ? 1.0 : 0.0);
-      for (int a = 0; a < BV; a++) bU[6 * (size_t)q + a] = bV_[(size_t)BV * q + a];
+          U[36 * (size_t)q + 6 * a + c] = (a < n && c < n) ? V_[(size_t)BV * BV * q + BV * a + c] : (a == c ? 1.0 : 0.0);
+      for (int a = 0; a < n; a++) bU[6 * (size_t)q + a] = bV_[(size_t)BV * q + a];
     }
     for (int u = 0; u < NU; u++) {
+      const int n = su(u);
       double* dst = &U[36 * (size_t)(NV + u)];
       for (int a = 0; a < 6; a++)
         for (int c = 0; c < 6; c++)
-          dst[6 * a + c] = (a < BU && c < BU) ? U_[(size_t)BU * BU * u + BU * a + c] : (a == c ? 1.0 : 0.0);
-      for (int a = 0; a < BU; a++) bU[6 * (size_t)(NV + u) + a] = bU_[(size_t)BU * u + a];
+          dst[6 * a + c] = (a < n && c < n) ? U_[(size_t)BU * BU * u + BU * a + c] : (a == c ? 1.0 : 0.0);
+      for (int a = 0; a < n; a++) bU[6 * (size_t)(NV + u) + a] = bU_[(size_t)BU * u + a];
     }
     for (size_t m = 0; m < mp_.size(); m++) {
       double* dst = &Uo[36 * m];
@@ -865,6 +896,7 @@ class DoubleSectionHessian : public SchurSystem {
   std::vector<double> U_, Uo_, V_, Vo_, W_, bU_, bV_;
   bool merged_ = false;
   std::vector<MergedBlock> mp_;
+  std::vector<int> suI_, svI_;  // per-parameter sizes, internal order (empty: the block size)
   double chi2_ = 0.0;
   mutable bool dirty_ = true;
   nsn_problem* h_ = nullptr;
@@ -900,6 +932,12 @@ class DoubleSection {
   // numbers of VARIABLE parameters (nelson/DoubleSection.h:40-166)
   virtual int numParametersU() const { return nU_; }
   virtual int numParametersV() const { return nV_; }
+  // per-parameter sizes, user order (the reference's mat::Variable block
+  // sizes, BaseNumSizeParametersU<mat::Variable, ...>::parameterUSize,
+  // nelson/DoubleSection.h:76-110): each at most BU / BV; empty (the
+  // default): every parameter has the block size BU / BV
+  virtual const std::vector<int>& parameterUSize() const { return noSizes_; }
+  virtual const std::vector<int>& parameterVSize() const { return noSizes_; }
 
   void parametersReady() {
     if (structured_) throw Error(NSN_ESTATE, "parametersReady after structureReady");
@@ -1052,7 +1090,26 @@ class DoubleSection {
           (int)(std::lower_bound(o.begin(), o.end(), std::make_pair(std::min(a, b), std::max(a, b))) - o.begin()));
       ed.pairTrans_.push_back(a > b ? 1 : 0);
     }
-    H_.setup_(this, numParametersU(), numParametersV(), w, o, vo, p, device);
+    std::vector<int> suI, svI;
+    {
+      const std::vector<int>& su = parameterUSize();
+      const std::vector<int>& sv = parameterVSize();
+      if (!su.empty()) {
+        if ((int)su.size() != numParametersU()) throw Error(NSN_EINVAL, "parameterUSize: one size per U parameter");
+        suI.assign(su.size(), 0);
+        for (size_t i = 0; i < su.size(); i++) {
+          if (su[i] < 1 || su[i] > BU) throw Error(NSN_EINVAL, "parameterUSize: sizes from 1 to the block size");
+          suI[u2i_[i]] = su[i];
+        }
+      }
+      if (!sv.empty()) {
+        if ((int)sv.size() != numParametersV()) throw Error(NSN_EINVAL, "parameterVSize: one size per V parameter");
+        for (int x : sv)
+          if (x < 1 || x > BV) throw Error(NSN_EINVAL, "parameterVSize: sizes from 1 to the block size");
+        svI = sv;
+      }
+    }
+    H_.setup_(this, numParametersU(), numParametersV(), w, o, vo, p, device, suI, svI);
     structured_ = true;
   }
 
@@ -1078,11 +1135,17 @@ class DoubleSection {
 
   // nelson/DoubleSection.h:300-311, with b of the last update(true)
   double computeRhoChi2Change(double mu, const HessianVectors& inc, double oldChi2) const {
-    const auto& bu = H_.bUvalues();
+    const auto& bu = H_.bUvalues();  // BU / BV values per parameter, its size used
     const auto& bv = H_.bVvalues();
     double du = 0, dv = 0;
-    for (size_t i = 0; i < bu.size(); i++) du += inc.bU()[i] * (mu * inc.bU()[i] - bu[i]);
-    for (size_t i = 0; i < bv.size(); i++) dv += inc.bV()[i] * (mu * inc.bV()[i] - bv[i]);
+    for (int u = 0; u < H_.numU(); u++) {
+      const ConstSeg x = inc.bU().segment(u);
+      for (int a = 0; a < x.size(); a++) du += x(a) * (mu * x(a) - bu[(size_t)BU * u + a]);
+    }
+    for (int q = 0; q < H_.numV(); q++) {
+      const ConstSeg x = inc.bV().segment(q);
+      for (int a = 0; a < x.size(); a++) dv += x(a) * (mu * x(a) - bv[(size_t)BV * q + a]);
+    }
     return (oldChi2 - H_.chi2()) / (du + dv);
   }
 
@@ -1127,6 +1190,7 @@ class DoubleSection {
   std::vector<std::pair<size_t, std::pair<int, int>>> vv_;       // (edge, (V id, V id))
   std::vector<std::pair<size_t, std::pair<int, int>>> nary_;    // (edge, variable pair), in pair order
   HessianT H_;
+  const std::vector<int> noSizes_;
 };
 
 // ---------------------------------------------------------------------------
@@ -1226,12 +1290,25 @@ class SolverCholeskySchur {
     inc_.origin = H.owner();
     inc_.u_.B_ = BU;
     inc_.v_.B_ = BV;
-    inc_.u_.v_.resize((size_t)BU * NU);
-    inc_.v_.v_.resize((size_t)BV * NV);
-    for (int u = 0; u < NU; u++)
-      for (int a = 0; a < BU; a++) inc_.u_.v_[(size_t)BU * u + a] = du[6 * (size_t)(offU + u) + a];
-    for (int q = 0; q < NV; q++)
-      for (int a = 0; a < BV; a++) inc_.v_.v_[(size_t)BV * q + a] = vu ? du[6 * (size_t)q + a] : dv[3 * (size_t)q + a];
+    // per-parameter sizes: block u at off_[u] (compact, as the reference's
+    // vector of a mat::Variable section); otherwise B values per block
+    const bool var = H.variableSizes();
+    inc_.u_.off_.assign(var ? NU + 1 : 0, 0);
+    inc_.v_.off_.assign(var ? NV + 1 : 0, 0);
+    if (var) {
+      for (int u = 0; u < NU; u++) inc_.u_.off_[u + 1] = inc_.u_.off_[u] + H.sizeU(u);
+      for (int q = 0; q < NV; q++) inc_.v_.off_[q + 1] = inc_.v_.off_[q] + H.sizeV(q);
+    }
+    inc_.u_.v_.resize(var ? (size_t)inc_.u_.off_[NU] : (size_t)BU * NU);
+    inc_.v_.v_.resize(var ? (size_t)inc_.v_.off_[NV] : (size_t)BV * NV);
+    for (int u = 0; u < NU; u++) {
+      const size_t o = var ? (size_t)inc_.u_.off_[u] : (size_t)BU * u;
+      for (int a = 0; a < H.sizeU(u); a++) inc_.u_.v_[o + a] = du[6 * (size_t)(offU + u) + a];
+    }
+    for (int q = 0; q < NV; q++) {
+      const size_t o = var ? (size_t)inc_.v_.off_[q] : (size_t)BV * q;
+      for (int a = 0; a < H.sizeV(q); a++) inc_.v_.v_[o + a] = vu ? du[6 * (size_t)q + a] : dv[3 * (size_t)q + a];
+    }
     // the padding's increment is exactly zero, so the norm is the device's
     sq_ = sq;
     return rc == NSN_OK;
