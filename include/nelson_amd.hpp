@@ -1226,6 +1226,10 @@ class SingleSection : public DoubleSection<Derived, Par, detail::NoParameter, B,
 
   // the user implements parameter(NodeId) (both constnesses)
   virtual int numParameters() const { return this->sizeU_(); }
+  // per-parameter sizes (mat::Variable, nelson/SingleSection.h:79-110): one
+  // size per parameter, each at most B; empty: every parameter has size B
+  virtual const std::vector<int>& parameterSize() const { return noSizes_; }
+  const std::vector<int>& parameterUSize() const override { return parameterSize(); }
   int numParametersU() const override { return static_cast<const Derived&>(*this).numParameters(); }
   int numParametersV() const override { return 0; }
   Par& parameterU(NodeId i) { return static_cast<Derived&>(*this).parameter(i); }
@@ -1235,6 +1239,9 @@ class SingleSection : public DoubleSection<Derived, Par, detail::NoParameter, B,
   void setUser2InternalIndexes(const std::vector<int>& u2i) { Base::setUser2InternalIndexesU(u2i); }
   void permuteAMD() { Base::permuteAMD_SchurU(); }
   const std::vector<int>& user2internalIndexes() const { return Base::user2internalIndexesU(); }
+
+ private:
+  const std::vector<int> noSizes_;
 };
 
 // ---------------------------------------------------------------------------

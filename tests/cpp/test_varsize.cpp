@@ -142,6 +142,35 @@ class EdgeVV : public Diff<EdgeBinaryVV<VarSection, EdgeVV>> {
   using Diff::Diff;
 };
 
+// the U parameters alone as a single section (nelson/SingleSection.h with
+// mat::Variable sizes: parameterSize()), unary and U-U edges
+class VarSingle : public SingleSection<VarSingle, Par, 3> {
+ public:
+  explicit VarSingle(const Data& d) : SingleSection(d.NU), d_(d), u_(d.pU) {}
+  const std::vector<int>& parameterSize() const override { return d_.su; }
+  Par& parameter(NodeId i) { return i.isVariable() ? u_.at(i.id()) : fixed_; }
+  const Par& parameter(NodeId i) const { return i.isVariable() ? u_.at(i.id()) : fixed_; }
+  void oplus(const HessianVectors& inc) {
+    for (int i = 0; i < d_.NU; i++) {
+      const ConstSeg x = inc.segment(user2internalIndexes()[i]);
+      for (int k = 0; k < x.size(); k++) u_[i].p[k] += x(k);
+    }
+  }
+  void backupSolution() { b_ = u_; }
+  void rollbackSolution() { u_ = b_; }
+  const std::vector<Par>& u() const { return u_; }
+
+ private:
+  const Data& d_;
+  std::vector<Par> u_, b_;
+  Par fixed_;
+};
+
+class SingleUU : public Diff<EdgeBinaryUU<VarSingle, SingleUU>> {
+ public:
+  using Diff::Diff;
+};
+
 static Data read(const char* path) {
   std::ifstream f(path);
   Data d;
@@ -244,6 +273,27 @@ int main(int argc, char** argv) {
         for (int k = 0; k < x.size(); k++) std::printf("%s%.17g", first ? "" : ", ", x(k)), first = false;
       }
       std::printf("], \"u_size\": %zu, \"v_size\": %zu}\n", inc.bU().size(), inc.bV().size());
+    }
+    {
+      // the single section: LM from the start, then its final state
+      VarSingle s(d);
+      s.parametersReady();
+      for (int i = 0; i < d.NU; i++) s.addEdge(i, new UnaryU<VarSingle>(d.gtU[i], d.su[i]));
+      double m3[3];
+      for (int i = 0; i + 1 < d.NU; i++) {
+        const int m = std::min(d.su[i], d.su[i + 1]);
+        for (int k = 0; k < 3; k++) m3[k] = k < m ? d.gtU[i].p[k] - d.gtU[i + 1].p[k] : 0.0;
+        s.addEdge(i, i + 1, new SingleUU(m3, m));
+      }
+      s.structureReady();
+      LevenbergMarquardt<SolverCholeskySchur> lm;
+      const TerminationReason tc = lm.solve(s);
+      std::printf("{\"solve\": \"single_lm\", \"reason\": \"%s\", \"final_chi2\": %.17g, \"x\": [",
+                  toString(tc).c_str(), s.hessian().chi2());
+      bool first = true;
+      for (int i = 0; i < d.NU; i++)
+        for (int k = 0; k < d.su[i]; k++) std::printf("%s%.17g", first ? "" : ", ", s.u()[i].p[k]), first = false;
+      std::printf("]}\n");
     }
   } catch (const Error& e) {
     std::fprintf(stderr, "nelson_amd error %d: %s\n", e.code(), e.what());

@@ -173,6 +173,11 @@ def test_variable_block_sizes(NU, NV, vv, seed, tmp_path):
             if y[2] < 1e-20:
                 break
         assert np.allclose(d["x"], t.x(), rtol=0, atol=1e-12)
+    # the U parameters alone as a single section (parameterSize()): LM
+    # reaches the ground truth
+    d = res["single_lm"]
+    assert d["final_chi2"] < 1e-12, d
+    assert np.allclose(d["x"], np.concatenate(t.gtU), rtol=0, atol=1e-6)
     # one damped step in the parameters' own sizes against a dense solve
     t = VarToy(NU, NV, vv, seed)
     H, b = t.dense()
