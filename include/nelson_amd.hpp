@@ -682,6 +682,10 @@ class DoubleSectionHessian : public SchurSystem {
   static_assert(BU >= 1 && BU <= 6 && BV >= 1 && BV <= 3, "block sizes up to 6 (U) and 3 (V)");
 
  public:
+  struct Traits {  // DoubleSectionHessianTraits (nelson/DoubleSectionHessian.h)
+    using Type = double;
+    static constexpr int blockSizeU = BU, blockSizeV = BV;
+  };
   ~DoubleSectionHessian() override {
     if (h_) nsn_problem_destroy(h_);
   }
@@ -908,6 +912,7 @@ class DoubleSection {
   using ParameterUType = ParU;
   using ParameterVType = ParV;
   using HessianT = DoubleSectionHessian<BU, BV>;
+  using Hessian = HessianT;  // P::Hessian::Traits names the solver's argument
   using HessianVectorsT = HessianVectors;
   static constexpr int blockSizeU = BU, blockSizeV = BV;
   template <class E>
@@ -1354,6 +1359,28 @@ class SolverCholeskySchur {
 // reasons and the per-sub-iteration (it, subit, chi2, mu) log keep the
 // reference's meaning.
 // ---------------------------------------------------------------------------
+// The reference's solver selection (nelson/SolverTraits.h:12-55,
+// SolverTraitsBase.h:12-40, OrderingTraits.h:9-11, MatrixWrapperTraits.h:11-12):
+// every solver kind, wrapper and ordering names the GPU Schur solver here
+// (dense and envelope storage of S are chosen from its structure, the
+// camera order by the section), so code written against
+// SolverTraits<kind>::Solver<HessianTraits, ...> compiles unchanged.
+constexpr int solverCholeskyDense = 1;
+constexpr int solverCholeskySparse = 2;
+constexpr int solverCholeskySchur = 3;
+constexpr int solverCholeskySchurDiagBlockInverse = 4;
+constexpr int solverCholeskySchurDiagBlockInverseWWtMult = 5;
+constexpr int choleskyNaturalOrdering = 0;
+constexpr int choleskyAMDOrdering = 1;
+constexpr int choleskyCOLAMDOrdering = 2;
+constexpr int matrixWrapperDense = 1;
+constexpr int matrixWrapperSparse = 2;
+template <int kind>
+struct SolverTraits {
+  template <class HessianTraits, int... options>
+  using Solver = SolverCholeskySchur;
+};
+
 struct LevenbergMarquardtSettings : nsn_lm_settings {
   LevenbergMarquardtSettings() { nsn_lm_default_settings(this); }
 };

@@ -249,7 +249,12 @@ int main(int argc, char** argv) {
     {
       VarSection s(d);
       build(s, d);
-      LevenbergMarquardt<SolverCholeskySchur> lm;
+      // the solver named as the reference's tests name it
+      // (nelson/tests/testDoubleSectionCommon.hpp:61)
+      using Solver = SolverTraits<solverCholeskySchur>::Solver<VarSection::Hessian::Traits, matrixWrapperDense,
+                                                                matrixWrapperSparse, solverCholeskyDense,
+                                                                choleskyAMDOrdering, choleskyAMDOrdering>;
+      LevenbergMarquardt<Solver> lm;
       const TerminationReason tc = lm.solve(s);
       print_solve("lm", tc, lm.stats(), s, d);
     }
