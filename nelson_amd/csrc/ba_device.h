@@ -10,6 +10,32 @@
 
 namespace nsn {
 
+// Workgroups are dealt to the device's 8 XCDs in turn (workgroup b runs on
+// XCD b % 8), each XCD with its own L2.  xcd_order<C>(b, n) maps workgroup b
+// of n to a work item so that every XCD takes runs of C consecutive items:
+// neighbouring cameras or S tiles, which read the same points' blocks (a
+// point's W / Z blocks are adjacent in memory), then share one L2 instead of
+// eight, while the runs still rotate over the XCDs (8 C items at a time), so
+// uneven items spread over all of them.  A bijection on [0, n) (the last
+// n mod 8C items keep their order); every item is computed as before.
+template <int C>
+__device__ __forceinline__ int xcd_order(int b, int n) {
+  constexpr int kXcd = 8, S = kXcd * C;
+  if (b >= (n / S) * S) return b;
+  const int r = b % S;
+  return b - r + (r % kXcd) * C + r / kXcd;
+}
+
+// one contiguous run per XCD (items 0 .. n/8 on the first, and so on), for
+// launches whose workgroups are all resident at once and whose items cost
+// about the same (one workgroup per camera)
+__device__ __forceinline__ int xcd_runs(int b, int n) {
+  constexpr int kXcd = 8;
+  const int per = (n + kXcd - 1) / kXcd, rem = n % kXcd;
+  const int x = b % kXcd, k = b / kXcd;
+  return x * per - (rem ? max(0, x - rem) : 0) + k;
+}
+
 __device__ __forceinline__ bool gate_open(const DevCtl* c, int gate) {
   switch (gate) {
     case GATE_ALWAYS: return true;

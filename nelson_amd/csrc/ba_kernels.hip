@@ -63,7 +63,7 @@ __global__ __launch_bounds__(256) void k_lin_cams(BAView v, T* __restrict__ U, T
   if (!gate_open(ctl, gate)) return;
   __shared__ double red[4][27];
   __shared__ double pri[27];  // the camera's unary prior: J^T J (21 packed), J^T r
-  const int u = blockIdx.x;
+  const int u = xcd_runs(blockIdx.x, gridDim.x);
   const int c = v.var_cam_user[u];
   if (threadIdx.x == 0) {
     for (int i = 0; i < 27; i++) pri[i] = 0.0;
@@ -208,7 +208,7 @@ __global__ __launch_bounds__(256) void k_s_diag(BAView v, const T* __restrict__ 
                                                 T* __restrict__ bS, DevCtl* ctl, int gate) {
   if (!gate_open(ctl, gate)) return;
   __shared__ double red[4][27];
-  const int u = blockIdx.x;
+  const int u = xcd_runs(blockIdx.x, gridDim.x);
   T acc[27];
 #pragma unroll
   for (int i = 0; i < 27; i++) acc[i] = (T)0;
@@ -284,7 +284,7 @@ __global__ __launch_bounds__(256) void k_s_tiles(TileView tv, const int* __restr
   if (!gate_open(ctl, gate)) return;
   __shared__ T lt[TB * TB];
   __shared__ T red[4][kSlots * 36];
-  const int q = blockIdx.x;
+  const int q = xcd_order<8>(blockIdx.x, gridDim.x);
   const int t = part_tile[q];
   const int rlo = part_rect[4 * q], rhi = part_rect[4 * q + 1];
   const int clo = part_rect[4 * q + 2], chi = part_rect[4 * q + 3];
@@ -439,7 +439,7 @@ __global__ __launch_bounds__(256) void k_ir_reduce(BAView v, const T* __restrict
     }
   }
   __shared__ double red[4][6];
-  const int u = blockIdx.x;
+  const int u = xcd_runs(blockIdx.x, gridDim.x);
   double acc[6] = {0, 0, 0, 0, 0, 0};
   const int i1 = v.cw_ptr[u + 1];
   for (int ii = v.cw_ptr[u] + threadIdx.x; ii < i1; ii += 256) {
